@@ -1,0 +1,415 @@
+// Hamiltonian assembly into canonical CSR on the device.
+//
+// Row-centric restatement of the reference's term-by-term triplet builders
+// (nbody_data*, localbody_data_par: edlgt/symmetries/sym_qmb_operations.py:
+// 297-602, driven per placement by LocalTerm / TwoBodyTerm / PlaquetteTerm)
+// followed by SciPy's COO->CSR coalescing (qmb_hamiltonian.py:206-209).
+//
+// One 64-lane wavefront owns one bra row at a time:
+//   1. lanes take placements (one term instance on concrete sites) in rounds
+//      of 64; each counts the product of its factors' transition counts for
+//      the row's local states; a wave prefix sum gives every placement a slot
+//      range in the wave's LDS buffer, in placement order;
+//   2. each lane enumerates its placement's transitions (mixed radix, last
+//      factor fastest), forming the target key by key arithmetic
+//      key' = key + sum_f (ket_f - bra_f) * stride[site_f]
+//      and the amplitude prod_f O_f * strength (conjugated for dagger
+//      placements, whose transitions are the column nonzeros of O_f);
+//   3. (key', slot) pairs are bitonic-sorted in LDS; equal keys are summed in
+//      slot order (duplicates summed, explicit zeros kept);
+//   4. count pass: number of distinct keys per row -> int64 scan -> row_ptr;
+//      fill pass: the head of every run binary-searches its key in the sorted
+//      basis (the diagonal is the row itself) and writes (col, value).
+// Pass 1 never searches; a key missing from the sector is detected in pass 2.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+
+#include "common.h"
+
+namespace edlgt {
+namespace {
+
+constexpr int kMaxOps = 8;
+constexpr double kOpTol = 1e-10;   // sym_qmb_operations.py:21
+constexpr double kRealTol = 1e-12; // dtype_config.py:101-112
+
+enum : int32_t { kErrTooMany = 1, kErrMissing = 2, kErrNotReal = 4, kErrOverflow = 8 };
+
+struct Placement {
+  int32_t nops;
+  int32_t dagger;
+  int32_t site[kMaxOps];
+  int32_t table[kMaxOps];
+  double s_re, s_im;
+};
+
+struct AsmDev {
+  int64_t n_states;
+  int n_place;
+  int d_max;
+  int cfg_stride;
+  int value_real;                 // 1: float64 output (values must be real)
+  const Placement* place;
+  const uint64_t* strides;        // [n_sites]
+  const uint64_t* keys;           // [N]
+  const uint8_t* configs;         // [N][cfg_stride]
+  // transitions: for (table t, side) with side 0 = row nonzeros O[a][*],
+  // side 1 = column nonzeros O[*][a]; entries of bra state a are
+  // trans_off[(2t+side)*(d_max+1) + a] .. +1
+  const int32_t* trans_off;
+  const uint8_t* trans_ket;
+  const double2* trans_val;
+  const double2* diag;            // [n_tables][d_max]  O[a][a]
+  int32_t* err;
+};
+
+__device__ inline int wave_excl_scan(int v, int* total) {
+  const int lane = threadIdx.x & 63;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline int64_t lower_bound_u64(const uint64_t* __restrict__ a, int64_t n, uint64_t k) {
+  int64_t lo = 0, len = n;
+  while (len > 0) {
+    const int64_t half = len >> 1;
+    const int64_t mid = lo + half;
+    if (a[mid] < k) {
+      lo = mid + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return lo;
+}
+
+// Number of raw entries placement P emits from the row with local states cfg.
+__device__ inline int placement_count(const AsmDev& A, const Placement& P, const uint8_t* cfg) {
+  if (P.nops == 1) {
+    const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
+    return (sqrt(v.x * v.x + v.y * v.y) >= kOpTol) ? 1 : 0;
+  }
+  int c = 1;
+  for (int f = 0; f < P.nops; ++f) {
+    const int a = cfg[P.site[f]];
+    const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
+    c *= off[a + 1] - off[a];
+  }
+  return c;
+}
+
+// Enumerates placement P's entries into LDS slots [slot0, slot0 + count).
+template <bool kFill>
+__device__ inline void placement_emit(const AsmDev& A, const Placement& P, const uint8_t* cfg,
+                                      uint64_t key, int slot0, uint64_t* skey, uint16_t* sidx,
+                                      double2* sval) {
+  if (P.nops == 1) {
+    skey[slot0] = key;
+    sidx[slot0] = (uint16_t)slot0;
+    if (kFill) {
+      const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
+      double2 r = cmul(v, make_double2(P.s_re, P.s_im));
+      if (A.value_real && fabs(r.y) > kRealTol) atomicOr(A.err, kErrNotReal);
+      sval[slot0] = r;
+    }
+    return;
+  }
+  int beg[kMaxOps], cnt[kMaxOps], idx[kMaxOps];
+  int64_t base_delta = 0;
+  for (int f = 0; f < P.nops; ++f) {
+    const int a = cfg[P.site[f]];
+    const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
+    beg[f] = off[a];
+    cnt[f] = off[a + 1] - off[a];
+    idx[f] = 0;
+    base_delta -= (int64_t)a * (int64_t)A.strides[P.site[f]];
+  }
+  int slot = slot0;
+  while (true) {
+    int64_t delta = base_delta;
+    double2 amp = make_double2(1.0, 0.0);
+    for (int f = 0; f < P.nops; ++f) {
+      const int e = beg[f] + idx[f];
+      delta += (int64_t)A.trans_ket[e] * (int64_t)A.strides[P.site[f]];
+      if (kFill) amp = cmul(amp, A.trans_val[e]);
+    }
+    skey[slot] = key + (uint64_t)delta;
+    sidx[slot] = (uint16_t)slot;
+    if (kFill) {
+      double2 r = cmul(amp, make_double2(P.s_re, P.s_im));
+      if (P.dagger) r.y = -r.y;
+      if (A.value_real && fabs(r.y) > kRealTol) atomicOr(A.err, kErrNotReal);
+      sval[slot] = r;
+    }
+    ++slot;
+    int f = P.nops - 1;
+    for (; f >= 0; --f) {
+      if (++idx[f] < cnt[f]) break;
+      idx[f] = 0;
+    }
+    if (f < 0) break;
+  }
+}
+
+// One 64-thread block == one wavefront; LDS holds the row's raw entries.
+template <int kCap, bool kFill>
+__global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restrict__ row_count,
+                                                    const int64_t* __restrict__ row_ptr,
+                                                    int32_t* __restrict__ out_col,
+                                                    double* __restrict__ out_val) {
+  __shared__ uint64_t skey[kCap];
+  __shared__ uint16_t sidx[kCap];
+  __shared__ double2 sval[kFill ? kCap : 1];
+  const int lane = threadIdx.x;
+  for (int64_t r = blockIdx.x; r < A.n_states; r += gridDim.x) {
+    const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
+    const uint64_t key = A.keys[r];
+    // 1+2: count placements, assign slots in placement order, emit.
+    int base = 0;
+    bool overflow = false;
+    for (int pb = 0; pb < A.n_place; pb += 64) {
+      const int p = pb + lane;
+      int c = 0;
+      Placement P;
+      if (p < A.n_place) {
+        P = A.place[p];
+        c = placement_count(A, P, cfg);
+      }
+      int tot;
+      const int off = wave_excl_scan(c, &tot);
+      if (base + tot > kCap) {
+        overflow = true;
+        break;
+      }
+      if (c > 0) placement_emit<kFill>(A, P, cfg, key, base + off, skey, sidx, sval);
+      base += tot;
+    }
+    if (overflow) {
+      if (lane == 0) atomicOr(A.err, kErrTooMany);
+      if (!kFill) row_count[r] = 0;
+      __syncthreads();
+      continue;
+    }
+    const int T = base;
+    int P2 = 1;
+    while (P2 < T) P2 <<= 1;
+    for (int i = T + lane; i < P2; i += 64) {
+      skey[i] = ~0ull;
+      sidx[i] = 0xffff;
+    }
+    __syncthreads();
+    // 3: bitonic sort of (key, slot) ascending.
+    for (int k = 2; k <= P2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = lane; i < P2; i += 64) {
+          const int ixj = i ^ j;
+          if (ixj > i) {
+            const uint64_t ka = skey[i], kb = skey[ixj];
+            const uint16_t ia = sidx[i], ib = sidx[ixj];
+            const bool a_gt_b = (ka > kb) || (ka == kb && ia > ib);
+            const bool up = (i & k) == 0;
+            if (a_gt_b == up) {
+              skey[i] = kb; skey[ixj] = ka;
+              sidx[i] = ib; sidx[ixj] = ia;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // 4: coalesce runs of equal keys.
+    int rank_base = 0;
+    for (int ib = 0; ib < T; ib += 64) {
+      const int i = ib + lane;
+      const bool head = i < T && (i == 0 || skey[i] != skey[i - 1]);
+      int tot;
+      const int rk = wave_excl_scan(head ? 1 : 0, &tot);
+      if (kFill && head) {
+        const uint64_t k = skey[i];
+        double2 v = sval[sidx[i]];
+        for (int q = i + 1; q < T && skey[q] == k; ++q) {
+          const double2 w = sval[sidx[q]];
+          v.x += w.x;
+          v.y += w.y;
+        }
+        int64_t col;
+        if (k == key) {
+          col = r;
+        } else {
+          col = lower_bound_u64(A.keys, A.n_states, k);
+          if (col >= A.n_states || A.keys[col] != k) {
+            atomicOr(A.err, kErrMissing);
+            col = 0;
+          }
+        }
+        const int64_t o = row_ptr[r] + rank_base + rk;
+        out_col[o] = (int32_t)col;
+        if (A.value_real) {
+          out_val[o] = v.x;
+        } else {
+          out_val[2 * o] = v.x;
+          out_val[2 * o + 1] = v.y;
+        }
+      }
+      rank_base += tot;
+    }
+    if (!kFill && lane == 0) row_count[r] = rank_base;
+    __syncthreads();
+  }
+}
+
+template <int kCap>
+int launch_pass(bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr, int32_t* col,
+                double* val, hipStream_t st) {
+  const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(A.n_states, 1), 1 << 18);
+  if (fill)
+    hipLaunchKernelGGL((assemble_rows<kCap, true>), dim3(grid), dim3(64), 0, st, A, counts, row_ptr,
+                       col, val);
+  else
+    hipLaunchKernelGGL((assemble_rows<kCap, false>), dim3(grid), dim3(64), 0, st, A, counts,
+                       row_ptr, col, val);
+  EDLGT_HIP(hipGetLastError());
+  return EDLGT_OK;
+}
+
+int launch_any(int cap, bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr,
+               int32_t* col, double* val, hipStream_t st) {
+  if (cap <= 64) return launch_pass<64>(fill, A, counts, row_ptr, col, val, st);
+  if (cap <= 128) return launch_pass<128>(fill, A, counts, row_ptr, col, val, st);
+  if (cap <= 256) return launch_pass<256>(fill, A, counts, row_ptr, col, val, st);
+  if (cap <= 512) return launch_pass<512>(fill, A, counts, row_ptr, col, val, st);
+  return launch_pass<1024>(fill, A, counts, row_ptr, col, val, st);
+}
+
+int check_err(const DevBuf& err, hipStream_t st) {
+  int32_t e = 0;
+  EDLGT_HIP(hipMemcpyAsync(&e, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  EDLGT_HIP(hipStreamSynchronize(st));
+  if (e & kErrMissing)
+    return set_error(EDLGT_ERR_KEY_NOT_FOUND, "a ket configuration is not in the symmetry sector");
+  if (e & kErrNotReal)
+    return set_error(EDLGT_ERR_NOT_REAL, "complex matrix element in real value mode (|Im| > 1e-12)");
+  if (e & kErrTooMany)
+    return set_error(EDLGT_ERR_INDEX_OVERFLOW, "more than 1024 raw entries in one row");
+  return EDLGT_OK;
+}
+
+}  // namespace
+
+int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
+                      int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                      const int32_t* place_tables, const double* place_strength,
+                      const int32_t* place_dagger, int32_t value_type, hipStream_t st, Csr* out) {
+  if (value_type != EDLGT_VALUE_F64 && value_type != EDLGT_VALUE_C128)
+    return set_error(EDLGT_ERR_INVALID_ARG, "value_type must be F64 or C128");
+  if (d_max < 1 || d_max > 255 || n_tables < 1 || n_place < 0)
+    return set_error(EDLGT_ERR_INVALID_ARG, "invalid table / placement counts");
+  // Host: transition lists (row and column nonzeros) of every table.
+  using cd = std::complex<double>;
+  std::vector<int32_t> trans_off((size_t)2 * n_tables * (d_max + 1));
+  std::vector<uint8_t> trans_ket;
+  std::vector<double2> trans_val, diag((size_t)n_tables * d_max);
+  for (int t = 0; t < n_tables; ++t) {
+    const double* O = tables + (size_t)t * d_max * d_max * 2;
+    auto at = [&](int a, int b) { return cd(O[2 * (a * d_max + b)], O[2 * (a * d_max + b) + 1]); };
+    for (int a = 0; a < d_max; ++a) diag[(size_t)t * d_max + a] = make_double2(at(a, a).real(), at(a, a).imag());
+    for (int side = 0; side < 2; ++side) {
+      int32_t* off = trans_off.data() + (size_t)(2 * t + side) * (d_max + 1);
+      for (int a = 0; a < d_max; ++a) {
+        off[a] = (int32_t)trans_ket.size();
+        for (int b = 0; b < d_max; ++b) {
+          // side 0: O[a][b] (bra a -> ket b); side 1: O[b][a] (column a)
+          const cd v = side == 0 ? at(a, b) : at(b, a);
+          if (std::abs(v) > kOpTol) {
+            trans_ket.push_back((uint8_t)b);
+            trans_val.push_back(make_double2(v.real(), v.imag()));
+          }
+        }
+      }
+      off[d_max] = (int32_t)trans_ket.size();
+    }
+  }
+  std::vector<Placement> place(n_place);
+  int64_t cap_bound = 0;
+  for (int p = 0; p < n_place; ++p) {
+    Placement& P = place[p];
+    P.nops = place_nops[p];
+    P.dagger = place_dagger[p] ? 1 : 0;
+    if (P.nops < 1 || P.nops > kMaxOps)
+      return set_error(EDLGT_ERR_INVALID_ARG, "placement with 0 or more than 8 factors");
+    int64_t bound = 1;
+    for (int f = 0; f < kMaxOps; ++f) {
+      P.site[f] = f < P.nops ? place_sites[kMaxOps * p + f] : 0;
+      P.table[f] = f < P.nops ? place_tables[kMaxOps * p + f] : 0;
+      if (f < P.nops) {
+        if (P.site[f] < 0 || P.site[f] >= B.n_sites || P.table[f] < 0 || P.table[f] >= n_tables)
+          return set_error(EDLGT_ERR_INVALID_ARG, "placement site/table out of range");
+        const int32_t* off = trans_off.data() + (size_t)(2 * P.table[f] + P.dagger) * (d_max + 1);
+        int mx = 0;
+        for (int a = 0; a < d_max; ++a) mx = std::max(mx, off[a + 1] - off[a]);
+        bound *= mx;
+      }
+    }
+    if (P.nops == 1) bound = 1;
+    cap_bound += bound;
+    P.s_re = place_strength[2 * p];
+    P.s_im = place_strength[2 * p + 1];
+  }
+  const int cap = (int)std::min<int64_t>(std::max<int64_t>(cap_bound, 1), 1024);
+
+  DevBuf b_place, b_str, b_toff, b_tket, b_tval, b_diag, b_err;
+  EDLGT_TRY(upload(b_place, place.data(), std::max<size_t>(place.size(), 1), st));
+  EDLGT_TRY(upload(b_str, B.strides.data(), B.strides.size(), st));
+  EDLGT_TRY(upload(b_toff, trans_off.data(), trans_off.size(), st));
+  const uint8_t z8 = 0;
+  const double2 z2 = make_double2(0, 0);
+  EDLGT_TRY(upload(b_tket, trans_ket.empty() ? &z8 : trans_ket.data(), std::max<size_t>(trans_ket.size(), 1), st));
+  EDLGT_TRY(upload(b_tval, trans_val.empty() ? &z2 : trans_val.data(), std::max<size_t>(trans_val.size(), 1), st));
+  EDLGT_TRY(upload(b_diag, diag.data(), diag.size(), st));
+  EDLGT_TRY(b_err.alloc(sizeof(int32_t)));
+  EDLGT_HIP(hipMemsetAsync(b_err.p, 0, sizeof(int32_t), st));
+
+  AsmDev A;
+  A.n_states = B.n_states; A.n_place = n_place; A.d_max = d_max; A.cfg_stride = B.cfg_stride;
+  A.value_real = value_type == EDLGT_VALUE_F64;
+  A.place = b_place.as<Placement>(); A.strides = b_str.as<uint64_t>();
+  A.keys = B.keys.as<uint64_t>(); A.configs = B.configs.as<uint8_t>();
+  A.trans_off = b_toff.as<int32_t>(); A.trans_ket = b_tket.as<uint8_t>();
+  A.trans_val = b_tval.as<double2>(); A.diag = b_diag.as<double2>(); A.err = b_err.as<int32_t>();
+
+  const int64_t N = B.n_states;
+  out->n_rows = N;
+  out->value_type = value_type;
+  EDLGT_TRY(out->row_ptr.alloc((N + 1) * sizeof(int64_t)));
+  DevBuf counts;
+  EDLGT_TRY(counts.alloc(std::max<int64_t>(N, 1) * sizeof(int32_t)));
+  if (N > 0) EDLGT_TRY(launch_any(cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr, st));
+  EDLGT_TRY(check_err(b_err, st));
+  EDLGT_TRY(exclusive_scan_i32_i64(counts.as<int32_t>(), out->row_ptr.as<int64_t>(), N, st));
+  int64_t nnz = 0;
+  EDLGT_HIP(hipMemcpyAsync(&nnz, out->row_ptr.as<int64_t>() + N, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, st));
+  EDLGT_HIP(hipStreamSynchronize(st));
+  out->nnz = nnz;
+  EDLGT_TRY(out->col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  EDLGT_TRY(out->val.alloc(std::max<int64_t>(nnz, 1) * value_bytes(value_type)));
+  if (N > 0)
+    EDLGT_TRY(launch_any(cap, true, A, nullptr, out->row_ptr.as<int64_t>(), out->col.as<int32_t>(),
+                         out->val.as<double>(), st));
+  EDLGT_TRY(check_err(b_err, st));
+  return EDLGT_OK;
+}
+
+}  // namespace edlgt

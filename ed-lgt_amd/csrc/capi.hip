@@ -1,0 +1,267 @@
+// C ABI of libedlgt_hip (declared in include/edlgt_hip.h): argument checks,
+// error capture and handle management around the device kernels.
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace edlgt {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+namespace {
+
+__device__ inline int64_t lb_u64(const uint64_t* a, int64_t n, uint64_t k) {
+  int64_t lo = 0, len = n;
+  while (len > 0) {
+    const int64_t half = len >> 1;
+    if (a[lo + half] < k) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return lo;
+}
+
+__global__ void lookup_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                              const uint64_t* __restrict__ q, int64_t nq, int64_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = lb_u64(keys, n, q[i]);
+    out[i] = (p < n && keys[p] == q[i]) ? p : -1;
+  }
+}
+
+}  // namespace
+
+int basis_lookup(const Basis& b, const uint16_t* configs, int64_t nq, int64_t* out_idx) {
+  std::vector<uint64_t> qk(std::max<int64_t>(nq, 1));
+  bool bad = false;
+  for (int64_t i = 0; i < nq; ++i) {
+    uint64_t k = 0;
+    for (int s = 0; s < b.n_sites; ++s) {
+      const uint16_t c = configs[i * b.n_sites + s];
+      if (c >= b.loc_dims[s]) bad = true;
+      k += (uint64_t)c * b.strides[s];
+    }
+    qk[i] = bad ? ~0ull : k;
+    bad = false;
+  }
+  if (nq <= 0) return EDLGT_OK;
+  DevBuf dq, dout;
+  EDLGT_TRY(upload(dq, qk.data(), nq, nullptr));
+  EDLGT_TRY(dout.alloc(nq * sizeof(int64_t)));
+  hipLaunchKernelGGL(lookup_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, nullptr,
+                     b.keys.as<uint64_t>(), b.n_states, dq.as<uint64_t>(), nq, dout.as<int64_t>());
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_HIP(hipMemcpy(out_idx, dout.p, nq * sizeof(int64_t), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < nq; ++i)
+    if (out_idx[i] < 0)
+      return set_error(EDLGT_ERR_KEY_NOT_FOUND, "config not compatible with the symmetry sector");
+  return EDLGT_OK;
+}
+
+}  // namespace edlgt
+
+using namespace edlgt;
+
+#define EDLGT_REQUIRE(cond, msg) \
+  do {                           \
+    if (!(cond)) return set_error(EDLGT_ERR_INVALID_ARG, msg); \
+  } while (0)
+
+extern "C" {
+
+int edlgt_last_error(char* buf, int64_t size) {
+  if (!buf || size <= 0) return EDLGT_ERR_INVALID_ARG;
+  const size_t n = std::min<size_t>((size_t)size - 1, g_last_error.size());
+  std::memcpy(buf, g_last_error.data(), n);
+  buf[n] = '\0';
+  return EDLGT_OK;
+}
+
+int edlgt_version(void) { return 1; }
+
+int edlgt_device_info(int32_t* device, int64_t* free_bytes, int64_t* total_bytes) {
+  int d = 0;
+  EDLGT_HIP(hipGetDevice(&d));
+  size_t fr = 0, tot = 0;
+  EDLGT_HIP(hipMemGetInfo(&fr, &tot));
+  if (device) *device = d;
+  if (free_bytes) *free_bytes = (int64_t)fr;
+  if (total_bytes) *total_bytes = (int64_t)tot;
+  return EDLGT_OK;
+}
+
+int edlgt_sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_t n_glob,
+                       int32_t glob_type, const double* glob_diags, const double* glob_sectors,
+                       int32_t n_dirs, const double* link_diags, const double* link_sectors,
+                       int32_t n_pairs, const int32_t* pair_dir, const int32_t* pair_site0,
+                       const int32_t* pair_site1, int32_t n_nbody, int32_t nbody_type,
+                       const double* nbody_diags, const double* nbody_sectors,
+                       const int32_t* nbody_site_offsets, const int32_t* nbody_sites,
+                       void* stream, edlgt_basis** out) {
+  EDLGT_REQUIRE(out && loc_dims, "null output or loc_dims");
+  *out = nullptr;
+  EDLGT_REQUIRE(n_glob == 0 || (glob_diags && glob_sectors), "null global tables");
+  EDLGT_REQUIRE(n_pairs == 0 || (pair_dir && pair_site0 && pair_site1 && link_diags && link_sectors),
+                "null link tables");
+  EDLGT_REQUIRE(n_nbody == 0 || (nbody_diags && nbody_sectors && nbody_site_offsets && nbody_sites),
+                "null n-body tables");
+  EDLGT_REQUIRE(n_nbody == 0 || nbody_type == EDLGT_SYM_U1 || nbody_type == EDLGT_SYM_Z2,
+                "nbody_type must be U1 or Z2");
+  edlgt_basis* b = new (std::nothrow) edlgt_basis();
+  if (!b) return set_error(EDLGT_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int s = sector_build(n_sites, loc_dims, d_max, n_glob, glob_type, glob_diags, glob_sectors, n_dirs,
+                       link_diags, link_sectors, n_pairs, pair_dir, pair_site0, pair_site1, n_nbody,
+                       nbody_type, nbody_diags, nbody_sectors, nbody_site_offsets, nbody_sites,
+                       (hipStream_t)stream, b);
+  if (s != EDLGT_OK) {
+    delete b;
+    return s;
+  }
+  *out = b;
+  return EDLGT_OK;
+}
+
+int edlgt_basis_info(const edlgt_basis* b, int64_t* n_states, int32_t* n_sites) {
+  EDLGT_REQUIRE(b, "null basis");
+  if (n_states) *n_states = b->n_states;
+  if (n_sites) *n_sites = b->n_sites;
+  return EDLGT_OK;
+}
+
+int edlgt_basis_copy_configs(const edlgt_basis* b, uint16_t* out_host) {
+  EDLGT_REQUIRE(b && out_host, "null argument");
+  const int64_t N = b->n_states;
+  if (N == 0) return EDLGT_OK;
+  std::vector<uint8_t> raw((size_t)N * b->cfg_stride);
+  EDLGT_HIP(hipMemcpy(raw.data(), b->configs.p, raw.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < N; ++i)
+    for (int s = 0; s < b->n_sites; ++s) out_host[i * b->n_sites + s] = raw[i * b->cfg_stride + s];
+  return EDLGT_OK;
+}
+
+int edlgt_basis_copy_keys(const edlgt_basis* b, uint64_t* out_host) {
+  EDLGT_REQUIRE(b && out_host, "null argument");
+  if (b->n_states == 0) return EDLGT_OK;
+  EDLGT_HIP(hipMemcpy(out_host, b->keys.p, b->n_states * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return EDLGT_OK;
+}
+
+int edlgt_basis_lookup(const edlgt_basis* b, const uint16_t* configs_host, int64_t n_query,
+                       int64_t* out_idx_host) {
+  EDLGT_REQUIRE(b && (n_query == 0 || (configs_host && out_idx_host)), "null argument");
+  return basis_lookup(*b, configs_host, n_query, out_idx_host);
+}
+
+int edlgt_basis_device_ptrs(const edlgt_basis* b, void** keys, void** configs, int32_t* cfg_stride) {
+  EDLGT_REQUIRE(b, "null basis");
+  if (keys) *keys = b->keys.p;
+  if (configs) *configs = b->configs.p;
+  if (cfg_stride) *cfg_stride = b->cfg_stride;
+  return EDLGT_OK;
+}
+
+int edlgt_basis_destroy(edlgt_basis* b) {
+  delete b;
+  return EDLGT_OK;
+}
+
+int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
+                            const double* tables, int32_t n_place, const int32_t* place_nops,
+                            const int32_t* place_sites, const int32_t* place_tables,
+                            const double* place_strength, const int32_t* place_dagger,
+                            int32_t value_type, void* stream, edlgt_csr** out) {
+  EDLGT_REQUIRE(b && out && tables, "null argument");
+  EDLGT_REQUIRE(n_place == 0 || (place_nops && place_sites && place_tables && place_strength &&
+                                 place_dagger),
+                "null placement arrays");
+  *out = nullptr;
+  edlgt_csr* h = new (std::nothrow) edlgt_csr();
+  if (!h) return set_error(EDLGT_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int s = hamiltonian_build(*b, d_max, n_tables, tables, n_place, place_nops, place_sites,
+                            place_tables, place_strength, place_dagger, value_type,
+                            (hipStream_t)stream, h);
+  if (s != EDLGT_OK) {
+    delete h;
+    return s;
+  }
+  *out = h;
+  return EDLGT_OK;
+}
+
+int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* value_type) {
+  EDLGT_REQUIRE(h, "null matrix");
+  if (n_rows) *n_rows = h->n_rows;
+  if (nnz) *nnz = h->nnz;
+  if (value_type) *value_type = h->value_type;
+  return EDLGT_OK;
+}
+
+int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val) {
+  EDLGT_REQUIRE(h, "null matrix");
+  if (row_ptr)
+    EDLGT_HIP(hipMemcpy(row_ptr, h->row_ptr.p, (h->n_rows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (col && h->nnz)
+    EDLGT_HIP(hipMemcpy(col, h->col.p, h->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (val && h->nnz)
+    EDLGT_HIP(hipMemcpy(val, h->val.p, h->nnz * value_bytes(h->value_type), hipMemcpyDeviceToHost));
+  return EDLGT_OK;
+}
+
+int edlgt_csr_device_ptrs(const edlgt_csr* h, void** row_ptr, void** col, void** val) {
+  EDLGT_REQUIRE(h, "null matrix");
+  if (row_ptr) *row_ptr = h->row_ptr.p;
+  if (col) *col = h->col.p;
+  if (val) *val = h->val.p;
+  return EDLGT_OK;
+}
+
+int edlgt_csr_make_low_precision(edlgt_csr* h, void* stream) {
+  EDLGT_REQUIRE(h, "null matrix");
+  return csr_make_low_precision(*h, (hipStream_t)stream);
+}
+
+int edlgt_csr_destroy(edlgt_csr* h) {
+  delete h;
+  return EDLGT_OK;
+}
+
+int edlgt_spmv(edlgt_csr* h, int32_t precision, const void* x, void* y, int32_t vec_type,
+               void* stream) {
+  return edlgt_spmm(h, precision, 1, x, y, vec_type, stream);
+}
+
+int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, void* Y,
+               int32_t vec_type, void* stream) {
+  EDLGT_REQUIRE(h && X && Y, "null argument");
+  EDLGT_REQUIRE(n_rhs >= 1, "n_rhs must be >= 1");
+  EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
+  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64, "vec_type must be C128 or F64");
+  EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_F64 && h->value_type != EDLGT_VALUE_F64),
+                "real vectors need a real matrix");
+  if (precision == EDLGT_PREC_MIXED) EDLGT_TRY(csr_make_low_precision(*h, (hipStream_t)stream));
+  return csr_spmm(*h, precision, vec_type, n_rhs, X, h->n_rows, Y, h->n_rows, (hipStream_t)stream);
+}
+
+int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis, int64_t max_iter,
+                  int32_t precision, int32_t vec_type, const void* v0_host, double* out_evals,
+                  void* out_evecs, double* out_resid, int64_t* out_matvecs, void* stream) {
+  EDLGT_REQUIRE(h && out_evals, "null argument");
+  EDLGT_REQUIRE(tol > 0.0, "tol must be positive");
+  EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
+  return lanczos_run(*h, n_eigs, tol, max_basis, max_iter, precision, vec_type, v0_host, out_evals,
+                     out_evecs, out_resid, out_matvecs, (hipStream_t)stream);
+}
+
+}  // extern "C"

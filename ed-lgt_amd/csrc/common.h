@@ -1,0 +1,121 @@
+// Shared internals of libedlgt_hip: status codes, error capture, device
+// buffers.  Everything here is internal; the public surface is include/edlgt_hip.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/edlgt_hip.h"
+
+namespace edlgt {
+
+// Records a message for edlgt_last_error() and returns the code.
+int set_error(int code, const std::string& msg);
+inline int hip_status(hipError_t e, const char* what) {
+  if (e == hipSuccess) return EDLGT_OK;
+  return set_error(e == hipErrorOutOfMemory ? EDLGT_ERR_OUT_OF_MEMORY : EDLGT_ERR_HIP,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define EDLGT_HIP(call)                                     \
+  do {                                                      \
+    hipError_t _e = (call);                                 \
+    if (_e != hipSuccess) return ::edlgt::hip_status(_e, #call); \
+  } while (0)
+
+#define EDLGT_TRY(call)                 \
+  do {                                  \
+    int _s = (call);                    \
+    if (_s != EDLGT_OK) return _s;      \
+  } while (0)
+
+// Owning device allocation (hipMalloc/hipFree), movable, not copyable.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_error(EDLGT_ERR_OUT_OF_MEMORY,
+                       "hipMalloc of " + std::to_string(n) + " bytes failed: " + hipGetErrorString(e));
+    }
+    bytes = n;
+    return EDLGT_OK;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+int upload(DevBuf& b, const T* host, size_t count, hipStream_t s) {
+  EDLGT_TRY(b.alloc(count * sizeof(T)));
+  if (count) EDLGT_HIP(hipMemcpyAsync(b.p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+  return EDLGT_OK;
+}
+
+// Device-wide exclusive scan of int32 counts into int64 offsets, with the total
+// written to offsets[n].  Implemented in scan.hip.
+int exclusive_scan_i32_i64(const int32_t* counts, int64_t* offsets, int64_t n, hipStream_t s);
+int exclusive_scan_i64(const int64_t* counts, int64_t* offsets, int64_t n, hipStream_t s);
+
+// Sorted sector basis kept on the device.
+struct Basis {
+  int32_t n_sites = 0;
+  int32_t cfg_stride = 0;     // bytes per configuration row (n_sites rounded up to 4)
+  int64_t n_states = 0;
+  std::vector<int32_t> loc_dims;
+  std::vector<uint64_t> strides;  // mixed-radix, last site fastest
+  DevBuf keys;                // uint64 [n_states], ascending
+  DevBuf configs;             // uint8  [n_states][cfg_stride]
+};
+
+// CSR Hamiltonian kept on the device.
+struct Csr {
+  int64_t n_rows = 0;
+  int64_t nnz = 0;
+  int32_t value_type = EDLGT_VALUE_C128;
+  int32_t g_override = 0;  // lanes per row of the SpMV kernels (0: from the mean row length)
+  DevBuf row_ptr;   // int64 [n_rows + 1]
+  DevBuf col;       // int32 [nnz]
+  DevBuf val;       // value_type [nnz]
+  DevBuf val_lo;    // reduced-precision copy (complex64 / float32), built on demand
+};
+
+inline size_t value_bytes(int32_t vt) {
+  switch (vt) {
+    case EDLGT_VALUE_F64: return 8;
+    case EDLGT_VALUE_C128: return 16;
+    case EDLGT_VALUE_F32: return 4;
+    case EDLGT_VALUE_C64: return 8;
+  }
+  return 0;
+}
+
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+}  // namespace edlgt
+
+struct edlgt_basis : edlgt::Basis {};
+struct edlgt_csr : edlgt::Csr {};

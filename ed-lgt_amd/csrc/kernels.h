@@ -1,0 +1,35 @@
+// Internal entry points shared between the translation units of libedlgt_hip.
+#pragma once
+#include "common.h"
+
+namespace edlgt {
+
+int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_t n_glob,
+                 int32_t glob_type, const double* glob_diags, const double* glob_sectors,
+                 int32_t n_dirs, const double* link_diags, const double* link_sectors,
+                 int32_t n_pairs, const int32_t* pair_dir, const int32_t* pair_site0,
+                 const int32_t* pair_site1, int32_t n_nbody, int32_t nbody_type,
+                 const double* nbody_diags, const double* nbody_sectors,
+                 const int32_t* nb_site_off, const int32_t* nb_sites, hipStream_t st, Basis* out);
+
+int basis_lookup(const Basis& b, const uint16_t* configs_host, int64_t n_query, int64_t* out_idx);
+
+int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
+                      int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                      const int32_t* place_tables, const double* place_strength,
+                      const int32_t* place_dagger, int32_t value_type, hipStream_t st, Csr* out);
+
+int lanes_per_row(const Csr& h);
+// Y[:, b] = H X[:, b] for b < n_rhs; xtype is the element type of X
+// (C128/F64, or C64/F32 with precision == MIXED); Y is FP64 (complex iff the
+// matrix or X is complex).
+int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx,
+             void* y, int64_t ldy, hipStream_t st);
+int csr_make_low_precision(Csr& h, hipStream_t st);
+int convert_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t st);
+
+int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter, int precision,
+                int vec_type, const void* v0_host, double* evals, void* evecs, double* resid,
+                int64_t* matvecs, hipStream_t st);
+
+}  // namespace edlgt

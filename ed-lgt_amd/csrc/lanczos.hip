@@ -1,0 +1,699 @@
+// Block thick-restart Lanczos for the lowest eigenpairs of a Hermitian CSR
+// matrix (replaces scipy eigsh(H, k, which="SA"), edlgt/modeling/
+// qmb_hamiltonian.py:260-332).
+//
+// Algorithm (Wu & Simon thick restart, block form):
+//   V = [V_0 .. V_{nv-1}] orthonormal (column-major, N x (M + b) on the
+//   device); the active block A = V[:, nv-b:nv] is multiplied by H with one
+//   SpMM (the matrix is streamed once for all b vectors), orthogonalised
+//   against the whole basis with two classical Gram-Schmidt passes
+//   (G = V^H W, W -= V G), and factored W = Q R by Cholesky QR twice.  The
+//   projected matrix T = V^H H V (Hermitian, host) receives G and R.  When
+//   the basis is full, T is diagonalised (cyclic complex Jacobi), residual
+//   norms ||R S_i|| decide convergence, and the l lowest Ritz vectors are
+//   kept (V <- V S[:, :l]) together with the residual block Q and the
+//   coupling R S.  Rank loss of W (invariant subspace) is repaired column by
+//   column with random vectors that carry no coupling.
+// Device kernels: tall-skinny Gram products V^H W with a deterministic
+// two-stage reduction, tall-skinny updates W = beta W + V C, and the CSR SpMM.
+// Mixed precision: the SpMM reads FP32 values and an FP32 copy of the active
+// block; every dot, the recurrence, the reorthogonalisation and T stay FP64.
+// The FP32 phase is followed by an FP64 phase started from its Ritz vectors.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace edlgt {
+namespace {
+
+using cd = std::complex<double>;
+
+// ---------------------------------------------------------------- scalars
+__device__ inline double2 cj_mul(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ inline double cj_mul(double a, double b) { return a * b; }
+__device__ inline double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double add(double a, double b) { return a + b; }
+__device__ inline double2 mul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// V element times a complex coefficient, in the vector's scalar type (real
+// vectors use the real part: their coefficients are real).
+__device__ inline double2 mul_s(double2 v, double2 c) { return mul(v, c); }
+__device__ inline double mul_s(double v, double2 c) { return v * c.x; }
+template <class S> __device__ inline S zero();
+template <> __device__ inline double zero<double>() { return 0.0; }
+template <> __device__ inline double2 zero<double2>() { return make_double2(0.0, 0.0); }
+__device__ inline double2 to_c(double2 a) { return a; }
+__device__ inline double2 to_c(double a) { return make_double2(a, 0.0); }
+
+constexpr int kGramBlocks = 512;
+constexpr int kTB = 256;
+
+// partial[blk][j][c] = sum over the block's rows of conj(V_j[r]) W_c[r]
+template <class S, int NW>
+__global__ void __launch_bounds__(kTB) gram_partial(const S* __restrict__ V, int64_t ldv, int nv,
+                                                    const S* __restrict__ W, int64_t ldw, int nw,
+                                                    int64_t n, double2* __restrict__ partial) {
+  constexpr int JB = NW >= 8 ? 1 : 8 / NW;
+  __shared__ double2 red[kTB];
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = (int64_t)blockIdx.x * chunk;
+  const int64_t r1 = min(n, r0 + chunk);
+  for (int j0 = 0; j0 < nv; j0 += JB) {
+    S acc[JB][NW];
+#pragma unroll
+    for (int a = 0; a < JB; ++a)
+#pragma unroll
+      for (int c = 0; c < NW; ++c) acc[a][c] = zero<S>();
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kTB) {
+      S w[NW];
+#pragma unroll
+      for (int c = 0; c < NW; ++c) w[c] = c < nw ? W[c * ldw + r] : zero<S>();
+#pragma unroll
+      for (int a = 0; a < JB; ++a) {
+        if (j0 + a < nv) {
+          const S v = V[(int64_t)(j0 + a) * ldv + r];
+#pragma unroll
+          for (int c = 0; c < NW; ++c) acc[a][c] = add(acc[a][c], cj_mul(v, w[c]));
+        }
+      }
+    }
+    // deterministic block tree reduction, one (a, c) pair at a time
+#pragma unroll
+    for (int a = 0; a < JB; ++a) {
+#pragma unroll
+      for (int c = 0; c < NW; ++c) {
+        red[threadIdx.x] = to_c(acc[a][c]);
+        __syncthreads();
+        for (int s = kTB / 2; s > 0; s >>= 1) {
+          if (threadIdx.x < s) red[threadIdx.x] = add(red[threadIdx.x], red[threadIdx.x + s]);
+          __syncthreads();
+        }
+        if (threadIdx.x == 0 && j0 + a < nv)
+          partial[((int64_t)blockIdx.x * nv + (j0 + a)) * NW + c] = red[0];
+        __syncthreads();
+      }
+    }
+  }
+}
+
+template <int NW>
+__global__ void gram_finish(const double2* __restrict__ partial, int nblk, int nv, int nw_real,
+                            double2* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nv * NW) return;
+  const int j = t / NW, c = t % NW;
+  if (c >= nw_real) return;
+  double2 s = make_double2(0.0, 0.0);
+  for (int b = 0; b < nblk; ++b) s = add(s, partial[((int64_t)b * nv + j) * NW + c]);
+  out[(int64_t)j * nw_real + c] = s;
+}
+
+// W[:, c] = beta * W[:, c] + sum_j V_j C[j][c]    (C: nv x nw, row-major, complex)
+template <class S, int NW>
+__global__ void __launch_bounds__(kTB) tall_update(const S* __restrict__ V, int64_t ldv, int nv,
+                                                   const double2* __restrict__ C, int ldc,
+                                                   S* __restrict__ W, int64_t ldw, int nw,
+                                                   int64_t n, int beta) {
+  extern __shared__ double2 cs[];
+  for (int i = threadIdx.x; i < nv * nw; i += blockDim.x) cs[i] = C[(i / nw) * ldc + (i % nw)];
+  __syncthreads();
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    S acc[NW];
+#pragma unroll
+    for (int c = 0; c < NW; ++c) acc[c] = (beta && c < nw) ? W[c * ldw + r] : zero<S>();
+    for (int j = 0; j < nv; ++j) {
+      const S v = V[(int64_t)j * ldv + r];
+#pragma unroll
+      for (int c = 0; c < NW; ++c)
+        if (c < nw) acc[c] = add(acc[c], mul_s(v, cs[j * nw + c]));
+    }
+#pragma unroll
+    for (int c = 0; c < NW; ++c)
+      if (c < nw) W[c * ldw + r] = acc[c];
+  }
+}
+
+// fills x[i] with deterministic pseudo-random values in [-0.5, 0.5)
+__device__ inline double hash_unit(uint64_t x) {
+  x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+}
+template <class S> __device__ inline S rand_val(uint64_t seed, int64_t i);
+template <> __device__ inline double rand_val<double>(uint64_t seed, int64_t i) {
+  return hash_unit(seed * 0x9e3779b97f4a7c15ull + (uint64_t)i);
+}
+template <> __device__ inline double2 rand_val<double2>(uint64_t seed, int64_t i) {
+  return make_double2(hash_unit(seed * 0x9e3779b97f4a7c15ull + 2 * (uint64_t)i),
+                      hash_unit(seed * 0x9e3779b97f4a7c15ull + 2 * (uint64_t)i + 1));
+}
+template <class S>
+__global__ void fill_random(S* __restrict__ x, int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = rand_val<S>(seed, i);
+}
+
+// ---------------------------------------------------------------- host side
+struct Dense {  // small dense complex matrix, row-major
+  int r = 0, c = 0;
+  std::vector<cd> a;
+  Dense() = default;
+  Dense(int rr, int cc) : r(rr), c(cc), a((size_t)rr * cc, cd(0, 0)) {}
+  cd& operator()(int i, int j) { return a[(size_t)i * c + j]; }
+  cd operator()(int i, int j) const { return a[(size_t)i * c + j]; }
+};
+
+// Cyclic Jacobi for a Hermitian matrix: eigenvalues ascending, eigenvectors
+// as the columns of S.
+void hermitian_eig(const Dense& A0, std::vector<double>& w, Dense& S) {
+  const int n = A0.r;
+  Dense A = A0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {  // enforce exact Hermitian symmetry
+      const cd m = 0.5 * (A(i, j) + std::conj(A(j, i)));
+      A(i, j) = m;
+      A(j, i) = std::conj(m);
+    }
+  for (int i = 0; i < n; ++i) A(i, i) = cd(A(i, i).real(), 0.0);
+  S = Dense(n, n);
+  for (int i = 0; i < n; ++i) S(i, i) = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i) {
+      diag += std::norm(A(i, i));
+      for (int j = i + 1; j < n; ++j) off += std::norm(A(i, j));
+    }
+    if (off <= 1e-31 * std::max(diag, 1e-300) || off == 0.0) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double b = std::abs(A(p, q));
+        if (b == 0.0) continue;
+        if (b < 1e-18 * (std::fabs(A(p, p).real()) + std::fabs(A(q, q).real()))) {
+          A(p, q) = 0.0;
+          A(q, p) = 0.0;
+          continue;
+        }
+        const cd ph = A(p, q) / b;  // e^{i phi}
+        const double app = A(p, p).real(), aqq = A(q, q).real();
+        const double th = (aqq - app) / (2.0 * b);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        const cd e = std::conj(ph);  // e^{-i phi}
+        for (int r = 0; r < n; ++r) {
+          if (r == p || r == q) continue;
+          const cd arp = A(r, p), arq = A(r, q);
+          const cd nrp = c * arp - s * e * arq;
+          const cd nrq = s * arp + c * e * arq;
+          A(r, p) = nrp; A(p, r) = std::conj(nrp);
+          A(r, q) = nrq; A(q, r) = std::conj(nrq);
+        }
+        A(p, p) = app - t * b;
+        A(q, q) = aqq + t * b;
+        A(p, q) = 0.0;
+        A(q, p) = 0.0;
+        for (int r = 0; r < n; ++r) {
+          const cd sp = S(r, p), sq = S(r, q);
+          S(r, p) = c * sp - s * e * sq;
+          S(r, q) = s * sp + c * e * sq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int x, int y) { return A(x, x).real() < A(y, y).real(); });
+  Dense S2(n, n);
+  w.assign(n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    w[k] = A(ord[k], ord[k]).real();
+    for (int r = 0; r < n; ++r) S2(r, k) = S(r, ord[k]);
+  }
+  S = S2;
+}
+
+// Cholesky of a small Hermitian positive matrix: G = R^H R, R upper.
+// Returns false if a pivot falls below floor2[c].
+bool cholesky_upper(const Dense& G, Dense& R, const std::vector<double>& floor2) {
+  const int n = G.r;
+  R = Dense(n, n);
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i <= j; ++i) {
+      cd s = G(i, j);
+      for (int k = 0; k < i; ++k) s -= std::conj(R(k, i)) * R(k, j);
+      if (i == j) {
+        const double d = s.real();
+        if (!(d > floor2[j])) return false;
+        R(j, j) = std::sqrt(d);
+      } else {
+        R(i, j) = s / R(i, i).real();
+      }
+    }
+  }
+  return true;
+}
+
+Dense upper_inverse(const Dense& R) {
+  const int n = R.r;
+  Dense X(n, n);
+  for (int j = 0; j < n; ++j) {
+    X(j, j) = 1.0 / R(j, j);
+    for (int i = j - 1; i >= 0; --i) {
+      cd s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += R(i, k) * X(k, j);
+      X(i, j) = -s / R(i, i);
+    }
+  }
+  return X;
+}
+
+Dense matmul(const Dense& A, const Dense& B) {
+  Dense C(A.r, B.c);
+  for (int i = 0; i < A.r; ++i)
+    for (int k = 0; k < A.c; ++k) {
+      const cd a = A(i, k);
+      if (a == cd(0, 0)) continue;
+      for (int j = 0; j < B.c; ++j) C(i, j) += a * B(k, j);
+    }
+  return C;
+}
+
+template <class S>
+struct Solver {
+  Csr& h;
+  hipStream_t st;
+  int precision;
+  int64_t N;
+  int b;       // block size
+  int M;       // basis capacity in vectors (multiple of b)
+  DevBuf V, W, Tmp, part, gdev, cdev, shadow;
+  int64_t mv64 = 0, mv32 = 0;
+  uint64_t seed = 12345;
+
+  Solver(Csr& hh, hipStream_t s, int prec) : h(hh), st(s), precision(prec), N(hh.n_rows) {}
+
+  S* col(int j) { return V.as<S>() + (int64_t)j * N; }
+
+  int setup() {
+    EDLGT_TRY(V.alloc((size_t)(M + b) * N * sizeof(S)));
+    EDLGT_TRY(W.alloc((size_t)b * N * sizeof(S)));
+    EDLGT_TRY(Tmp.alloc((size_t)std::max(M, 8) * N * sizeof(S)));
+    EDLGT_TRY(part.alloc((size_t)kGramBlocks * (M + b) * 8 * sizeof(double2)));
+    EDLGT_TRY(gdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
+    EDLGT_TRY(cdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
+    if (precision == EDLGT_PREC_MIXED) {
+      EDLGT_TRY(csr_make_low_precision(h, st));
+      EDLGT_TRY(shadow.alloc((size_t)b * N * sizeof(S) / 2));
+    }
+    return EDLGT_OK;
+  }
+
+  // G (nv x nw) = V[:, j0:j0+nv]^H X[:, 0:nw]   (X with leading dimension N)
+  int gram(const S* Vp, int nv, const S* X, int nw, Dense& G) {
+    G = Dense(nv, nw);
+    if (nv == 0 || nw == 0) return EDLGT_OK;
+    int nblk = (int)std::min<int64_t>(kGramBlocks, std::max<int64_t>(1, (N + 2047) / 2048));
+#define GRAM_CASE(NWC)                                                                          \
+  case NWC:                                                                                     \
+    hipLaunchKernelGGL((gram_partial<S, NWC>), dim3(nblk), dim3(kTB), 0, st, Vp, N, nv, X, N, nw, \
+                       N, part.as<double2>());                                                   \
+    hipLaunchKernelGGL((gram_finish<NWC>), dim3((nv * NWC + 255) / 256), dim3(256), 0, st,        \
+                       part.as<double2>(), nblk, nv, nw, gdev.as<double2>());                    \
+    break;
+    int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : 8;
+    if (nw > 8) return set_error(EDLGT_ERR_INVALID_ARG, "block size above 8");
+    switch (nwp) { GRAM_CASE(1) GRAM_CASE(2) GRAM_CASE(4) GRAM_CASE(8) }
+#undef GRAM_CASE
+    EDLGT_HIP(hipGetLastError());
+    std::vector<double2> hbuf((size_t)nv * nw);
+    EDLGT_HIP(hipMemcpyAsync(hbuf.data(), gdev.p, hbuf.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+    EDLGT_HIP(hipStreamSynchronize(st));
+    for (int j = 0; j < nv; ++j)
+      for (int c = 0; c < nw; ++c) G(j, c) = cd(hbuf[(size_t)j * nw + c].x, hbuf[(size_t)j * nw + c].y);
+    return EDLGT_OK;
+  }
+
+  // X[:, 0:nw] = beta X + Vp[:, 0:nv] C      (C: nv x nw)
+  int update(const S* Vp, int nv, const Dense& C, S* X, int nw, int beta) {
+    if (nw == 0) return EDLGT_OK;
+    if (nv == 0 && beta) return EDLGT_OK;
+    std::vector<double2> hc((size_t)std::max(nv, 1) * nw);
+    for (int j = 0; j < nv; ++j)
+      for (int c = 0; c < nw; ++c) hc[(size_t)j * nw + c] = make_double2(C(j, c).real(), C(j, c).imag());
+    EDLGT_HIP(hipMemcpyAsync(cdev.p, hc.data(), hc.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    const size_t shmem = (size_t)std::max(nv, 1) * nw * sizeof(double2);
+    if (shmem > 60000) {
+      // split the coefficient rows: X = beta X + V[:, :h] C[:h] + V[:, h:] C[h:]
+      const int half = nv / 2;
+      Dense C1(half, nw), C2(nv - half, nw);
+      for (int j = 0; j < nv; ++j)
+        for (int c = 0; c < nw; ++c) (j < half ? C1(j, c) : C2(j - half, c)) = C(j, c);
+      EDLGT_HIP(hipStreamSynchronize(st));
+      EDLGT_TRY(update(Vp, half, C1, X, nw, beta));
+      return update(Vp + (int64_t)half * N, nv - half, C2, X, nw, 1);
+    }
+    const unsigned g = grid_for(N, kTB, 4096);
+#define UPD_CASE(NWC)                                                                            \
+  case NWC:                                                                                      \
+    hipLaunchKernelGGL((tall_update<S, NWC>), dim3(g), dim3(kTB), shmem, st, Vp, N, nv,           \
+                       cdev.as<double2>(), nw, X, N, nw, N, beta);                                \
+    break;
+    if (nw > 8) return set_error(EDLGT_ERR_INVALID_ARG, "update width above 8");
+    int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : 8;
+    switch (nwp) { UPD_CASE(1) UPD_CASE(2) UPD_CASE(4) UPD_CASE(8) }
+#undef UPD_CASE
+    EDLGT_HIP(hipGetLastError());
+    EDLGT_HIP(hipStreamSynchronize(st));  // cdev is reused by the next call
+    return EDLGT_OK;
+  }
+
+  int matvec(const S* X, S* Y, int nw) {
+    const int vt = sizeof(S) == 16 ? EDLGT_VALUE_C128 : EDLGT_VALUE_F64;
+    if (precision == EDLGT_PREC_MIXED) {
+      const int64_t ncomp = (int64_t)nw * N * (sizeof(S) / 8);
+      EDLGT_TRY(convert_f64_to_f32(reinterpret_cast<const double*>(X), shadow.as<float>(), ncomp, st));
+      const int lt = sizeof(S) == 16 ? EDLGT_VALUE_C64 : EDLGT_VALUE_F32;
+      EDLGT_TRY(csr_spmm(h, EDLGT_PREC_MIXED, lt, nw, shadow.p, N, Y, N, st));
+      mv32 += nw;
+    } else {
+      EDLGT_TRY(csr_spmm(h, EDLGT_PREC_FP64, vt, nw, X, N, Y, N, st));
+      mv64 += nw;
+    }
+    return EDLGT_OK;
+  }
+
+  int randomize(S* X, int64_t count) {
+    hipLaunchKernelGGL(fill_random<S>, dim3(grid_for(count, 256, 4096)), dim3(256), 0, st, X, count, seed++);
+    EDLGT_HIP(hipGetLastError());
+    return EDLGT_OK;
+  }
+
+  // Orthonormalises X (nw columns) against V[:, :nv] and within itself.
+  // On return X = Q, and Rout (nw x nw) with (X_in - V G) = Q Rout; columns
+  // that were (numerically) dependent are replaced by random directions and
+  // get zero rows in Rout.  Gout receives the V-coefficients.
+  int orthonormalize(int nv, S* X, int nw, Dense& Gout, Dense& Rout) {
+    Dense G0;
+    EDLGT_TRY(gram(X, nw, X, nw, G0));
+    std::vector<double> n0(nw);
+    for (int c = 0; c < nw; ++c) n0[c] = std::max(G0(c, c).real(), 0.0);
+    Dense G1, G2;
+    Gout = Dense(nv, nw);
+    for (int pass = 0; pass < 2 && nv > 0; ++pass) {
+      Dense G;
+      EDLGT_TRY(gram(V.as<S>(), nv, X, nw, G));
+      Dense mG = G;
+      for (auto& z : mG.a) z = -z;
+      EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1));
+      for (size_t i = 0; i < G.a.size(); ++i) Gout.a[i] += G.a[i];
+    }
+    // Cholesky QR, twice.
+    std::vector<double> floor2(nw);
+    for (int c = 0; c < nw; ++c) floor2[c] = 1e-20 * std::max(n0[c], 1e-280);
+    Dense Gq, R1;
+    Dense Rpre(nw, nw);
+    for (int c = 0; c < nw; ++c) Rpre(c, c) = 1.0;
+    EDLGT_TRY(gram(X, nw, X, nw, Gq));
+    if (cholesky_upper(Gq, R1, floor2)) {
+      Dense R1i = upper_inverse(R1);
+      EDLGT_TRY(update(X, nw, R1i, Tmp.as<S>(), nw, 0));
+      EDLGT_HIP(hipMemcpyAsync(X, Tmp.p, (size_t)nw * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+      Rpre = R1;
+      for (int c = 0; c < nw; ++c) n0[c] = 1.0;  // X now has unit-scale columns
+      Dense Gq2, R2;
+      std::vector<double> fl2(nw, 0.25);
+      EDLGT_TRY(gram(X, nw, X, nw, Gq2));
+      if (cholesky_upper(Gq2, R2, fl2)) {
+        Dense R2i = upper_inverse(R2);
+        EDLGT_TRY(update(X, nw, R2i, Tmp.as<S>(), nw, 0));
+        EDLGT_HIP(hipMemcpyAsync(X, Tmp.p, (size_t)nw * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+        Rout = matmul(R2, R1);
+        return EDLGT_OK;
+      }
+    }
+    // Column-by-column repair (rank loss).
+    Rout = Dense(nw, nw);
+    for (int c = 0; c < nw; ++c) {
+      S* xc = X + (int64_t)c * N;
+      // against previously accepted columns of X
+      for (int pass = 0; pass < 2 && c > 0; ++pass) {
+        Dense g;
+        EDLGT_TRY(gram(X, c, xc, 1, g));
+        Dense mg = g;
+        for (auto& z : mg.a) z = -z;
+        EDLGT_TRY(update(X, c, mg, xc, 1, 1));
+        for (int i = 0; i < c; ++i) Rout(i, c) += g(i, 0);
+      }
+      Dense nn;
+      EDLGT_TRY(gram(xc, 1, xc, 1, nn));
+      const double nrm = std::sqrt(std::max(nn(0, 0).real(), 0.0));
+      if (nrm > 1e-10 * std::sqrt(std::max(n0[c], 1e-280))) {
+        Dense sc(1, 1);
+        sc(0, 0) = 1.0 / nrm;
+        EDLGT_TRY(update(xc, 1, sc, Tmp.as<S>(), 1, 0));
+        EDLGT_HIP(hipMemcpyAsync(xc, Tmp.p, (size_t)N * sizeof(S), hipMemcpyDeviceToDevice, st));
+        Rout(c, c) = nrm;
+      } else {
+        if (nv + c + 1 > N) return set_error(EDLGT_ERR_INVALID_ARG, "basis exceeds the space");
+        EDLGT_TRY(randomize(xc, N));
+        for (int pass = 0; pass < 2; ++pass) {
+          Dense g;
+          if (nv > 0) {
+            EDLGT_TRY(gram(V.as<S>(), nv, xc, 1, g));
+            for (auto& z : g.a) z = -z;
+            EDLGT_TRY(update(V.as<S>(), nv, g, xc, 1, 1));
+          }
+          if (c > 0) {
+            EDLGT_TRY(gram(X, c, xc, 1, g));
+            for (auto& z : g.a) z = -z;
+            EDLGT_TRY(update(X, c, g, xc, 1, 1));
+          }
+        }
+        EDLGT_TRY(gram(xc, 1, xc, 1, nn));
+        Dense sc(1, 1);
+        sc(0, 0) = 1.0 / std::sqrt(nn(0, 0).real());
+        EDLGT_TRY(update(xc, 1, sc, Tmp.as<S>(), 1, 0));
+        EDLGT_HIP(hipMemcpyAsync(xc, Tmp.p, (size_t)N * sizeof(S), hipMemcpyDeviceToDevice, st));
+        Rout(c, c) = 0.0;
+      }
+    }
+    Rout = matmul(Rout, Rpre);
+    return EDLGT_OK;
+  }
+
+  // Runs thick-restart block Lanczos from the b start columns already in
+  // V[:, 0:b] (not necessarily orthonormal).  Fills theta (k values), and the
+  // Ritz vectors Y (k columns) into Tmp[:, 0:k]; resid gets the estimates.
+  int run(int k, double tol, int64_t max_mv, std::vector<double>& theta, std::vector<double>& est,
+          bool& converged) {
+    Dense Gd, R;
+    EDLGT_TRY(orthonormalize(0, col(0), b, Gd, R));
+    int nv = b;
+    int l = 0;
+    Dense T(M + b, M + b);
+    converged = false;
+    while (true) {
+      // expand until full
+      const int a0 = nv - b;
+      EDLGT_TRY(matvec(col(a0), W.as<S>(), b));
+      Dense G, Rb;
+      if (nv >= N) {
+        // the basis spans the whole space: T = V^H H V is exact, no residual
+        EDLGT_TRY(gram(V.as<S>(), nv, W.as<S>(), b, G));
+        Rb = Dense(b, b);
+      } else {
+        EDLGT_TRY(orthonormalize(nv, W.as<S>(), b, G, Rb));
+      }
+      for (int i = 0; i < nv; ++i)
+        for (int c = 0; c < b; ++c) {
+          T(i, a0 + c) = G(i, c);
+          T(a0 + c, i) = std::conj(G(i, c));
+        }
+      const bool full = nv + b > M || nv + b > N;
+      // residual block Q -> V[:, nv:nv+b] (capacity M + b)
+      EDLGT_HIP(hipMemcpyAsync(col(nv), W.p, (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+      if (!full) {
+        for (int c = 0; c < b; ++c)
+          for (int i = 0; i < b; ++i) {
+            T(nv + i, a0 + c) = Rb(i, c);
+            T(a0 + c, nv + i) = std::conj(Rb(i, c));
+          }
+        nv += b;
+        continue;
+      }
+      // Rayleigh-Ritz
+      Dense Tn(nv, nv);
+      for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j) Tn(i, j) = T(i, j);
+      std::vector<double> w;
+      Dense Sv;
+      hermitian_eig(Tn, w, Sv);
+      const bool exhausted = nv >= N;
+      theta.assign(w.begin(), w.begin() + k);
+      est.assign(k, 0.0);
+      bool ok = true;
+      for (int i = 0; i < k; ++i) {
+        double s2 = 0.0;
+        if (!exhausted) {
+          for (int c = 0; c < b; ++c) {
+            cd z = 0.0;
+            for (int q = 0; q < b; ++q) z += Rb(c, q) * Sv(a0 + q, i);
+            s2 += std::norm(z);
+          }
+        }
+        est[i] = std::sqrt(s2);
+        if (est[i] > tol * std::max(1.0, std::fabs(w[i]))) ok = false;
+      }
+      const int64_t mv = mv64 + mv32;
+      if (ok || exhausted || mv >= max_mv) {
+        converged = ok || exhausted;
+        Dense Sk(nv, k);
+        for (int j = 0; j < nv; ++j)
+          for (int i = 0; i < k; ++i) Sk(j, i) = Sv(j, i);
+        EDLGT_TRY(update_blocked(col(0), nv, Sk, Tmp.as<S>(), k));
+        return EDLGT_OK;
+      }
+      // thick restart: keep the lowest l Ritz vectors and the residual block
+      l = std::max(k, std::min(M - 2 * b, k + (M - k) / 2));
+      l = std::min(l, nv - b);
+      Dense Sl(nv, l);
+      for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < l; ++i) Sl(j, i) = Sv(j, i);
+      EDLGT_TRY(update_blocked(col(0), nv, Sl, Tmp.as<S>(), l));
+      EDLGT_HIP(hipMemcpyAsync(col(0), Tmp.p, (size_t)l * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+      EDLGT_HIP(hipMemcpyAsync(col(l), col(nv), (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+      for (auto& z : T.a) z = 0.0;
+      for (int i = 0; i < l; ++i) T(i, i) = w[i];
+      for (int c = 0; c < b; ++c)
+        for (int i = 0; i < l; ++i) {
+          cd z = 0.0;
+          for (int q = 0; q < b; ++q) z += Rb(c, q) * Sv(a0 + q, i);
+          T(l + c, i) = z;
+          T(i, l + c) = std::conj(z);
+        }
+      nv = l + b;
+    }
+  }
+
+  // X[:, 0:nc] = V[:, 0:nv] C for any nc (chunks of 8 columns).
+  int update_blocked(const S* Vp, int nv, const Dense& C, S* X, int nc) {
+    for (int c0 = 0; c0 < nc; c0 += 8) {
+      const int w = std::min(8, nc - c0);
+      Dense Cc(nv, w);
+      for (int j = 0; j < nv; ++j)
+        for (int c = 0; c < w; ++c) Cc(j, c) = C(j, c0 + c);
+      EDLGT_TRY(update(Vp, nv, Cc, X + (int64_t)c0 * N, w, 0));
+    }
+    return EDLGT_OK;
+  }
+
+  // True residual norms ||H y_i - theta_i y_i|| in FP64 for Y = Tmp[:, 0:k].
+  int residuals(int k, const std::vector<double>& theta, std::vector<double>& out) {
+    out.assign(k, 0.0);
+    const int saved = precision;
+    precision = EDLGT_PREC_FP64;
+    for (int i = 0; i < k; ++i) {
+      S* y = Tmp.as<S>() + (int64_t)i * N;
+      EDLGT_TRY(matvec(y, W.as<S>(), 1));
+      Dense c(1, 1);
+      c(0, 0) = -theta[i];
+      EDLGT_TRY(update(y, 1, c, W.as<S>(), 1, 1));
+      Dense nn;
+      EDLGT_TRY(gram(W.as<S>(), 1, W.as<S>(), 1, nn));
+      out[i] = std::sqrt(std::max(nn(0, 0).real(), 0.0));
+    }
+    precision = saved;
+    return EDLGT_OK;
+  }
+};
+
+template <class S>
+int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, int precision,
+                  const void* v0_host, double* evals, void* evecs, double* resid, int64_t* matvecs,
+                  hipStream_t st) {
+  const int64_t N = h.n_rows;
+  if (k < 1 || k > N) return set_error(EDLGT_ERR_INVALID_ARG, "n_eigs must be in [1, n]");
+  Solver<S> sv(h, st, precision);
+  // Block size: up to 4 so that small degeneracies are resolved; a basis
+  // that can hold the whole space uses single vectors and runs to exhaustion.
+  int b = std::min(k, 4);
+  int M = std::max(max_basis, 2 * k + 2 * b);
+  M = (M + b - 1) / b * b;
+  if (N <= 512 && M >= N) {
+    // the basis can hold the whole space: single vectors, run to exhaustion
+    b = 1;
+    M = (int)N;
+  } else if (M > N / 2) {
+    M = (int)std::max<int64_t>(2 * b, (N / 2) / b * b);
+  }
+  sv.b = b;
+  sv.M = M;
+  EDLGT_TRY(sv.setup());
+  // start block: caller's v0 in column 0, random columns after it
+  if (v0_host) {
+    EDLGT_HIP(hipMemcpyAsync(sv.col(0), v0_host, (size_t)N * sizeof(S), hipMemcpyHostToDevice, st));
+  } else {
+    EDLGT_TRY(sv.randomize(sv.col(0), N));
+  }
+  if (b > 1) EDLGT_TRY(sv.randomize(sv.col(1), (int64_t)(b - 1) * N));
+  std::vector<double> theta, est;
+  bool conv = false;
+  if (precision == EDLGT_PREC_MIXED) {
+    // FP32 phase to a loose tolerance, then FP64 from its Ritz block
+    const double tol32 = std::max(tol, 2e-6);
+    EDLGT_TRY(sv.run(k, tol32, max_iter, theta, est, conv));
+    const int nk = std::min(k, b);
+    EDLGT_HIP(hipMemcpyAsync(sv.col(0), sv.Tmp.p, (size_t)nk * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+    if (nk < b) EDLGT_TRY(sv.randomize(sv.col(nk), (int64_t)(b - nk) * N));
+    if (k > b) {
+      // more wanted vectors than the block: fold the rest into the block
+      Dense c(1, 1);
+      c(0, 0) = 1.0;
+      for (int i = b; i < k; ++i)
+        EDLGT_TRY(sv.update(sv.Tmp.template as<S>() + (int64_t)i * N, 1, c, sv.col(i % b), 1, 1));
+    }
+    sv.precision = EDLGT_PREC_FP64;
+  }
+  EDLGT_TRY(sv.run(k, tol, max_iter, theta, est, conv));
+  for (int i = 0; i < k; ++i) evals[i] = theta[i];
+  if (evecs)
+    EDLGT_HIP(hipMemcpyAsync(evecs, sv.Tmp.p, (size_t)k * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+  if (resid) {
+    std::vector<double> r;
+    EDLGT_TRY(sv.residuals(k, theta, r));
+    for (int i = 0; i < k; ++i) resid[i] = r[i];
+  }
+  if (matvecs) {
+    matvecs[0] = sv.mv64;
+    matvecs[1] = sv.mv32;
+  }
+  EDLGT_HIP(hipStreamSynchronize(st));
+  if (!conv) return set_error(EDLGT_ERR_NOT_CONVERGED, "Lanczos did not converge within max_iter");
+  return EDLGT_OK;
+}
+
+}  // namespace
+
+int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter, int precision,
+                int vec_type, const void* v0_host, double* evals, void* evecs, double* resid,
+                int64_t* matvecs, hipStream_t st) {
+  if (vec_type == EDLGT_VALUE_C128)
+    return lanczos_typed<double2>(h, n_eigs, tol, max_basis, max_iter, precision, v0_host, evals,
+                                  evecs, resid, matvecs, st);
+  if (vec_type == EDLGT_VALUE_F64) {
+    if (h.value_type != EDLGT_VALUE_F64)
+      return set_error(EDLGT_ERR_INVALID_ARG, "real vectors need a real matrix");
+    return lanczos_typed<double>(h, n_eigs, tol, max_basis, max_iter, precision, v0_host, evals,
+                                 evecs, resid, matvecs, st);
+  }
+  return set_error(EDLGT_ERR_INVALID_ARG, "vec_type must be C128 or F64");
+}
+
+}  // namespace edlgt

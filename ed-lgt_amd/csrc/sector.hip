@@ -1,0 +1,381 @@
+// Symmetry-sector enumeration on the device.
+//
+// The sector is the set of configurations c (c_s < loc_dims[s]) that satisfy
+// the global / link / n-body constraints, in lexicographic order with site 0
+// most significant (the row order of the reference's sector_configs,
+// edlgt/symmetries/symmetry_sector.py:785-967 and :1329-1469).
+//
+// Design (MI355X):
+//  1. Breadth-first frontier of valid prefixes.  A prefix of depth p is its
+//     mixed-radix index over sites [0, p).  Expanding a frontier by one site
+//     launches one thread per candidate (prefix*d_p + s), checks only the
+//     constraints that become decidable at site p (U(1) reachability with
+//     suffix min/max bounds, link pairs and n-body groups whose last site is
+//     p), and compacts the survivors with a flag scan.  The expansion keeps
+//     prefix order, so the frontier stays sorted.
+//  2. Once the frontier holds enough prefixes to fill the chip (or the next
+//     candidate set would be too large), every frontier prefix is completed
+//     by one thread with an iterative depth-first search over the remaining
+//     sites, in lexicographic order: a count pass, an int64 scan of the
+//     counts, and a fill pass that writes keys and configuration bytes at
+//     the scanned offsets.  Output order = (prefix order, DFS order) =
+//     lexicographic order, with no global sort.
+// Integer output is deterministic and independent of launch shapes.
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+
+namespace edlgt {
+namespace {
+
+constexpr int kMaxSites = 48;
+constexpr int kMaxGlob = 4;
+constexpr double kSymTol = 1e-13;  // symmetry_sector.py:17
+
+struct SectorDev {
+  int n_sites, d_max;
+  int n_glob, glob_type;
+  const double* glob_diags;    // [n_glob][n_sites][d_max]
+  const double* glob_sectors;  // [n_glob]
+  const double* rem_min;       // [n_glob][n_sites+1]  (U1 suffix bounds)
+  const double* rem_max;
+  const int32_t* act_link_off; // [n_sites+1]
+  const int32_t* act_link;     // triples (dir, i, j) activated at their max site
+  const double* link_diags;    // [n_dirs][2][n_sites][d_max]
+  const double* link_sectors;  // [n_dirs]
+  const int32_t* act_nb_off;   // [n_sites+1]
+  const int32_t* act_nb;       // n-body ids activated at their max site
+  int nbody_type;
+  const double* nbody_diags;   // [n_nb][n_sites][d_max]
+  const double* nbody_sectors;
+  const int32_t* nb_site_off;  // [n_nb+1]
+  const int32_t* nb_sites;
+  const int32_t* loc_dims;     // [n_sites]
+  const uint64_t* strides;     // [n_sites]
+};
+
+// Checks every constraint that becomes decidable when site s is assigned.
+// gin: global partial values of sites [0, s); gout receives those of [0, s].
+__device__ inline bool check_site(const SectorDev& D, int s, const uint8_t* dig,
+                                  const double* gin, double* gout) {
+  const int n = D.n_sites;
+  const int a = dig[s];
+  for (int g = 0; g < D.n_glob; ++g) {
+    const double v = D.glob_diags[((size_t)g * n + s) * D.d_max + a];
+    if (D.glob_type == EDLGT_SYM_U1) {
+      const double nv = (s == 0 ? 0.0 : gin[g]) + v;
+      gout[g] = nv;
+      const double t = D.glob_sectors[g];
+      if (s < n - 1) {
+        const double lo = nv + D.rem_min[g * (n + 1) + s + 1];
+        const double hi = nv + D.rem_max[g * (n + 1) + s + 1];
+        if (t < lo - kSymTol || t > hi + kSymTol) return false;
+      } else if (fabs(nv - t) > kSymTol) {
+        return false;
+      }
+    } else {
+      const double nv = (s == 0 ? 1.0 : gin[g]) * v;
+      gout[g] = nv;
+      if (s == n - 1 && fabs(nv - D.glob_sectors[g]) > kSymTol) return false;
+    }
+  }
+  for (int k = D.act_link_off[s]; k < D.act_link_off[s + 1]; ++k) {
+    const int dir = D.act_link[3 * k], i = D.act_link[3 * k + 1], j = D.act_link[3 * k + 2];
+    const double v = D.link_diags[(((size_t)dir * 2 + 0) * n + i) * D.d_max + dig[i]] +
+                     D.link_diags[(((size_t)dir * 2 + 1) * n + j) * D.d_max + dig[j]];
+    if (fabs(v - D.link_sectors[dir]) > kSymTol) return false;
+  }
+  for (int k = D.act_nb_off[s]; k < D.act_nb_off[s + 1]; ++k) {
+    const int id = D.act_nb[k];
+    double v = D.nbody_type == EDLGT_SYM_U1 ? 0.0 : 1.0;
+    for (int q = D.nb_site_off[id]; q < D.nb_site_off[id + 1]; ++q) {
+      const int site = D.nb_sites[q];
+      const double x = D.nbody_diags[((size_t)id * n + site) * D.d_max + dig[site]];
+      v = D.nbody_type == EDLGT_SYM_U1 ? v + x : v * x;
+    }
+    if (fabs(v - D.nbody_sectors[id]) > kSymTol) return false;
+  }
+  return true;
+}
+
+// Decodes a depth-p prefix index into digits [0, p).
+__device__ inline void decode_prefix(const SectorDev& D, uint64_t idx, int p, uint8_t* dig) {
+  for (int s = p - 1; s >= 0; --s) {
+    const uint64_t d = (uint64_t)D.loc_dims[s];
+    dig[s] = (uint8_t)(idx % d);
+    idx /= d;
+  }
+}
+
+// Re-accumulates the global partial values of a (valid) prefix of depth p.
+__device__ inline void prefix_globals(const SectorDev& D, const uint8_t* dig, int p, double* g) {
+  for (int k = 0; k < D.n_glob; ++k) {
+    double v = D.glob_type == EDLGT_SYM_U1 ? 0.0 : 1.0;
+    for (int s = 0; s < p; ++s) {
+      const double x = D.glob_diags[((size_t)k * D.n_sites + s) * D.d_max + dig[s]];
+      v = D.glob_type == EDLGT_SYM_U1 ? v + x : v * x;
+    }
+    g[k] = v;
+  }
+}
+
+// One thread per candidate prefix*d_p + s of the next frontier level.
+__global__ void expand_flags(SectorDev D, const uint64_t* __restrict__ frontier, int64_t n_front,
+                             int p, int32_t* __restrict__ flags) {
+  const int64_t dp = D.loc_dims[p];
+  const int64_t n_cand = n_front * dp;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_cand;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = t / dp;
+    const int s = (int)(t - f * dp);
+    uint8_t dig[kMaxSites];
+    decode_prefix(D, frontier[f], p, dig);
+    dig[p] = (uint8_t)s;
+    double gin[kMaxGlob], gout[kMaxGlob];
+    prefix_globals(D, dig, p, gin);
+    flags[t] = check_site(D, p, dig, gin, gout) ? 1 : 0;
+  }
+}
+
+__global__ void expand_compact(const uint64_t* __restrict__ frontier, int64_t n_front, int64_t dp,
+                               const int32_t* __restrict__ flags, const int64_t* __restrict__ pos,
+                               uint64_t* __restrict__ next) {
+  const int64_t n_cand = n_front * dp;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_cand;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (flags[t]) {
+      const int64_t f = t / dp;
+      next[pos[t]] = frontier[f] * (uint64_t)dp + (uint64_t)(t - f * dp);
+    }
+  }
+}
+
+// Depth-first completion of one frontier prefix.  kFill=false counts the
+// completions, kFill=true writes them (keys + configuration bytes) from
+// offsets[f] on, in lexicographic order.
+template <bool kFill>
+__global__ void complete_prefixes(SectorDev D, const uint64_t* __restrict__ frontier,
+                                  int64_t n_front, int p, int64_t* __restrict__ counts,
+                                  const int64_t* __restrict__ offsets, uint64_t* __restrict__ keys,
+                                  uint8_t* __restrict__ configs, int cfg_stride) {
+  const int n = D.n_sites;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n_front;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    uint8_t dig[kMaxSites];
+    double gsum[kMaxSites + 1][kMaxGlob];  // gsum[s] = globals of sites [0, s)
+    decode_prefix(D, frontier[f], p, dig);
+    prefix_globals(D, dig, p, gsum[p]);
+    uint64_t pkey = 0;
+    for (int s = 0; s < p; ++s) pkey += (uint64_t)dig[s] * D.strides[s];
+    int64_t count = 0;
+    int64_t out = kFill ? offsets[f] : 0;
+    if (p == n) {
+      // The prefix already is a full configuration (validated by expansion).
+      count = 1;
+      if (kFill) {
+        keys[out] = pkey;
+        for (int s = 0; s < n; ++s) configs[out * cfg_stride + s] = dig[s];
+      }
+    } else {
+      uint64_t kpart[kMaxSites + 1];
+      kpart[p] = pkey;
+      int s = p;
+      int cur = -1;
+      while (true) {
+        ++cur;
+        if (cur >= D.loc_dims[s]) {
+          if (s == p) break;
+          --s;
+          cur = dig[s];
+          continue;
+        }
+        dig[s] = (uint8_t)cur;
+        if (!check_site(D, s, dig, gsum[s], gsum[s + 1])) continue;
+        const uint64_t k = kpart[s] + (uint64_t)cur * D.strides[s];
+        if (s == n - 1) {
+          if (kFill) {
+            keys[out] = k;
+            uint8_t* row = configs + out * cfg_stride;
+            for (int q = 0; q < n; ++q) row[q] = dig[q];
+            ++out;
+          }
+          ++count;
+        } else {
+          kpart[s + 1] = k;
+          ++s;
+          cur = -1;
+        }
+      }
+    }
+    if (!kFill) counts[f] = count;
+  }
+}
+
+}  // namespace
+
+int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_t n_glob,
+                 int32_t glob_type, const double* glob_diags, const double* glob_sectors,
+                 int32_t n_dirs, const double* link_diags, const double* link_sectors,
+                 int32_t n_pairs, const int32_t* pair_dir, const int32_t* pair_site0,
+                 const int32_t* pair_site1, int32_t n_nbody, int32_t nbody_type,
+                 const double* nbody_diags, const double* nbody_sectors,
+                 const int32_t* nb_site_off_h, const int32_t* nb_sites_h, hipStream_t st,
+                 Basis* out) {
+  if (n_sites < 1 || n_sites > kMaxSites)
+    return set_error(EDLGT_ERR_INVALID_ARG, "n_sites must be in [1, 48]");
+  if (d_max < 1 || n_glob < 0 || n_glob > kMaxGlob || n_pairs < 0 || n_nbody < 0 || n_dirs < 0)
+    return set_error(EDLGT_ERR_INVALID_ARG, "invalid constraint counts");
+  if (n_glob && (glob_type != EDLGT_SYM_U1 && glob_type != EDLGT_SYM_Z2))
+    return set_error(EDLGT_ERR_INVALID_ARG, "glob_type must be U1 or Z2");
+  // Mixed-radix strides (last site fastest); the key range must fit 64 bits.
+  std::vector<uint64_t> strides(n_sites);
+  unsigned __int128 prodd = 1;
+  for (int s = n_sites - 1; s >= 0; --s) {
+    if (loc_dims[s] < 1 || loc_dims[s] > 255 || loc_dims[s] > d_max)
+      return set_error(EDLGT_ERR_INVALID_ARG, "loc_dims must be in [1, min(255, d_max)]");
+    strides[s] = (uint64_t)prodd;
+    prodd *= (unsigned)loc_dims[s];
+    if (prodd > (unsigned __int128)UINT64_MAX)
+      return set_error(EDLGT_ERR_INDEX_OVERFLOW, "prod(loc_dims) does not fit 64-bit keys");
+  }
+  // Host-side tables: U(1) suffix bounds, activation lists.
+  std::vector<double> rem_min((size_t)std::max(n_glob, 1) * (n_sites + 1), 0.0),
+      rem_max(rem_min.size(), 0.0);
+  if (glob_type == EDLGT_SYM_U1) {
+    for (int g = 0; g < n_glob; ++g)
+      for (int s = n_sites - 1; s >= 0; --s) {
+        const double* v = glob_diags + ((size_t)g * n_sites + s) * d_max;
+        double mn = v[0], mx = v[0];
+        for (int a = 1; a < loc_dims[s]; ++a) { mn = std::min(mn, v[a]); mx = std::max(mx, v[a]); }
+        rem_min[g * (n_sites + 1) + s] = rem_min[g * (n_sites + 1) + s + 1] + mn;
+        rem_max[g * (n_sites + 1) + s] = rem_max[g * (n_sites + 1) + s + 1] + mx;
+      }
+  }
+  std::vector<std::vector<int32_t>> by_site_link(n_sites), by_site_nb(n_sites);
+  for (int q = 0; q < n_pairs; ++q) {
+    const int d = pair_dir[q], i = pair_site0[q], j = pair_site1[q];
+    if (d < 0 || d >= n_dirs || i < 0 || j < 0 || i >= n_sites || j >= n_sites)
+      return set_error(EDLGT_ERR_INVALID_ARG, "link pair out of range");
+    auto& v = by_site_link[std::max(i, j)];
+    v.push_back(d); v.push_back(i); v.push_back(j);
+  }
+  for (int k = 0; k < n_nbody; ++k) {
+    int mx = -1;
+    for (int q = nb_site_off_h[k]; q < nb_site_off_h[k + 1]; ++q) {
+      if (nb_sites_h[q] < 0 || nb_sites_h[q] >= n_sites)
+        return set_error(EDLGT_ERR_INVALID_ARG, "n-body site out of range");
+      mx = std::max(mx, nb_sites_h[q]);
+    }
+    if (mx >= 0) by_site_nb[mx].push_back(k);
+  }
+  std::vector<int32_t> act_link_off(n_sites + 1, 0), act_link, act_nb_off(n_sites + 1, 0), act_nb;
+  for (int s = 0; s < n_sites; ++s) {
+    act_link.insert(act_link.end(), by_site_link[s].begin(), by_site_link[s].end());
+    act_link_off[s + 1] = (int32_t)(act_link.size() / 3);
+    act_nb.insert(act_nb.end(), by_site_nb[s].begin(), by_site_nb[s].end());
+    act_nb_off[s + 1] = (int32_t)act_nb.size();
+  }
+  // Upload.
+  DevBuf b_gd, b_gs, b_rmin, b_rmax, b_alo, b_al, b_ld, b_ls, b_ano, b_an, b_nd, b_ns, b_nso, b_nsi,
+      b_ldims, b_str;
+  const double zero = 0.0;
+  const int32_t izero = 0;
+  EDLGT_TRY(upload(b_gd, n_glob ? glob_diags : &zero, n_glob ? (size_t)n_glob * n_sites * d_max : 1, st));
+  EDLGT_TRY(upload(b_gs, n_glob ? glob_sectors : &zero, n_glob ? (size_t)n_glob : 1, st));
+  EDLGT_TRY(upload(b_rmin, rem_min.data(), rem_min.size(), st));
+  EDLGT_TRY(upload(b_rmax, rem_max.data(), rem_max.size(), st));
+  EDLGT_TRY(upload(b_alo, act_link_off.data(), act_link_off.size(), st));
+  EDLGT_TRY(upload(b_al, act_link.empty() ? &izero : act_link.data(), std::max<size_t>(act_link.size(), 1), st));
+  EDLGT_TRY(upload(b_ld, n_dirs ? link_diags : &zero, n_dirs ? (size_t)n_dirs * 2 * n_sites * d_max : 1, st));
+  EDLGT_TRY(upload(b_ls, n_dirs ? link_sectors : &zero, n_dirs ? (size_t)n_dirs : 1, st));
+  EDLGT_TRY(upload(b_ano, act_nb_off.data(), act_nb_off.size(), st));
+  EDLGT_TRY(upload(b_an, act_nb.empty() ? &izero : act_nb.data(), std::max<size_t>(act_nb.size(), 1), st));
+  EDLGT_TRY(upload(b_nd, n_nbody ? nbody_diags : &zero, n_nbody ? (size_t)n_nbody * n_sites * d_max : 1, st));
+  EDLGT_TRY(upload(b_ns, n_nbody ? nbody_sectors : &zero, n_nbody ? (size_t)n_nbody : 1, st));
+  EDLGT_TRY(upload(b_nso, n_nbody ? nb_site_off_h : &izero, n_nbody ? (size_t)n_nbody + 1 : 1, st));
+  const size_t n_nbs = n_nbody ? (size_t)nb_site_off_h[n_nbody] : 0;
+  EDLGT_TRY(upload(b_nsi, n_nbs ? nb_sites_h : &izero, std::max<size_t>(n_nbs, 1), st));
+  EDLGT_TRY(upload(b_ldims, loc_dims, n_sites, st));
+  EDLGT_TRY(upload(b_str, strides.data(), strides.size(), st));
+
+  SectorDev D;
+  D.n_sites = n_sites; D.d_max = d_max; D.n_glob = n_glob; D.glob_type = glob_type;
+  D.glob_diags = b_gd.as<double>(); D.glob_sectors = b_gs.as<double>();
+  D.rem_min = b_rmin.as<double>(); D.rem_max = b_rmax.as<double>();
+  D.act_link_off = b_alo.as<int32_t>(); D.act_link = b_al.as<int32_t>();
+  D.link_diags = b_ld.as<double>(); D.link_sectors = b_ls.as<double>();
+  D.act_nb_off = b_ano.as<int32_t>(); D.act_nb = b_an.as<int32_t>();
+  D.nbody_type = nbody_type; D.nbody_diags = b_nd.as<double>(); D.nbody_sectors = b_ns.as<double>();
+  D.nb_site_off = b_nso.as<int32_t>(); D.nb_sites = b_nsi.as<int32_t>();
+  D.loc_dims = b_ldims.as<int32_t>(); D.strides = b_str.as<uint64_t>();
+
+  // Phase 1: breadth-first frontier.
+  const int64_t kTargetFrontier = 1 << 18;
+  const int64_t kMaxCandidates = (int64_t)1 << 27;
+  DevBuf front;
+  EDLGT_TRY(front.alloc(sizeof(uint64_t)));
+  EDLGT_HIP(hipMemsetAsync(front.p, 0, sizeof(uint64_t), st));
+  int64_t n_front = 1;
+  int p = 0;
+  while (p < n_sites && n_front > 0) {
+    const int64_t n_cand = n_front * loc_dims[p];
+    if (p > 0 && (n_front >= kTargetFrontier || n_cand > kMaxCandidates)) break;
+    DevBuf flags, pos, next;
+    EDLGT_TRY(flags.alloc(n_cand * sizeof(int32_t)));
+    EDLGT_TRY(pos.alloc((n_cand + 1) * sizeof(int64_t)));
+    hipLaunchKernelGGL(expand_flags, dim3(grid_for(n_cand, 256)), dim3(256), 0, st, D,
+                       front.as<uint64_t>(), n_front, p, flags.as<int32_t>());
+    EDLGT_HIP(hipGetLastError());
+    EDLGT_TRY(exclusive_scan_i32_i64(flags.as<int32_t>(), pos.as<int64_t>(), n_cand, st));
+    int64_t n_next = 0;
+    EDLGT_HIP(hipMemcpyAsync(&n_next, pos.as<int64_t>() + n_cand, sizeof(int64_t),
+                             hipMemcpyDeviceToHost, st));
+    EDLGT_HIP(hipStreamSynchronize(st));
+    EDLGT_TRY(next.alloc(std::max<int64_t>(n_next, 1) * sizeof(uint64_t)));
+    hipLaunchKernelGGL(expand_compact, dim3(grid_for(n_cand, 256)), dim3(256), 0, st,
+                       front.as<uint64_t>(), n_front, (int64_t)loc_dims[p], flags.as<int32_t>(),
+                       pos.as<int64_t>(), next.as<uint64_t>());
+    EDLGT_HIP(hipGetLastError());
+    EDLGT_HIP(hipStreamSynchronize(st));
+    front = std::move(next);
+    n_front = n_next;
+    ++p;
+  }
+
+  out->n_sites = n_sites;
+  out->cfg_stride = (n_sites + 3) & ~3;
+  out->loc_dims.assign(loc_dims, loc_dims + n_sites);
+  out->strides = strides;
+  if (n_front == 0) {
+    out->n_states = 0;
+    EDLGT_TRY(out->keys.alloc(8));
+    EDLGT_TRY(out->configs.alloc(8));
+    return EDLGT_OK;
+  }
+  // Phase 2: depth-first completion, count -> scan -> fill.
+  DevBuf counts, offs;
+  EDLGT_TRY(counts.alloc(n_front * sizeof(int64_t)));
+  EDLGT_TRY(offs.alloc((n_front + 1) * sizeof(int64_t)));
+  const unsigned g2 = grid_for(n_front, 64);
+  hipLaunchKernelGGL(complete_prefixes<false>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
+                     n_front, p, counts.as<int64_t>(), (const int64_t*)nullptr, (uint64_t*)nullptr,
+                     (uint8_t*)nullptr, 0);
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_TRY(exclusive_scan_i64(counts.as<int64_t>(), offs.as<int64_t>(), n_front, st));
+  int64_t N = 0;
+  EDLGT_HIP(hipMemcpyAsync(&N, offs.as<int64_t>() + n_front, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  EDLGT_HIP(hipStreamSynchronize(st));
+  out->n_states = N;
+  EDLGT_TRY(out->keys.alloc(std::max<int64_t>(N, 1) * sizeof(uint64_t)));
+  EDLGT_TRY(out->configs.alloc(std::max<int64_t>(N, 1) * out->cfg_stride));
+  EDLGT_HIP(hipMemsetAsync(out->configs.p, 0, std::max<int64_t>(N, 1) * out->cfg_stride, st));
+  hipLaunchKernelGGL(complete_prefixes<true>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
+                     n_front, p, (int64_t*)nullptr, offs.as<int64_t>(), out->keys.as<uint64_t>(),
+                     out->configs.as<uint8_t>(), out->cfg_stride);
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_HIP(hipStreamSynchronize(st));
+  return EDLGT_OK;
+}
+
+}  // namespace edlgt

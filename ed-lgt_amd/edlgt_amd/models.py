@@ -1,0 +1,284 @@
+"""Model API of the engine, mirroring edlgt's QuantumModel / QED_Model /
+QMB_hamiltonian / QMB_state (edlgt/models/quantum_model.py,
+edlgt/models/QED_model.py, edlgt/modeling/qmb_hamiltonian.py,
+edlgt/modeling/qmb_state.py): same constructor arguments, method names,
+attributes and per-site energy convention.  Sector enumeration, Hamiltonian
+assembly, eigensolves and time evolution run on the GPU through
+libedlgt_hip; only the tiny per-site operator tables are built on the host.
+"""
+import logging
+from math import prod
+
+import numpy as np
+
+from . import _lib as L
+from .device import CSRMatrix, SectorBasis
+from .lattice import Lattice
+from . import operators as opx
+
+logger = logging.getLogger(__name__)
+
+__all__ = ["QuantumModel", "QED_Model", "QMB_hamiltonian", "QMB_state"]
+
+
+class QMB_state:
+    """State vector wrapper (edlgt/modeling/qmb_state.py:41).  `psi` is the host
+    copy (complex128 numpy), `psi_device` the device tensor."""
+
+    def __init__(self, psi_device, lvals, loc_dims):
+        self.psi_device = psi_device
+        self.lvals = lvals
+        self.loc_dims = loc_dims
+        self._host = None
+
+    @property
+    def psi(self):
+        if self._host is None:
+            self._host = self.psi_device.detach().cpu().numpy()
+        return self._host
+
+    def get_norm(self):
+        import torch
+        return float(torch.linalg.vector_norm(self.psi_device).item())
+
+
+class QMB_hamiltonian:
+    """Hamiltonian container (edlgt/modeling/qmb_hamiltonian.py:24-426) whose
+    matrix lives on the GPU as canonical CSR."""
+
+    def __init__(self, lvals, size):
+        self.lvals = list(lvals)
+        self.shape = (size, size)
+        self.Ham = None
+        self.Ham_type = None
+
+    def set_matrix(self, csr, fmt):
+        self.Ham = csr
+        self.Ham_type = fmt
+        self.nnz = csr.nnz
+
+    def diagonalize(self, n_eigs, format, loc_dims, print_results=False, tol=1e-12,
+                    max_basis=64, precision=L.PREC_FP64, v0=None):
+        """Lowest n_eigs eigenpairs; Nenergies are energies per site
+        (qmb_hamiltonian.py:260-332)."""
+        if isinstance(n_eigs, str):
+            raise NotImplementedError("full spectra (n_eigs='full') are not on the device path")
+        if not isinstance(n_eigs, (int, np.integer)) or n_eigs < 1:
+            raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")
+        if n_eigs > self.shape[0]:
+            raise ValueError(f"n_eigs must be < H.shape[0]={self.shape[0]}, got {n_eigs}")
+        self.loc_dims = loc_dims
+        self.n_eigs = int(n_eigs)
+        if v0 is None:
+            v0 = np.random.default_rng(0).random(self.shape[0])
+        cplx = self.Ham.is_complex
+        v0 = np.asarray(v0, dtype=np.complex128 if cplx else np.float64)
+        v0 = v0 / np.linalg.norm(v0)
+        evals, evecs, resid, mvs = self.Ham.lanczos(self.n_eigs, tol=tol, max_basis=max_basis,
+                                                    precision=precision, v0=v0)
+        self.energies = evals
+        self.residuals = resid
+        self.matvecs = mvs
+        nsites = prod(self.lvals)
+        self.Nenergies = evals / nsites
+        self.Npsi = [QMB_state(evecs[i], self.lvals, loc_dims) for i in range(self.n_eigs)]
+        self.GSenergy = self.Nenergies[0]
+        self.GSpsi = self.Npsi[0]
+        if print_results:
+            for e in evals:
+                logger.info(f"{e:.10f}  {e / nsites:.10f}")
+
+
+class QuantumModel:
+    """Model base (edlgt/models/quantum_model.py:120-615)."""
+
+    def __init__(self, lvals, has_obc, ham_format="sparse", basis_projector=None):
+        if basis_projector is not None:
+            raise NotImplementedError("basis_projector is not supported by the device engine")
+        self.lvals = list(lvals)
+        self.has_obc = list(has_obc)
+        self.lattice = Lattice(self.lvals, self.has_obc)
+        self.dim = len(self.lvals)
+        self.directions = "xyz"[: self.dim]
+        self.n_sites = prod(self.lvals)
+        self.ham_format = ham_format
+        self.momentum_basis = None
+        self.staggered_basis = False
+        self.basis = None
+        self.res = {}
+
+    # -- per-site operator tables (quantum_model.py:400-502)
+    def _project(self, site, op_builder, labels):
+        self.lattice_labels = labels
+        self.loc_dims = np.array([site.dim(l) for l in labels], dtype=int)
+        d_max = int(self.loc_dims.max())
+        per_label = {l: op_builder(l) for l in sorted(set(labels))}
+        names = list(next(iter(per_label.values())).keys())
+        self.ops = {}
+        for name in names:
+            arr = np.zeros((self.n_sites, d_max, d_max), dtype=np.complex128)
+            for j, l in enumerate(labels):
+                d = self.loc_dims[j]
+                arr[j, :d, :d] = per_label[l][name]
+            self.ops[name] = arr
+        self._label_ops = per_label
+        self.gauge_states_per_site = [site.values[l] for l in labels]
+
+    @property
+    def sector_configs(self):
+        return None if self.basis is None else self.basis.configs()
+
+    @property
+    def sector_dim(self):
+        return None if self.basis is None else self.basis.n_states
+
+    def default_params(self):
+        self.H = QMB_hamiltonian(self.lvals, size=self.sector_dim)
+
+    # -- Hamiltonian program: tables + placements
+    def _new_program(self):
+        self._tables = []
+        self._table_ids = {}
+        self._placements = []
+
+    def _table(self, name, site):
+        key = (name, self.lattice_labels[site])
+        if key not in self._table_ids:
+            self._table_ids[key] = len(self._tables)
+            self._tables.append(self.ops[name][site])
+        return self._table_ids[key]
+
+    def _add_local(self, name, strength, sites=None):
+        """LocalTerm.get_Hamiltonian (local_term.py:55-117): diagonal entries
+        O[a][a] with |O[a][a]| >= 1e-10, times strength, on the masked sites."""
+        for i in range(self.n_sites) if sites is None else sites:
+            self._placements.append(dict(sites=[i], tables=[self._table(name, i)], strength=strength))
+
+    def _add_nbody(self, names, site_lists, strength, add_dagger):
+        """TwoBodyTerm / PlaquetteTerm / NBodyTerm (twobody_term.py:116-158,
+        plaquette_term.py:116-163): one placement per site tuple, plus the
+        Hermitian-conjugate rows when add_dagger."""
+        for sl in site_lists:
+            tabs = [self._table(n, s) for n, s in zip(names, sl)]
+            self._placements.append(dict(sites=list(sl), tables=tabs, strength=strength, dagger=0))
+            if add_dagger:
+                self._placements.append(dict(sites=list(sl), tables=tabs, strength=strength, dagger=1))
+
+    def _build_program(self, real):
+        tables = np.stack(self._tables)
+        vt = L.VALUE_F64 if real else L.VALUE_C128
+        csr = CSRMatrix(self.basis, tables, self._placements, vt)
+        self.H.set_matrix(csr, self.ham_format)
+        return csr
+
+    def diagonalize_Hamiltonian(self, n_eigs, format, print_results=False, **kw):
+        self.H.diagonalize(n_eigs, format, self.loc_dims, print_results, **kw)
+        self.n_eigs = self.H.n_eigs
+        self.res["energy"] = self.H.Nenergies
+
+    def get_qmb_state_from_configs(self, configs):
+        """Equal-weight normalised superposition of basis configurations
+        (quantum_model.py:826-862); raises ValueError for a config outside
+        the sector."""
+        import torch
+        idx = self.basis.lookup(np.asarray(configs))
+        state = torch.zeros(self.sector_dim, dtype=torch.complex128, device="cuda")
+        state[torch.as_tensor(idx, device="cuda")] = complex(1 / np.sqrt(len(idx)), 0)
+        return state
+
+
+class QED_Model(QuantumModel):
+    """U(1) lattice gauge theory with optional staggered fermions
+    (edlgt/models/QED_model.py:20-394, 719-776)."""
+
+    def __init__(self, spin, pure_theory, bg_list=None, plaq_basis=False, link_symmetries=True,
+                 get_only_bulk=False, **kwargs):
+        super().__init__(**kwargs)
+        if bg_list is not None and any(b != 0 for b in bg_list):
+            raise NotImplementedError("background charges are not supported by the device engine")
+        if plaq_basis or not link_symmetries or get_only_bulk:
+            raise NotImplementedError("only the dressed-site basis with link symmetries is supported")
+        if not isinstance(spin, (int, np.integer)) or spin < 1:
+            raise NotImplementedError("spin must be a positive integer truncation")
+        self.spin = int(spin)
+        self.pure_theory = bool(pure_theory)
+        self.staggered_basis = not self.pure_theory
+        self.link_symmetries = True
+        site = opx.qed_site(self.spin, self.pure_theory, self.dim)
+        labels = [self.lattice.label(i, self.staggered_basis) for i in range(self.n_sites)]
+        self._project(site, lambda l: opx.qed_operators(site, self.spin, self.pure_theory, self.dim, l), labels)
+        d_max = self.ops["E_px"].shape[1]
+
+        def diag(name):
+            return np.real(np.einsum("sii->si", self.ops[name]))
+
+        glob = None
+        if not self.pure_theory:
+            glob = (diag("N")[None], np.array([float(self.n_sites // 2)]), "U")
+        pairs = self.lattice.link_pairs()
+        link = (np.stack([np.stack([diag(f"E_p{d}"), -diag(f"E_m{d}")]) for d in self.directions]),
+                np.zeros(self.dim), pairs)
+        nbody = None
+        if self.pure_theory and not any(self.has_obc) and self.dim == 2:
+            line_of = {"x": "y", "y": "x"}
+            nbody = (np.stack([diag(f"E_p{d}") for d in self.directions]), np.zeros(self.dim),
+                     [self.lattice.origin_line(line_of[d]) for d in self.directions], "U")
+        elif self.pure_theory and not any(self.has_obc) and self.dim == 3:
+            raise NotImplementedError("3D pure PBC flux sectors are not supported yet")
+        self.basis = SectorBasis(self.loc_dims, glob=glob, link=link, nbody=nbody)
+        logger.info(f"SEC DIM: {self.basis.n_states}")
+        self.default_params()
+
+    def QED_Hamiltonian_couplings(self, g, m=None, theta=0.0):
+        """QED_model.py:719-776 (dressed-site normalisation)."""
+        c = {"g": g, "E": g / 2, "B": -1 / (2 * g), "theta": -theta * g}
+        if not self.pure_theory and m is not None:
+            t = 0.5
+            c.update({"t": -complex(0, t), "tx_even": -complex(0, t), "tx_odd": -complex(0, t),
+                      "ty_even": -t, "ty_odd": t, "tz_even": -complex(0, t), "tz_odd": complex(0, t),
+                      "m": m, "m_odd": -m, "m_even": m})
+        self.coeffs = c
+
+    def _dtype_real(self, theta, dtype_mode):
+        mode = dtype_mode
+        if isinstance(mode, (bool, np.bool_)):
+            mode = "complex" if mode else "real"
+        if mode is None or mode == "auto":
+            return self.pure_theory and abs(theta) <= 1e-12
+        if mode not in ("real", "complex"):
+            raise ValueError(f"dtype_mode must be one of ['auto', 'complex', 'real'], got {dtype_mode!r}")
+        return mode == "real"
+
+    def build_Hamiltonian(self, g, m=None, theta=0.0, dtype_mode="auto"):
+        """build_truncated_Hamiltonian term order (QED_model.py:235-394)."""
+        real = self._dtype_real(theta, dtype_mode)
+        self.QED_Hamiltonian_couplings(g, m, theta)
+        c = self.coeffs
+        lat = self.lattice
+        self._new_program()
+        self._add_local("E2", c["E"])
+        if self.dim > 1:
+            names = ["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"]
+            plaq = lat.plaquettes(("x", "y"))
+            self._add_nbody(names, plaq, c["B"], True)
+        if self.dim == 3:
+            self._add_nbody(["C_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], lat.plaquettes(("x", "z")), c["B"], True)
+            self._add_nbody(["C_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], lat.plaquettes(("y", "z")), c["B"], True)
+        if self.dim == 2 and abs(c["theta"]) > 10e-10:
+            self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], lat.plaquettes(("x", "y")),
+                            c["theta"], True)
+        if self.dim == 3 and abs(c["theta"]) > 10e-10:
+            self._add_nbody(["EzC_px,py", "C_py,mx", "C_my,px", "C_mx,my"], lat.plaquettes(("x", "y")), c["theta"], True)
+            self._add_nbody(["EyC_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], lat.plaquettes(("x", "z")), c["theta"], True)
+            self._add_nbody(["ExC_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], lat.plaquettes(("y", "z")), c["theta"], True)
+        if not self.pure_theory:
+            if m is None:
+                raise ValueError("m is required with matter fields")
+            even = lat.even_mask()
+            self._add_local("N", c["m_even"], [i for i in range(self.n_sites) if even[i]])
+            self._add_local("N", c["m_odd"], [i for i in range(self.n_sites) if not even[i]])
+            for d in self.directions:
+                for stag in ("even", "odd"):
+                    bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
+                    self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
+        return self._build_program(real)

@@ -1,0 +1,191 @@
+/*
+ * libedlgt_hip -- MI355X (gfx950) engine for the exact-diagonalization hot
+ * path of edlgt (gcataldi96/ed-lgt 0.2.3): symmetry-sector enumeration,
+ * Hamiltonian assembly into canonical CSR, SpMV/SpMM, Lanczos and Krylov
+ * exp(-iHt).
+ *
+ * C ABI: plain pointers and sizes only.  Every function returns an int32
+ * status (EDLGT_OK == 0, negative on error); edlgt_last_error() gives the
+ * message of the most recent failure on the calling thread.
+ *
+ * Memory: arrays documented as "host" are read or written synchronously
+ * through host memory; arrays documented as "device" are device pointers the
+ * caller owns (e.g. from hipMalloc or a torch tensor).  Long-lived device
+ * state (sector basis, CSR matrix, solver workspaces) lives in opaque handles
+ * allocated by the library and released by the matching *_destroy call.  The
+ * library never frees caller memory.  `stream` may be NULL (legacy default
+ * stream) or a hipStream_t; all work of one call is ordered on that stream.
+ * A handle may be used by one thread at a time.
+ */
+#ifndef EDLGT_HIP_H
+#define EDLGT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the Python layer maps them to ValueError/MemoryError/...) */
+#define EDLGT_OK 0
+#define EDLGT_ERR_INVALID_ARG -1
+#define EDLGT_ERR_KEY_NOT_FOUND -2
+#define EDLGT_ERR_INDEX_OVERFLOW -3
+#define EDLGT_ERR_OUT_OF_MEMORY -4
+#define EDLGT_ERR_HIP -5
+#define EDLGT_ERR_NOT_CONVERGED -6
+#define EDLGT_ERR_NOT_REAL -7
+
+/* ---- value / vector element types */
+#define EDLGT_VALUE_F64 0  /* double                       */
+#define EDLGT_VALUE_C128 1 /* complex128 as (re, im) double pairs */
+#define EDLGT_VALUE_F32 2  /* float                        */
+#define EDLGT_VALUE_C64 3  /* complex64 as (re, im) float pairs   */
+
+/* ---- precision modes of SpMV-driven solvers */
+#define EDLGT_PREC_FP64 0  /* matrix values and vectors as built (FP64)                     */
+#define EDLGT_PREC_MIXED 1 /* FP32 matrix values and gathered vector, FP64 accumulation,
+                              dots, recurrence, reorthogonalisation and final refinement   */
+
+/* ---- symmetry types of global / n-body constraints */
+#define EDLGT_SYM_U1 0 /* additive: sum of diagonal values equals the sector value        */
+#define EDLGT_SYM_Z2 1 /* multiplicative: product of diagonal values equals the sector    */
+
+typedef struct edlgt_basis edlgt_basis; /* sorted sector basis on the device */
+typedef struct edlgt_csr edlgt_csr;     /* CSR Hamiltonian on the device     */
+
+int edlgt_last_error(char* buf, int64_t size);
+int edlgt_version(void);
+/* Device of the calling thread and its free / total memory in bytes. */
+int edlgt_device_info(int32_t* device, int64_t* free_bytes, int64_t* total_bytes);
+
+/* ======================================================================
+ * Symmetry-sector enumeration.
+ *
+ * Replaces symmetry_sector_configs (edlgt/symmetries/symmetry_sector.py:68-145,
+ * kernel iterative_sitebased_sym_sector_configs :785-967) and
+ * get_link_sector_configs (:1167-1234, kernels iterative_link_sector_configs
+ * :1237-1326 and iterative_link_sector_configs_plus :1329-1469).
+ *
+ * Produces every configuration (c_0..c_{n-1}), 0 <= c_s < loc_dims[s], that
+ * satisfies all constraints, in lexicographic order (site 0 most
+ * significant) -- the order of the reference's sector_configs rows:
+ *   global:  sum_s glob_diags[g][s][c_s] == glob_sectors[g]   (EDLGT_SYM_U1)
+ *            prod_s glob_diags[g][s][c_s] == glob_sectors[g]  (EDLGT_SYM_Z2)
+ *   link:    link_diags[d][0][i][c_i] + link_diags[d][1][j][c_j] == link_sectors[d]
+ *            for every pair p (d = pair_dir[p], i = pair_site0[p], j = pair_site1[p])
+ *   n-body:  sum (or product) of nbody_diags[k][s][c_s] over the sites
+ *            nbody_sites[nbody_site_offsets[k] .. nbody_site_offsets[k+1]) ==
+ *            nbody_sectors[k]
+ * each within 1e-13 (symmetry_sector.py:17).  All table pointers are host
+ * arrays; diag tables are padded to d_max states per site.  n_glob, n_pairs,
+ * n_nbody may be 0 (their pointers may then be NULL).  Requires
+ * prod(loc_dims) < 2^64 and loc_dims[s] <= 255.
+ * ====================================================================== */
+int edlgt_sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max,
+                       int32_t n_glob, int32_t glob_type, const double* glob_diags,
+                       const double* glob_sectors,
+                       int32_t n_dirs, const double* link_diags, const double* link_sectors,
+                       int32_t n_pairs, const int32_t* pair_dir, const int32_t* pair_site0,
+                       const int32_t* pair_site1,
+                       int32_t n_nbody, int32_t nbody_type, const double* nbody_diags,
+                       const double* nbody_sectors, const int32_t* nbody_site_offsets,
+                       const int32_t* nbody_sites,
+                       void* stream, edlgt_basis** out);
+int edlgt_basis_info(const edlgt_basis* b, int64_t* n_states, int32_t* n_sites);
+/* Sector configurations as uint16 rows (host, n_states x n_sites), the layout of
+ * the reference's sector_configs. */
+int edlgt_basis_copy_configs(const edlgt_basis* b, uint16_t* out_host);
+/* Mixed-radix keys (host, n_states), key = sum_s c_s * prod_{t>s} loc_dims[t];
+ * ascending (edlgt/tools/magic.py:40-80 convention). */
+int edlgt_basis_copy_keys(const edlgt_basis* b, uint64_t* out_host);
+/* Index of each query configuration (host uint16 rows) in the basis; replaces
+ * config_to_index_binarysearch (edlgt/symmetries/generate_configs.py:145-173).
+ * Misses are written as -1 and reported as EDLGT_ERR_KEY_NOT_FOUND. */
+int edlgt_basis_lookup(const edlgt_basis* b, const uint16_t* configs_host, int64_t n_query,
+                       int64_t* out_idx_host);
+/* Device pointers of the keys (uint64 [n]) and configs (uint8 [n][cfg_stride]). */
+int edlgt_basis_device_ptrs(const edlgt_basis* b, void** keys, void** configs,
+                            int32_t* cfg_stride);
+int edlgt_basis_destroy(edlgt_basis* b);
+
+/* ======================================================================
+ * Hamiltonian assembly into canonical CSR.
+ *
+ * Replaces nbody_term and its kernels (edlgt/symmetries/sym_qmb_operations.py:
+ * 32-602), the term drivers LocalTerm/TwoBodyTerm/PlaquetteTerm/NBodyTerm
+ * .get_Hamiltonian (edlgt/modeling/{local,twobody,plaquette,nbody}_term.py) and QMB_hamiltonian.add_term /
+ * build("sparse") (edlgt/modeling/qmb_hamiltonian.py:155-225).
+ *
+ * tables: host, n_tables dense d_max x d_max complex matrices (interleaved
+ *   re, im), each one site-projected local operator O[bra][ket].
+ * A placement is one term instance on concrete sites (one site for a local
+ *   term, the bond or plaquette sites otherwise):
+ *   place_nops[p]          number of factors (1..8); 1 means a diagonal local
+ *                          term (localbody_data_par semantics: keep
+ *                          |O[a][a]| >= 1e-10 before scaling)
+ *   place_sites[8p+f]      site of factor f
+ *   place_tables[8p+f]     table of factor f
+ *   place_strength[2p..]   complex strength (re, im)
+ *   place_dagger[p]        0: emit <bra| s * prod_f O_f |ket>;
+ *                          1: emit the Hermitian-conjugate block of the same
+ *                          placement (the reference's add_dagger rows)
+ * Factor transitions keep |O[a][b]| > 1e-10 (sym_qmb_operations.py:21,85).
+ * The result is the canonical CSR of the summed triplets: rows ascending,
+ * columns ascending within a row, duplicates summed, explicit zeros kept.
+ * value_type EDLGT_VALUE_F64 requires every value's |Im| <= 1e-12
+ * (EDLGT_ERR_NOT_REAL otherwise); EDLGT_VALUE_C128 keeps complex values.
+ * A ket configuration outside the sector is EDLGT_ERR_KEY_NOT_FOUND.
+ * ====================================================================== */
+int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
+                            const double* tables, int32_t n_place, const int32_t* place_nops,
+                            const int32_t* place_sites, const int32_t* place_tables,
+                            const double* place_strength, const int32_t* place_dagger,
+                            int32_t value_type, void* stream, edlgt_csr** out);
+int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* value_type);
+/* Copy CSR arrays to host: row_ptr int64 [n+1], col int32 [nnz], val [nnz]. */
+int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val);
+int edlgt_csr_device_ptrs(const edlgt_csr* h, void** row_ptr, void** col, void** val);
+/* Build an FP32 (complex64 / float32) copy of the values for EDLGT_PREC_MIXED. */
+int edlgt_csr_make_low_precision(edlgt_csr* h, void* stream);
+int edlgt_csr_destroy(edlgt_csr* h);
+
+/* ======================================================================
+ * Sparse products.  Replace csr_spmv / csr_matmat (edlgt/modeling/
+ * qmb_hamiltonian.py:705-764) behind the LinearOperator matvec/matmat seam
+ * (:186-216).  x, y, X, Y are device arrays of vec_type (EDLGT_VALUE_C128 or
+ * EDLGT_VALUE_F64; F64 only for real matrices).  X, Y are n_rhs vectors of
+ * length n stored one after the other (column-major N x n_rhs).
+ * precision EDLGT_PREC_MIXED reads the FP32 value copy (built on demand).
+ * ====================================================================== */
+int edlgt_spmv(edlgt_csr* h, int32_t precision, const void* x, void* y, int32_t vec_type,
+               void* stream);
+int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, void* Y,
+               int32_t vec_type, void* stream);
+
+/* ======================================================================
+ * Lowest eigenpairs by thick-restart Lanczos with full reorthogonalisation.
+ * Replaces scipy eigsh(H, k, which="SA") in QMB_hamiltonian.diagonalize
+ * (edlgt/modeling/qmb_hamiltonian.py:260-332).
+ *   v0_host      start vector (host, vec_type, length n); NULL -> all ones
+ *   tol          stop when every wanted residual ||H y - theta y|| <= tol * max(1,|theta|)
+ *   max_basis    Krylov basis size m (>= n_eigs + 2)
+ *   max_iter     total matvec budget
+ *   precision    EDLGT_PREC_FP64, or EDLGT_PREC_MIXED (FP32 SpMV, then FP64
+ *                refinement restarted from the FP32 Ritz vectors)
+ *   out_evals    host double [n_eigs], ascending total energies
+ *   out_evecs    device, vec_type, n x n_eigs column-major, or NULL
+ *   out_resid    host double [n_eigs] final FP64 residual norms, or NULL
+ *   out_matvecs  host int64 [2]: FP64 and FP32 matvec counts, or NULL
+ * Returns EDLGT_ERR_NOT_CONVERGED (with the best estimates written) when the
+ * budget is exhausted.
+ * ====================================================================== */
+int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis,
+                  int64_t max_iter, int32_t precision, int32_t vec_type, const void* v0_host,
+                  double* out_evals, void* out_evecs, double* out_resid,
+                  int64_t* out_matvecs, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
