@@ -1,0 +1,198 @@
+"""Benchmark of the exact-diagonalization hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload qed4x3|qed4x2] [--no-cpu]
+
+Workload (default): 2D U(1) QED, 4x3 periodic lattice, staggered fermions,
+E in {-1,0,1}, g=1, m=0.5, Gauss-law + zero-charge sector (N = 28,904,615).
+The matrix is built on the GPU (sector enumeration + CSR assembly), then one
+"step" is one Lanczos matvec y = H x (complex128 CSR SpMV) over the whole
+sector, inputs resident in HBM.  `value` is the SpMV's algorithmic bandwidth
+
+    B = nnz*(16 + 4) + (N+1)*8 + N*(16 + 16)   bytes per launch
+
+divided by the mean launch time (HIP events around K launches on the stream
+the kernel runs on).  The JSON line also reports the build times, the FP64
+and mixed-precision lowest-eigenvalue solves, the FP32 SpMV rate and the
+CPU baseline (the C/OpenMP restatement of the reference's csr_spmv timed on
+the host cores over the same matrix).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "ed-lgt_amd"):
+    sys.path.insert(0, str(p))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "qed4x3": dict(lvals=[4, 3], g=1.0, m=0.5, desc="2D U(1) QED 4x3 PBC, staggered fermions, E in {-1,0,1}, g=1, m=0.5"),
+    "qed4x2": dict(lvals=[4, 2], g=1.0, m=0.5, desc="2D U(1) QED 4x2 PBC, staggered fermions, E in {-1,0,1}, g=1, m=0.5"),
+}
+
+
+def spmv_bytes(n, nnz, s_v=16, s_x=16):
+    return nnz * (s_v + 4) + (n + 1) * 8 + n * (s_x + 16)
+
+
+def time_launches(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def cpu_baseline(H, x_dev, max_seconds=12.0):
+    """oracle/refcpu csr_spmv (row-parallel, complex128) on the host cores over
+    the same CSR; returns GB/s by the same algorithmic-bytes formula."""
+    from oracle import refcpu
+    rp, col, val = H.to_host()
+    x = x_dev.cpu().numpy()
+    y = np.empty_like(x)
+    refcpu.spmv(rp, col, val, x, y)  # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        refcpu.spmv(rp, col, val, x, y)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > max_seconds or reps >= 50:
+            break
+    t = el / reps
+    return dict(value=round(spmv_bytes(H.n, H.nnz) / t / 1e9, 2), unit="GB/s", cores=refcpu.threads(),
+                kind="port", sample=f"{reps} full complex128 CSR SpMV passes over the same matrix "
+                                    f"(N={H.n}, nnz={H.nnz}), {t*1e3:.1f} ms each")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also launch plain device copies of a known size (PMC byte-counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded design is "
+              "described in DESIGN.md and no multi-GPU path is built.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from edlgt_amd import QED_Model, PREC_MIXED
+    from edlgt_amd import _lib as L
+
+    wl = WORKLOADS[args.workload]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model = QED_Model(spin=1, pure_theory=False, lvals=wl["lvals"], has_obc=[False, False])
+    torch.cuda.synchronize()
+    t_sector = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    H = model.build_Hamiltonian(wl["g"], wl["m"])
+    torch.cuda.synchronize()
+    t_assembly = time.perf_counter() - t0
+    N, nnz = H.n, H.nnz
+
+    # launch-shape autotuner (lanes per row x workgroups), timed on the device
+    tune = H.autotune(n_rhs=1)
+    tune32 = H.autotune(n_rhs=1, precision=L.PREC_MIXED)
+    H.set_launch(*tune[:2])
+
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn(N, dtype=torch.complex128, device="cuda", generator=gen)
+    y = torch.empty_like(x)
+    ms = time_launches(lambda: H.matvec(x, y), args.steps, args.warmup)
+    B = spmv_bytes(N, nnz)
+    gbs = B / (ms * 1e-3) / 1e9
+
+    if args.calibrate:
+        # 5 elementwise passes over N complex128: 16*N bytes read, 16*N written each
+        for _ in range(5):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+
+    # mixed precision: FP32 values and FP32 gathered vector, FP64 accumulation
+    x32 = x.to(torch.complex64)
+    lib = L.load()
+
+    def mixed():  # uses the FP32-tuned launch shape
+        H.set_launch(*tune32[:2])
+        L.check(lib.edlgt_spmm(H.handle, L.PREC_MIXED, 1, x32.data_ptr(), y.data_ptr(), L.VALUE_C64,
+                               L.stream_handle()))
+
+    mixed()
+    ms32 = time_launches(mixed, args.steps, args.warmup)
+    H.set_launch(*tune[:2])
+    gbs32 = spmv_bytes(N, nnz, s_v=8, s_x=8) / (ms32 * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_solve:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        model.diagonalize_Hamiltonian(1, "sparse")
+        torch.cuda.synchronize()
+        t_fp64 = time.perf_counter() - t0
+        e64 = float(model.H.energies[0])
+        mv64 = [int(v) for v in model.H.matvecs]
+        t0 = time.perf_counter()
+        model.diagonalize_Hamiltonian(1, "sparse", precision=PREC_MIXED)
+        torch.cuda.synchronize()
+        t_mix = time.perf_counter() - t0
+        emix = float(model.H.energies[0])
+        extra = dict(E0_fp64=e64, E0_per_site=e64 / np.prod(wl["lvals"]), lanczos_fp64_s=round(t_fp64, 3),
+                     lanczos_fp64_matvecs=mv64, lanczos_mixed_s=round(t_mix, 3),
+                     lanczos_mixed_matvecs=[int(v) for v in model.H.matvecs],
+                     dE0_mixed_vs_fp64=abs(emix - e64),
+                     build_plus_E0_s=round(t_sector + t_assembly + t_fp64, 3))
+
+    traffic = None
+    tf = ROOT / "profiles" / f"spmv_{args.workload}_traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(H, x)
+
+    out = {
+        "metric": "Lanczos SpMV GB/s (% HBM peak)",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (random complex128 input vector, seed 0); matrix built on the GPU from the model",
+        "config": {"workload": wl["desc"], "sector_dim": N, "nnz": nnz, "step": "one CSR SpMV y=Hx",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": B},
+        "cpu_baseline": cpu,
+        "extra": dict(sector_s=round(t_sector, 3), assembly_s=round(t_assembly, 3),
+                      spmv_launch={"lanes_per_row": tune[0], "workgroups": tune[1]},
+                      spmv_mixed_launch={"lanes_per_row": tune32[0], "workgroups": tune32[1]},
+                      spmv_mixed_ms=round(ms32, 4), spmv_mixed_GBs=round(gbs32, 2),
+                      pct_hbm_peak=round(100 * gbs / HBM_PEAK_GBS, 2), **extra),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

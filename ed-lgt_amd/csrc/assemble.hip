@@ -96,16 +96,21 @@ __device__ inline int64_t lower_bound_u64(const uint64_t* __restrict__ a, int64_
 }
 
 // Number of raw entries placement P emits from the row with local states cfg.
+// Loops over factors are unrolled to kMaxOps with guards so that every
+// per-factor value stays in registers (no private-memory arrays).
 __device__ inline int placement_count(const AsmDev& A, const Placement& P, const uint8_t* cfg) {
   if (P.nops == 1) {
     const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
     return (sqrt(v.x * v.x + v.y * v.y) >= kOpTol) ? 1 : 0;
   }
   int c = 1;
-  for (int f = 0; f < P.nops; ++f) {
-    const int a = cfg[P.site[f]];
-    const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
-    c *= off[a + 1] - off[a];
+#pragma unroll
+  for (int f = 0; f < kMaxOps; ++f) {
+    if (f < P.nops) {
+      const int a = cfg[P.site[f]];
+      const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
+      c *= off[a + 1] - off[a];
+    }
   }
   return c;
 }
@@ -127,23 +132,34 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
     return;
   }
   int beg[kMaxOps], cnt[kMaxOps], idx[kMaxOps];
+  int64_t stride[kMaxOps];
   int64_t base_delta = 0;
-  for (int f = 0; f < P.nops; ++f) {
-    const int a = cfg[P.site[f]];
-    const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
-    beg[f] = off[a];
-    cnt[f] = off[a + 1] - off[a];
+#pragma unroll
+  for (int f = 0; f < kMaxOps; ++f) {
+    beg[f] = 0;
+    cnt[f] = 1;
     idx[f] = 0;
-    base_delta -= (int64_t)a * (int64_t)A.strides[P.site[f]];
+    stride[f] = 0;
+    if (f < P.nops) {
+      const int a = cfg[P.site[f]];
+      const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
+      beg[f] = off[a];
+      cnt[f] = off[a + 1] - off[a];
+      stride[f] = (int64_t)A.strides[P.site[f]];
+      base_delta -= (int64_t)a * stride[f];
+    }
   }
   int slot = slot0;
   while (true) {
     int64_t delta = base_delta;
     double2 amp = make_double2(1.0, 0.0);
-    for (int f = 0; f < P.nops; ++f) {
-      const int e = beg[f] + idx[f];
-      delta += (int64_t)A.trans_ket[e] * (int64_t)A.strides[P.site[f]];
-      if (kFill) amp = cmul(amp, A.trans_val[e]);
+#pragma unroll
+    for (int f = 0; f < kMaxOps; ++f) {
+      if (f < P.nops) {
+        const int e = beg[f] + idx[f];
+        delta += (int64_t)A.trans_ket[e] * stride[f];
+        if (kFill) amp = cmul(amp, A.trans_val[e]);
+      }
     }
     skey[slot] = key + (uint64_t)delta;
     sidx[slot] = (uint16_t)slot;
@@ -154,12 +170,19 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
       sval[slot] = r;
     }
     ++slot;
-    int f = P.nops - 1;
-    for (; f >= 0; --f) {
-      if (++idx[f] < cnt[f]) break;
-      idx[f] = 0;
+    // mixed-radix increment, last factor fastest (cnt == 1 beyond nops)
+    bool carry = true;
+#pragma unroll
+    for (int f = kMaxOps - 1; f >= 0; --f) {
+      if (carry && f < P.nops) {
+        if (++idx[f] < cnt[f]) {
+          carry = false;
+        } else {
+          idx[f] = 0;
+        }
+      }
     }
-    if (f < 0) break;
+    if (carry) break;
   }
 }
 

@@ -247,11 +247,38 @@ int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, vo
   EDLGT_REQUIRE(h && X && Y, "null argument");
   EDLGT_REQUIRE(n_rhs >= 1, "n_rhs must be >= 1");
   EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
-  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64, "vec_type must be C128 or F64");
-  EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_F64 && h->value_type != EDLGT_VALUE_F64),
+  const bool lo_vec = vec_type == EDLGT_VALUE_C64 || vec_type == EDLGT_VALUE_F32;
+  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64 ||
+                    (lo_vec && precision == EDLGT_PREC_MIXED),
+                "vec_type must be C128 or F64 (or C64/F32 with EDLGT_PREC_MIXED)");
+  EDLGT_REQUIRE(!((vec_type == EDLGT_VALUE_F64 || vec_type == EDLGT_VALUE_F32) &&
+                  h->value_type != EDLGT_VALUE_F64),
                 "real vectors need a real matrix");
+  EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_C64 && h->value_type != EDLGT_VALUE_C128),
+                "complex64 vectors need a complex matrix");
   if (precision == EDLGT_PREC_MIXED) EDLGT_TRY(csr_make_low_precision(*h, (hipStream_t)stream));
   return csr_spmm(*h, precision, vec_type, n_rhs, X, h->n_rows, Y, h->n_rows, (hipStream_t)stream);
+}
+
+int edlgt_csr_autotune(edlgt_csr* h, int32_t precision, int32_t vec_type, int32_t n_rhs,
+                       const void* X, void* Y, void* stream, int32_t* out_lanes, int64_t* out_grid,
+                       float* out_ms) {
+  EDLGT_REQUIRE(h && X && Y && n_rhs >= 1, "null argument");
+  EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
+  if (precision == EDLGT_PREC_MIXED) EDLGT_TRY(csr_make_low_precision(*h, (hipStream_t)stream));
+  return csr_autotune(*h, precision, vec_type, n_rhs, X, Y, (hipStream_t)stream, out_lanes, out_grid,
+                      out_ms);
+}
+
+int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid) {
+  EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 ||
+                    lanes_per_row == 16 || lanes_per_row == 32 || lanes_per_row == 64,
+                "lanes_per_row must be 0, 4, 8, 16, 32 or 64");
+  EDLGT_REQUIRE(grid >= 0, "grid must be >= 0");
+  h->g_override = lanes_per_row;
+  h->grid_override = grid;
+  return EDLGT_OK;
 }
 
 int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis, int64_t max_iter,

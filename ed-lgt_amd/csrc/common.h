@@ -91,7 +91,8 @@ struct Csr {
   int64_t n_rows = 0;
   int64_t nnz = 0;
   int32_t value_type = EDLGT_VALUE_C128;
-  int32_t g_override = 0;  // lanes per row of the SpMV kernels (0: from the mean row length)
+  int32_t g_override = 0;     // lanes per row of the SpMV kernels (0: from the mean row length)
+  int64_t grid_override = 0;  // SpMV workgroups (0: default), set by the autotuner
   DevBuf row_ptr;   // int64 [n_rows + 1]
   DevBuf col;       // int32 [nnz]
   DevBuf val;       // value_type [nnz]

@@ -22,6 +22,7 @@
 //     lexicographic order, with no global sort.
 // Integer output is deterministic and independent of launch shapes.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "common.h"
@@ -212,6 +213,122 @@ __global__ void complete_prefixes(SectorDev D, const uint64_t* __restrict__ fron
   }
 }
 
+// Depth-first completion with all per-level state outside private memory:
+// digits live in LDS (64 bytes x 64 lanes), the global U(1) partial sums are
+// exact integers (diagonal values scaled by 2^k on the host) kept in registers
+// and updated incrementally on descent / backtrack, and the mixed-radix key is
+// updated the same way.  Used whenever the global constraints are U(1) with
+// dyadic-rational diagonal values (every model of this engine); the generic
+// kernel above covers the rest.
+struct IntGlob {
+  const int64_t* tab;     // [n_glob][n_sites][d_max]
+  const int64_t* rmin;    // [n_glob][n_sites+1]
+  const int64_t* rmax;
+  const int64_t* target;  // [n_glob]
+};
+
+__device__ inline bool check_links_nb(const SectorDev& D, int s, const uint8_t* dig) {
+  const int n = D.n_sites;
+  for (int k = D.act_link_off[s]; k < D.act_link_off[s + 1]; ++k) {
+    const int dir = D.act_link[3 * k], i = D.act_link[3 * k + 1], j = D.act_link[3 * k + 2];
+    const double v = D.link_diags[(((size_t)dir * 2 + 0) * n + i) * D.d_max + dig[i]] +
+                     D.link_diags[(((size_t)dir * 2 + 1) * n + j) * D.d_max + dig[j]];
+    if (fabs(v - D.link_sectors[dir]) > kSymTol) return false;
+  }
+  for (int k = D.act_nb_off[s]; k < D.act_nb_off[s + 1]; ++k) {
+    const int id = D.act_nb[k];
+    double v = D.nbody_type == EDLGT_SYM_U1 ? 0.0 : 1.0;
+    for (int q = D.nb_site_off[id]; q < D.nb_site_off[id + 1]; ++q) {
+      const int site = D.nb_sites[q];
+      const double x = D.nbody_diags[((size_t)id * n + site) * D.d_max + dig[site]];
+      v = D.nbody_type == EDLGT_SYM_U1 ? v + x : v * x;
+    }
+    if (fabs(v - D.nbody_sectors[id]) > kSymTol) return false;
+  }
+  return true;
+}
+
+template <bool kFill>
+__global__ void __launch_bounds__(64) complete_prefixes_int(
+    SectorDev D, IntGlob G, const uint64_t* __restrict__ frontier, int64_t n_front, int p,
+    int64_t* __restrict__ counts, const int64_t* __restrict__ offsets, uint64_t* __restrict__ keys,
+    uint8_t* __restrict__ configs, int cfg_stride) {
+  __shared__ uint8_t sdig[64 * 64];
+  uint8_t* dig = sdig + threadIdx.x * 64;
+  const int n = D.n_sites;
+  const int ng = D.n_glob;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n_front;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    decode_prefix(D, frontier[f], p, dig);
+    int64_t g[kMaxGlob];
+    uint64_t key = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxGlob; ++k) g[k] = 0;
+    for (int s = 0; s < p; ++s) {
+      key += (uint64_t)dig[s] * D.strides[s];
+#pragma unroll
+      for (int k = 0; k < kMaxGlob; ++k)
+        if (k < ng) g[k] += G.tab[((size_t)k * n + s) * D.d_max + dig[s]];
+    }
+    int64_t count = 0;
+    int64_t out = kFill ? offsets[f] : 0;
+    if (p == n) {
+      count = 1;
+      if (kFill) {
+        keys[out] = key;
+        for (int q = 0; q < n; ++q) configs[out * cfg_stride + q] = dig[q];
+      }
+    } else {
+      int s = p;
+      int cur = -1;
+      while (true) {
+        ++cur;
+        if (cur >= D.loc_dims[s]) {
+          if (s == p) break;
+          --s;
+          cur = dig[s];
+          key -= (uint64_t)cur * D.strides[s];
+#pragma unroll
+          for (int k = 0; k < kMaxGlob; ++k)
+            if (k < ng) g[k] -= G.tab[((size_t)k * n + s) * D.d_max + cur];
+          continue;
+        }
+        dig[s] = (uint8_t)cur;
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < kMaxGlob; ++k) {
+          if (k < ng && ok) {
+            const int64_t gs = g[k] + G.tab[((size_t)k * n + s) * D.d_max + cur];
+            const int64_t t = G.target[k];
+            if (s < n - 1)
+              ok = t >= gs + G.rmin[k * (n + 1) + s + 1] && t <= gs + G.rmax[k * (n + 1) + s + 1];
+            else
+              ok = gs == t;
+          }
+        }
+        if (!ok || !check_links_nb(D, s, dig)) continue;
+        if (s == n - 1) {
+          if (kFill) {
+            keys[out] = key + (uint64_t)cur * D.strides[s];
+            uint8_t* row = configs + out * cfg_stride;
+            for (int q = 0; q < n; ++q) row[q] = dig[q];
+            ++out;
+          }
+          ++count;
+        } else {
+          key += (uint64_t)cur * D.strides[s];
+#pragma unroll
+          for (int k = 0; k < kMaxGlob; ++k)
+            if (k < ng) g[k] += G.tab[((size_t)k * n + s) * D.d_max + cur];
+          ++s;
+          cur = -1;
+        }
+      }
+    }
+    if (!kFill) counts[f] = count;
+  }
+}
+
 }  // namespace
 
 int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_t n_glob,
@@ -310,8 +427,61 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   D.nb_site_off = b_nso.as<int32_t>(); D.nb_sites = b_nsi.as<int32_t>();
   D.loc_dims = b_ldims.as<int32_t>(); D.strides = b_str.as<uint64_t>();
 
+  // Exact integer form of the U(1) constraints: every diagonal value and
+  // target scaled by 2^k (k <= 12) must be an integer of modest size.
+  bool int_mode = (n_glob == 0 || glob_type == EDLGT_SYM_U1) && n_sites <= 64;
+  std::vector<int64_t> gtab_i, rmin_i, rmax_i, tgt_i;
+  if (int_mode && n_glob > 0) {
+    int scale_pow = -1;
+    for (int k = 0; k <= 12 && scale_pow < 0; ++k) {
+      const double sc = std::ldexp(1.0, k);
+      bool ok = true;
+      for (int g = 0; g < n_glob && ok; ++g) {
+        ok = std::fabs(glob_sectors[g] * sc - std::nearbyint(glob_sectors[g] * sc)) < 1e-9;
+        for (int s = 0; s < n_sites && ok; ++s)
+          for (int a = 0; a < loc_dims[s] && ok; ++a) {
+            const double v = glob_diags[((size_t)g * n_sites + s) * d_max + a] * sc;
+            ok = std::fabs(v - std::nearbyint(v)) < 1e-9 && std::fabs(v) < 1e12;
+          }
+      }
+      if (ok) scale_pow = k;
+    }
+    int_mode = scale_pow >= 0;
+    if (int_mode) {
+      const double sc = std::ldexp(1.0, scale_pow);
+      gtab_i.assign((size_t)n_glob * n_sites * d_max, 0);
+      rmin_i.assign((size_t)n_glob * (n_sites + 1), 0);
+      rmax_i.assign((size_t)n_glob * (n_sites + 1), 0);
+      tgt_i.assign(n_glob, 0);
+      for (int g = 0; g < n_glob; ++g) {
+        tgt_i[g] = (int64_t)std::llround(glob_sectors[g] * sc);
+        for (int s = n_sites - 1; s >= 0; --s) {
+          int64_t mn = INT64_MAX, mx = INT64_MIN;
+          for (int a = 0; a < loc_dims[s]; ++a) {
+            const int64_t v = (int64_t)std::llround(glob_diags[((size_t)g * n_sites + s) * d_max + a] * sc);
+            gtab_i[((size_t)g * n_sites + s) * d_max + a] = v;
+            mn = std::min(mn, v);
+            mx = std::max(mx, v);
+          }
+          rmin_i[g * (n_sites + 1) + s] = rmin_i[g * (n_sites + 1) + s + 1] + mn;
+          rmax_i[g * (n_sites + 1) + s] = rmax_i[g * (n_sites + 1) + s + 1] + mx;
+        }
+      }
+    }
+  }
+  DevBuf b_gti, b_rmi, b_rxi, b_tgi;
+  IntGlob IG{nullptr, nullptr, nullptr, nullptr};
+  if (int_mode) {
+    const int64_t z = 0;
+    EDLGT_TRY(upload(b_gti, gtab_i.empty() ? &z : gtab_i.data(), std::max<size_t>(gtab_i.size(), 1), st));
+    EDLGT_TRY(upload(b_rmi, rmin_i.empty() ? &z : rmin_i.data(), std::max<size_t>(rmin_i.size(), 1), st));
+    EDLGT_TRY(upload(b_rxi, rmax_i.empty() ? &z : rmax_i.data(), std::max<size_t>(rmax_i.size(), 1), st));
+    EDLGT_TRY(upload(b_tgi, tgt_i.empty() ? &z : tgt_i.data(), std::max<size_t>(tgt_i.size(), 1), st));
+    IG = IntGlob{b_gti.as<int64_t>(), b_rmi.as<int64_t>(), b_rxi.as<int64_t>(), b_tgi.as<int64_t>()};
+  }
+
   // Phase 1: breadth-first frontier.
-  const int64_t kTargetFrontier = 1 << 18;
+  const int64_t kTargetFrontier = 1 << 20;
   const int64_t kMaxCandidates = (int64_t)1 << 27;
   DevBuf front;
   EDLGT_TRY(front.alloc(sizeof(uint64_t)));
@@ -358,9 +528,14 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   EDLGT_TRY(counts.alloc(n_front * sizeof(int64_t)));
   EDLGT_TRY(offs.alloc((n_front + 1) * sizeof(int64_t)));
   const unsigned g2 = grid_for(n_front, 64);
-  hipLaunchKernelGGL(complete_prefixes<false>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
-                     n_front, p, counts.as<int64_t>(), (const int64_t*)nullptr, (uint64_t*)nullptr,
-                     (uint8_t*)nullptr, 0);
+  if (int_mode)
+    hipLaunchKernelGGL(complete_prefixes_int<false>, dim3(g2), dim3(64), 0, st, D, IG,
+                       front.as<uint64_t>(), n_front, p, counts.as<int64_t>(),
+                       (const int64_t*)nullptr, (uint64_t*)nullptr, (uint8_t*)nullptr, 0);
+  else
+    hipLaunchKernelGGL(complete_prefixes<false>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
+                       n_front, p, counts.as<int64_t>(), (const int64_t*)nullptr, (uint64_t*)nullptr,
+                       (uint8_t*)nullptr, 0);
   EDLGT_HIP(hipGetLastError());
   EDLGT_TRY(exclusive_scan_i64(counts.as<int64_t>(), offs.as<int64_t>(), n_front, st));
   int64_t N = 0;
@@ -370,9 +545,14 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   EDLGT_TRY(out->keys.alloc(std::max<int64_t>(N, 1) * sizeof(uint64_t)));
   EDLGT_TRY(out->configs.alloc(std::max<int64_t>(N, 1) * out->cfg_stride));
   EDLGT_HIP(hipMemsetAsync(out->configs.p, 0, std::max<int64_t>(N, 1) * out->cfg_stride, st));
-  hipLaunchKernelGGL(complete_prefixes<true>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
-                     n_front, p, (int64_t*)nullptr, offs.as<int64_t>(), out->keys.as<uint64_t>(),
-                     out->configs.as<uint8_t>(), out->cfg_stride);
+  if (int_mode)
+    hipLaunchKernelGGL(complete_prefixes_int<true>, dim3(g2), dim3(64), 0, st, D, IG,
+                       front.as<uint64_t>(), n_front, p, (int64_t*)nullptr, offs.as<int64_t>(),
+                       out->keys.as<uint64_t>(), out->configs.as<uint8_t>(), out->cfg_stride);
+  else
+    hipLaunchKernelGGL(complete_prefixes<true>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
+                       n_front, p, (int64_t*)nullptr, offs.as<int64_t>(), out->keys.as<uint64_t>(),
+                       out->configs.as<uint8_t>(), out->cfg_stride);
   EDLGT_HIP(hipGetLastError());
   EDLGT_HIP(hipStreamSynchronize(st));
   return EDLGT_OK;

@@ -14,6 +14,8 @@
 // Value/vector types: FP64 real or complex values with FP64 vectors; FP32
 // values (mixed precision) with FP64 or FP32 gathered vectors; accumulation
 // is always FP64.
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -73,15 +75,26 @@ __device__ inline double2 group_sum(double2 v) {
 // VT: stored value type; XT: gathered vector element type; AT: accumulator /
 // output element type (double or double2); NB right-hand sides, column-major
 // with leading dimension ld.
+//
+// Row partition: the grid is a multiple of the 8 XCDs and workgroup b runs on
+// XCD b % 8, so logical block L = (b % 8) * (G / 8) + b / 8 gives every XCD one
+// contiguous slab of rows, and each block a contiguous chunk inside it.  Rows
+// that are close share most gathered x lines (a term moves a row to nearby
+// columns for nearby rows), so those lines are fetched once into the XCD's L2
+// instead of once per XCD.
 template <class VT, class XT, class AT, int G, int NB>
 __global__ void __launch_bounds__(256) csr_spmm_kernel(int64_t n, const int64_t* __restrict__ row_ptr,
                                                        const int32_t* __restrict__ col,
                                                        const VT* __restrict__ val,
                                                        const XT* __restrict__ x, int64_t ldx,
-                                                       AT* __restrict__ y, int64_t ldy) {
+                                                       AT* __restrict__ y, int64_t ldy,
+                                                       int64_t rows_per_block) {
   const int sub = threadIdx.x & (G - 1);
-  const int64_t groups_per_grid = (int64_t)gridDim.x * (256 / G);
-  for (int64_t r = (int64_t)blockIdx.x * (256 / G) + (threadIdx.x / G); r < n; r += groups_per_grid) {
+  const unsigned nb = gridDim.x;
+  const unsigned L = (nb % 8u == 0u) ? (blockIdx.x % 8u) * (nb / 8u) + blockIdx.x / 8u : blockIdx.x;
+  const int64_t r_beg = (int64_t)L * rows_per_block;
+  const int64_t r_end = min(n, r_beg + rows_per_block);
+  for (int64_t r = r_beg + (threadIdx.x / G); r < r_end; r += 256 / G) {
     const int64_t beg = row_ptr[r], end = row_ptr[r + 1];
     AT acc[NB];
 #pragma unroll
@@ -101,13 +114,19 @@ __global__ void __launch_bounds__(256) csr_spmm_kernel(int64_t n, const int64_t*
 }
 
 template <class VT, class XT, class AT, int NB>
-int launch_g(int g, int64_t n, const int64_t* rp, const int32_t* col, const VT* val, const XT* x,
-             int64_t ldx, AT* y, int64_t ldy, hipStream_t st) {
-  auto grid = [&](int G) { return grid_for(n, 256 / G, (int64_t)1 << 20); };
-#define EDLGT_SPMM_CASE(GG)                                                                     \
-  case GG:                                                                                      \
-    hipLaunchKernelGGL((csr_spmm_kernel<VT, XT, AT, GG, NB>), dim3(grid(GG)), dim3(256), 0, st, \
-                       n, rp, col, val, x, ldx, y, ldy);                                        \
+int launch_g(int g, int64_t grid, int64_t n, const int64_t* rp, const int32_t* col, const VT* val,
+             const XT* x, int64_t ldx, AT* y, int64_t ldy, hipStream_t st) {
+  // Default: 8 XCDs x 32 CUs with ~8 resident 256-thread blocks each -> 2048
+  // blocks, every block owning a contiguous chunk of rows.
+  const int rows_per_pass = 256 / g;
+  int64_t blocks = std::min<int64_t>(grid > 0 ? grid : 2048,
+                                     std::max<int64_t>(8, (n + rows_per_pass - 1) / rows_per_pass));
+  blocks = (blocks + 7) / 8 * 8;
+  const int64_t rpb = (n + blocks - 1) / blocks;
+#define EDLGT_SPMM_CASE(GG)                                                                        \
+  case GG:                                                                                         \
+    hipLaunchKernelGGL((csr_spmm_kernel<VT, XT, AT, GG, NB>), dim3((unsigned)blocks), dim3(256), 0, \
+                       st, n, rp, col, val, x, ldx, y, ldy, rpb);                                   \
     break;
   switch (g) {
     EDLGT_SPMM_CASE(4)
@@ -124,13 +143,13 @@ int launch_g(int g, int64_t n, const int64_t* rp, const int32_t* col, const VT* 
 }
 
 template <class VT, class XT, class AT>
-int launch_nb(int g, int nb, int64_t n, const int64_t* rp, const int32_t* col, const VT* val,
-              const XT* x, int64_t ldx, AT* y, int64_t ldy, hipStream_t st) {
+int launch_nb(int g, int64_t grid, int nb, int64_t n, const int64_t* rp, const int32_t* col,
+              const VT* val, const XT* x, int64_t ldx, AT* y, int64_t ldy, hipStream_t st) {
   switch (nb) {
-    case 1: return launch_g<VT, XT, AT, 1>(g, n, rp, col, val, x, ldx, y, ldy, st);
-    case 2: return launch_g<VT, XT, AT, 2>(g, n, rp, col, val, x, ldx, y, ldy, st);
-    case 4: return launch_g<VT, XT, AT, 4>(g, n, rp, col, val, x, ldx, y, ldy, st);
-    case 8: return launch_g<VT, XT, AT, 8>(g, n, rp, col, val, x, ldx, y, ldy, st);
+    case 1: return launch_g<VT, XT, AT, 1>(g, grid, n, rp, col, val, x, ldx, y, ldy, st);
+    case 2: return launch_g<VT, XT, AT, 2>(g, grid, n, rp, col, val, x, ldx, y, ldy, st);
+    case 4: return launch_g<VT, XT, AT, 4>(g, grid, n, rp, col, val, x, ldx, y, ldy, st);
+    case 8: return launch_g<VT, XT, AT, 8>(g, grid, n, rp, col, val, x, ldx, y, ldy, st);
   }
   return set_error(EDLGT_ERR_INVALID_ARG, "unsupported block width");
 }
@@ -170,24 +189,24 @@ int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, i
     int s;
     if (cplx) {
       if (!lo && xtype == EDLGT_VALUE_C128)
-        s = launch_nb(g, nb, n, rp, col, h.val.as<double2>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val.as<double2>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
       else if (lo && xtype == EDLGT_VALUE_C128)
-        s = launch_nb(g, nb, n, rp, col, h.val_lo.as<float2>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val_lo.as<float2>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
       else if (lo && xtype == EDLGT_VALUE_C64)
-        s = launch_nb(g, nb, n, rp, col, h.val_lo.as<float2>(), (const float2*)xb, ldx, (double2*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val_lo.as<float2>(), (const float2*)xb, ldx, (double2*)yb, ldy, st);
       else
         return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
     } else {
       if (!lo && xtype == EDLGT_VALUE_F64)
-        s = launch_nb(g, nb, n, rp, col, h.val.as<double>(), (const double*)xb, ldx, (double*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val.as<double>(), (const double*)xb, ldx, (double*)yb, ldy, st);
       else if (!lo && xtype == EDLGT_VALUE_C128)
-        s = launch_nb(g, nb, n, rp, col, h.val.as<double>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val.as<double>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
       else if (lo && xtype == EDLGT_VALUE_F64)
-        s = launch_nb(g, nb, n, rp, col, h.val_lo.as<float>(), (const double*)xb, ldx, (double*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val_lo.as<float>(), (const double*)xb, ldx, (double*)yb, ldy, st);
       else if (lo && xtype == EDLGT_VALUE_F32)
-        s = launch_nb(g, nb, n, rp, col, h.val_lo.as<float>(), (const float*)xb, ldx, (double*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val_lo.as<float>(), (const float*)xb, ldx, (double*)yb, ldy, st);
       else if (lo && xtype == EDLGT_VALUE_C128)
-        s = launch_nb(g, nb, n, rp, col, h.val_lo.as<float>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
+        s = launch_nb(g, h.grid_override, nb, n, rp, col, h.val_lo.as<float>(), (const double2*)xb, ldx, (double2*)yb, ldy, st);
       else
         return set_error(EDLGT_ERR_INVALID_ARG, "unsupported vector type for a real matrix");
     }
@@ -218,6 +237,59 @@ int csr_make_low_precision(Csr& h, hipStream_t st) {
   EDLGT_TRY(h.val_lo.alloc(std::max<int64_t>(ncomp, 1) * sizeof(float)));
   EDLGT_TRY(convert_f64_to_f32(h.val.as<double>(), h.val_lo.as<float>(), ncomp, st));
   EDLGT_HIP(hipStreamSynchronize(st));
+  return EDLGT_OK;
+}
+
+int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
+                 hipStream_t st, int32_t* best_g, int64_t* best_grid, float* best_ms) {
+  const double avg = h.n_rows > 0 ? (double)h.nnz / (double)h.n_rows : 1.0;
+  const int gs[] = {4, 8, 16, 32, 64};
+  const int64_t grids[] = {1024, 2048, 4096, 8192, 16384};
+  hipEvent_t e0, e1;
+  EDLGT_HIP(hipEventCreate(&e0));
+  EDLGT_HIP(hipEventCreate(&e1));
+  const int32_t save_g = h.g_override;
+  const int64_t save_grid = h.grid_override;
+  float best = 1e30f;
+  int32_t bg = 0;
+  int64_t bgrid = 0;
+  int status = EDLGT_OK;
+  for (int g : gs) {
+    if (g > 8 && g > 2 * avg) continue;
+    for (int64_t grid : grids) {
+      h.g_override = g;
+      h.grid_override = grid;
+      status = csr_spmm(h, precision, xtype, n_rhs, x, h.n_rows, y, h.n_rows, st);  // warm
+      if (status != EDLGT_OK) break;
+      (void)hipEventRecord(e0, st);
+      const int reps = 3;
+      for (int r = 0; r < reps && status == EDLGT_OK; ++r)
+        status = csr_spmm(h, precision, xtype, n_rhs, x, h.n_rows, y, h.n_rows, st);
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= reps;
+      if (status == EDLGT_OK && ms < best) {
+        best = ms;
+        bg = g;
+        bgrid = grid;
+      }
+    }
+    if (status != EDLGT_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (status != EDLGT_OK) {
+    h.g_override = save_g;
+    h.grid_override = save_grid;
+    return status;
+  }
+  h.g_override = bg;
+  h.grid_override = bgrid;
+  if (best_g) *best_g = bg;
+  if (best_grid) *best_grid = bgrid;
+  if (best_ms) *best_ms = best;
   return EDLGT_OK;
 }
 

@@ -57,6 +57,8 @@ SIGNATURES = {
     "edlgt_csr_destroy": (_i32, [_p]),
     "edlgt_spmv": (_i32, [_p, _i32, _p, _p, _i32, _p]),
     "edlgt_spmm": (_i32, [_p, _i32, _i32, _p, _p, _i32, _p]),
+    "edlgt_csr_autotune": (_i32, [_p, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p]),
+    "edlgt_csr_set_launch": (_i32, [_p, _i32, _i64]),
     "edlgt_lanczos": (_i32, [_p, _i32, _f64, _i32, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
 }
 

@@ -214,6 +214,23 @@ class CSRMatrix:
                                         L.stream_handle(stream)))
         return y
 
+    def autotune(self, n_rhs=1, precision=L.PREC_FP64, complex_vectors=None, stream=None):
+        """Times the SpMM launch shapes on scratch vectors and keeps the fastest.
+        Returns (lanes_per_row, workgroups, ms)."""
+        torch = _torch()
+        vt = self.vec_type(complex_vectors)
+        dt = torch.complex128 if vt == L.VALUE_C128 else torch.float64
+        X = torch.ones((n_rhs, self.n), dtype=dt, device="cuda")
+        Y = torch.empty_like(X)
+        g, grid, ms = C.c_int32(), C.c_int64(), C.c_float()
+        L.check(L.load().edlgt_csr_autotune(self._h, precision, vt, n_rhs, L.ptr(X), L.ptr(Y),
+                                            L.stream_handle(stream), C.byref(g), C.byref(grid),
+                                            C.byref(ms)))
+        return int(g.value), int(grid.value), float(ms.value)
+
+    def set_launch(self, lanes_per_row=0, grid=0):
+        L.check(L.load().edlgt_csr_set_launch(self._h, lanes_per_row, grid))
+
     def lanczos(self, n_eigs, tol=1e-12, max_basis=64, max_iter=100000, precision=L.PREC_FP64,
                 v0=None, complex_vectors=None, want_vectors=True, stream=None):
         """Lowest eigenpairs: (evals ascending, evecs tensor (k, N) or None, resid, matvecs)."""

@@ -153,15 +153,27 @@ int edlgt_csr_destroy(edlgt_csr* h);
 /* ======================================================================
  * Sparse products.  Replace csr_spmv / csr_matmat (edlgt/modeling/
  * qmb_hamiltonian.py:705-764) behind the LinearOperator matvec/matmat seam
- * (:186-216).  x, y, X, Y are device arrays of vec_type (EDLGT_VALUE_C128 or
- * EDLGT_VALUE_F64; F64 only for real matrices).  X, Y are n_rhs vectors of
- * length n stored one after the other (column-major N x n_rhs).
- * precision EDLGT_PREC_MIXED reads the FP32 value copy (built on demand).
+ * (:186-216).  x, X are device arrays of vec_type: EDLGT_VALUE_C128 or
+ * EDLGT_VALUE_F64 (F64 only for real matrices); with EDLGT_PREC_MIXED also
+ * EDLGT_VALUE_C64 / EDLGT_VALUE_F32 (FP32 gathered vector).  y, Y are FP64
+ * (complex128 for a complex matrix or complex vectors, else float64).  X, Y
+ * are n_rhs vectors of length n stored one after the other (column-major
+ * N x n_rhs).  precision EDLGT_PREC_MIXED reads the FP32 value copy (built on
+ * demand); accumulation is FP64 in every mode.
  * ====================================================================== */
 int edlgt_spmv(edlgt_csr* h, int32_t precision, const void* x, void* y, int32_t vec_type,
                void* stream);
 int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, void* Y,
                int32_t vec_type, void* stream);
+
+/* Times the SpMM launch shapes (lanes per row x workgroups) on X -> Y with HIP
+ * events and keeps the fastest for this matrix (no counterpart in the
+ * reference: CPU csr_spmv has no launch shape).  Outputs may be NULL. */
+int edlgt_csr_autotune(edlgt_csr* h, int32_t precision, int32_t vec_type, int32_t n_rhs,
+                       const void* X, void* Y, void* stream, int32_t* out_lanes, int64_t* out_grid,
+                       float* out_ms);
+/* Sets the launch shape explicitly (lanes_per_row 0 and grid 0 restore the defaults). */
+int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid);
 
 /* ======================================================================
  * Lowest eigenpairs by thick-restart Lanczos with full reorthogonalisation.
