@@ -26,6 +26,7 @@
 #include <complex>
 
 #include "common.h"
+#include "kernels.h"
 
 namespace edlgt {
 namespace {
@@ -432,6 +433,7 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
     EDLGT_TRY(launch_any(cap, true, A, nullptr, out->row_ptr.as<int64_t>(), out->col.as<int32_t>(),
                          out->val.as<double>(), st));
   EDLGT_TRY(check_err(b_err, st));
+  EDLGT_TRY(csr_prepare_stream(*out, st));
   return EDLGT_OK;
 }
 

@@ -291,4 +291,31 @@ int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis, i
                      out_evecs, out_resid, out_matvecs, (hipStream_t)stream);
 }
 
+int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
+                          double* out_host, void* stream) {
+  EDLGT_REQUIRE(b && psi && out_host, "null argument");
+  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64, "vec_type must be C128 or F64");
+  const int bins = b->n_sites * d_max;
+  EDLGT_REQUIRE(d_max >= 1 && bins <= 4096, "n_sites * d_max must be <= 4096");
+  DevBuf out;
+  EDLGT_TRY(out.alloc((size_t)bins * sizeof(double)));
+  EDLGT_TRY(basis_marginals(*b, psi, vec_type == EDLGT_VALUE_C128, d_max, out.as<double>(),
+                            (hipStream_t)stream));
+  EDLGT_HIP(hipMemcpy(out_host, out.p, (size_t)bins * sizeof(double), hipMemcpyDeviceToHost));
+  return EDLGT_OK;
+}
+
+int edlgt_krylov_evolve(edlgt_csr* h, const edlgt_basis* b, int32_t precision, void* psi, double dt,
+                        int64_t n_steps, int32_t m, int32_t reorth, int32_t d_max,
+                        double* out_marginals, double* out_err, void* psi_history, void* stream) {
+  EDLGT_REQUIRE(h && psi, "null argument");
+  EDLGT_REQUIRE(n_steps >= 0 && m >= 1, "n_steps must be >= 0 and m >= 1");
+  EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
+  EDLGT_REQUIRE(!out_marginals || (b && d_max >= 1 && b->n_sites * d_max <= 4096),
+                "marginals need the basis and n_sites * d_max <= 4096");
+  EDLGT_REQUIRE(!b || b->n_states == h->n_rows, "basis and matrix sizes differ");
+  return krylov_run(*h, b, precision, psi, dt, n_steps, m, reorth, d_max, out_marginals, out_err,
+                    psi_history, (hipStream_t)stream);
+}
+
 }  // extern "C"

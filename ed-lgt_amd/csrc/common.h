@@ -97,6 +97,9 @@ struct Csr {
   DevBuf col;       // int32 [nnz]
   DevBuf val;       // value_type [nnz]
   DevBuf val_lo;    // reduced-precision copy (complex64 / float32), built on demand
+  DevBuf stream_blocks;      // int64 [n_stream_blocks + 1] first row of every CSR-stream tile
+  DevBuf stream_pos;         // int64 [n_stream_blocks + 1] first entry of every tile
+  int64_t n_stream_blocks = 0;
 };
 
 inline size_t value_bytes(int32_t vt) {

@@ -26,6 +26,8 @@ int lanes_per_row(const Csr& h);
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx,
              void* y, int64_t ldy, hipStream_t st);
 int csr_make_low_precision(Csr& h, hipStream_t st);
+// Row partition of the CSR-stream SpMV (run once per matrix).
+int csr_prepare_stream(Csr& h, hipStream_t st);
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
                  hipStream_t st, int32_t* best_g, int64_t* best_grid, float* best_ms);
 int convert_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t st);
@@ -33,5 +35,12 @@ int convert_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t st);
 int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter, int precision,
                 int vec_type, const void* v0_host, double* evals, void* evecs, double* resid,
                 int64_t* matvecs, hipStream_t st);
+
+int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
+               int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
+               hipStream_t st);
+
+int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
+                    hipStream_t st);
 
 }  // namespace edlgt

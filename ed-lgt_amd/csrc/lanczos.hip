@@ -296,6 +296,7 @@ struct Solver {
   DevBuf V, W, Tmp, part, gdev, cdev, shadow;
   int64_t mv64 = 0, mv32 = 0;
   uint64_t seed = 12345;
+  bool need_tmp = true;  // restart / Ritz buffer (not needed by Krylov stepping)
 
   Solver(Csr& hh, hipStream_t s, int prec) : h(hh), st(s), precision(prec), N(hh.n_rows) {}
 
@@ -304,7 +305,7 @@ struct Solver {
   int setup() {
     EDLGT_TRY(V.alloc((size_t)(M + b) * N * sizeof(S)));
     EDLGT_TRY(W.alloc((size_t)b * N * sizeof(S)));
-    EDLGT_TRY(Tmp.alloc((size_t)std::max(M, 8) * N * sizeof(S)));
+    if (need_tmp) EDLGT_TRY(Tmp.alloc((size_t)std::max(M, 8) * N * sizeof(S)));
     EDLGT_TRY(part.alloc((size_t)kGramBlocks * (M + b) * 8 * sizeof(double2)));
     EDLGT_TRY(gdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
     EDLGT_TRY(cdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
@@ -594,6 +595,84 @@ struct Solver {
     return EDLGT_OK;
   }
 
+  // One Krylov step psi <- exp(-i H dt) psi (Lanczos basis of dimension m,
+  // three-term recurrence plus optional full CGS2 reorthogonalisation).
+  // V[:, 0..m] holds the basis; the projected exponential is formed on the
+  // host from the eigendecomposition of the real tridiagonal T.
+  // err receives the standard a-posteriori estimate beta_m |(e^{-iT dt} e_1)_m|.
+  int krylov_step(S* psi, double dt, int m, bool reorth, double& err, int& m_used) {
+    Dense nn;
+    EDLGT_TRY(gram(psi, 1, psi, 1, nn));
+    const double beta0 = std::sqrt(std::max(nn(0, 0).real(), 0.0));
+    err = 0.0;
+    if (beta0 == 0.0) {
+      m_used = 0;
+      return EDLGT_OK;
+    }
+    Dense sc(1, 1);
+    sc(0, 0) = 1.0 / beta0;
+    EDLGT_TRY(update(psi, 1, sc, col(0), 1, 0));
+    std::vector<double> alpha, beta;
+    double hnorm = 0.0;
+    int me = m;
+    for (int j = 0; j < m; ++j) {
+      EDLGT_TRY(matvec(col(j), W.as<S>(), 1));
+      double a = 0.0;
+      if (reorth) {
+        for (int pass = 0; pass < 2; ++pass) {
+          Dense G;
+          EDLGT_TRY(gram(V.as<S>(), j + 1, W.as<S>(), 1, G));
+          a += G(j, 0).real();
+          for (auto& z : G.a) z = -z;
+          EDLGT_TRY(update(V.as<S>(), j + 1, G, W.as<S>(), 1, 1));
+        }
+      } else {
+        Dense G;
+        EDLGT_TRY(gram(col(j), 1, W.as<S>(), 1, G));
+        a = G(0, 0).real();
+        const int lo = j > 0 ? j - 1 : j;
+        Dense C(j - lo + 1, 1);
+        if (j > 0) C(0, 0) = -beta[j - 1];
+        C(j - lo, 0) = -a;
+        EDLGT_TRY(update(col(lo), j - lo + 1, C, W.as<S>(), 1, 1));
+      }
+      alpha.push_back(a);
+      Dense bn;
+      EDLGT_TRY(gram(W.as<S>(), 1, W.as<S>(), 1, bn));
+      const double b = std::sqrt(std::max(bn(0, 0).real(), 0.0));
+      beta.push_back(b);
+      hnorm = std::max(hnorm, std::fabs(a) + b + (j > 0 ? beta[j - 1] : 0.0));
+      if (b <= 1e-14 * std::max(hnorm, 1e-300)) {  // invariant subspace: exact
+        me = j + 1;
+        break;
+      }
+      Dense s1(1, 1);
+      s1(0, 0) = 1.0 / b;
+      EDLGT_TRY(update(W.as<S>(), 1, s1, col(j + 1), 1, 0));
+    }
+    Dense T(me, me);
+    for (int j = 0; j < me; ++j) {
+      T(j, j) = alpha[j];
+      if (j + 1 < me) {
+        T(j, j + 1) = beta[j];
+        T(j + 1, j) = beta[j];
+      }
+    }
+    std::vector<double> lam;
+    Dense Sv;
+    hermitian_eig(T, lam, Sv);
+    Dense c(me, 1);
+    for (int k = 0; k < me; ++k) {
+      cd z = 0.0;
+      for (int l = 0; l < me; ++l) z += Sv(k, l) * std::exp(cd(0.0, -lam[l] * dt)) * std::conj(Sv(0, l));
+      c(k, 0) = beta0 * z;
+    }
+    err = (me == m) ? beta[m - 1] * std::abs(c(m - 1, 0)) / beta0 : 0.0;
+    EDLGT_TRY(update(V.as<S>(), me, c, psi, 1, 0));
+    m_used = me;
+    return EDLGT_OK;
+  }
+
   // True residual norms ||H y_i - theta_i y_i|| in FP64 for Y = Tmp[:, 0:k].
   int residuals(int k, const std::vector<double>& theta, std::vector<double>& out) {
     out.assign(k, 0.0);
@@ -679,7 +758,53 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   return EDLGT_OK;
 }
 
+int krylov_typed(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
+                 int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
+                 hipStream_t st) {
+  using S = double2;
+  const int64_t N = h.n_rows;
+  if (m < 1 || (int64_t)m > N) m = (int)std::max<int64_t>(1, std::min<int64_t>(m, N));
+  Solver<S> sv(h, st, precision);
+  sv.b = 1;
+  sv.M = m;
+  sv.need_tmp = false;
+  EDLGT_TRY(sv.setup());
+  DevBuf marg;
+  const int bins = basis ? basis->n_sites * d_max : 0;
+  if (out_marg) {
+    if (!basis) return set_error(EDLGT_ERR_INVALID_ARG, "marginals need the sector basis");
+    EDLGT_TRY(marg.alloc((size_t)bins * sizeof(double)));
+  }
+  auto record = [&](int64_t t) -> int {
+    if (out_marg) {
+      EDLGT_TRY(basis_marginals(*basis, psi, true, d_max, marg.as<double>(), st));
+      EDLGT_HIP(hipMemcpy(out_marg + t * bins, marg.p, (size_t)bins * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (history)
+      EDLGT_HIP(hipMemcpyAsync(static_cast<S*>(history) + t * N, psi, (size_t)N * sizeof(S),
+                               hipMemcpyDeviceToDevice, st));
+    return EDLGT_OK;
+  };
+  EDLGT_TRY(record(0));
+  for (int64_t t = 0; t < n_steps; ++t) {
+    double err = 0.0;
+    int mu = 0;
+    EDLGT_TRY(sv.krylov_step(static_cast<S*>(psi), dt, m, reorth != 0, err, mu));
+    if (out_err) out_err[t] = err;
+    EDLGT_TRY(record(t + 1));
+  }
+  EDLGT_HIP(hipStreamSynchronize(st));
+  return EDLGT_OK;
+}
+
 }  // namespace
+
+int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
+               int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
+               hipStream_t st) {
+  return krylov_typed(h, basis, precision, psi, dt, n_steps, m, reorth, d_max, out_marg, out_err,
+                      history, st);
+}
 
 int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter, int precision,
                 int vec_type, const void* v0_host, double* evals, void* evecs, double* resid,

@@ -1,0 +1,92 @@
+// Diagonal local observables on the device.
+//
+// Every diagonal single-site operator O_i has <psi|O_i|psi> =
+// sum_a P_i(a) O_i[a][a], with P_i(a) = sum_r |psi_r|^2 [c_r[i] == a] the
+// marginal distribution of site i's local state.  One pass over psi and the
+// configuration bytes therefore gives every LocalTerm.get_expval result
+// (edlgt/modeling/local_term.py:119-237, exp_val_data qmb_state.py:630-653)
+// for all sites and all diagonal operators at once (the reference recomputes
+// the triplets of each site and operator per call).
+//
+// Determinism: |psi_r|^2 is converted to 2^62-scaled fixed point and summed
+// with integer atomics in LDS (associative, so the result is independent of
+// scheduling); per-block partial histograms are summed in block order.  The
+// absolute error is at most N * 2^-62 (< 1e-10 for N < 4.6e8).
+#include <algorithm>
+
+#include "common.h"
+
+namespace edlgt {
+namespace {
+
+constexpr int kMaxBins = 4096;
+constexpr double kScale = 4611686018427387904.0;  // 2^62
+
+template <bool kCplx>
+__global__ void __launch_bounds__(256) marginal_partial(const double* __restrict__ psi, int64_t n,
+                                                       const uint8_t* __restrict__ cfg, int cfg_stride,
+                                                       int n_sites, int d_max,
+                                                       unsigned long long* __restrict__ partial) {
+  __shared__ unsigned long long hist[kMaxBins];
+  const int bins = n_sites * d_max;
+  for (int i = threadIdx.x; i < bins; i += blockDim.x) hist[i] = 0ull;
+  __syncthreads();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = (int64_t)blockIdx.x * chunk;
+  const int64_t r1 = min(n, r0 + chunk);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    double w;
+    if (kCplx) {
+      const double re = psi[2 * r], im = psi[2 * r + 1];
+      w = re * re + im * im;
+    } else {
+      w = psi[r] * psi[r];
+    }
+    const unsigned long long q = (unsigned long long)(w * kScale);
+    if (q == 0ull) continue;
+    const uint8_t* row = cfg + r * cfg_stride;
+    for (int i = 0; i < n_sites; ++i) atomicAdd(&hist[i * d_max + row[i]], q);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < bins; i += blockDim.x)
+    partial[(int64_t)blockIdx.x * bins + i] = hist[i];
+}
+
+__global__ void marginal_finish(const unsigned long long* __restrict__ partial, int nblk, int bins,
+                                double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= bins) return;
+  unsigned long long s = 0ull;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * bins + i];
+  out[i] = (double)s / kScale;
+}
+
+}  // namespace
+
+// out_dev: device double [n_sites * d_max]
+int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
+                    hipStream_t st) {
+  const int bins = B.n_sites * d_max;
+  if (bins > kMaxBins) return set_error(EDLGT_ERR_INVALID_ARG, "n_sites * d_max exceeds 4096");
+  for (int s = 0; s < B.n_sites; ++s)
+    if (B.loc_dims[s] > d_max) return set_error(EDLGT_ERR_INVALID_ARG, "d_max below a local dimension");
+  const int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (B.n_states + 4095) / 4096));
+  DevBuf part;
+  EDLGT_TRY(part.alloc((size_t)nblk * bins * sizeof(unsigned long long)));
+  if (cplx)
+    hipLaunchKernelGGL(marginal_partial<true>, dim3(nblk), dim3(256), 0, st, (const double*)psi,
+                       B.n_states, B.configs.as<uint8_t>(), B.cfg_stride, B.n_sites, d_max,
+                       part.as<unsigned long long>());
+  else
+    hipLaunchKernelGGL(marginal_partial<false>, dim3(nblk), dim3(256), 0, st, (const double*)psi,
+                       B.n_states, B.configs.as<uint8_t>(), B.cfg_stride, B.n_sites, d_max,
+                       part.as<unsigned long long>());
+  EDLGT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(marginal_finish, dim3((bins + 255) / 256), dim3(256), 0, st,
+                     part.as<unsigned long long>(), nblk, bins, out_dev);
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_HIP(hipStreamSynchronize(st));
+  return EDLGT_OK;
+}
+
+}  // namespace edlgt

@@ -55,6 +55,8 @@ __device__ inline void fma_acc(double2& acc, double a, double2 b) {
 }
 __device__ inline void fma_acc(double& acc, double a, double b) { acc += a * b; }
 
+__device__ inline double2 operator+(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
 template <class A> __device__ inline A zero_acc();
 template <> __device__ inline double zero_acc<double>() { return 0.0; }
 template <> __device__ inline double2 zero_acc<double2>() { return make_double2(0.0, 0.0); }
@@ -72,29 +74,22 @@ __device__ inline double2 group_sum(double2 v) {
   return v;
 }
 
+// Vector-CSR product (G lanes per row), used for several right-hand sides.
 // VT: stored value type; XT: gathered vector element type; AT: accumulator /
 // output element type (double or double2); NB right-hand sides, column-major
-// with leading dimension ld.
-//
-// Row partition: the grid is a multiple of the 8 XCDs and workgroup b runs on
-// XCD b % 8, so logical block L = (b % 8) * (G / 8) + b / 8 gives every XCD one
-// contiguous slab of rows, and each block a contiguous chunk inside it.  Rows
-// that are close share most gathered x lines (a term moves a row to nearby
-// columns for nearby rows), so those lines are fetched once into the XCD's L2
-// instead of once per XCD.
+// with leading dimension ld.  Blocks take row groups in grid-stride order so
+// that the blocks in flight cover one compact window of rows: the gathered x
+// lines of neighbouring rows are then reused from L2 / MALL (measured faster
+// on MI355X than giving each XCD its own contiguous slab of rows).
 template <class VT, class XT, class AT, int G, int NB>
 __global__ void __launch_bounds__(256) csr_spmm_kernel(int64_t n, const int64_t* __restrict__ row_ptr,
                                                        const int32_t* __restrict__ col,
                                                        const VT* __restrict__ val,
                                                        const XT* __restrict__ x, int64_t ldx,
-                                                       AT* __restrict__ y, int64_t ldy,
-                                                       int64_t rows_per_block) {
+                                                       AT* __restrict__ y, int64_t ldy) {
   const int sub = threadIdx.x & (G - 1);
-  const unsigned nb = gridDim.x;
-  const unsigned L = (nb % 8u == 0u) ? (blockIdx.x % 8u) * (nb / 8u) + blockIdx.x / 8u : blockIdx.x;
-  const int64_t r_beg = (int64_t)L * rows_per_block;
-  const int64_t r_end = min(n, r_beg + rows_per_block);
-  for (int64_t r = r_beg + (threadIdx.x / G); r < r_end; r += 256 / G) {
+  const int64_t stride = (int64_t)gridDim.x * (256 / G);
+  for (int64_t r = (int64_t)blockIdx.x * (256 / G) + (threadIdx.x / G); r < n; r += stride) {
     const int64_t beg = row_ptr[r], end = row_ptr[r + 1];
     AT acc[NB];
 #pragma unroll
@@ -113,20 +108,163 @@ __global__ void __launch_bounds__(256) csr_spmm_kernel(int64_t n, const int64_t*
   }
 }
 
+// CSR-stream product for short rows (one right-hand side).
+//
+// The nonzeros are cut into tiles of kTile entries; tile k's block owns the
+// rows that start inside [k*kTile, (k+1)*kTile) (row_blocks[k] =
+// lower_bound(row_ptr, k*kTile), found on the device).  A kBS-thread block
+// streams its contiguous entries with coalesced non-temporal loads, each
+// thread keeping kUnroll independent gathers of x in flight, writes the
+// products to LDS, and then reduces every row sequentially from LDS (fixed
+// order -> bitwise reproducible).  Blocks run in row order, so the whole chip
+// sweeps a compact window of rows and the gathered x lines are reused from
+// L2 / MALL.  A block whose entries exceed the LDS capacity (a very long row)
+// falls back to a block-cooperative reduction per row.
+constexpr int kBS = 512;     // threads per block of the stream kernel
+constexpr int kTile = 2048;  // nonzeros per tile (measured best on MI355X among 512..4096)
+constexpr int kCap = kTile + 256;
+constexpr int kUnroll = 4;
+
+// 16-byte non-temporal value loads (one dwordx4 instead of two dwordx2)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+__device__ inline double2 load_val16(const double2* v, int64_t p) {
+  const dvec2 t = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(v) + p);
+  return make_double2(t.x, t.y);
+}
+__device__ inline double2 load_val16(const float2* v, int64_t p) {
+  const fvec2 t = __builtin_nontemporal_load(reinterpret_cast<const fvec2*>(v) + p);
+  return make_double2((double)t.x, (double)t.y);
+}
+__device__ inline double load_val16(const double* v, int64_t p) { return ldnt(v + p); }
+__device__ inline double load_val16(const float* v, int64_t p) { return (double)ldnt(v + p); }
+
+template <int G>
+__device__ inline double sum_lanes(double v) {
+#pragma unroll
+  for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <int G>
+__device__ inline double2 sum_lanes(double2 v) {
+  return make_double2(sum_lanes<G>(v.x), sum_lanes<G>(v.y));
+}
+
+// Row sums from the LDS products with G lanes per row (fixed per block).
+template <int G, class AT>
+__device__ inline void reduce_rows(const AT* prod, const int64_t* __restrict__ row_ptr, int64_t r0,
+                                   int64_t nr, int64_t p0, AT* __restrict__ y) {
+  const int sub = threadIdx.x & (G - 1);
+  constexpr int per = kBS / G;
+  for (int64_t i = threadIdx.x / G; i < (nr + per - 1) / per * per; i += per) {
+    AT s = zero_acc<AT>();
+    if (i < nr) {
+      const int64_t b = row_ptr[r0 + i] - p0, e = row_ptr[r0 + i + 1] - p0;
+      for (int64_t q = b + sub; q < e; q += G) s = s + prod[q];
+    }
+    s = sum_lanes<G>(s);
+    if (sub == 0 && i < nr) y[r0 + i] = s;
+  }
+}
+
+template <class VT, class XT, class AT>
+__global__ void __launch_bounds__(kBS) csr_stream_kernel(
+    int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const VT* __restrict__ val, const XT* __restrict__ x, AT* __restrict__ y,
+    const int64_t* __restrict__ row_blocks, const int64_t* __restrict__ block_pos) {
+  __shared__ AT prod[kCap];
+  const int64_t r0 = row_blocks[blockIdx.x], r1 = row_blocks[blockIdx.x + 1];
+  if (r0 >= r1) return;
+  const int64_t p0 = block_pos[blockIdx.x], p1 = block_pos[blockIdx.x + 1];
+  const int64_t cnt = p1 - p0;
+  const int64_t nr = r1 - r0;
+  const int tid = threadIdx.x;
+  if (cnt <= kCap) {
+    for (int64_t base = 0; base < cnt; base += kBS * kUnroll) {
+      int32_t c[kUnroll];
+      decltype(load_val16(val, 0)) v[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t i = base + u * kBS + tid;
+        if (i < cnt) {
+          c[u] = ldnt(col + p0 + i);
+          v[u] = load_val16(val, p0 + i);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t i = base + u * kBS + tid;
+        if (i < cnt) {
+          AT a = zero_acc<AT>();
+          fma_acc(a, v[u], load_x(x, c[u]));
+          prod[i] = a;
+        }
+      }
+    }
+    __syncthreads();
+    if (nr <= 64) reduce_rows<8>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= 128) reduce_rows<4>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= 256) reduce_rows<2>(prod, row_ptr, r0, nr, p0, y);
+    else reduce_rows<1>(prod, row_ptr, r0, nr, p0, y);
+  } else {
+    // long rows: all threads on one row at a time, fixed-shape tree sum
+    AT* red = prod;  // kCap >= kBS
+    for (int64_t r = r0; r < r1; ++r) {
+      AT s = zero_acc<AT>();
+      for (int64_t p = row_ptr[r] + tid; p < row_ptr[r + 1]; p += kBS)
+        fma_acc(s, load_val16(val, p), load_x(x, ldnt(col + p)));
+      red[tid] = s;
+      __syncthreads();
+      for (int o = kBS / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = red[tid] + red[tid + o];
+        __syncthreads();
+      }
+      if (tid == 0) y[r] = red[0];
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void stream_block_pos(const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ rb,
+                                 int64_t n_blocks, int64_t* __restrict__ pos) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n_blocks;
+       k += (int64_t)gridDim.x * blockDim.x)
+    pos[k] = row_ptr[rb[k]];
+}
+
+__global__ void stream_row_blocks(const int64_t* __restrict__ row_ptr, int64_t n, int64_t n_blocks,
+                                  int64_t* __restrict__ rb) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n_blocks;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    if (k == n_blocks) {
+      rb[k] = n;
+      continue;
+    }
+    const int64_t target = k * (int64_t)kTile;
+    int64_t lo = 0, len = n;  // first row with row_ptr[r] >= target
+    while (len > 0) {
+      const int64_t half = len >> 1;
+      if (row_ptr[lo + half] < target) {
+        lo += half + 1;
+        len -= half + 1;
+      } else {
+        len = half;
+      }
+    }
+    rb[k] = lo;
+  }
+}
+
 template <class VT, class XT, class AT, int NB>
 int launch_g(int g, int64_t grid, int64_t n, const int64_t* rp, const int32_t* col, const VT* val,
              const XT* x, int64_t ldx, AT* y, int64_t ldy, hipStream_t st) {
-  // Default: 8 XCDs x 32 CUs with ~8 resident 256-thread blocks each -> 2048
-  // blocks, every block owning a contiguous chunk of rows.
   const int rows_per_pass = 256 / g;
-  int64_t blocks = std::min<int64_t>(grid > 0 ? grid : 2048,
-                                     std::max<int64_t>(8, (n + rows_per_pass - 1) / rows_per_pass));
-  blocks = (blocks + 7) / 8 * 8;
-  const int64_t rpb = (n + blocks - 1) / blocks;
+  const int64_t blocks = std::min<int64_t>(grid > 0 ? grid : 65536,
+                                           std::max<int64_t>(1, (n + rows_per_pass - 1) / rows_per_pass));
 #define EDLGT_SPMM_CASE(GG)                                                                        \
   case GG:                                                                                         \
     hipLaunchKernelGGL((csr_spmm_kernel<VT, XT, AT, GG, NB>), dim3((unsigned)blocks), dim3(256), 0, \
-                       st, n, rp, col, val, x, ldx, y, ldy, rpb);                                   \
+                       st, n, rp, col, val, x, ldx, y, ldy);                                        \
     break;
   switch (g) {
     EDLGT_SPMM_CASE(4)
@@ -167,8 +305,68 @@ int lanes_per_row(const Csr& h) {
   return 64;
 }
 
+template <class VT, class XT, class AT>
+int launch_stream(const Csr& h, const VT* val, const XT* x, AT* y, hipStream_t st) {
+  hipLaunchKernelGGL((csr_stream_kernel<VT, XT, AT>), dim3((unsigned)h.n_stream_blocks), dim3(kBS), 0, st,
+                     h.n_rows, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), val, x, y,
+                     h.stream_blocks.as<int64_t>(), h.stream_pos.as<int64_t>());
+  EDLGT_HIP(hipGetLastError());
+  return EDLGT_OK;
+}
+
+
+int csr_prepare_stream(Csr& h, hipStream_t st) {
+  if (h.stream_blocks.p) return EDLGT_OK;
+  const int64_t nb = std::max<int64_t>(1, (h.nnz + kTile - 1) / kTile);
+  if (nb > 0x7fffffffLL) return set_error(EDLGT_ERR_INDEX_OVERFLOW, "too many stream blocks");
+  EDLGT_TRY(h.stream_blocks.alloc((nb + 1) * sizeof(int64_t)));
+  hipLaunchKernelGGL(stream_row_blocks, dim3(grid_for(nb + 1, 256)), dim3(256), 0, st,
+                     h.row_ptr.as<int64_t>(), h.n_rows, nb, h.stream_blocks.as<int64_t>());
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_TRY(h.stream_pos.alloc((nb + 1) * sizeof(int64_t)));
+  hipLaunchKernelGGL(stream_block_pos, dim3(grid_for(nb + 1, 256)), dim3(256), 0, st,
+                     h.row_ptr.as<int64_t>(), h.stream_blocks.as<int64_t>(), nb, h.stream_pos.as<int64_t>());
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_HIP(hipStreamSynchronize(st));
+  h.n_stream_blocks = nb;
+  return EDLGT_OK;
+}
+
+
+// Single right-hand side through the CSR-stream kernel.
+int spmv_stream(const Csr& h, int precision, int xtype, const void* xb, void* yb, hipStream_t st) {
+  const bool cplx = h.value_type == EDLGT_VALUE_C128;
+  const bool lo = precision == EDLGT_PREC_MIXED;
+  if (cplx) {
+    if (!lo && xtype == EDLGT_VALUE_C128)
+      return launch_stream(h, h.val.as<double2>(), (const double2*)xb, (double2*)yb, st);
+    if (lo && xtype == EDLGT_VALUE_C128)
+      return launch_stream(h, h.val_lo.as<float2>(), (const double2*)xb, (double2*)yb, st);
+    if (lo && xtype == EDLGT_VALUE_C64)
+      return launch_stream(h, h.val_lo.as<float2>(), (const float2*)xb, (double2*)yb, st);
+    return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
+  }
+  if (!lo && xtype == EDLGT_VALUE_F64)
+    return launch_stream(h, h.val.as<double>(), (const double*)xb, (double*)yb, st);
+  if (!lo && xtype == EDLGT_VALUE_C128)
+    return launch_stream(h, h.val.as<double>(), (const double2*)xb, (double2*)yb, st);
+  if (lo && xtype == EDLGT_VALUE_F64)
+    return launch_stream(h, h.val_lo.as<float>(), (const double*)xb, (double*)yb, st);
+  if (lo && xtype == EDLGT_VALUE_F32)
+    return launch_stream(h, h.val_lo.as<float>(), (const float*)xb, (double*)yb, st);
+  if (lo && xtype == EDLGT_VALUE_C128)
+    return launch_stream(h, h.val_lo.as<float>(), (const double2*)xb, (double2*)yb, st);
+  return set_error(EDLGT_ERR_INVALID_ARG, "unsupported vector type for a real matrix");
+}
+
+
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx, void* y,
              int64_t ldy, hipStream_t st) {
+  if (n_rhs == 1 && h.g_override <= 0 && h.stream_blocks.p) {
+    if (precision == EDLGT_PREC_MIXED && !h.val_lo.p)
+      return set_error(EDLGT_ERR_INVALID_ARG, "low-precision values not built");
+    return spmv_stream(h, precision, xtype, x, y, st);
+  }
   const int g = lanes_per_row(h);
   const int64_t n = h.n_rows;
   const int64_t* rp = h.row_ptr.as<int64_t>();
@@ -244,7 +442,7 @@ int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, voi
                  hipStream_t st, int32_t* best_g, int64_t* best_grid, float* best_ms) {
   const double avg = h.n_rows > 0 ? (double)h.nnz / (double)h.n_rows : 1.0;
   const int gs[] = {4, 8, 16, 32, 64};
-  const int64_t grids[] = {1024, 2048, 4096, 8192, 16384};
+  const int64_t grids[] = {4096, 16384, 65536, 262144};
   hipEvent_t e0, e1;
   EDLGT_HIP(hipEventCreate(&e0));
   EDLGT_HIP(hipEventCreate(&e1));
@@ -254,6 +452,21 @@ int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, voi
   int32_t bg = 0;
   int64_t bgrid = 0;
   int status = EDLGT_OK;
+  if (n_rhs == 1 && h.stream_blocks.p) {  // CSR-stream candidate (lanes 0)
+    h.g_override = 0;
+    h.grid_override = 0;
+    status = csr_spmm(h, precision, xtype, n_rhs, x, h.n_rows, y, h.n_rows, st);
+    if (status == EDLGT_OK) {
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < 3 && status == EDLGT_OK; ++r)
+        status = csr_spmm(h, precision, xtype, n_rhs, x, h.n_rows, y, h.n_rows, st);
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      best = ms / 3;
+    }
+  }
   for (int g : gs) {
     if (g > 8 && g > 2 * avg) continue;
     for (int64_t grid : grids) {

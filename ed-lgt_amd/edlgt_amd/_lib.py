@@ -60,6 +60,8 @@ SIGNATURES = {
     "edlgt_csr_autotune": (_i32, [_p, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "edlgt_csr_set_launch": (_i32, [_p, _i32, _i64]),
     "edlgt_lanczos": (_i32, [_p, _i32, _f64, _i32, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
+    "edlgt_basis_marginals": (_i32, [_p, _p, _i32, _i32, _p, _p]),
+    "edlgt_krylov_evolve": (_i32, [_p, _p, _i32, _p, _f64, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
 }
 
 _lib = None

@@ -89,6 +89,50 @@ class QMB_hamiltonian:
                 logger.info(f"{e:.10f}  {e / nsites:.10f}")
 
 
+def _time_evolution(self, initial_state, time_line, loc_dims, basis=None, d_max=None, m=30,
+                    reorth=True, precision=L.PREC_FP64, store_states=True):
+    """exp(-iHt) psi0 on a uniform time grid by device Krylov stepping
+    (replaces expm_multiply in QMB_hamiltonian.time_evolution,
+    qmb_hamiltonian.py:334-426).  psi_time[k] = exp(-i H t_k) psi0."""
+    import ctypes as C
+    import torch
+    time_line = np.asarray(time_line, dtype=float)
+    if time_line.size < 2:
+        raise ValueError("time_line needs at least two points")
+    dt = float(time_line[1] - time_line[0])
+    if not np.allclose(np.diff(time_line), dt, rtol=1e-10, atol=1e-12):
+        raise ValueError("time_line must be uniformly spaced")
+    self.loc_dims = loc_dims
+    N = self.shape[0]
+    if hasattr(initial_state, "data_ptr"):
+        psi = initial_state.to(device="cuda", dtype=torch.complex128).clone()
+    else:
+        psi = torch.as_tensor(np.asarray(initial_state, dtype=np.complex128), device="cuda").clone()
+    if psi.numel() != N:
+        raise ValueError("initial_state has the wrong length")
+    lib = L.load()
+    st = L.stream_handle()
+    if abs(time_line[0]) > 0:
+        L.check(lib.edlgt_krylov_evolve(self.Ham.handle, None, precision, L.ptr(psi), float(time_line[0]),
+                                        1, m, int(reorth), 0, None, None, None, st))
+    n_steps = time_line.size - 1
+    hist = torch.empty((n_steps + 1, N), dtype=torch.complex128, device="cuda") if store_states else None
+    marg = None
+    if basis is not None:
+        marg = np.zeros((n_steps + 1, basis.n_sites, d_max))
+    err = np.zeros(max(n_steps, 1))
+    L.check(lib.edlgt_krylov_evolve(self.Ham.handle, None if basis is None else basis.handle, precision,
+                                    L.ptr(psi), dt, n_steps, m, int(reorth), int(d_max or 0),
+                                    L.ptr(marg), L.ptr(err), L.ptr(hist), st))
+    self.krylov_err = err[:n_steps]
+    self.marginals = marg
+    self.psi_final = psi
+    self.psi_time = [QMB_state(hist[k], self.lvals, loc_dims) for k in range(n_steps + 1)] if store_states else None
+
+
+QMB_hamiltonian.time_evolution = _time_evolution
+
+
 class QuantumModel:
     """Model base (edlgt/models/quantum_model.py:120-615)."""
 
@@ -134,6 +178,8 @@ class QuantumModel:
 
     def default_params(self):
         self.H = QMB_hamiltonian(self.lvals, size=self.sector_dim)
+        self.H.psi_time = None
+        self.H.marginals = None
 
     # -- Hamiltonian program: tables + placements
     def _new_program(self):
@@ -175,6 +221,78 @@ class QuantumModel:
         self.H.diagonalize(n_eigs, format, self.loc_dims, print_results, **kw)
         self.n_eigs = self.H.n_eigs
         self.res["energy"] = self.H.Nenergies
+
+    def time_evolution_Hamiltonian(self, initial_state, time_line, **kw):
+        """quantum_model.py:613-615; per-step site marginals are kept in
+        self.H.marginals so that dynamics observables need no extra pass."""
+        d_max = int(self.loc_dims.max())
+        self.H.time_evolution(initial_state, time_line, self.loc_dims, basis=self.basis, d_max=d_max, **kw)
+
+    # -- observables (quantum_model.py:1158-1333)
+    def get_observables(self, local_obs=[], twobody_obs=[], plaquette_obs=[], nbody_obs=[],
+                        nbody_dist=[], twobody_axes=None):
+        if twobody_obs or nbody_obs:
+            raise NotImplementedError("two-body / n-body correlators are not on the device path")
+        self.local_obs = list(local_obs)
+        self.plaquette_obs = [list(p) for p in plaquette_obs]
+        self._plaq_ops = {}
+        for names in self.plaquette_obs:
+            axes = (names[-1][3], names[-1][6])
+            plaqs = self.lattice.plaquettes(axes)
+            self._new_program()
+            for sl in plaqs:
+                tabs = [self._table(n, q) for n, q in zip(names, sl)]
+                self._placements.append(dict(sites=list(sl), tables=tabs, strength=1.0 / len(plaqs), dagger=0))
+            self._plaq_ops["_".join(names)] = CSRMatrix(self.basis, np.stack(self._tables),
+                                                        self._placements, L.VALUE_C128)
+
+    def _marginals(self, psi_device):
+        d_max = int(self.loc_dims.max())
+        out = np.zeros((self.n_sites, d_max))
+        vt = L.VALUE_C128 if psi_device.is_complex() else L.VALUE_F64
+        L.check(L.load().edlgt_basis_marginals(self.basis.handle, L.ptr(psi_device.contiguous()), vt,
+                                               d_max, out.ctypes.data, L.stream_handle()))
+        return out
+
+    def local_expvals(self, marg, name):
+        """<O_s> per site of a diagonal local operator from site marginals
+        (LocalTerm in the sector is diagonal-only: sym_qmb_operations.py:565-602)."""
+        diag = np.real(np.einsum("sii->si", self.ops[name]))
+        return np.einsum("sa,sa->s", marg, diag[:, : marg.shape[1]])
+
+    def measure_observables(self, index, dynamics=False):
+        import torch
+        if not dynamics:
+            state = self.H.Npsi[index]
+            marg = self._marginals(state.psi_device)
+        else:
+            state = self.H.psi_time[index] if self.H.psi_time is not None else None
+            marg = self.H.marginals[index] if self.H.marginals is not None else self._marginals(state.psi_device)
+        for obs in self.local_obs:
+            self.res[obs] = self.local_expvals(marg, obs)
+        for names in self.plaquette_obs:
+            key = "_".join(names)
+            psi = state.psi_device.to(torch.complex128)
+            y = self._plaq_ops[key].matvec(psi)
+            self.res[key] = float(torch.vdot(psi, y).real)
+
+    def link_avg(self, obs_name):
+        """Average of res[f'{obs_name}_p{d}'] over the existing links (quantum_model.py:1271-1299)."""
+        tot, cnt = 0.0, 0
+        for i in range(self.n_sites):
+            for d in self.directions:
+                if self.lattice.step(i, d) is not None:
+                    tot += self.res[f"{obs_name}_p{d}"][i]
+                    cnt += 1
+        return tot / cnt
+
+    def stag_avg(self, obs_name, staggered_avg=None):
+        """Mean of res[obs_name] over all / even / odd sites (quantum_model.py:1301-1333)."""
+        v = np.asarray(self.res[obs_name])
+        if staggered_avg is None:
+            return float(np.mean(v))
+        even = self.lattice.even_mask()
+        return float(np.mean(v[even if staggered_avg == "even" else ~even]))
 
     def get_qmb_state_from_configs(self, configs):
         """Equal-weight normalised superposition of basis configurations

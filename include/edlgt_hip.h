@@ -197,6 +197,36 @@ int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis,
                   double* out_evals, void* out_evecs, double* out_resid,
                   int64_t* out_matvecs, void* stream);
 
+/* ======================================================================
+ * Diagonal local observables.  Per-site marginals of |psi|^2:
+ *   out_host[s * d_max + a] = sum_r |psi_r|^2 [config_r[s] == a]
+ * so that <psi|O_s|psi> = sum_a out[s][a] O_s[a][a] for every diagonal local
+ * operator -- replaces LocalTerm.get_expval + exp_val_data
+ * (edlgt/modeling/local_term.py:119-237, edlgt/modeling/qmb_state.py:630-653)
+ * for diagonal operators.  psi: device, vec_type C128 or F64, ||psi|| <= 1.
+ * Deterministic (fixed-point integer accumulation, |error| <= N * 2^-62).
+ * ====================================================================== */
+int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
+                          double* out_host, void* stream);
+
+/* ======================================================================
+ * Real-time evolution psi <- exp(-i H dt)^n_steps psi by Krylov (Lanczos)
+ * exponentiation with an m-dimensional basis per step; replaces
+ * QMB_hamiltonian.time_evolution -> expm_multiply(-1j*H, psi0, ...)
+ * (edlgt/modeling/qmb_hamiltonian.py:334-426).
+ *   psi            device complex128 [n], overwritten by the final state
+ *   reorth         1: full CGS2 reorthogonalisation of the Krylov basis; 0: three-term
+ *   out_marginals  host [(n_steps+1) * n_sites * d_max] per-step marginals
+ *                  (step 0 = input state), or NULL (then b may be NULL)
+ *   out_err        host [n_steps] a-posteriori Krylov error estimates, or NULL
+ *   psi_history    device complex128 [(n_steps+1) * n] states per step, or NULL
+ * precision EDLGT_PREC_MIXED runs the matvecs with FP32 values and an FP32
+ * copy of the Krylov vector; the recurrence and exponential stay FP64.
+ * ====================================================================== */
+int edlgt_krylov_evolve(edlgt_csr* h, const edlgt_basis* b, int32_t precision, void* psi, double dt,
+                        int64_t n_steps, int32_t m, int32_t reorth, int32_t d_max,
+                        double* out_marginals, double* out_err, void* psi_history, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
