@@ -52,8 +52,8 @@ struct AsmDev {
   int cfg_stride;
   int value_real;                 // 1: float64 output (values must be real)
   const Placement* place;
-  const uint64_t* strides;        // [n_sites]
-  const uint64_t* keys;           // [N]
+  const void* strides;            // KeyT [n_sites]
+  const void* keys;               // KeyT [N]
   const uint8_t* configs;         // [N][cfg_stride]
   // transitions: for (table t, side) with side 0 = row nonzeros O[a][*],
   // side 1 = column nonzeros O[*][a]; entries of bra state a are
@@ -81,7 +81,8 @@ __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-__device__ inline int64_t lower_bound_u64(const uint64_t* __restrict__ a, int64_t n, uint64_t k) {
+template <class K>
+__device__ inline int64_t lower_bound_key(const K* __restrict__ a, int64_t n, K k) {
   int64_t lo = 0, len = n;
   while (len > 0) {
     const int64_t half = len >> 1;
@@ -117,10 +118,11 @@ __device__ inline int placement_count(const AsmDev& A, const Placement& P, const
 }
 
 // Enumerates placement P's entries into LDS slots [slot0, slot0 + count).
-template <bool kFill>
+template <bool kFill, class KeyT>
 __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const uint8_t* cfg,
-                                      uint64_t key, int slot0, uint64_t* skey, uint16_t* sidx,
+                                      KeyT key, int slot0, KeyT* skey, uint16_t* sidx,
                                       double2* sval) {
+  const KeyT* strides = static_cast<const KeyT*>(A.strides);
   if (P.nops == 1) {
     skey[slot0] = key;
     sidx[slot0] = (uint16_t)slot0;
@@ -133,8 +135,8 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
     return;
   }
   int beg[kMaxOps], cnt[kMaxOps], idx[kMaxOps];
-  int64_t stride[kMaxOps];
-  int64_t base_delta = 0;
+  KeyT stride[kMaxOps];
+  KeyT base_delta = 0;  // modular (wrap-around) key arithmetic
 #pragma unroll
   for (int f = 0; f < kMaxOps; ++f) {
     beg[f] = 0;
@@ -146,23 +148,23 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
       const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
       beg[f] = off[a];
       cnt[f] = off[a + 1] - off[a];
-      stride[f] = (int64_t)A.strides[P.site[f]];
-      base_delta -= (int64_t)a * stride[f];
+      stride[f] = strides[P.site[f]];
+      base_delta -= (KeyT)a * stride[f];
     }
   }
   int slot = slot0;
   while (true) {
-    int64_t delta = base_delta;
+    KeyT delta = base_delta;
     double2 amp = make_double2(1.0, 0.0);
 #pragma unroll
     for (int f = 0; f < kMaxOps; ++f) {
       if (f < P.nops) {
         const int e = beg[f] + idx[f];
-        delta += (int64_t)A.trans_ket[e] * stride[f];
+        delta += (KeyT)A.trans_ket[e] * stride[f];
         if (kFill) amp = cmul(amp, A.trans_val[e]);
       }
     }
-    skey[slot] = key + (uint64_t)delta;
+    skey[slot] = key + delta;
     sidx[slot] = (uint16_t)slot;
     if (kFill) {
       double2 r = cmul(amp, make_double2(P.s_re, P.s_im));
@@ -188,18 +190,19 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
 }
 
 // One 64-thread block == one wavefront; LDS holds the row's raw entries.
-template <int kCap, bool kFill>
+template <int kCap, bool kFill, class KeyT>
 __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restrict__ row_count,
                                                     const int64_t* __restrict__ row_ptr,
                                                     int32_t* __restrict__ out_col,
                                                     double* __restrict__ out_val) {
-  __shared__ uint64_t skey[kCap];
+  __shared__ KeyT skey[kCap];
+  const KeyT* keys = static_cast<const KeyT*>(A.keys);
   __shared__ uint16_t sidx[kCap];
   __shared__ double2 sval[kFill ? kCap : 1];
   const int lane = threadIdx.x;
   for (int64_t r = blockIdx.x; r < A.n_states; r += gridDim.x) {
     const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
-    const uint64_t key = A.keys[r];
+    const KeyT key = keys[r];
     // 1+2: count placements, assign slots in placement order, emit.
     int base = 0;
     bool overflow = false;
@@ -217,7 +220,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
         overflow = true;
         break;
       }
-      if (c > 0) placement_emit<kFill>(A, P, cfg, key, base + off, skey, sidx, sval);
+      if (c > 0) placement_emit<kFill, KeyT>(A, P, cfg, key, base + off, skey, sidx, sval);
       base += tot;
     }
     if (overflow) {
@@ -230,7 +233,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
     int P2 = 1;
     while (P2 < T) P2 <<= 1;
     for (int i = T + lane; i < P2; i += 64) {
-      skey[i] = ~0ull;
+      skey[i] = ~(KeyT)0;
       sidx[i] = 0xffff;
     }
     __syncthreads();
@@ -240,7 +243,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
         for (int i = lane; i < P2; i += 64) {
           const int ixj = i ^ j;
           if (ixj > i) {
-            const uint64_t ka = skey[i], kb = skey[ixj];
+            const KeyT ka = skey[i], kb = skey[ixj];
             const uint16_t ia = sidx[i], ib = sidx[ixj];
             const bool a_gt_b = (ka > kb) || (ka == kb && ia > ib);
             const bool up = (i & k) == 0;
@@ -261,7 +264,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
       int tot;
       const int rk = wave_excl_scan(head ? 1 : 0, &tot);
       if (kFill && head) {
-        const uint64_t k = skey[i];
+        const KeyT k = skey[i];
         double2 v = sval[sidx[i]];
         for (int q = i + 1; q < T && skey[q] == k; ++q) {
           const double2 w = sval[sidx[q]];
@@ -272,8 +275,8 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
         if (k == key) {
           col = r;
         } else {
-          col = lower_bound_u64(A.keys, A.n_states, k);
-          if (col >= A.n_states || A.keys[col] != k) {
+          col = lower_bound_key<KeyT>(keys, A.n_states, k);
+          if (col >= A.n_states || keys[col] != k) {
             atomicOr(A.err, kErrMissing);
             col = 0;
           }
@@ -294,27 +297,34 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
   }
 }
 
-template <int kCap>
+template <int kCap, class KeyT>
 int launch_pass(bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr, int32_t* col,
                 double* val, hipStream_t st) {
   const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(A.n_states, 1), 1 << 18);
   if (fill)
-    hipLaunchKernelGGL((assemble_rows<kCap, true>), dim3(grid), dim3(64), 0, st, A, counts, row_ptr,
-                       col, val);
+    hipLaunchKernelGGL((assemble_rows<kCap, true, KeyT>), dim3(grid), dim3(64), 0, st, A, counts,
+                       row_ptr, col, val);
   else
-    hipLaunchKernelGGL((assemble_rows<kCap, false>), dim3(grid), dim3(64), 0, st, A, counts,
+    hipLaunchKernelGGL((assemble_rows<kCap, false, KeyT>), dim3(grid), dim3(64), 0, st, A, counts,
                        row_ptr, col, val);
   EDLGT_HIP(hipGetLastError());
   return EDLGT_OK;
 }
 
-int launch_any(int cap, bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr,
+template <class KeyT>
+int launch_cap(int cap, bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr,
                int32_t* col, double* val, hipStream_t st) {
-  if (cap <= 64) return launch_pass<64>(fill, A, counts, row_ptr, col, val, st);
-  if (cap <= 128) return launch_pass<128>(fill, A, counts, row_ptr, col, val, st);
-  if (cap <= 256) return launch_pass<256>(fill, A, counts, row_ptr, col, val, st);
-  if (cap <= 512) return launch_pass<512>(fill, A, counts, row_ptr, col, val, st);
-  return launch_pass<1024>(fill, A, counts, row_ptr, col, val, st);
+  if (cap <= 64) return launch_pass<64, KeyT>(fill, A, counts, row_ptr, col, val, st);
+  if (cap <= 128) return launch_pass<128, KeyT>(fill, A, counts, row_ptr, col, val, st);
+  if (cap <= 256) return launch_pass<256, KeyT>(fill, A, counts, row_ptr, col, val, st);
+  if (cap <= 512) return launch_pass<512, KeyT>(fill, A, counts, row_ptr, col, val, st);
+  return launch_pass<1024, KeyT>(fill, A, counts, row_ptr, col, val, st);
+}
+
+int launch_any(int key_words, int cap, bool fill, const AsmDev& A, int32_t* counts,
+               const int64_t* row_ptr, int32_t* col, double* val, hipStream_t st) {
+  if (key_words == 2) return launch_cap<u128>(cap, fill, A, counts, row_ptr, col, val, st);
+  return launch_cap<uint64_t>(cap, fill, A, counts, row_ptr, col, val, st);
 }
 
 int check_err(const DevBuf& err, hipStream_t st) {
@@ -395,7 +405,10 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
 
   DevBuf b_place, b_str, b_toff, b_tket, b_tval, b_diag, b_err;
   EDLGT_TRY(upload(b_place, place.data(), std::max<size_t>(place.size(), 1), st));
-  EDLGT_TRY(upload(b_str, B.strides.data(), B.strides.size(), st));
+  if (B.key_words == 2)
+    EDLGT_TRY(upload(b_str, B.strides128.data(), B.strides128.size(), st));
+  else
+    EDLGT_TRY(upload(b_str, B.strides.data(), B.strides.size(), st));
   EDLGT_TRY(upload(b_toff, trans_off.data(), trans_off.size(), st));
   const uint8_t z8 = 0;
   const double2 z2 = make_double2(0, 0);
@@ -408,8 +421,8 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   AsmDev A;
   A.n_states = B.n_states; A.n_place = n_place; A.d_max = d_max; A.cfg_stride = B.cfg_stride;
   A.value_real = value_type == EDLGT_VALUE_F64;
-  A.place = b_place.as<Placement>(); A.strides = b_str.as<uint64_t>();
-  A.keys = B.keys.as<uint64_t>(); A.configs = B.configs.as<uint8_t>();
+  A.place = b_place.as<Placement>(); A.strides = b_str.p;
+  A.keys = B.keys.p; A.configs = B.configs.as<uint8_t>();
   A.trans_off = b_toff.as<int32_t>(); A.trans_ket = b_tket.as<uint8_t>();
   A.trans_val = b_tval.as<double2>(); A.diag = b_diag.as<double2>(); A.err = b_err.as<int32_t>();
 
@@ -419,7 +432,7 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   EDLGT_TRY(out->row_ptr.alloc((N + 1) * sizeof(int64_t)));
   DevBuf counts;
   EDLGT_TRY(counts.alloc(std::max<int64_t>(N, 1) * sizeof(int32_t)));
-  if (N > 0) EDLGT_TRY(launch_any(cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr, st));
+  if (N > 0) EDLGT_TRY(launch_any(B.key_words, cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr, st));
   EDLGT_TRY(check_err(b_err, st));
   EDLGT_TRY(exclusive_scan_i32_i64(counts.as<int32_t>(), out->row_ptr.as<int64_t>(), N, st));
   int64_t nnz = 0;
@@ -430,7 +443,7 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   EDLGT_TRY(out->col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
   EDLGT_TRY(out->val.alloc(std::max<int64_t>(nnz, 1) * value_bytes(value_type)));
   if (N > 0)
-    EDLGT_TRY(launch_any(cap, true, A, nullptr, out->row_ptr.as<int64_t>(), out->col.as<int32_t>(),
+    EDLGT_TRY(launch_any(B.key_words, cap, true, A, nullptr, out->row_ptr.as<int64_t>(), out->col.as<int32_t>(),
                          out->val.as<double>(), st));
   EDLGT_TRY(check_err(b_err, st));
   EDLGT_TRY(csr_prepare_stream(*out, st));

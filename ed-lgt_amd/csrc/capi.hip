@@ -18,7 +18,8 @@ int set_error(int code, const std::string& msg) {
 
 namespace {
 
-__device__ inline int64_t lb_u64(const uint64_t* a, int64_t n, uint64_t k) {
+template <class K>
+__device__ inline int64_t lb_key(const K* a, int64_t n, K k) {
   int64_t lo = 0, len = n;
   while (len > 0) {
     const int64_t half = len >> 1;
@@ -32,36 +33,47 @@ __device__ inline int64_t lb_u64(const uint64_t* a, int64_t n, uint64_t k) {
   return lo;
 }
 
-__global__ void lookup_kernel(const uint64_t* __restrict__ keys, int64_t n,
-                              const uint64_t* __restrict__ q, int64_t nq, int64_t* __restrict__ out) {
+template <class K>
+__global__ void lookup_kernel(const K* __restrict__ keys, int64_t n, const K* __restrict__ q,
+                              const uint8_t* __restrict__ bad, int64_t nq, int64_t* __restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = lb_u64(keys, n, q[i]);
-    out[i] = (p < n && keys[p] == q[i]) ? p : -1;
+    const int64_t p = lb_key<K>(keys, n, q[i]);
+    out[i] = (!bad[i] && p < n && keys[p] == q[i]) ? p : -1;
   }
 }
 
 }  // namespace
 
 int basis_lookup(const Basis& b, const uint16_t* configs, int64_t nq, int64_t* out_idx) {
-  std::vector<uint64_t> qk(std::max<int64_t>(nq, 1));
-  bool bad = false;
+  if (nq <= 0) return EDLGT_OK;
+  std::vector<u128> qk(nq);
+  std::vector<uint8_t> bad(nq, 0);
   for (int64_t i = 0; i < nq; ++i) {
-    uint64_t k = 0;
+    u128 k = 0;
     for (int s = 0; s < b.n_sites; ++s) {
       const uint16_t c = configs[i * b.n_sites + s];
-      if (c >= b.loc_dims[s]) bad = true;
-      k += (uint64_t)c * b.strides[s];
+      if (c >= b.loc_dims[s]) bad[i] = 1;
+      k += (u128)c * b.strides128[s];
     }
-    qk[i] = bad ? ~0ull : k;
-    bad = false;
+    qk[i] = k;
   }
-  if (nq <= 0) return EDLGT_OK;
-  DevBuf dq, dout;
-  EDLGT_TRY(upload(dq, qk.data(), nq, nullptr));
+  DevBuf dq, dbad, dout;
   EDLGT_TRY(dout.alloc(nq * sizeof(int64_t)));
-  hipLaunchKernelGGL(lookup_kernel, dim3(grid_for(nq, 256)), dim3(256), 0, nullptr,
-                     b.keys.as<uint64_t>(), b.n_states, dq.as<uint64_t>(), nq, dout.as<int64_t>());
+  EDLGT_TRY(upload(dbad, bad.data(), nq, nullptr));
+  if (b.key_words == 2) {
+    EDLGT_TRY(upload(dq, qk.data(), nq, nullptr));
+    hipLaunchKernelGGL(lookup_kernel<u128>, dim3(grid_for(nq, 256)), dim3(256), 0, nullptr,
+                       b.keys.as<u128>(), b.n_states, dq.as<u128>(), dbad.as<uint8_t>(), nq,
+                       dout.as<int64_t>());
+  } else {
+    std::vector<uint64_t> q64(nq);
+    for (int64_t i = 0; i < nq; ++i) q64[i] = (uint64_t)qk[i];
+    EDLGT_TRY(upload(dq, q64.data(), nq, nullptr));
+    hipLaunchKernelGGL(lookup_kernel<uint64_t>, dim3(grid_for(nq, 256)), dim3(256), 0, nullptr,
+                       b.keys.as<uint64_t>(), b.n_states, dq.as<uint64_t>(), dbad.as<uint8_t>(), nq,
+                       dout.as<int64_t>());
+  }
   EDLGT_HIP(hipGetLastError());
   EDLGT_HIP(hipMemcpy(out_idx, dout.p, nq * sizeof(int64_t), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < nq; ++i)
@@ -140,6 +152,12 @@ int edlgt_basis_info(const edlgt_basis* b, int64_t* n_states, int32_t* n_sites) 
   return EDLGT_OK;
 }
 
+int edlgt_basis_key_words(const edlgt_basis* b, int32_t* key_words) {
+  EDLGT_REQUIRE(b && key_words, "null argument");
+  *key_words = b->key_words;
+  return EDLGT_OK;
+}
+
 int edlgt_basis_copy_configs(const edlgt_basis* b, uint16_t* out_host) {
   EDLGT_REQUIRE(b && out_host, "null argument");
   const int64_t N = b->n_states;
@@ -154,7 +172,7 @@ int edlgt_basis_copy_configs(const edlgt_basis* b, uint16_t* out_host) {
 int edlgt_basis_copy_keys(const edlgt_basis* b, uint64_t* out_host) {
   EDLGT_REQUIRE(b && out_host, "null argument");
   if (b->n_states == 0) return EDLGT_OK;
-  EDLGT_HIP(hipMemcpy(out_host, b->keys.p, b->n_states * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  EDLGT_HIP(hipMemcpy(out_host, b->keys.p, b->n_states * 8 * b->key_words, hipMemcpyDeviceToHost));
   return EDLGT_OK;
 }
 

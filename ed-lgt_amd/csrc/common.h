@@ -10,6 +10,8 @@
 
 namespace edlgt {
 
+typedef unsigned __int128 u128;
+
 // Records a message for edlgt_last_error() and returns the code.
 int set_error(int code, const std::string& msg);
 inline int hip_status(hipError_t e, const char* what) {
@@ -81,8 +83,10 @@ struct Basis {
   int32_t cfg_stride = 0;     // bytes per configuration row (n_sites rounded up to 4)
   int64_t n_states = 0;
   std::vector<int32_t> loc_dims;
-  std::vector<uint64_t> strides;  // mixed-radix, last site fastest
-  DevBuf keys;                // uint64 [n_states], ascending
+  std::vector<uint64_t> strides;  // mixed-radix, last site fastest (64-bit keys)
+  std::vector<u128> strides128;   // same strides for 128-bit keys
+  int32_t key_words = 1;      // 1: uint64 keys; 2: unsigned 128-bit keys (prod(loc_dims) >= 2^64)
+  DevBuf keys;                // key_words * 8 bytes per state, ascending
   DevBuf configs;             // uint8  [n_states][cfg_stride]
 };
 

@@ -248,11 +248,11 @@ __device__ inline bool check_links_nb(const SectorDev& D, int s, const uint8_t* 
   return true;
 }
 
-template <bool kFill>
+template <bool kFill, class KeyT>
 __global__ void __launch_bounds__(64) complete_prefixes_int(
-    SectorDev D, IntGlob G, const uint64_t* __restrict__ frontier, int64_t n_front, int p,
-    int64_t* __restrict__ counts, const int64_t* __restrict__ offsets, uint64_t* __restrict__ keys,
-    uint8_t* __restrict__ configs, int cfg_stride) {
+    SectorDev D, IntGlob G, const KeyT* __restrict__ strides, const uint64_t* __restrict__ frontier,
+    int64_t n_front, int p, int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+    KeyT* __restrict__ keys, uint8_t* __restrict__ configs, int cfg_stride) {
   __shared__ uint8_t sdig[64 * 64];
   uint8_t* dig = sdig + threadIdx.x * 64;
   const int n = D.n_sites;
@@ -261,11 +261,11 @@ __global__ void __launch_bounds__(64) complete_prefixes_int(
        f += (int64_t)gridDim.x * blockDim.x) {
     decode_prefix(D, frontier[f], p, dig);
     int64_t g[kMaxGlob];
-    uint64_t key = 0;
+    KeyT key = 0;
 #pragma unroll
     for (int k = 0; k < kMaxGlob; ++k) g[k] = 0;
     for (int s = 0; s < p; ++s) {
-      key += (uint64_t)dig[s] * D.strides[s];
+      key += (KeyT)dig[s] * strides[s];
 #pragma unroll
       for (int k = 0; k < kMaxGlob; ++k)
         if (k < ng) g[k] += G.tab[((size_t)k * n + s) * D.d_max + dig[s]];
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(64) complete_prefixes_int(
           if (s == p) break;
           --s;
           cur = dig[s];
-          key -= (uint64_t)cur * D.strides[s];
+          key -= (KeyT)cur * strides[s];
 #pragma unroll
           for (int k = 0; k < kMaxGlob; ++k)
             if (k < ng) g[k] -= G.tab[((size_t)k * n + s) * D.d_max + cur];
@@ -309,14 +309,14 @@ __global__ void __launch_bounds__(64) complete_prefixes_int(
         if (!ok || !check_links_nb(D, s, dig)) continue;
         if (s == n - 1) {
           if (kFill) {
-            keys[out] = key + (uint64_t)cur * D.strides[s];
+            keys[out] = key + (KeyT)cur * strides[s];
             uint8_t* row = configs + out * cfg_stride;
             for (int q = 0; q < n; ++q) row[q] = dig[q];
             ++out;
           }
           ++count;
         } else {
-          key += (uint64_t)cur * D.strides[s];
+          key += (KeyT)cur * strides[s];
 #pragma unroll
           for (int k = 0; k < kMaxGlob; ++k)
             if (k < ng) g[k] += G.tab[((size_t)k * n + s) * D.d_max + cur];
@@ -345,16 +345,21 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
     return set_error(EDLGT_ERR_INVALID_ARG, "invalid constraint counts");
   if (n_glob && (glob_type != EDLGT_SYM_U1 && glob_type != EDLGT_SYM_Z2))
     return set_error(EDLGT_ERR_INVALID_ARG, "glob_type must be U1 or Z2");
-  // Mixed-radix strides (last site fastest); the key range must fit 64 bits.
+  // Mixed-radix strides (last site fastest); keys are 64-bit when the whole
+  // product space fits, 128-bit otherwise.
   std::vector<uint64_t> strides(n_sites);
-  unsigned __int128 prodd = 1;
+  std::vector<u128> strides128(n_sites);
+  u128 prodd = 1;
+  bool wide = false;
   for (int s = n_sites - 1; s >= 0; --s) {
     if (loc_dims[s] < 1 || loc_dims[s] > 255 || loc_dims[s] > d_max)
       return set_error(EDLGT_ERR_INVALID_ARG, "loc_dims must be in [1, min(255, d_max)]");
+    strides128[s] = prodd;
     strides[s] = (uint64_t)prodd;
+    if (prodd > (((u128)1) << 120) / 256)
+      return set_error(EDLGT_ERR_INDEX_OVERFLOW, "prod(loc_dims) does not fit 128-bit keys");
     prodd *= (unsigned)loc_dims[s];
-    if (prodd > (unsigned __int128)UINT64_MAX)
-      return set_error(EDLGT_ERR_INDEX_OVERFLOW, "prod(loc_dims) does not fit 64-bit keys");
+    if (prodd > (u128)UINT64_MAX) wide = true;
   }
   // Host-side tables: U(1) suffix bounds, activation lists.
   std::vector<double> rem_min((size_t)std::max(n_glob, 1) * (n_sites + 1), 0.0),
@@ -469,6 +474,9 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
       }
     }
   }
+  if (wide && !int_mode)
+    return set_error(EDLGT_ERR_INVALID_ARG,
+                     "128-bit keys need additive global constraints with dyadic values");
   DevBuf b_gti, b_rmi, b_rxi, b_tgi;
   IntGlob IG{nullptr, nullptr, nullptr, nullptr};
   if (int_mode) {
@@ -491,6 +499,8 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   while (p < n_sites && n_front > 0) {
     const int64_t n_cand = n_front * loc_dims[p];
     if (p > 0 && (n_front >= kTargetFrontier || n_cand > kMaxCandidates)) break;
+    if (p > 0 && prodd / strides128[p] > (u128)(UINT64_MAX / 2))
+      break;  // prefix indices of depth p+1 would not fit 64 bits
     DevBuf flags, pos, next;
     EDLGT_TRY(flags.alloc(n_cand * sizeof(int32_t)));
     EDLGT_TRY(pos.alloc((n_cand + 1) * sizeof(int64_t)));
@@ -517,9 +527,14 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   out->cfg_stride = (n_sites + 3) & ~3;
   out->loc_dims.assign(loc_dims, loc_dims + n_sites);
   out->strides = strides;
+  out->strides128 = strides128;
+  out->key_words = wide ? 2 : 1;
+  const size_t kb = wide ? 16 : 8;
+  DevBuf b_str128;
+  EDLGT_TRY(upload(b_str128, strides128.data(), strides128.size(), st));
   if (n_front == 0) {
     out->n_states = 0;
-    EDLGT_TRY(out->keys.alloc(8));
+    EDLGT_TRY(out->keys.alloc(16));
     EDLGT_TRY(out->configs.alloc(8));
     return EDLGT_OK;
   }
@@ -528,9 +543,13 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   EDLGT_TRY(counts.alloc(n_front * sizeof(int64_t)));
   EDLGT_TRY(offs.alloc((n_front + 1) * sizeof(int64_t)));
   const unsigned g2 = grid_for(n_front, 64);
-  if (int_mode)
-    hipLaunchKernelGGL(complete_prefixes_int<false>, dim3(g2), dim3(64), 0, st, D, IG,
-                       front.as<uint64_t>(), n_front, p, counts.as<int64_t>(),
+  if (int_mode && wide)
+    hipLaunchKernelGGL((complete_prefixes_int<false, u128>), dim3(g2), dim3(64), 0, st, D, IG,
+                       b_str128.as<u128>(), front.as<uint64_t>(), n_front, p, counts.as<int64_t>(),
+                       (const int64_t*)nullptr, (u128*)nullptr, (uint8_t*)nullptr, 0);
+  else if (int_mode)
+    hipLaunchKernelGGL((complete_prefixes_int<false, uint64_t>), dim3(g2), dim3(64), 0, st, D, IG,
+                       b_str.as<uint64_t>(), front.as<uint64_t>(), n_front, p, counts.as<int64_t>(),
                        (const int64_t*)nullptr, (uint64_t*)nullptr, (uint8_t*)nullptr, 0);
   else
     hipLaunchKernelGGL(complete_prefixes<false>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
@@ -542,13 +561,19 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   EDLGT_HIP(hipMemcpyAsync(&N, offs.as<int64_t>() + n_front, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   EDLGT_HIP(hipStreamSynchronize(st));
   out->n_states = N;
-  EDLGT_TRY(out->keys.alloc(std::max<int64_t>(N, 1) * sizeof(uint64_t)));
+  EDLGT_TRY(out->keys.alloc(std::max<int64_t>(N, 1) * kb));
   EDLGT_TRY(out->configs.alloc(std::max<int64_t>(N, 1) * out->cfg_stride));
   EDLGT_HIP(hipMemsetAsync(out->configs.p, 0, std::max<int64_t>(N, 1) * out->cfg_stride, st));
-  if (int_mode)
-    hipLaunchKernelGGL(complete_prefixes_int<true>, dim3(g2), dim3(64), 0, st, D, IG,
-                       front.as<uint64_t>(), n_front, p, (int64_t*)nullptr, offs.as<int64_t>(),
-                       out->keys.as<uint64_t>(), out->configs.as<uint8_t>(), out->cfg_stride);
+  if (int_mode && wide)
+    hipLaunchKernelGGL((complete_prefixes_int<true, u128>), dim3(g2), dim3(64), 0, st, D, IG,
+                       b_str128.as<u128>(), front.as<uint64_t>(), n_front, p, (int64_t*)nullptr,
+                       offs.as<int64_t>(), out->keys.as<u128>(), out->configs.as<uint8_t>(),
+                       out->cfg_stride);
+  else if (int_mode)
+    hipLaunchKernelGGL((complete_prefixes_int<true, uint64_t>), dim3(g2), dim3(64), 0, st, D, IG,
+                       b_str.as<uint64_t>(), front.as<uint64_t>(), n_front, p, (int64_t*)nullptr,
+                       offs.as<int64_t>(), out->keys.as<uint64_t>(), out->configs.as<uint8_t>(),
+                       out->cfg_stride);
   else
     hipLaunchKernelGGL(complete_prefixes<true>, dim3(g2), dim3(64), 0, st, D, front.as<uint64_t>(),
                        n_front, p, (int64_t*)nullptr, offs.as<int64_t>(), out->keys.as<uint64_t>(),

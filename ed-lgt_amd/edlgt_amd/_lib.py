@@ -26,6 +26,11 @@ PREC_FP64, PREC_MIXED = 0, 1
 SYM_U1, SYM_Z2 = 0, 1
 
 
+def SYM_MOD(n):
+    """Additive modulo-n n-body constraint type (Z_n flux sectors)."""
+    return 16 + int(n)
+
+
 class EdlgtError(RuntimeError):
     """Device/runtime failure inside libedlgt_hip."""
 
@@ -45,6 +50,7 @@ SIGNATURES = {
                                   _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "edlgt_basis_info": (_i32, [_p, _p, _p]),
     "edlgt_basis_copy_configs": (_i32, [_p, _p]),
+    "edlgt_basis_key_words": (_i32, [_p, _p]),
     "edlgt_basis_copy_keys": (_i32, [_p, _p]),
     "edlgt_basis_lookup": (_i32, [_p, _p, _i64, _p]),
     "edlgt_basis_device_ptrs": (_i32, [_p, _p, _p, _p]),

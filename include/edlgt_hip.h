@@ -50,6 +50,7 @@ extern "C" {
 /* ---- symmetry types of global / n-body constraints */
 #define EDLGT_SYM_U1 0 /* additive: sum of diagonal values equals the sector value        */
 #define EDLGT_SYM_Z2 1 /* multiplicative: product of diagonal values equals the sector    */
+#define EDLGT_SYM_MOD(n) (16 + (n)) /* additive modulo n (n >= 2), n-body constraints only    */
 
 typedef struct edlgt_basis edlgt_basis; /* sorted sector basis on the device */
 typedef struct edlgt_csr edlgt_csr;     /* CSR Hamiltonian on the device     */
@@ -79,8 +80,11 @@ int edlgt_device_info(int32_t* device, int64_t* free_bytes, int64_t* total_bytes
  *            nbody_sectors[k]
  * each within 1e-13 (symmetry_sector.py:17).  All table pointers are host
  * arrays; diag tables are padded to d_max states per site.  n_glob, n_pairs,
- * n_nbody may be 0 (their pointers may then be NULL).  Requires
- * prod(loc_dims) < 2^64 and loc_dims[s] <= 255.
+ * n_nbody may be 0 (their pointers may then be NULL).  nbody_type may also be
+ * EDLGT_SYM_MOD(n): additive modulo n (sum == sector mod n; integer values),
+ * the flux constraint of Z_n gauge theories.  Requires loc_dims[s] <= 255 and
+ * prod(loc_dims) < 2^120; keys are 128-bit above 2^64 (then the global
+ * constraints must be additive with dyadic-rational values).
  * ====================================================================== */
 int edlgt_sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max,
                        int32_t n_glob, int32_t glob_type, const double* glob_diags,
@@ -96,8 +100,11 @@ int edlgt_basis_info(const edlgt_basis* b, int64_t* n_states, int32_t* n_sites);
 /* Sector configurations as uint16 rows (host, n_states x n_sites), the layout of
  * the reference's sector_configs. */
 int edlgt_basis_copy_configs(const edlgt_basis* b, uint16_t* out_host);
-/* Mixed-radix keys (host, n_states), key = sum_s c_s * prod_{t>s} loc_dims[t];
- * ascending (edlgt/tools/magic.py:40-80 convention). */
+/* Mixed-radix keys, key = sum_s c_s * prod_{t>s} loc_dims[t], ascending
+ * (edlgt/tools/magic.py:40-80 convention).  64-bit when prod(loc_dims) < 2^64,
+ * else 128-bit (little-endian word pairs); out_host holds key_words * n_states
+ * uint64 words. */
+int edlgt_basis_key_words(const edlgt_basis* b, int32_t* key_words);
 int edlgt_basis_copy_keys(const edlgt_basis* b, uint64_t* out_host);
 /* Index of each query configuration (host uint16 rows) in the basis; replaces
  * config_to_index_binarysearch (edlgt/symmetries/generate_configs.py:145-173).
