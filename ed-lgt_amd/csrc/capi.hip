@@ -129,8 +129,9 @@ int edlgt_sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, 
                 "null link tables");
   EDLGT_REQUIRE(n_nbody == 0 || (nbody_diags && nbody_sectors && nbody_site_offsets && nbody_sites),
                 "null n-body tables");
-  EDLGT_REQUIRE(n_nbody == 0 || nbody_type == EDLGT_SYM_U1 || nbody_type == EDLGT_SYM_Z2,
-                "nbody_type must be U1 or Z2");
+  EDLGT_REQUIRE(n_nbody == 0 || nbody_type == EDLGT_SYM_U1 || nbody_type == EDLGT_SYM_Z2 ||
+                    nbody_type >= 18,
+                "nbody_type must be U1, Z2 or EDLGT_SYM_MOD(n >= 2)");
   edlgt_basis* b = new (std::nothrow) edlgt_basis();
   if (!b) return set_error(EDLGT_ERR_OUT_OF_MEMORY, "host allocation failed");
   int s = sector_build(n_sites, loc_dims, d_max, n_glob, glob_type, glob_diags, glob_sectors, n_dirs,

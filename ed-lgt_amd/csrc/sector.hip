@@ -56,6 +56,25 @@ struct SectorDev {
   const uint64_t* strides;     // [n_sites]
 };
 
+// n-body constraint `id` on a full assignment of its sites: additive (U1),
+// multiplicative (Z2) or additive modulo m (type EDLGT_SYM_MOD(m), integer values).
+__device__ inline bool nbody_ok(const SectorDev& D, int id, const uint8_t* dig) {
+  const int n = D.n_sites;
+  const bool mult = D.nbody_type == EDLGT_SYM_Z2;
+  double v = mult ? 1.0 : 0.0;
+  for (int q = D.nb_site_off[id]; q < D.nb_site_off[id + 1]; ++q) {
+    const int site = D.nb_sites[q];
+    const double x = D.nbody_diags[((size_t)id * n + site) * D.d_max + dig[site]];
+    v = mult ? v * x : v + x;
+  }
+  if (D.nbody_type >= 16) {
+    const long long m = D.nbody_type - 16;
+    long long r = (llround(v) - llround(D.nbody_sectors[id])) % m;
+    return r == 0;
+  }
+  return fabs(v - D.nbody_sectors[id]) <= kSymTol;
+}
+
 // Checks every constraint that becomes decidable when site s is assigned.
 // gin: global partial values of sites [0, s); gout receives those of [0, s].
 __device__ inline bool check_site(const SectorDev& D, int s, const uint8_t* dig,
@@ -87,16 +106,8 @@ __device__ inline bool check_site(const SectorDev& D, int s, const uint8_t* dig,
                      D.link_diags[(((size_t)dir * 2 + 1) * n + j) * D.d_max + dig[j]];
     if (fabs(v - D.link_sectors[dir]) > kSymTol) return false;
   }
-  for (int k = D.act_nb_off[s]; k < D.act_nb_off[s + 1]; ++k) {
-    const int id = D.act_nb[k];
-    double v = D.nbody_type == EDLGT_SYM_U1 ? 0.0 : 1.0;
-    for (int q = D.nb_site_off[id]; q < D.nb_site_off[id + 1]; ++q) {
-      const int site = D.nb_sites[q];
-      const double x = D.nbody_diags[((size_t)id * n + site) * D.d_max + dig[site]];
-      v = D.nbody_type == EDLGT_SYM_U1 ? v + x : v * x;
-    }
-    if (fabs(v - D.nbody_sectors[id]) > kSymTol) return false;
-  }
+  for (int k = D.act_nb_off[s]; k < D.act_nb_off[s + 1]; ++k)
+    if (!nbody_ok(D, D.act_nb[k], dig)) return false;
   return true;
 }
 
@@ -235,16 +246,8 @@ __device__ inline bool check_links_nb(const SectorDev& D, int s, const uint8_t* 
                      D.link_diags[(((size_t)dir * 2 + 1) * n + j) * D.d_max + dig[j]];
     if (fabs(v - D.link_sectors[dir]) > kSymTol) return false;
   }
-  for (int k = D.act_nb_off[s]; k < D.act_nb_off[s + 1]; ++k) {
-    const int id = D.act_nb[k];
-    double v = D.nbody_type == EDLGT_SYM_U1 ? 0.0 : 1.0;
-    for (int q = D.nb_site_off[id]; q < D.nb_site_off[id + 1]; ++q) {
-      const int site = D.nb_sites[q];
-      const double x = D.nbody_diags[((size_t)id * n + site) * D.d_max + dig[site]];
-      v = D.nbody_type == EDLGT_SYM_U1 ? v + x : v * x;
-    }
-    if (fabs(v - D.nbody_sectors[id]) > kSymTol) return false;
-  }
+  for (int k = D.act_nb_off[s]; k < D.act_nb_off[s + 1]; ++k)
+    if (!nbody_ok(D, D.act_nb[k], dig)) return false;
   return true;
 }
 
@@ -381,6 +384,12 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
       return set_error(EDLGT_ERR_INVALID_ARG, "link pair out of range");
     auto& v = by_site_link[std::max(i, j)];
     v.push_back(d); v.push_back(i); v.push_back(j);
+  }
+  if (n_nbody > 0 && nbody_type >= 16) {
+    if (nbody_type - 16 < 2) return set_error(EDLGT_ERR_INVALID_ARG, "modulus must be >= 2");
+    for (size_t i = 0; i < (size_t)n_nbody * n_sites * d_max; ++i)
+      if (std::fabs(nbody_diags[i] - std::nearbyint(nbody_diags[i])) > 1e-9)
+        return set_error(EDLGT_ERR_INVALID_ARG, "modular n-body constraints need integer values");
   }
   for (int k = 0; k < n_nbody; ++k) {
     int mx = -1;

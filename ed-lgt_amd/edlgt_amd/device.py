@@ -61,7 +61,7 @@ class SectorBasis:
             nb_diags = np.asarray(nb_diags, dtype=np.float64)
             d_max = max(d_max, nb_diags.shape[-1])
             n_nb = nb_diags.shape[0]
-            nb_type = L.SYM_U1 if nb_t == "U" else L.SYM_Z2
+            nb_type = L.SYM_U1 if nb_t == "U" else L.SYM_Z2 if nb_t == "Z" else int(nb_t)
 
         def pad(a):
             out = np.zeros(a.shape[:-1] + (d_max,), dtype=np.float64)
@@ -112,9 +112,15 @@ class SectorBasis:
         return self._configs
 
     def keys(self):
-        out = np.empty(self.n_states, dtype=np.uint64)
+        """Keys as uint64 (n,) or, for 128-bit keys, Python ints."""
+        kw = C.c_int32()
+        L.check(L.load().edlgt_basis_key_words(self._h, C.byref(kw)))
+        out = np.empty(self.n_states * kw.value, dtype=np.uint64)
         L.check(L.load().edlgt_basis_copy_keys(self._h, out.ctypes.data))
-        return out
+        if kw.value == 1:
+            return out
+        w = out.reshape(-1, 2)
+        return [int(lo) | (int(hi) << 64) for lo, hi in w]
 
     def lookup(self, configs):
         """Indices of configurations (raises ValueError if one is not in the sector)."""

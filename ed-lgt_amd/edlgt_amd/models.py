@@ -18,7 +18,7 @@ from . import operators as opx
 
 logger = logging.getLogger(__name__)
 
-__all__ = ["QuantumModel", "QED_Model", "QMB_hamiltonian", "QMB_state"]
+__all__ = ["QuantumModel", "QED_Model", "Zn_Model", "QMB_hamiltonian", "QMB_state"]
 
 
 class QMB_state:
@@ -400,3 +400,67 @@ class QED_Model(QuantumModel):
                     bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
                     self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
         return self._build_program(real)
+
+
+class Zn_Model(QuantumModel):
+    """Pure Z_n lattice gauge theory on a 2D lattice (no model class exists in
+    the reference; built from its Z_n operator factories,
+    edlgt/operators/Zn_operators.py:130-360, in the shape of QED_Model).
+
+    Sector: Gauss law in the local basis, link constraints
+    E_p{d}(i) + E_m{d}(i+d) = 0, and for PBC optionally the electric flux through
+    the origin lines fixed modulo n (flux_sector=(fx, fy); None keeps all flux
+    sectors).  H = -J sum_plaq (U_plaq + h.c.) - h sum_links (E_l + E_l^dag)."""
+
+    def __init__(self, n, pure_theory=True, flux_sector=(0, 0), **kwargs):
+        super().__init__(**kwargs)
+        if not pure_theory:
+            raise NotImplementedError("Z_n with matter is not on the device path")
+        if self.dim != 2:
+            raise NotImplementedError("Z_n models are 2D")
+        self.n = int(n)
+        self.pure_theory = True
+        site = opx.zn_site(self.n)
+        labels = [self.lattice.label(i, False) for i in range(self.n_sites)]
+        self._project(site, lambda l: opx.zn_operators(site, self.n, l), labels)
+
+        def diag(name):
+            return np.real(np.einsum("sii->si", self.ops[name]))
+
+        link = (np.stack([np.stack([diag(f"E_p{d}"), diag(f"E_m{d}")]) for d in self.directions]),
+                np.zeros(self.dim), self.lattice.link_pairs())
+        nbody = None
+        self.flux_sector = None
+        if flux_sector is not None and not any(self.has_obc):
+            if self.n % 2 == 0:
+                raise NotImplementedError("flux sectors are defined for odd n (integer E)")
+            line_of = {"x": "y", "y": "x"}
+            nbody = (np.stack([diag(f"E_p{d}") for d in self.directions]),
+                     np.array(flux_sector, dtype=float),
+                     [self.lattice.origin_line(line_of[d]) for d in self.directions], L.SYM_MOD(self.n))
+            self.flux_sector = tuple(flux_sector)
+        self.basis = SectorBasis(self.loc_dims, link=link, nbody=nbody)
+        self.default_params()
+
+    def build_Hamiltonian(self, J, h, dtype_mode="auto"):
+        self.coeffs = {"J": J, "h": h}
+        self._new_program()
+        self._add_local("E_clock", -h)
+        self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], self.lattice.plaquettes(("x", "y")),
+                        -J, True)
+        real = dtype_mode in ("auto", "real", False, None)
+        return self._build_program(real)
+
+    def multi_term_program(self):
+        """(tables, placements with term ids) of H = J*H_J + h*H_h for batched solves:
+        term 0 = -sum_plaq(U + h.c.), term 1 = -sum_links(E + E^dag)."""
+        self._new_program()
+        self._add_local("E_clock", -1.0)
+        for p in self._placements:
+            p["term"] = 1
+        k0 = len(self._placements)
+        self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], self.lattice.plaquettes(("x", "y")),
+                        -1.0, True)
+        for p in self._placements[k0:]:
+            p["term"] = 0
+        return np.stack(self._tables), list(self._placements)

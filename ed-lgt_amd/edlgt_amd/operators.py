@@ -216,3 +216,68 @@ def qed_operators(site, spin, pure_theory, dim, label):
         out["EyC_px,pz"] = -1j * (out["E_py"] + out["E_my"]) @ out["C_px,pz"]
         out["ExC_py,pz"] = 1j * (out["E_px"] + out["E_mx"]) @ out["C_py,pz"]
     return out
+
+
+# ---------------------------------------------------------------- Z_n (pure gauge, 2D)
+def zn_rishon_matrices(n):
+    """Z_n link/rishon matrices, pure theory (Zn_operators.py:130-201): U is the
+    cyclic raising shift (U|k> = |k+1 mod n>), E = n_op - (n-1)/2, P = 1."""
+    U = np.zeros((n, n))
+    for k in range(n):
+        U[(k + 1) % n, k] = 1.0
+    E = np.diag(np.arange(n, dtype=float) - 0.5 * (n - 1))
+    return {"IDz": np.eye(n), "U": U, "Ut": U.T.copy(), "n": np.diag(np.arange(n, dtype=float)),
+            "E": E, "E_square": E @ E}
+
+
+def zn_site(n, dim=2):
+    """Gauss law sum(rishon indices) = dim*(n-1) (mod n) over (m_x, m_y, p_x, p_y)
+    (Zn_operators.py:269-360); OBC border labels require index 0 on the named
+    half-links (Zn_border_configs, Zn_operators.py:385-437)."""
+    if dim != 2:
+        raise NotImplementedError("Z_n dressed sites are defined for 2D lattices")
+    dofs = [(f"n_{s}", n) for s in ("mx", "my", "px", "py")]
+    pos = {"mx": 0, "my": 1, "px": 2, "py": 3}
+    sets = {"site": ([], [])}
+    for parts in _border_parts(2):
+        sets["site_" + "_".join(parts)] = ([], [])
+    for idx in product(*[range(n)] * 4):
+        if (sum(idx) - dim * (n - 1)) % n != 0:
+            continue
+        for lab, (ii, vv) in sets.items():
+            parts = lab.split("_")[1:]
+            if all(idx[pos[p]] == 0 for p in parts):
+                ii.append(idx)
+                vv.append(idx)
+    return DressedSite(dofs, sets)
+
+
+def zn_operators(site, n, label):
+    """Dressed-site Z_n operators on one label (Zn_operators.py:204-266), plus the
+    clock electric operator E_clock = sum over the label's linked half-links of
+    cos(2 pi E_h / n) (so that summing it over sites gives sum_links (E + E^dag))."""
+    m = zn_rishon_matrices(n)
+    order = ["mx", "my", "px", "py"]
+
+    def op(assign, coeff=1.0):
+        return site.matrix(label, [m[assign.get(s, "IDz")] for s in order], coeff)
+
+    out = {}
+    for name in ("n", "E", "E_square"):
+        for s in order:
+            out[f"{name}_{s}"] = op({s: name})
+    out["E_square"] = sum(0.5 * out[f"E_square_{s}"] for s in order)
+    out["C_px,py"] = op({"px": "U", "py": "Ut"})
+    out["C_py,mx"] = op({"mx": "Ut", "py": "U"})
+    out["C_mx,my"] = op({"mx": "U", "my": "Ut"})
+    out["C_my,px"] = op({"my": "U", "px": "Ut"})
+    outer = set(label.split("_")[1:])
+    clock = np.zeros_like(out["E_mx"])
+    st = site.states[label]
+    for k, s in enumerate(order):
+        if s in outer:
+            continue
+        e = st[:, k] - 0.5 * (n - 1)
+        clock = clock + np.diag(np.cos(2 * np.pi * e / n))
+    out["E_clock"] = clock
+    return out

@@ -69,3 +69,19 @@ def restrict_to_sector(H_full, configs, loc_dims):
         st[i] = st[i + 1] * int(loc_dims[i + 1])
     idx = (configs.astype(np.int64) * st).sum(axis=1)
     return H_full[idx][:, idx]
+
+
+def zn_full_hamiltonian(model, J, h):
+    """Full-space Z_n Hamiltonian (same terms as ZnOracle.hamiltonian)."""
+    ld = model.loc_dims
+    H = 0
+    for s in range(model.n_sites):
+        H = H + (-h) * _embed({s: model.ops["E_clock"][s]}, ld)
+    names = ["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"]
+    for i in range(model.n_sites):
+        p = lat.plaquette_sites(lat.zig_zag(model.lvals, i), model.lvals, ["x", "y"], model.has_obc)
+        if p is None:
+            continue
+        T = (-J) * _embed({s: model.ops[n][s] for s, n in zip(p, names)}, ld)
+        H = H + T + T.conj().T
+    return sp.csr_matrix(H)

@@ -127,15 +127,19 @@ def sector_configs(loc_dims, glob_diags=None, glob_sectors=None, glob_type="U",
             v = link_diags[dd, 0, i, cand[:, i]] + link_diags[dd, 1, j, cand[:, j]]
             ok &= np.abs(v - link_sectors[dd]) <= SYM_TOL
         for k in act_nb[s]:
-            if nbody_type == "U":
-                v = np.zeros(cand.shape[0])
-                for site in nbody_sites[k]:
-                    v += nbody_diags[k, site, cand[:, site]]
-            else:
+            if nbody_type == "Z":
                 v = np.ones(cand.shape[0])
                 for site in nbody_sites[k]:
                     v *= nbody_diags[k, site, cand[:, site]]
-            ok &= np.abs(v - nbody_sectors[k]) <= SYM_TOL
+            else:
+                v = np.zeros(cand.shape[0])
+                for site in nbody_sites[k]:
+                    v += nbody_diags[k, site, cand[:, site]]
+            if nbody_type.startswith("M"):  # additive modulo m (Z_m flux)
+                m = int(nbody_type[1:])
+                ok &= (np.rint(v).astype(np.int64) - int(round(nbody_sectors[k]))) % m == 0
+            else:
+                ok &= np.abs(v - nbody_sectors[k]) <= SYM_TOL
         configs = cand[ok]
         gvals = gnew[ok]
     return configs.astype(np.uint16)
@@ -374,3 +378,53 @@ def lowest_eigs(H, k, v0=None, tol=0.0):
 def expm_path(H, psi0, t0, t1, num):
     """expm_multiply(-iH, psi0, t0..t1, num, endpoint) (qmb_hamiltonian.py:377-386)."""
     return expm_multiply(-1j * H, psi0, start=t0, stop=t1, num=num, endpoint=True)
+
+
+class ZnOracle:
+    """Pure Z_n gauge theory in 2D (oracle): local basis / operators restated from
+    Zn_operators.py:130-360, link constraint E_p(i) + E_m(i+d) = 0, optional
+    flux sector sum_{origin line} E_p{d} = f_d (mod n),
+    H = -J sum_plaq (C_px,py C_py,mx C_my,px C_mx,my + h.c.) - h sum_sites E_clock."""
+
+    def __init__(self, n, lvals, has_obc, flux_sector=(0, 0)):
+        from . import zn_ops
+        self.n = n
+        self.lvals, self.has_obc = list(lvals), list(has_obc)
+        self.n_sites = prod(lvals)
+        self.dim = 2
+        basis, states = zn_ops.gauge_invariant_states(n)
+        ops = zn_ops.dressed_site_operators(n)
+        self.labels = [lat.site_label(lat.zig_zag(self.lvals, i), self.lvals, self.has_obc, False)
+                       for i in range(self.n_sites)]
+        self.ops, self.loc_dims = project_operators(ops, basis, self.labels)
+        clock = {l: (basis[l].T @ zn_ops.clock_operator(n, l) @ basis[l]).toarray() for l in set(self.labels)}
+        dmax = int(self.loc_dims.max())
+        self.ops["E_clock"] = np.zeros((self.n_sites, dmax, dmax), dtype=complex)
+        for j, l in enumerate(self.labels):
+            d = self.loc_dims[j]
+            self.ops["E_clock"][j, :d, :d] = clock[l]
+        dg = lambda name: np.real(np.einsum("sii->si", self.ops[name]))
+        pairs = lat.link_pairs(self.lvals, self.has_obc)
+        link_diags = np.stack([np.stack([dg(f"E_p{d}"), dg(f"E_m{d}")]) for d in "xy"])
+        if flux_sector is not None and not any(self.has_obc):
+            lines = lat.origin_lines_2d(self.lvals)
+            nb_sites = [lines["y"], lines["x"]]
+            nb_diags = np.stack([dg("E_px"), dg("E_py")])
+            self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, np.zeros(2), pairs,
+                                          nb_diags, np.array(flux_sector, float), nb_sites, f"M{n}")
+        else:
+            self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, np.zeros(2), pairs)
+        self.keys = encode(self.configs, self.loc_dims) if np.prod(self.loc_dims.astype(float)) < 2**62 else None
+        self.N = self.configs.shape[0]
+
+    def hamiltonian(self, J, h):
+        if self.keys is None:
+            raise NotImplementedError("oracle triplets need 64-bit keys")
+        T = Triplets(self.N)
+        args = (self.configs, self.keys, self.loc_dims)
+        add_local(T, self.ops["E_clock"], *args, -h)
+        names = ["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"]
+        plaqs = [p for p in (lat.plaquette_sites(lat.zig_zag(self.lvals, i), self.lvals, ["x", "y"],
+                                                 self.has_obc) for i in range(self.n_sites)) if p]
+        add_nbody_list(T, [self.ops[k] for k in names], plaqs, *args, -J, True)
+        return T.csr(real=True)
