@@ -40,6 +40,8 @@ enum : int32_t { kErrTooMany = 1, kErrMissing = 2, kErrNotReal = 4, kErrOverflow
 struct Placement {
   int32_t nops;
   int32_t dagger;
+  int32_t term;  // term id of multi-term matrices (0 otherwise)
+  int32_t pad_;
   int32_t site[kMaxOps];
   int32_t table[kMaxOps];
   double s_re, s_im;
@@ -121,11 +123,12 @@ __device__ inline int placement_count(const AsmDev& A, const Placement& P, const
 template <bool kFill, class KeyT>
 __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const uint8_t* cfg,
                                       KeyT key, int slot0, KeyT* skey, uint16_t* sidx,
-                                      double2* sval) {
+                                      uint8_t* sterm, double2* sval) {
   const KeyT* strides = static_cast<const KeyT*>(A.strides);
   if (P.nops == 1) {
     skey[slot0] = key;
     sidx[slot0] = (uint16_t)slot0;
+    sterm[slot0] = (uint8_t)P.term;
     if (kFill) {
       const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
       double2 r = cmul(v, make_double2(P.s_re, P.s_im));
@@ -166,6 +169,7 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
     }
     skey[slot] = key + delta;
     sidx[slot] = (uint16_t)slot;
+    sterm[slot] = (uint8_t)P.term;
     if (kFill) {
       double2 r = cmul(amp, make_double2(P.s_re, P.s_im));
       if (P.dagger) r.y = -r.y;
@@ -194,10 +198,12 @@ template <int kCap, bool kFill, class KeyT>
 __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restrict__ row_count,
                                                     const int64_t* __restrict__ row_ptr,
                                                     int32_t* __restrict__ out_col,
-                                                    double* __restrict__ out_val) {
+                                                    double* __restrict__ out_val,
+                                                    uint8_t* __restrict__ out_term) {
   __shared__ KeyT skey[kCap];
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
   __shared__ uint16_t sidx[kCap];
+  __shared__ uint8_t sterm[kCap];
   __shared__ double2 sval[kFill ? kCap : 1];
   const int lane = threadIdx.x;
   for (int64_t r = blockIdx.x; r < A.n_states; r += gridDim.x) {
@@ -220,7 +226,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
         overflow = true;
         break;
       }
-      if (c > 0) placement_emit<kFill, KeyT>(A, P, cfg, key, base + off, skey, sidx, sval);
+      if (c > 0) placement_emit<kFill, KeyT>(A, P, cfg, key, base + off, skey, sidx, sterm, sval);
       base += tot;
     }
     if (overflow) {
@@ -245,7 +251,9 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
           if (ixj > i) {
             const KeyT ka = skey[i], kb = skey[ixj];
             const uint16_t ia = sidx[i], ib = sidx[ixj];
-            const bool a_gt_b = (ka > kb) || (ka == kb && ia > ib);
+            // (key, term, slot) order; padding slots (0xffff) carry no term
+            const int ta = ia == 0xffff ? 255 : sterm[ia], tb = ib == 0xffff ? 255 : sterm[ib];
+            const bool a_gt_b = (ka > kb) || (ka == kb && (ta > tb || (ta == tb && ia > ib)));
             const bool up = (i & k) == 0;
             if (a_gt_b == up) {
               skey[i] = kb; skey[ixj] = ka;
@@ -260,13 +268,15 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
     int rank_base = 0;
     for (int ib = 0; ib < T; ib += 64) {
       const int i = ib + lane;
-      const bool head = i < T && (i == 0 || skey[i] != skey[i - 1]);
+      const bool head = i < T && (i == 0 || skey[i] != skey[i - 1] ||
+                                  sterm[sidx[i]] != sterm[sidx[i - 1]]);
       int tot;
       const int rk = wave_excl_scan(head ? 1 : 0, &tot);
       if (kFill && head) {
         const KeyT k = skey[i];
+        const uint8_t tm = sterm[sidx[i]];
         double2 v = sval[sidx[i]];
-        for (int q = i + 1; q < T && skey[q] == k; ++q) {
+        for (int q = i + 1; q < T && skey[q] == k && sterm[sidx[q]] == tm; ++q) {
           const double2 w = sval[sidx[q]];
           v.x += w.x;
           v.y += w.y;
@@ -283,6 +293,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
         }
         const int64_t o = row_ptr[r] + rank_base + rk;
         out_col[o] = (int32_t)col;
+        if (out_term) out_term[o] = tm;
         if (A.value_real) {
           out_val[o] = v.x;
         } else {
@@ -299,32 +310,32 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
 
 template <int kCap, class KeyT>
 int launch_pass(bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr, int32_t* col,
-                double* val, hipStream_t st) {
+                double* val, uint8_t* term, hipStream_t st) {
   const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(A.n_states, 1), 1 << 18);
   if (fill)
     hipLaunchKernelGGL((assemble_rows<kCap, true, KeyT>), dim3(grid), dim3(64), 0, st, A, counts,
-                       row_ptr, col, val);
+                       row_ptr, col, val, term);
   else
     hipLaunchKernelGGL((assemble_rows<kCap, false, KeyT>), dim3(grid), dim3(64), 0, st, A, counts,
-                       row_ptr, col, val);
+                       row_ptr, col, val, term);
   EDLGT_HIP(hipGetLastError());
   return EDLGT_OK;
 }
 
 template <class KeyT>
 int launch_cap(int cap, bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr,
-               int32_t* col, double* val, hipStream_t st) {
-  if (cap <= 64) return launch_pass<64, KeyT>(fill, A, counts, row_ptr, col, val, st);
-  if (cap <= 128) return launch_pass<128, KeyT>(fill, A, counts, row_ptr, col, val, st);
-  if (cap <= 256) return launch_pass<256, KeyT>(fill, A, counts, row_ptr, col, val, st);
-  if (cap <= 512) return launch_pass<512, KeyT>(fill, A, counts, row_ptr, col, val, st);
-  return launch_pass<1024, KeyT>(fill, A, counts, row_ptr, col, val, st);
+               int32_t* col, double* val, uint8_t* term, hipStream_t st) {
+  if (cap <= 64) return launch_pass<64, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
+  if (cap <= 128) return launch_pass<128, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
+  if (cap <= 256) return launch_pass<256, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
+  if (cap <= 512) return launch_pass<512, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
+  return launch_pass<1024, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
 }
 
 int launch_any(int key_words, int cap, bool fill, const AsmDev& A, int32_t* counts,
-               const int64_t* row_ptr, int32_t* col, double* val, hipStream_t st) {
-  if (key_words == 2) return launch_cap<u128>(cap, fill, A, counts, row_ptr, col, val, st);
-  return launch_cap<uint64_t>(cap, fill, A, counts, row_ptr, col, val, st);
+               const int64_t* row_ptr, int32_t* col, double* val, uint8_t* term, hipStream_t st) {
+  if (key_words == 2) return launch_cap<u128>(cap, fill, A, counts, row_ptr, col, val, term, st);
+  return launch_cap<uint64_t>(cap, fill, A, counts, row_ptr, col, val, term, st);
 }
 
 int check_err(const DevBuf& err, hipStream_t st) {
@@ -345,7 +356,10 @@ int check_err(const DevBuf& err, hipStream_t st) {
 int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
                       int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
                       const int32_t* place_tables, const double* place_strength,
-                      const int32_t* place_dagger, int32_t value_type, hipStream_t st, Csr* out) {
+                      const int32_t* place_dagger, const int32_t* place_term, int32_t n_terms,
+                      int32_t value_type, hipStream_t st, Csr* out) {
+  if (n_terms < 1 || n_terms > 16 || (n_terms > 1 && !place_term))
+    return set_error(EDLGT_ERR_INVALID_ARG, "n_terms must be in [1, 16] (with place_term when > 1)");
   if (value_type != EDLGT_VALUE_F64 && value_type != EDLGT_VALUE_C128)
     return set_error(EDLGT_ERR_INVALID_ARG, "value_type must be F64 or C128");
   if (d_max < 1 || d_max > 255 || n_tables < 1 || n_place < 0)
@@ -381,6 +395,9 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
     Placement& P = place[p];
     P.nops = place_nops[p];
     P.dagger = place_dagger[p] ? 1 : 0;
+    P.term = n_terms > 1 ? place_term[p] : 0;
+    P.pad_ = 0;
+    if (P.term < 0 || P.term >= n_terms) return set_error(EDLGT_ERR_INVALID_ARG, "place_term out of range");
     if (P.nops < 1 || P.nops > kMaxOps)
       return set_error(EDLGT_ERR_INVALID_ARG, "placement with 0 or more than 8 factors");
     int64_t bound = 1;
@@ -432,7 +449,9 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   EDLGT_TRY(out->row_ptr.alloc((N + 1) * sizeof(int64_t)));
   DevBuf counts;
   EDLGT_TRY(counts.alloc(std::max<int64_t>(N, 1) * sizeof(int32_t)));
-  if (N > 0) EDLGT_TRY(launch_any(B.key_words, cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr, st));
+  if (N > 0)
+    EDLGT_TRY(launch_any(B.key_words, cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr,
+                         nullptr, st));
   EDLGT_TRY(check_err(b_err, st));
   EDLGT_TRY(exclusive_scan_i32_i64(counts.as<int32_t>(), out->row_ptr.as<int64_t>(), N, st));
   int64_t nnz = 0;
@@ -442,9 +461,12 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   out->nnz = nnz;
   EDLGT_TRY(out->col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
   EDLGT_TRY(out->val.alloc(std::max<int64_t>(nnz, 1) * value_bytes(value_type)));
+  out->n_terms = n_terms;
+  if (n_terms > 1) EDLGT_TRY(out->term.alloc(std::max<int64_t>(nnz, 1)));
   if (N > 0)
-    EDLGT_TRY(launch_any(B.key_words, cap, true, A, nullptr, out->row_ptr.as<int64_t>(), out->col.as<int32_t>(),
-                         out->val.as<double>(), st));
+    EDLGT_TRY(launch_any(B.key_words, cap, true, A, nullptr, out->row_ptr.as<int64_t>(),
+                         out->col.as<int32_t>(), out->val.as<double>(),
+                         n_terms > 1 ? out->term.as<uint8_t>() : nullptr, st));
   EDLGT_TRY(check_err(b_err, st));
   EDLGT_TRY(csr_prepare_stream(*out, st));
   return EDLGT_OK;

@@ -209,13 +209,45 @@ int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_table
   edlgt_csr* h = new (std::nothrow) edlgt_csr();
   if (!h) return set_error(EDLGT_ERR_OUT_OF_MEMORY, "host allocation failed");
   int s = hamiltonian_build(*b, d_max, n_tables, tables, n_place, place_nops, place_sites,
-                            place_tables, place_strength, place_dagger, value_type,
+                            place_tables, place_strength, place_dagger, nullptr, 1, value_type,
                             (hipStream_t)stream, h);
   if (s != EDLGT_OK) {
     delete h;
     return s;
   }
   *out = h;
+  return EDLGT_OK;
+}
+
+int edlgt_hamiltonian_build_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
+                                  const double* tables, int32_t n_place, const int32_t* place_nops,
+                                  const int32_t* place_sites, const int32_t* place_tables,
+                                  const double* place_strength, const int32_t* place_dagger,
+                                  const int32_t* place_term, int32_t n_terms, int32_t value_type,
+                                  void* stream, edlgt_csr** out) {
+  EDLGT_REQUIRE(b && out && tables && place_term, "null argument");
+  EDLGT_REQUIRE(n_place == 0 || (place_nops && place_sites && place_tables && place_strength &&
+                                 place_dagger),
+                "null placement arrays");
+  *out = nullptr;
+  edlgt_csr* h = new (std::nothrow) edlgt_csr();
+  if (!h) return set_error(EDLGT_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int s = hamiltonian_build(*b, d_max, n_tables, tables, n_place, place_nops, place_sites,
+                            place_tables, place_strength, place_dagger, place_term, n_terms,
+                            value_type, (hipStream_t)stream, h);
+  if (s != EDLGT_OK) {
+    delete h;
+    return s;
+  }
+  *out = h;
+  return EDLGT_OK;
+}
+
+int edlgt_csr_copy_terms(const edlgt_csr* h, uint8_t* term_host, int32_t* n_terms) {
+  EDLGT_REQUIRE(h, "null matrix");
+  if (n_terms) *n_terms = h->n_terms;
+  if (term_host && h->n_terms > 1 && h->nnz)
+    EDLGT_HIP(hipMemcpy(term_host, h->term.p, h->nnz, hipMemcpyDeviceToHost));
   return EDLGT_OK;
 }
 
@@ -279,6 +311,42 @@ int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, vo
   return csr_spmm(*h, precision, vec_type, n_rhs, X, h->n_rows, Y, h->n_rows, (hipStream_t)stream);
 }
 
+int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t vec_type,
+                    const double* coef_host, void* stream) {
+  EDLGT_REQUIRE(h && X && Y && coef_host, "null argument");
+  EDLGT_REQUIRE(n_rhs >= 1, "n_rhs must be >= 1");
+  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64, "vec_type must be C128 or F64");
+  EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_F64 && h->value_type != EDLGT_VALUE_F64),
+                "real vectors need a real matrix");
+  hipStream_t st = (hipStream_t)stream;
+  const int nt = h->n_terms;
+  const size_t es = vec_type == EDLGT_VALUE_C128 ? 16 : 8;
+  DevBuf xs, ys, cf;
+  EDLGT_TRY(xs.alloc(16 * sizeof(void*)));
+  EDLGT_TRY(ys.alloc(16 * sizeof(void*)));
+  EDLGT_TRY(cf.alloc(16 * 16 * sizeof(double)));
+  std::vector<const void*> hx(16);
+  std::vector<void*> hy(16);
+  std::vector<double> hc(16 * 16);
+  int done = 0;
+  while (done < n_rhs) {
+    const int left = n_rhs - done;
+    const int nb = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    for (int c = 0; c < nb; ++c) {
+      hx[c] = (const char*)X + (size_t)(done + c) * h->n_rows * es;
+      hy[c] = (char*)Y + (size_t)(done + c) * h->n_rows * es;
+      for (int t = 0; t < 16; ++t) hc[c * 16 + t] = t < nt ? coef_host[(size_t)(done + c) * nt + t] : 0.0;
+    }
+    EDLGT_HIP(hipMemcpyAsync(xs.p, hx.data(), nb * sizeof(void*), hipMemcpyHostToDevice, st));
+    EDLGT_HIP(hipMemcpyAsync(ys.p, hy.data(), nb * sizeof(void*), hipMemcpyHostToDevice, st));
+    EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data(), nb * 16 * sizeof(double), hipMemcpyHostToDevice, st));
+    EDLGT_TRY(csr_spmm_coef(*h, vec_type, nb, xs.p, ys.p, cf.as<double>(), st));
+    EDLGT_HIP(hipStreamSynchronize(st));
+    done += nb;
+  }
+  return EDLGT_OK;
+}
+
 int edlgt_csr_autotune(edlgt_csr* h, int32_t precision, int32_t vec_type, int32_t n_rhs,
                        const void* X, void* Y, void* stream, int32_t* out_lanes, int64_t* out_grid,
                        float* out_ms) {
@@ -335,6 +403,70 @@ int edlgt_krylov_evolve(edlgt_csr* h, const edlgt_basis* b, int32_t precision, v
   EDLGT_REQUIRE(!b || b->n_states == h->n_rows, "basis and matrix sizes differ");
   return krylov_run(*h, b, precision, psi, dt, n_steps, m, reorth, d_max, out_marginals, out_err,
                     psi_history, (hipStream_t)stream);
+}
+
+struct edlgt_server {
+  void* impl = nullptr;
+  int32_t n_eigs = 0;
+  int32_t max_slots = 0;
+  int32_t n_terms = 1;
+};
+
+int edlgt_server_create(edlgt_csr* h, int32_t vec_type, int32_t max_slots, int32_t n_eigs, double tol,
+                        int32_t max_basis, int64_t max_matvecs, void* stream, edlgt_server** out) {
+  EDLGT_REQUIRE(h && out, "null argument");
+  EDLGT_REQUIRE(max_slots >= 1 && max_slots <= 64, "max_slots must be in [1, 64]");
+  EDLGT_REQUIRE(n_eigs >= 1 && n_eigs <= h->n_rows && tol > 0.0, "bad n_eigs / tol");
+  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_F64 || vec_type == EDLGT_VALUE_C128, "vec_type must be F64 or C128");
+  EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_F64 && h->value_type != EDLGT_VALUE_F64),
+                "real vectors need a real matrix");
+  *out = nullptr;
+  int status = EDLGT_OK;
+  void* impl = server_create(*h, vec_type, max_slots, n_eigs, tol, max_basis, max_matvecs,
+                             (hipStream_t)stream, &status);
+  if (!impl) return status;
+  auto* sv = new edlgt_server();
+  sv->impl = impl;
+  sv->n_eigs = n_eigs;
+  sv->max_slots = max_slots;
+  sv->n_terms = h->n_terms;
+  *out = sv;
+  return EDLGT_OK;
+}
+
+int edlgt_server_submit(edlgt_server* s, int64_t request_id, const double* coef, uint64_t seed,
+                        int32_t* slot_out) {
+  EDLGT_REQUIRE(s && coef && slot_out, "null argument");
+  int slot = -1;
+  EDLGT_TRY(server_submit(s->impl, request_id, coef, seed, &slot));
+  *slot_out = slot;
+  return EDLGT_OK;
+}
+
+int edlgt_server_active(const edlgt_server* s, int32_t* active, int32_t* max_slots) {
+  EDLGT_REQUIRE(s, "null server");
+  if (active) *active = server_active(s->impl);
+  if (max_slots) *max_slots = s->max_slots;
+  return EDLGT_OK;
+}
+
+int edlgt_server_step(edlgt_server* s, int32_t max_done, int64_t* done_ids, double* done_evals,
+                      double* done_resid_est, int64_t* done_matvecs, int32_t* done_converged,
+                      int32_t* n_done) {
+  EDLGT_REQUIRE(s && done_ids && done_evals && n_done, "null argument");
+  int nd = 0;
+  const int st = server_step(s->impl, max_done, done_ids, done_evals, done_resid_est, done_matvecs,
+                             done_converged, &nd);
+  *n_done = nd;
+  return st;
+}
+
+int edlgt_server_destroy(edlgt_server* s) {
+  if (s) {
+    server_destroy(s->impl);
+    delete s;
+  }
+  return EDLGT_OK;
 }
 
 }  // extern "C"

@@ -101,6 +101,8 @@ struct Csr {
   DevBuf col;       // int32 [nnz]
   DevBuf val;       // value_type [nnz]
   DevBuf val_lo;    // reduced-precision copy (complex64 / float32), built on demand
+  DevBuf term;      // uint8 [nnz] term id of every entry (multi-term matrices only)
+  int32_t n_terms = 1;
   DevBuf stream_blocks;      // int64 [n_stream_blocks + 1] first row of every CSR-stream tile
   DevBuf stream_pos;         // int64 [n_stream_blocks + 1] first entry of every tile
   int64_t n_stream_blocks = 0;

@@ -17,7 +17,8 @@ int basis_lookup(const Basis& b, const uint16_t* configs_host, int64_t n_query, 
 int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
                       int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
                       const int32_t* place_tables, const double* place_strength,
-                      const int32_t* place_dagger, int32_t value_type, hipStream_t st, Csr* out);
+                      const int32_t* place_dagger, const int32_t* place_term, int32_t n_terms,
+                      int32_t value_type, hipStream_t st, Csr* out);
 
 int lanes_per_row(const Csr& h);
 // Y[:, b] = H X[:, b] for b < n_rhs; xtype is the element type of X
@@ -26,6 +27,10 @@ int lanes_per_row(const Csr& h);
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx,
              void* y, int64_t ldy, hipStream_t st);
 int csr_make_low_precision(Csr& h, hipStream_t st);
+// Fused multi-coefficient SpMM (see spmv.hip): xs/ys device pointer arrays,
+// coef device [nb][16]; nb in {1, 2, 4, 8, 16}.
+int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const double* coef,
+                  hipStream_t st);
 // Row partition of the CSR-stream SpMV (run once per matrix).
 int csr_prepare_stream(Csr& h, hipStream_t st);
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
@@ -39,6 +44,14 @@ int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter,
 int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
                int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
                hipStream_t st);
+
+void* server_create(Csr& h, int vec_type, int max_slots, int n_eigs, double tol, int max_basis,
+                    int64_t max_mv, hipStream_t st, int* status);
+void server_destroy(void* p);
+int server_submit(void* p, int64_t req, const double* coef, uint64_t seed, int* slot);
+int server_active(void* p);
+int server_step(void* p, int max_done, int64_t* ids, double* evals, double* est, int64_t* mvs,
+                int32_t* conv, int* n_done);
 
 int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
                     hipStream_t st);

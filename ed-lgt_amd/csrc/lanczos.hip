@@ -23,6 +23,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <memory>
 
 #include "common.h"
 #include "kernels.h"
@@ -296,7 +297,8 @@ struct Solver {
   DevBuf V, W, Tmp, part, gdev, cdev, shadow;
   int64_t mv64 = 0, mv32 = 0;
   uint64_t seed = 12345;
-  bool need_tmp = true;  // restart / Ritz buffer (not needed by Krylov stepping)
+  bool need_tmp = true;      // restart / Ritz buffer (not needed by Krylov stepping)
+  bool want_vectors = true;  // form the Ritz vectors when a run finishes
 
   Solver(Csr& hh, hipStream_t s, int prec) : h(hh), st(s), precision(prec), N(hh.n_rows) {}
 
@@ -492,94 +494,134 @@ struct Solver {
   // Runs thick-restart block Lanczos from the b start columns already in
   // V[:, 0:b] (not necessarily orthonormal).  Fills theta (k values), and the
   // Ritz vectors Y (k columns) into Tmp[:, 0:k]; resid gets the estimates.
-  int run(int k, double tol, int64_t max_mv, std::vector<double>& theta, std::vector<double>& est,
-          bool& converged) {
+  // ---- resumable thick-restart state (one expansion step per matvec)
+  int st_nv = 0, st_l = 0, st_k = 0;
+  double st_tol = 0.0;
+  int64_t st_max_mv = 0;
+  Dense st_T;
+
+  // Orthonormalises the start block V[:, 0:b] and resets the state.
+  int begin(int k, double tol, int64_t max_mv) {
     Dense Gd, R;
     EDLGT_TRY(orthonormalize(0, col(0), b, Gd, R));
-    int nv = b;
-    int l = 0;
-    Dense T(M + b, M + b);
-    converged = false;
-    while (true) {
-      // expand until full
-      const int a0 = nv - b;
-      EDLGT_TRY(matvec(col(a0), W.as<S>(), b));
-      Dense G, Rb;
-      if (nv >= N) {
-        // the basis spans the whole space: T = V^H H V is exact, no residual
-        EDLGT_TRY(gram(V.as<S>(), nv, W.as<S>(), b, G));
-        Rb = Dense(b, b);
-      } else {
-        EDLGT_TRY(orthonormalize(nv, W.as<S>(), b, G, Rb));
+    st_nv = b;
+    st_l = 0;
+    st_k = k;
+    st_tol = tol;
+    st_max_mv = max_mv;
+    st_T = Dense(M + b, M + b);
+    return EDLGT_OK;
+  }
+
+  // The block the next matvec must multiply: V[:, nv-b : nv].
+  S* active_block() { return col(st_nv - b); }
+
+  // Consumes W = H * active_block(): orthogonalisation, T update, and when the
+  // basis is full Rayleigh-Ritz + convergence test + thick restart.  Sets
+  // finished when the wanted pairs converged (or the budget ran out); then
+  // theta/est hold the results and Tmp[:, 0:k] the Ritz vectors.
+  int after_matvec(bool& finished, std::vector<double>& theta, std::vector<double>& est,
+                   bool& converged) {
+    finished = false;
+    const int k = st_k;
+    int& nv = st_nv;
+    int& l = st_l;
+    Dense& T = st_T;
+    const int a0 = nv - b;
+    Dense G, Rb;
+    if (nv >= N) {
+      // the basis spans the whole space: T = V^H H V is exact, no residual
+      EDLGT_TRY(gram(V.as<S>(), nv, W.as<S>(), b, G));
+      Rb = Dense(b, b);
+    } else {
+      EDLGT_TRY(orthonormalize(nv, W.as<S>(), b, G, Rb));
+    }
+    for (int i = 0; i < nv; ++i)
+      for (int c = 0; c < b; ++c) {
+        T(i, a0 + c) = G(i, c);
+        T(a0 + c, i) = std::conj(G(i, c));
       }
-      for (int i = 0; i < nv; ++i)
+    const bool full = nv + b > M || nv + b > N;
+    // residual block Q -> V[:, nv:nv+b] (capacity M + b)
+    EDLGT_HIP(hipMemcpyAsync(col(nv), W.p, (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+    if (!full) {
+      for (int c = 0; c < b; ++c)
+        for (int i = 0; i < b; ++i) {
+          T(nv + i, a0 + c) = Rb(i, c);
+          T(a0 + c, nv + i) = std::conj(Rb(i, c));
+        }
+      nv += b;
+      return EDLGT_OK;
+    }
+    // Rayleigh-Ritz
+    Dense Tn(nv, nv);
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < nv; ++j) Tn(i, j) = T(i, j);
+    std::vector<double> w;
+    Dense Sv;
+    hermitian_eig(Tn, w, Sv);
+    const bool exhausted = nv >= N;
+    theta.assign(w.begin(), w.begin() + k);
+    est.assign(k, 0.0);
+    bool ok = true;
+    for (int i = 0; i < k; ++i) {
+      double s2 = 0.0;
+      if (!exhausted) {
         for (int c = 0; c < b; ++c) {
-          T(i, a0 + c) = G(i, c);
-          T(a0 + c, i) = std::conj(G(i, c));
+          cd z = 0.0;
+          for (int q = 0; q < b; ++q) z += Rb(c, q) * Sv(a0 + q, i);
+          s2 += std::norm(z);
         }
-      const bool full = nv + b > M || nv + b > N;
-      // residual block Q -> V[:, nv:nv+b] (capacity M + b)
-      EDLGT_HIP(hipMemcpyAsync(col(nv), W.p, (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
-      if (!full) {
-        for (int c = 0; c < b; ++c)
-          for (int i = 0; i < b; ++i) {
-            T(nv + i, a0 + c) = Rb(i, c);
-            T(a0 + c, nv + i) = std::conj(Rb(i, c));
-          }
-        nv += b;
-        continue;
       }
-      // Rayleigh-Ritz
-      Dense Tn(nv, nv);
-      for (int i = 0; i < nv; ++i)
-        for (int j = 0; j < nv; ++j) Tn(i, j) = T(i, j);
-      std::vector<double> w;
-      Dense Sv;
-      hermitian_eig(Tn, w, Sv);
-      const bool exhausted = nv >= N;
-      theta.assign(w.begin(), w.begin() + k);
-      est.assign(k, 0.0);
-      bool ok = true;
-      for (int i = 0; i < k; ++i) {
-        double s2 = 0.0;
-        if (!exhausted) {
-          for (int c = 0; c < b; ++c) {
-            cd z = 0.0;
-            for (int q = 0; q < b; ++q) z += Rb(c, q) * Sv(a0 + q, i);
-            s2 += std::norm(z);
-          }
-        }
-        est[i] = std::sqrt(s2);
-        if (est[i] > tol * std::max(1.0, std::fabs(w[i]))) ok = false;
-      }
-      const int64_t mv = mv64 + mv32;
-      if (ok || exhausted || mv >= max_mv) {
-        converged = ok || exhausted;
+      est[i] = std::sqrt(s2);
+      if (est[i] > st_tol * std::max(1.0, std::fabs(w[i]))) ok = false;
+    }
+    const int64_t mv = mv64 + mv32;
+    if (ok || exhausted || mv >= st_max_mv) {
+      converged = ok || exhausted;
+      finished = true;
+      if (want_vectors) {
         Dense Sk(nv, k);
         for (int j = 0; j < nv; ++j)
           for (int i = 0; i < k; ++i) Sk(j, i) = Sv(j, i);
         EDLGT_TRY(update_blocked(col(0), nv, Sk, Tmp.as<S>(), k));
-        return EDLGT_OK;
       }
-      // thick restart: keep the lowest l Ritz vectors and the residual block
-      l = std::max(k, std::min(M - 2 * b, k + (M - k) / 2));
-      l = std::min(l, nv - b);
-      Dense Sl(nv, l);
-      for (int j = 0; j < nv; ++j)
-        for (int i = 0; i < l; ++i) Sl(j, i) = Sv(j, i);
-      EDLGT_TRY(update_blocked(col(0), nv, Sl, Tmp.as<S>(), l));
-      EDLGT_HIP(hipMemcpyAsync(col(0), Tmp.p, (size_t)l * N * sizeof(S), hipMemcpyDeviceToDevice, st));
-      EDLGT_HIP(hipMemcpyAsync(col(l), col(nv), (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
-      for (auto& z : T.a) z = 0.0;
-      for (int i = 0; i < l; ++i) T(i, i) = w[i];
-      for (int c = 0; c < b; ++c)
-        for (int i = 0; i < l; ++i) {
-          cd z = 0.0;
-          for (int q = 0; q < b; ++q) z += Rb(c, q) * Sv(a0 + q, i);
-          T(l + c, i) = z;
-          T(i, l + c) = std::conj(z);
-        }
-      nv = l + b;
+      return EDLGT_OK;
+    }
+    // thick restart: keep the lowest l Ritz vectors and the residual block
+    l = std::max(k, std::min(M - 2 * b, k + (M - k) / 2));
+    l = std::min(l, nv - b);
+    Dense Sl(nv, l);
+    for (int j = 0; j < nv; ++j)
+      for (int i = 0; i < l; ++i) Sl(j, i) = Sv(j, i);
+    EDLGT_TRY(update_blocked(col(0), nv, Sl, Tmp.as<S>(), l));
+    EDLGT_HIP(hipMemcpyAsync(col(0), Tmp.p, (size_t)l * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+    EDLGT_HIP(hipMemcpyAsync(col(l), col(nv), (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+    for (auto& z : T.a) z = 0.0;
+    for (int i = 0; i < l; ++i) T(i, i) = w[i];
+    for (int c = 0; c < b; ++c)
+      for (int i = 0; i < l; ++i) {
+        cd z = 0.0;
+        for (int q = 0; q < b; ++q) z += Rb(c, q) * Sv(a0 + q, i);
+        T(l + c, i) = z;
+        T(i, l + c) = std::conj(z);
+      }
+    nv = l + b;
+    return EDLGT_OK;
+  }
+
+  // Runs thick-restart block Lanczos from the b start columns already in
+  // V[:, 0:b] (not necessarily orthonormal).  Fills theta (k values), and the
+  // Ritz vectors Y (k columns) into Tmp[:, 0:k]; est gets the estimates.
+  int run(int k, double tol, int64_t max_mv, std::vector<double>& theta, std::vector<double>& est,
+          bool& converged) {
+    EDLGT_TRY(begin(k, tol, max_mv));
+    converged = false;
+    while (true) {
+      EDLGT_TRY(matvec(active_block(), W.as<S>(), b));
+      bool finished = false;
+      EDLGT_TRY(after_matvec(finished, theta, est, converged));
+      if (finished) return EDLGT_OK;
     }
   }
 
@@ -797,6 +839,152 @@ int krylov_typed(Csr& h, const Basis* basis, int precision, void* psi, double dt
   return EDLGT_OK;
 }
 
+// ---------------------------------------------------------------- serving
+// Continuous-batching eigensolver: up to max_slots independent requests, each
+// a block thick-restart Lanczos run for the lowest k eigenvalues of
+// H(c) = sum_t c[t] H_t on one shared multi-term CSR pattern.  One step
+// advances every active slot by one block expansion: a single fused SpMM
+// applies each slot's own coefficients to its active block (the matrix is
+// streamed once for all slots), then each slot orthogonalises, updates its
+// projected matrix and, when its basis is full, runs Rayleigh-Ritz, tests
+// convergence and restarts.  Finished slots are freed at once and can take a
+// new request before the next step.  A request's start block depends only on
+// its seed and every column is computed in the same order whatever the batch,
+// so a result is bitwise the same alone or in any mix.
+template <class S>
+struct Server {
+  Csr& h;
+  hipStream_t st;
+  int vec_type, b, M, k;
+  double tol;
+  int64_t max_mv;
+  struct Slot {
+    std::unique_ptr<Solver<S>> sv;
+    bool active = false;
+    int64_t req = -1;
+    double coef[16] = {0};
+  };
+  std::vector<Slot> slots;
+  DevBuf xs, ys, cf;
+  std::vector<const void*> hx;
+  std::vector<void*> hy;
+  std::vector<double> hc;
+
+  Server(Csr& hh, hipStream_t s, int vt) : h(hh), st(s), vec_type(vt) {}
+
+  int setup(int max_slots, int n_eigs, double tol_, int max_basis, int64_t max_mv_) {
+    const int64_t N = h.n_rows;
+    k = n_eigs;
+    tol = tol_;
+    max_mv = max_mv_;
+    b = std::min(k, 4);
+    M = std::max(max_basis, 2 * k + 2 * b);
+    M = (M + b - 1) / b * b;
+    if (N <= 512 && M >= N) {
+      b = 1;
+      M = (int)N;
+    } else if (M > N / 2) {
+      M = (int)std::max<int64_t>(2 * b, (N / 2) / b * b);
+    }
+    slots.resize(max_slots);
+    for (auto& sl : slots) {
+      sl.sv.reset(new Solver<S>(h, st, EDLGT_PREC_FP64));
+      sl.sv->b = b;
+      sl.sv->M = M;
+      sl.sv->want_vectors = false;
+      EDLGT_TRY(sl.sv->setup());
+    }
+    EDLGT_TRY(xs.alloc(64 * sizeof(void*)));
+    EDLGT_TRY(ys.alloc(64 * sizeof(void*)));
+    EDLGT_TRY(cf.alloc(16 * 16 * sizeof(double)));
+    return EDLGT_OK;
+  }
+
+  int active() const {
+    int a = 0;
+    for (auto& sl : slots) a += sl.active ? 1 : 0;
+    return a;
+  }
+
+  int submit(int64_t req, const double* coef, uint64_t seed, int* slot_out) {
+    *slot_out = -1;
+    for (size_t i = 0; i < slots.size(); ++i) {
+      Slot& sl = slots[i];
+      if (sl.active) continue;
+      for (int t = 0; t < 16; ++t) sl.coef[t] = t < h.n_terms ? coef[t] : 0.0;
+      sl.sv->seed = seed;
+      sl.sv->mv64 = sl.sv->mv32 = 0;
+      EDLGT_TRY(sl.sv->randomize(sl.sv->col(0), (int64_t)b * h.n_rows));
+      EDLGT_TRY(sl.sv->begin(k, tol, max_mv));
+      sl.req = req;
+      sl.active = true;
+      *slot_out = (int)i;
+      return EDLGT_OK;
+    }
+    return EDLGT_OK;
+  }
+
+  // Fused SpMM over the active blocks of all active slots, in chunks of at most
+  // 16 columns (chunk widths 16/8/4/2/1).
+  int fused_matvec() {
+    hx.clear();
+    hy.clear();
+    hc.clear();
+    for (auto& sl : slots) {
+      if (!sl.active) continue;
+      for (int c = 0; c < b; ++c) {
+        hx.push_back(sl.sv->active_block() + (int64_t)c * h.n_rows);
+        hy.push_back(sl.sv->W.template as<S>() + (int64_t)c * h.n_rows);
+        for (int t = 0; t < 16; ++t) hc.push_back(sl.coef[t]);
+        sl.sv->mv64 += 1;
+      }
+    }
+    size_t done = 0;
+    while (done < hx.size()) {
+      const size_t left = hx.size() - done;
+      const int nb = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+      EDLGT_HIP(hipMemcpyAsync(xs.p, hx.data() + done, nb * sizeof(void*), hipMemcpyHostToDevice, st));
+      EDLGT_HIP(hipMemcpyAsync(ys.p, hy.data() + done, nb * sizeof(void*), hipMemcpyHostToDevice, st));
+      EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data() + done * 16, nb * 16 * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+      EDLGT_TRY(csr_spmm_coef(h, vec_type, nb, xs.p, ys.p, cf.as<double>(), st));
+      EDLGT_HIP(hipStreamSynchronize(st));  // the pointer/coefficient buffers are reused
+      done += nb;
+    }
+    return EDLGT_OK;
+  }
+
+  struct Done {
+    int64_t req;
+    std::vector<double> theta, est;
+    int64_t matvecs;
+    bool converged;
+  };
+
+  int step(std::vector<Done>& out) {
+    if (active() == 0) return EDLGT_OK;
+    EDLGT_TRY(fused_matvec());
+    for (auto& sl : slots) {
+      if (!sl.active) continue;
+      bool finished = false, conv = false;
+      std::vector<double> th, es;
+      EDLGT_TRY(sl.sv->after_matvec(finished, th, es, conv));
+      if (finished) {
+        out.push_back(Done{sl.req, th, es, sl.sv->mv64, conv});
+        sl.active = false;
+        sl.req = -1;
+      }
+    }
+    return EDLGT_OK;
+  }
+};
+
+struct ServerAny {
+  int vec_type;
+  std::unique_ptr<Server<double>> r;
+  std::unique_ptr<Server<double2>> c;
+};
+
 }  // namespace
 
 int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
@@ -819,6 +1007,67 @@ int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter,
                                  evecs, resid, matvecs, st);
   }
   return set_error(EDLGT_ERR_INVALID_ARG, "vec_type must be C128 or F64");
+}
+
+void* server_create(Csr& h, int vec_type, int max_slots, int n_eigs, double tol, int max_basis,
+                    int64_t max_mv, hipStream_t st, int* status) {
+  auto* sa = new ServerAny();
+  sa->vec_type = vec_type;
+  int s;
+  if (vec_type == EDLGT_VALUE_F64) {
+    sa->r.reset(new Server<double>(h, st, vec_type));
+    s = sa->r->setup(max_slots, n_eigs, tol, max_basis, max_mv);
+  } else {
+    sa->c.reset(new Server<double2>(h, st, vec_type));
+    s = sa->c->setup(max_slots, n_eigs, tol, max_basis, max_mv);
+  }
+  *status = s;
+  if (s != EDLGT_OK) {
+    delete sa;
+    return nullptr;
+  }
+  return sa;
+}
+
+void server_destroy(void* p) { delete static_cast<ServerAny*>(p); }
+
+int server_submit(void* p, int64_t req, const double* coef, uint64_t seed, int* slot) {
+  auto* sa = static_cast<ServerAny*>(p);
+  return sa->r ? sa->r->submit(req, coef, seed, slot) : sa->c->submit(req, coef, seed, slot);
+}
+
+int server_active(void* p) {
+  auto* sa = static_cast<ServerAny*>(p);
+  return sa->r ? sa->r->active() : sa->c->active();
+}
+
+int server_step(void* p, int max_done, int64_t* ids, double* evals, double* est, int64_t* mvs,
+                int32_t* conv, int* n_done) {
+  auto* sa = static_cast<ServerAny*>(p);
+  *n_done = 0;
+  auto emit = [&](auto& done) {
+    for (auto& d : done) {
+      if (*n_done >= max_done) return set_error(EDLGT_ERR_INVALID_ARG, "max_done too small");
+      const int i = (*n_done)++;
+      ids[i] = d.req;
+      const int k = (int)d.theta.size();
+      for (int j = 0; j < k; ++j) {
+        evals[(int64_t)i * k + j] = d.theta[j];
+        if (est) est[(int64_t)i * k + j] = d.est[j];
+      }
+      if (mvs) mvs[i] = d.matvecs;
+      if (conv) conv[i] = d.converged ? 1 : 0;
+    }
+    return (int)EDLGT_OK;
+  };
+  if (sa->r) {
+    std::vector<Server<double>::Done> done;
+    EDLGT_TRY(sa->r->step(done));
+    return emit(done);
+  }
+  std::vector<Server<double2>::Done> done;
+  EDLGT_TRY(sa->c->step(done));
+  return emit(done);
 }
 
 }  // namespace edlgt

@@ -438,6 +438,116 @@ int csr_make_low_precision(Csr& h, hipStream_t st) {
   return EDLGT_OK;
 }
 
+namespace {
+
+// Multi-coefficient product: for right-hand side b, y_b = sum_k coef[b][k] H_k x_b
+// where H = sum_k H_k shares one CSR pattern and every entry carries its term
+// id k.  One pass streams col/val/term once for all NB right-hand sides (the
+// fused SpMM of batched parameter sweeps and served solves); each column's
+// accumulation order is the same for any NB, so a right-hand side gets the
+// same bits alone or in a batch.
+constexpr int kMaxTerms = 16;
+
+__device__ inline double2 scale(double c, double2 v) { return make_double2(c * v.x, c * v.y); }
+__device__ inline double scale(double c, double v) { return c * v; }
+
+template <class VT, class XT, class AT, int G, int NB>
+__global__ void __launch_bounds__(256) csr_spmm_coef_kernel(
+    int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const VT* __restrict__ val, const uint8_t* __restrict__ term,
+    const XT* const* __restrict__ xs, AT* const* __restrict__ ys, const double* __restrict__ coef) {
+  __shared__ double cs[NB * kMaxTerms];
+  for (int i = threadIdx.x; i < NB * kMaxTerms; i += blockDim.x) cs[i] = coef[i];
+  __syncthreads();
+  const XT* xp[NB];
+  AT* yp[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    xp[b] = xs[b];
+    yp[b] = ys[b];
+  }
+  const int sub = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (256 / G);
+  for (int64_t r = (int64_t)blockIdx.x * (256 / G) + (threadIdx.x / G); r < n; r += stride) {
+    const int64_t beg = row_ptr[r], end = row_ptr[r + 1];
+    AT acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = zero_acc<AT>();
+    for (int64_t p = beg + sub; p < end; p += G) {
+      const int32_t c = ldnt(col + p);
+      const auto v = load_val(val, p);
+      const int t = term ? (int)ldnt(term + p) : 0;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) fma_acc(acc[b], scale(cs[b * kMaxTerms + t], v), load_x(xp[b], c));
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const AT s = group_sum<G>(acc[b]);
+      if (sub == 0) yp[b][r] = s;
+    }
+  }
+}
+
+template <class VT, class XT, class AT, int G>
+int launch_coef_nb(int nb, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
+                   const double* coef, hipStream_t st) {
+  const int64_t blocks = std::min<int64_t>(65536, std::max<int64_t>(1, (n + 256 / G - 1) / (256 / G)));
+#define EDLGT_COEF_CASE(NBC)                                                                     \
+  case NBC:                                                                                      \
+    hipLaunchKernelGGL((csr_spmm_coef_kernel<VT, XT, AT, G, NBC>), dim3((unsigned)blocks),        \
+                       dim3(256), 0, st, n, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), val,   \
+                       h.n_terms > 1 ? h.term.as<uint8_t>() : (const uint8_t*)nullptr,         \
+                       (const XT* const*)xs, (AT* const*)ys, coef);                             \
+    break;
+  switch (nb) {
+    EDLGT_COEF_CASE(1)
+    EDLGT_COEF_CASE(2)
+    EDLGT_COEF_CASE(4)
+    EDLGT_COEF_CASE(8)
+    EDLGT_COEF_CASE(16)
+    default:
+      return set_error(EDLGT_ERR_INVALID_ARG, "unsupported batch width");
+  }
+#undef EDLGT_COEF_CASE
+  EDLGT_HIP(hipGetLastError());
+  return EDLGT_OK;
+}
+
+template <class VT, class XT, class AT>
+int launch_coef(int g, int nb, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
+                const double* coef, hipStream_t st) {
+  switch (g) {
+    case 4: return launch_coef_nb<VT, XT, AT, 4>(nb, n, h, val, xs, ys, coef, st);
+    case 8: return launch_coef_nb<VT, XT, AT, 8>(nb, n, h, val, xs, ys, coef, st);
+    case 16: return launch_coef_nb<VT, XT, AT, 16>(nb, n, h, val, xs, ys, coef, st);
+    case 32: return launch_coef_nb<VT, XT, AT, 32>(nb, n, h, val, xs, ys, coef, st);
+  }
+  return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");
+}
+
+}  // namespace
+
+int coef_lanes(const Csr& h) {
+  const double avg = h.n_rows > 0 ? (double)h.nnz / (double)h.n_rows : 1.0;
+  return avg <= 6 ? 4 : avg <= 14 ? 8 : avg <= 40 ? 16 : 32;
+}
+
+// xs / ys: device arrays of nb device pointers; coef: device [nb][16] doubles.
+// nb must be 1, 2, 4, 8 or 16.  Vectors: complex128 for a complex matrix,
+// float64 (F64) or complex128 (C128) for a real matrix.
+int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const double* coef,
+                  hipStream_t st) {
+  const int g = coef_lanes(h);
+  const int64_t n = h.n_rows;
+  if (h.value_type == EDLGT_VALUE_C128) {
+    if (vec_type != EDLGT_VALUE_C128) return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
+    return launch_coef<double2, double2, double2>(g, nb, n, h, h.val.as<double2>(), xs, ys, coef, st);
+  }
+  if (vec_type == EDLGT_VALUE_F64)
+    return launch_coef<double, double, double>(g, nb, n, h, h.val.as<double>(), xs, ys, coef, st);
+  return launch_coef<double, double2, double2>(g, nb, n, h, h.val.as<double>(), xs, ys, coef, st);
+}
+
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
                  hipStream_t st, int32_t* best_g, int64_t* best_grid, float* best_ms) {
   const double avg = h.n_rows > 0 ? (double)h.nnz / (double)h.n_rows : 1.0;

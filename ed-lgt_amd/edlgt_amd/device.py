@@ -1,7 +1,9 @@
 """Python handles around the device objects of libedlgt_hip.
 
 SectorBasis  -- sorted symmetry-sector basis (keys + configurations) on the GPU
-CSRMatrix    -- canonical CSR Hamiltonian on the GPU, with SpMV/SpMM and Lanczos
+CSRMatrix    -- canonical CSR Hamiltonian on the GPU, with SpMV/SpMM and Lanczos;
+                optionally multi-term (H = sum_t H_t, coefficients per product)
+Server       -- continuous-batching lowest-eigenvalue solver on a multi-term matrix
 """
 import ctypes as C
 
@@ -145,9 +147,11 @@ class CSRMatrix:
     """Canonical CSR Hamiltonian on the device (rows ascending, columns sorted
     within rows, duplicates summed, explicit zeros kept)."""
 
-    def __init__(self, basis, tables, placements, value_type, stream=None):
+    def __init__(self, basis, tables, placements, value_type, stream=None, n_terms=1):
         """tables: (n_tables, d, d) complex; placements: list of dicts
-        {sites, tables, strength, dagger}."""
+        {sites, tables, strength, dagger[, term]}.  With n_terms > 1 every
+        placement names its term (0 <= term < n_terms) and the matrix keeps the
+        terms apart on a shared pattern (matvec_coef, Server)."""
         lib = L.load()
         tables = np.ascontiguousarray(tables, dtype=np.complex128)
         n_tables, d_max = tables.shape[0], tables.shape[1]
@@ -157,6 +161,7 @@ class CSRMatrix:
         tabs = np.zeros((n_place, 8), dtype=np.int32)
         stren = np.zeros((n_place, 2), dtype=np.float64)
         dag = np.zeros(n_place, dtype=np.int32)
+        term = np.zeros(n_place, dtype=np.int32)
         for p, pl in enumerate(placements):
             k = len(pl["sites"])
             nops[p] = k
@@ -165,13 +170,21 @@ class CSRMatrix:
             s = complex(pl["strength"])
             stren[p] = (s.real, s.imag)
             dag[p] = 1 if pl.get("dagger") else 0
+            term[p] = int(pl.get("term", 0))
         h = C.c_void_p()
         st = L.stream_handle(stream) if stream is not None else None
-        L.check(lib.edlgt_hamiltonian_build(
-            basis.handle, d_max, n_tables, tables.ctypes.data, n_place, nops.ctypes.data,
-            sites.ctypes.data, tabs.ctypes.data, stren.ctypes.data, dag.ctypes.data,
-            int(value_type), st, C.byref(h)))
+        if n_terms == 1:
+            L.check(lib.edlgt_hamiltonian_build(
+                basis.handle, d_max, n_tables, tables.ctypes.data, n_place, nops.ctypes.data,
+                sites.ctypes.data, tabs.ctypes.data, stren.ctypes.data, dag.ctypes.data,
+                int(value_type), st, C.byref(h)))
+        else:
+            L.check(lib.edlgt_hamiltonian_build_terms(
+                basis.handle, d_max, n_tables, tables.ctypes.data, n_place, nops.ctypes.data,
+                sites.ctypes.data, tabs.ctypes.data, stren.ctypes.data, dag.ctypes.data,
+                term.ctypes.data, int(n_terms), int(value_type), st, C.byref(h)))
         self._h = h
+        self.n_terms = int(n_terms)
         n, nnz, vt = C.c_int64(), C.c_int64(), C.c_int32()
         L.check(lib.edlgt_csr_info(h, C.byref(n), C.byref(nnz), C.byref(vt)))
         self.n, self.nnz, self.value_type = int(n.value), int(nnz.value), int(vt.value)
@@ -197,10 +210,25 @@ class CSRMatrix:
         L.check(L.load().edlgt_csr_copy(self._h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
         return rp, col, val
 
-    def to_scipy(self):
+    def terms(self):
+        """Per-entry term ids (uint8, nnz); all zero for a single-term matrix."""
+        t = np.zeros(self.nnz, dtype=np.uint8)
+        if self.n_terms > 1:
+            L.check(L.load().edlgt_csr_copy_terms(self._h, t.ctypes.data, None))
+        return t
+
+    def to_scipy(self, coef=None):
+        """The matrix as scipy CSR; for a multi-term matrix, sum_t coef[t] H_t
+        (coef defaults to all ones).  Entries of different terms on one
+        (row, col) are summed by scipy."""
         import scipy.sparse as sp
         rp, col, val = self.to_host()
-        return sp.csr_matrix((val, col, rp), shape=self.shape)
+        if self.n_terms > 1 and coef is not None:
+            val = val * np.asarray(coef, dtype=np.float64)[self.terms()]
+        m = sp.csr_matrix((val, col, rp), shape=self.shape)
+        if self.n_terms > 1:
+            m.sum_duplicates()
+        return m
 
     def vec_type(self, complex_vectors=None):
         cplx = self.is_complex if complex_vectors is None else complex_vectors
@@ -218,6 +246,21 @@ class CSRMatrix:
             # x: (n_rhs, N) row-major == N x n_rhs column-major
             L.check(L.load().edlgt_spmm(self._h, precision, x.shape[0], L.ptr(x), L.ptr(y), vt,
                                         L.stream_handle(stream)))
+        return y
+
+    def matvec_coef(self, x, coef, y=None, stream=None):
+        """Y[j] = (sum_t coef[j, t] H_t) X[j] for device X of shape (n_rhs, N)
+        (complex128, or float64 for a real H); coef: (n_rhs, n_terms)."""
+        torch = _torch()
+        if x.dim() == 1:
+            x = x.view(1, -1)
+        if y is None:
+            y = torch.empty_like(x)
+        coef = np.ascontiguousarray(np.broadcast_to(np.asarray(coef, dtype=np.float64),
+                                                    (x.shape[0], self.n_terms)))
+        vt = L.VALUE_C128 if x.dtype == torch.complex128 else L.VALUE_F64
+        L.check(L.load().edlgt_spmm_coef(self._h, x.shape[0], L.ptr(x), L.ptr(y), vt,
+                                         coef.ctypes.data, L.stream_handle(stream)))
         return y
 
     def autotune(self, n_rhs=1, precision=L.PREC_FP64, complex_vectors=None, stream=None):
@@ -266,3 +309,75 @@ class CSRMatrix:
             except Exception:
                 pass
             self._h = None
+
+
+class Server:
+    """Served lowest-eigenvalue solves with continuous batching on a
+    multi-term matrix (edlgt_server_*).  Each request is a coefficient vector
+    c (H(c) = sum_t c[t] H_t) and a seed; up to max_slots requests advance
+    together, one fused SpMM per step, and a finished request frees its slot
+    at once for the next one.  A request's result does not depend on which
+    other requests share the batch."""
+
+    def __init__(self, csr, n_eigs=1, tol=1e-10, max_basis=32, max_matvecs=100000,
+                 max_slots=8, complex_vectors=None, stream=None):
+        lib = L.load()
+        self.csr = csr
+        self.n_eigs = int(n_eigs)
+        self.max_slots = int(max_slots)
+        self._stream = L.stream_handle(stream)
+        vt = csr.vec_type(complex_vectors)
+        s = C.c_void_p()
+        L.check(lib.edlgt_server_create(csr.handle, vt, self.max_slots, self.n_eigs, float(tol),
+                                        int(max_basis), int(max_matvecs), self._stream, C.byref(s)))
+        self._s = s
+        self._ids = np.zeros(self.max_slots, dtype=np.int64)
+        self._ev = np.zeros((self.max_slots, self.n_eigs))
+        self._es = np.zeros((self.max_slots, self.n_eigs))
+        self._mv = np.zeros(self.max_slots, dtype=np.int64)
+        self._cv = np.zeros(self.max_slots, dtype=np.int32)
+
+    def submit(self, request_id, coef, seed):
+        """Puts the request into a free slot; returns False when all are busy."""
+        c = np.zeros(self.csr.n_terms)
+        c[:] = np.asarray(coef, dtype=np.float64)
+        slot = C.c_int32()
+        L.check(L.load().edlgt_server_submit(self._s, int(request_id), c.ctypes.data,
+                                             C.c_uint64(int(seed)), C.byref(slot)))
+        return slot.value >= 0
+
+    @property
+    def active(self):
+        a = C.c_int32()
+        L.check(L.load().edlgt_server_active(self._s, C.byref(a), None))
+        return int(a.value)
+
+    def step(self):
+        """One fused expansion step; returns the requests that finished as a
+        list of (request_id, evals, resid_est, matvecs, converged)."""
+        n = C.c_int32()
+        L.check(L.load().edlgt_server_step(self._s, self.max_slots, self._ids.ctypes.data,
+                                           self._ev.ctypes.data, self._es.ctypes.data,
+                                           self._mv.ctypes.data, self._cv.ctypes.data, C.byref(n)))
+        return [(int(self._ids[i]), self._ev[i].copy(), self._es[i].copy(), int(self._mv[i]),
+                 bool(self._cv[i])) for i in range(n.value)]
+
+    def solve_all(self, requests):
+        """Runs a list of (request_id, coef, seed) to completion, admitting in
+        order as slots free up; returns {request_id: (evals, resid, matvecs, converged)}."""
+        out, queue = {}, list(requests)
+        while queue or self.active:
+            while queue and self.submit(*queue[0]):
+                queue.pop(0)
+            for rid, ev, es, mv, cv in self.step():
+                out[rid] = (ev, es, mv, cv)
+        return out
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            try:
+                L.load().edlgt_server_destroy(s)
+            except Exception:
+                pass
+            self._s = None

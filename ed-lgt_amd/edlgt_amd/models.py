@@ -464,3 +464,10 @@ class Zn_Model(QuantumModel):
         for p in self._placements[k0:]:
             p["term"] = 0
         return np.stack(self._tables), list(self._placements)
+
+    def build_multi_term(self):
+        """Assembles H_J and H_h once on one shared pattern (real CSR with
+        term ids): H(J, h) = J * H_J + h * H_h for any couplings, without
+        re-assembly.  Returns the CSRMatrix (coefficients [J, h])."""
+        tables, placements = self.multi_term_program()
+        return CSRMatrix(self.basis, tables, placements, L.VALUE_F64, n_terms=2)

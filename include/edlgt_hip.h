@@ -54,6 +54,7 @@ extern "C" {
 
 typedef struct edlgt_basis edlgt_basis; /* sorted sector basis on the device */
 typedef struct edlgt_csr edlgt_csr;     /* CSR Hamiltonian on the device     */
+typedef struct edlgt_server edlgt_server; /* continuous-batching eigensolver   */
 
 int edlgt_last_error(char* buf, int64_t size);
 int edlgt_version(void);
@@ -149,6 +150,21 @@ int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_table
                             const int32_t* place_sites, const int32_t* place_tables,
                             const double* place_strength, const int32_t* place_dagger,
                             int32_t value_type, void* stream, edlgt_csr** out);
+/* Multi-term variant: place_term[p] in [0, n_terms) tags every placement; the
+ * result is H = sum_t H_t on one pattern whose entries keep their term id
+ * (entries of different terms on the same (row, col) stay separate, sorted by
+ * term).  Products then take per-right-hand-side coefficients c[t]
+ * (edlgt_spmm_coef, edlgt_server_*), replacing one assembly per parameter
+ * value (H(g) = sum_t c_t(g) H_t).  n_terms <= 16.  No reference
+ * counterpart: the reference re-assembles H for every coupling. */
+int edlgt_hamiltonian_build_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
+                                  const double* tables, int32_t n_place, const int32_t* place_nops,
+                                  const int32_t* place_sites, const int32_t* place_tables,
+                                  const double* place_strength, const int32_t* place_dagger,
+                                  const int32_t* place_term, int32_t n_terms, int32_t value_type,
+                                  void* stream, edlgt_csr** out);
+/* Term ids (host uint8 [nnz], only when n_terms > 1) and the term count. */
+int edlgt_csr_copy_terms(const edlgt_csr* h, uint8_t* term_host, int32_t* n_terms);
 int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* value_type);
 /* Copy CSR arrays to host: row_ptr int64 [n+1], col int32 [nnz], val [nnz]. */
 int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val);
@@ -172,6 +188,16 @@ int edlgt_spmv(edlgt_csr* h, int32_t precision, const void* x, void* y, int32_t 
                void* stream);
 int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, void* Y,
                int32_t vec_type, void* stream);
+
+/* Y[:, j] = (sum_t coef_host[j * n_terms + t] H_t) X[:, j] for j < n_rhs on a
+ * multi-term matrix (edlgt_hamiltonian_build_terms; a single-term matrix has
+ * n_terms = 1).  X, Y: device, column-major with leading dimension n_rows,
+ * complex128 (C128) or, for a real matrix, float64 (F64).  Every column's
+ * result is independent of n_rhs and of the other columns, bit for bit.
+ * Synchronous with respect to stream.  No reference counterpart: it fuses the
+ * reference's one-matrix-per-coupling products of a parameter sweep. */
+int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t vec_type,
+                    const double* coef_host, void* stream);
 
 /* Times the SpMM launch shapes (lanes per row x workgroups) on X -> Y with HIP
  * events and keeps the fastest for this matrix (no counterpart in the
@@ -203,6 +229,31 @@ int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis,
                   int64_t max_iter, int32_t precision, int32_t vec_type, const void* v0_host,
                   double* out_evals, void* out_evecs, double* out_resid,
                   int64_t* out_matvecs, void* stream);
+
+/* ======================================================================
+ * Served lowest-eigenvalue solves with continuous batching (no reference
+ * counterpart; each request replaces one build_Hamiltonian +
+ * diagonalize_Hamiltonian of the reference at its own couplings).
+ * A server owns max_slots solver slots on a multi-term matrix h; a request is
+ * a coefficient vector c (n_terms doubles: H(c) = sum_t c[t] H_t) and a seed
+ * for its random start block.  edlgt_server_submit puts it in a free slot
+ * (*slot_out = -1 when all slots are busy: retry after a step).
+ * edlgt_server_step advances every active slot by one block-Lanczos expansion
+ * (one fused SpMM streams the matrix once for all slots) and reports the
+ * requests that finished: ids, n_eigs ascending eigenvalues each (total
+ * energies), residual estimates, matvec counts, converged flags.  Slot memory
+ * is allocated at creation (about (max_basis + 2 b + 8) vectors per slot).
+ * Results do not depend on the other requests in the batch.
+ * ====================================================================== */
+int edlgt_server_create(edlgt_csr* h, int32_t vec_type, int32_t max_slots, int32_t n_eigs, double tol,
+                        int32_t max_basis, int64_t max_matvecs, void* stream, edlgt_server** out);
+int edlgt_server_submit(edlgt_server* s, int64_t request_id, const double* coef, uint64_t seed,
+                        int32_t* slot_out);
+int edlgt_server_active(const edlgt_server* s, int32_t* active, int32_t* max_slots);
+int edlgt_server_step(edlgt_server* s, int32_t max_done, int64_t* done_ids, double* done_evals,
+                      double* done_resid_est, int64_t* done_matvecs, int32_t* done_converged,
+                      int32_t* n_done);
+int edlgt_server_destroy(edlgt_server* s);
 
 /* ======================================================================
  * Diagonal local observables.  Per-site marginals of |psi|^2:

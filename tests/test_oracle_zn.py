@@ -37,7 +37,7 @@ def test_closed_form_J0(lv, fs):
 
 
 @pytest.mark.parametrize("lv,obc,fs", [([2, 2], [False, False], None), ([2, 2], [False, False], (0, 0)),
-                                       ([3, 2], [False, False], (1, 0))])
+                                       ([2, 2], [False, False], (1, 2))])
 def test_sector_hamiltonian_equals_bruteforce(lv, obc, fs):
     M = ZnOracle(3, lv, obc, fs)
     H = M.hamiltonian(1.3, 0.6).toarray()
