@@ -321,17 +321,17 @@ int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t
   hipStream_t st = (hipStream_t)stream;
   const int nt = h->n_terms;
   const size_t es = vec_type == EDLGT_VALUE_C128 ? 16 : 8;
-  DevBuf xs, ys, cf;
-  EDLGT_TRY(xs.alloc(16 * sizeof(void*)));
-  EDLGT_TRY(ys.alloc(16 * sizeof(void*)));
-  EDLGT_TRY(cf.alloc(16 * 16 * sizeof(double)));
-  std::vector<const void*> hx(16);
-  std::vector<void*> hy(16);
-  std::vector<double> hc(16 * 16);
+  DevBuf xs, ys, cf, pack;
+  EDLGT_TRY(xs.alloc(kCoefMaxRhs * sizeof(void*)));
+  EDLGT_TRY(ys.alloc(kCoefMaxRhs * sizeof(void*)));
+  EDLGT_TRY(cf.alloc(kCoefMaxRhs * 16 * sizeof(double)));
+  EDLGT_TRY(pack.alloc(coef_pack_bytes(*h, vec_type, std::min(n_rhs, kCoefMaxRhs))));
+  std::vector<const void*> hx(kCoefMaxRhs);
+  std::vector<void*> hy(kCoefMaxRhs);
+  std::vector<double> hc(kCoefMaxRhs * 16, 0.0);
   int done = 0;
   while (done < n_rhs) {
-    const int left = n_rhs - done;
-    const int nb = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    const int nb = std::min(n_rhs - done, kCoefMaxRhs);
     for (int c = 0; c < nb; ++c) {
       hx[c] = (const char*)X + (size_t)(done + c) * h->n_rows * es;
       hy[c] = (char*)Y + (size_t)(done + c) * h->n_rows * es;
@@ -339,8 +339,8 @@ int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t
     }
     EDLGT_HIP(hipMemcpyAsync(xs.p, hx.data(), nb * sizeof(void*), hipMemcpyHostToDevice, st));
     EDLGT_HIP(hipMemcpyAsync(ys.p, hy.data(), nb * sizeof(void*), hipMemcpyHostToDevice, st));
-    EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data(), nb * 16 * sizeof(double), hipMemcpyHostToDevice, st));
-    EDLGT_TRY(csr_spmm_coef(*h, vec_type, nb, xs.p, ys.p, cf.as<double>(), st));
+    EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data(), kCoefMaxRhs * 16 * sizeof(double), hipMemcpyHostToDevice, st));
+    EDLGT_TRY(csr_spmm_coef(*h, vec_type, nb, xs.p, ys.p, cf.as<double>(), pack.p, st));
     EDLGT_HIP(hipStreamSynchronize(st));
     done += nb;
   }
@@ -413,16 +413,18 @@ struct edlgt_server {
 };
 
 int edlgt_server_create(edlgt_csr* h, int32_t vec_type, int32_t max_slots, int32_t n_eigs, double tol,
-                        int32_t max_basis, int64_t max_matvecs, void* stream, edlgt_server** out) {
+                        int32_t max_basis, int64_t max_matvecs, int32_t warm_cache, void* stream,
+                        edlgt_server** out) {
   EDLGT_REQUIRE(h && out, "null argument");
   EDLGT_REQUIRE(max_slots >= 1 && max_slots <= 64, "max_slots must be in [1, 64]");
   EDLGT_REQUIRE(n_eigs >= 1 && n_eigs <= h->n_rows && tol > 0.0, "bad n_eigs / tol");
   EDLGT_REQUIRE(vec_type == EDLGT_VALUE_F64 || vec_type == EDLGT_VALUE_C128, "vec_type must be F64 or C128");
   EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_F64 && h->value_type != EDLGT_VALUE_F64),
                 "real vectors need a real matrix");
+  EDLGT_REQUIRE(warm_cache >= 0 && warm_cache <= 4096, "warm_cache must be in [0, 4096]");
   *out = nullptr;
   int status = EDLGT_OK;
-  void* impl = server_create(*h, vec_type, max_slots, n_eigs, tol, max_basis, max_matvecs,
+  void* impl = server_create(*h, vec_type, max_slots, n_eigs, tol, max_basis, max_matvecs, warm_cache,
                              (hipStream_t)stream, &status);
   if (!impl) return status;
   auto* sv = new edlgt_server();
@@ -443,10 +445,17 @@ int edlgt_server_submit(edlgt_server* s, int64_t request_id, const double* coef,
   return EDLGT_OK;
 }
 
-int edlgt_server_active(const edlgt_server* s, int32_t* active, int32_t* max_slots) {
+int edlgt_server_info(const edlgt_server* s, int32_t* active, int32_t* allocated, int32_t* max_slots,
+                      int64_t* slot_bytes, int64_t* shared_bytes) {
   EDLGT_REQUIRE(s, "null server");
-  if (active) *active = server_active(s->impl);
+  int a = 0, al = 0;
+  int64_t sb = 0, wb = 0;
+  server_info(s->impl, &a, &al, &sb, &wb);
+  if (active) *active = a;
+  if (allocated) *allocated = al;
   if (max_slots) *max_slots = s->max_slots;
+  if (slot_bytes) *slot_bytes = sb;
+  if (shared_bytes) *shared_bytes = wb;
   return EDLGT_OK;
 }
 
@@ -459,6 +468,12 @@ int edlgt_server_step(edlgt_server* s, int32_t max_done, int64_t* done_ids, doub
                              done_converged, &nd);
   *n_done = nd;
   return st;
+}
+
+int edlgt_server_trim(edlgt_server* s) {
+  EDLGT_REQUIRE(s, "null server");
+  server_trim(s->impl);
+  return EDLGT_OK;
 }
 
 int edlgt_server_destroy(edlgt_server* s) {

@@ -29,8 +29,11 @@ int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, i
 int csr_make_low_precision(Csr& h, hipStream_t st);
 // Fused multi-coefficient SpMM (see spmv.hip): xs/ys device pointer arrays,
 // coef device [nb][16]; nb in {1, 2, 4, 8, 16}.
+constexpr int kCoefMaxRhs = 16;
+int coef_pad(int nb);
+size_t coef_pack_bytes(const Csr& h, int vec_type, int nb);
 int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const double* coef,
-                  hipStream_t st);
+                  void* pack, hipStream_t st);
 // Row partition of the CSR-stream SpMV (run once per matrix).
 int csr_prepare_stream(Csr& h, hipStream_t st);
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
@@ -46,10 +49,11 @@ int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, 
                hipStream_t st);
 
 void* server_create(Csr& h, int vec_type, int max_slots, int n_eigs, double tol, int max_basis,
-                    int64_t max_mv, hipStream_t st, int* status);
+                    int64_t max_mv, int warm_cap, hipStream_t st, int* status);
 void server_destroy(void* p);
 int server_submit(void* p, int64_t req, const double* coef, uint64_t seed, int* slot);
-int server_active(void* p);
+void server_trim(void* p);
+void server_info(void* p, int* active, int* allocated, int64_t* slot_bytes, int64_t* warm_bytes);
 int server_step(void* p, int max_done, int64_t* ids, double* evals, double* est, int64_t* mvs,
                 int32_t* conv, int* n_done);
 

@@ -53,67 +53,87 @@ template <> __device__ inline double2 zero<double2>() { return make_double2(0.0,
 __device__ inline double2 to_c(double2 a) { return a; }
 __device__ inline double2 to_c(double a) { return make_double2(a, 0.0); }
 
-constexpr int kGramBlocks = 512;
+constexpr int kGramBlocks = 1024;
+constexpr int kGramCols = 64;  // V columns per Gram launch (one per lane)
+constexpr int kGramRows = 8;   // rows per thread per sweep
 constexpr int kTB = 256;
 
-// partial[blk][j][c] = sum over the block's rows of conj(V_j[r]) W_c[r]
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double2 wave_sum(double2 v) { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
+
+// partial[blk][j][c] = sum over the block's rows of conj(V_j[r]) W_c[r],
+// j < nv <= 64, c < nw <= NW.  Each thread keeps kGramRows rows of W in
+// registers for a sweep, so W and V are each read once; for every column j
+// the lane products are summed across the wave with a fixed xor butterfly
+// and lane j keeps the running sum of column j.  The four waves' sums are
+// combined in fixed order: the result is bitwise reproducible.
 template <class S, int NW>
 __global__ void __launch_bounds__(kTB) gram_partial(const S* __restrict__ V, int64_t ldv, int nv,
                                                     const S* __restrict__ W, int64_t ldw, int nw,
                                                     int64_t n, double2* __restrict__ partial) {
-  constexpr int JB = NW >= 8 ? 1 : 8 / NW;
-  __shared__ double2 red[kTB];
+  __shared__ double2 red[kTB / 64][kGramCols][NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = (int64_t)blockIdx.x * chunk;
   const int64_t r1 = min(n, r0 + chunk);
-  for (int j0 = 0; j0 < nv; j0 += JB) {
-    S acc[JB][NW];
+  S mine[NW];
 #pragma unroll
-    for (int a = 0; a < JB; ++a)
+  for (int c = 0; c < NW; ++c) mine[c] = zero<S>();
+  for (int64_t base = r0; base < r1; base += (int64_t)kTB * kGramRows) {
+    S w[kGramRows][NW];
 #pragma unroll
-      for (int c = 0; c < NW; ++c) acc[a][c] = zero<S>();
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kTB) {
-      S w[NW];
+    for (int i = 0; i < kGramRows; ++i) {
+      const int64_t r = base + (int64_t)i * kTB + threadIdx.x;
 #pragma unroll
-      for (int c = 0; c < NW; ++c) w[c] = c < nw ? W[c * ldw + r] : zero<S>();
-#pragma unroll
-      for (int a = 0; a < JB; ++a) {
-        if (j0 + a < nv) {
-          const S v = V[(int64_t)(j0 + a) * ldv + r];
-#pragma unroll
-          for (int c = 0; c < NW; ++c) acc[a][c] = add(acc[a][c], cj_mul(v, w[c]));
-        }
-      }
+      for (int c = 0; c < NW; ++c) w[i][c] = (r < r1 && c < nw) ? W[c * ldw + r] : zero<S>();
     }
-    // deterministic block tree reduction, one (a, c) pair at a time
+    for (int j = 0; j < nv; ++j) {
+      S acc[NW];
 #pragma unroll
-    for (int a = 0; a < JB; ++a) {
+      for (int c = 0; c < NW; ++c) acc[c] = zero<S>();
+#pragma unroll
+      for (int i = 0; i < kGramRows; ++i) {
+        const int64_t r = base + (int64_t)i * kTB + threadIdx.x;
+        const S v = r < r1 ? V[(int64_t)j * ldv + r] : zero<S>();
+#pragma unroll
+        for (int c = 0; c < NW; ++c) acc[c] = add(acc[c], cj_mul(v, w[i][c]));
+      }
 #pragma unroll
       for (int c = 0; c < NW; ++c) {
-        red[threadIdx.x] = to_c(acc[a][c]);
-        __syncthreads();
-        for (int s = kTB / 2; s > 0; s >>= 1) {
-          if (threadIdx.x < s) red[threadIdx.x] = add(red[threadIdx.x], red[threadIdx.x + s]);
-          __syncthreads();
-        }
-        if (threadIdx.x == 0 && j0 + a < nv)
-          partial[((int64_t)blockIdx.x * nv + (j0 + a)) * NW + c] = red[0];
-        __syncthreads();
+        const S t = wave_sum(acc[c]);
+        if (lane == j) mine[c] = add(mine[c], t);
       }
     }
   }
+#pragma unroll
+  for (int c = 0; c < NW; ++c) red[wave][lane][c] = to_c(mine[c]);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nv * NW; t += kTB) {
+    const int j = t / NW, c = t % NW;
+    double2 s = red[0][j][c];
+#pragma unroll
+    for (int q = 1; q < kTB / 64; ++q) s = add(s, red[q][j][c]);
+    partial[((int64_t)blockIdx.x * nv + j) * NW + c] = s;
+  }
 }
 
+// out[(j0 + j) * nw_real + c] = sum over blocks of partial[b][j][c]: one wave
+// per output element, lanes over blocks, fixed butterfly.
 template <int NW>
-__global__ void gram_finish(const double2* __restrict__ partial, int nblk, int nv, int nw_real,
-                            double2* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nv * NW) return;
-  const int j = t / NW, c = t % NW;
-  if (c >= nw_real) return;
+__global__ void __launch_bounds__(kTB) gram_finish(const double2* __restrict__ partial, int nblk, int nv,
+                                                   int nw_real, int j0, double2* __restrict__ out) {
+  const int e = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (e >= nv * NW) return;  // whole waves exit together
+  const int j = e / NW, c = e % NW;
   double2 s = make_double2(0.0, 0.0);
-  for (int b = 0; b < nblk; ++b) s = add(s, partial[((int64_t)b * nv + j) * NW + c]);
-  out[(int64_t)j * nw_real + c] = s;
+  for (int b = lane; b < nblk; b += 64) s = add(s, partial[((int64_t)b * nv + j) * NW + c]);
+  s = wave_sum(s);
+  if (lane == 0 && c < nw_real) out[(int64_t)(j0 + j) * nw_real + c] = s;
 }
 
 // W[:, c] = beta * W[:, c] + sum_j V_j C[j][c]    (C: nv x nw, row-major, complex)
@@ -299,6 +319,7 @@ struct Solver {
   uint64_t seed = 12345;
   bool need_tmp = true;      // restart / Ritz buffer (not needed by Krylov stepping)
   bool want_vectors = true;  // form the Ritz vectors when a run finishes
+  bool need_w = true;        // spare block (Krylov stepping, true residuals)
 
   Solver(Csr& hh, hipStream_t s, int prec) : h(hh), st(s), precision(prec), N(hh.n_rows) {}
 
@@ -306,9 +327,9 @@ struct Solver {
 
   int setup() {
     EDLGT_TRY(V.alloc((size_t)(M + b) * N * sizeof(S)));
-    EDLGT_TRY(W.alloc((size_t)b * N * sizeof(S)));
+    if (need_w) EDLGT_TRY(W.alloc((size_t)b * N * sizeof(S)));
     if (need_tmp) EDLGT_TRY(Tmp.alloc((size_t)std::max(M, 8) * N * sizeof(S)));
-    EDLGT_TRY(part.alloc((size_t)kGramBlocks * (M + b) * 8 * sizeof(double2)));
+    EDLGT_TRY(part.alloc((size_t)kGramBlocks * kGramCols * 8 * sizeof(double2)));
     EDLGT_TRY(gdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
     EDLGT_TRY(cdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
     if (precision == EDLGT_PREC_MIXED) {
@@ -318,23 +339,28 @@ struct Solver {
     return EDLGT_OK;
   }
 
-  // G (nv x nw) = V[:, j0:j0+nv]^H X[:, 0:nw]   (X with leading dimension N)
+  // G (nv x nw) = Vp[:, 0:nv]^H X[:, 0:nw]   (X with leading dimension N)
   int gram(const S* Vp, int nv, const S* X, int nw, Dense& G) {
     G = Dense(nv, nw);
     if (nv == 0 || nw == 0) return EDLGT_OK;
-    int nblk = (int)std::min<int64_t>(kGramBlocks, std::max<int64_t>(1, (N + 2047) / 2048));
-#define GRAM_CASE(NWC)                                                                          \
-  case NWC:                                                                                     \
-    hipLaunchKernelGGL((gram_partial<S, NWC>), dim3(nblk), dim3(kTB), 0, st, Vp, N, nv, X, N, nw, \
-                       N, part.as<double2>());                                                   \
-    hipLaunchKernelGGL((gram_finish<NWC>), dim3((nv * NWC + 255) / 256), dim3(256), 0, st,        \
-                       part.as<double2>(), nblk, nv, nw, gdev.as<double2>());                    \
-    break;
-    int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : 8;
     if (nw > 8) return set_error(EDLGT_ERR_INVALID_ARG, "block size above 8");
-    switch (nwp) { GRAM_CASE(1) GRAM_CASE(2) GRAM_CASE(4) GRAM_CASE(8) }
+    const int nblk = (int)std::min<int64_t>(kGramBlocks, std::max<int64_t>(1, (N + 2047) / 2048));
+    const int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : 8;
+    for (int j0 = 0; j0 < nv; j0 += kGramCols) {
+      const int nc = std::min(kGramCols, nv - j0);
+      const S* Vc = Vp + (int64_t)j0 * N;
+      const unsigned fin = (unsigned)((nc * nwp + kTB / 64 - 1) / (kTB / 64));
+#define GRAM_CASE(NWC)                                                                              \
+  case NWC:                                                                                         \
+    hipLaunchKernelGGL((gram_partial<S, NWC>), dim3(nblk), dim3(kTB), 0, st, Vc, N, nc, X, N, nw, N, \
+                       part.as<double2>());                                                          \
+    hipLaunchKernelGGL((gram_finish<NWC>), dim3(fin), dim3(kTB), 0, st, part.as<double2>(), nblk, nc, \
+                       nw, j0, gdev.as<double2>());                                                  \
+    break;
+      switch (nwp) { GRAM_CASE(1) GRAM_CASE(2) GRAM_CASE(4) GRAM_CASE(8) }
 #undef GRAM_CASE
-    EDLGT_HIP(hipGetLastError());
+      EDLGT_HIP(hipGetLastError());
+    }
     std::vector<double2> hbuf((size_t)nv * nw);
     EDLGT_HIP(hipMemcpyAsync(hbuf.data(), gdev.p, hbuf.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
     EDLGT_HIP(hipStreamSynchronize(st));
@@ -491,9 +517,6 @@ struct Solver {
     return EDLGT_OK;
   }
 
-  // Runs thick-restart block Lanczos from the b start columns already in
-  // V[:, 0:b] (not necessarily orthonormal).  Fills theta (k values), and the
-  // Ritz vectors Y (k columns) into Tmp[:, 0:k]; resid gets the estimates.
   // ---- resumable thick-restart state (one expansion step per matvec)
   int st_nv = 0, st_l = 0, st_k = 0;
   double st_tol = 0.0;
@@ -509,17 +532,26 @@ struct Solver {
     st_k = k;
     st_tol = tol;
     st_max_mv = max_mv;
+    st_steps = 0;
     st_T = Dense(M + b, M + b);
     return EDLGT_OK;
   }
 
-  // The block the next matvec must multiply: V[:, nv-b : nv].
+  // The block the next matvec must multiply, V[:, nv-b : nv], and where its
+  // product goes, V[:, nv : nv+b] (the basis has room for M + b columns).
   S* active_block() { return col(st_nv - b); }
+  S* out_block() { return col(st_nv); }
 
-  // Consumes W = H * active_block(): orthogonalisation, T update, and when the
-  // basis is full Rayleigh-Ritz + convergence test + thick restart.  Sets
-  // finished when the wanted pairs converged (or the budget ran out); then
-  // theta/est hold the results and Tmp[:, 0:k] the Ritz vectors.
+  // Consumes out_block() = H * active_block(): orthogonalisation, T update, and when the
+  // basis is full Rayleigh-Ritz + convergence test + thick restart.  With
+  // early_every > 0 the convergence test also runs every early_every block
+  // steps before the basis is full (Rayleigh-Ritz of the projected matrix on
+  // the host; the residual of every Ritz pair is exact at any step:
+  // H V = V T + Q Rb E^T).  Sets finished when the wanted pairs converged (or
+  // the budget ran out); then theta/est hold the results and, with
+  // want_vectors, Tmp[:, 0:k] the Ritz vectors.
+  int early_every = 0;
+  int st_steps = 0;
   int after_matvec(bool& finished, std::vector<double>& theta, std::vector<double>& est,
                    bool& converged) {
     finished = false;
@@ -528,13 +560,15 @@ struct Solver {
     int& l = st_l;
     Dense& T = st_T;
     const int a0 = nv - b;
+    S* Wp = out_block();
     Dense G, Rb;
     if (nv >= N) {
       // the basis spans the whole space: T = V^H H V is exact, no residual
-      EDLGT_TRY(gram(V.as<S>(), nv, W.as<S>(), b, G));
+      EDLGT_TRY(gram(V.as<S>(), nv, Wp, b, G));
       Rb = Dense(b, b);
     } else {
-      EDLGT_TRY(orthonormalize(nv, W.as<S>(), b, G, Rb));
+      // the residual block Q is orthonormalised in place: V[:, nv:nv+b]
+      EDLGT_TRY(orthonormalize(nv, Wp, b, G, Rb));
     }
     for (int i = 0; i < nv; ++i)
       for (int c = 0; c < b; ++c) {
@@ -542,14 +576,16 @@ struct Solver {
         T(a0 + c, i) = std::conj(G(i, c));
       }
     const bool full = nv + b > M || nv + b > N;
-    // residual block Q -> V[:, nv:nv+b] (capacity M + b)
-    EDLGT_HIP(hipMemcpyAsync(col(nv), W.p, (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
     if (!full) {
       for (int c = 0; c < b; ++c)
         for (int i = 0; i < b; ++i) {
           T(nv + i, a0 + c) = Rb(i, c);
           T(a0 + c, nv + i) = std::conj(Rb(i, c));
         }
+    }
+    ++st_steps;
+    const bool test = full || (early_every > 0 && nv >= k && st_steps % early_every == 0);
+    if (!test) {
       nv += b;
       return EDLGT_OK;
     }
@@ -588,6 +624,10 @@ struct Solver {
       }
       return EDLGT_OK;
     }
+    if (!full) {
+      nv += b;
+      return EDLGT_OK;
+    }
     // thick restart: keep the lowest l Ritz vectors and the residual block
     l = std::max(k, std::min(M - 2 * b, k + (M - k) / 2));
     l = std::min(l, nv - b);
@@ -618,7 +658,7 @@ struct Solver {
     EDLGT_TRY(begin(k, tol, max_mv));
     converged = false;
     while (true) {
-      EDLGT_TRY(matvec(active_block(), W.as<S>(), b));
+      EDLGT_TRY(matvec(active_block(), out_block(), b));
       bool finished = false;
       EDLGT_TRY(after_matvec(finished, theta, est, converged));
       if (finished) return EDLGT_OK;
@@ -846,11 +886,17 @@ int krylov_typed(Csr& h, const Basis* basis, int precision, void* psi, double dt
 // advances every active slot by one block expansion: a single fused SpMM
 // applies each slot's own coefficients to its active block (the matrix is
 // streamed once for all slots), then each slot orthogonalises, updates its
-// projected matrix and, when its basis is full, runs Rayleigh-Ritz, tests
-// convergence and restarts.  Finished slots are freed at once and can take a
-// new request before the next step.  A request's start block depends only on
-// its seed and every column is computed in the same order whatever the batch,
-// so a result is bitwise the same alone or in any mix.
+// projected matrix, tests convergence and, when its basis is full, restarts.
+// Finished slots are freed at once and take a new request before the next
+// step.  Slots are allocated on first use (slot_bytes() each), so a caller can
+// admit by device-memory footprint.
+//
+// Start blocks: cold (warm_cap == 0) from the request's seed only, so a
+// result is bitwise the same alone or in any mix (every column of the fused
+// SpMM is computed in the same order for any batch).  Warm (warm_cap > 0):
+// the server keeps the Ritz vectors of up to warm_cap finished requests on the
+// device and starts a new request from those of the nearest coefficient
+// vector; the result then agrees with the cold one to the requested tolerance.
 template <class S>
 struct Server {
   Csr& h;
@@ -858,6 +904,7 @@ struct Server {
   int vec_type, b, M, k;
   double tol;
   int64_t max_mv;
+  int max_slots = 0;
   struct Slot {
     std::unique_ptr<Solver<S>> sv;
     bool active = false;
@@ -865,18 +912,24 @@ struct Server {
     double coef[16] = {0};
   };
   std::vector<Slot> slots;
-  DevBuf xs, ys, cf;
+  DevBuf xs, ys, cf, pack;
   std::vector<const void*> hx;
   std::vector<void*> hy;
   std::vector<double> hc;
+  // warm-start cache
+  int warm_cap = 0, warm_n = 0;
+  DevBuf warm;
+  std::vector<std::vector<double>> warm_coef;
 
   Server(Csr& hh, hipStream_t s, int vt) : h(hh), st(s), vec_type(vt) {}
 
-  int setup(int max_slots, int n_eigs, double tol_, int max_basis, int64_t max_mv_) {
+  int setup(int max_slots_, int n_eigs, double tol_, int max_basis, int64_t max_mv_, int warm_cap_) {
     const int64_t N = h.n_rows;
     k = n_eigs;
     tol = tol_;
     max_mv = max_mv_;
+    max_slots = max_slots_;
+    warm_cap = warm_cap_;
     b = std::min(k, 4);
     M = std::max(max_basis, 2 * k + 2 * b);
     M = (M + b - 1) / b * b;
@@ -887,17 +940,21 @@ struct Server {
       M = (int)std::max<int64_t>(2 * b, (N / 2) / b * b);
     }
     slots.resize(max_slots);
-    for (auto& sl : slots) {
-      sl.sv.reset(new Solver<S>(h, st, EDLGT_PREC_FP64));
-      sl.sv->b = b;
-      sl.sv->M = M;
-      sl.sv->want_vectors = false;
-      EDLGT_TRY(sl.sv->setup());
-    }
-    EDLGT_TRY(xs.alloc(64 * sizeof(void*)));
-    EDLGT_TRY(ys.alloc(64 * sizeof(void*)));
-    EDLGT_TRY(cf.alloc(16 * 16 * sizeof(double)));
+    EDLGT_TRY(xs.alloc(kCoefMaxRhs * sizeof(void*)));
+    EDLGT_TRY(ys.alloc(kCoefMaxRhs * sizeof(void*)));
+    EDLGT_TRY(cf.alloc(kCoefMaxRhs * 16 * sizeof(double)));
     return EDLGT_OK;
+  }
+
+  size_t slot_bytes() const {
+    const size_t N = (size_t)h.n_rows;
+    return ((size_t)(M + b) + std::max(M, 8)) * N * sizeof(S) +
+           (size_t)kGramBlocks * kGramCols * 8 * sizeof(double2) +
+           2 * (size_t)(M + b) * std::max(M, 8) * sizeof(double2);
+  }
+  // warm cache plus the packed right-hand-side workspace of the fused SpMM
+  size_t shared_bytes() const {
+    return (size_t)warm_cap * k * (size_t)h.n_rows * sizeof(S) + coef_pack_bytes(h, vec_type, kCoefMaxRhs);
   }
 
   int active() const {
@@ -905,27 +962,101 @@ struct Server {
     for (auto& sl : slots) a += sl.active ? 1 : 0;
     return a;
   }
+  int allocated() const {
+    int a = 0;
+    for (auto& sl : slots) a += sl.sv ? 1 : 0;
+    return a;
+  }
 
+  // Frees the memory of every idle slot.
+  void trim() {
+    for (auto& sl : slots)
+      if (sl.sv && !sl.active) sl.sv.reset();
+  }
+
+  int nearest_warm(const double* c) const {
+    int best = -1;
+    double bd = 0.0;
+    for (int e = 0; e < warm_n; ++e) {
+      double d = 0.0;
+      for (int t = 0; t < h.n_terms; ++t) d += (warm_coef[e][t] - c[t]) * (warm_coef[e][t] - c[t]);
+      if (best < 0 || d < bd) {
+        best = e;
+        bd = d;
+      }
+    }
+    return best;
+  }
+
+  // Prefers an allocated free slot; allocates a new one (up to max_slots)
+  // otherwise.  *slot_out = -1 when every slot is busy.
   int submit(int64_t req, const double* coef, uint64_t seed, int* slot_out) {
     *slot_out = -1;
-    for (size_t i = 0; i < slots.size(); ++i) {
-      Slot& sl = slots[i];
-      if (sl.active) continue;
-      for (int t = 0; t < 16; ++t) sl.coef[t] = t < h.n_terms ? coef[t] : 0.0;
-      sl.sv->seed = seed;
-      sl.sv->mv64 = sl.sv->mv32 = 0;
-      EDLGT_TRY(sl.sv->randomize(sl.sv->col(0), (int64_t)b * h.n_rows));
-      EDLGT_TRY(sl.sv->begin(k, tol, max_mv));
-      sl.req = req;
-      sl.active = true;
-      *slot_out = (int)i;
-      return EDLGT_OK;
+    int pick = -1;
+    for (size_t i = 0; i < slots.size() && pick < 0; ++i)
+      if (slots[i].sv && !slots[i].active) pick = (int)i;
+    for (size_t i = 0; i < slots.size() && pick < 0; ++i)
+      if (!slots[i].sv) {
+        Slot& sl = slots[i];
+        sl.sv.reset(new Solver<S>(h, st, EDLGT_PREC_FP64));
+        sl.sv->b = b;
+        sl.sv->M = M;
+        sl.sv->want_vectors = warm_cap > 0;
+        sl.sv->need_w = false;
+        const int r = sl.sv->setup();
+        if (r != EDLGT_OK) {
+          sl.sv.reset();
+          return r;
+        }
+        pick = (int)i;
+      }
+    if (pick < 0) return EDLGT_OK;
+    Slot& sl = slots[pick];
+    for (int t = 0; t < 16; ++t) sl.coef[t] = t < h.n_terms ? coef[t] : 0.0;
+    sl.sv->seed = seed;
+    sl.sv->mv64 = sl.sv->mv32 = 0;
+    const int64_t N = h.n_rows;
+    const int e = warm_cap > 0 ? nearest_warm(sl.coef) : -1;
+    // A warm start block approximates every wanted eigenvector, so the
+    // convergence test runs at every step; a random start block is tested only
+    // when the basis is full (a small Krylov space can hold converged Ritz
+    // pairs while a lower or degenerate eigenvector is still missing).
+    sl.sv->early_every = e >= 0 ? 1 : 0;
+    if (e >= 0) {
+      EDLGT_HIP(hipMemcpyAsync(sl.sv->col(0), warm.as<S>() + (size_t)e * k * N, (size_t)b * N * sizeof(S),
+                               hipMemcpyDeviceToDevice, st));
+    } else {
+      EDLGT_TRY(sl.sv->randomize(sl.sv->col(0), (int64_t)b * N));
     }
+    EDLGT_TRY(sl.sv->begin(k, tol, max_mv));
+    sl.req = req;
+    sl.active = true;
+    *slot_out = pick;
+    return EDLGT_OK;
+  }
+
+  // Keeps the Ritz vectors of a finished request (Tmp[:, 0:k]); when the cache
+  // is full the entry nearest in coefficients is replaced, which keeps the
+  // cached couplings spread over the requested range.
+  int remember(Slot& sl) {
+    if (warm_cap <= 0) return EDLGT_OK;
+    const int64_t N = h.n_rows;
+    if (!warm.p) EDLGT_TRY(warm.alloc((size_t)warm_cap * k * (size_t)h.n_rows * sizeof(S)));
+    int e;
+    if (warm_n < warm_cap) {
+      e = warm_n++;
+      warm_coef.emplace_back(sl.coef, sl.coef + 16);
+    } else {
+      e = nearest_warm(sl.coef);
+      warm_coef[e].assign(sl.coef, sl.coef + 16);
+    }
+    EDLGT_HIP(hipMemcpyAsync(warm.as<S>() + (size_t)e * k * N, sl.sv->Tmp.p, (size_t)k * N * sizeof(S),
+                             hipMemcpyDeviceToDevice, st));
     return EDLGT_OK;
   }
 
   // Fused SpMM over the active blocks of all active slots, in chunks of at most
-  // 16 columns (chunk widths 16/8/4/2/1).
+  // 16 columns; each slot's product lands in V[:, nv:nv+b] (out_block).
   int fused_matvec() {
     hx.clear();
     hy.clear();
@@ -934,20 +1065,21 @@ struct Server {
       if (!sl.active) continue;
       for (int c = 0; c < b; ++c) {
         hx.push_back(sl.sv->active_block() + (int64_t)c * h.n_rows);
-        hy.push_back(sl.sv->W.template as<S>() + (int64_t)c * h.n_rows);
+        hy.push_back(sl.sv->out_block() + (int64_t)c * h.n_rows);
         for (int t = 0; t < 16; ++t) hc.push_back(sl.coef[t]);
         sl.sv->mv64 += 1;
       }
     }
+    if (!pack.p) EDLGT_TRY(pack.alloc(coef_pack_bytes(h, vec_type, kCoefMaxRhs)));
+    hc.resize((hx.size() + kCoefMaxRhs) * 16, 0.0);
     size_t done = 0;
     while (done < hx.size()) {
-      const size_t left = hx.size() - done;
-      const int nb = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+      const int nb = (int)std::min<size_t>(hx.size() - done, kCoefMaxRhs);
       EDLGT_HIP(hipMemcpyAsync(xs.p, hx.data() + done, nb * sizeof(void*), hipMemcpyHostToDevice, st));
       EDLGT_HIP(hipMemcpyAsync(ys.p, hy.data() + done, nb * sizeof(void*), hipMemcpyHostToDevice, st));
-      EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data() + done * 16, nb * 16 * sizeof(double),
+      EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data() + done * 16, kCoefMaxRhs * 16 * sizeof(double),
                                hipMemcpyHostToDevice, st));
-      EDLGT_TRY(csr_spmm_coef(h, vec_type, nb, xs.p, ys.p, cf.as<double>(), st));
+      EDLGT_TRY(csr_spmm_coef(h, vec_type, nb, xs.p, ys.p, cf.as<double>(), pack.p, st));
       EDLGT_HIP(hipStreamSynchronize(st));  // the pointer/coefficient buffers are reused
       done += nb;
     }
@@ -970,6 +1102,7 @@ struct Server {
       std::vector<double> th, es;
       EDLGT_TRY(sl.sv->after_matvec(finished, th, es, conv));
       if (finished) {
+        if (conv) EDLGT_TRY(remember(sl));
         out.push_back(Done{sl.req, th, es, sl.sv->mv64, conv});
         sl.active = false;
         sl.req = -1;
@@ -1010,16 +1143,16 @@ int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter,
 }
 
 void* server_create(Csr& h, int vec_type, int max_slots, int n_eigs, double tol, int max_basis,
-                    int64_t max_mv, hipStream_t st, int* status) {
+                    int64_t max_mv, int warm_cap, hipStream_t st, int* status) {
   auto* sa = new ServerAny();
   sa->vec_type = vec_type;
   int s;
   if (vec_type == EDLGT_VALUE_F64) {
     sa->r.reset(new Server<double>(h, st, vec_type));
-    s = sa->r->setup(max_slots, n_eigs, tol, max_basis, max_mv);
+    s = sa->r->setup(max_slots, n_eigs, tol, max_basis, max_mv, warm_cap);
   } else {
     sa->c.reset(new Server<double2>(h, st, vec_type));
-    s = sa->c->setup(max_slots, n_eigs, tol, max_basis, max_mv);
+    s = sa->c->setup(max_slots, n_eigs, tol, max_basis, max_mv, warm_cap);
   }
   *status = s;
   if (s != EDLGT_OK) {
@@ -1036,9 +1169,27 @@ int server_submit(void* p, int64_t req, const double* coef, uint64_t seed, int* 
   return sa->r ? sa->r->submit(req, coef, seed, slot) : sa->c->submit(req, coef, seed, slot);
 }
 
-int server_active(void* p) {
+void server_trim(void* p) {
   auto* sa = static_cast<ServerAny*>(p);
-  return sa->r ? sa->r->active() : sa->c->active();
+  if (sa->r)
+    sa->r->trim();
+  else
+    sa->c->trim();
+}
+
+void server_info(void* p, int* active, int* allocated, int64_t* slot_bytes, int64_t* warm_bytes) {
+  auto* sa = static_cast<ServerAny*>(p);
+  if (sa->r) {
+    *active = sa->r->active();
+    *allocated = sa->r->allocated();
+    *slot_bytes = (int64_t)sa->r->slot_bytes();
+    *warm_bytes = (int64_t)sa->r->shared_bytes();
+  } else {
+    *active = sa->c->active();
+    *allocated = sa->c->allocated();
+    *slot_bytes = (int64_t)sa->c->slot_bytes();
+    *warm_bytes = (int64_t)sa->c->shared_bytes();
+  }
 }
 
 int server_step(void* p, int max_done, int64_t* ids, double* evals, double* est, int64_t* mvs,

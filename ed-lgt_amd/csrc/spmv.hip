@@ -442,64 +442,99 @@ namespace {
 
 // Multi-coefficient product: for right-hand side b, y_b = sum_k coef[b][k] H_k x_b
 // where H = sum_k H_k shares one CSR pattern and every entry carries its term
-// id k.  One pass streams col/val/term once for all NB right-hand sides (the
-// fused SpMM of batched parameter sweeps and served solves); each column's
-// accumulation order is the same for any NB, so a right-hand side gets the
-// same bits alone or in a batch.
+// id k.  The nb right-hand sides are first packed row-interleaved
+// (xi[r * NB + b] = x_b[r]), so that the gather for one stored entry fetches
+// the NB values of its column as one contiguous line instead of NB scattered
+// words from NB separate vectors; one pass then streams col/val/term once for
+// all NB right-hand sides (the fused SpMM of batched parameter sweeps and
+// served solves).  Each column's accumulation order is the same for any NB
+// and any batch composition, so a right-hand side gets the same bits alone or
+// in a batch.
 constexpr int kMaxTerms = 16;
 
 __device__ inline double2 scale(double c, double2 v) { return make_double2(c * v.x, c * v.y); }
 __device__ inline double scale(double c, double v) { return c * v; }
 
-template <class VT, class XT, class AT, int G, int NB>
-__global__ void __launch_bounds__(256) csr_spmm_coef_kernel(
+template <class S, int NB>
+__global__ void __launch_bounds__(256) pack_rows(const S* const* __restrict__ xs, int nb, int64_t n,
+                                                 S* __restrict__ xi) {
+  const S* xp[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) xp[b] = b < nb ? xs[b] : nullptr;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) xi[r * NB + b] = b < nb ? xp[b][r] : zero_acc<S>();
+  }
+}
+
+// NB consecutive vector elements of one packed row, 16-byte loads.
+template <int NB>
+__device__ inline void load_row(const double* __restrict__ q, double (&x)[NB]) {
+  if constexpr (NB == 1) {
+    x[0] = q[0];
+  } else {
+    const double2* q2 = reinterpret_cast<const double2*>(q);
+#pragma unroll
+    for (int i = 0; i < NB / 2; ++i) {
+      const double2 v = q2[i];
+      x[2 * i] = v.x;
+      x[2 * i + 1] = v.y;
+    }
+  }
+}
+template <int NB>
+__device__ inline void load_row(const double2* __restrict__ q, double2 (&x)[NB]) {
+#pragma unroll
+  for (int i = 0; i < NB; ++i) x[i] = q[i];
+}
+
+template <class VT, class S, int G, int NB>
+__global__ void __launch_bounds__(256) csr_spmm_coef_rows(
     int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-    const VT* __restrict__ val, const uint8_t* __restrict__ term,
-    const XT* const* __restrict__ xs, AT* const* __restrict__ ys, const double* __restrict__ coef) {
+    const VT* __restrict__ val, const uint8_t* __restrict__ term, const S* __restrict__ xi,
+    S* const* __restrict__ ys, int nb, const double* __restrict__ coef) {
   __shared__ double cs[NB * kMaxTerms];
   for (int i = threadIdx.x; i < NB * kMaxTerms; i += blockDim.x) cs[i] = coef[i];
   __syncthreads();
-  const XT* xp[NB];
-  AT* yp[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    xp[b] = xs[b];
-    yp[b] = ys[b];
-  }
   const int sub = threadIdx.x & (G - 1);
   const int64_t stride = (int64_t)gridDim.x * (256 / G);
   for (int64_t r = (int64_t)blockIdx.x * (256 / G) + (threadIdx.x / G); r < n; r += stride) {
     const int64_t beg = row_ptr[r], end = row_ptr[r + 1];
-    AT acc[NB];
+    S acc[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = zero_acc<AT>();
+    for (int b = 0; b < NB; ++b) acc[b] = zero_acc<S>();
     for (int64_t p = beg + sub; p < end; p += G) {
       const int32_t c = ldnt(col + p);
       const auto v = load_val(val, p);
       const int t = term ? (int)ldnt(term + p) : 0;
+      S x[NB];
+      load_row<NB>(xi + (int64_t)c * NB, x);
 #pragma unroll
-      for (int b = 0; b < NB; ++b) fma_acc(acc[b], scale(cs[b * kMaxTerms + t], v), load_x(xp[b], c));
+      for (int b = 0; b < NB; ++b) fma_acc(acc[b], scale(cs[b * kMaxTerms + t], v), x[b]);
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const AT s = group_sum<G>(acc[b]);
-      if (sub == 0) yp[b][r] = s;
+      const S s = group_sum<G>(acc[b]);
+      if (sub == 0 && b < nb) ys[b][r] = s;
     }
   }
 }
 
-template <class VT, class XT, class AT, int G>
-int launch_coef_nb(int nb, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
-                   const double* coef, hipStream_t st) {
+template <class VT, class S, int G>
+int launch_coef_nb(int nb, int nbp, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
+                   const double* coef, void* pack, hipStream_t st) {
+  const unsigned pgrid = grid_for(n, 256, 65536);
   const int64_t blocks = std::min<int64_t>(65536, std::max<int64_t>(1, (n + 256 / G - 1) / (256 / G)));
+  const uint8_t* term = h.n_terms > 1 ? h.term.as<uint8_t>() : (const uint8_t*)nullptr;
 #define EDLGT_COEF_CASE(NBC)                                                                     \
   case NBC:                                                                                      \
-    hipLaunchKernelGGL((csr_spmm_coef_kernel<VT, XT, AT, G, NBC>), dim3((unsigned)blocks),        \
-                       dim3(256), 0, st, n, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), val,   \
-                       h.n_terms > 1 ? h.term.as<uint8_t>() : (const uint8_t*)nullptr,         \
-                       (const XT* const*)xs, (AT* const*)ys, coef);                             \
+    hipLaunchKernelGGL((pack_rows<S, NBC>), dim3(pgrid), dim3(256), 0, st, (const S* const*)xs, nb, n, \
+                       (S*)pack);                                                                \
+    hipLaunchKernelGGL((csr_spmm_coef_rows<VT, S, G, NBC>), dim3((unsigned)blocks), dim3(256), 0, st, \
+                       n, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), val, term, (const S*)pack, \
+                       (S* const*)ys, nb, coef);                                                 \
     break;
-  switch (nb) {
+  switch (nbp) {
     EDLGT_COEF_CASE(1)
     EDLGT_COEF_CASE(2)
     EDLGT_COEF_CASE(4)
@@ -513,14 +548,14 @@ int launch_coef_nb(int nb, int64_t n, const Csr& h, const VT* val, const void* x
   return EDLGT_OK;
 }
 
-template <class VT, class XT, class AT>
-int launch_coef(int g, int nb, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
-                const double* coef, hipStream_t st) {
+template <class VT, class S>
+int launch_coef(int g, int nb, int nbp, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
+                const double* coef, void* pack, hipStream_t st) {
   switch (g) {
-    case 4: return launch_coef_nb<VT, XT, AT, 4>(nb, n, h, val, xs, ys, coef, st);
-    case 8: return launch_coef_nb<VT, XT, AT, 8>(nb, n, h, val, xs, ys, coef, st);
-    case 16: return launch_coef_nb<VT, XT, AT, 16>(nb, n, h, val, xs, ys, coef, st);
-    case 32: return launch_coef_nb<VT, XT, AT, 32>(nb, n, h, val, xs, ys, coef, st);
+    case 4: return launch_coef_nb<VT, S, 4>(nb, nbp, n, h, val, xs, ys, coef, pack, st);
+    case 8: return launch_coef_nb<VT, S, 8>(nb, nbp, n, h, val, xs, ys, coef, pack, st);
+    case 16: return launch_coef_nb<VT, S, 16>(nb, nbp, n, h, val, xs, ys, coef, pack, st);
+    case 32: return launch_coef_nb<VT, S, 32>(nb, nbp, n, h, val, xs, ys, coef, pack, st);
   }
   return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");
 }
@@ -532,20 +567,31 @@ int coef_lanes(const Csr& h) {
   return avg <= 6 ? 4 : avg <= 14 ? 8 : avg <= 40 ? 16 : 32;
 }
 
-// xs / ys: device arrays of nb device pointers; coef: device [nb][16] doubles.
-// nb must be 1, 2, 4, 8 or 16.  Vectors: complex128 for a complex matrix,
-// float64 (F64) or complex128 (C128) for a real matrix.
+int coef_pad(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : 16; }
+
+size_t coef_pack_bytes(const Csr& h, int vec_type, int nb) {
+  return (size_t)h.n_rows * coef_pad(nb) * (vec_type == EDLGT_VALUE_C128 ? 16 : 8);
+}
+
+// xs / ys: device arrays of nb device pointers; coef: device [nb][16] doubles
+// (padded to the kernel width: unused rows may hold anything finite); pack:
+// device workspace of coef_pack_bytes(h, vec_type, nb).  1 <= nb <= 16.
+// Vectors: complex128 for a complex matrix, float64 (F64) or complex128 (C128)
+// for a real matrix.  The padded right-hand sides read zeros and are not
+// written.
 int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const double* coef,
-                  hipStream_t st) {
+                  void* pack, hipStream_t st) {
+  if (nb < 1 || nb > kCoefMaxRhs) return set_error(EDLGT_ERR_INVALID_ARG, "batch width must be in [1, 16]");
   const int g = coef_lanes(h);
+  const int nbp = coef_pad(nb);
   const int64_t n = h.n_rows;
   if (h.value_type == EDLGT_VALUE_C128) {
     if (vec_type != EDLGT_VALUE_C128) return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
-    return launch_coef<double2, double2, double2>(g, nb, n, h, h.val.as<double2>(), xs, ys, coef, st);
+    return launch_coef<double2, double2>(g, nb, nbp, n, h, h.val.as<double2>(), xs, ys, coef, pack, st);
   }
   if (vec_type == EDLGT_VALUE_F64)
-    return launch_coef<double, double, double>(g, nb, n, h, h.val.as<double>(), xs, ys, coef, st);
-  return launch_coef<double, double2, double2>(g, nb, n, h, h.val.as<double>(), xs, ys, coef, st);
+    return launch_coef<double, double>(g, nb, nbp, n, h, h.val.as<double>(), xs, ys, coef, pack, st);
+  return launch_coef<double, double2>(g, nb, nbp, n, h, h.val.as<double>(), xs, ys, coef, pack, st);
 }
 
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,

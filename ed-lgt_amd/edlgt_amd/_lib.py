@@ -68,10 +68,11 @@ SIGNATURES = {
     "edlgt_hamiltonian_build_terms": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _p, _i32, _i32, _p, _p]),
     "edlgt_csr_copy_terms": (_i32, [_p, _p, _p]),
     "edlgt_spmm_coef": (_i32, [_p, _i32, _p, _p, _i32, _p, _p]),
-    "edlgt_server_create": (_i32, [_p, _i32, _i32, _i32, _f64, _i32, _i64, _p, _p]),
+    "edlgt_server_create": (_i32, [_p, _i32, _i32, _i32, _f64, _i32, _i64, _i32, _p, _p]),
     "edlgt_server_submit": (_i32, [_p, _i64, _p, C.c_uint64, _p]),
-    "edlgt_server_active": (_i32, [_p, _p, _p]),
+    "edlgt_server_info": (_i32, [_p, _p, _p, _p, _p, _p]),
     "edlgt_server_step": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _p]),
+    "edlgt_server_trim": (_i32, [_p]),
     "edlgt_server_destroy": (_i32, [_p]),
     "edlgt_lanczos": (_i32, [_p, _i32, _f64, _i32, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "edlgt_basis_marginals": (_i32, [_p, _p, _i32, _i32, _p, _p]),

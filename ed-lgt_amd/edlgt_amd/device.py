@@ -316,11 +316,14 @@ class Server:
     multi-term matrix (edlgt_server_*).  Each request is a coefficient vector
     c (H(c) = sum_t c[t] H_t) and a seed; up to max_slots requests advance
     together, one fused SpMM per step, and a finished request frees its slot
-    at once for the next one.  A request's result does not depend on which
-    other requests share the batch."""
+    at once for the next one.  Slots are allocated on first use
+    (slot_bytes each).  With warm_cache = 0 a request's result does not depend
+    on which other requests share the batch (bitwise); with warm_cache = C the
+    server starts new requests from the stored Ritz vectors of the nearest of
+    up to C finished couplings (results equal to tol)."""
 
     def __init__(self, csr, n_eigs=1, tol=1e-10, max_basis=32, max_matvecs=100000,
-                 max_slots=8, complex_vectors=None, stream=None):
+                 max_slots=8, complex_vectors=None, warm_cache=0, stream=None):
         lib = L.load()
         self.csr = csr
         self.n_eigs = int(n_eigs)
@@ -329,7 +332,8 @@ class Server:
         vt = csr.vec_type(complex_vectors)
         s = C.c_void_p()
         L.check(lib.edlgt_server_create(csr.handle, vt, self.max_slots, self.n_eigs, float(tol),
-                                        int(max_basis), int(max_matvecs), self._stream, C.byref(s)))
+                                        int(max_basis), int(max_matvecs), int(warm_cache),
+                                        self._stream, C.byref(s)))
         self._s = s
         self._ids = np.zeros(self.max_slots, dtype=np.int64)
         self._ev = np.zeros((self.max_slots, self.n_eigs))
@@ -346,11 +350,22 @@ class Server:
                                              C.c_uint64(int(seed)), C.byref(slot)))
         return slot.value >= 0
 
+    def info(self):
+        """dict(active, allocated, max_slots, slot_bytes, shared_bytes)."""
+        a, al, mx = C.c_int32(), C.c_int32(), C.c_int32()
+        sb, wb = C.c_int64(), C.c_int64()
+        L.check(L.load().edlgt_server_info(self._s, C.byref(a), C.byref(al), C.byref(mx), C.byref(sb),
+                                           C.byref(wb)))
+        return dict(active=a.value, allocated=al.value, max_slots=mx.value, slot_bytes=sb.value,
+                    shared_bytes=wb.value)
+
     @property
     def active(self):
-        a = C.c_int32()
-        L.check(L.load().edlgt_server_active(self._s, C.byref(a), None))
-        return int(a.value)
+        return self.info()["active"]
+
+    def trim(self):
+        """Frees the memory of every idle slot."""
+        L.check(L.load().edlgt_server_trim(self._s))
 
     def step(self):
         """One fused expansion step; returns the requests that finished as a

@@ -241,18 +241,30 @@ int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis,
  * edlgt_server_step advances every active slot by one block-Lanczos expansion
  * (one fused SpMM streams the matrix once for all slots) and reports the
  * requests that finished: ids, n_eigs ascending eigenvalues each (total
- * energies), residual estimates, matvec counts, converged flags.  Slot memory
- * is allocated at creation (about (max_basis + 2 b + 8) vectors per slot).
- * Results do not depend on the other requests in the batch.
+ * energies), residual estimates, matvec counts, converged flags.
+ * Slots are allocated on first use, slot_bytes each (edlgt_server_info), so
+ * a caller admits by device-memory footprint: submit only while a free
+ * allocated slot exists or the budget covers one more slot.
+ * warm_cache = 0: a request starts from its seed alone and its result is
+ * bitwise independent of the other requests.  warm_cache = C > 0: the server
+ * keeps the Ritz vectors of up to C finished requests and starts each new
+ * request from those of the nearest coefficient vector; results then agree
+ * with the cold ones to tol.  shared_bytes = that cache plus the packed
+ * right-hand-side workspace of the fused SpMM (both allocated on first use).
  * ====================================================================== */
 int edlgt_server_create(edlgt_csr* h, int32_t vec_type, int32_t max_slots, int32_t n_eigs, double tol,
-                        int32_t max_basis, int64_t max_matvecs, void* stream, edlgt_server** out);
+                        int32_t max_basis, int64_t max_matvecs, int32_t warm_cache, void* stream,
+                        edlgt_server** out);
 int edlgt_server_submit(edlgt_server* s, int64_t request_id, const double* coef, uint64_t seed,
                         int32_t* slot_out);
-int edlgt_server_active(const edlgt_server* s, int32_t* active, int32_t* max_slots);
+int edlgt_server_info(const edlgt_server* s, int32_t* active, int32_t* allocated, int32_t* max_slots,
+                      int64_t* slot_bytes, int64_t* shared_bytes);
 int edlgt_server_step(edlgt_server* s, int32_t max_done, int64_t* done_ids, double* done_evals,
                       double* done_resid_est, int64_t* done_matvecs, int32_t* done_converged,
                       int32_t* n_done);
+/* Frees the device memory of every idle slot (for a budget shared with other
+ * servers); busy slots and the warm cache are kept. */
+int edlgt_server_trim(edlgt_server* s);
 int edlgt_server_destroy(edlgt_server* s);
 
 /* ======================================================================

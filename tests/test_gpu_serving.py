@@ -104,3 +104,28 @@ def test_server_complex_vectors_and_full_slots(gpu):
     for rid, coef in [(0, [1.0, 0.5]), (1, [0.5, 1.0])]:
         ref = lowest_eigs(orc.hamiltonian(*coef), 2)
         assert np.max(np.abs(out[rid] - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_server_warm_start_matches_cold(gpu):
+    from edlgt_amd import Server
+    orc, mod = _zn([3, 3])
+    H2 = mod.build_multi_term()
+    reqs = [(i, [1.0, float(h)], 500 + i) for i, h in enumerate(np.linspace(0.4, 2.8, 9))]
+    cold = Server(H2, n_eigs=3, tol=1e-10, max_basis=24, max_slots=3).solve_all(reqs)
+    srv = Server(H2, n_eigs=3, tol=1e-10, max_basis=24, max_slots=3, warm_cache=4)
+    warm = srv.solve_all(reqs)
+    for rid, coef, _ in reqs:
+        ref = lowest_eigs(orc.hamiltonian(*coef), 3)
+        assert warm[rid][3]
+        assert np.max(np.abs(warm[rid][0] - cold[rid][0])) < 1e-8 * max(1.0, np.max(np.abs(ref)))
+        assert np.max(np.abs(warm[rid][0] - ref)) < 1e-8 * max(1.0, np.max(np.abs(ref)))
+    assert sum(v[2] for v in warm.values()) < sum(v[2] for v in cold.values())
+    inf = srv.info()
+    assert inf["active"] == 0 and inf["allocated"] == 3
+    assert inf["shared_bytes"] >= 4 * 3 * H2.n * 8
+    srv.trim()
+    assert srv.info()["allocated"] == 0
+    # a trimmed server allocates again on demand
+    again = srv.solve_all([(99, [1.0, 1.1], 7)])
+    ref = lowest_eigs(orc.hamiltonian(1.0, 1.1), 3)
+    assert np.max(np.abs(again[99][0] - ref)) < 1e-8 * max(1.0, np.max(np.abs(ref)))
