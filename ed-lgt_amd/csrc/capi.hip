@@ -251,6 +251,15 @@ int edlgt_csr_copy_terms(const edlgt_csr* h, uint8_t* term_host, int32_t* n_term
   return EDLGT_OK;
 }
 
+int edlgt_csr_compress(edlgt_csr* h, void* stream, int32_t* n_dict, int32_t* code_bytes) {
+  EDLGT_REQUIRE(h, "null matrix");
+  int done = 0;
+  EDLGT_TRY(csr_compress(*h, (hipStream_t)stream, &done));
+  if (n_dict) *n_dict = h->n_dict;
+  if (code_bytes) *code_bytes = h->code_bytes;
+  return EDLGT_OK;
+}
+
 int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* value_type) {
   EDLGT_REQUIRE(h, "null matrix");
   if (n_rows) *n_rows = h->n_rows;
@@ -340,7 +349,7 @@ int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t
     EDLGT_HIP(hipMemcpyAsync(xs.p, hx.data(), nb * sizeof(void*), hipMemcpyHostToDevice, st));
     EDLGT_HIP(hipMemcpyAsync(ys.p, hy.data(), nb * sizeof(void*), hipMemcpyHostToDevice, st));
     EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data(), kCoefMaxRhs * 16 * sizeof(double), hipMemcpyHostToDevice, st));
-    EDLGT_TRY(csr_spmm_coef(*h, vec_type, nb, xs.p, ys.p, cf.as<double>(), pack.p, st));
+    EDLGT_TRY(csr_spmm_coef(*h, vec_type, nb, xs.p, ys.p, hx.data(), hy.data(), cf.as<double>(), pack.p, st));
     EDLGT_HIP(hipStreamSynchronize(st));
     done += nb;
   }

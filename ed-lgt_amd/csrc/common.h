@@ -106,6 +106,15 @@ struct Csr {
   DevBuf stream_blocks;      // int64 [n_stream_blocks + 1] first row of every CSR-stream tile
   DevBuf stream_pos;         // int64 [n_stream_blocks + 1] first entry of every tile
   int64_t n_stream_blocks = 0;
+  // dictionary coding of the values (edlgt_csr_compress): entry p holds
+  // dict_val[code[p]] of term dict_term[code[p]]; code is uint8 when
+  // n_dict <= 256, else uint16 (n_dict <= 4096)
+  int32_t n_dict = 0;
+  int32_t code_bytes = 0;
+  DevBuf code;
+  DevBuf dict_val;   // double2 [n_dict] (imaginary part 0 for real matrices)
+  DevBuf dict_term;  // int32 [n_dict]
+  DevBuf tab_unit;   // dict values with all term coefficients 1 (double or double2)
 };
 
 inline size_t value_bytes(int32_t vt) {

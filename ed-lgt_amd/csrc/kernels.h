@@ -32,10 +32,12 @@ int csr_make_low_precision(Csr& h, hipStream_t st);
 constexpr int kCoefMaxRhs = 16;
 int coef_pad(int nb);
 size_t coef_pack_bytes(const Csr& h, int vec_type, int nb);
-int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const double* coef,
-                  void* pack, hipStream_t st);
+bool coef_fused(const Csr& h, int vec_type);
+int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const void* const* hx,
+                  void* const* hy, const double* coef, void* pack, hipStream_t st);
 // Row partition of the CSR-stream SpMV (run once per matrix).
 int csr_prepare_stream(Csr& h, hipStream_t st);
+int csr_compress(Csr& h, hipStream_t st, int* compressed);
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
                  hipStream_t st, int32_t* best_g, int64_t* best_grid, float* best_ms);
 int convert_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t st);

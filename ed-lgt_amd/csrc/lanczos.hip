@@ -136,15 +136,57 @@ __global__ void __launch_bounds__(kTB) gram_finish(const double2* __restrict__ p
   if (lane == 0 && c < nw_real) out[(int64_t)(j0 + j) * nw_real + c] = s;
 }
 
-// W[:, c] = beta * W[:, c] + sum_j V_j C[j][c]    (C: nv x nw, row-major, complex)
+// Per-thread Gram accumulator of the NW values a thread writes for one row
+// (g[i][c] += conj(y_i) y_c), reduced over the block at the end (fixed wave
+// butterfly, waves in order) into partial[blk][i][c].
 template <class S, int NW>
+struct RowGram {
+  S g[NW][NW];
+  __device__ void clear() {
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+#pragma unroll
+      for (int c = 0; c < NW; ++c) g[i][c] = zero<S>();
+  }
+  __device__ void add_row(const S (&y)[NW]) {
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+#pragma unroll
+      for (int c = 0; c < NW; ++c) g[i][c] = add(g[i][c], cj_mul(y[i], y[c]));
+  }
+  __device__ void flush(double2* __restrict__ partial) {
+    __shared__ double2 red[kTB / 64][NW * NW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+#pragma unroll
+      for (int c = 0; c < NW; ++c) {
+        const S t = wave_sum(g[i][c]);
+        if (lane == 0) red[wave][i * NW + c] = to_c(t);
+      }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NW * NW; t += kTB) {
+      double2 s = red[0][t];
+#pragma unroll
+      for (int q = 1; q < kTB / 64; ++q) s = add(s, red[q][t]);
+      partial[(int64_t)blockIdx.x * NW * NW + t] = s;
+    }
+  }
+};
+
+// W[:, c] = beta * W[:, c] + sum_j V_j C[j][c]    (C: nv x nw, row-major, complex)
+// With kGram the Gram matrix of the new W columns is accumulated on the way
+// (partial[blk][i][c], NW x NW per block).
+template <class S, int NW, bool kGram>
 __global__ void __launch_bounds__(kTB) tall_update(const S* __restrict__ V, int64_t ldv, int nv,
                                                    const double2* __restrict__ C, int ldc,
                                                    S* __restrict__ W, int64_t ldw, int nw,
-                                                   int64_t n, int beta) {
+                                                   int64_t n, int beta, double2* __restrict__ partial) {
   extern __shared__ double2 cs[];
   for (int i = threadIdx.x; i < nv * nw; i += blockDim.x) cs[i] = C[(i / nw) * ldc + (i % nw)];
   __syncthreads();
+  RowGram<S, kGram ? NW : 1> rg;
+  if (kGram) rg.clear();
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
     S acc[NW];
@@ -159,7 +201,40 @@ __global__ void __launch_bounds__(kTB) tall_update(const S* __restrict__ V, int6
 #pragma unroll
     for (int c = 0; c < NW; ++c)
       if (c < nw) W[c * ldw + r] = acc[c];
+    if constexpr (kGram) rg.add_row(acc);
   }
+  if constexpr (kGram) rg.flush(partial);
+}
+
+// In place X[:, c] = sum_j X[:, j] C[j][c] for c, j < nw (each thread reads
+// its row's nw values before writing them); optionally accumulates the Gram
+// matrix of the new columns like tall_update.
+template <class S, int NW, bool kGram>
+__global__ void __launch_bounds__(kTB) row_transform(S* X, int64_t ldx, int nw, const double2* __restrict__ C,
+                                                     int64_t n, double2* __restrict__ partial) {
+  __shared__ double2 cs[NW * NW];
+  for (int i = threadIdx.x; i < nw * nw; i += blockDim.x) cs[i] = C[i];
+  __syncthreads();
+  RowGram<S, kGram ? NW : 1> rg;
+  if (kGram) rg.clear();
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    S x[NW], y[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) x[j] = j < nw ? X[j * ldx + r] : zero<S>();
+#pragma unroll
+    for (int c = 0; c < NW; ++c) {
+      y[c] = zero<S>();
+#pragma unroll
+      for (int j = 0; j < NW; ++j)
+        if (j < nw && c < nw) y[c] = add(y[c], mul_s(x[j], cs[j * nw + c]));
+    }
+#pragma unroll
+    for (int c = 0; c < NW; ++c)
+      if (c < nw) X[c * ldx + r] = y[c];
+    if constexpr (kGram) rg.add_row(y);
+  }
+  if constexpr (kGram) rg.flush(partial);
 }
 
 // fills x[i] with deterministic pseudo-random values in [-0.5, 0.5)
@@ -328,7 +403,7 @@ struct Solver {
   int setup() {
     EDLGT_TRY(V.alloc((size_t)(M + b) * N * sizeof(S)));
     if (need_w) EDLGT_TRY(W.alloc((size_t)b * N * sizeof(S)));
-    if (need_tmp) EDLGT_TRY(Tmp.alloc((size_t)std::max(M, 8) * N * sizeof(S)));
+    if (need_tmp) EDLGT_TRY(Tmp.alloc((size_t)std::max(M + b, 8) * N * sizeof(S)));
     EDLGT_TRY(part.alloc((size_t)kGramBlocks * kGramCols * 8 * sizeof(double2)));
     EDLGT_TRY(gdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
     EDLGT_TRY(cdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
@@ -369,14 +444,30 @@ struct Solver {
     return EDLGT_OK;
   }
 
-  // X[:, 0:nw] = beta X + Vp[:, 0:nv] C      (C: nv x nw)
-  int update(const S* Vp, int nv, const Dense& C, S* X, int nw, int beta) {
-    if (nw == 0) return EDLGT_OK;
-    if (nv == 0 && beta) return EDLGT_OK;
+  // Reads back a Gram matrix reduced by gram_finish into gdev.
+  int fetch_gram(int nw, Dense& G) {
+    std::vector<double2> hbuf((size_t)nw * nw);
+    EDLGT_HIP(hipMemcpyAsync(hbuf.data(), gdev.p, hbuf.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+    EDLGT_HIP(hipStreamSynchronize(st));
+    G = Dense(nw, nw);
+    for (int j = 0; j < nw; ++j)
+      for (int c = 0; c < nw; ++c) G(j, c) = cd(hbuf[(size_t)j * nw + c].x, hbuf[(size_t)j * nw + c].y);
+    return EDLGT_OK;
+  }
+
+  int upload_coef(const Dense& C, int nv, int nw) {
     std::vector<double2> hc((size_t)std::max(nv, 1) * nw);
     for (int j = 0; j < nv; ++j)
       for (int c = 0; c < nw; ++c) hc[(size_t)j * nw + c] = make_double2(C(j, c).real(), C(j, c).imag());
     EDLGT_HIP(hipMemcpyAsync(cdev.p, hc.data(), hc.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    return EDLGT_OK;
+  }
+
+  // X[:, 0:nw] = beta X + Vp[:, 0:nv] C      (C: nv x nw).  With gout the
+  // Gram matrix of the new X columns is computed in the same pass (nw <= 4).
+  int update(const S* Vp, int nv, const Dense& C, S* X, int nw, int beta, Dense* gout = nullptr) {
+    if (nw == 0) return EDLGT_OK;
+    if (nv == 0 && beta && !gout) return EDLGT_OK;
     const size_t shmem = (size_t)std::max(nv, 1) * nw * sizeof(double2);
     if (shmem > 60000) {
       // split the coefficient rows: X = beta X + V[:, :h] C[:h] + V[:, h:] C[h:]
@@ -384,22 +475,63 @@ struct Solver {
       Dense C1(half, nw), C2(nv - half, nw);
       for (int j = 0; j < nv; ++j)
         for (int c = 0; c < nw; ++c) (j < half ? C1(j, c) : C2(j - half, c)) = C(j, c);
-      EDLGT_HIP(hipStreamSynchronize(st));
       EDLGT_TRY(update(Vp, half, C1, X, nw, beta));
-      return update(Vp + (int64_t)half * N, nv - half, C2, X, nw, 1);
+      return update(Vp + (int64_t)half * N, nv - half, C2, X, nw, 1, gout);
     }
+    EDLGT_TRY(upload_coef(C, nv, nw));
     const unsigned g = grid_for(N, kTB, 4096);
-#define UPD_CASE(NWC)                                                                            \
-  case NWC:                                                                                      \
-    hipLaunchKernelGGL((tall_update<S, NWC>), dim3(g), dim3(kTB), shmem, st, Vp, N, nv,           \
-                       cdev.as<double2>(), nw, X, N, nw, N, beta);                                \
+    if (gout) {
+      if (nw > 4) return set_error(EDLGT_ERR_INVALID_ARG, "fused Gram needs width <= 4");
+      const int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : 4;
+#define UPDG_CASE(NWC)                                                                              \
+  case NWC:                                                                                         \
+    hipLaunchKernelGGL((tall_update<S, NWC, true>), dim3(g), dim3(kTB), shmem, st, Vp, N, nv,        \
+                       cdev.as<double2>(), nw, X, N, nw, N, beta, part.as<double2>());              \
+    hipLaunchKernelGGL((gram_finish<NWC>), dim3((NWC * NWC + 3) / 4), dim3(kTB), 0, st,              \
+                       part.as<double2>(), (int)g, NWC, nw, 0, gdev.as<double2>());                 \
     break;
-    if (nw > 8) return set_error(EDLGT_ERR_INVALID_ARG, "update width above 8");
-    int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : 8;
-    switch (nwp) { UPD_CASE(1) UPD_CASE(2) UPD_CASE(4) UPD_CASE(8) }
+      switch (nwp) { UPDG_CASE(1) UPDG_CASE(2) UPDG_CASE(4) }
+#undef UPDG_CASE
+      EDLGT_HIP(hipGetLastError());
+      return fetch_gram(nw, *gout);  // synchronises: cdev is free again
+    }
+#define UPD_CASE(NWC)                                                                             \
+  case NWC:                                                                                       \
+    hipLaunchKernelGGL((tall_update<S, NWC, false>), dim3(g), dim3(kTB), shmem, st, Vp, N, nv,     \
+                       cdev.as<double2>(), nw, X, N, nw, N, beta, (double2*)nullptr);             \
+    break;
+    if (nw > 16) return set_error(EDLGT_ERR_INVALID_ARG, "update width above 16");
+    int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : nw <= 8 ? 8 : 16;
+    switch (nwp) { UPD_CASE(1) UPD_CASE(2) UPD_CASE(4) UPD_CASE(8) UPD_CASE(16) }
 #undef UPD_CASE
     EDLGT_HIP(hipGetLastError());
     EDLGT_HIP(hipStreamSynchronize(st));  // cdev is reused by the next call
+    return EDLGT_OK;
+  }
+
+  // In place X = X C (C: nw x nw, nw <= 4); with gout also the new Gram matrix.
+  int transform(S* X, int nw, const Dense& C, Dense* gout) {
+    if (nw > 4) return set_error(EDLGT_ERR_INVALID_ARG, "transform width above 4");
+    EDLGT_TRY(upload_coef(C, nw, nw));
+    const unsigned g = grid_for(N, kTB, 4096);
+    const int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : 4;
+#define TRF_CASE(NWC)                                                                               \
+  case NWC:                                                                                         \
+    if (gout) {                                                                                     \
+      hipLaunchKernelGGL((row_transform<S, NWC, true>), dim3(g), dim3(kTB), 0, st, X, N, nw,         \
+                         cdev.as<double2>(), N, part.as<double2>());                                \
+      hipLaunchKernelGGL((gram_finish<NWC>), dim3((NWC * NWC + 3) / 4), dim3(kTB), 0, st,            \
+                         part.as<double2>(), (int)g, NWC, nw, 0, gdev.as<double2>());               \
+    } else {                                                                                        \
+      hipLaunchKernelGGL((row_transform<S, NWC, false>), dim3(g), dim3(kTB), 0, st, X, N, nw,        \
+                         cdev.as<double2>(), N, (double2*)nullptr);                                 \
+    }                                                                                               \
+    break;
+    switch (nwp) { TRF_CASE(1) TRF_CASE(2) TRF_CASE(4) }
+#undef TRF_CASE
+    EDLGT_HIP(hipGetLastError());
+    if (gout) return fetch_gram(nw, *gout);
+    EDLGT_HIP(hipStreamSynchronize(st));
     return EDLGT_OK;
   }
 
@@ -429,40 +561,58 @@ struct Solver {
   // that were (numerically) dependent are replaced by random directions and
   // get zero rows in Rout.  Gout receives the V-coefficients.
   int orthonormalize(int nv, S* X, int nw, Dense& Gout, Dense& Rout) {
-    Dense G0;
-    EDLGT_TRY(gram(X, nw, X, nw, G0));
     std::vector<double> n0(nw);
-    for (int c = 0; c < nw; ++c) n0[c] = std::max(G0(c, c).real(), 0.0);
-    Dense G1, G2;
+    Dense Gq;  // Gram matrix of X after the Gram-Schmidt passes
     Gout = Dense(nv, nw);
-    for (int pass = 0; pass < 2 && nv > 0; ++pass) {
-      Dense G;
-      EDLGT_TRY(gram(V.as<S>(), nv, X, nw, G));
+    if (nv == 0) {
+      EDLGT_TRY(gram(X, nw, X, nw, Gq));
+      for (int c = 0; c < nw; ++c) n0[c] = std::max(Gq(c, c).real(), 0.0);
+    } else {
+      // pass 1: when X directly follows V, one Gram over [V | X] gives both
+      // V^H X and X^H X
+      Dense G(nv, nw), G0(nw, nw);
+      if (X == V.as<S>() + (int64_t)nv * N) {
+        Dense GG;
+        EDLGT_TRY(gram(V.as<S>(), nv + nw, X, nw, GG));
+        for (int c = 0; c < nw; ++c) {
+          for (int j = 0; j < nv; ++j) G(j, c) = GG(j, c);
+          for (int i = 0; i < nw; ++i) G0(i, c) = GG(nv + i, c);
+        }
+      } else {
+        EDLGT_TRY(gram(X, nw, X, nw, G0));
+        EDLGT_TRY(gram(V.as<S>(), nv, X, nw, G));
+      }
+      for (int c = 0; c < nw; ++c) n0[c] = std::max(G0(c, c).real(), 0.0);
       Dense mG = G;
       for (auto& z : mG.a) z = -z;
       EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1));
+      Gout = G;
+      // pass 2, with the Gram matrix of the result formed in the same sweep
+      EDLGT_TRY(gram(V.as<S>(), nv, X, nw, G));
+      mG = G;
+      for (auto& z : mG.a) z = -z;
+      if (nw <= 4) {
+        EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1, &Gq));
+      } else {
+        EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1));
+        EDLGT_TRY(gram(X, nw, X, nw, Gq));
+      }
       for (size_t i = 0; i < G.a.size(); ++i) Gout.a[i] += G.a[i];
     }
-    // Cholesky QR, twice.
+    // Cholesky QR, twice (in place: X <- X R^-1).
     std::vector<double> floor2(nw);
     for (int c = 0; c < nw; ++c) floor2[c] = 1e-20 * std::max(n0[c], 1e-280);
-    Dense Gq, R1;
+    Dense R1;
     Dense Rpre(nw, nw);
     for (int c = 0; c < nw; ++c) Rpre(c, c) = 1.0;
-    EDLGT_TRY(gram(X, nw, X, nw, Gq));
-    if (cholesky_upper(Gq, R1, floor2)) {
-      Dense R1i = upper_inverse(R1);
-      EDLGT_TRY(update(X, nw, R1i, Tmp.as<S>(), nw, 0));
-      EDLGT_HIP(hipMemcpyAsync(X, Tmp.p, (size_t)nw * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+    if (nw <= 4 && cholesky_upper(Gq, R1, floor2)) {
+      Dense Gq2, R2;
+      EDLGT_TRY(transform(X, nw, upper_inverse(R1), &Gq2));
       Rpre = R1;
       for (int c = 0; c < nw; ++c) n0[c] = 1.0;  // X now has unit-scale columns
-      Dense Gq2, R2;
       std::vector<double> fl2(nw, 0.25);
-      EDLGT_TRY(gram(X, nw, X, nw, Gq2));
       if (cholesky_upper(Gq2, R2, fl2)) {
-        Dense R2i = upper_inverse(R2);
-        EDLGT_TRY(update(X, nw, R2i, Tmp.as<S>(), nw, 0));
-        EDLGT_HIP(hipMemcpyAsync(X, Tmp.p, (size_t)nw * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+        EDLGT_TRY(transform(X, nw, upper_inverse(R2), nullptr));
         Rout = matmul(R2, R1);
         return EDLGT_OK;
       }
@@ -486,8 +636,7 @@ struct Solver {
       if (nrm > 1e-10 * std::sqrt(std::max(n0[c], 1e-280))) {
         Dense sc(1, 1);
         sc(0, 0) = 1.0 / nrm;
-        EDLGT_TRY(update(xc, 1, sc, Tmp.as<S>(), 1, 0));
-        EDLGT_HIP(hipMemcpyAsync(xc, Tmp.p, (size_t)N * sizeof(S), hipMemcpyDeviceToDevice, st));
+        EDLGT_TRY(transform(xc, 1, sc, nullptr));
         Rout(c, c) = nrm;
       } else {
         if (nv + c + 1 > N) return set_error(EDLGT_ERR_INVALID_ARG, "basis exceeds the space");
@@ -508,8 +657,7 @@ struct Solver {
         EDLGT_TRY(gram(xc, 1, xc, 1, nn));
         Dense sc(1, 1);
         sc(0, 0) = 1.0 / std::sqrt(nn(0, 0).real());
-        EDLGT_TRY(update(xc, 1, sc, Tmp.as<S>(), 1, 0));
-        EDLGT_HIP(hipMemcpyAsync(xc, Tmp.p, (size_t)N * sizeof(S), hipMemcpyDeviceToDevice, st));
+        EDLGT_TRY(transform(xc, 1, sc, nullptr));
         Rout(c, c) = 0.0;
       }
     }
@@ -634,9 +782,13 @@ struct Solver {
     Dense Sl(nv, l);
     for (int j = 0; j < nv; ++j)
       for (int i = 0; i < l; ++i) Sl(j, i) = Sv(j, i);
+    // Ritz vectors into Tmp[:, 0:l], the residual block after them, then Tmp
+    // becomes the basis (both buffers hold M + b columns)
     EDLGT_TRY(update_blocked(col(0), nv, Sl, Tmp.as<S>(), l));
-    EDLGT_HIP(hipMemcpyAsync(col(0), Tmp.p, (size_t)l * N * sizeof(S), hipMemcpyDeviceToDevice, st));
-    EDLGT_HIP(hipMemcpyAsync(col(l), col(nv), (size_t)b * N * sizeof(S), hipMemcpyDeviceToDevice, st));
+    EDLGT_HIP(hipMemcpyAsync(Tmp.as<S>() + (int64_t)l * N, col(nv), (size_t)b * N * sizeof(S),
+                             hipMemcpyDeviceToDevice, st));
+    std::swap(V.p, Tmp.p);
+    std::swap(V.bytes, Tmp.bytes);
     for (auto& z : T.a) z = 0.0;
     for (int i = 0; i < l; ++i) T(i, i) = w[i];
     for (int c = 0; c < b; ++c)
@@ -665,10 +817,10 @@ struct Solver {
     }
   }
 
-  // X[:, 0:nc] = V[:, 0:nv] C for any nc (chunks of 8 columns).
+  // X[:, 0:nc] = V[:, 0:nv] C for any nc (chunks of 16 columns).
   int update_blocked(const S* Vp, int nv, const Dense& C, S* X, int nc) {
-    for (int c0 = 0; c0 < nc; c0 += 8) {
-      const int w = std::min(8, nc - c0);
+    for (int c0 = 0; c0 < nc; c0 += 16) {
+      const int w = std::min(16, nc - c0);
       Dense Cc(nv, w);
       for (int j = 0; j < nv; ++j)
         for (int c = 0; c < w; ++c) Cc(j, c) = C(j, c0 + c);
@@ -948,7 +1100,7 @@ struct Server {
 
   size_t slot_bytes() const {
     const size_t N = (size_t)h.n_rows;
-    return ((size_t)(M + b) + std::max(M, 8)) * N * sizeof(S) +
+    return ((size_t)(M + b) + std::max(M + b, 8)) * N * sizeof(S) +
            (size_t)kGramBlocks * kGramCols * 8 * sizeof(double2) +
            2 * (size_t)(M + b) * std::max(M, 8) * sizeof(double2);
   }
@@ -1070,7 +1222,8 @@ struct Server {
         sl.sv->mv64 += 1;
       }
     }
-    if (!pack.p) EDLGT_TRY(pack.alloc(coef_pack_bytes(h, vec_type, kCoefMaxRhs)));
+    const size_t need = coef_pack_bytes(h, vec_type, kCoefMaxRhs);
+    if (pack.bytes < need) EDLGT_TRY(pack.alloc(need));  // also after the matrix was coded
     hc.resize((hx.size() + kCoefMaxRhs) * 16, 0.0);
     size_t done = 0;
     while (done < hx.size()) {
@@ -1079,7 +1232,8 @@ struct Server {
       EDLGT_HIP(hipMemcpyAsync(ys.p, hy.data() + done, nb * sizeof(void*), hipMemcpyHostToDevice, st));
       EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data() + done * 16, kCoefMaxRhs * 16 * sizeof(double),
                                hipMemcpyHostToDevice, st));
-      EDLGT_TRY(csr_spmm_coef(h, vec_type, nb, xs.p, ys.p, cf.as<double>(), pack.p, st));
+      EDLGT_TRY(csr_spmm_coef(h, vec_type, nb, xs.p, ys.p, hx.data() + done, hy.data() + done, cf.as<double>(),
+                              pack.p, st));
       EDLGT_HIP(hipStreamSynchronize(st));  // the pointer/coefficient buffers are reused
       done += nb;
     }

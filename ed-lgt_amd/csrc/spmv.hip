@@ -255,6 +255,76 @@ __global__ void stream_row_blocks(const int64_t* __restrict__ row_ptr, int64_t n
   }
 }
 
+// ---- dictionary-coded values (csr_compress): entry p carries code[p] and
+// its value is tab[code[p]], a table already multiplied by the product's term
+// coefficients.  Same loop structure and summation order as the kernels
+// above; the streamed bytes per entry drop to 4 (col) + 1 or 2 (code).
+constexpr int kTabLds = 1024;      // tables up to this many entries sit in LDS
+constexpr int kRowsTabLds = 2048;  // NB x n_tab bound for the multi-RHS kernel
+
+template <class TT, class CT, class XT, class AT, bool kLds>
+__global__ void __launch_bounds__(kBS) csr_stream_coded(
+    int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const CT* __restrict__ code, const TT* __restrict__ tab, int n_tab, const XT* __restrict__ x,
+    AT* __restrict__ y, const int64_t* __restrict__ row_blocks, const int64_t* __restrict__ block_pos) {
+  __shared__ AT prod[kCap];
+  __shared__ TT stab[kLds ? kTabLds : 1];
+  if (kLds) {
+    for (int i = threadIdx.x; i < n_tab; i += kBS) stab[i] = tab[i];
+    __syncthreads();
+  }
+  const TT* tb = kLds ? stab : tab;
+  const int64_t r0 = row_blocks[blockIdx.x], r1 = row_blocks[blockIdx.x + 1];
+  if (r0 >= r1) return;
+  const int64_t p0 = block_pos[blockIdx.x], p1 = block_pos[blockIdx.x + 1];
+  const int64_t cnt = p1 - p0;
+  const int64_t nr = r1 - r0;
+  const int tid = threadIdx.x;
+  if (cnt <= kCap) {
+    for (int64_t base = 0; base < cnt; base += kBS * kUnroll) {
+      int32_t c[kUnroll];
+      int k[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t i = base + u * kBS + tid;
+        if (i < cnt) {
+          c[u] = ldnt(col + p0 + i);
+          k[u] = (int)ldnt(code + p0 + i);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t i = base + u * kBS + tid;
+        if (i < cnt) {
+          AT a = zero_acc<AT>();
+          fma_acc(a, tb[k[u]], load_x(x, c[u]));
+          prod[i] = a;
+        }
+      }
+    }
+    __syncthreads();
+    if (nr <= 64) reduce_rows<8>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= 128) reduce_rows<4>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= 256) reduce_rows<2>(prod, row_ptr, r0, nr, p0, y);
+    else reduce_rows<1>(prod, row_ptr, r0, nr, p0, y);
+  } else {
+    AT* red = prod;
+    for (int64_t r = r0; r < r1; ++r) {
+      AT s = zero_acc<AT>();
+      for (int64_t p = row_ptr[r] + tid; p < row_ptr[r + 1]; p += kBS)
+        fma_acc(s, tb[(int)ldnt(code + p)], load_x(x, ldnt(col + p)));
+      red[tid] = s;
+      __syncthreads();
+      for (int o = kBS / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = red[tid] + red[tid + o];
+        __syncthreads();
+      }
+      if (tid == 0) y[r] = red[0];
+      __syncthreads();
+    }
+  }
+}
+
 template <class VT, class XT, class AT, int NB>
 int launch_g(int g, int64_t grid, int64_t n, const int64_t* rp, const int32_t* col, const VT* val,
              const XT* x, int64_t ldx, AT* y, int64_t ldy, hipStream_t st) {
@@ -360,8 +430,51 @@ int spmv_stream(const Csr& h, int precision, int xtype, const void* xb, void* yb
 }
 
 
+template <class TT, class XT, class AT>
+int launch_stream_coded(const Csr& h, const TT* tab, const XT* x, AT* y, hipStream_t st) {
+  const bool lds = h.n_dict <= kTabLds;
+#define EDLGT_SC(CT, LDS)                                                                          \
+  hipLaunchKernelGGL((csr_stream_coded<TT, CT, XT, AT, LDS>), dim3((unsigned)h.n_stream_blocks),    \
+                     dim3(kBS), 0, st, h.n_rows, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(),       \
+                     h.code.as<CT>(), tab, h.n_dict, x, y, h.stream_blocks.as<int64_t>(),           \
+                     h.stream_pos.as<int64_t>())
+  if (h.code_bytes == 1) {
+    if (lds) EDLGT_SC(uint8_t, true);
+    else EDLGT_SC(uint8_t, false);
+  } else {
+    if (lds) EDLGT_SC(uint16_t, true);
+    else EDLGT_SC(uint16_t, false);
+  }
+#undef EDLGT_SC
+  EDLGT_HIP(hipGetLastError());
+  return EDLGT_OK;
+}
+
+// One right-hand side of a coded matrix; tab: scaled table (double for a real
+// matrix, double2 for a complex one).
+int spmv_coded(const Csr& h, const void* tab, int xtype, const void* xb, void* yb, hipStream_t st) {
+  if (h.value_type == EDLGT_VALUE_C128) {
+    if (xtype != EDLGT_VALUE_C128) return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
+    return launch_stream_coded(h, (const double2*)tab, (const double2*)xb, (double2*)yb, st);
+  }
+  if (xtype == EDLGT_VALUE_F64)
+    return launch_stream_coded(h, (const double*)tab, (const double*)xb, (double*)yb, st);
+  if (xtype == EDLGT_VALUE_C128)
+    return launch_stream_coded(h, (const double*)tab, (const double2*)xb, (double2*)yb, st);
+  return set_error(EDLGT_ERR_INVALID_ARG, "unsupported vector type for a coded matrix");
+}
+
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx, void* y,
              int64_t ldy, hipStream_t st) {
+  if (h.n_dict > 0 && precision == EDLGT_PREC_FP64 && h.stream_blocks.p) {
+    // coded values: one stream pass per right-hand side (all term coefficients 1)
+    const size_t xe = xtype == EDLGT_VALUE_C128 ? 16 : 8;
+    const size_t ye = h.value_type == EDLGT_VALUE_C128 || xtype == EDLGT_VALUE_C128 ? 16 : 8;
+    for (int j = 0; j < n_rhs; ++j)
+      EDLGT_TRY(spmv_coded(h, h.tab_unit.p, xtype, static_cast<const char*>(x) + (size_t)j * ldx * xe,
+                           static_cast<char*>(y) + (size_t)j * ldy * ye, st));
+    return EDLGT_OK;
+  }
   if (n_rhs == 1 && h.g_override <= 0 && h.stream_blocks.p) {
     if (precision == EDLGT_PREC_MIXED && !h.val_lo.p)
       return set_error(EDLGT_ERR_INVALID_ARG, "low-precision values not built");
@@ -520,6 +633,89 @@ __global__ void __launch_bounds__(256) csr_spmm_coef_rows(
   }
 }
 
+// Per-right-hand-side tables of a coded matrix: out[b][j] = coef[b][term_j] * value_j.
+__device__ inline void set_tab(double& o, double c, double2 v) { o = c * v.x; }
+__device__ inline void set_tab(double2& o, double c, double2 v) { o = scale(c, v); }
+
+template <class TT>
+__global__ void scale_dict(const double2* __restrict__ dval, const int32_t* __restrict__ dterm, int n_dict,
+                           const double* __restrict__ coef, int nb, TT* __restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb * n_dict; i += gridDim.x * blockDim.x) {
+    const int b = i / n_dict, j = i % n_dict;
+    set_tab(out[i], coef[b * kMaxTerms + dterm[j]], dval[j]);
+  }
+}
+
+template <class TT, class CT, class S, int G, int NB, bool kLds>
+__global__ void __launch_bounds__(256) csr_rows_coded(
+    int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const CT* __restrict__ code, const TT* __restrict__ tab, int n_tab, const S* __restrict__ xi,
+    S* const* __restrict__ ys, int nb) {
+  __shared__ TT stab[kLds ? kRowsTabLds : 1];
+  if (kLds) {
+    for (int i = threadIdx.x; i < nb * n_tab; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+  }
+  const TT* tb = kLds ? stab : tab;
+  const int sub = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (256 / G);
+  for (int64_t r = (int64_t)blockIdx.x * (256 / G) + (threadIdx.x / G); r < n; r += stride) {
+    const int64_t beg = row_ptr[r], end = row_ptr[r + 1];
+    S acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = zero_acc<S>();
+    for (int64_t p = beg + sub; p < end; p += G) {
+      const int32_t c = ldnt(col + p);
+      const int k = (int)ldnt(code + p);
+      S x[NB];
+      load_row<NB>(xi + (int64_t)c * NB, x);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) fma_acc(acc[b], tb[b * n_tab + k], x[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const S s = group_sum<G>(acc[b]);
+      if (sub == 0 && b < nb) ys[b][r] = s;
+    }
+  }
+}
+
+template <class TT, class S, int G>
+int launch_rows_coded(int nb, int nbp, const Csr& h, const TT* tab, const void* xs, void* ys, void* pack,
+                      hipStream_t st) {
+  const int64_t n = h.n_rows;
+  const unsigned pgrid = grid_for(n, 256, 65536);
+  const int64_t blocks = std::min<int64_t>(65536, std::max<int64_t>(1, (n + 256 / G - 1) / (256 / G)));
+#define EDLGT_RC(CT, NBC, LDS)                                                                      \
+  hipLaunchKernelGGL((csr_rows_coded<TT, CT, S, G, NBC, LDS>), dim3((unsigned)blocks), dim3(256), 0, st, \
+                     n, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), h.code.as<CT>(), tab, h.n_dict,  \
+                     (const S*)pack, (S* const*)ys, nb)
+#define EDLGT_RC_NB(NBC)                                                                            \
+  case NBC:                                                                                         \
+    hipLaunchKernelGGL((pack_rows<S, NBC>), dim3(pgrid), dim3(256), 0, st, (const S* const*)xs, nb, n, \
+                       (S*)pack);                                                                   \
+    if (h.code_bytes == 1) {                                                                        \
+      if (lds) EDLGT_RC(uint8_t, NBC, true); else EDLGT_RC(uint8_t, NBC, false);                    \
+    } else {                                                                                        \
+      if (lds) EDLGT_RC(uint16_t, NBC, true); else EDLGT_RC(uint16_t, NBC, false);                  \
+    }                                                                                               \
+    break;
+  const bool lds = nbp * h.n_dict <= kRowsTabLds;
+  switch (nbp) {
+    EDLGT_RC_NB(1)
+    EDLGT_RC_NB(2)
+    EDLGT_RC_NB(4)
+    EDLGT_RC_NB(8)
+    EDLGT_RC_NB(16)
+    default:
+      return set_error(EDLGT_ERR_INVALID_ARG, "unsupported batch width");
+  }
+#undef EDLGT_RC_NB
+#undef EDLGT_RC
+  EDLGT_HIP(hipGetLastError());
+  return EDLGT_OK;
+}
+
 template <class VT, class S, int G>
 int launch_coef_nb(int nb, int nbp, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
                    const double* coef, void* pack, hipStream_t st) {
@@ -569,26 +765,73 @@ int coef_lanes(const Csr& h) {
 
 int coef_pad(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : 16; }
 
-size_t coef_pack_bytes(const Csr& h, int vec_type, int nb) {
-  return (size_t)h.n_rows * coef_pad(nb) * (vec_type == EDLGT_VALUE_C128 ? 16 : 8);
+// Whether the multi-RHS kernel (row-interleaved right-hand sides) is used:
+// always for uncoded matrices; for coded ones only while a full 16-wide packed
+// block stays within a cache-sized footprint, otherwise every right-hand side
+// takes its own stream pass (the gathered vector then stays in the MALL).
+// The choice depends on the matrix alone, so a column's arithmetic does not
+// depend on the batch it is in.
+constexpr size_t kFuseBytes = size_t(64) << 20;
+bool coef_fused(const Csr& h, int vec_type) {
+  if (h.n_dict == 0) return true;
+  const size_t es = vec_type == EDLGT_VALUE_C128 ? 16 : 8;
+  return (size_t)h.n_rows * es * kCoefMaxRhs <= kFuseBytes;
 }
 
-// xs / ys: device arrays of nb device pointers; coef: device [nb][16] doubles
-// (padded to the kernel width: unused rows may hold anything finite); pack:
-// device workspace of coef_pack_bytes(h, vec_type, nb).  1 <= nb <= 16.
-// Vectors: complex128 for a complex matrix, float64 (F64) or complex128 (C128)
-// for a real matrix.  The padded right-hand sides read zeros and are not
-// written.
-int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const double* coef,
-                  void* pack, hipStream_t st) {
+static size_t coef_x_bytes(const Csr& h, int vec_type, int nb) {
+  if (!coef_fused(h, vec_type)) return 0;
+  const size_t b = (size_t)h.n_rows * coef_pad(nb) * (vec_type == EDLGT_VALUE_C128 ? 16 : 8);
+  return (b + 255) / 256 * 256;
+}
+
+size_t coef_pack_bytes(const Csr& h, int vec_type, int nb) {
+  return coef_x_bytes(h, vec_type, nb) + (h.n_dict > 0 ? (size_t)kCoefMaxRhs * h.n_dict * 16 : 0) + 256;
+}
+
+// xs / ys: device arrays of nb device pointers (hx / hy: the same pointers on
+// the host); coef: device [nb][16] doubles (padded to the kernel width: unused
+// rows may hold anything); pack: device workspace of coef_pack_bytes(h,
+// vec_type, nb).  1 <= nb <= 16.  Vectors: complex128 for a complex matrix,
+// float64 (F64) or complex128 (C128) for a real matrix.
+int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, const void* const* hx,
+                  void* const* hy, const double* coef, void* pack, hipStream_t st) {
   if (nb < 1 || nb > kCoefMaxRhs) return set_error(EDLGT_ERR_INVALID_ARG, "batch width must be in [1, 16]");
   const int g = coef_lanes(h);
   const int nbp = coef_pad(nb);
   const int64_t n = h.n_rows;
-  if (h.value_type == EDLGT_VALUE_C128) {
-    if (vec_type != EDLGT_VALUE_C128) return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
-    return launch_coef<double2, double2>(g, nb, nbp, n, h, h.val.as<double2>(), xs, ys, coef, pack, st);
+  const bool cplx = h.value_type == EDLGT_VALUE_C128;
+  if (cplx && vec_type != EDLGT_VALUE_C128)
+    return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
+  if (h.n_dict > 0) {
+    char* tab = static_cast<char*>(pack) + coef_x_bytes(h, vec_type, nb);
+    const unsigned tg = grid_for((int64_t)nb * h.n_dict, 256, 1024);
+    if (cplx)
+      hipLaunchKernelGGL(scale_dict<double2>, dim3(tg), dim3(256), 0, st, h.dict_val.as<double2>(),
+                         h.dict_term.as<int32_t>(), h.n_dict, coef, nb, (double2*)tab);
+    else
+      hipLaunchKernelGGL(scale_dict<double>, dim3(tg), dim3(256), 0, st, h.dict_val.as<double2>(),
+                         h.dict_term.as<int32_t>(), h.n_dict, coef, nb, (double*)tab);
+    EDLGT_HIP(hipGetLastError());
+    if (!coef_fused(h, vec_type)) {
+      const size_t te = cplx ? 16 : 8;
+      for (int b = 0; b < nb; ++b)
+        EDLGT_TRY(spmv_coded(h, tab + (size_t)b * h.n_dict * te, vec_type, hx[b], hy[b], st));
+      return EDLGT_OK;
+    }
+#define EDLGT_RCG(TT, S)                                                                          \
+  switch (g) {                                                                                    \
+    case 4: return launch_rows_coded<TT, S, 4>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);      \
+    case 8: return launch_rows_coded<TT, S, 8>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);      \
+    case 16: return launch_rows_coded<TT, S, 16>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);    \
+    case 32: return launch_rows_coded<TT, S, 32>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);    \
+  }                                                                                               \
+  return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");
+    if (cplx) { EDLGT_RCG(double2, double2) }
+    if (vec_type == EDLGT_VALUE_F64) { EDLGT_RCG(double, double) }
+    EDLGT_RCG(double, double2)
+#undef EDLGT_RCG
   }
+  if (cplx) return launch_coef<double2, double2>(g, nb, nbp, n, h, h.val.as<double2>(), xs, ys, coef, pack, st);
   if (vec_type == EDLGT_VALUE_F64)
     return launch_coef<double, double>(g, nb, nbp, n, h, h.val.as<double>(), xs, ys, coef, pack, st);
   return launch_coef<double, double2>(g, nb, nbp, n, h, h.val.as<double>(), xs, ys, coef, pack, st);

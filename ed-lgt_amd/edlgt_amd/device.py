@@ -210,6 +210,14 @@ class CSRMatrix:
         L.check(L.load().edlgt_csr_copy(self._h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
         return rp, col, val
 
+    def compress(self, stream=None):
+        """Dictionary-codes the values (edlgt_csr_compress); returns the number
+        of distinct (value, term) pairs, 0 when the matrix stays uncoded."""
+        nd, cb = C.c_int32(), C.c_int32()
+        L.check(L.load().edlgt_csr_compress(self._h, L.stream_handle(stream), C.byref(nd), C.byref(cb)))
+        self.n_dict, self.code_bytes = int(nd.value), int(cb.value)
+        return self.n_dict
+
     def terms(self):
         """Per-entry term ids (uint8, nnz); all zero for a single-term matrix."""
         t = np.zeros(self.nnz, dtype=np.uint8)

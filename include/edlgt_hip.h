@@ -166,6 +166,15 @@ int edlgt_hamiltonian_build_terms(const edlgt_basis* b, int32_t d_max, int32_t n
 /* Term ids (host uint8 [nnz], only when n_terms > 1) and the term count. */
 int edlgt_csr_copy_terms(const edlgt_csr* h, uint8_t* term_host, int32_t* n_terms);
 int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* value_type);
+/* Dictionary coding of the values: when the matrix has at most 4096 distinct
+ * (value, term) pairs (compared bitwise), every entry gets a uint8 (<= 256)
+ * or uint16 code into a small table, and from then on every FP64 product
+ * (edlgt_spmv / edlgt_spmm / edlgt_spmm_coef, the eigensolvers, the servers)
+ * streams col + code (5-6 bytes per entry instead of 12-20) and multiplies by
+ * exactly the stored values.  *n_dict = 0: too many distinct values, the
+ * matrix stays as it was.  FP64 matrices only.  No reference counterpart
+ * (the reference stores complex128 triplets / CSC). */
+int edlgt_csr_compress(edlgt_csr* h, void* stream, int32_t* n_dict, int32_t* code_bytes);
 /* Copy CSR arrays to host: row_ptr int64 [n+1], col int32 [nnz], val [nnz]. */
 int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val);
 int edlgt_csr_device_ptrs(const edlgt_csr* h, void** row_ptr, void** col, void** val);
