@@ -8,9 +8,9 @@ from . import _lib
 from ._lib import EdlgtError, NotConvergedError, PREC_FP64, PREC_MIXED
 from .device import CSRMatrix, SectorBasis, Server
 from .lattice import Lattice
-from .models import QED_Model, QMB_hamiltonian, QMB_state, QuantumModel, Zn_Model
+from .models import QED_Model, QMB_hamiltonian, QMB_state, QuantumModel, SU2_Model, Zn_Model
 
 __all__ = [
-    "QED_Model", "Zn_Model", "QuantumModel", "QMB_hamiltonian", "QMB_state", "SectorBasis", "CSRMatrix", "Server",
+    "QED_Model", "SU2_Model", "Zn_Model", "QuantumModel", "QMB_hamiltonian", "QMB_state", "SectorBasis", "CSRMatrix", "Server",
     "Lattice", "EdlgtError", "NotConvergedError", "PREC_FP64", "PREC_MIXED",
 ]

@@ -18,7 +18,7 @@ from . import operators as opx
 
 logger = logging.getLogger(__name__)
 
-__all__ = ["QuantumModel", "QED_Model", "Zn_Model", "QMB_hamiltonian", "QMB_state"]
+__all__ = ["QuantumModel", "QED_Model", "SU2_Model", "Zn_Model", "QMB_hamiltonian", "QMB_state"]
 
 
 class QMB_state:
@@ -40,6 +40,20 @@ class QMB_state:
     def get_norm(self):
         import torch
         return float(torch.linalg.vector_norm(self.psi_device).item())
+
+    def entanglement_entropy(self, keep_indices, partitions_dict):
+        """Von Neumann entropy (base 2, Schmidt weights > 1e-10) of the sites
+        keep_indices (qmb_state.py:245-291), from the subsystem/environment
+        maps of QuantumModel._get_partition.  The state is copied to the host
+        and the Schmidt values come from a dense SVD (small systems)."""
+        part = partitions_dict[tuple(sorted(keep_indices))]
+        M = np.zeros((part["subsys_dim"], part["env_dim"]), dtype=np.complex128)
+        M[part["subsys_map"], part["env_map"]] = self.psi
+        if min(M.shape) == 1:
+            return 0.0
+        lam = np.linalg.svd(M, compute_uv=False) ** 2
+        lam = lam[lam > 1e-10]
+        return float(-np.sum(lam * np.log2(lam)))
 
 
 class QMB_hamiltonian:
@@ -294,6 +308,22 @@ class QuantumModel:
         even = self.lattice.even_mask()
         return float(np.mean(v[even if staggered_avg == "even" else ~even]))
 
+    def _get_partition(self, keep_indices):
+        """Subsystem / environment configuration maps of the sector basis
+        (quantum_model.py:715-791), cached by the kept site set."""
+        if not hasattr(self, "_partition_cache"):
+            self._partition_cache = {}
+        key = tuple(sorted(keep_indices))
+        if key not in self._partition_cache:
+            cfg = self.sector_configs
+            env = [i for i in range(self.n_sites) if i not in key]
+            _, sub_map = np.unique(cfg[:, list(key)], axis=0, return_inverse=True)
+            _, env_map = np.unique(cfg[:, env], axis=0, return_inverse=True)
+            sub_map, env_map = sub_map.ravel(), env_map.ravel()
+            self._partition_cache[key] = dict(subsys_map=sub_map, env_map=env_map,
+                                              subsys_dim=int(sub_map.max()) + 1, env_dim=int(env_map.max()) + 1)
+        return self._partition_cache[key]
+
     def get_qmb_state_from_configs(self, configs):
         """Equal-weight normalised superposition of basis configurations
         (quantum_model.py:826-862); raises ValueError for a config outside
@@ -471,3 +501,123 @@ class Zn_Model(QuantumModel):
         re-assembly.  Returns the CSRMatrix (coefficients [J, h])."""
         tables, placements = self.multi_term_program()
         return CSRMatrix(self.basis, tables, placements, L.VALUE_F64, n_terms=2)
+
+
+class SU2_Model(QuantumModel):
+    """SU(2) lattice gauge theory with spin-1/2 rishons and optional two-colour
+    staggered matter (edlgt/models/SU2_model.py:28-355, 486-624), 1D and 2D.
+
+    Sector: global N_tot = sectors[0] with matter, link constraints
+    T2_p{d}(i) = T2_m{d}(i+d), and for the pure theory on periodic 2D lattices
+    the parity of the flux through the origin lines fixed to +1."""
+
+    def __init__(self, spin, pure_theory, bg_list=None, sectors=None, use_generic_model=False, **kwargs):
+        super().__init__(**kwargs)
+        if float(spin) != 0.5:
+            raise NotImplementedError("the device engine builds the spin-1/2 SU(2) model")
+        if bg_list is not None and any(b != 0 for b in bg_list):
+            raise NotImplementedError("background charges are not supported by the device engine")
+        if use_generic_model:
+            raise NotImplementedError("the generalised SU(2) operator set is not on the device path")
+        if self.dim > 2:
+            raise NotImplementedError("SU(2) models are built in 1D and 2D")
+        from .su2_operators import SU2Site
+        self.spin = 0.5
+        self.pure_theory = bool(pure_theory)
+        self.use_generic_model = False
+        self.background = 0
+        site = SU2Site(self.pure_theory, self.dim)
+        self._site = site
+        labels = [self.lattice.label(i, False) for i in range(self.n_sites)]
+        self._project(site, site.operators, labels)
+
+        def diag(name):
+            return np.real(np.einsum("sii->si", self.ops[name]))
+
+        glob = None
+        if not self.pure_theory:
+            if sectors is None:
+                raise ValueError("sectors=[N_tot] is required with matter fields")
+            glob = (diag("N_tot")[None], np.array([float(sectors[0])]), "U")
+            self.zero_density = sectors[0] == self.n_sites
+        link = (np.stack([np.stack([diag(f"T2_p{d}"), -diag(f"T2_m{d}")]) for d in self.directions]),
+                np.zeros(self.dim), self.lattice.link_pairs())
+        nbody = None
+        if self.pure_theory and not any(self.has_obc) and self.dim == 2:
+            line_of = {"x": "y", "y": "x"}
+            nbody = (np.stack([diag(f"P_p{d}") for d in self.directions]), np.ones(self.dim),
+                     [self.lattice.origin_line(line_of[d]) for d in self.directions], "Z")
+        self.basis = SectorBasis(self.loc_dims, glob=glob, link=link, nbody=nbody)
+        if self.basis.n_states == 0:
+            raise ValueError("No configurations found for the given symmetry sectors")
+        logger.info(f"SEC DIM: {self.basis.n_states}")
+        self.default_params()
+
+    def SU2_Hamiltonian_couplings(self, g, m=None, theta=0.0):
+        """SU2_model.py:555-624 (hopping rescaled to 2 sqrt(2))."""
+        if self.dim == 1:
+            E, B = 8 * g / 3, 0
+        else:
+            E, B = g / 2, -1 / (2 * g)
+        c = {"g": g, "E": E, "B": B, "theta": -theta * g}
+        if not self.pure_theory and m is not None:
+            t = 2 * np.sqrt(2)
+            c.update({"tx_even": -complex(0, t), "tx_odd": -complex(0, t), "ty_even": -t, "ty_odd": t,
+                      "tz_even": -t, "tz_odd": t, "m": m, "m_odd": -m, "m_even": m})
+        self.coeffs = c
+
+    def build_Hamiltonian(self, g, m=None, theta=0.0, lambda_noise=0.0, dtype_mode="auto"):
+        """build_base_Hamiltonian term order (SU2_model.py:200-355)."""
+        mode = dtype_mode
+        if isinstance(mode, (bool, np.bool_)):
+            mode = "complex" if mode else "real"
+        if mode in (None, "auto"):
+            real = self.pure_theory and abs(theta) <= 1e-12
+        elif mode in ("real", "complex"):
+            real = mode == "real"
+        else:
+            raise ValueError(f"dtype_mode must be one of ['auto', 'complex', 'real'], got {dtype_mode!r}")
+        if abs(theta) > 1e-12 or lambda_noise:
+            raise NotImplementedError("theta and Gauss-law noise terms are not on the device path")
+        self.SU2_Hamiltonian_couplings(g, m, theta)
+        c = self.coeffs
+        lat = self.lattice
+        self._new_program()
+        self._add_local("E_square", c["E"])
+        if self.dim > 1:
+            self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], lat.plaquettes(("x", "y")), c["B"], True)
+        if not self.pure_theory:
+            if m is None:
+                raise ValueError("m is required with matter fields")
+            even = lat.even_mask()
+            self._add_local("N-1", c["m_even"], [i for i in range(self.n_sites) if even[i]])
+            self._add_local("N-1", c["m_odd"], [i for i in range(self.n_sites) if not even[i]])
+            for d in self.directions:
+                for stag in ("even", "odd"):
+                    bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
+                    self._add_nbody([f"Qp{d}_dag", f"Qm{d}"], bonds, c[f"t{d}_{stag}"], True)
+        return self._build_program(real)
+
+    def overlap_QMB_state(self, name):
+        """Reference product configurations (SU2_model.py:486-553)."""
+        if self.dim == 1:
+            table = {"V": (0, 4, 0, 2), "PV": (1, 5, 1, 1), "T": (6, 12, 1, 1), "M": (2, 3, 1, 1),
+                     "B": (7, 11, 1, 1)}
+            s1, s2, Lc, Rc = table.get(name, (0, 0, 0, 0))
+            cfg = [s1 if i % 2 == 0 else s2 for i in range(self.n_sites)]
+            if name == "DW":
+                s1, s2, Lc, Rc = 0, 4, 0, 2
+                cfg = [s1 if i < self.n_sites // 2 else s2 for i in range(self.n_sites)]
+            elif name == "DW2":
+                s1, s2, Lc, Rc = 0, 4, 0, 2
+                cfg = [s1 if (i // 2) % 2 == 0 else s2 for i in range(self.n_sites)]
+            if self.has_obc[0]:
+                cfg[0], cfg[-1] = Lc, Rc
+            return np.array(cfg)
+        if not self.pure_theory:
+            if self.has_obc[0]:
+                table = {"V": [0, 9, 0, 4, 4, 0, 9, 0], "PV1": [1, 12, 3, 5, 5, 2, 11, 1],
+                         "PV2": [1, 11, 1, 5, 5, 3, 10, 1]}
+                return np.array(table[name])
+            return np.array([0, 9, 0, 9, 9, 0, 9, 0])
+        return np.zeros(self.n_sites, dtype=int)

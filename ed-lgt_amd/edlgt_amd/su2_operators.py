@@ -1,0 +1,294 @@
+"""SU(2) dressed sites (spin-1/2 rishons, optional two-colour staggered matter).
+
+A dressed site is matter (4 states: vacuum, r = m +1/2, g = m -1/2, pair)
+times one rishon per half-link (3 states: j = 0; j = 1/2, m = +1/2; j = 1/2,
+m = -1/2), half-links ordered m_x, (m_y, m_z,) p_x, (p_y, p_z)
+(edlgt/operators/SU2_operators.py:35-300, SU2_rishons.py:50-80,
+bose_fermi_operators.py:14-70).  The gauge-invariant local states are the
+SU(2) singlets of the site's spins, built by coupling the spins one after the
+other (matter first) and grouped by their intermediate total spins
+(SU2_singlets.py:30-249); their order follows SU2_gauge_invariant_states
+(SU2_operators.py:625-714): spin configurations in product order, then
+singlets sorted by intermediate spins.  Operators are Kronecker products on the
+full site space projected onto each label's singlet basis, B^T O B
+(quantum_model.py:400-502); the site spaces are tiny (36 states in 1D, 324 in
+2D).
+
+Clebsch-Gordan coefficients come from the Racah formula in exact rational
+arithmetic (a coefficient is sign * sqrt(rational); a singlet amplitude, a
+product of them, is rounded once).
+"""
+from fractions import Fraction
+from itertools import product
+from math import factorial, sqrt
+
+import numpy as np
+
+HALF = Fraction(1, 2)
+
+
+# ---------------------------------------------------------------- Clebsch-Gordan
+def _fact(x):
+    if x.denominator != 1 or x < 0:
+        raise ValueError("non-integer factorial argument")
+    return factorial(int(x))
+
+
+def cg_exact(j1, m1, j2, m2, J, M):
+    """<j1 m1 j2 m2 | J M> as (sign_rational, radicand) with value
+    sign_rational * sqrt(radicand); arguments are Fractions."""
+    if m1 + m2 != M or abs(m1) > j1 or abs(m2) > j2 or abs(M) > J:
+        return Fraction(0), Fraction(0)
+    if J < abs(j1 - j2) or J > j1 + j2:
+        return Fraction(0), Fraction(0)
+    for a in (j1 + m1, j2 + m2, J + M, j1 + j2 + J):
+        if a.denominator != 1:
+            return Fraction(0), Fraction(0)
+    rad = Fraction((2 * J + 1).numerator, (2 * J + 1).denominator)
+    rad *= Fraction(_fact(J + j1 - j2) * _fact(J - j1 + j2) * _fact(j1 + j2 - J), _fact(j1 + j2 + J + 1))
+    rad *= _fact(J + M) * _fact(J - M) * _fact(j1 - m1) * _fact(j1 + m1) * _fact(j2 - m2) * _fact(j2 + m2)
+    s = Fraction(0)
+    k = 0
+    while True:
+        args = [j1 + j2 - J - k, j1 - m1 - k, j2 + m2 - k, J - j2 + m1 + k, J - j1 - m2 + k]
+        if any(a < 0 for a in args[:3]):
+            break
+        if all(a >= 0 for a in args):
+            den = factorial(k)
+            for a in args:
+                den *= _fact(Fraction(a))
+            s += Fraction((-1) ** k, den)
+        k += 1
+    return s, rad
+
+
+def m_values(j):
+    """m = j, j-1, ..., -j (descending, spin_operators.py:25-28)."""
+    return [j - k for k in range(int(2 * j) + 1)]
+
+
+def _couple(j1, j2, singlet, m1_fixed=None):
+    """Non-zero couplings of j1 (all m1, or m1 = m1_fixed) with j2, in the
+    reference's enumeration order: m1, then m2 (descending), then J
+    ascending (only J = 0 for a singlet).  Entries (m1, m2, cg, J, M)."""
+    m1s = [m1_fixed] if m1_fixed is not None else m_values(j1)
+    Js = [Fraction(0)] if singlet else [abs(j1 - j2) + k for k in range(int(j1 + j2 - abs(j1 - j2)) + 1)]
+    out = []
+    for m1 in m1s:
+        for m2 in m_values(j2):
+            for J in Js:
+                M = m1 + m2
+                if abs(M) > J:
+                    continue
+                s, r = cg_exact(j1, m1, j2, m2, J, M)
+                if s != 0:
+                    out.append((m1, m2, (s, r), J, M))
+    return out
+
+
+def su2_singlets(spins):
+    """Singlets of the spins (Fractions) coupled left to right.  Returns a list
+    of (intermediate_J_tuple, [(m_tuple, amplitude_float), ...]) in the
+    reference's order (stable sort by intermediate spins)."""
+    n = len(spins)
+    if n < 2:
+        raise ValueError("at least two spins are needed")
+    chains = []  # (m list, (sign, rad) product, J list, M)
+    for m1, m2, c, J, M in _couple(spins[0], spins[1], n == 2):
+        chains.append(([m1, m2], c, [J], M))
+    for k in range(2, n):
+        nxt = []
+        for ms, (s, r), Js, M in chains:
+            for _, m2, (s2, r2), J, M2 in _couple(Js[-1], spins[k], k == n - 1, m1_fixed=M):
+                nxt.append((ms + [m2], (s * s2, r * r2), Js + [J], M2))
+        chains = nxt
+    if not chains:
+        return []
+    chains = sorted(chains, key=lambda c: tuple(c[2]))  # stable, like the reference
+    groups = []
+    for ms, (s, r), Js, _ in chains:
+        amp = float(s) * sqrt(float(r))
+        if groups and groups[-1][0] == tuple(Js):
+            groups[-1][1].append((tuple(ms), amp))
+        else:
+            groups.append((tuple(Js), [(tuple(ms), amp)]))
+    return groups
+
+
+# ---------------------------------------------------------------- single-dof matrices
+def rishon_matrices():
+    """Spin-1/2 rishon algebra (SU2_rishons.py:50-80) and generators
+    (spin_operators.py:91-127): basis (j=0), (1/2, +1/2), (1/2, -1/2)."""
+    cf = 2.0 ** -0.25
+    P = np.diag([1.0, -1.0, -1.0])
+    Zr = np.zeros((3, 3))
+    Zr[0, 1] = cf
+    Zr[2, 0] = cf
+    Zg = np.zeros((3, 3))
+    Zg[0, 2] = cf
+    Zg[1, 0] = -cf
+    Tz = np.diag([0.0, 0.5, -0.5])
+    Tp = np.zeros((3, 3))
+    Tp[1, 2] = 1.0
+    Tm = Tp.T.copy()
+    m = {"Iz": np.eye(3), "P": P, "Zr": Zr, "Zg": Zg, "Tz": Tz, "Tp": Tp, "Tm": Tm,
+         "Tx": 0.5 * (Tp + Tm), "Ty": -0.5j * (Tp - Tm), "T2": np.diag([0.0, 0.75, 0.75])}
+    for c in "rg":
+        Z = m[f"Z{c}"]
+        m[f"Z{c}_dag"] = Z.T.copy()
+        m[f"Z{c}_P"] = Z @ P
+        m[f"P_Z{c}_dag"] = P @ Z.T
+        m[f"Z{c}_dag_P"] = Z.T @ P
+    return m
+
+
+def matter_matrices():
+    """Two-component fermion on one site (bose_fermi_operators.py:43-67):
+    index = 2 * n_down + n_up, r = up (m = +1/2), g = down (m = -1/2)."""
+    psi = np.array([[0.0, 1.0], [0.0, 0.0]])
+    Pp = np.diag([1.0, -1.0])
+    I2 = np.eye(2)
+    N = np.diag([0.0, 1.0])
+    m = {"ID_psi": np.eye(4), "P_psi": np.kron(Pp, Pp)}
+    m["psi_g"] = np.kron(psi, I2)          # down
+    m["psi_r"] = np.kron(Pp, psi)          # up, with the parity string of the down mode
+    m["N_g"] = np.kron(N, I2)
+    m["N_r"] = np.kron(I2, N)
+    m["N_pair"] = m["N_r"] @ m["N_g"]
+    m["N_tot"] = m["N_r"] + m["N_g"]
+    m["N_single"] = m["N_tot"] - 2 * m["N_pair"]
+    m["N_zero"] = np.eye(4) - m["N_pair"] - m["N_single"]
+    for c in "rg":
+        m[f"psi_{c}_dag"] = m[f"psi_{c}"].T.copy()
+        m[f"psi_{c}_dag_P"] = m[f"psi_{c}_dag"] @ m["P_psi"]
+    return m
+
+
+def _kron(mats):
+    out = np.array([[1.0]])
+    for a in mats:
+        out = np.kron(out, a)
+    return out
+
+
+# ---------------------------------------------------------------- dressed site
+MATTER_J = [("V", Fraction(0)), ("S", HALF), ("S", HALF), ("P", Fraction(0))]
+MATTER_M = [Fraction(0), HALF, -HALF, Fraction(0)]
+RISHON_J = [Fraction(0), HALF, HALF]
+RISHON_M = [Fraction(0), HALF, -HALF]
+
+
+class SU2Site:
+    """Singlet bases per site label ('site' and OBC border labels) as
+    isometries basis[label] (full site dim x d_loc)."""
+
+    def __init__(self, pure_theory, dim):
+        self.pure = bool(pure_theory)
+        self.dim_lat = int(dim)
+        axes = "xyz"[:dim]
+        self.links = [f"m{a}" for a in axes] + [f"p{a}" for a in axes]
+        n_link = len(self.links)
+        self.full_dim = (1 if self.pure else 4) * 3 ** n_link
+        from .operators import _border_parts
+        labels = ["site"] + ["site_" + "_".join(p) for p in _border_parts(dim)]
+        cols = {lab: [] for lab in labels}
+        vals = {lab: [] for lab in labels}
+        link_choices = [[Fraction(0), HALF]] * n_link
+        matter_choices = [] if self.pure else [["V", "S", "P"]]
+        for combo in product(*(matter_choices + link_choices)):
+            if self.pure:
+                mkind, links = None, list(combo)
+                spins = links
+            else:
+                mkind, links = combo[0], list(combo[1:])
+                spins = [HALF if mkind == "S" else Fraction(0)] + links
+            j2 = [int(2 * s) for s in spins]
+            tot = sum(j2)
+            if tot % 2 or max(j2) > tot - max(j2):
+                continue
+            for Js, terms in su2_singlets(spins):
+                vec = np.zeros(self.full_dim)
+                for ms, amp in terms:
+                    vec[self._index(mkind, spins, ms)] = amp
+                zero_links = {self.links[k] for k, s in enumerate(links) if s == 0}
+                jv = ([{"V": 0, "S": 1, "P": 2}[mkind]] if not self.pure else []) + [int(2 * s) for s in links]
+                for lab in labels:
+                    parts = lab.split("_")[1:]
+                    if all(p in zero_links for p in parts):
+                        cols[lab].append(vec)
+                        vals[lab].append(jv)
+        self.basis = {lab: np.column_stack(cols[lab]) for lab in labels if cols[lab]}
+        self.values = {lab: np.asarray(vals[lab], dtype=np.int64) for lab in labels if cols[lab]}
+
+    def _index(self, mkind, spins, ms):
+        idx = 0
+        k0 = 0
+        if not self.pure:
+            if mkind == "V":
+                mi = 0
+            elif mkind == "P":
+                mi = 3
+            else:
+                mi = 1 if ms[0] > 0 else 2
+            idx = mi
+            k0 = 1
+        for s, m in zip(spins[k0:], ms[k0:]):
+            ri = 0 if s == 0 else (1 if m > 0 else 2)
+            idx = idx * 3 + ri
+        return idx
+
+    def dim(self, label):
+        return self.basis[label].shape[1]
+
+    def full_operators(self):
+        """Kronecker-product operators on the full site space
+        (SU2_operators.py:35-300, spin 1/2; 1D and 2D)."""
+        R = rishon_matrices()
+        Mt = None if self.pure else matter_matrices()
+        links = self.links
+        n_link = len(links)
+
+        def op(assign, matter="ID_psi"):
+            mats = [] if self.pure else [Mt[matter]]
+            mats += [R[assign.get(l, "Iz")] for l in links]
+            return _kron(mats)
+
+        ops = {}
+        for name in ("T2", "P", "Tx", "Ty", "Tz"):
+            for l in links:
+                ops[f"{name}_{l}"] = op({l: name})
+        if self.dim_lat == 1:
+            ops["U"] = sum(op({"mx": f"Z{c}_dag_P", "px": f"Z{c}"}) for c in "rg") / np.sqrt(2)
+        elif self.dim_lat == 2:
+            ops["C_px,py"] = sum(op({"px": f"Z{c}_P", "py": f"Z{c}_dag"}) for c in "rg")
+            ops["C_py,mx"] = sum(op({"mx": f"P_Z{c}_dag", "my": "P", "px": "P", "py": f"Z{c}"}) for c in "rg")
+            ops["C_mx,my"] = sum(op({"mx": f"Z{c}_P", "my": f"Z{c}_dag"}) for c in "rg")
+            ops["C_my,px"] = sum(op({"my": f"Z{c}_P", "px": f"Z{c}_dag"}) for c in "rg")
+        else:
+            raise NotImplementedError("SU(2) dressed sites are built for 1D and 2D lattices")
+        if not self.pure:
+            axes = "xyz"[: self.dim_lat]
+            for a in axes:
+                ops[f"Qm{a}_dag"] = 0
+                ops[f"Qp{a}_dag"] = 0
+            for c in "rg":
+                # string of parities over the half-links before the acted one
+                for k, l in enumerate(links):
+                    assign = {links[q]: "P" for q in range(k)}
+                    assign[l] = f"Z{c}"
+                    ops[f"Q{l}_dag"] = ops[f"Q{l}_dag"] + op(assign, matter=f"psi_{c}_dag_P")
+            for a in axes:
+                for s in "mp":
+                    ops[f"Q{s}{a}"] = ops[f"Q{s}{a}_dag"].conj().T.copy()
+            for lab in ("r", "g", "tot", "single", "pair", "zero"):
+                ops[f"N_{lab}"] = op({}, matter=f"N_{lab}")
+            ops["N-1"] = ops["N_tot"] - np.eye(self.full_dim)
+        ops["E_square"] = sum(0.5 * ops[f"T2_{l}"] for l in links)
+        return ops
+
+    def operators(self, label):
+        """Every operator projected on the label's singlet basis, B^T O B."""
+        if not hasattr(self, "_full"):
+            self._full = self.full_operators()
+        B = self.basis[label]
+        return {k: B.T @ v @ B for k, v in self._full.items()}

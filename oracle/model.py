@@ -428,3 +428,81 @@ class ZnOracle:
                                                  self.has_obc) for i in range(self.n_sites)) if p]
         add_nbody_list(T, [self.ops[k] for k in names], plaqs, *args, -J, True)
         return T.csr(real=True)
+
+
+class SU2Oracle:
+    """SU2_Model (j = 1/2, hard-coded operators, no background) restated
+    (models/SU2_model.py:31-171, 200-355, 555-624), 1D and 2D."""
+
+    def __init__(self, pure_theory, lvals, has_obc, sectors=None):
+        from . import su2_ops
+        self.pure = pure_theory
+        self.lvals, self.has_obc = list(lvals), list(has_obc)
+        self.dim = len(lvals)
+        self.n_sites = prod(lvals)
+        self.dirs = "xyz"[: self.dim]
+        basis = su2_ops.gauge_invariant_states(pure_theory, self.dim)
+        ops = su2_ops.dressed_site_operators(pure_theory, self.dim)
+        self.labels = [lat.site_label(lat.zig_zag(self.lvals, i), self.lvals, self.has_obc, False)
+                       for i in range(self.n_sites)]
+        self.site_basis = basis
+        self.ops, self.loc_dims = project_operators(ops, basis, self.labels)
+        dg = lambda name: np.real(np.einsum("sii->si", self.ops[name]))
+        pairs = lat.link_pairs(self.lvals, self.has_obc)
+        link_diags = np.stack([np.stack([dg(f"T2_p{d}"), -dg(f"T2_m{d}")]) for d in self.dirs])
+        link_sectors = np.zeros(self.dim)
+        if not pure_theory:
+            self.configs = sector_configs(self.loc_dims, dg("N_tot")[None], np.array([float(sectors[0])]),
+                                          "U", link_diags, link_sectors, pairs)
+        elif not any(self.has_obc) and self.dim == 2:
+            lines = lat.origin_lines_2d(self.lvals)
+            line_of = {"x": "y", "y": "x"}
+            self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, link_sectors, pairs,
+                                          np.stack([dg(f"P_p{d}") for d in self.dirs]), np.ones(self.dim),
+                                          [lines[line_of[d]] for d in self.dirs], "Z")
+        else:
+            self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, link_sectors, pairs)
+        self.keys = encode(self.configs, self.loc_dims)
+        self.N = self.configs.shape[0]
+
+    def couplings(self, g, m=None):
+        """SU2_Hamiltonian_couplings (SU2_model.py:555-624)."""
+        c = {"E": 8 * g / 3 if self.dim == 1 else g / 2, "B": 0 if self.dim == 1 else -1 / (2 * g)}
+        if not self.pure and m is not None:
+            t = 2 * np.sqrt(2)
+            c.update({"tx_even": -1j * t, "tx_odd": -1j * t, "ty_even": -t, "ty_odd": t,
+                      "m_even": m, "m_odd": -m})
+        return c
+
+    def triplets(self, g, m=None):
+        """build_base_Hamiltonian term order (SU2_model.py:200-355)."""
+        c = self.couplings(g, m)
+        T = Triplets(self.N)
+        args = (self.configs, self.keys, self.loc_dims)
+        add_local(T, self.ops["E_square"], *args, c["E"])
+        if self.dim > 1:
+            names = ["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"]
+            plaqs = [p for p in (lat.plaquette_sites(lat.zig_zag(self.lvals, i), self.lvals, ["x", "y"],
+                                                     self.has_obc) for i in range(self.n_sites)) if p]
+            add_nbody_list(T, [self.ops[n] for n in names], plaqs, *args, c["B"], True)
+        if not self.pure:
+            for stag in ["even", "odd"]:
+                mask = lat.staggered_mask(self.lvals, stag)
+                ms = [mask[lat.zig_zag(self.lvals, i)] for i in range(self.n_sites)]
+                add_local(T, self.ops["N-1"], *args, c[f"m_{stag}"], ms)
+            for d in self.dirs:
+                for stag in ["even", "odd"]:
+                    mask = lat.staggered_mask(self.lvals, stag)
+                    bonds = []
+                    for i in range(self.n_sites):
+                        co = lat.zig_zag(self.lvals, i)
+                        b = lat.neighbor_sites(co, self.lvals, d, self.has_obc)
+                        if b is not None and mask[co]:
+                            bonds.append(b)
+                    if bonds:
+                        add_nbody_list(T, [self.ops[f"Qp{d}_dag"], self.ops[f"Qm{d}"]], bonds, *args,
+                                       c[f"t{d}_{stag}"], True)
+        return T
+
+    def hamiltonian(self, g, m=None):
+        return self.triplets(g, m).csr(real=self.pure)
