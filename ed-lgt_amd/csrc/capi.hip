@@ -245,6 +245,7 @@ int edlgt_hamiltonian_build_terms(const edlgt_basis* b, int32_t d_max, int32_t n
 
 int edlgt_csr_copy_terms(const edlgt_csr* h, uint8_t* term_host, int32_t* n_terms) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!term_host || h->n_terms == 1 || h->term.p, "the term array was released (coded matrix)");
   if (n_terms) *n_terms = h->n_terms;
   if (term_host && h->n_terms > 1 && h->nnz)
     EDLGT_HIP(hipMemcpy(term_host, h->term.p, h->nnz, hipMemcpyDeviceToHost));
@@ -260,6 +261,16 @@ int edlgt_csr_compress(edlgt_csr* h, void* stream, int32_t* n_dict, int32_t* cod
   return EDLGT_OK;
 }
 
+int edlgt_csr_release_values(edlgt_csr* h) {
+  EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(h->n_dict > 0, "only a coded matrix can release its value array");
+  EDLGT_HIP(hipDeviceSynchronize());
+  h->val.release();
+  h->val_lo.release();
+  h->term.release();
+  return EDLGT_OK;
+}
+
 int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* value_type) {
   EDLGT_REQUIRE(h, "null matrix");
   if (n_rows) *n_rows = h->n_rows;
@@ -270,6 +281,7 @@ int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* v
 
 int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!val || h->val.p || h->nnz == 0, "the value array was released (coded matrix)");
   if (row_ptr)
     EDLGT_HIP(hipMemcpy(row_ptr, h->row_ptr.p, (h->n_rows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
   if (col && h->nnz)
@@ -371,7 +383,15 @@ int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid) {
   EDLGT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 ||
                     lanes_per_row == 16 || lanes_per_row == 32 || lanes_per_row == 64,
                 "lanes_per_row must be 0, 4, 8, 16, 32 or 64");
-  EDLGT_REQUIRE(grid >= 0, "grid must be >= 0");
+  if (grid < 0) {
+    // coded matrices: grid = -(variant + 1) selects a coded stream launch variant
+    EDLGT_REQUIRE(h->n_dict > 0 && lanes_per_row == 0 && -grid - 1 < 5, "bad coded launch variant");
+    const int v = (int)(-grid - 1);
+    const int tiles[5] = {2048, 1024, 4096, 4096, 2048};
+    EDLGT_TRY(csr_prepare_tiles(*h, tiles[v], nullptr));
+    h->coded_variant = v;
+    return EDLGT_OK;
+  }
   h->g_override = lanes_per_row;
   h->grid_override = grid;
   return EDLGT_OK;

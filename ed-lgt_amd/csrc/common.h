@@ -103,9 +103,14 @@ struct Csr {
   DevBuf val_lo;    // reduced-precision copy (complex64 / float32), built on demand
   DevBuf term;      // uint8 [nnz] term id of every entry (multi-term matrices only)
   int32_t n_terms = 1;
-  DevBuf stream_blocks;      // int64 [n_stream_blocks + 1] first row of every CSR-stream tile
-  DevBuf stream_pos;         // int64 [n_stream_blocks + 1] first entry of every tile
+  // CSR-stream tiles of 1024 / 2048 / 4096 nonzeros (tile_blocks[t]: first
+  // row of every tile, tile_pos[t]: its first entry); 2048 is the default
+  DevBuf tile_blocks[3], tile_pos[3];
+  int64_t n_tile_blocks[3] = {0, 0, 0};
+  DevBuf& stream_blocks = tile_blocks[1];
+  DevBuf& stream_pos = tile_pos[1];
   int64_t n_stream_blocks = 0;
+  int32_t coded_variant = 0;  // launch variant of the coded stream kernel
   // dictionary coding of the values (edlgt_csr_compress): entry p holds
   // dict_val[code[p]] of term dict_term[code[p]]; code is uint8 when
   // n_dict <= 256, else uint16 (n_dict <= 4096)

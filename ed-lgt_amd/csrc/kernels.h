@@ -37,6 +37,7 @@ int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, 
                   void* const* hy, const double* coef, void* pack, hipStream_t st);
 // Row partition of the CSR-stream SpMV (run once per matrix).
 int csr_prepare_stream(Csr& h, hipStream_t st);
+int csr_prepare_tiles(Csr& h, int tile, hipStream_t st);
 int csr_compress(Csr& h, hipStream_t st, int* compressed);
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
                  hipStream_t st, int32_t* best_g, int64_t* best_grid, float* best_ms);

@@ -151,11 +151,11 @@ __device__ inline double2 sum_lanes(double2 v) {
 }
 
 // Row sums from the LDS products with G lanes per row (fixed per block).
-template <int G, class AT>
+template <int G, class AT, int BS = kBS>
 __device__ inline void reduce_rows(const AT* prod, const int64_t* __restrict__ row_ptr, int64_t r0,
                                    int64_t nr, int64_t p0, AT* __restrict__ y) {
   const int sub = threadIdx.x & (G - 1);
-  constexpr int per = kBS / G;
+  constexpr int per = BS / G;
   for (int64_t i = threadIdx.x / G; i < (nr + per - 1) / per * per; i += per) {
     AT s = zero_acc<AT>();
     if (i < nr) {
@@ -233,14 +233,14 @@ __global__ void stream_block_pos(const int64_t* __restrict__ row_ptr, const int6
 }
 
 __global__ void stream_row_blocks(const int64_t* __restrict__ row_ptr, int64_t n, int64_t n_blocks,
-                                  int64_t* __restrict__ rb) {
+                                  int64_t* __restrict__ rb, int64_t tile) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n_blocks;
        k += (int64_t)gridDim.x * blockDim.x) {
     if (k == n_blocks) {
       rb[k] = n;
       continue;
     }
-    const int64_t target = k * (int64_t)kTile;
+    const int64_t target = k * tile;
     int64_t lo = 0, len = n;  // first row with row_ptr[r] >= target
     while (len > 0) {
       const int64_t half = len >> 1;
@@ -262,15 +262,25 @@ __global__ void stream_row_blocks(const int64_t* __restrict__ row_ptr, int64_t n
 constexpr int kTabLds = 1024;      // tables up to this many entries sit in LDS
 constexpr int kRowsTabLds = 2048;  // NB x n_tab bound for the multi-RHS kernel
 
-template <class TT, class CT, class XT, class AT, bool kLds>
-__global__ void __launch_bounds__(kBS) csr_stream_coded(
+// Launch variants (threads per block, nonzeros per tile, independent gathers
+// in flight per thread), picked per matrix by csr_autotune.
+struct CodedVariant {
+  int bs, tile, unroll;
+};
+constexpr CodedVariant kCodedVariants[] = {{512, 2048, 4}, {256, 1024, 4}, {512, 4096, 8}, {1024, 4096, 4},
+                                           {256, 2048, 8}};
+constexpr int kNumCodedVariants = 5;
+
+template <class TT, class CT, class XT, class AT, bool kLds, int BS, int TILE, int U>
+__global__ void __launch_bounds__(BS) csr_stream_coded(
     int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const CT* __restrict__ code, const TT* __restrict__ tab, int n_tab, const XT* __restrict__ x,
     AT* __restrict__ y, const int64_t* __restrict__ row_blocks, const int64_t* __restrict__ block_pos) {
-  __shared__ AT prod[kCap];
+  constexpr int CAP = TILE + 256;
+  __shared__ AT prod[CAP];
   __shared__ TT stab[kLds ? kTabLds : 1];
   if (kLds) {
-    for (int i = threadIdx.x; i < n_tab; i += kBS) stab[i] = tab[i];
+    for (int i = threadIdx.x; i < n_tab; i += BS) stab[i] = tab[i];
     __syncthreads();
   }
   const TT* tb = kLds ? stab : tab;
@@ -280,21 +290,21 @@ __global__ void __launch_bounds__(kBS) csr_stream_coded(
   const int64_t cnt = p1 - p0;
   const int64_t nr = r1 - r0;
   const int tid = threadIdx.x;
-  if (cnt <= kCap) {
-    for (int64_t base = 0; base < cnt; base += kBS * kUnroll) {
-      int32_t c[kUnroll];
-      int k[kUnroll];
+  if (cnt <= CAP) {
+    for (int64_t base = 0; base < cnt; base += BS * U) {
+      int32_t c[U];
+      int k[U];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int64_t i = base + u * kBS + tid;
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = base + u * BS + tid;
         if (i < cnt) {
           c[u] = ldnt(col + p0 + i);
           k[u] = (int)ldnt(code + p0 + i);
         }
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int64_t i = base + u * kBS + tid;
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = base + u * BS + tid;
         if (i < cnt) {
           AT a = zero_acc<AT>();
           fma_acc(a, tb[k[u]], load_x(x, c[u]));
@@ -303,19 +313,19 @@ __global__ void __launch_bounds__(kBS) csr_stream_coded(
       }
     }
     __syncthreads();
-    if (nr <= 64) reduce_rows<8>(prod, row_ptr, r0, nr, p0, y);
-    else if (nr <= 128) reduce_rows<4>(prod, row_ptr, r0, nr, p0, y);
-    else if (nr <= 256) reduce_rows<2>(prod, row_ptr, r0, nr, p0, y);
-    else reduce_rows<1>(prod, row_ptr, r0, nr, p0, y);
+    if (nr <= BS / 8) reduce_rows<8, AT, BS>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= BS / 4) reduce_rows<4, AT, BS>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= BS / 2) reduce_rows<2, AT, BS>(prod, row_ptr, r0, nr, p0, y);
+    else reduce_rows<1, AT, BS>(prod, row_ptr, r0, nr, p0, y);
   } else {
     AT* red = prod;
     for (int64_t r = r0; r < r1; ++r) {
       AT s = zero_acc<AT>();
-      for (int64_t p = row_ptr[r] + tid; p < row_ptr[r + 1]; p += kBS)
+      for (int64_t p = row_ptr[r] + tid; p < row_ptr[r + 1]; p += BS)
         fma_acc(s, tb[(int)ldnt(code + p)], load_x(x, ldnt(col + p)));
       red[tid] = s;
       __syncthreads();
-      for (int o = kBS / 2; o > 0; o >>= 1) {
+      for (int o = BS / 2; o > 0; o >>= 1) {
         if (tid < o) red[tid] = red[tid] + red[tid + o];
         __syncthreads();
       }
@@ -385,23 +395,66 @@ int launch_stream(const Csr& h, const VT* val, const XT* x, AT* y, hipStream_t s
 }
 
 
-int csr_prepare_stream(Csr& h, hipStream_t st) {
-  if (h.stream_blocks.p) return EDLGT_OK;
-  const int64_t nb = std::max<int64_t>(1, (h.nnz + kTile - 1) / kTile);
+int spmv_coded(const Csr& h, const void* tab, int xtype, const void* xb, void* yb, hipStream_t st);
+int tile_index(int tile) { return tile == 1024 ? 0 : tile == 2048 ? 1 : 2; }
+
+// Row-start partition of the nonzeros into tiles of `tile` entries (the
+// stream kernels' work units): tile_blocks[k] = first row starting at or
+// after k * tile, tile_pos[k] = its first entry.
+int csr_prepare_tiles(Csr& h, int tile, hipStream_t st) {
+  const int ti = tile_index(tile);
+  if (h.tile_blocks[ti].p) return EDLGT_OK;
+  const int64_t nb = std::max<int64_t>(1, (h.nnz + tile - 1) / tile);
   if (nb > 0x7fffffffLL) return set_error(EDLGT_ERR_INDEX_OVERFLOW, "too many stream blocks");
-  EDLGT_TRY(h.stream_blocks.alloc((nb + 1) * sizeof(int64_t)));
+  EDLGT_TRY(h.tile_blocks[ti].alloc((nb + 1) * sizeof(int64_t)));
   hipLaunchKernelGGL(stream_row_blocks, dim3(grid_for(nb + 1, 256)), dim3(256), 0, st,
-                     h.row_ptr.as<int64_t>(), h.n_rows, nb, h.stream_blocks.as<int64_t>());
+                     h.row_ptr.as<int64_t>(), h.n_rows, nb, h.tile_blocks[ti].as<int64_t>(), (int64_t)tile);
   EDLGT_HIP(hipGetLastError());
-  EDLGT_TRY(h.stream_pos.alloc((nb + 1) * sizeof(int64_t)));
+  EDLGT_TRY(h.tile_pos[ti].alloc((nb + 1) * sizeof(int64_t)));
   hipLaunchKernelGGL(stream_block_pos, dim3(grid_for(nb + 1, 256)), dim3(256), 0, st,
-                     h.row_ptr.as<int64_t>(), h.stream_blocks.as<int64_t>(), nb, h.stream_pos.as<int64_t>());
+                     h.row_ptr.as<int64_t>(), h.tile_blocks[ti].as<int64_t>(), nb, h.tile_pos[ti].as<int64_t>());
   EDLGT_HIP(hipGetLastError());
   EDLGT_HIP(hipStreamSynchronize(st));
-  h.n_stream_blocks = nb;
+  h.n_tile_blocks[ti] = nb;
   return EDLGT_OK;
 }
 
+int csr_prepare_stream(Csr& h, hipStream_t st) {
+  EDLGT_TRY(csr_prepare_tiles(h, kTile, st));
+  h.n_stream_blocks = h.n_tile_blocks[tile_index(kTile)];
+  return EDLGT_OK;
+}
+
+// Times the coded stream variants on x -> y (one right-hand side, FP64
+// tables) and keeps the fastest in h.coded_variant.
+int coded_autotune(Csr& h, int xtype, const void* x, void* y, hipStream_t st, float* best_ms) {
+  hipEvent_t e0, e1;
+  EDLGT_HIP(hipEventCreate(&e0));
+  EDLGT_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bv = 0, status = EDLGT_OK;
+  for (int v = 0; v < kNumCodedVariants && status == EDLGT_OK; ++v) {
+    status = csr_prepare_tiles(h, kCodedVariants[v].tile, st);
+    if (status != EDLGT_OK) break;
+    h.coded_variant = v;
+    status = spmv_coded(h, h.tab_unit.p, xtype, x, y, st);  // warm
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < 3 && status == EDLGT_OK; ++r) status = spmv_coded(h, h.tab_unit.p, xtype, x, y, st);
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms / 3 < best) {
+      best = ms / 3;
+      bv = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h.coded_variant = bv;
+  if (best_ms) *best_ms = best;
+  return status;
+}
 
 // Single right-hand side through the CSR-stream kernel.
 int spmv_stream(const Csr& h, int precision, int xtype, const void* xb, void* yb, hipStream_t st) {
@@ -430,21 +483,35 @@ int spmv_stream(const Csr& h, int precision, int xtype, const void* xb, void* yb
 }
 
 
+int tile_index(int tile);
+
 template <class TT, class XT, class AT>
 int launch_stream_coded(const Csr& h, const TT* tab, const XT* x, AT* y, hipStream_t st) {
   const bool lds = h.n_dict <= kTabLds;
-#define EDLGT_SC(CT, LDS)                                                                          \
-  hipLaunchKernelGGL((csr_stream_coded<TT, CT, XT, AT, LDS>), dim3((unsigned)h.n_stream_blocks),    \
-                     dim3(kBS), 0, st, h.n_rows, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(),       \
-                     h.code.as<CT>(), tab, h.n_dict, x, y, h.stream_blocks.as<int64_t>(),           \
-                     h.stream_pos.as<int64_t>())
-  if (h.code_bytes == 1) {
-    if (lds) EDLGT_SC(uint8_t, true);
-    else EDLGT_SC(uint8_t, false);
-  } else {
-    if (lds) EDLGT_SC(uint16_t, true);
-    else EDLGT_SC(uint16_t, false);
+  const int v = h.coded_variant;
+  const int ti = tile_index(kCodedVariants[v].tile);
+  if (!h.tile_blocks[ti].p) return set_error(EDLGT_ERR_INVALID_ARG, "stream tiles not prepared");
+  const unsigned nb = (unsigned)h.n_tile_blocks[ti];
+  const int64_t* rb = h.tile_blocks[ti].as<int64_t>();
+  const int64_t* pos = h.tile_pos[ti].as<int64_t>();
+#define EDLGT_SC(CT, LDS, BS, TILE, U)                                                              \
+  hipLaunchKernelGGL((csr_stream_coded<TT, CT, XT, AT, LDS, BS, TILE, U>), dim3(nb), dim3(BS), 0, st, \
+                     h.n_rows, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), h.code.as<CT>(), tab,   \
+                     h.n_dict, x, y, rb, pos)
+#define EDLGT_SCV(CT, LDS)                                          \
+  switch (v) {                                                      \
+    case 0: EDLGT_SC(CT, LDS, 512, 2048, 4); break;                 \
+    case 1: EDLGT_SC(CT, LDS, 256, 1024, 4); break;                 \
+    case 2: EDLGT_SC(CT, LDS, 512, 4096, 8); break;                 \
+    case 3: EDLGT_SC(CT, LDS, 1024, 4096, 4); break;                \
+    default: EDLGT_SC(CT, LDS, 256, 2048, 8); break;                \
   }
+  if (h.code_bytes == 1) {
+    if (lds) { EDLGT_SCV(uint8_t, true) } else { EDLGT_SCV(uint8_t, false) }
+  } else {
+    if (lds) { EDLGT_SCV(uint16_t, true) } else { EDLGT_SCV(uint16_t, false) }
+  }
+#undef EDLGT_SCV
 #undef EDLGT_SC
   EDLGT_HIP(hipGetLastError());
   return EDLGT_OK;
@@ -454,21 +521,27 @@ int launch_stream_coded(const Csr& h, const TT* tab, const XT* x, AT* y, hipStre
 // matrix, double2 for a complex one).
 int spmv_coded(const Csr& h, const void* tab, int xtype, const void* xb, void* yb, hipStream_t st) {
   if (h.value_type == EDLGT_VALUE_C128) {
-    if (xtype != EDLGT_VALUE_C128) return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
-    return launch_stream_coded(h, (const double2*)tab, (const double2*)xb, (double2*)yb, st);
+    if (xtype == EDLGT_VALUE_C128)
+      return launch_stream_coded(h, (const double2*)tab, (const double2*)xb, (double2*)yb, st);
+    if (xtype == EDLGT_VALUE_C64)  // mixed precision: complex64 gathered vector, FP64 values and sums
+      return launch_stream_coded(h, (const double2*)tab, (const float2*)xb, (double2*)yb, st);
+    return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
   }
   if (xtype == EDLGT_VALUE_F64)
     return launch_stream_coded(h, (const double*)tab, (const double*)xb, (double*)yb, st);
   if (xtype == EDLGT_VALUE_C128)
     return launch_stream_coded(h, (const double*)tab, (const double2*)xb, (double2*)yb, st);
+  if (xtype == EDLGT_VALUE_F32)
+    return launch_stream_coded(h, (const double*)tab, (const float*)xb, (double*)yb, st);
   return set_error(EDLGT_ERR_INVALID_ARG, "unsupported vector type for a coded matrix");
 }
 
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx, void* y,
              int64_t ldy, hipStream_t st) {
-  if (h.n_dict > 0 && precision == EDLGT_PREC_FP64 && h.stream_blocks.p) {
-    // coded values: one stream pass per right-hand side (all term coefficients 1)
-    const size_t xe = xtype == EDLGT_VALUE_C128 ? 16 : 8;
+  if (h.n_dict > 0 && h.stream_blocks.p) {
+    // coded values: one stream pass per right-hand side (all term coefficients
+    // 1); with a reduced-precision vector only the gathers are FP32
+    const size_t xe = xtype == EDLGT_VALUE_C128 ? 16 : xtype == EDLGT_VALUE_F32 ? 4 : 8;
     const size_t ye = h.value_type == EDLGT_VALUE_C128 || xtype == EDLGT_VALUE_C128 ? 16 : 8;
     for (int j = 0; j < n_rhs; ++j)
       EDLGT_TRY(spmv_coded(h, h.tab_unit.p, xtype, static_cast<const char*>(x) + (size_t)j * ldx * xe,
@@ -543,6 +616,7 @@ int convert_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t st) 
 }
 
 int csr_make_low_precision(Csr& h, hipStream_t st) {
+  if (h.n_dict > 0) return EDLGT_OK;  // coded values: products read the exact table
   if (h.val_lo.p) return EDLGT_OK;
   const int64_t ncomp = h.nnz * (h.value_type == EDLGT_VALUE_C128 ? 2 : 1);
   EDLGT_TRY(h.val_lo.alloc(std::max<int64_t>(ncomp, 1) * sizeof(float)));
@@ -839,6 +913,14 @@ int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, 
 
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,
                  hipStream_t st, int32_t* best_g, int64_t* best_grid, float* best_ms) {
+  if (h.n_dict > 0 && h.stream_blocks.p) {
+    // coded values: the launch variant of the coded stream kernel
+    if (best_g) *best_g = 0;
+    if (best_grid) *best_grid = -h.coded_variant - 1;
+    EDLGT_TRY(coded_autotune(h, xtype, x, y, st, best_ms));
+    if (best_grid) *best_grid = -h.coded_variant - 1;
+    return EDLGT_OK;
+  }
   const double avg = h.n_rows > 0 ? (double)h.nnz / (double)h.n_rows : 1.0;
   const int gs[] = {4, 8, 16, 32, 64};
   const int64_t grids[] = {4096, 16384, 65536, 262144};

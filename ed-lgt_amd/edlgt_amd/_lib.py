@@ -68,6 +68,7 @@ SIGNATURES = {
     "edlgt_hamiltonian_build_terms": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _p, _i32, _i32, _p, _p]),
     "edlgt_csr_copy_terms": (_i32, [_p, _p, _p]),
     "edlgt_csr_compress": (_i32, [_p, _p, _p, _p]),
+    "edlgt_csr_release_values": (_i32, [_p]),
     "edlgt_spmm_coef": (_i32, [_p, _i32, _p, _p, _i32, _p, _p]),
     "edlgt_server_create": (_i32, [_p, _i32, _i32, _i32, _f64, _i32, _i64, _i32, _p, _p]),
     "edlgt_server_submit": (_i32, [_p, _i64, _p, C.c_uint64, _p]),

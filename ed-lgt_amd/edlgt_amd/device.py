@@ -218,6 +218,11 @@ class CSRMatrix:
         self.n_dict, self.code_bytes = int(nd.value), int(cb.value)
         return self.n_dict
 
+    def release_values(self):
+        """Frees the full-width value arrays of a coded matrix (products keep
+        working from the codes; to_host can no longer return values)."""
+        L.check(L.load().edlgt_csr_release_values(self._h))
+
     def terms(self):
         """Per-entry term ids (uint8, nnz); all zero for a single-term matrix."""
         t = np.zeros(self.nnz, dtype=np.uint8)

@@ -175,6 +175,11 @@ int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* v
  * matrix stays as it was.  FP64 matrices only.  No reference counterpart
  * (the reference stores complex128 triplets / CSC). */
 int edlgt_csr_compress(edlgt_csr* h, void* stream, int32_t* n_dict, int32_t* code_bytes);
+/* Frees the value (and term-id and reduced-precision) arrays of a coded
+ * matrix: products keep working from the codes; edlgt_csr_copy can then no
+ * longer return values.  For matrices whose full-width values would crowd
+ * out the Krylov vectors (e.g. 1D SU(2), 20 sites: 3.8e8 states). */
+int edlgt_csr_release_values(edlgt_csr* h);
 /* Copy CSR arrays to host: row_ptr int64 [n+1], col int32 [nnz], val [nnz]. */
 int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val);
 int edlgt_csr_device_ptrs(const edlgt_csr* h, void** row_ptr, void** col, void** val);
@@ -210,11 +215,16 @@ int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t
 
 /* Times the SpMM launch shapes (lanes per row x workgroups) on X -> Y with HIP
  * events and keeps the fastest for this matrix (no counterpart in the
- * reference: CPU csr_spmv has no launch shape).  Outputs may be NULL. */
+ * reference: CPU csr_spmv has no launch shape).  Outputs may be NULL.  For a
+ * coded matrix (edlgt_csr_compress) the candidates are the coded stream
+ * kernel's variants (threads per block x nonzeros per tile x gathers in
+ * flight); *out_lanes = 0 and *out_grid = -(variant + 1). */
 int edlgt_csr_autotune(edlgt_csr* h, int32_t precision, int32_t vec_type, int32_t n_rhs,
                        const void* X, void* Y, void* stream, int32_t* out_lanes, int64_t* out_grid,
                        float* out_ms);
-/* Sets the launch shape explicitly (lanes_per_row 0 and grid 0 restore the defaults). */
+/* Sets the launch shape explicitly (lanes_per_row 0 and grid 0 restore the
+ * defaults; on a coded matrix lanes_per_row 0 and grid -(v + 1) select the
+ * coded stream variant v, 0 <= v < 5). */
 int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid);
 
 /* ======================================================================
