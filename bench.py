@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload qed4x3|qed4x2] [--no-cpu]
                     [--serve-requests 512] [--serve-rate 20] [--plain-sample 32] [--no-serve]
+    python bench.py --workload su2c3 [--sites 20] [--dyn-steps 200]
 
 Workload (default): 2D U(1) QED, 4x3 periodic lattice, staggered fermions,
 E in {-1,0,1}, g=1, m=0.5, Gauss-law + zero-charge sector (N = 28,904,615).
@@ -30,6 +31,17 @@ stream one request at a time, each re-assembling H(J, h) and running the FP64
 block Lanczos cold; timed on the first --plain-sample requests of the stream
 (its full 512-request run takes many minutes).  |dE| of the served against the
 plain eigenvalues is reported over that sample.
+
+Workload su2c3 (dynamics): 1D SU(2), spin-1/2 rishons with staggered
+two-colour matter, 20 sites, OBC, N_tot = 20, g = 1, m = 1 (N = 377,379,369
+states).  Quench from the bare vacuum ("V"); exp(-iH dt) psi by Krylov steps
+with a 30-vector Lanczos basis, dt = 0.05 to t = 10 (200 steps), once with the
+FP64 products and once with complex64 gathered vectors (FP64 values, sums and
+recurrence).  The matrix is dictionary-coded and its value array released
+(5 bytes per nonzero) so that the Krylov basis fits in HBM next to it.  The
+line reports the build times, seconds per step for both precisions and, per
+step, the site occupations and the electric energy of both runs with their
+difference.
 """
 import argparse
 import json
@@ -90,6 +102,87 @@ def cpu_baseline(H, x_dev, max_seconds=12.0):
                                     f"(N={H.n}, nnz={H.nnz}), {t*1e3:.1f} ms each")
 
 
+def su2_dynamics(args):
+    """1D SU(2) quench dynamics at full size (see the module docstring)."""
+    import torch
+    from edlgt_amd import PREC_MIXED, SU2_Model
+    Ls = args.sites
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[Ls], lvals=[Ls], has_obc=[True])
+    torch.cuda.synchronize()
+    t_sector = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    H = mod.build_Hamiltonian(1.0, 1.0)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    n_dict = H.compress()
+    H.release_values()
+    torch.cuda.synchronize()
+    t_code = time.perf_counter() - t0
+    x = torch.randn(H.n, dtype=torch.complex128, device="cuda")
+    y = torch.empty_like(x)
+    g, grid, ms_tune = H.autotune()
+    ms_spmv = time_launches(lambda: H.matvec(x, y), 10, 2)
+    del x, y
+    torch.cuda.empty_cache()
+    names = ["T2_px"] + [f"N_{l}" for l in ("tot", "single", "pair", "zero")]
+    mod.get_observables(names)
+    psi0 = mod.get_qmb_state_from_configs([mod.overlap_QMB_state("V")])
+    tl = np.arange(args.dyn_steps + 1) * 0.05
+    runs = {}
+    chunk = 20
+    for prec, name in ((0, "fp64"), (PREC_MIXED, "mixed")):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        psi, margs, errs = psi0, [], []
+        for c0 in range(0, args.dyn_steps, chunk):
+            n = min(chunk, args.dyn_steps - c0)
+            mod.time_evolution_Hamiltonian(psi, tl[c0:c0 + n + 1], m=30, reorth=False, store_states=False,
+                                           precision=prec)
+            psi = mod.H.psi_final
+            margs.extend(list(mod.H.marginals[(0 if c0 == 0 else 1):]))
+            errs.extend(list(mod.H.krylov_err))
+            print(f"{name}: {c0 + n}/{args.dyn_steps} steps, {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+                  flush=True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        mod.H.krylov_err = np.asarray(errs)
+        obs = {"E2": [], "N_tot": [], "N_single": [], "N_pair": [], "N_zero": []}
+        even = mod.lattice.even_mask()
+        for k in range(len(tl)):
+            marg = margs[k]
+            v = {n: mod.local_expvals(marg, n) for n in names}
+            obs["E2"].append(float(v["T2_px"][:-1].mean()))
+            obs["N_single"].append(float(v["N_single"].mean()))
+            npair = 0.5 * v["N_pair"][even].mean() + 0.5 * v["N_zero"][~even].mean()
+            obs["N_pair"].append(float(npair))
+            obs["N_zero"].append(float(0.5 * v["N_zero"][even].mean() + 0.5 * v["N_pair"][~even].mean()))
+            obs["N_tot"].append(float(v["N_single"].mean() + 2 * npair))
+        runs[name] = dict(seconds=round(el, 2), s_per_step=round(el / args.dyn_steps, 4),
+                          krylov_err_max=float(mod.H.krylov_err.max()), obs=obs)
+    err = {k: [abs(a - b) for a, b in zip(runs["mixed"]["obs"][k], runs["fp64"]["obs"][k])]
+           for k in runs["fp64"]["obs"]}
+    out = {
+        "metric": "Krylov exp(-iH dt) step time (m=30), 1D SU(2) quench",
+        "value": runs["fp64"]["s_per_step"], "unit": "s/step", "n_gpus": 1, "steps": args.dyn_steps,
+        "warmup": 0, "ms_per_step": round(1e3 * runs["fp64"]["s_per_step"], 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic: bare-vacuum quench built on the GPU from the model",
+        "config": {"workload": f"1D SU(2) j=1/2 with matter, {Ls} sites OBC, N_tot={Ls}, g=1, m=1, "
+                               "m_krylov=30, dt=0.05", "sector_dim": H.n, "nnz": H.nnz},
+        "build": dict(sector_s=round(t_sector, 3), assembly_s=round(t_asm, 3), code_s=round(t_code, 3),
+                      n_dict=n_dict, coded_spmv_ms=round(ms_spmv, 3), coded_variant=-grid - 1),
+        "fp64": {k: v for k, v in runs["fp64"].items() if k != "obs"},
+        "mixed": {k: v for k, v in runs["mixed"].items() if k != "obs"},
+        "max_abs_err_mixed_vs_fp64": {k: max(v) for k, v in err.items()},
+        "per_step": {"t": [round(float(t), 4) for t in tl], "fp64": runs["fp64"]["obs"],
+                     "abs_err_mixed": err},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def serve_leg(args):
     """Served vs plain lowest-eigenvalue solves on the Z3 request stream."""
     import torch
@@ -144,9 +237,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--sites", type=int, default=20)
+    ap.add_argument("--dyn-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--plain-sample", type=int, default=32)
@@ -158,6 +253,8 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded design is "
               "described in DESIGN.md and no multi-GPU path is built.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "su2c3":
+        return su2_dynamics(args)
     import torch
     from edlgt_amd import QED_Model, PREC_MIXED
     from edlgt_amd import _lib as L

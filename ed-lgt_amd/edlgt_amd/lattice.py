@@ -91,5 +91,13 @@ class Lattice:
             return [self.index((x, 0)) for x in range(self.lvals[0])]
         return [self.index((0, y)) for y in range(self.lvals[1])]
 
+    def origin_face(self, axis):
+        """3D: sites of the coordinate plane through the origin orthogonal to
+        `axis` ('x': x = 0, 'y': y = 0, 'z': z = 0), lattice_geometry.py:225-287."""
+        if self.dim != 3:
+            raise NotImplementedError("origin faces are defined for 3D lattices")
+        a = AXES.index(axis)
+        return [i for i in range(self.n_sites) if self.coords(i)[a] == 0]
+
     def even_mask(self):
         return np.array([self.is_even(i) for i in range(self.n_sites)])

@@ -372,7 +372,9 @@ class QED_Model(QuantumModel):
             nbody = (np.stack([diag(f"E_p{d}") for d in self.directions]), np.zeros(self.dim),
                      [self.lattice.origin_line(line_of[d]) for d in self.directions], "U")
         elif self.pure_theory and not any(self.has_obc) and self.dim == 3:
-            raise NotImplementedError("3D pure PBC flux sectors are not supported yet")
+            # electric flux through the three coordinate planes at the origin
+            nbody = (np.stack([diag(f"E_p{d}") for d in self.directions]), np.zeros(3),
+                     [self.lattice.origin_face(d) for d in self.directions], "U")
         self.basis = SectorBasis(self.loc_dims, glob=glob, link=link, nbody=nbody)
         logger.info(f"SEC DIM: {self.basis.n_states}")
         self.default_params()

@@ -102,34 +102,33 @@ def staggered_mask(lvals, label):
 
 
 def border_labels(lattice_dim):
-    """lattice_geometry.py:309-342."""
-    if lattice_dim == 1:
-        return ["mx", "px"]
-    if lattice_dim == 2:
-        return ["mx", "px", "my", "py", "mx_my", "mx_py", "px_my", "px_py"]
-    raise NotImplementedError("oracle covers 1D and 2D lattices")
+    """lattice_geometry.py:309-342: every non-empty set of axes (in x, y, z
+    order) with a lower (m) or upper (p) side per axis, e.g. 'px_my'."""
+    from itertools import combinations, product
+    axes = "xyz"[:lattice_dim]
+    out = []
+    for k in range(1, lattice_dim + 1):
+        for combo in combinations(axes, k):
+            for sides in product("mp", repeat=k):
+                out.append("_".join(f"{sd}{a}" for sd, a in zip(sides, combo)))
+    return out
 
 
 def border_configs(links, lattice_dim, offset=0):
     """Which border subsets a local link configuration belongs to
-    (LGT_border_configs, lattice_geometry.py:345-487); links are the
-    half-link values ordered (m_x, m_y, ..., p_x, p_y, ...)."""
-    lab = []
-    if lattice_dim == 1:
-        if links[0] == offset:
-            lab.append("mx")
-        if links[1] == offset:
-            lab.append("px")
-    elif lattice_dim == 2:
-        mx, my, px, py = [links[i] == offset for i in range(4)]
-        if mx: lab.append("mx")
-        if my: lab.append("my")
-        if px: lab.append("px")
-        if py: lab.append("py")
-        if mx and my: lab.append("mx_my")
-        if mx and py: lab.append("mx_py")
-        if my and px: lab.append("px_my")
-        if px and py: lab.append("px_py")
-    else:
-        raise NotImplementedError
-    return lab
+    (LGT_border_configs, lattice_geometry.py:345-487): a label applies when
+    every half-link it names holds the offset value; links are ordered
+    (m_x, m_y, ..., p_x, p_y, ...)."""
+    pos = {f"m{a}": i for i, a in enumerate("xyz"[:lattice_dim])}
+    pos.update({f"p{a}": lattice_dim + i for i, a in enumerate("xyz"[:lattice_dim])})
+    return [lab for lab in border_labels(lattice_dim)
+            if all(links[pos[part]] == offset for part in lab.split("_"))]
+
+
+def origin_faces_3d(lvals):
+    """Coordinate planes through the origin (lattice_geometry.py:225-287):
+    'yz' (x=0), 'xz' (y=0), 'xy' (z=0) as site lists."""
+    Lx, Ly, Lz = lvals
+    return {"yz": [inverse_zig_zag(lvals, (0, y, z)) for y in range(Ly) for z in range(Lz)],
+            "xz": [inverse_zig_zag(lvals, (x, 0, z)) for x in range(Lx) for z in range(Lz)],
+            "xy": [inverse_zig_zag(lvals, (x, y, 0)) for x in range(Lx) for y in range(Ly)]}

@@ -314,6 +314,12 @@ class QEDOracle:
             nb_diags = np.stack([dg(f"E_p{d}") for d in self.dirs])
             self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, link_sectors,
                                           pairs, nb_diags, np.zeros(self.dim), nb_sites, "U")
+        elif not any(self.has_obc) and self.dim == 3:
+            faces = lat.origin_faces_3d(self.lvals)
+            face_of = {"x": "yz", "y": "xz", "z": "xy"}
+            self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, link_sectors, pairs,
+                                          np.stack([dg(f"E_p{d}") for d in self.dirs]), np.zeros(3),
+                                          [faces[face_of[d]] for d in self.dirs], "U")
         else:
             self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags,
                                           link_sectors, pairs)
@@ -340,8 +346,21 @@ class QEDOracle:
             plaqs = [p for p in (lat.plaquette_sites(lat.zig_zag(self.lvals, i), self.lvals, ["x", "y"],
                                                      self.has_obc) for i in range(self.n_sites)) if p]
             add_nbody_list(T, [self.ops[n] for n in names], plaqs, *args, c["B"], True)
-            if abs(c["theta"]) > 10e-10:
+            if self.dim == 3:
+                for ax, nm in ((["x", "z"], ["C_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"]),
+                               (["y", "z"], ["C_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"])):
+                    pl = [p for p in (lat.plaquette_sites(lat.zig_zag(self.lvals, i), self.lvals, ax,
+                                                          self.has_obc) for i in range(self.n_sites)) if p]
+                    add_nbody_list(T, [self.ops[n] for n in nm], pl, *args, c["B"], True)
+            if self.dim == 2 and abs(c["theta"]) > 10e-10:
                 add_nbody_list(T, [self.ops[n] for n in names], plaqs, *args, c["theta"], True)
+            if self.dim == 3 and abs(c["theta"]) > 10e-10:
+                for ax, nm in ((["x", "y"], ["EzC_px,py", "C_py,mx", "C_my,px", "C_mx,my"]),
+                               (["x", "z"], ["EyC_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"]),
+                               (["y", "z"], ["ExC_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"])):
+                    pl = [p for p in (lat.plaquette_sites(lat.zig_zag(self.lvals, i), self.lvals, ax,
+                                                          self.has_obc) for i in range(self.n_sites)) if p]
+                    add_nbody_list(T, [self.ops[n] for n in nm], pl, *args, c["theta"], True)
         if not self.pure:
             for stag in ["even", "odd"]:
                 mask = lat.staggered_mask(self.lvals, stag)

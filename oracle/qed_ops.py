@@ -59,7 +59,7 @@ def fermi_operators():
 
 
 def dressed_site_operators(spin, pure_theory, lattice_dim):
-    """QED_dressed_site_operators for 1D/2D (QED_operators.py:120-330)."""
+    """QED_dressed_site_operators for 1D/2D/3D (QED_operators.py:120-330)."""
     r = rishon_operators(spin)
     in_ops = dict(r)
     if not pure_theory:
@@ -100,8 +100,49 @@ def dressed_site_operators(spin, pure_theory, lattice_dim):
                 ops[k[:-4]] = csr_matrix(ops[k].conj().transpose())
             ops["N"] = _kron_list(in_ops, ["N", "Iz", "Iz", "Iz", "Iz"])
             ops["N_zero"] = _kron_list(in_ops, ["N_zero", "Iz", "Iz", "Iz", "Iz"])
+    elif lattice_dim == 3:
+        names6 = ["mx", "my", "mz", "px", "py", "pz"]
+        for op in ["E", "E2", "P"]:
+            for pos, name in enumerate(names6):
+                lst = ["Iz"] * 6
+                lst[pos] = op
+                ops[f"{op}_{name}"] = _kron_list(in_ops, lst)
+        corners = {
+            "C_px,py": (-1, ["Iz", "Iz", "Iz", "Zp_P", "Zp_dag", "Iz"]),
+            "C_py,mx": (1, ["P_Zm_dag", "P", "P", "P", "Zp", "Iz"]),
+            "C_mx,my": (1, ["Zm_P", "Zm_dag", "Iz", "Iz", "Iz", "Iz"]),
+            "C_my,px": (1, ["Iz", "Zm_P", "P", "Zp_dag", "Iz", "Iz"]),
+            "C_px,pz": (-1, ["Iz", "Iz", "Iz", "Zp_P", "P", "Zp_dag"]),
+            "C_pz,mx": (1, ["P_Zm_dag", "P", "P", "P", "P", "Zp"]),
+            "C_mx,mz": (1, ["Zm_P", "P", "Zm_dag", "Iz", "Iz", "Iz"]),
+            "C_mz,px": (1, ["Iz", "Iz", "Zm_P", "Zp_dag", "Iz", "Iz"]),
+            "C_py,pz": (-1, ["Iz", "Iz", "Iz", "Iz", "Zp_P", "Zp_dag"]),
+            "C_pz,my": (1, ["Iz", "P_Zm_dag", "P", "P", "P", "Zp"]),
+            "C_my,mz": (1, ["Iz", "Zm_P", "Zm_dag", "Iz", "Iz", "Iz"]),
+            "C_mz,py": (1, ["Iz", "Iz", "Zm_P", "P", "Zp_dag", "Iz"]),
+        }
+        for name, (sg, lst) in corners.items():
+            ops[name] = csr_matrix(sg * _kron_list(in_ops, lst))
+        ops["EzC_px,py"] = csr_matrix(1j * (ops["E_pz"] + ops["E_mz"]) @ ops["C_px,py"])
+        ops["EyC_px,pz"] = csr_matrix(-1j * (ops["E_py"] + ops["E_my"]) @ ops["C_px,pz"])
+        ops["ExC_py,pz"] = csr_matrix(1j * (ops["E_px"] + ops["E_mx"]) @ ops["C_py,pz"])
+        if not pure_theory:
+            for k in list(ops):
+                ops[k] = csr_matrix(kron(in_ops["ID_psi"], ops[k]))
+            hops = {"Q_mx_dag": ["psi_dag", "Zm", "Iz", "Iz", "Iz", "Iz", "Iz"],
+                    "Q_my_dag": ["psi_dag", "P", "Zm", "Iz", "Iz", "Iz", "Iz"],
+                    "Q_mz_dag": ["psi_dag", "P", "P", "Zm", "Iz", "Iz", "Iz"],
+                    "Q_px_dag": ["psi_dag", "P", "P", "P", "Zp", "Iz", "Iz"],
+                    "Q_py_dag": ["psi_dag", "P", "P", "P", "P", "Zp", "Iz"],
+                    "Q_pz_dag": ["psi_dag", "P", "P", "P", "P", "P", "Zp"]}
+            for k, lst in hops.items():
+                ops[k] = _kron_list(in_ops, lst)
+            for k in [k for k in ops if k.endswith("_dag")]:
+                ops[k[:-4]] = csr_matrix(ops[k].conj().transpose())
+            ops["N"] = _kron_list(in_ops, ["N"] + ["Iz"] * 6)
+            ops["N_zero"] = _kron_list(in_ops, ["N_zero"] + ["Iz"] * 6)
     else:
-        raise NotImplementedError("oracle covers 1D and 2D QED")
+        raise NotImplementedError("oracle covers 1D, 2D and 3D QED")
     E2 = 0
     for d in "xyz"[:lattice_dim]:
         for s in "mp":

@@ -119,3 +119,19 @@ def test_lookup_and_initial_state(gpu):
         mod.basis.lookup(bad)
     psi = mod.get_qmb_state_from_configs(cfg[[1, 2]])
     assert abs(float(psi.abs().pow(2).sum()) - 1.0) < 1e-14
+
+
+def test_qed_3d_theta_sector_csr_energy(gpu):
+    """3D pure PBC with the theta term (complex H): sector and CSR against the
+    oracle, energy against the reference's KAT."""
+    from edlgt_amd import QED_Model
+    print("QED 2x2x2 PBC pure theta", flush=True)
+    orc = QEDOracle(1, True, [2, 2, 2], [False, False, False])
+    mod = QED_Model(spin=1, pure_theory=True, lvals=[2, 2, 2], has_obc=[False, False, False])
+    assert mod.sector_dim == orc.N
+    assert np.array_equal(mod.sector_configs, orc.configs)
+    Hg = mod.build_Hamiltonian(2.3, theta=0.41)
+    Ho = orc.hamiltonian(2.3, theta=0.41)
+    _csr_match(Hg, Ho)
+    mod.diagonalize_Hamiltonian(1, "sparse", tol=1e-12)
+    assert abs(mod.H.Nenergies[0] - (-0.37555632787741716)) < 1e-10

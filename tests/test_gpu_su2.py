@@ -90,3 +90,17 @@ def test_su2_gold_dynamics(gpu):
     mod.time_evolution_Hamiltonian(psi0, t, precision=PREC_MIXED)
     E2m = np.array([_obs(mod, i, dynamics=True)["E2"] for i in range(len(t))])
     assert np.max(np.abs(E2m - E2)) < 1e-6
+
+
+def test_su2_krylov_without_full_reorth(gpu):
+    """The three-term Krylov recurrence (no full reorthogonalisation, used for
+    the 20-site quench) against expm_multiply on a 10-site OBC chain."""
+    orc = SU2Oracle(False, [10], [True], sectors=[10])
+    mod = _model([10], [True], False, 10)
+    mod.build_Hamiltonian(1.0, 1.0)
+    psi0 = mod.get_qmb_state_from_configs([mod.overlap_QMB_state("V")])
+    t = np.arange(21) * 0.05
+    mod.time_evolution_Hamiltonian(psi0, t, m=30, reorth=False)
+    path = expm_path(orc.hamiltonian(1.0, 1.0), psi0.cpu().numpy(), t[0], t[-1], len(t))
+    got = np.stack([s.psi for s in mod.H.psi_time])
+    assert np.max(np.abs(got - path)) < 1e-8

@@ -37,3 +37,11 @@ def test_qed_2d_pbc_kat_and_c2_dimension():
 
 def test_c1_dimension():
     assert QEDOracle(1, False, [10], [True]).N == 197
+
+
+def test_qed_3d_theta_kat():
+    """validation/qed/test_qed_3d_theta.py: 2x2x2 pure PBC (flux through the
+    three coordinate planes at the origin fixed to 0), g=2.3, theta=0.41."""
+    M = QEDOracle(1, True, [2, 2, 2], [False, False, False])
+    e = lowest_eigs(M.hamiltonian(2.3, theta=0.41), 1)[0] / 8
+    assert abs(e - (-0.37555632787741716)) < 1e-10
