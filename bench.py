@@ -63,6 +63,10 @@ WORKLOADS = {
 }
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def spmv_bytes(n, nnz, s_v=16, s_x=16):
     return nnz * (s_v + 4) + (n + 1) * 8 + n * (s_x + 16)
 
@@ -183,6 +187,38 @@ def su2_dynamics(args):
     print(json.dumps(out), flush=True)
 
 
+def c2_sweep():
+    """QED 4x2 PBC with matter (N = 140,647), m = 0.5, g in {0.5, 1, 2}, lowest 4
+    eigenvalues: one build + FP64 Lanczos per coupling (plain) vs one
+    multi-term assembly solved for all couplings by one server (batched)."""
+    import torch
+    from edlgt_amd import QED_Model, Server
+    gs = [0.5, 1.0, 2.0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mod = QED_Model(spin=1, pure_theory=False, lvals=[4, 2], has_obc=[False, False])
+    torch.cuda.synchronize()
+    t_sector = time.perf_counter() - t0
+    plain, t0 = {}, time.perf_counter()
+    for g in gs:
+        H = mod.build_Hamiltonian(g, 0.5)
+        ev, _, _, _ = H.lanczos(4, tol=1e-10, want_vectors=False)
+        plain[g] = ev
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    H3 = mod.build_multi_term(0.5)
+    res = Server(H3, n_eigs=4, tol=1e-10, max_basis=64, max_slots=3).solve_all(
+        [(i, mod.multi_term_coefficients(g), 7 + i) for i, g in enumerate(gs)])
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    return dict(workload="QED 4x2 PBC with matter, m=0.5, g in {0.5,1,2}, lowest 4 eigenvalues, tol 1e-10",
+                sector_dim=mod.sector_dim, sector_s=round(t_sector, 3), plain_s=round(t_plain, 3),
+                batched_s=round(t_batch, 3),
+                E0_per_site={str(g): float(plain[g][0] / 8) for g in gs},
+                max_dE_batched_vs_plain=max(float(np.max(np.abs(res[i][0] - plain[g]))) for i, g in enumerate(gs)))
+
+
 def serve_leg(args):
     """Served vs plain lowest-eigenvalue solves on the Z3 request stream."""
     import torch
@@ -215,6 +251,7 @@ def serve_leg(args):
         return ev
 
     sample = sorted(reqs, key=lambda r: r["arrival"])[:args.plain_sample]
+    progress(f"served {served['requests']} requests in {served['wall_s']:.1f}s; plain sample")
     pres = serve_plain(solve_one, sample)
     plain = latency_summary(pres, serve_plain.wall)
     busy = sum(r["done"] - r["admitted"] for r in pres.values())
@@ -260,6 +297,7 @@ def main():
     from edlgt_amd import _lib as L
 
     wl = WORKLOADS[args.workload]
+    progress(f"building {wl['desc']}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     model = QED_Model(spin=1, pure_theory=False, lvals=wl["lvals"], has_obc=[False, False])
@@ -271,6 +309,7 @@ def main():
     t_assembly = time.perf_counter() - t0
     N, nnz = H.n, H.nnz
 
+    progress(f"N={H.n} nnz={H.nnz}: sector {t_sector:.2f}s assembly {t_assembly:.2f}s; autotuning")
     # launch-shape autotuner (lanes per row x workgroups), timed on the device
     tune = H.autotune(n_rhs=1)
     tune32 = H.autotune(n_rhs=1, precision=L.PREC_MIXED)
@@ -303,6 +342,7 @@ def main():
     H.set_launch(*tune[:2])
     gbs32 = spmv_bytes(N, nnz, s_v=8, s_x=8) / (ms32 * 1e-3) / 1e9
 
+    progress(f"SpMV {ms:.3f} ms ({gbs:.0f} GB/s); mixed {ms32:.3f} ms; solving")
     extra = {}
     if not args.no_solve:
         torch.cuda.synchronize()
@@ -325,7 +365,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        progress("CPU baseline")
         cpu = cpu_baseline(H, x)
+    progress("coded SpMV")
 
     # dictionary-coded values: same products bit for bit, 5 bytes per entry
     y_ref = y.clone()
@@ -351,7 +393,10 @@ def main():
     torch.cuda.empty_cache()
 
     serving = None
+    progress("C2 coupling sweep")
+    sweep = c2_sweep()
     if not args.no_serve:
+        progress("serving leg")
         serving = serve_leg(args)
 
     traffic = None
@@ -382,6 +427,7 @@ def main():
                       spmv_mixed_launch={"lanes_per_row": tune32[0], "workgroups": tune32[1]},
                       spmv_mixed_ms=round(ms32, 4), spmv_mixed_GBs=round(gbs32, 2),
                       pct_hbm_peak=round(100 * gbs / HBM_PEAK_GBS, 2), coded=coded, **extra),
+        "c2_sweep": sweep,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -937,6 +937,10 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   // Block size: up to 4 so that small degeneracies are resolved; a basis
   // that can hold the whole space uses single vectors and runs to exhaustion.
   int b = std::min(k, 4);
+  // max_basis <= 0: ARPACK's default basis size ncv = max(2k + 1, 20), the
+  // size the reference's eigsh uses (measured the faster choice for a few
+  // eigenvalues: the reorthogonalisation cost grows with the basis)
+  if (max_basis <= 0) max_basis = std::max(2 * k + 1, 20);
   int M = std::max(max_basis, 2 * k + 2 * b);
   M = (M + b - 1) / b * b;
   if (N <= 512 && M >= N) {

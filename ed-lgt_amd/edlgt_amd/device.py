@@ -293,9 +293,10 @@ class CSRMatrix:
     def set_launch(self, lanes_per_row=0, grid=0):
         L.check(L.load().edlgt_csr_set_launch(self._h, lanes_per_row, grid))
 
-    def lanczos(self, n_eigs, tol=1e-12, max_basis=64, max_iter=100000, precision=L.PREC_FP64,
+    def lanczos(self, n_eigs, tol=1e-12, max_basis=0, max_iter=100000, precision=L.PREC_FP64,
                 v0=None, complex_vectors=None, want_vectors=True, stream=None):
-        """Lowest eigenpairs: (evals ascending, evecs tensor (k, N) or None, resid, matvecs)."""
+        """Lowest eigenpairs: (evals ascending, evecs tensor (k, N) or None, resid, matvecs).
+        max_basis 0: eigsh's default basis size max(2k + 1, 20)."""
         torch = _torch()
         vt = self.vec_type(complex_vectors)
         if v0 is not None:

@@ -72,7 +72,7 @@ class QMB_hamiltonian:
         self.nnz = csr.nnz
 
     def diagonalize(self, n_eigs, format, loc_dims, print_results=False, tol=1e-12,
-                    max_basis=64, precision=L.PREC_FP64, v0=None):
+                    max_basis=0, precision=L.PREC_FP64, v0=None):
         """Lowest n_eigs eigenpairs; Nenergies are energies per site
         (qmb_hamiltonian.py:260-332)."""
         if isinstance(n_eigs, str):
@@ -432,6 +432,61 @@ class QED_Model(QuantumModel):
                     bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
                     self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
         return self._build_program(real)
+
+
+def _qed_multi_term(self, m=None):
+    """(tables, placements with term ids) of H(g) = (g/2) H_E + (-1/(2g)) H_B + H_0
+    for a g sweep at fixed m: term 0 = E2, term 1 = xy plaquettes (+ h.c.; the
+    3D planes too), term 2 = mass + hopping at mass m (empty without matter).
+    Coefficients of a coupling g: QED_Model.multi_term_coefficients(g)."""
+    lat = self.lattice
+    self._new_program()
+    self._add_local("E2", 1.0)
+    for p in self._placements:
+        p["term"] = 0
+    k0 = len(self._placements)
+    if self.dim > 1:
+        self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], lat.plaquettes(("x", "y")), 1.0, True)
+    if self.dim == 3:
+        self._add_nbody(["C_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], lat.plaquettes(("x", "z")), 1.0, True)
+        self._add_nbody(["C_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], lat.plaquettes(("y", "z")), 1.0, True)
+    for p in self._placements[k0:]:
+        p["term"] = 1
+    k1 = len(self._placements)
+    if not self.pure_theory:
+        if m is None:
+            raise ValueError("m is required with matter fields")
+        self.QED_Hamiltonian_couplings(1.0, m)
+        c = self.coeffs
+        even = lat.even_mask()
+        self._add_local("N", c["m_even"], [i for i in range(self.n_sites) if even[i]])
+        self._add_local("N", c["m_odd"], [i for i in range(self.n_sites) if not even[i]])
+        for d in self.directions:
+            for stag in ("even", "odd"):
+                bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
+                self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
+    for p in self._placements[k1:]:
+        p["term"] = 2
+    return np.stack(self._tables), list(self._placements)
+
+
+def _qed_build_multi_term(self, m=None, dtype_mode="auto"):
+    """Assembles the three coupling-independent parts of H(g) once on one
+    pattern; the matrix at coupling g is sum_t multi_term_coefficients(g)[t] H_t
+    (products: CSRMatrix.matvec_coef, Server)."""
+    tables, placements = self.multi_term_program(m)
+    real = self._dtype_real(0.0, dtype_mode)
+    return CSRMatrix(self.basis, tables, placements, L.VALUE_F64 if real else L.VALUE_C128, n_terms=3)
+
+
+def _qed_coefficients(self, g):
+    """Term coefficients of H(g) (QED_Hamiltonian_couplings, QED_model.py:719-776)."""
+    return [g / 2, -1 / (2 * g), 1.0]
+
+
+QED_Model.multi_term_program = _qed_multi_term
+QED_Model.build_multi_term = _qed_build_multi_term
+QED_Model.multi_term_coefficients = _qed_coefficients
 
 
 class Zn_Model(QuantumModel):

@@ -233,7 +233,8 @@ int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid);
  * (edlgt/modeling/qmb_hamiltonian.py:260-332).
  *   v0_host      start vector (host, vec_type, length n); NULL -> all ones
  *   tol          stop when every wanted residual ||H y - theta y|| <= tol * max(1,|theta|)
- *   max_basis    Krylov basis size m (>= n_eigs + 2)
+ *   max_basis    Krylov basis size m (>= n_eigs + 2); <= 0 selects
+ *                max(2 n_eigs + 1, 20), eigsh's default ncv
  *   max_iter     total matvec budget
  *   precision    EDLGT_PREC_FP64, or EDLGT_PREC_MIXED (FP32 SpMV, then FP64
  *                refinement restarted from the FP32 Ritz vectors)

@@ -129,3 +129,26 @@ def test_server_warm_start_matches_cold(gpu):
     again = srv.solve_all([(99, [1.0, 1.1], 7)])
     ref = lowest_eigs(orc.hamiltonian(1.0, 1.1), 3)
     assert np.max(np.abs(again[99][0] - ref)) < 1e-8 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_qed_c2_g_sweep_multi_term(gpu):
+    """QED 4x2 PBC with matter (N = 140,647), m = 0.5, g in {0.5, 1, 2}: one
+    multi-term assembly; at every g the summed matrix has the oracle's pattern
+    and values, and the served lowest 4 eigenvalues match the oracle."""
+    from edlgt_amd import QED_Model, Server
+    from oracle.model import QEDOracle
+    orc = QEDOracle(1, False, [4, 2], [False, False])
+    mod = QED_Model(spin=1, pure_theory=False, lvals=[4, 2], has_obc=[False, False])
+    H3 = mod.build_multi_term(0.5)
+    assert H3.n_terms == 3 and H3.n == 140647
+    gs = [0.5, 1.0, 2.0]
+    for g in gs:
+        Ho = orc.hamiltonian(g, 0.5)
+        Hm = H3.to_scipy(coef=mod.multi_term_coefficients(g))
+        assert np.array_equal(Hm.indptr, Ho.indptr) and np.array_equal(Hm.indices, Ho.indices)
+        assert np.max(np.abs(Hm.data - Ho.data)) <= 1e-14 * np.max(np.abs(Ho.data))
+    srv = Server(H3, n_eigs=4, tol=1e-10, max_basis=64, max_slots=3)
+    res = srv.solve_all([(i, mod.multi_term_coefficients(g), 7 + i) for i, g in enumerate(gs)])
+    for i, g in enumerate(gs):
+        ref = lowest_eigs(orc.hamiltonian(g, 0.5), 4)
+        assert np.max(np.abs(res[i][0] - ref)) < 1e-9
