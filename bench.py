@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload qed4x3|qed4x2] [--no-cpu]
                     [--serve-requests 512] [--serve-rate 20] [--plain-sample 32] [--no-serve]
     python bench.py --workload su2c3 [--sites 20] [--dyn-steps 200]
+    python bench.py --workload cpu_qed4x3
 
 Workload (default): 2D U(1) QED, 4x3 periodic lattice, staggered fermions,
 E in {-1,0,1}, g=1, m=0.5, Gauss-law + zero-charge sector (N = 28,904,615).
@@ -42,6 +43,11 @@ recurrence).  The matrix is dictionary-coded and its value array released
 line reports the build times, seconds per step for both precisions and, per
 step, the site occupations and the electric energy of both runs with their
 difference.
+
+Workload cpu_qed4x3: the full build + lowest-eigenvalue path of C5 on the
+host cores (oracle/refcpu: C/OpenMP sector enumeration and two-pass triplets,
+SciPy coalescing into CSR, eigsh on the C SpMV), beside the same path on the
+GPU, for the build+E0 ratio.
 """
 import argparse
 import json
@@ -104,6 +110,39 @@ def cpu_baseline(H, x_dev, max_seconds=12.0):
     return dict(value=round(spmv_bytes(H.n, H.nnz) / t / 1e9, 2), unit="GB/s", cores=refcpu.threads(),
                 kind="port", sample=f"{reps} full complex128 CSR SpMV passes over the same matrix "
                                     f"(N={H.n}, nnz={H.nnz}), {t*1e3:.1f} ms each")
+
+
+def cpu_build_e0(args):
+    """C5 build + E0: host cores (refcpu restatement) vs GPU."""
+    import torch
+    from edlgt_amd import QED_Model
+    from oracle import refcpu
+    progress("GPU build + E0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model = QED_Model(spin=1, pure_theory=False, lvals=[4, 3], has_obc=[False, False])
+    model.build_Hamiltonian(1.0, 0.5)
+    model.diagonalize_Hamiltonian(1, "sparse")
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    e_gpu = float(model.H.energies[0])
+    del model
+    torch.cuda.empty_cache()
+    progress(f"GPU {t_gpu:.2f}s; CPU path on {refcpu.threads()} threads")
+    t0 = time.perf_counter()
+    e_cpu, N, nnz, T = refcpu.qed_build_and_e0([4, 3], [False, False], 1.0, 0.5, log=progress)
+    t_cpu = time.perf_counter() - t0
+    out = {"metric": "C5 build + lowest eigenvalue, wall time", "value": round(t_gpu, 3), "unit": "s",
+           "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(1e3 * t_gpu, 1),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic: matrix built from the model", "config": {"workload": WORKLOADS["qed4x3"]["desc"],
+                                                                      "sector_dim": N, "nnz": nnz},
+           "gpu": {"build_plus_E0_s": round(t_gpu, 3), "E0": e_gpu},
+           "cpu_baseline": {"kind": "port", "cores": refcpu.threads(), "build_plus_E0_s": round(t_cpu, 2),
+                            "phases_s": {k: round(v, 2) for k, v in T.items()}, "E0": e_cpu,
+                            "sample": "full C5 path once: sector, triplets, SciPy CSR, eigsh(k=1, SA)"},
+           "speedup": round(t_cpu / t_gpu, 1), "dE0_gpu_vs_cpu": abs(e_gpu - e_cpu)}
+    print(json.dumps(out), flush=True)
 
 
 def su2_dynamics(args):
@@ -274,7 +313,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3"])
+    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3", "cpu_qed4x3"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--sites", type=int, default=20)
@@ -292,6 +331,8 @@ def main():
         sys.exit(2)
     if args.workload == "su2c3":
         return su2_dynamics(args)
+    if args.workload == "cpu_qed4x3":
+        return cpu_build_e0(args)
     import torch
     from edlgt_amd import QED_Model, PREC_MIXED
     from edlgt_amd import _lib as L

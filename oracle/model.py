@@ -284,7 +284,7 @@ class QEDOracle:
     """QED_Model (truncated spin, ladder U, no background) restated
     (models/QED_model.py:23-233, 235-394, 719-776)."""
 
-    def __init__(self, spin, pure_theory, lvals, has_obc):
+    def __init__(self, spin, pure_theory, lvals, has_obc, enumerate_sector=True):
         self.spin, self.pure = spin, pure_theory
         self.lvals, self.has_obc = list(lvals), list(has_obc)
         self.dim = len(lvals)
@@ -303,6 +303,9 @@ class QEDOracle:
         pairs = lat.link_pairs(self.lvals, self.has_obc)
         link_diags = np.stack([np.stack([dg(f"E_p{d}"), -dg(f"E_m{d}")]) for d in self.dirs])
         link_sectors = np.zeros(self.dim)
+        self.pairs, self.link_diags = pairs, link_diags
+        if not enumerate_sector:
+            return
         if not pure_theory:
             glob = dg("N")[None]
             self.configs = sector_configs(self.loc_dims, glob, np.array([self.n_sites // 2], float),

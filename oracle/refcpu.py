@@ -125,3 +125,116 @@ def nbody(configs, op_stack, sites, op_tol=1e-10):
                                  kets.ctypes.data, vals.ctypes.data, d_max, offsets.ctypes.data, rows.ctypes.data,
                                  cols.ctypes.data, out.ctypes.data)
     return rows, cols, out, int(miss)
+
+
+def qed_build_and_e0(lvals, has_obc, g, m, log=None):
+    """The reference's build + lowest-eigenvalue path for QED with matter on
+    the host cores, through this C/OpenMP restatement of its kernels:
+    sector enumeration (symmetry_sector.py:785-967), two-pass triplets with
+    binary search per term placement in the reference's term order
+    (sym_qmb_operations.py:95-602, QED_model.py:235-394), SciPy coalescing of
+    the triplets into CSR (qmb_hamiltonian.py:173-225; done per row block),
+    and eigsh(which="SA", k=1) on a LinearOperator over the C SpMV
+    (qmb_hamiltonian.py:260-332, ham_format="linear").  Returns (E0, N, nnz,
+    timings)."""
+    from time import perf_counter
+
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import LinearOperator, eigsh
+
+    from . import lattice as lat
+    from .model import OP_TOL, QEDOracle
+
+    say = log or (lambda msg: None)
+    T = {}
+    t0 = perf_counter()
+    orc = QEDOracle(1, False, lvals, has_obc, enumerate_sector=False)
+    dg = lambda name: np.real(np.einsum("sii->si", orc.ops[name]))
+    cfg = sector(orc.loc_dims, dg("N")[None], np.array([orc.n_sites // 2], float), orc.link_diags,
+                 np.zeros(orc.dim), orc.pairs)
+    N = cfg.shape[0]
+    T["sector_s"] = perf_counter() - t0
+    say(f"sector N={N} in {T['sector_s']:.1f}s")
+    c = orc.couplings(g, m)
+    blocks = []
+
+    def local(name, strength, mask=None):
+        for s_ in range(orc.n_sites):
+            if mask is not None and not mask[s_]:
+                continue
+            v = orc.ops[name][s_][cfg[:, s_], cfg[:, s_]]
+            keep = np.nonzero(np.abs(v) >= OP_TOL)[0].astype(np.int32)
+            blocks.append((keep, keep, v[keep] * strength))
+
+    def nbody_list(names, site_lists, strength, dagger):
+        stack = np.stack([orc.ops[n] for n in names])
+        for sites in site_lists:
+            r, cc, v, miss = nbody(cfg, stack, sites)
+            if miss:
+                raise RuntimeError("lookup miss")
+            blocks.append((r, cc, v * strength))
+            if dagger:
+                blocks.append((cc, r, np.conj(v * strength)))
+        say(f"term {names[0]}.. on {len(site_lists)} placements done ({perf_counter() - t0:.1f}s)")
+
+    t0 = perf_counter()
+    local("E2", c["E"])
+    plaqs = [p for p in (lat.plaquette_sites(lat.zig_zag(orc.lvals, i), orc.lvals, ["x", "y"], orc.has_obc)
+                         for i in range(orc.n_sites)) if p]
+    nbody_list(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], plaqs, c["B"], True)
+    for stag in ("even", "odd"):
+        mask = lat.staggered_mask(orc.lvals, stag)
+        local("N", c[f"m_{stag}"], [mask[lat.zig_zag(orc.lvals, i)] for i in range(orc.n_sites)])
+    for d in orc.dirs:
+        for stag in ("even", "odd"):
+            mask = lat.staggered_mask(orc.lvals, stag)
+            bonds = []
+            for i in range(orc.n_sites):
+                co = lat.zig_zag(orc.lvals, i)
+                b = lat.neighbor_sites(co, orc.lvals, d, orc.has_obc)
+                if b is not None and mask[co]:
+                    bonds.append(b)
+            nbody_list([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
+    T["triplets_s"] = perf_counter() - t0
+    raw = sum(b[0].size for b in blocks)
+    say(f"{raw} triplets in {T['triplets_s']:.1f}s")
+    # coalesce per row block (same result as one SciPy call over all triplets)
+    t0 = perf_counter()
+    n_chunks = 8
+    edges = np.linspace(0, N, n_chunks + 1).astype(np.int64)
+    parts = []
+    for k in range(n_chunks):
+        lo, hi = edges[k], edges[k + 1]
+        rs, cs, vs = [], [], []
+        for r, cc, v in blocks:
+            sel = np.nonzero((r >= lo) & (r < hi))[0]
+            rs.append(r[sel] - lo); cs.append(cc[sel]); vs.append(v[sel])
+        part = sp.coo_matrix((np.concatenate(vs), (np.concatenate(rs), np.concatenate(cs))),
+                             shape=(hi - lo, N)).tocsr()
+        part.sum_duplicates()
+        part.sort_indices()
+        parts.append(part)
+        say(f"CSR rows {lo}..{hi} done ({perf_counter() - t0:.1f}s)")
+    blocks.clear()
+    H = sp.vstack(parts, format="csr")
+    T["csr_s"] = perf_counter() - t0
+    nnz = H.nnz
+    indptr = H.indptr.astype(np.int64)
+    indices = H.indices.astype(np.int32)
+    data = H.data.astype(np.complex128)
+    del H, parts
+    say(f"nnz={nnz}; eigsh")
+    t0 = perf_counter()
+    y = np.empty(N, dtype=np.complex128)
+
+    def mv(x):
+        x = np.ascontiguousarray(x, dtype=np.complex128).ravel()
+        spmv(indptr, indices, data, x, y)
+        return y.copy()
+
+    op = LinearOperator((N, N), matvec=mv, dtype=np.complex128)
+    v0 = np.random.default_rng(0).random(N).astype(np.complex128)
+    w = eigsh(op, k=1, which="SA", v0=v0 / np.linalg.norm(v0), return_eigenvectors=False)
+    T["eigsh_s"] = perf_counter() - t0
+    say(f"E0={w[0]} in {T['eigsh_s']:.1f}s")
+    return float(np.min(w)), N, nnz, T

@@ -63,3 +63,12 @@ def test_nbody_triplets_match_oracle(qed):
         assert miss == 0
         assert np.array_equal(r, ro) and np.array_equal(c, co)
         assert np.max(np.abs(v - vo), initial=0.0) == 0.0
+
+
+def test_cpu_build_and_e0_matches_kat_and_oracle():
+    """The host baseline's full path (sector, triplets, CSR, eigsh) on the
+    reference's 4x2 PBC KAT (validation/qed/test_qed_2d_matter.py:43-74)."""
+    e0, N, nnz, T = refcpu.qed_build_and_e0([4, 2], [False, False], 1.0, 1.0)
+    assert N == 140647
+    assert nnz == QEDOracle(1, False, [4, 2], [False, False]).hamiltonian(1.0, 1.0).nnz
+    assert abs(e0 / 8 - (-0.8830087027925408)) < 1e-10
