@@ -95,6 +95,11 @@ def load(path=None):
             f"libedlgt_hip.so not found at {p}; build it with `python ed-lgt_amd/build.py` "
             "(the engine has no CPU fallback)"
         )
+    # torch ships its own libamdhip64 with the same soname: loading it first makes
+    # the dynamic linker bind this library to that one runtime.  The other order
+    # maps two HIP runtimes into the process, and torch then finds no device and
+    # cannot share streams or pointers with the engine.
+    import torch  # noqa: F401
     lib = C.CDLL(str(p))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -41,3 +41,17 @@ def test_last_error_roundtrip_without_gpu():
                                 0, 0, None, None, None, None, None, ctypes.byref(h))
     assert st == _lib.ERR_INVALID_ARG
     assert "null" in _lib.last_error() or "n_sites" in _lib.last_error()
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """Loading the engine before torch still maps a single HIP runtime (two
+    runtimes in one process leave torch without a device)."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from edlgt_amd import _lib\n_lib.load()\nimport torch\n"
+            "print(len({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}))\n"
+            % str(ROOT / "ed-lgt_amd"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "1"
