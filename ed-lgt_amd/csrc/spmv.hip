@@ -839,13 +839,16 @@ int coef_lanes(const Csr& h) {
 
 int coef_pad(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : 16; }
 
-// Whether the multi-RHS kernel (row-interleaved right-hand sides) is used:
-// always for uncoded matrices; for coded ones only while a full 16-wide packed
-// block stays within a cache-sized footprint, otherwise every right-hand side
-// takes its own stream pass (the gathered vector then stays in the MALL).
-// The choice depends on the matrix alone, so a column's arithmetic does not
-// depend on the batch it is in.
+// Whether the whole batch goes through one multi-RHS launch (row-interleaved
+// right-hand sides): always for uncoded matrices; for coded ones while a full
+// 16-wide packed block stays within a cache-sized footprint.  Larger coded
+// matrices take groups of kGroupRhs columns per launch, zero-padded: a wider
+// packed row would turn every gather into whole cache lines of a vector far
+// larger than the MALL, while 4 columns still share one line per gather.
+// Either way a column's arithmetic (kernel, lanes per row, summation order)
+// depends on the matrix alone, never on the batch it is in.
 constexpr size_t kFuseBytes = size_t(64) << 20;
+constexpr int kGroupRhs = 4;
 bool coef_fused(const Csr& h, int vec_type) {
   if (h.n_dict == 0) return true;
   const size_t es = vec_type == EDLGT_VALUE_C128 ? 16 : 8;
@@ -853,8 +856,8 @@ bool coef_fused(const Csr& h, int vec_type) {
 }
 
 static size_t coef_x_bytes(const Csr& h, int vec_type, int nb) {
-  if (!coef_fused(h, vec_type)) return 0;
-  const size_t b = (size_t)h.n_rows * coef_pad(nb) * (vec_type == EDLGT_VALUE_C128 ? 16 : 8);
+  const int w = coef_fused(h, vec_type) ? coef_pad(nb) : kGroupRhs;
+  const size_t b = (size_t)h.n_rows * w * (vec_type == EDLGT_VALUE_C128 ? 16 : 8);
   return (b + 255) / 256 * 256;
 }
 
@@ -886,24 +889,33 @@ int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, 
       hipLaunchKernelGGL(scale_dict<double>, dim3(tg), dim3(256), 0, st, h.dict_val.as<double2>(),
                          h.dict_term.as<int32_t>(), h.n_dict, coef, nb, (double*)tab);
     EDLGT_HIP(hipGetLastError());
-    if (!coef_fused(h, vec_type)) {
-      const size_t te = cplx ? 16 : 8;
-      for (int b = 0; b < nb; ++b)
-        EDLGT_TRY(spmv_coded(h, tab + (size_t)b * h.n_dict * te, vec_type, hx[b], hy[b], st));
-      return EDLGT_OK;
-    }
+    const bool fused = coef_fused(h, vec_type);
+    const int gw = fused ? nb : kGroupRhs;  // columns per launch
+    const size_t te = cplx ? 16 : 8;
+    for (int g0 = 0; g0 < nb; g0 += gw) {
+      const int nbg = std::min(gw, nb - g0);
+      const int wp = fused ? nbp : kGroupRhs;
+      const char* tg = tab + (size_t)g0 * h.n_dict * te;
+      const void* xg = static_cast<const char*>(xs) + (size_t)g0 * sizeof(void*);
+      void* yg = static_cast<char*>(ys) + (size_t)g0 * sizeof(void*);
 #define EDLGT_RCG(TT, S)                                                                          \
   switch (g) {                                                                                    \
-    case 4: return launch_rows_coded<TT, S, 4>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);      \
-    case 8: return launch_rows_coded<TT, S, 8>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);      \
-    case 16: return launch_rows_coded<TT, S, 16>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);    \
-    case 32: return launch_rows_coded<TT, S, 32>(nb, nbp, h, (const TT*)tab, xs, ys, pack, st);    \
-  }                                                                                               \
-  return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");
-    if (cplx) { EDLGT_RCG(double2, double2) }
-    if (vec_type == EDLGT_VALUE_F64) { EDLGT_RCG(double, double) }
-    EDLGT_RCG(double, double2)
+    case 4: EDLGT_TRY((launch_rows_coded<TT, S, 4>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break;   \
+    case 8: EDLGT_TRY((launch_rows_coded<TT, S, 8>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break;   \
+    case 16: EDLGT_TRY((launch_rows_coded<TT, S, 16>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break; \
+    case 32: EDLGT_TRY((launch_rows_coded<TT, S, 32>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break; \
+    default: return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");                \
+  }
+      if (cplx) {
+        EDLGT_RCG(double2, double2)
+      } else if (vec_type == EDLGT_VALUE_F64) {
+        EDLGT_RCG(double, double)
+      } else {
+        EDLGT_RCG(double, double2)
+      }
 #undef EDLGT_RCG
+    }
+    return EDLGT_OK;
   }
   if (cplx) return launch_coef<double2, double2>(g, nb, nbp, n, h, h.val.as<double2>(), xs, ys, coef, pack, st);
   if (vec_type == EDLGT_VALUE_F64)
