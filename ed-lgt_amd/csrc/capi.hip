@@ -329,6 +329,16 @@ int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, vo
   EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_C64 && h->value_type != EDLGT_VALUE_C128),
                 "complex64 vectors need a complex matrix");
   if (precision == EDLGT_PREC_MIXED) EDLGT_TRY(csr_make_low_precision(*h, (hipStream_t)stream));
+  if (n_rhs > 1 && precision == EDLGT_PREC_FP64 && spmm_packable(*h, vec_type)) {
+    // row-interleaved groups of columns in a temporary workspace; the call
+    // returns once the product is done (the workspace is released)
+    DevBuf pack;
+    EDLGT_TRY(pack.alloc(spmm_pack_bytes(*h, vec_type)));
+    EDLGT_TRY(csr_spmm(*h, precision, vec_type, n_rhs, X, h->n_rows, Y, h->n_rows, (hipStream_t)stream, pack.p,
+                       pack.bytes));
+    EDLGT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return EDLGT_OK;
+  }
   return csr_spmm(*h, precision, vec_type, n_rhs, X, h->n_rows, Y, h->n_rows, (hipStream_t)stream);
 }
 

@@ -24,8 +24,12 @@ int lanes_per_row(const Csr& h);
 // Y[:, b] = H X[:, b] for b < n_rhs; xtype is the element type of X
 // (C128/F64, or C64/F32 with precision == MIXED); Y is FP64 (complex iff the
 // matrix or X is complex).
+// pack (optional, pack_bytes >= spmm_pack_bytes): workspace for the
+// row-interleaved FP64 block product (n_rhs > 1, spmm_packable).
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx,
-             void* y, int64_t ldy, hipStream_t st);
+             void* y, int64_t ldy, hipStream_t st, void* pack = nullptr, size_t pack_bytes = 0);
+size_t spmm_pack_bytes(const Csr& h, int xtype);
+bool spmm_packable(const Csr& h, int xtype);
 int csr_make_low_precision(Csr& h, hipStream_t st);
 // Fused multi-coefficient SpMM (see spmv.hip): xs/ys device pointer arrays,
 // coef device [nb][16]; nb in {1, 2, 4, 8, 16}.

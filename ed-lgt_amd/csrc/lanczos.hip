@@ -389,7 +389,7 @@ struct Solver {
   int64_t N;
   int b;       // block size
   int M;       // basis capacity in vectors (multiple of b)
-  DevBuf V, W, Tmp, part, gdev, cdev, shadow;
+  DevBuf V, W, Tmp, part, gdev, cdev, shadow, pack;
   int64_t mv64 = 0, mv32 = 0;
   uint64_t seed = 12345;
   bool need_tmp = true;      // restart / Ritz buffer (not needed by Krylov stepping)
@@ -544,7 +544,8 @@ struct Solver {
       EDLGT_TRY(csr_spmm(h, EDLGT_PREC_MIXED, lt, nw, shadow.p, N, Y, N, st));
       mv32 += nw;
     } else {
-      EDLGT_TRY(csr_spmm(h, EDLGT_PREC_FP64, vt, nw, X, N, Y, N, st));
+      if (nw > 1 && !pack.p && spmm_packable(h, vt)) EDLGT_TRY(pack.alloc(spmm_pack_bytes(h, vt)));
+      EDLGT_TRY(csr_spmm(h, EDLGT_PREC_FP64, vt, nw, X, N, Y, N, st, pack.p, pack.bytes));
       mv64 += nw;
     }
     return EDLGT_OK;

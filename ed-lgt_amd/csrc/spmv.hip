@@ -536,8 +536,14 @@ int spmv_coded(const Csr& h, const void* tab, int xtype, const void* xb, void* y
   return set_error(EDLGT_ERR_INVALID_ARG, "unsupported vector type for a coded matrix");
 }
 
+int spmm_packed(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                void* pack, hipStream_t st);
+
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx, void* y,
-             int64_t ldy, hipStream_t st) {
+             int64_t ldy, hipStream_t st, void* pack, size_t pack_bytes) {
+  if (n_rhs > 1 && precision == EDLGT_PREC_FP64 && pack && pack_bytes >= spmm_pack_bytes(h, xtype) &&
+      spmm_packable(h, xtype))
+    return spmm_packed(h, xtype, n_rhs, x, ldx, y, ldy, pack, st);
   if (h.n_dict > 0 && h.stream_blocks.p) {
     // coded values: one stream pass per right-hand side (all term coefficients
     // 1); with a reduced-precision vector only the gathers are FP32
@@ -642,12 +648,28 @@ constexpr int kMaxTerms = 16;
 __device__ inline double2 scale(double c, double2 v) { return make_double2(c * v.x, c * v.y); }
 __device__ inline double scale(double c, double v) { return c * v; }
 
+// Columns of a block of vectors: a device array of per-column pointers, or
+// column j at base + j * ld.
+template <class S>
+struct InCols {
+  const S* const* ptr;
+  const S* base;
+  int64_t ld;
+  __device__ const S* col(int j) const { return ptr ? ptr[j] : base + (int64_t)j * ld; }
+};
+template <class S>
+struct OutCols {
+  S* const* ptr;
+  S* base;
+  int64_t ld;
+  __device__ S* col(int j) const { return ptr ? ptr[j] : base + (int64_t)j * ld; }
+};
+
 template <class S, int NB>
-__global__ void __launch_bounds__(256) pack_rows(const S* const* __restrict__ xs, int nb, int64_t n,
-                                                 S* __restrict__ xi) {
+__global__ void __launch_bounds__(256) pack_rows(InCols<S> xs, int nb, int64_t n, S* __restrict__ xi) {
   const S* xp[NB];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) xp[b] = b < nb ? xs[b] : nullptr;
+  for (int b = 0; b < NB; ++b) xp[b] = b < nb ? xs.col(b) : nullptr;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) xi[r * NB + b] = b < nb ? xp[b][r] : zero_acc<S>();
@@ -679,9 +701,10 @@ template <class VT, class S, int G, int NB>
 __global__ void __launch_bounds__(256) csr_spmm_coef_rows(
     int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const VT* __restrict__ val, const uint8_t* __restrict__ term, const S* __restrict__ xi,
-    S* const* __restrict__ ys, int nb, const double* __restrict__ coef) {
+    OutCols<S> ys, int nb, const double* __restrict__ coef) {
+  // coef == nullptr: every coefficient 1 (a plain block product)
   __shared__ double cs[NB * kMaxTerms];
-  for (int i = threadIdx.x; i < NB * kMaxTerms; i += blockDim.x) cs[i] = coef[i];
+  for (int i = threadIdx.x; i < NB * kMaxTerms; i += blockDim.x) cs[i] = coef ? coef[i] : 1.0;
   __syncthreads();
   const int sub = threadIdx.x & (G - 1);
   const int64_t stride = (int64_t)gridDim.x * (256 / G);
@@ -702,7 +725,7 @@ __global__ void __launch_bounds__(256) csr_spmm_coef_rows(
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const S s = group_sum<G>(acc[b]);
-      if (sub == 0 && b < nb) ys[b][r] = s;
+      if (sub == 0 && b < nb) ys.col(b)[r] = s;
     }
   }
 }
@@ -723,11 +746,13 @@ __global__ void scale_dict(const double2* __restrict__ dval, const int32_t* __re
 template <class TT, class CT, class S, int G, int NB, bool kLds>
 __global__ void __launch_bounds__(256) csr_rows_coded(
     int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-    const CT* __restrict__ code, const TT* __restrict__ tab, int n_tab, const S* __restrict__ xi,
-    S* const* __restrict__ ys, int nb) {
+    const CT* __restrict__ code, const TT* __restrict__ tab, int n_tab, int tab_stride,
+    const S* __restrict__ xi, OutCols<S> ys, int nb) {
+  // column b's value of code k: tab[b * tab_stride + k] (tab_stride 0: one
+  // table shared by every column)
   __shared__ TT stab[kLds ? kRowsTabLds : 1];
   if (kLds) {
-    for (int i = threadIdx.x; i < nb * n_tab; i += blockDim.x) stab[i] = tab[i];
+    for (int i = threadIdx.x; i < (tab_stride ? nb : 1) * n_tab; i += blockDim.x) stab[i] = tab[i];
     __syncthreads();
   }
   const TT* tb = kLds ? stab : tab;
@@ -744,37 +769,36 @@ __global__ void __launch_bounds__(256) csr_rows_coded(
       S x[NB];
       load_row<NB>(xi + (int64_t)c * NB, x);
 #pragma unroll
-      for (int b = 0; b < NB; ++b) fma_acc(acc[b], tb[b * n_tab + k], x[b]);
+      for (int b = 0; b < NB; ++b) fma_acc(acc[b], tb[b * tab_stride + k], x[b]);
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const S s = group_sum<G>(acc[b]);
-      if (sub == 0 && b < nb) ys[b][r] = s;
+      if (sub == 0 && b < nb) ys.col(b)[r] = s;
     }
   }
 }
 
 template <class TT, class S, int G>
-int launch_rows_coded(int nb, int nbp, const Csr& h, const TT* tab, const void* xs, void* ys, void* pack,
-                      hipStream_t st) {
+int launch_rows_coded(int nb, int nbp, const Csr& h, const TT* tab, int tab_stride, InCols<S> xs,
+                      OutCols<S> ys, void* pack, hipStream_t st) {
   const int64_t n = h.n_rows;
   const unsigned pgrid = grid_for(n, 256, 65536);
   const int64_t blocks = std::min<int64_t>(65536, std::max<int64_t>(1, (n + 256 / G - 1) / (256 / G)));
 #define EDLGT_RC(CT, NBC, LDS)                                                                      \
   hipLaunchKernelGGL((csr_rows_coded<TT, CT, S, G, NBC, LDS>), dim3((unsigned)blocks), dim3(256), 0, st, \
                      n, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), h.code.as<CT>(), tab, h.n_dict,  \
-                     (const S*)pack, (S* const*)ys, nb)
+                     tab_stride, (const S*)pack, ys, nb)
 #define EDLGT_RC_NB(NBC)                                                                            \
   case NBC:                                                                                         \
-    hipLaunchKernelGGL((pack_rows<S, NBC>), dim3(pgrid), dim3(256), 0, st, (const S* const*)xs, nb, n, \
-                       (S*)pack);                                                                   \
+    hipLaunchKernelGGL((pack_rows<S, NBC>), dim3(pgrid), dim3(256), 0, st, xs, nb, n, (S*)pack);    \
     if (h.code_bytes == 1) {                                                                        \
       if (lds) EDLGT_RC(uint8_t, NBC, true); else EDLGT_RC(uint8_t, NBC, false);                    \
     } else {                                                                                        \
       if (lds) EDLGT_RC(uint16_t, NBC, true); else EDLGT_RC(uint16_t, NBC, false);                  \
     }                                                                                               \
     break;
-  const bool lds = nbp * h.n_dict <= kRowsTabLds;
+  const bool lds = (tab_stride ? nbp : 1) * h.n_dict <= kRowsTabLds;
   switch (nbp) {
     EDLGT_RC_NB(1)
     EDLGT_RC_NB(2)
@@ -791,18 +815,17 @@ int launch_rows_coded(int nb, int nbp, const Csr& h, const TT* tab, const void* 
 }
 
 template <class VT, class S, int G>
-int launch_coef_nb(int nb, int nbp, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
+int launch_coef_nb(int nb, int nbp, int64_t n, const Csr& h, const VT* val, InCols<S> xs, OutCols<S> ys,
                    const double* coef, void* pack, hipStream_t st) {
   const unsigned pgrid = grid_for(n, 256, 65536);
   const int64_t blocks = std::min<int64_t>(65536, std::max<int64_t>(1, (n + 256 / G - 1) / (256 / G)));
   const uint8_t* term = h.n_terms > 1 ? h.term.as<uint8_t>() : (const uint8_t*)nullptr;
 #define EDLGT_COEF_CASE(NBC)                                                                     \
   case NBC:                                                                                      \
-    hipLaunchKernelGGL((pack_rows<S, NBC>), dim3(pgrid), dim3(256), 0, st, (const S* const*)xs, nb, n, \
-                       (S*)pack);                                                                \
+    hipLaunchKernelGGL((pack_rows<S, NBC>), dim3(pgrid), dim3(256), 0, st, xs, nb, n, (S*)pack);   \
     hipLaunchKernelGGL((csr_spmm_coef_rows<VT, S, G, NBC>), dim3((unsigned)blocks), dim3(256), 0, st, \
                        n, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), val, term, (const S*)pack, \
-                       (S* const*)ys, nb, coef);                                                 \
+                       ys, nb, coef);                                                            \
     break;
   switch (nbp) {
     EDLGT_COEF_CASE(1)
@@ -819,7 +842,7 @@ int launch_coef_nb(int nb, int nbp, int64_t n, const Csr& h, const VT* val, cons
 }
 
 template <class VT, class S>
-int launch_coef(int g, int nb, int nbp, int64_t n, const Csr& h, const VT* val, const void* xs, void* ys,
+int launch_coef(int g, int nb, int nbp, int64_t n, const Csr& h, const VT* val, InCols<S> xs, OutCols<S> ys,
                 const double* coef, void* pack, hipStream_t st) {
   switch (g) {
     case 4: return launch_coef_nb<VT, S, 4>(nb, nbp, n, h, val, xs, ys, coef, pack, st);
@@ -899,12 +922,17 @@ int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, 
       const void* xg = static_cast<const char*>(xs) + (size_t)g0 * sizeof(void*);
       void* yg = static_cast<char*>(ys) + (size_t)g0 * sizeof(void*);
 #define EDLGT_RCG(TT, S)                                                                          \
-  switch (g) {                                                                                    \
-    case 4: EDLGT_TRY((launch_rows_coded<TT, S, 4>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break;   \
-    case 8: EDLGT_TRY((launch_rows_coded<TT, S, 8>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break;   \
-    case 16: EDLGT_TRY((launch_rows_coded<TT, S, 16>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break; \
-    case 32: EDLGT_TRY((launch_rows_coded<TT, S, 32>(nbg, wp, h, (const TT*)tg, xg, yg, pack, st))); break; \
-    default: return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");                \
+  {                                                                                               \
+    const InCols<S> xc{(const S* const*)xg, nullptr, 0};                                          \
+    const OutCols<S> yc{(S* const*)yg, nullptr, 0};                                               \
+    const int ts = h.n_dict;                                                                      \
+    switch (g) {                                                                                  \
+      case 4: EDLGT_TRY((launch_rows_coded<TT, S, 4>(nbg, wp, h, (const TT*)tg, ts, xc, yc, pack, st))); break;   \
+      case 8: EDLGT_TRY((launch_rows_coded<TT, S, 8>(nbg, wp, h, (const TT*)tg, ts, xc, yc, pack, st))); break;   \
+      case 16: EDLGT_TRY((launch_rows_coded<TT, S, 16>(nbg, wp, h, (const TT*)tg, ts, xc, yc, pack, st))); break; \
+      case 32: EDLGT_TRY((launch_rows_coded<TT, S, 32>(nbg, wp, h, (const TT*)tg, ts, xc, yc, pack, st))); break; \
+      default: return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");              \
+    }                                                                                             \
   }
       if (cplx) {
         EDLGT_RCG(double2, double2)
@@ -917,10 +945,71 @@ int csr_spmm_coef(const Csr& h, int vec_type, int nb, const void* xs, void* ys, 
     }
     return EDLGT_OK;
   }
-  if (cplx) return launch_coef<double2, double2>(g, nb, nbp, n, h, h.val.as<double2>(), xs, ys, coef, pack, st);
+  if (cplx)
+    return launch_coef<double2, double2>(g, nb, nbp, n, h, h.val.as<double2>(),
+                                         InCols<double2>{(const double2* const*)xs, nullptr, 0},
+                                         OutCols<double2>{(double2* const*)ys, nullptr, 0}, coef, pack, st);
   if (vec_type == EDLGT_VALUE_F64)
-    return launch_coef<double, double>(g, nb, nbp, n, h, h.val.as<double>(), xs, ys, coef, pack, st);
-  return launch_coef<double, double2>(g, nb, nbp, n, h, h.val.as<double>(), xs, ys, coef, pack, st);
+    return launch_coef<double, double>(g, nb, nbp, n, h, h.val.as<double>(),
+                                       InCols<double>{(const double* const*)xs, nullptr, 0},
+                                       OutCols<double>{(double* const*)ys, nullptr, 0}, coef, pack, st);
+  return launch_coef<double, double2>(g, nb, nbp, n, h, h.val.as<double>(),
+                                      InCols<double2>{(const double2* const*)xs, nullptr, 0},
+                                      OutCols<double2>{(double2* const*)ys, nullptr, 0}, coef, pack, st);
+}
+
+// Block product Y = H X (FP64, 2 <= n_rhs) through row-interleaved groups of
+// up to kGroupRhs columns: one gather per stored entry fetches that entry's x
+// for the whole group.  Same kernels as the coefficient product with every
+// coefficient 1, so a column's bits do not depend on the other columns.
+size_t spmm_pack_bytes(const Csr& h, int xtype) {
+  const size_t es = xtype == EDLGT_VALUE_C128 ? 16 : 8;
+  return ((size_t)h.n_rows * kGroupRhs * es + 255) / 256 * 256;
+}
+
+bool spmm_packable(const Csr& h, int xtype) {
+  if (h.value_type == EDLGT_VALUE_C128) return xtype == EDLGT_VALUE_C128;
+  return xtype == EDLGT_VALUE_F64 || xtype == EDLGT_VALUE_C128;
+}
+
+int spmm_packed(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                void* pack, hipStream_t st) {
+  const int g = coef_lanes(h);
+  const int64_t n = h.n_rows;
+  const bool cplx = h.value_type == EDLGT_VALUE_C128;
+  const size_t es = xtype == EDLGT_VALUE_C128 ? 16 : 8;
+  for (int g0 = 0; g0 < n_rhs; g0 += kGroupRhs) {
+    const int nb = std::min(kGroupRhs, n_rhs - g0);
+    const int nbp = coef_pad(nb);
+    const char* xg = static_cast<const char*>(x) + (size_t)g0 * ldx * es;
+    char* yg = static_cast<char*>(y) + (size_t)g0 * ldy * es;
+#define EDLGT_PK(TT, S)                                                                           \
+  {                                                                                               \
+    const InCols<S> xc{nullptr, (const S*)xg, ldx};                                               \
+    const OutCols<S> yc{nullptr, (S*)yg, ldy};                                                    \
+    if (h.n_dict > 0) {                                                                           \
+      const TT* tu = h.tab_unit.as<TT>();                                                         \
+      switch (g) {                                                                                \
+        case 4: EDLGT_TRY((launch_rows_coded<TT, S, 4>(nb, nbp, h, tu, 0, xc, yc, pack, st))); break;   \
+        case 8: EDLGT_TRY((launch_rows_coded<TT, S, 8>(nb, nbp, h, tu, 0, xc, yc, pack, st))); break;   \
+        case 16: EDLGT_TRY((launch_rows_coded<TT, S, 16>(nb, nbp, h, tu, 0, xc, yc, pack, st))); break; \
+        case 32: EDLGT_TRY((launch_rows_coded<TT, S, 32>(nb, nbp, h, tu, 0, xc, yc, pack, st))); break; \
+        default: return set_error(EDLGT_ERR_INVALID_ARG, "unsupported lanes-per-row");            \
+      }                                                                                           \
+    } else {                                                                                      \
+      EDLGT_TRY((launch_coef<TT, S>(g, nb, nbp, n, h, h.val.as<TT>(), xc, yc, nullptr, pack, st))); \
+    }                                                                                             \
+  }
+    if (cplx) {
+      EDLGT_PK(double2, double2)
+    } else if (xtype == EDLGT_VALUE_F64) {
+      EDLGT_PK(double, double)
+    } else {
+      EDLGT_PK(double, double2)
+    }
+#undef EDLGT_PK
+  }
+  return EDLGT_OK;
 }
 
 int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, void* y,

@@ -2,8 +2,8 @@
 
 A coded product multiplies by exactly the stored values in the same order as
 the uncoded kernels, so SpMV and the fused coefficient SpMM give the same bits
-coded and uncoded (where both take the same kernel shape); the per-column
-stream path of large coded matrices is checked against the uncoded product at
+coded and uncoded (where both take the same kernel shape); the grouped
+products of large coded matrices are checked against the uncoded product at
 1e-12 relative and for batch independence (bitwise).  Eigenvalues of coded
 matrices are checked against the oracle."""
 import numpy as np
@@ -27,14 +27,20 @@ def test_coded_spmv_bitwise_complex(gpu):
     x = torch.as_tensor(rng.standard_normal(H.n) + 1j * rng.standard_normal(H.n), device=gpu)
     X = torch.as_tensor(rng.standard_normal((3, H.n)) + 1j * rng.standard_normal((3, H.n)), device=gpu)
     y0 = H.matvec(x).cpu().numpy()
+    Y0 = H.matvec(X).cpu().numpy()
     nd = H.compress()
     print(f"QED 4x2 n_dict={nd} code_bytes={H.code_bytes}", flush=True)
     assert 0 < nd <= 4096
     y1 = H.matvec(x).cpu().numpy()
     assert np.array_equal(y0, y1)
     Y1 = H.matvec(X).cpu().numpy()
+    # block products: coded == uncoded bitwise, a column's bits independent of
+    # the block; single-vector products (another kernel) agree to rounding
+    assert np.array_equal(Y0, Y1)
+    assert np.array_equal(H.matvec(X[1:]).cpu().numpy(), Y1[1:])
     for j in range(3):
-        assert np.array_equal(Y1[j], H.matvec(X[j].contiguous()).cpu().numpy())
+        yj = H.matvec(X[j].contiguous()).cpu().numpy()
+        assert np.max(np.abs(Y1[j] - yj)) <= 1e-13 * np.max(np.abs(yj))
     ref = H.to_scipy() @ X.cpu().numpy().T
     assert np.max(np.abs(Y1 - ref.T)) <= 1e-13 * np.max(np.abs(ref))
     # eigenvalues of the coded matrix against the oracle
