@@ -182,7 +182,8 @@ def su2_dynamics(args):
         psi, margs, errs = psi0, [], []
         for c0 in range(0, args.dyn_steps, chunk):
             n = min(chunk, args.dyn_steps - c0)
-            mod.time_evolution_Hamiltonian(psi, tl[c0:c0 + n + 1], m=30, reorth=False, store_states=False,
+            # the time line of a call is measured from the state passed in
+            mod.time_evolution_Hamiltonian(psi, tl[:n + 1], m=30, reorth=False, store_states=False,
                                            precision=prec)
             psi = mod.H.psi_final
             margs.extend(list(mod.H.marginals[(0 if c0 == 0 else 1):]))

@@ -1,6 +1,7 @@
 // C ABI of libedlgt_hip (declared in include/edlgt_hip.h): argument checks,
 // error capture and handle management around the device kernels.
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 
@@ -442,6 +443,41 @@ int edlgt_krylov_evolve(edlgt_csr* h, const edlgt_basis* b, int32_t precision, v
   EDLGT_REQUIRE(!b || b->n_states == h->n_rows, "basis and matrix sizes differ");
   return krylov_run(*h, b, precision, psi, dt, n_steps, m, reorth, d_max, out_marginals, out_err,
                     psi_history, (hipStream_t)stream);
+}
+
+struct edlgt_krylov {
+  KrylovSession* s = nullptr;
+  const edlgt_basis* b = nullptr;
+  int64_t n = 0;
+  ~edlgt_krylov() { krylov_destroy(s); }
+};
+
+int edlgt_krylov_create(edlgt_csr* h, const edlgt_basis* b, int32_t precision, int32_t m, int32_t reorth,
+                        int32_t d_max, void* stream, edlgt_krylov** out) {
+  EDLGT_REQUIRE(h && out, "null argument");
+  EDLGT_REQUIRE(m >= 1, "m must be >= 1");
+  EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
+  EDLGT_REQUIRE(!b || b->n_states == h->n_rows, "basis and matrix sizes differ");
+  EDLGT_REQUIRE(!b || d_max < 1 || b->n_sites * d_max <= 4096, "n_sites * d_max must be <= 4096");
+  *out = nullptr;
+  auto k = std::make_unique<edlgt_krylov>();
+  EDLGT_TRY(krylov_create(*h, b, precision, m, reorth, b ? d_max : 0, (hipStream_t)stream, &k->s));
+  k->b = b;
+  k->n = h->n_rows;
+  *out = k.release();
+  return EDLGT_OK;
+}
+
+int edlgt_krylov_run(edlgt_krylov* k, void* psi, double dt, int64_t n_steps, double* out_marginals,
+                     double* out_err, void* psi_history) {
+  EDLGT_REQUIRE(k && psi, "null argument");
+  EDLGT_REQUIRE(n_steps >= 0, "n_steps must be >= 0");
+  return krylov_session_run(*k->s, psi, dt, n_steps, out_marginals, out_err, psi_history);
+}
+
+int edlgt_krylov_destroy(edlgt_krylov* k) {
+  delete k;
+  return EDLGT_OK;
 }
 
 struct edlgt_server {

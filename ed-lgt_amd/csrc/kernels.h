@@ -54,6 +54,12 @@ int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter,
 int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
                int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
                hipStream_t st);
+struct KrylovSession;
+int krylov_create(Csr& h, const Basis* basis, int precision, int m, int reorth, int d_max, hipStream_t st,
+                  KrylovSession** out);
+int krylov_session_run(KrylovSession& ks, void* psi, double dt, int64_t n_steps, double* out_marg,
+                       double* out_err, void* history);
+void krylov_destroy(KrylovSession* ks);
 
 void* server_create(Csr& h, int vec_type, int max_slots, int n_eigs, double tol, int max_basis,
                     int64_t max_mv, int warm_cap, hipStream_t st, int* status);

@@ -237,6 +237,28 @@ __global__ void __launch_bounds__(kTB) row_transform(S* X, int64_t ldx, int nw, 
   if constexpr (kGram) rg.flush(partial);
 }
 
+__device__ inline double2 rscale(double c, double2 v) { return make_double2(c * v.x, c * v.y); }
+__device__ inline double rscale(double c, double v) { return c * v; }
+
+// One three-term Krylov update in place: w = c2 w + c1 b + c0 a (real
+// coefficients; a may alias b), accumulating |w|^2 per block like
+// tall_update.  Reads three vectors and writes one: the recurrence, the new
+// norm and the next basis vector in a single pass.
+template <class S>
+__global__ void __launch_bounds__(kTB) three_term(const S* a, const S* b, S* w, double c0, double c1, double c2,
+                                                  int64_t n, double2* __restrict__ partial) {
+  RowGram<S, 1> rg;
+  rg.clear();
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    S y[1];
+    y[0] = add(add(rscale(c2, w[r]), rscale(c1, b[r])), rscale(c0, a[r]));
+    w[r] = y[0];
+    rg.add_row(y);
+  }
+  rg.flush(partial);
+}
+
 // fills x[i] with deterministic pseudo-random values in [-0.5, 0.5)
 __device__ inline double hash_unit(uint64_t x) {
   x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
@@ -532,6 +554,19 @@ struct Solver {
     EDLGT_HIP(hipGetLastError());
     if (gout) return fetch_gram(nw, *gout);
     EDLGT_HIP(hipStreamSynchronize(st));
+    return EDLGT_OK;
+  }
+
+  // In place w = c2 w + c1 b + c0 a; nrm2 = |w|^2 after the update.
+  int three_term_update(const S* a, const S* bv, S* w, double c0, double c1, double c2, double& nrm2) {
+    const unsigned g = grid_for(N, kTB, 4096);
+    hipLaunchKernelGGL(three_term<S>, dim3(g), dim3(kTB), 0, st, a, bv, w, c0, c1, c2, N, part.as<double2>());
+    hipLaunchKernelGGL((gram_finish<1>), dim3(1), dim3(kTB), 0, st, part.as<double2>(), (int)g, 1, 1, 0,
+                       gdev.as<double2>());
+    EDLGT_HIP(hipGetLastError());
+    Dense G;
+    EDLGT_TRY(fetch_gram(1, G));
+    nrm2 = G(0, 0).real();
     return EDLGT_OK;
   }
 
@@ -848,12 +883,16 @@ struct Solver {
     sc(0, 0) = 1.0 / beta0;
     EDLGT_TRY(update(psi, 1, sc, col(0), 1, 0));
     std::vector<double> alpha, beta;
+    // Without reorthogonalisation the basis is stored unnormalised, u_j =
+    // nrm[j] v_j: the product lands in column j+1 and one fused pass forms
+    // u_{j+1} = H v_j - alpha_j v_j - beta_{j-1} v_{j-1} and its norm.
+    std::vector<double> nrm(1, 1.0);
     double hnorm = 0.0;
     int me = m;
     for (int j = 0; j < m; ++j) {
-      EDLGT_TRY(matvec(col(j), W.as<S>(), 1));
-      double a = 0.0;
+      double a = 0.0, b = 0.0;
       if (reorth) {
+        EDLGT_TRY(matvec(col(j), W.as<S>(), 1));
         for (int pass = 0; pass < 2; ++pass) {
           Dense G;
           EDLGT_TRY(gram(V.as<S>(), j + 1, W.as<S>(), 1, G));
@@ -861,29 +900,37 @@ struct Solver {
           for (auto& z : G.a) z = -z;
           EDLGT_TRY(update(V.as<S>(), j + 1, G, W.as<S>(), 1, 1));
         }
+        Dense bn;
+        EDLGT_TRY(gram(W.as<S>(), 1, W.as<S>(), 1, bn));
+        b = std::sqrt(std::max(bn(0, 0).real(), 0.0));
       } else {
+        S* u = col(j);
+        S* w = col(j + 1);
+        EDLGT_TRY(matvec(u, w, 1));
         Dense G;
-        EDLGT_TRY(gram(col(j), 1, W.as<S>(), 1, G));
-        a = G(0, 0).real();
-        const int lo = j > 0 ? j - 1 : j;
-        Dense C(j - lo + 1, 1);
-        if (j > 0) C(0, 0) = -beta[j - 1];
-        C(j - lo, 0) = -a;
-        EDLGT_TRY(update(col(lo), j - lo + 1, C, W.as<S>(), 1, 1));
+        EDLGT_TRY(gram(u, 1, w, 1, G));
+        const double nj = nrm[j];
+        a = G(0, 0).real() / (nj * nj);
+        const double cp = j > 0 ? -beta[j - 1] / nrm[j - 1] : 0.0;
+        double nn = 0.0;
+        EDLGT_TRY(three_term_update(j > 0 ? col(j - 1) : u, u, w, cp, -a / nj, 1.0 / nj, nn));
+        b = std::sqrt(std::max(nn, 0.0));
       }
       alpha.push_back(a);
-      Dense bn;
-      EDLGT_TRY(gram(W.as<S>(), 1, W.as<S>(), 1, bn));
-      const double b = std::sqrt(std::max(bn(0, 0).real(), 0.0));
       beta.push_back(b);
       hnorm = std::max(hnorm, std::fabs(a) + b + (j > 0 ? beta[j - 1] : 0.0));
       if (b <= 1e-14 * std::max(hnorm, 1e-300)) {  // invariant subspace: exact
         me = j + 1;
         break;
       }
-      Dense s1(1, 1);
-      s1(0, 0) = 1.0 / b;
-      EDLGT_TRY(update(W.as<S>(), 1, s1, col(j + 1), 1, 0));
+      if (reorth) {
+        Dense s1(1, 1);
+        s1(0, 0) = 1.0 / b;
+        EDLGT_TRY(update(W.as<S>(), 1, s1, col(j + 1), 1, 0));
+        nrm.push_back(1.0);
+      } else {
+        nrm.push_back(b);
+      }
     }
     Dense T(me, me);
     for (int j = 0; j < me; ++j) {
@@ -903,6 +950,7 @@ struct Solver {
       c(k, 0) = beta0 * z;
     }
     err = (me == m) ? beta[m - 1] * std::abs(c(m - 1, 0)) / beta0 : 0.0;
+    for (int k = 0; k < me; ++k) c(k, 0) /= nrm[k];
     EDLGT_TRY(update(V.as<S>(), me, c, psi, 1, 0));
     m_used = me;
     return EDLGT_OK;
@@ -997,43 +1045,82 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   return EDLGT_OK;
 }
 
-int krylov_typed(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
-                 int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
-                 hipStream_t st) {
-  using S = double2;
-  const int64_t N = h.n_rows;
-  if (m < 1 || (int64_t)m > N) m = (int)std::max<int64_t>(1, std::min<int64_t>(m, N));
-  Solver<S> sv(h, st, precision);
-  sv.b = 1;
-  sv.M = m;
-  sv.need_tmp = false;
-  EDLGT_TRY(sv.setup());
+}  // namespace
+
+// Krylov stepping state that outlives one call: the basis of m+1 vectors
+// (m+1 full-length vectors: 187 GB for 377M complex128 states) and the
+// marginal buffer stay allocated between runs, so a trajectory computed in
+// chunks pays the allocation once.
+struct KrylovSession {
+  Csr& h;
+  const Basis* basis;
+  int m, reorth, d_max, bins;
+  hipStream_t st;
+  std::unique_ptr<Solver<double2>> sv;
   DevBuf marg;
-  const int bins = basis ? basis->n_sites * d_max : 0;
-  if (out_marg) {
-    if (!basis) return set_error(EDLGT_ERR_INVALID_ARG, "marginals need the sector basis");
-    EDLGT_TRY(marg.alloc((size_t)bins * sizeof(double)));
-  }
+  KrylovSession(Csr& hh, const Basis* b, hipStream_t s) : h(hh), basis(b), st(s) {}
+};
+
+int krylov_create(Csr& h, const Basis* basis, int precision, int m, int reorth, int d_max, hipStream_t st,
+                  KrylovSession** out) {
+  const int64_t N = h.n_rows;
+  m = (int)std::max<int64_t>(1, std::min<int64_t>(m, N));
+  auto ks = std::make_unique<KrylovSession>(h, basis, st);
+  ks->m = m;
+  ks->reorth = reorth;
+  ks->d_max = d_max;
+  ks->bins = basis && d_max > 0 ? basis->n_sites * d_max : 0;
+  ks->sv = std::make_unique<Solver<double2>>(h, st, precision);
+  ks->sv->b = 1;
+  ks->sv->M = m;
+  ks->sv->need_tmp = false;
+  ks->sv->need_w = reorth != 0;
+  EDLGT_TRY(ks->sv->setup());
+  if (ks->bins > 0) EDLGT_TRY(ks->marg.alloc((size_t)ks->bins * sizeof(double)));
+  *out = ks.release();
+  return EDLGT_OK;
+}
+
+void krylov_destroy(KrylovSession* ks) { delete ks; }
+
+int krylov_session_run(KrylovSession& ks, void* psi, double dt, int64_t n_steps, double* out_marg,
+                       double* out_err, void* history) {
+  using S = double2;
+  const int64_t N = ks.h.n_rows;
+  if (out_marg && ks.bins == 0) return set_error(EDLGT_ERR_INVALID_ARG, "marginals need the sector basis");
   auto record = [&](int64_t t) -> int {
     if (out_marg) {
-      EDLGT_TRY(basis_marginals(*basis, psi, true, d_max, marg.as<double>(), st));
-      EDLGT_HIP(hipMemcpy(out_marg + t * bins, marg.p, (size_t)bins * sizeof(double), hipMemcpyDeviceToHost));
+      EDLGT_TRY(basis_marginals(*ks.basis, psi, true, ks.d_max, ks.marg.as<double>(), ks.st));
+      EDLGT_HIP(hipMemcpy(out_marg + t * ks.bins, ks.marg.p, (size_t)ks.bins * sizeof(double),
+                          hipMemcpyDeviceToHost));
     }
     if (history)
       EDLGT_HIP(hipMemcpyAsync(static_cast<S*>(history) + t * N, psi, (size_t)N * sizeof(S),
-                               hipMemcpyDeviceToDevice, st));
+                               hipMemcpyDeviceToDevice, ks.st));
     return EDLGT_OK;
   };
   EDLGT_TRY(record(0));
   for (int64_t t = 0; t < n_steps; ++t) {
     double err = 0.0;
     int mu = 0;
-    EDLGT_TRY(sv.krylov_step(static_cast<S*>(psi), dt, m, reorth != 0, err, mu));
+    EDLGT_TRY(ks.sv->krylov_step(static_cast<S*>(psi), dt, ks.m, ks.reorth != 0, err, mu));
     if (out_err) out_err[t] = err;
     EDLGT_TRY(record(t + 1));
   }
-  EDLGT_HIP(hipStreamSynchronize(st));
+  EDLGT_HIP(hipStreamSynchronize(ks.st));
   return EDLGT_OK;
+}
+
+namespace {
+
+int krylov_typed(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
+                 int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
+                 hipStream_t st) {
+  if (out_marg && !basis) return set_error(EDLGT_ERR_INVALID_ARG, "marginals need the sector basis");
+  KrylovSession* ks = nullptr;
+  EDLGT_TRY(krylov_create(h, basis, precision, m, reorth, out_marg ? d_max : 0, st, &ks));
+  std::unique_ptr<KrylovSession> hold(ks);
+  return krylov_session_run(*ks, psi, dt, n_steps, out_marg, out_err, history);
 }
 
 // ---------------------------------------------------------------- serving

@@ -79,6 +79,9 @@ SIGNATURES = {
     "edlgt_lanczos": (_i32, [_p, _i32, _f64, _i32, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "edlgt_basis_marginals": (_i32, [_p, _p, _i32, _i32, _p, _p]),
     "edlgt_krylov_evolve": (_i32, [_p, _p, _i32, _p, _f64, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
+    "edlgt_krylov_create": (_i32, [_p, _p, _i32, _i32, _i32, _i32, _p, _p]),
+    "edlgt_krylov_run": (_i32, [_p, _p, _f64, _i64, _p, _p, _p]),
+    "edlgt_krylov_destroy": (_i32, [_p]),
 }
 
 _lib = None

@@ -410,3 +410,39 @@ class Server:
             except Exception:
                 pass
             self._s = None
+
+
+class KrylovSession:
+    """Krylov exp(-iH dt) stepping state kept on the device between calls
+    (edlgt_krylov_create / run / destroy): the m+1 basis vectors and the
+    marginal buffer are allocated once, so a trajectory evolved in chunks does
+    not re-allocate them per chunk."""
+
+    def __init__(self, csr, basis=None, precision=L.PREC_FP64, m=30, reorth=True, d_max=0, stream=None):
+        k = C.c_void_p()
+        L.check(L.load().edlgt_krylov_create(csr.handle, None if basis is None else basis.handle,
+                                             int(precision), int(m), int(reorth), int(d_max or 0),
+                                             L.stream_handle(stream), C.byref(k)))
+        self._k = k
+        self.csr, self.basis = csr, basis
+        self.key = (id(csr), None if basis is None else id(basis), int(precision), int(m), bool(reorth),
+                    int(d_max or 0))
+
+    def run(self, psi, dt, n_steps, marginals=None, err=None, history=None):
+        """psi <- exp(-i H dt)^n_steps psi (device complex128, in place); per-step
+        marginals (host, (n_steps+1, n_sites, d_max)), error estimates (host) and
+        states (device, (n_steps+1, N)) when given."""
+        L.check(L.load().edlgt_krylov_run(self._k, L.ptr(psi), float(dt), int(n_steps), L.ptr(marginals),
+                                          L.ptr(err), L.ptr(history)))
+
+    def close(self):
+        k = getattr(self, "_k", None)
+        if k is not None and k.value:
+            try:
+                L.load().edlgt_krylov_destroy(k)
+            except Exception:
+                pass
+        self._k = None
+
+    def __del__(self):
+        self.close()

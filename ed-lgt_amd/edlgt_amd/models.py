@@ -107,9 +107,11 @@ def _time_evolution(self, initial_state, time_line, loc_dims, basis=None, d_max=
                     reorth=True, precision=L.PREC_FP64, store_states=True):
     """exp(-iHt) psi0 on a uniform time grid by device Krylov stepping
     (replaces expm_multiply in QMB_hamiltonian.time_evolution,
-    qmb_hamiltonian.py:334-426).  psi_time[k] = exp(-i H t_k) psi0."""
-    import ctypes as C
+    qmb_hamiltonian.py:334-426).  psi_time[k] = exp(-i H t_k) psi0.  The
+    stepping state (m+1 basis vectors) stays on the device for the next call
+    with the same matrix and settings; release_krylov() frees it."""
     import torch
+    from .device import KrylovSession
     time_line = np.asarray(time_line, dtype=float)
     if time_line.size < 2:
         raise ValueError("time_line needs at least two points")
@@ -124,27 +126,37 @@ def _time_evolution(self, initial_state, time_line, loc_dims, basis=None, d_max=
         psi = torch.as_tensor(np.asarray(initial_state, dtype=np.complex128), device="cuda").clone()
     if psi.numel() != N:
         raise ValueError("initial_state has the wrong length")
-    lib = L.load()
-    st = L.stream_handle()
+    dm = int(d_max or 0) if basis is not None else 0
+    key = (id(self.Ham), None if basis is None else id(basis), int(precision), int(m), bool(reorth), dm)
+    ks = getattr(self, "_krylov", None)
+    if ks is None or ks.key != key or ks.csr is not self.Ham:
+        self.release_krylov()
+        ks = self._krylov = KrylovSession(self.Ham, basis, precision, m, reorth, dm)
     if abs(time_line[0]) > 0:
-        L.check(lib.edlgt_krylov_evolve(self.Ham.handle, None, precision, L.ptr(psi), float(time_line[0]),
-                                        1, m, int(reorth), 0, None, None, None, st))
+        ks.run(psi, float(time_line[0]), 1)
     n_steps = time_line.size - 1
     hist = torch.empty((n_steps + 1, N), dtype=torch.complex128, device="cuda") if store_states else None
     marg = None
     if basis is not None:
         marg = np.zeros((n_steps + 1, basis.n_sites, d_max))
     err = np.zeros(max(n_steps, 1))
-    L.check(lib.edlgt_krylov_evolve(self.Ham.handle, None if basis is None else basis.handle, precision,
-                                    L.ptr(psi), dt, n_steps, m, int(reorth), int(d_max or 0),
-                                    L.ptr(marg), L.ptr(err), L.ptr(hist), st))
+    ks.run(psi, dt, n_steps, marg, err, hist)
     self.krylov_err = err[:n_steps]
     self.marginals = marg
     self.psi_final = psi
     self.psi_time = [QMB_state(hist[k], self.lvals, loc_dims) for k in range(n_steps + 1)] if store_states else None
 
 
+def _release_krylov(self):
+    """Frees the device state kept by time_evolution."""
+    ks = getattr(self, "_krylov", None)
+    if ks is not None:
+        ks.close()
+    self._krylov = None
+
+
 QMB_hamiltonian.time_evolution = _time_evolution
+QMB_hamiltonian.release_krylov = _release_krylov
 
 
 class QuantumModel:

@@ -55,6 +55,7 @@ extern "C" {
 typedef struct edlgt_basis edlgt_basis; /* sorted sector basis on the device */
 typedef struct edlgt_csr edlgt_csr;     /* CSR Hamiltonian on the device     */
 typedef struct edlgt_server edlgt_server; /* continuous-batching eigensolver   */
+typedef struct edlgt_krylov edlgt_krylov; /* Krylov stepping state (basis)      */
 
 int edlgt_last_error(char* buf, int64_t size);
 int edlgt_version(void);
@@ -316,6 +317,22 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
 int edlgt_krylov_evolve(edlgt_csr* h, const edlgt_basis* b, int32_t precision, void* psi, double dt,
                         int64_t n_steps, int32_t m, int32_t reorth, int32_t d_max,
                         double* out_marginals, double* out_err, void* psi_history, void* stream);
+
+/* The same stepping with its state kept between calls: create allocates the
+ * m+1 basis vectors (and the marginal buffer when b is given and d_max >= 1)
+ * once; each run advances psi by n_steps steps of dt exactly as
+ * edlgt_krylov_evolve does (same arguments, same results); destroy frees
+ * them.  Lets a caller evolve a trajectory in chunks (observables, progress,
+ * checkpoints between chunks) without re-allocating the basis, which for
+ * hundreds of millions of states is a hundred-GB allocation.  The session
+ * uses h (and b) until destroyed; one host thread at a time.
+ * Replaces the same expm_multiply call as edlgt_krylov_evolve
+ * (edlgt/modeling/qmb_hamiltonian.py:334-426). */
+int edlgt_krylov_create(edlgt_csr* h, const edlgt_basis* b, int32_t precision, int32_t m, int32_t reorth,
+                        int32_t d_max, void* stream, edlgt_krylov** out);
+int edlgt_krylov_run(edlgt_krylov* k, void* psi, double dt, int64_t n_steps, double* out_marginals,
+                     double* out_err, void* psi_history);
+int edlgt_krylov_destroy(edlgt_krylov* k);
 
 #ifdef __cplusplus
 }

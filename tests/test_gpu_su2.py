@@ -104,3 +104,16 @@ def test_su2_krylov_without_full_reorth(gpu):
     path = expm_path(orc.hamiltonian(1.0, 1.0), psi0.cpu().numpy(), t[0], t[-1], len(t))
     got = np.stack([s.psi for s in mod.H.psi_time])
     assert np.max(np.abs(got - path)) < 1e-8
+    # chunked evolution through the kept Krylov session (each chunk's time line
+    # starts at 0: the time line is measured from the state passed in): same
+    # bits as one run
+    psi_full = mod.H.psi_final.clone()
+    marg_full = mod.H.marginals
+    mod.time_evolution_Hamiltonian(psi0, t[:11], m=30, reorth=False, store_states=False)
+    ks = mod.H._krylov
+    mod.time_evolution_Hamiltonian(mod.H.psi_final, t[:11], m=30, reorth=False, store_states=False)
+    assert mod.H._krylov is ks
+    assert bool((mod.H.psi_final == psi_full).all())
+    assert np.array_equal(mod.H.marginals, marg_full[10:])
+    mod.H.release_krylov()
+    assert mod.H._krylov is None
