@@ -353,11 +353,21 @@ int check_err(const DevBuf& err, hipStream_t st) {
 
 }  // namespace
 
-int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
-                      int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
-                      const int32_t* place_tables, const double* place_strength,
-                      const int32_t* place_dagger, const int32_t* place_term, int32_t n_terms,
-                      int32_t value_type, hipStream_t st, Csr* out) {
+namespace {
+
+// Term program on the device: the transition lists of every table (row and
+// column nonzeros above OP_TOL), the placements, the basis' keys, strides and
+// configurations, as the assembly and expectation kernels read them.
+struct Program {
+  DevBuf place, str, toff, tket, tval, diag, err;
+  AsmDev A;
+  int cap = 1;  // raw entries per row bound (assembly LDS size)
+};
+
+int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables, int32_t n_place,
+                    const int32_t* place_nops, const int32_t* place_sites, const int32_t* place_tables,
+                    const double* place_strength, const int32_t* place_dagger, const int32_t* place_term,
+                    int32_t n_terms, int32_t value_type, hipStream_t st, Program& prog) {
   if (n_terms < 1 || n_terms > 16 || (n_terms > 1 && !place_term))
     return set_error(EDLGT_ERR_INVALID_ARG, "n_terms must be in [1, 16] (with place_term when > 1)");
   if (value_type != EDLGT_VALUE_F64 && value_type != EDLGT_VALUE_C128)
@@ -394,7 +404,7 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   for (int p = 0; p < n_place; ++p) {
     Placement& P = place[p];
     P.nops = place_nops[p];
-    P.dagger = place_dagger[p] ? 1 : 0;
+    P.dagger = place_dagger && place_dagger[p] ? 1 : 0;
     P.term = n_terms > 1 ? place_term[p] : 0;
     P.pad_ = 0;
     if (P.term < 0 || P.term >= n_terms) return set_error(EDLGT_ERR_INVALID_ARG, "place_term out of range");
@@ -418,31 +428,181 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
     P.s_re = place_strength[2 * p];
     P.s_im = place_strength[2 * p + 1];
   }
-  const int cap = (int)std::min<int64_t>(std::max<int64_t>(cap_bound, 1), 1024);
+  prog.cap = (int)std::min<int64_t>(std::max<int64_t>(cap_bound, 1), 1024);
 
-  DevBuf b_place, b_str, b_toff, b_tket, b_tval, b_diag, b_err;
-  EDLGT_TRY(upload(b_place, place.data(), std::max<size_t>(place.size(), 1), st));
+  EDLGT_TRY(upload(prog.place, place.data(), std::max<size_t>(place.size(), 1), st));
   if (B.key_words == 2)
-    EDLGT_TRY(upload(b_str, B.strides128.data(), B.strides128.size(), st));
+    EDLGT_TRY(upload(prog.str, B.strides128.data(), B.strides128.size(), st));
   else
-    EDLGT_TRY(upload(b_str, B.strides.data(), B.strides.size(), st));
-  EDLGT_TRY(upload(b_toff, trans_off.data(), trans_off.size(), st));
+    EDLGT_TRY(upload(prog.str, B.strides.data(), B.strides.size(), st));
+  EDLGT_TRY(upload(prog.toff, trans_off.data(), trans_off.size(), st));
   const uint8_t z8 = 0;
   const double2 z2 = make_double2(0, 0);
-  EDLGT_TRY(upload(b_tket, trans_ket.empty() ? &z8 : trans_ket.data(), std::max<size_t>(trans_ket.size(), 1), st));
-  EDLGT_TRY(upload(b_tval, trans_val.empty() ? &z2 : trans_val.data(), std::max<size_t>(trans_val.size(), 1), st));
-  EDLGT_TRY(upload(b_diag, diag.data(), diag.size(), st));
-  EDLGT_TRY(b_err.alloc(sizeof(int32_t)));
-  EDLGT_HIP(hipMemsetAsync(b_err.p, 0, sizeof(int32_t), st));
+  EDLGT_TRY(upload(prog.tket, trans_ket.empty() ? &z8 : trans_ket.data(), std::max<size_t>(trans_ket.size(), 1), st));
+  EDLGT_TRY(upload(prog.tval, trans_val.empty() ? &z2 : trans_val.data(), std::max<size_t>(trans_val.size(), 1), st));
+  EDLGT_TRY(upload(prog.diag, diag.data(), diag.size(), st));
+  EDLGT_TRY(prog.err.alloc(sizeof(int32_t)));
+  EDLGT_HIP(hipMemsetAsync(prog.err.p, 0, sizeof(int32_t), st));
 
-  AsmDev A;
+  AsmDev& A = prog.A;
   A.n_states = B.n_states; A.n_place = n_place; A.d_max = d_max; A.cfg_stride = B.cfg_stride;
   A.value_real = value_type == EDLGT_VALUE_F64;
-  A.place = b_place.as<Placement>(); A.strides = b_str.p;
+  A.place = prog.place.as<Placement>(); A.strides = prog.str.p;
   A.keys = B.keys.p; A.configs = B.configs.as<uint8_t>();
-  A.trans_off = b_toff.as<int32_t>(); A.trans_ket = b_tket.as<uint8_t>();
-  A.trans_val = b_tval.as<double2>(); A.diag = b_diag.as<double2>(); A.err = b_err.as<int32_t>();
+  A.trans_off = prog.toff.as<int32_t>(); A.trans_ket = prog.tket.as<uint8_t>();
+  A.trans_val = prog.tval.as<double2>(); A.diag = prog.diag.as<double2>(); A.err = prog.err.as<int32_t>();
 
+  return EDLGT_OK;
+}
+
+
+// ------------------------------------------------------------ expectation values
+// E_p = sum_r conj(psi_r) sum_{transitions of placement p from row r} amp * psi[col]
+// for a batch of placements (term instances on concrete sites), without
+// materialising their matrices: replaces one nbody_term + exp_val_data pass
+// per site pair / plaquette / n-body instance (twobody_term.py:232-264,
+// plaquette_term.py:272-279, nbody_term.py:194-217, qmb_state.py:630-653).
+// Four waves per block walk rows; lane l of a wave owns placements l, l+64,
+// ... of the batch and accumulates into its own LDS slots (rows in a fixed
+// order per wave), the block sums its waves in order, and a finish pass sums
+// the blocks in order: deterministic.
+constexpr int kExpRounds = 8;  // placements per launch: 64 * kExpRounds
+constexpr int kExpWaves = 4;
+
+__device__ inline double2 to_c2(double v) { return make_double2(v, 0.0); }
+__device__ inline double2 to_c2(double2 v) { return v; }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// sum over placement P's transitions from this row of amp * psi[col]
+template <class KeyT, class PT>
+__device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, const uint8_t* cfg, KeyT key,
+                                          int64_t r, const PT* __restrict__ psi) {
+  const KeyT* keys = static_cast<const KeyT*>(A.keys);
+  const KeyT* strides = static_cast<const KeyT*>(A.strides);
+  if (P.nops == 1) {
+    const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
+    if (sqrt(v.x * v.x + v.y * v.y) < kOpTol) return make_double2(0.0, 0.0);
+    return cmul(v, to_c2(psi[r]));
+  }
+  int beg[kMaxOps], cnt[kMaxOps], idx[kMaxOps];
+  KeyT stride[kMaxOps];
+  KeyT base_delta = 0;
+  bool empty = false;
+#pragma unroll
+  for (int f = 0; f < kMaxOps; ++f) {
+    beg[f] = 0;
+    cnt[f] = 1;
+    idx[f] = 0;
+    stride[f] = 0;
+    if (f < P.nops) {
+      const int a = cfg[P.site[f]];
+      const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
+      beg[f] = off[a];
+      cnt[f] = off[a + 1] - off[a];
+      if (cnt[f] == 0) empty = true;
+      stride[f] = strides[P.site[f]];
+      base_delta -= (KeyT)a * stride[f];
+    }
+  }
+  double2 sum = make_double2(0.0, 0.0);
+  if (empty) return sum;
+  while (true) {
+    KeyT delta = base_delta;
+    double2 amp = make_double2(1.0, 0.0);
+#pragma unroll
+    for (int f = 0; f < kMaxOps; ++f) {
+      if (f < P.nops) {
+        const int e = beg[f] + idx[f];
+        delta += (KeyT)A.trans_ket[e] * stride[f];
+        amp = cmul(amp, A.trans_val[e]);
+      }
+    }
+    if (P.dagger) amp.y = -amp.y;
+    const KeyT k2 = key + delta;
+    int64_t col = r;
+    if (k2 != key) {
+      col = lower_bound_key<KeyT>(keys, A.n_states, k2);
+      if (col >= A.n_states || keys[col] != k2) {
+        atomicOr(A.err, kErrMissing);
+        col = -1;
+      }
+    }
+    if (col >= 0) sum = cadd(sum, cmul(amp, to_c2(psi[col])));
+    bool carry = true;
+#pragma unroll
+    for (int f = kMaxOps - 1; f >= 0; --f) {
+      if (carry && f < P.nops) {
+        if (++idx[f] < cnt[f]) {
+          carry = false;
+        } else {
+          idx[f] = 0;
+        }
+      }
+    }
+    if (carry) break;
+  }
+  return sum;
+}
+
+template <class KeyT, class PT>
+__global__ void __launch_bounds__(64 * kExpWaves) expval_rows(AsmDev A, int p0, int np, const PT* __restrict__ psi,
+                                                              double2* __restrict__ partial) {
+  __shared__ double2 acc[kExpWaves][64 * kExpRounds];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = lane; i < 64 * kExpRounds; i += 64) acc[wave][i] = make_double2(0.0, 0.0);
+  const KeyT* keys = static_cast<const KeyT*>(A.keys);
+  for (int64_t r = (int64_t)blockIdx.x * kExpWaves + wave; r < A.n_states;
+       r += (int64_t)gridDim.x * kExpWaves) {
+    const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
+    const KeyT key = keys[r];
+    double2 pr = to_c2(psi[r]);
+    pr.y = -pr.y;  // conj(psi_r)
+#pragma unroll 1
+    for (int p = lane; p < np; p += 64) {
+      const Placement P = A.place[p0 + p];
+      const double2 s = cmul(placement_apply<KeyT, PT>(A, P, cfg, key, r, psi), make_double2(P.s_re, P.s_im));
+      acc[wave][p] = cadd(acc[wave][p], cmul(pr, s));
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    double2 t = acc[0][p];
+#pragma unroll
+    for (int w = 1; w < kExpWaves; ++w) t = cadd(t, acc[w][p]);
+    partial[(int64_t)blockIdx.x * np + p] = t;
+  }
+}
+
+// out[p] = sum over blocks of partial[b][p]: one wave per placement, lanes over
+// blocks in order, fixed butterfly.
+__global__ void __launch_bounds__(256) expval_finish(const double2* __restrict__ partial, int nblk, int np,
+                                                     double2* __restrict__ out) {
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;  // whole waves exit together
+  double2 s = make_double2(0.0, 0.0);
+  for (int b = lane; b < nblk; b += 64) s = cadd(s, partial[(int64_t)b * np + p]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    s.x += __shfl_xor(s.x, o, 64);
+    s.y += __shfl_xor(s.y, o, 64);
+  }
+  if (lane == 0) out[p] = s;
+}
+
+}  // namespace
+
+int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
+                      int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                      const int32_t* place_tables, const double* place_strength,
+                      const int32_t* place_dagger, const int32_t* place_term, int32_t n_terms,
+                      int32_t value_type, hipStream_t st, Csr* out) {
+  Program prog;
+  EDLGT_TRY(prepare_program(B, d_max, n_tables, tables, n_place, place_nops, place_sites, place_tables,
+                            place_strength, place_dagger, place_term, n_terms, value_type, st, prog));
+  const AsmDev& A = prog.A;
+  const int cap = prog.cap;
+  DevBuf& b_err = prog.err;
   const int64_t N = B.n_states;
   out->n_rows = N;
   out->value_type = value_type;
@@ -469,6 +629,42 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
                          n_terms > 1 ? out->term.as<uint8_t>() : nullptr, st));
   EDLGT_TRY(check_err(b_err, st));
   EDLGT_TRY(csr_prepare_stream(*out, st));
+  return EDLGT_OK;
+}
+
+int placements_expval(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables, int32_t n_place,
+                      const int32_t* place_nops, const int32_t* place_sites, const int32_t* place_tables,
+                      const double* place_strength, const void* psi, int32_t vec_type, double* out_host,
+                      hipStream_t st) {
+  if (vec_type != EDLGT_VALUE_F64 && vec_type != EDLGT_VALUE_C128)
+    return set_error(EDLGT_ERR_INVALID_ARG, "psi must be F64 or C128");
+  Program prog;
+  EDLGT_TRY(prepare_program(B, d_max, n_tables, tables, n_place, place_nops, place_sites, place_tables,
+                            place_strength, nullptr, nullptr, 1, EDLGT_VALUE_C128, st, prog));
+  if (n_place == 0) return EDLGT_OK;
+  const int64_t N = B.n_states;
+  constexpr int kBatch = 64 * kExpRounds;
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((N + kExpWaves - 1) / kExpWaves, 2048));
+  DevBuf partial, out;
+  EDLGT_TRY(partial.alloc((size_t)nblk * kBatch * sizeof(double2)));
+  EDLGT_TRY(out.alloc((size_t)n_place * sizeof(double2)));
+  for (int p0 = 0; p0 < n_place; p0 += kBatch) {
+    const int np = std::min(kBatch, n_place - p0);
+#define EXP_LAUNCH(KT, PT)                                                                          \
+  hipLaunchKernelGGL((expval_rows<KT, PT>), dim3(nblk), dim3(64 * kExpWaves), 0, st, prog.A, p0, np, \
+                     static_cast<const PT*>(psi), partial.as<double2>())
+    if (B.key_words == 2) {
+      if (vec_type == EDLGT_VALUE_C128) EXP_LAUNCH(u128, double2); else EXP_LAUNCH(u128, double);
+    } else {
+      if (vec_type == EDLGT_VALUE_C128) EXP_LAUNCH(uint64_t, double2); else EXP_LAUNCH(uint64_t, double);
+    }
+#undef EXP_LAUNCH
+    hipLaunchKernelGGL(expval_finish, dim3((np + 3) / 4), dim3(256), 0, st, partial.as<double2>(), nblk, np,
+                       out.as<double2>() + p0);
+    EDLGT_HIP(hipGetLastError());
+  }
+  EDLGT_HIP(hipMemcpyAsync(out_host, out.p, (size_t)n_place * sizeof(double2), hipMemcpyDeviceToHost, st));
+  EDLGT_TRY(check_err(prog.err, st));  // synchronises
   return EDLGT_OK;
 }
 

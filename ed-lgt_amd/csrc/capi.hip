@@ -197,6 +197,17 @@ int edlgt_basis_destroy(edlgt_basis* b) {
   return EDLGT_OK;
 }
 
+int edlgt_expval_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables, const double* tables,
+                       int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                       const int32_t* place_tables, const double* place_strength, const void* psi,
+                       int32_t vec_type, double* out_host, void* stream) {
+  EDLGT_REQUIRE(b && tables && psi && (out_host || n_place == 0), "null argument");
+  EDLGT_REQUIRE(n_place == 0 || (place_nops && place_sites && place_tables && place_strength),
+                "null placement arrays");
+  return placements_expval(*b, d_max, n_tables, tables, n_place, place_nops, place_sites, place_tables,
+                           place_strength, psi, vec_type, out_host, (hipStream_t)stream);
+}
+
 int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
                             const double* tables, int32_t n_place, const int32_t* place_nops,
                             const int32_t* place_sites, const int32_t* place_tables,

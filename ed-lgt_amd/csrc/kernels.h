@@ -19,6 +19,10 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
                       const int32_t* place_tables, const double* place_strength,
                       const int32_t* place_dagger, const int32_t* place_term, int32_t n_terms,
                       int32_t value_type, hipStream_t st, Csr* out);
+int placements_expval(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables, int32_t n_place,
+                      const int32_t* place_nops, const int32_t* place_sites, const int32_t* place_tables,
+                      const double* place_strength, const void* psi, int32_t vec_type, double* out_host,
+                      hipStream_t st);
 
 int lanes_per_row(const Csr& h);
 // Y[:, b] = H X[:, b] for b < n_rhs; xtype is the element type of X

@@ -78,6 +78,7 @@ SIGNATURES = {
     "edlgt_server_destroy": (_i32, [_p]),
     "edlgt_lanczos": (_i32, [_p, _i32, _f64, _i32, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "edlgt_basis_marginals": (_i32, [_p, _p, _i32, _i32, _p, _p]),
+    "edlgt_expval_terms": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "edlgt_krylov_evolve": (_i32, [_p, _p, _i32, _p, _f64, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
     "edlgt_krylov_create": (_i32, [_p, _p, _i32, _i32, _i32, _i32, _p, _p]),
     "edlgt_krylov_run": (_i32, [_p, _p, _f64, _i64, _p, _p, _p]),

@@ -143,6 +143,50 @@ class SectorBasis:
             self._h = None
 
 
+def _pack_placements(placements):
+    """Placement dicts {sites, tables, strength[, dagger, term]} -> the C ABI's
+    arrays (nops, sites[8], tables[8], strength (re, im), dagger, term)."""
+    n_place = len(placements)
+    nops = np.zeros(n_place, dtype=np.int32)
+    sites = np.zeros((n_place, 8), dtype=np.int32)
+    tabs = np.zeros((n_place, 8), dtype=np.int32)
+    stren = np.zeros((n_place, 2), dtype=np.float64)
+    dag = np.zeros(n_place, dtype=np.int32)
+    term = np.zeros(n_place, dtype=np.int32)
+    for p, pl in enumerate(placements):
+        k = len(pl["sites"])
+        if k > 8:
+            raise ValueError("a placement acts on at most 8 sites")
+        nops[p] = k
+        sites[p, :k] = pl["sites"]
+        tabs[p, :k] = pl["tables"]
+        s = complex(pl.get("strength", 1.0))
+        stren[p] = (s.real, s.imag)
+        dag[p] = 1 if pl.get("dagger") else 0
+        term[p] = int(pl.get("term", 0))
+    return nops, sites, tabs, stren, dag, term
+
+
+def expval_terms(basis, tables, placements, psi, stream=None):
+    """<psi| strength_p prod_f O_f |psi> for every placement (complex array),
+    one device pass over the basis for all of them (edlgt_expval_terms).
+    psi: device tensor (complex128 or float64) over the sector basis."""
+    tables = np.ascontiguousarray(tables, dtype=np.complex128)
+    n_tables, d_max = tables.shape[0], tables.shape[1]
+    nops, sites, tabs, stren, _, _ = _pack_placements(placements)
+    torch = _torch()
+    psi = psi.contiguous()
+    vt = L.VALUE_C128 if psi.dtype == torch.complex128 else L.VALUE_F64
+    if vt == L.VALUE_F64 and psi.dtype != torch.float64:
+        raise ValueError("psi must be complex128 or float64")
+    out = np.zeros((len(placements), 2))
+    L.check(L.load().edlgt_expval_terms(basis.handle, d_max, n_tables, tables.ctypes.data, len(placements),
+                                        nops.ctypes.data, sites.ctypes.data, tabs.ctypes.data,
+                                        stren.ctypes.data, L.ptr(psi), vt, out.ctypes.data,
+                                        L.stream_handle(stream)))
+    return out[:, 0] + 1j * out[:, 1]
+
+
 class CSRMatrix:
     """Canonical CSR Hamiltonian on the device (rows ascending, columns sorted
     within rows, duplicates summed, explicit zeros kept)."""
@@ -156,21 +200,7 @@ class CSRMatrix:
         tables = np.ascontiguousarray(tables, dtype=np.complex128)
         n_tables, d_max = tables.shape[0], tables.shape[1]
         n_place = len(placements)
-        nops = np.zeros(n_place, dtype=np.int32)
-        sites = np.zeros((n_place, 8), dtype=np.int32)
-        tabs = np.zeros((n_place, 8), dtype=np.int32)
-        stren = np.zeros((n_place, 2), dtype=np.float64)
-        dag = np.zeros(n_place, dtype=np.int32)
-        term = np.zeros(n_place, dtype=np.int32)
-        for p, pl in enumerate(placements):
-            k = len(pl["sites"])
-            nops[p] = k
-            sites[p, :k] = pl["sites"]
-            tabs[p, :k] = pl["tables"]
-            s = complex(pl["strength"])
-            stren[p] = (s.real, s.imag)
-            dag[p] = 1 if pl.get("dagger") else 0
-            term[p] = int(pl.get("term", 0))
+        nops, sites, tabs, stren, dag, term = _pack_placements(placements)
         h = C.c_void_p()
         st = L.stream_handle(stream) if stream is not None else None
         if n_terms == 1:

@@ -69,6 +69,23 @@ class Lattice:
                 out.append((i, j))
         return out
 
+    def nbody_sites(self, i, distances):
+        """NBodyTerm.get_nbody_neighbors (nbody_term.py:222-262): site i plus the
+        sites at each displacement from it (periodic axes wrap), or None when a
+        displacement leaves an open axis."""
+        c0 = self.coords(i)
+        out = [i]
+        for dist in distances:
+            c = list(c0)
+            for a in range(self.dim):
+                c[a] += int(dist[a])
+                if not self.has_obc[a]:
+                    c[a] %= self.lvals[a]
+            if not all(0 <= c[a] < self.lvals[a] for a in range(self.dim)):
+                return None
+            out.append(self.index(c))
+        return out
+
     def plaquettes(self, axes=("x", "y")):
         """[(i, i+a, i+b, i+a+b)] in site order."""
         out = []

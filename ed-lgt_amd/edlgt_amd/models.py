@@ -257,20 +257,61 @@ class QuantumModel:
     # -- observables (quantum_model.py:1158-1333)
     def get_observables(self, local_obs=[], twobody_obs=[], plaquette_obs=[], nbody_obs=[],
                         nbody_dist=[], twobody_axes=None):
-        if twobody_obs or nbody_obs:
-            raise NotImplementedError("two-body / n-body correlators are not on the device path")
+        """quantum_model.py:1158-1235.  Local (diagonal) observables come from
+        site marginals; two-body correlators, plaquettes and n-body terms from
+        one device expectation pass each (edlgt_expval_terms)."""
         self.local_obs = list(local_obs)
+        self.twobody_obs = [list(p) for p in twobody_obs]
+        self.twobody_axes = twobody_axes
         self.plaquette_obs = [list(p) for p in plaquette_obs]
-        self._plaq_ops = {}
+        self.nbody_obs = [list(p) for p in nbody_obs]
+        if len(nbody_dist) < len(self.nbody_obs):
+            raise ValueError("nbody_dist needs one distance list per n-body observable")
+        self.nbody_dist = [list(d) for d in nbody_dist]
+        for names, dist in zip(self.nbody_obs, self.nbody_dist):
+            if len(dist) != len(names) - 1:
+                raise ValueError(f"The distances length should be len(op_list)-1, not {len(dist)}")
+            if any(len(d) != self.dim for d in dist):
+                raise ValueError(f"Each entry in distances must have length len(lvals)={self.dim}.")
+        if twobody_axes is not None:
+            for ax in twobody_axes:
+                if ax not in self.directions:
+                    raise ValueError(f"axis should be in {self.directions}: got {ax}")
+        self._obs_groups = {}
+        for names in self.twobody_obs:
+            if len(names) != 2:
+                raise ValueError("a two-body observable names two operators")
+            pairs = [(i, j) for i in range(self.n_sites) for j in range(i + 1, self.n_sites)]
+            self._obs_groups["_".join(names)] = ("twobody", names, pairs)
         for names in self.plaquette_obs:
             axes = (names[-1][3], names[-1][6])
-            plaqs = self.lattice.plaquettes(axes)
-            self._new_program()
-            for sl in plaqs:
-                tabs = [self._table(n, q) for n, q in zip(names, sl)]
-                self._placements.append(dict(sites=list(sl), tables=tabs, strength=1.0 / len(plaqs), dagger=0))
-            self._plaq_ops["_".join(names)] = CSRMatrix(self.basis, np.stack(self._tables),
-                                                        self._placements, L.VALUE_C128)
+            self._obs_groups["_".join(names)] = ("plaquette", names, self.lattice.plaquettes(axes))
+        for names, dist in zip(self.nbody_obs, self.nbody_dist):
+            sl = [self.lattice.nbody_sites(i, dist) for i in range(self.n_sites)]
+            self._obs_groups["_".join(names)] = ("nbody", names, [x for x in sl if x is not None])
+
+    def _group_expvals(self, psi_device):
+        """Complex <psi|O_p|psi> of every placement of every correlator group, in
+        one device pass."""
+        from .device import expval_terms
+        tables, ids, placements, spans = [], {}, [], {}
+
+        def tab(name, site):
+            key = (name, self.lattice_labels[site])
+            if key not in ids:
+                ids[key] = len(tables)
+                tables.append(self.ops[name][site])
+            return ids[key]
+
+        for key, (kind, names, site_lists) in self._obs_groups.items():
+            a = len(placements)
+            for sl in site_lists:
+                placements.append(dict(sites=list(sl), tables=[tab(n, q) for n, q in zip(names, sl)]))
+            spans[key] = (a, len(placements))
+        if not placements:
+            return {}
+        vals = expval_terms(self.basis, np.stack(tables), placements, psi_device)
+        return {k: vals[a:b] for k, (a, b) in spans.items()}
 
     def _marginals(self, psi_device):
         d_max = int(self.loc_dims.max())
@@ -296,11 +337,24 @@ class QuantumModel:
             marg = self.H.marginals[index] if self.H.marginals is not None else self._marginals(state.psi_device)
         for obs in self.local_obs:
             self.res[obs] = self.local_expvals(marg, obs)
-        for names in self.plaquette_obs:
-            key = "_".join(names)
-            psi = state.psi_device.to(torch.complex128)
-            y = self._plaq_ops[key].matvec(psi)
-            self.res[key] = float(torch.vdot(psi, y).real)
+        groups = getattr(self, "_obs_groups", {})
+        if groups:
+            vals = self._group_expvals(state.psi_device)
+            for key, (kind, names, site_lists) in groups.items():
+                v = np.real(vals[key])
+                if kind == "twobody":
+                    # lattice_twobody_exp_val (twobody_term.py:232-264): symmetric, zero diagonal
+                    corr = np.zeros((self.n_sites, self.n_sites))
+                    for (i, j), x in zip(site_lists, v):
+                        corr[i, j] = corr[j, i] = x
+                    self.res[key] = corr
+                elif kind == "plaquette":
+                    # PlaquetteTerm.get_expval (plaquette_term.py:246-261): mean over plaquettes
+                    self.res[key] = float(np.mean(v)) if v.size else float("nan")
+                    self.res[key + "_obs"] = v
+                else:
+                    # NBodyTerm.get_expval (nbody_term.py:194-220): one value per valid start site
+                    self.res[key] = v
 
     def link_avg(self, obs_name):
         """Average of res[f'{obs_name}_p{d}'] over the existing links (quantum_model.py:1271-1299)."""

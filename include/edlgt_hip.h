@@ -301,6 +301,27 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
                           double* out_host, void* stream);
 
 /* ======================================================================
+ * Expectation values of term instances without building their matrices:
+ *   out_host[2p], out_host[2p+1] = Re, Im of <psi| strength_p prod_f O_f |psi>
+ * for every placement p (same tables / placement arrays as
+ * edlgt_hamiltonian_build, no dagger flags: each placement is one operator
+ * product on its sites, bra transitions O[bra][ket], transitions below 1e-10
+ * dropped, single-factor placements diagonal-only as in the build).
+ * Replaces the per-instance nbody_term + exp_val_data passes of
+ * lattice_twobody_exp_val (edlgt/modeling/twobody_term.py:232-264),
+ * PlaquetteTerm.get_expval (edlgt/modeling/plaquette_term.py:165-263) and
+ * NBodyTerm.get_expval (edlgt/modeling/nbody_term.py:164-220) with
+ * exp_val_data (edlgt/modeling/qmb_state.py:630-653): one pass over the
+ * basis serves every placement.  psi: device, vec_type C128 or F64.  A ket
+ * outside the sector is an error (EDLGT_ERR_KEY_NOT_FOUND).  Deterministic
+ * (fixed summation order).
+ * ====================================================================== */
+int edlgt_expval_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables, const double* tables,
+                       int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                       const int32_t* place_tables, const double* place_strength, const void* psi,
+                       int32_t vec_type, double* out_host, void* stream);
+
+/* ======================================================================
  * Real-time evolution psi <- exp(-i H dt)^n_steps psi by Krylov (Lanczos)
  * exponentiation with an m-dimensional basis per step; replaces
  * QMB_hamiltonian.time_evolution -> expm_multiply(-1j*H, psi0, ...)

@@ -223,6 +223,25 @@ def nbody_triplets(op_stack, sites, configs, keys, loc_dims):
     return np.concatenate(rows_out), np.concatenate(cols_out), np.concatenate(vals_out)
 
 
+def expval_nbody(op_list, sites, configs, keys, loc_dims, psi):
+    """<psi|O|psi> of one operator product on concrete sites: exp_val_data
+    (qmb_state.py:630-653) over the sector triplets of nbody_term
+    (sym_qmb_operations.py:167-602)."""
+    r, c, v = nbody_triplets(np.stack(op_list), sites, configs, keys, loc_dims)
+    return complex(np.sum(np.conj(psi[r]) * v * psi[c]))
+
+
+def twobody_corr(op1, op2, configs, keys, loc_dims, psi):
+    """lattice_twobody_exp_val (twobody_term.py:232-264): corr[i, j] =
+    corr[j, i] = Re <psi|O1_i O2_j|psi> for i < j, zero diagonal."""
+    n = configs.shape[1]
+    corr = np.zeros((n, n))
+    for i in range(n):
+        for j in range(i + 1, n):
+            corr[i, j] = corr[j, i] = expval_nbody([op1, op2], [i, j], configs, keys, loc_dims, psi).real
+    return corr
+
+
 class Triplets:
     """Accumulator mirroring QMB_hamiltonian.add_term/build (qmb_hamiltonian.py:155-225)."""
 

@@ -1,5 +1,6 @@
-"""GPU: observables (diagonal locals via site marginals, plaquettes via an
-assembled average operator) against the reference's KAT observables, and
+"""GPU: observables (diagonal locals via site marginals, plaquettes via the
+device expectation pass over all plaquettes) against the reference's KAT
+observables, and
 Krylov exp(-iHt) against the oracle's expm_multiply / exact dense propagation."""
 import numpy as np
 import pytest

@@ -117,3 +117,34 @@ def test_su2_krylov_without_full_reorth(gpu):
     assert np.array_equal(mod.H.marginals, marg_full[10:])
     mod.H.release_krylov()
     assert mod.H._krylov is None
+
+
+@pytest.mark.parametrize("obc", [[False], [True]])
+def test_su2_correlators_and_energy_closure(gpu, obc):
+    """Two-body correlators (all pairs) and n-body hopping expectation values
+    from one device pass each, against the oracle's triplet expectation values
+    at 1e-12; and the ground-state energy re-assembled from the measured local,
+    mass and hopping terms equals the Lanczos eigenvalue (1e-10)."""
+    from oracle.model import expval_nbody, twobody_corr
+    mod = _model([6], obc, False, 6)
+    mod.build_Hamiltonian(1.0, 5.0)
+    mod.diagonalize_Hamiltonian(1, "sparse")
+    mod.get_observables(local_obs=["E_square", "N-1"], twobody_obs=[["N_tot", "N_single"]],
+                        nbody_obs=[["Qpx_dag", "Qmx"]], nbody_dist=[[(1,)]])
+    mod.measure_observables(0)
+    psi = mod.H.Npsi[0].psi
+    orc = SU2Oracle(False, [6], obc, sectors=[6])
+    assert np.array_equal(orc.configs, mod.sector_configs)
+    args = (orc.configs, orc.keys, orc.loc_dims, psi)
+    corr = twobody_corr(orc.ops["N_tot"], orc.ops["N_single"], *args)
+    assert np.max(np.abs(mod.res["N_tot_N_single"] - corr)) < 1e-12
+    bonds = [(i, i + 1) for i in range(5)] + ([] if obc[0] else [(5, 0)])
+    hop = np.array([expval_nbody([orc.ops["Qpx_dag"], orc.ops["Qmx"]], list(b), *args) for b in bonds])
+    assert mod.res["Qpx_dag_Qmx"].shape == (len(bonds),)
+    assert np.max(np.abs(mod.res["Qpx_dag_Qmx"] - hop.real)) < 1e-12
+    # closure: <H> from the measured pieces (SU2_model.py:200-355 term list)
+    c = orc.couplings(1.0, 5.0)
+    mass = np.array([c["m_even"] if i % 2 == 0 else c["m_odd"] for i in range(6)])
+    e = c["E"] * np.sum(mod.res["E_square"]) + np.sum(mass * mod.res["N-1"])
+    e += sum(2 * (c["tx_even" if b[0] % 2 == 0 else "tx_odd"] * h).real for b, h in zip(bonds, hop))
+    assert abs(e - mod.H.Nenergies[0] * 6) < 1e-10

@@ -177,3 +177,36 @@ def test_cg_against_sympy():
 def test_su2_sector_counts():
     assert SU2Oracle(False, [10], [True], sectors=[10]).N == 8953
     assert SU2Oracle(True, [3, 2], [False, False]).N > 0
+
+
+def test_expval_oracle_equals_bruteforce():
+    """The oracle's n-body expectation values (sector triplets + exp_val_data)
+    equal psi^H (O_i x O_j) psi over the full Kronecker space, L=4 PBC, for a
+    diagonal two-body pair at every distance and the nearest-neighbour hopping."""
+    from oracle.model import expval_nbody, twobody_corr
+    orc = SU2Oracle(False, [4], [False], sectors=[4])
+    d = 6
+    rng = np.random.default_rng(4)
+    psi = rng.standard_normal(orc.N) + 1j * rng.standard_normal(orc.N)
+    psi /= np.linalg.norm(psi)
+    full = np.zeros(d ** 4, dtype=complex)
+    full[encode(orc.configs, orc.loc_dims)] = psi
+
+    def place(mats):
+        out = sp.identity(1, format="csr")
+        for s in range(4):
+            out = sp.kron(out, sp.csr_matrix(mats.get(s, np.eye(d))), format="csr")
+        return out
+
+    corr = twobody_corr(orc.ops["N_tot"], orc.ops["N_single"], orc.configs, orc.keys, orc.loc_dims, psi)
+    for i in range(4):
+        assert corr[i, i] == 0.0
+        for j in range(i + 1, 4):
+            O = place({i: orc.ops["N_tot"][i][:d, :d], j: orc.ops["N_single"][j][:d, :d]})
+            ref = np.vdot(full, O @ full).real
+            assert abs(corr[i, j] - ref) < 1e-13 and corr[j, i] == corr[i, j]
+    for i in range(4):
+        j = (i + 1) % 4
+        got = expval_nbody([orc.ops["Qpx_dag"], orc.ops["Qmx"]], [i, j], orc.configs, orc.keys, orc.loc_dims, psi)
+        O = place({i: orc.ops["Qpx_dag"][i][:d, :d]}) @ place({j: orc.ops["Qmx"][j][:d, :d]})
+        assert abs(got - np.vdot(full, O @ full)) < 1e-13
