@@ -456,6 +456,50 @@ int edlgt_krylov_evolve(edlgt_csr* h, const edlgt_basis* b, int32_t precision, v
                     psi_history, (hipStream_t)stream);
 }
 
+int edlgt_momentum_build(const edlgt_basis* b, int32_t n_dims, const int32_t* lvals, const int32_t* unit_cell,
+                         const int32_t* k_vals, void* stream, edlgt_momentum** out) {
+  EDLGT_REQUIRE(b && lvals && unit_cell && k_vals && out, "null argument");
+  *out = nullptr;
+  auto m = std::make_unique<edlgt_momentum>();
+  EDLGT_TRY(momentum_build(*b, n_dims, lvals, unit_cell, k_vals, (hipStream_t)stream, m.get()));
+  *out = m.release();
+  return EDLGT_OK;
+}
+
+int edlgt_momentum_info(const edlgt_momentum* m, int64_t* n_rows, int64_t* n_cols, int64_t* nnz,
+                        int32_t* is_complex) {
+  EDLGT_REQUIRE(m, "null argument");
+  if (n_rows) *n_rows = m->n_rows;
+  if (n_cols) *n_cols = m->n_cols;
+  if (nnz) *nnz = m->nnz;
+  if (is_complex) *is_complex = m->is_complex;
+  return EDLGT_OK;
+}
+
+int edlgt_momentum_copy(const edlgt_momentum* m, int64_t* col_ptr, int32_t* row_idx, double* data) {
+  EDLGT_REQUIRE(m && col_ptr && row_idx && data, "null argument");
+  return momentum_copy(*m, col_ptr, row_idx, data);
+}
+
+int edlgt_momentum_apply(const edlgt_momentum* m, int32_t forward, const void* in, void* out, void* stream) {
+  EDLGT_REQUIRE(m && in && out, "null argument");
+  return momentum_apply(*m, forward ? 1 : 0, in, out, (hipStream_t)stream);
+}
+
+int edlgt_csr_momentum(const edlgt_csr* h, const edlgt_momentum* m, void* stream, edlgt_csr** out) {
+  EDLGT_REQUIRE(h && m && out, "null argument");
+  *out = nullptr;
+  auto c = std::make_unique<edlgt_csr>();
+  EDLGT_TRY(csr_momentum(*h, *m, (hipStream_t)stream, c.get()));
+  *out = c.release();
+  return EDLGT_OK;
+}
+
+int edlgt_momentum_destroy(edlgt_momentum* m) {
+  delete m;
+  return EDLGT_OK;
+}
+
 struct edlgt_krylov {
   KrylovSession* s = nullptr;
   const edlgt_basis* b = nullptr;

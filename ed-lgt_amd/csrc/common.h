@@ -122,6 +122,18 @@ struct Csr {
   DevBuf tab_unit;   // dict values with all term coefficients 1 (double or double2)
 };
 
+// Momentum-sector projector B (n_rows x n_cols): CSC (col_ptr, row_idx,
+// data) and, per sector row, its column (-1: in no kept column) and amplitude.
+struct Momentum {
+  int64_t n_rows = 0, n_cols = 0, nnz = 0;
+  int32_t is_complex = 0;  // 1 for k != 0 (complex Bloch phases)
+  DevBuf col_ptr;  // int64 [n_cols + 1]
+  DevBuf row_idx;  // int32 [nnz], ascending within a column
+  DevBuf data;     // double2 [nnz]
+  DevBuf row_col;  // int32 [n_rows]
+  DevBuf row_val;  // double2 [n_rows]
+};
+
 inline size_t value_bytes(int32_t vt) {
   switch (vt) {
     case EDLGT_VALUE_F64: return 8;
@@ -143,3 +155,4 @@ inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
 
 struct edlgt_basis : edlgt::Basis {};
 struct edlgt_csr : edlgt::Csr {};
+struct edlgt_momentum : edlgt::Momentum {};

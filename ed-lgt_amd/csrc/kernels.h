@@ -58,6 +58,12 @@ int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter,
 int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
                int m, int reorth, int d_max, double* out_marg, double* out_err, void* history,
                hipStream_t st);
+int momentum_build(const Basis& B, int n_dims, const int32_t* lvals, const int32_t* unit_cell,
+                   const int32_t* k_vals, hipStream_t st, Momentum* out);
+int momentum_copy(const Momentum& m, int64_t* col_ptr, int32_t* row_idx, double* data);
+int momentum_apply(const Momentum& m, int forward, const void* in, void* out, hipStream_t st);
+int csr_momentum(const Csr& h, const Momentum& m, hipStream_t st, Csr* out);
+
 struct KrylovSession;
 int krylov_create(Csr& h, const Basis* basis, int precision, int m, int reorth, int d_max, hipStream_t st,
                   KrylovSession** out);

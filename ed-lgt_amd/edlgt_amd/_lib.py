@@ -80,6 +80,12 @@ SIGNATURES = {
     "edlgt_basis_marginals": (_i32, [_p, _p, _i32, _i32, _p, _p]),
     "edlgt_expval_terms": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "edlgt_krylov_evolve": (_i32, [_p, _p, _i32, _p, _f64, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
+    "edlgt_momentum_build": (_i32, [_p, _i32, _p, _p, _p, _p, _p]),
+    "edlgt_momentum_info": (_i32, [_p, _p, _p, _p, _p]),
+    "edlgt_momentum_copy": (_i32, [_p, _p, _p, _p]),
+    "edlgt_momentum_apply": (_i32, [_p, _i32, _p, _p, _p]),
+    "edlgt_csr_momentum": (_i32, [_p, _p, _p, _p]),
+    "edlgt_momentum_destroy": (_i32, [_p]),
     "edlgt_krylov_create": (_i32, [_p, _p, _i32, _i32, _i32, _i32, _p, _p]),
     "edlgt_krylov_run": (_i32, [_p, _p, _f64, _i64, _p, _p, _p]),
     "edlgt_krylov_destroy": (_i32, [_p]),

@@ -220,6 +220,27 @@ class CSRMatrix:
         self.n, self.nnz, self.value_type = int(n.value), int(nnz.value), int(vt.value)
         self.basis = basis
 
+    @classmethod
+    def _wrap(cls, h, basis, n_terms=1):
+        """A CSRMatrix owning an existing edlgt_csr handle."""
+        self = cls.__new__(cls)
+        self._h = h
+        self.n_terms = int(n_terms)
+        n, nnz, vt = C.c_int64(), C.c_int64(), C.c_int32()
+        L.check(L.load().edlgt_csr_info(h, C.byref(n), C.byref(nnz), C.byref(vt)))
+        self.n, self.nnz, self.value_type = int(n.value), int(nnz.value), int(vt.value)
+        self.basis = basis
+        return self
+
+    def momentum(self, mb, stream=None):
+        """B^H H B on momentum sector mb (edlgt_csr_momentum): a new matrix of
+        mb.n_cols rows; this one is left unchanged."""
+        h = C.c_void_p()
+        L.check(L.load().edlgt_csr_momentum(self._h, mb.handle, L.stream_handle(stream), C.byref(h)))
+        out = CSRMatrix._wrap(h, None)
+        out.momentum_basis = mb
+        return out
+
     @property
     def handle(self):
         return self._h
@@ -476,3 +497,67 @@ class KrylovSession:
 
     def __del__(self):
         self.close()
+
+
+class MomentumBasis:
+    """Projector B onto momentum k of the block translations by unit_cell
+    along each axis (edlgt_momentum_*; get_momentum_basis,
+    translational_sym.py:1307-1359).  n_cols columns, one per compatible
+    translation orbit; to_scipy() gives B, expand() / project() apply B and B^H
+    to device vectors."""
+
+    def __init__(self, basis, lvals, unit_cell, k_vals, stream=None):
+        lv = np.ascontiguousarray(lvals, dtype=np.int32)
+        uc = np.ascontiguousarray(unit_cell, dtype=np.int32)
+        kv = np.ascontiguousarray(np.atleast_1d(k_vals), dtype=np.int32)
+        if not (lv.size == uc.size == kv.size):
+            raise ValueError("lvals, unit_cell and k_vals need one entry per axis")
+        m = C.c_void_p()
+        L.check(L.load().edlgt_momentum_build(basis.handle, int(lv.size), lv.ctypes.data, uc.ctypes.data,
+                                              kv.ctypes.data, L.stream_handle(stream), C.byref(m)))
+        self._m = m
+        self.basis = basis
+        self.k_vals, self.unit_cell = kv.copy(), uc.copy()
+        nr, nc, nnz, cx = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        L.check(L.load().edlgt_momentum_info(m, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(cx)))
+        self.n_rows, self.n_cols, self.nnz, self.is_complex = int(nr.value), int(nc.value), int(nnz.value), bool(cx.value)
+
+    @property
+    def handle(self):
+        return self._m
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        cp = np.zeros(self.n_cols + 1, dtype=np.int64)
+        ri = np.zeros(max(self.nnz, 1), dtype=np.int32)
+        dv = np.zeros((max(self.nnz, 1), 2))
+        L.check(L.load().edlgt_momentum_copy(self._m, cp.ctypes.data, ri.ctypes.data, dv.ctypes.data))
+        v = dv[: self.nnz, 0] + 1j * dv[: self.nnz, 1]
+        return sp.csc_matrix((v, ri[: self.nnz], cp), shape=(self.n_rows, self.n_cols))
+
+    def _apply(self, forward, x, stream=None):
+        torch = _torch()
+        x = x.to(device="cuda", dtype=torch.complex128).contiguous()
+        if x.numel() != (self.n_cols if forward else self.n_rows):
+            raise ValueError("vector length does not match the momentum basis")
+        y = torch.empty(self.n_rows if forward else self.n_cols, dtype=torch.complex128, device="cuda")
+        L.check(L.load().edlgt_momentum_apply(self._m, 1 if forward else 0, L.ptr(x), L.ptr(y),
+                                              L.stream_handle(stream)))
+        return y
+
+    def expand(self, psi_k, stream=None):
+        """B psi_k: a momentum-sector vector in the sector basis."""
+        return self._apply(True, psi_k, stream)
+
+    def project(self, psi, stream=None):
+        """B^H psi."""
+        return self._apply(False, psi, stream)
+
+    def __del__(self):
+        m = getattr(self, "_m", None)
+        if m is not None and m.value:
+            try:
+                L.load().edlgt_momentum_destroy(m)
+            except Exception:
+                pass
+            self._m = None

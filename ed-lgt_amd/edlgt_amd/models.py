@@ -240,8 +240,41 @@ class QuantumModel:
         tables = np.stack(self._tables)
         vt = L.VALUE_F64 if real else L.VALUE_C128
         csr = CSRMatrix(self.basis, tables, self._placements, vt)
+        if self.momentum_basis is not None:
+            # B^H H B in the momentum sector (the reference's momentum triplets,
+            # translational_sym.py:1394-1969, behind QMB_hamiltonian.build)
+            csr = csr.momentum(self.momentum_basis)
+            self.H.shape = (csr.n, csr.n)
         self.H.set_matrix(csr, self.ham_format)
         return csr
+
+    # -- momentum sectors (quantum_model.py:213-276)
+    def set_momentum_sector(self, k_unit_cell_size, k_vals, TC_symmetry=False):
+        """Restricts the following builds to momentum k_vals of the block
+        translations by k_unit_cell_size sites (axis order and coordinates of
+        the reference's translational_sym module).  The Hamiltonian becomes
+        B^H H B on the n_cols momentum states; measure_observables maps the
+        eigenvectors back through B."""
+        from .device import MomentumBasis
+        if self.basis is None:
+            raise ValueError("symmetry sector_configs not defined yet")
+        if self.has_obc[0]:
+            raise ValueError("Momentum is not conserved with OBC.")
+        if TC_symmetry:
+            raise NotImplementedError("TC_symmetry momentum sectors are not on the device path")
+        k_vals = np.atleast_1d(np.asarray(k_vals, dtype=np.int32))
+        if k_vals.size != self.dim:
+            raise ValueError(f"expected k_vals shape {self.dim}, got {k_vals.shape}")
+        uc = np.ascontiguousarray(k_unit_cell_size, dtype=np.int32)
+        self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals)
+        logger.info(f"Momentum basis shape ({self.momentum_basis.n_rows},{self.momentum_basis.n_cols})")
+
+    def _sector_state(self, psi_device):
+        """A state over the sector basis (momentum states expanded through B)."""
+        mb = self.momentum_basis
+        if mb is not None and psi_device.numel() == mb.n_cols and mb.n_cols != mb.n_rows:
+            return mb.expand(psi_device)
+        return psi_device
 
     def diagonalize_Hamiltonian(self, n_eigs, format, print_results=False, **kw):
         self.H.diagonalize(n_eigs, format, self.loc_dims, print_results, **kw)
@@ -252,7 +285,9 @@ class QuantumModel:
         """quantum_model.py:613-615; per-step site marginals are kept in
         self.H.marginals so that dynamics observables need no extra pass."""
         d_max = int(self.loc_dims.max())
-        self.H.time_evolution(initial_state, time_line, self.loc_dims, basis=self.basis, d_max=d_max, **kw)
+        # per-step marginals need sector-basis states: not in a momentum sector
+        basis = self.basis if self.momentum_basis is None else None
+        self.H.time_evolution(initial_state, time_line, self.loc_dims, basis=basis, d_max=d_max, **kw)
 
     # -- observables (quantum_model.py:1158-1333)
     def get_observables(self, local_obs=[], twobody_obs=[], plaquette_obs=[], nbody_obs=[],
@@ -331,15 +366,17 @@ class QuantumModel:
         import torch
         if not dynamics:
             state = self.H.Npsi[index]
-            marg = self._marginals(state.psi_device)
+            psi_dev = self._sector_state(state.psi_device)
+            marg = self._marginals(psi_dev)
         else:
             state = self.H.psi_time[index] if self.H.psi_time is not None else None
-            marg = self.H.marginals[index] if self.H.marginals is not None else self._marginals(state.psi_device)
+            psi_dev = None if state is None else self._sector_state(state.psi_device)
+            marg = self.H.marginals[index] if self.H.marginals is not None else self._marginals(psi_dev)
         for obs in self.local_obs:
             self.res[obs] = self.local_expvals(marg, obs)
         groups = getattr(self, "_obs_groups", {})
         if groups:
-            vals = self._group_expvals(state.psi_device)
+            vals = self._group_expvals(psi_dev)
             for key, (kind, names, site_lists) in groups.items():
                 v = np.real(vals[key])
                 if kind == "twobody":

@@ -56,6 +56,7 @@ typedef struct edlgt_basis edlgt_basis; /* sorted sector basis on the device */
 typedef struct edlgt_csr edlgt_csr;     /* CSR Hamiltonian on the device     */
 typedef struct edlgt_server edlgt_server; /* continuous-batching eigensolver   */
 typedef struct edlgt_krylov edlgt_krylov; /* Krylov stepping state (basis)      */
+typedef struct edlgt_momentum edlgt_momentum; /* momentum-sector projector     */
 
 int edlgt_last_error(char* buf, int64_t size);
 int edlgt_version(void);
@@ -320,6 +321,38 @@ int edlgt_expval_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables, co
                        int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
                        const int32_t* place_tables, const double* place_strength, const void* psi,
                        int32_t vec_type, double* out_host, void* stream);
+
+/* ======================================================================
+ * Momentum sectors.  edlgt_momentum_build forms the projector B (n_rows x
+ * n_cols) onto momentum k of the block translations by unit_cell[d] sites
+ * along each axis of lvals (n_dims axes; axis order and site coordinates as
+ * the reference's translational_sym module: row-major, last axis fastest):
+ * one column per translation orbit compatible with k (its smallest basis
+ * index, ascending), the normalised Bloch sum over the orbit's period box,
+ * equal images accumulated, |amplitude| <= 1e-14 dropped for k != 0 (a
+ * cancelled column stays, empty).  Replaces get_momentum_basis
+ * (edlgt/symmetries/translational_sym.py:1307-1359, :568-950) used by
+ * QuantumModel.set_momentum_sector (edlgt/models/quantum_model.py:213-276);
+ * TC_symmetry is not supported.  The sector must be closed under the
+ * translations (else EDLGT_ERR_KEY_NOT_FOUND).
+ *   edlgt_momentum_copy: CSC of B to the host (col_ptr [n_cols+1], row_idx
+ *                        [nnz], data [nnz] complex as (re, im) pairs)
+ *   edlgt_momentum_apply: forward 1: out (n_rows) = B in (n_cols); 0: out
+ *                        (n_cols) = B^H in (n_rows); complex128 device vectors
+ *   edlgt_csr_momentum:  new CSR B^H H B (n_cols x n_cols, canonical: columns
+ *                        sorted, duplicates summed), float64 when H is real
+ *                        and k = 0 -- replaces the momentum triplet builders
+ *                        nbody_data_momentum* (translational_sym.py:1394-1969)
+ *                        behind QMB_hamiltonian.build; single-term, values kept
+ * ====================================================================== */
+int edlgt_momentum_build(const edlgt_basis* b, int32_t n_dims, const int32_t* lvals, const int32_t* unit_cell,
+                         const int32_t* k_vals, void* stream, edlgt_momentum** out);
+int edlgt_momentum_info(const edlgt_momentum* m, int64_t* n_rows, int64_t* n_cols, int64_t* nnz,
+                        int32_t* is_complex);
+int edlgt_momentum_copy(const edlgt_momentum* m, int64_t* col_ptr, int32_t* row_idx, double* data);
+int edlgt_momentum_apply(const edlgt_momentum* m, int32_t forward, const void* in, void* out, void* stream);
+int edlgt_csr_momentum(const edlgt_csr* h, const edlgt_momentum* m, void* stream, edlgt_csr** out);
+int edlgt_momentum_destroy(edlgt_momentum* m);
 
 /* ======================================================================
  * Real-time evolution psi <- exp(-i H dt)^n_steps psi by Krylov (Lanczos)

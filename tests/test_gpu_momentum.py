@@ -1,0 +1,87 @@
+"""GPU: momentum sectors against the oracle's restatement of the reference's
+projector (same columns, rows and amplitudes), B^H H B against the oracle's
+sparse product, eigenvalues per sector against the oracle, and the sector
+ground states against the full-sector ground state and the reference's SU(2)
+known answers."""
+import numpy as np
+import pytest
+
+from oracle.model import SU2Oracle, ZnOracle, lowest_eigs
+from oracle.momentum import momentum_basis
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(mb, Bo, Hk, Ho):
+    B = mb.to_scipy()
+    assert B.shape == Bo.shape
+    assert np.array_equal(B.indptr, Bo.indptr) and np.array_equal(B.indices, Bo.indices)
+    assert np.max(np.abs(B.data - Bo.data), initial=0.0) < 1e-15
+    ref = (Bo.conj().T @ Ho @ Bo).toarray()
+    got = Hk.to_scipy().toarray()
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref), initial=0.0) < 1e-13 * max(1.0, np.max(np.abs(ref), initial=0.0))
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_su2_chain_momentum(gpu, k):
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    orc = SU2Oracle(False, [6], [False], sectors=[6])
+    mod.set_momentum_sector([2], [k])
+    Hk = mod.build_Hamiltonian(1.0, 5.0)
+    Bo = momentum_basis(orc.configs, orc.keys, orc.loc_dims, [6], [2], [k])
+    Ho = orc.hamiltonian(1.0, 5.0)
+    _compare(mod.momentum_basis, Bo, Hk, Ho)
+    mod.diagonalize_Hamiltonian(3, "sparse")
+    ref = np.linalg.eigvalsh((Bo.conj().T @ Ho @ Bo).toarray())[:3]
+    assert np.max(np.abs(mod.H.energies - ref)) < 1e-10
+
+
+def test_su2_ground_state_in_k0_matches_kat(gpu):
+    """The L=6 PBC ground state lies in k = 0: energy and observables of the
+    reference's known answers (validation/su2/test_su2_spectrum.py)."""
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    mod.set_momentum_sector([2], [0])
+    mod.build_Hamiltonian(1.0, 5.0)
+    mod.diagonalize_Hamiltonian(1, "sparse")
+    assert abs(mod.H.Nenergies[0] - (-6.173752796132477)) < 1e-10
+    mod.get_observables(["T2_px", "N_single", "N_pair", "N_zero"])
+    mod.measure_observables(0)
+    e2 = mod.link_avg("T2")
+    assert abs(e2 - 0.0553604161539725) < 1e-10
+    assert abs(mod.stag_avg("N_single") - 0.14157936434695081) < 1e-10
+
+
+def test_z3_square_momentum_sectors(gpu):
+    from edlgt_amd import Zn_Model
+    z = ZnOracle(3, [3, 3], [False, False], flux_sector=(0, 0))
+    Ho = z.hamiltonian(1.0, 0.7)
+    e_full = lowest_eigs(Ho, 1)[0]
+    best = np.inf
+    tot = 0
+    for kx in range(3):
+        for ky in range(3):
+            mod = Zn_Model(3, lvals=[3, 3], has_obc=[False, False], flux_sector=(0, 0))
+            mod.set_momentum_sector([1, 1], [kx, ky])
+            Hk = mod.build_Hamiltonian(1.0, 0.7)
+            Bo = momentum_basis(z.configs, z.keys, z.loc_dims, [3, 3], [1, 1], [kx, ky])
+            _compare(mod.momentum_basis, Bo, Hk, Ho)
+            tot += int(np.sum(np.diff(Bo.indptr) > 0))
+            mod.diagonalize_Hamiltonian(1, "sparse")
+            best = min(best, mod.H.energies[0])
+    assert tot == z.N
+    assert abs(best - e_full) < 1e-10
+
+
+def test_momentum_errors(gpu):
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[True])
+    with pytest.raises(ValueError):
+        mod.set_momentum_sector([2], [0])
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    with pytest.raises(ValueError):
+        mod.set_momentum_sector([4], [0])  # 4 does not divide 6
+    with pytest.raises(ValueError):
+        mod.set_momentum_sector([2], [0, 1])
