@@ -1,0 +1,80 @@
+"""GPU edge cases: an empty symmetry sector, a one-state sector, a sector
+solved in full (k = N), invalid requests that must raise the reference's
+exception types, and observables/expectation values on tiny sectors."""
+import numpy as np
+import pytest
+
+from oracle.model import SU2Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _su2(lv, obc, n):
+    from edlgt_amd import SU2_Model
+    return SU2_Model(spin=0.5, pure_theory=False, sectors=[n], lvals=lv, has_obc=obc)
+
+
+def test_empty_sector(gpu):
+    """N_tot = 100 on 4 sites has no states: SU2_Model raises ValueError, as
+    the reference does (SU2_model.py:170-171)."""
+    with pytest.raises(ValueError, match="No configurations"):
+        _su2([4], [True], 100)
+
+
+def test_one_state_sector(gpu):
+    """N_tot = 0 on an open 2-site chain: one state, its diagonal is the energy."""
+    mod = _su2([2], [True], 0)
+    orc = SU2Oracle(False, [2], [True], sectors=[0])
+    assert mod.sector_dim == 1 and np.array_equal(mod.sector_configs, orc.configs)
+    mod.build_Hamiltonian(1.0, 1.0)
+    mod.diagonalize_Hamiltonian(1, "sparse")
+    e = orc.hamiltonian(1.0, 1.0).toarray()[0, 0].real
+    assert abs(mod.H.energies[0] - e) < 1e-12
+    mod.get_observables(["N_tot"])
+    mod.measure_observables(0)
+    assert np.allclose(mod.res["N_tot"], 0.0)
+
+
+def test_full_spectrum_of_a_small_sector(gpu):
+    """k = N on a 3-state sector: every eigenvalue, against dense eigvalsh."""
+    mod = _su2([2], [True], 2)
+    orc = SU2Oracle(False, [2], [True], sectors=[2])
+    assert mod.sector_dim == 3
+    mod.build_Hamiltonian(0.7, 1.3)
+    mod.diagonalize_Hamiltonian(3, "sparse")
+    ref = np.linalg.eigvalsh(orc.hamiltonian(0.7, 1.3).toarray())
+    assert np.max(np.abs(mod.H.energies - ref)) < 1e-12
+    with pytest.raises(ValueError):
+        mod.diagonalize_Hamiltonian(4, "sparse")
+    with pytest.raises(ValueError):
+        mod.diagonalize_Hamiltonian(0, "sparse")
+
+
+def test_invalid_requests_raise(gpu):
+    import torch
+    mod = _su2([6], [False], 6)
+    mod.build_Hamiltonian(1.0, 5.0)
+    with pytest.raises(ValueError):
+        mod.get_qmb_state_from_configs([[5] * 6])  # not in the sector
+    psi0 = mod.get_qmb_state_from_configs([mod.overlap_QMB_state("V")])
+    with pytest.raises(ValueError):
+        mod.time_evolution_Hamiltonian(psi0, [0.0])  # one time point
+    with pytest.raises(ValueError):
+        mod.time_evolution_Hamiltonian(psi0, [0.0, 0.1, 0.3])  # not uniform
+    with pytest.raises(ValueError):
+        mod.time_evolution_Hamiltonian(torch.zeros(3, dtype=torch.complex128), [0.0, 0.1])
+    with pytest.raises(ValueError):
+        mod.get_observables(nbody_obs=[["Qpx_dag", "Qmx"]], nbody_dist=[[(1,), (2,)]])
+
+
+def test_expectation_of_a_ket_outside_the_sector_raises(gpu):
+    """A single hopping factor leaves the Gauss-law sector: an error, not a
+    value (the reference's lookup would return -1 and read psi[-1])."""
+    from edlgt_amd.device import expval_terms
+    mod = _su2([6], [False], 6)
+    mod.build_Hamiltonian(1.0, 5.0)
+    mod.diagonalize_Hamiltonian(1, "sparse")
+    psi = mod.H.Npsi[0].psi_device
+    tab = np.stack([mod.ops["Qpx_dag"][0]])
+    with pytest.raises(ValueError):
+        expval_terms(mod.basis, tab, [dict(sites=[0, 2], tables=[0, 0])], psi)
