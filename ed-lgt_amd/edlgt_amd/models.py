@@ -55,6 +55,33 @@ class QMB_state:
         lam = lam[lam > 1e-10]
         return float(-np.sum(lam * np.log2(lam)))
 
+    def reduced_density_matrix(self, keep_indices, partitions_dict):
+        """Dense reduced density matrix of the sites keep_indices, psi_matrix
+        psi_matrix^H over the sector's subsystem configurations
+        (qmb_state.py:211-243); host, small systems."""
+        part = partitions_dict[tuple(sorted(keep_indices))]
+        M = np.zeros((part["subsys_dim"], part["env_dim"]), dtype=np.complex128)
+        M[part["subsys_map"], part["env_map"]] = self.psi
+        return M @ M.conj().T
+
+    def get_state_configurations(self, threshold, sector_configs):
+        """(configurations, probabilities) of the basis states with |psi|^2 >
+        threshold, largest first (qmb_state.py get_state_configurations)."""
+        p = np.abs(self.psi) ** 2
+        idx = np.nonzero(p > threshold)[0]
+        idx = idx[np.argsort(-p[idx], kind="stable")]
+        for i in idx:
+            logger.info(f"{sector_configs[i]}  {p[i]:.6f}")
+        return sector_configs[idx], p[idx]
+
+
+def diagonalize_density_matrix(rho):
+    """(eigenvalues ascending, eigenvectors) of a Hermitian density matrix
+    (qmb_state.py:526-546)."""
+    if hasattr(rho, "toarray"):
+        rho = rho.toarray()
+    return np.linalg.eigh(np.asarray(rho))
+
 
 class QMB_hamiltonian:
     """Hamiltonian container (edlgt/modeling/qmb_hamiltonian.py:24-426) whose
@@ -65,6 +92,11 @@ class QMB_hamiltonian:
         self.shape = (size, size)
         self.Ham = None
         self.Ham_type = None
+
+    def print_energy(self, en_state):
+        """qmb_hamiltonian.py:603-612."""
+        logger.info("====================================================")
+        logger.info(f"{en_state} ENERGY: {round(self.Nenergies[en_state], 16)}")
 
     def set_matrix(self, csr, fmt):
         self.Ham = csr
@@ -426,6 +458,25 @@ class QuantumModel:
             self._partition_cache[key] = dict(subsys_map=sub_map, env_map=env_map,
                                               subsys_dim=int(sub_map.max()) + 1, env_dim=int(env_map.max()) + 1)
         return self._partition_cache[key]
+
+    def measure_fidelity(self, state, index, dynamics=False, print_value=False):
+        """|<state|psi_index>|^2 with an eigenstate or a time-evolved state
+        (quantum_model.py:864-902); state: host array or device tensor over
+        the Hamiltonian's basis."""
+        import torch
+        ref = self.H.Npsi[index] if not dynamics else self.H.psi_time[index]
+        n = self.H.shape[0]
+        if hasattr(state, "data_ptr"):
+            st = state.to(device="cuda", dtype=torch.complex128).reshape(-1)
+        else:
+            st = torch.as_tensor(np.asarray(state, dtype=np.complex128).reshape(-1), device="cuda")
+        if st.numel() != n:
+            raise ValueError(f"len(state) must be {n} not {st.numel()}")
+        fid = float(abs(torch.vdot(st, ref.psi_device.to(torch.complex128)).item()) ** 2)
+        if print_value:
+            logger.info("----------------------------------------------------")
+            logger.info(f"FIDELITY: {fid}")
+        return fid
 
     def get_qmb_state_from_configs(self, configs):
         """Equal-weight normalised superposition of basis configurations
