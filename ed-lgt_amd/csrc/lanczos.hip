@@ -53,6 +53,7 @@ template <> __device__ inline double2 zero<double2>() { return make_double2(0.0,
 __device__ inline double2 to_c(double2 a) { return a; }
 __device__ inline double2 to_c(double a) { return make_double2(a, 0.0); }
 
+constexpr double kDgks = 0.5;  // squared-norm ratio above which one Gram-Schmidt pass suffices
 constexpr int kGramBlocks = 1024;
 constexpr int kGramCols = 64;  // V columns per Gram launch (one per lane)
 constexpr int kGramRows = 8;   // rows per thread per sweep
@@ -413,6 +414,7 @@ struct Solver {
   int M;       // basis capacity in vectors (multiple of b)
   DevBuf V, W, Tmp, part, gdev, cdev, shadow, pack;
   int64_t mv64 = 0, mv32 = 0;
+  int64_t ortho_single = 0, ortho_double = 0;  // orthogonalisations with one / two passes
   uint64_t seed = 12345;
   bool need_tmp = true;      // restart / Ritz buffer (not needed by Krylov stepping)
   bool want_vectors = true;  // form the Ritz vectors when a run finishes
@@ -621,19 +623,38 @@ struct Solver {
       for (int c = 0; c < nw; ++c) n0[c] = std::max(G0(c, c).real(), 0.0);
       Dense mG = G;
       for (auto& z : mG.a) z = -z;
-      EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1));
       Gout = G;
-      // pass 2, with the Gram matrix of the result formed in the same sweep
-      EDLGT_TRY(gram(V.as<S>(), nv, X, nw, G));
-      mG = G;
-      for (auto& z : mG.a) z = -z;
+      // Second pass only where the first lost more than half of a column's
+      // squared norm (Daniel-Gragg-Kaufman-Stewart criterion): below that the
+      // cancellation leaves the result orthogonal to V to working precision.
+      bool again = true;
       if (nw <= 4) {
-        EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1, &Gq));
+        Dense G1;
+        EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1, &G1));
+        again = false;
+        for (int c = 0; c < nw; ++c)
+          if (!(G1(c, c).real() >= kDgks * n0[c])) again = true;
+        if (!again) {
+          Gq = G1;
+          ++ortho_single;
+        }
       } else {
         EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1));
-        EDLGT_TRY(gram(X, nw, X, nw, Gq));
       }
-      for (size_t i = 0; i < G.a.size(); ++i) Gout.a[i] += G.a[i];
+      if (again) {
+        // pass 2, with the Gram matrix of the result formed in the same sweep
+        EDLGT_TRY(gram(V.as<S>(), nv, X, nw, G));
+        mG = G;
+        for (auto& z : mG.a) z = -z;
+        if (nw <= 4) {
+          EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1, &Gq));
+        } else {
+          EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1));
+          EDLGT_TRY(gram(X, nw, X, nw, Gq));
+        }
+        for (size_t i = 0; i < G.a.size(); ++i) Gout.a[i] += G.a[i];
+        ++ortho_double;
+      }
     }
     // Cholesky QR, twice (in place: X <- X R^-1).
     std::vector<double> floor2(nw);
