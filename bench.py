@@ -78,6 +78,20 @@ def spmv_bytes(n, nnz, s_v=16, s_x=16):
     return nnz * (s_v + 4) + (n + 1) * 8 + n * (s_x + 16)
 
 
+STREAM_VARIANTS = [(512, 2048, 4), (256, 1024, 4), (512, 4096, 8), (1024, 4096, 4), (256, 2048, 8)]
+
+
+def launch_desc(tune):
+    """Autotuner result as a readable launch shape."""
+    g, grid = int(tune[0]), int(tune[1])
+    if g == 0 and grid < 0:
+        bs, tile, u = STREAM_VARIANTS[-grid - 1]
+        return {"kernel": "csr_stream", "threads": bs, "tile_nnz": tile, "gathers_in_flight": u}
+    if g == 0:
+        return {"kernel": "csr_stream", "threads": 512, "tile_nnz": 2048, "gathers_in_flight": 4}
+    return {"kernel": "vector_csr", "lanes_per_row": g, "workgroups": grid}
+
+
 def time_launches(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -509,8 +523,7 @@ def main():
                      "algorithmic_bytes": B},
         "cpu_baseline": cpu,
         "extra": dict(sector_s=round(t_sector, 3), assembly_s=round(t_assembly, 3),
-                      spmv_launch={"lanes_per_row": tune[0], "workgroups": tune[1]},
-                      spmv_mixed_launch={"lanes_per_row": tune32[0], "workgroups": tune32[1]},
+                      spmv_launch=launch_desc(tune), spmv_mixed_launch=launch_desc(tune32),
                       spmv_mixed_ms=round(ms32, 4), spmv_mixed_GBs=round(gbs32, 2),
                       pct_hbm_peak=round(100 * gbs / HBM_PEAK_GBS, 2), coded=coded, **extra),
         "c2_sweep": sweep,

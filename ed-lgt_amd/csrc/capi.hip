@@ -406,12 +406,18 @@ int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid) {
                     lanes_per_row == 16 || lanes_per_row == 32 || lanes_per_row == 64,
                 "lanes_per_row must be 0, 4, 8, 16, 32 or 64");
   if (grid < 0) {
-    // coded matrices: grid = -(variant + 1) selects a coded stream launch variant
-    EDLGT_REQUIRE(h->n_dict > 0 && lanes_per_row == 0 && -grid - 1 < 5, "bad coded launch variant");
+    // grid = -(variant + 1) selects a launch variant of the stream kernel (the
+    // coded one for a coded matrix); lanes_per_row must be 0
+    EDLGT_REQUIRE(lanes_per_row == 0 && -grid - 1 < 5, "bad stream launch variant");
     const int v = (int)(-grid - 1);
     const int tiles[5] = {2048, 1024, 4096, 4096, 2048};
     EDLGT_TRY(csr_prepare_tiles(*h, tiles[v], nullptr));
-    h->coded_variant = v;
+    if (h->n_dict > 0)
+      h->coded_variant = v;
+    else
+      h->plain_variant = v;
+    h->g_override = 0;
+    h->grid_override = 0;
     return EDLGT_OK;
   }
   h->g_override = lanes_per_row;

@@ -111,6 +111,7 @@ struct Csr {
   DevBuf& stream_pos = tile_pos[1];
   int64_t n_stream_blocks = 0;
   int32_t coded_variant = 0;  // launch variant of the coded stream kernel
+  int32_t plain_variant = 0;  // launch variant of the uncoded stream kernel
   // dictionary coding of the values (edlgt_csr_compress): entry p holds
   // dict_val[code[p]] of term dict_term[code[p]]; code is uint8 when
   // n_dict <= 256, else uint16 (n_dict <= 4096)

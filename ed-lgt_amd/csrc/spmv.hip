@@ -167,33 +167,34 @@ __device__ inline void reduce_rows(const AT* prod, const int64_t* __restrict__ r
   }
 }
 
-template <class VT, class XT, class AT>
-__global__ void __launch_bounds__(kBS) csr_stream_kernel(
+template <class VT, class XT, class AT, int BS, int TILE, int U>
+__global__ void __launch_bounds__(BS) csr_stream_kernel(
     int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const VT* __restrict__ val, const XT* __restrict__ x, AT* __restrict__ y,
     const int64_t* __restrict__ row_blocks, const int64_t* __restrict__ block_pos) {
-  __shared__ AT prod[kCap];
+  constexpr int CAP = TILE + 256;
+  __shared__ AT prod[CAP];
   const int64_t r0 = row_blocks[blockIdx.x], r1 = row_blocks[blockIdx.x + 1];
   if (r0 >= r1) return;
   const int64_t p0 = block_pos[blockIdx.x], p1 = block_pos[blockIdx.x + 1];
   const int64_t cnt = p1 - p0;
   const int64_t nr = r1 - r0;
   const int tid = threadIdx.x;
-  if (cnt <= kCap) {
-    for (int64_t base = 0; base < cnt; base += kBS * kUnroll) {
-      int32_t c[kUnroll];
-      decltype(load_val16(val, 0)) v[kUnroll];
+  if (cnt <= CAP) {
+    for (int64_t base = 0; base < cnt; base += BS * U) {
+      int32_t c[U];
+      decltype(load_val16(val, 0)) v[U];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int64_t i = base + u * kBS + tid;
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = base + u * BS + tid;
         if (i < cnt) {
           c[u] = ldnt(col + p0 + i);
           v[u] = load_val16(val, p0 + i);
         }
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int64_t i = base + u * kBS + tid;
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = base + u * BS + tid;
         if (i < cnt) {
           AT a = zero_acc<AT>();
           fma_acc(a, v[u], load_x(x, c[u]));
@@ -202,20 +203,20 @@ __global__ void __launch_bounds__(kBS) csr_stream_kernel(
       }
     }
     __syncthreads();
-    if (nr <= 64) reduce_rows<8>(prod, row_ptr, r0, nr, p0, y);
-    else if (nr <= 128) reduce_rows<4>(prod, row_ptr, r0, nr, p0, y);
-    else if (nr <= 256) reduce_rows<2>(prod, row_ptr, r0, nr, p0, y);
-    else reduce_rows<1>(prod, row_ptr, r0, nr, p0, y);
+    if (nr <= BS / 8) reduce_rows<8, AT, BS>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= BS / 4) reduce_rows<4, AT, BS>(prod, row_ptr, r0, nr, p0, y);
+    else if (nr <= BS / 2) reduce_rows<2, AT, BS>(prod, row_ptr, r0, nr, p0, y);
+    else reduce_rows<1, AT, BS>(prod, row_ptr, r0, nr, p0, y);
   } else {
     // long rows: all threads on one row at a time, fixed-shape tree sum
-    AT* red = prod;  // kCap >= kBS
+    AT* red = prod;  // CAP >= BS
     for (int64_t r = r0; r < r1; ++r) {
       AT s = zero_acc<AT>();
-      for (int64_t p = row_ptr[r] + tid; p < row_ptr[r + 1]; p += kBS)
+      for (int64_t p = row_ptr[r] + tid; p < row_ptr[r + 1]; p += BS)
         fma_acc(s, load_val16(val, p), load_x(x, ldnt(col + p)));
       red[tid] = s;
       __syncthreads();
-      for (int o = kBS / 2; o > 0; o >>= 1) {
+      for (int o = BS / 2; o > 0; o >>= 1) {
         if (tid < o) red[tid] = red[tid] + red[tid + o];
         __syncthreads();
       }
@@ -262,14 +263,6 @@ __global__ void stream_row_blocks(const int64_t* __restrict__ row_ptr, int64_t n
 constexpr int kTabLds = 1024;      // tables up to this many entries sit in LDS
 constexpr int kRowsTabLds = 2048;  // NB x n_tab bound for the multi-RHS kernel
 
-// Launch variants (threads per block, nonzeros per tile, independent gathers
-// in flight per thread), picked per matrix by csr_autotune.
-struct CodedVariant {
-  int bs, tile, unroll;
-};
-constexpr CodedVariant kCodedVariants[] = {{512, 2048, 4}, {256, 1024, 4}, {512, 4096, 8}, {1024, 4096, 4},
-                                           {256, 2048, 8}};
-constexpr int kNumCodedVariants = 5;
 
 template <class TT, class CT, class XT, class AT, bool kLds, int BS, int TILE, int U>
 __global__ void __launch_bounds__(BS) csr_stream_coded(
@@ -385,11 +378,37 @@ int lanes_per_row(const Csr& h) {
   return 64;
 }
 
+int tile_index(int tile);
+
+// Launch variants of both stream kernels (threads per block, nonzeros per
+// tile, independent gathers in flight per thread), picked per matrix by
+// csr_autotune (plain_variant / coded_variant).
+struct StreamVariant {
+  int bs, tile, unroll;
+};
+constexpr StreamVariant kStreamVariants[] = {{512, 2048, 4}, {256, 1024, 4}, {512, 4096, 8}, {1024, 4096, 4},
+                                             {256, 2048, 8}};
+constexpr int kNumStreamVariants = 5;
+
 template <class VT, class XT, class AT>
 int launch_stream(const Csr& h, const VT* val, const XT* x, AT* y, hipStream_t st) {
-  hipLaunchKernelGGL((csr_stream_kernel<VT, XT, AT>), dim3((unsigned)h.n_stream_blocks), dim3(kBS), 0, st,
-                     h.n_rows, h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), val, x, y,
-                     h.stream_blocks.as<int64_t>(), h.stream_pos.as<int64_t>());
+  const int v = h.plain_variant;
+  const int ti = tile_index(kStreamVariants[v].tile);
+  if (!h.tile_blocks[ti].p) return set_error(EDLGT_ERR_INVALID_ARG, "stream tiles not prepared");
+  const unsigned nb = (unsigned)h.n_tile_blocks[ti];
+  const int64_t* rb = h.tile_blocks[ti].as<int64_t>();
+  const int64_t* pos = h.tile_pos[ti].as<int64_t>();
+#define EDLGT_SP(BS, TILE, U)                                                                     \
+  hipLaunchKernelGGL((csr_stream_kernel<VT, XT, AT, BS, TILE, U>), dim3(nb), dim3(BS), 0, st, h.n_rows, \
+                     h.row_ptr.as<int64_t>(), h.col.as<int32_t>(), val, x, y, rb, pos)
+  switch (v) {
+    case 0: EDLGT_SP(512, 2048, 4); break;
+    case 1: EDLGT_SP(256, 1024, 4); break;
+    case 2: EDLGT_SP(512, 4096, 8); break;
+    case 3: EDLGT_SP(1024, 4096, 4); break;
+    default: EDLGT_SP(256, 2048, 8); break;
+  }
+#undef EDLGT_SP
   EDLGT_HIP(hipGetLastError());
   return EDLGT_OK;
 }
@@ -433,8 +452,8 @@ int coded_autotune(Csr& h, int xtype, const void* x, void* y, hipStream_t st, fl
   EDLGT_HIP(hipEventCreate(&e1));
   float best = 1e30f;
   int bv = 0, status = EDLGT_OK;
-  for (int v = 0; v < kNumCodedVariants && status == EDLGT_OK; ++v) {
-    status = csr_prepare_tiles(h, kCodedVariants[v].tile, st);
+  for (int v = 0; v < kNumStreamVariants && status == EDLGT_OK; ++v) {
+    status = csr_prepare_tiles(h, kStreamVariants[v].tile, st);
     if (status != EDLGT_OK) break;
     h.coded_variant = v;
     status = spmv_coded(h, h.tab_unit.p, xtype, x, y, st);  // warm
@@ -489,7 +508,7 @@ template <class TT, class XT, class AT>
 int launch_stream_coded(const Csr& h, const TT* tab, const XT* x, AT* y, hipStream_t st) {
   const bool lds = h.n_dict <= kTabLds;
   const int v = h.coded_variant;
-  const int ti = tile_index(kCodedVariants[v].tile);
+  const int ti = tile_index(kStreamVariants[v].tile);
   if (!h.tile_blocks[ti].p) return set_error(EDLGT_ERR_INVALID_ARG, "stream tiles not prepared");
   const unsigned nb = (unsigned)h.n_tile_blocks[ti];
   const int64_t* rb = h.tile_blocks[ti].as<int64_t>();
@@ -1034,11 +1053,17 @@ int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, voi
   int32_t bg = 0;
   int64_t bgrid = 0;
   int status = EDLGT_OK;
-  if (n_rhs == 1 && h.stream_blocks.p) {  // CSR-stream candidate (lanes 0)
+  int best_plain = -1;
+  if (n_rhs == 1 && h.stream_blocks.p) {  // CSR-stream candidates (lanes 0), every launch variant
     h.g_override = 0;
     h.grid_override = 0;
-    status = csr_spmm(h, precision, xtype, n_rhs, x, h.n_rows, y, h.n_rows, st);
-    if (status == EDLGT_OK) {
+    const int save_v = h.plain_variant;
+    for (int v = 0; v < kNumStreamVariants && status == EDLGT_OK; ++v) {
+      status = csr_prepare_tiles(h, kStreamVariants[v].tile, st);
+      if (status != EDLGT_OK) break;
+      h.plain_variant = v;
+      status = csr_spmm(h, precision, xtype, n_rhs, x, h.n_rows, y, h.n_rows, st);  // warm
+      if (status != EDLGT_OK) break;
       (void)hipEventRecord(e0, st);
       for (int r = 0; r < 3 && status == EDLGT_OK; ++r)
         status = csr_spmm(h, precision, xtype, n_rhs, x, h.n_rows, y, h.n_rows, st);
@@ -1046,8 +1071,12 @@ int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, voi
       (void)hipEventSynchronize(e1);
       float ms = 0;
       (void)hipEventElapsedTime(&ms, e0, e1);
-      best = ms / 3;
+      if (ms / 3 < best) {
+        best = ms / 3;
+        best_plain = v;
+      }
     }
+    h.plain_variant = best_plain >= 0 ? best_plain : save_v;
   }
   for (int g : gs) {
     if (g > 8 && g > 2 * avg) continue;
@@ -1083,7 +1112,8 @@ int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, voi
   h.g_override = bg;
   h.grid_override = bgrid;
   if (best_g) *best_g = bg;
-  if (best_grid) *best_grid = bgrid;
+  // the stream kernel won: its launch variant is reported as grid = -(variant + 1)
+  if (best_grid) *best_grid = (bg == 0 && best_plain >= 0) ? -(int64_t)h.plain_variant - 1 : bgrid;
   if (best_ms) *best_ms = best;
   return EDLGT_OK;
 }

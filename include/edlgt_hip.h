@@ -215,18 +215,19 @@ int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, vo
 int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t vec_type,
                     const double* coef_host, void* stream);
 
-/* Times the SpMM launch shapes (lanes per row x workgroups) on X -> Y with HIP
- * events and keeps the fastest for this matrix (no counterpart in the
- * reference: CPU csr_spmv has no launch shape).  Outputs may be NULL.  For a
- * coded matrix (edlgt_csr_compress) the candidates are the coded stream
- * kernel's variants (threads per block x nonzeros per tile x gathers in
- * flight); *out_lanes = 0 and *out_grid = -(variant + 1). */
+/* Times the SpMM launch shapes with HIP events and keeps the fastest for this
+ * matrix (no counterpart in the reference: CPU csr_spmv has no launch shape):
+ * for one right-hand side the stream kernel's variants (threads per block x
+ * nonzeros per tile x gathers in flight) against the vector kernel's lanes
+ * per row x workgroups.  Outputs may be NULL.  A stream winner is reported as
+ * *out_lanes = 0, *out_grid = -(variant + 1); a coded matrix
+ * (edlgt_csr_compress) only has stream candidates. */
 int edlgt_csr_autotune(edlgt_csr* h, int32_t precision, int32_t vec_type, int32_t n_rhs,
                        const void* X, void* Y, void* stream, int32_t* out_lanes, int64_t* out_grid,
                        float* out_ms);
 /* Sets the launch shape explicitly (lanes_per_row 0 and grid 0 restore the
- * defaults; on a coded matrix lanes_per_row 0 and grid -(v + 1) select the
- * coded stream variant v, 0 <= v < 5). */
+ * defaults; lanes_per_row 0 and grid -(v + 1) select stream variant v,
+ * 0 <= v < 5, of the coded kernel on a coded matrix, else of the plain one). */
 int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid);
 
 /* ======================================================================
