@@ -239,7 +239,6 @@ __global__ void __launch_bounds__(kTB) row_transform(S* X, int64_t ldx, int nw, 
 }
 
 __device__ inline double2 rscale(double c, double2 v) { return make_double2(c * v.x, c * v.y); }
-__device__ inline double rscale(double c, double v) { return c * v; }
 
 // One three-term Krylov update in place: w = c2 w + c1 b + c0 a (real
 // coefficients; a may alias b), accumulating |w|^2 per block like

@@ -110,20 +110,20 @@ __global__ void __launch_bounds__(256) csr_spmm_kernel(int64_t n, const int64_t*
 
 // CSR-stream product for short rows (one right-hand side).
 //
-// The nonzeros are cut into tiles of kTile entries; tile k's block owns the
-// rows that start inside [k*kTile, (k+1)*kTile) (row_blocks[k] =
-// lower_bound(row_ptr, k*kTile), found on the device).  A kBS-thread block
+// The nonzeros are cut into tiles of TILE entries; tile k's block owns the
+// rows that start inside [k*TILE, (k+1)*TILE) (row_blocks[k] =
+// lower_bound(row_ptr, k*TILE), found on the device).  A BS-thread block
 // streams its contiguous entries with coalesced non-temporal loads, each
-// thread keeping kUnroll independent gathers of x in flight, writes the
-// products to LDS, and then reduces every row sequentially from LDS (fixed
-// order -> bitwise reproducible).  Blocks run in row order, so the whole chip
-// sweeps a compact window of rows and the gathered x lines are reused from
-// L2 / MALL.  A block whose entries exceed the LDS capacity (a very long row)
-// falls back to a block-cooperative reduction per row.
-constexpr int kBS = 512;     // threads per block of the stream kernel
-constexpr int kTile = 2048;  // nonzeros per tile (measured best on MI355X among 512..4096)
-constexpr int kCap = kTile + 256;
-constexpr int kUnroll = 4;
+// thread keeping U independent gathers of x in flight, writes the products
+// to LDS, and then reduces every row sequentially from LDS (fixed order ->
+// bitwise reproducible).  Blocks run in row order, so the whole chip sweeps a
+// compact window of rows and the gathered x lines are reused from L2 / MALL.
+// A block whose entries exceed the LDS capacity (a very long row) falls back
+// to a block-cooperative reduction per row.  (BS, TILE, U) is a launch
+// variant chosen per matrix by the autotuner; (512, 2048, 4) is the default
+// and was the fastest on every matrix measured on MI355X.
+constexpr int kBS = 512;     // default threads per block of the stream kernel
+constexpr int kTile = 2048;  // default nonzeros per tile
 
 // 16-byte non-temporal value loads (one dwordx4 instead of two dwordx2)
 typedef double dvec2 __attribute__((ext_vector_type(2)));
