@@ -44,14 +44,14 @@ class QMB_state:
     def entanglement_entropy(self, keep_indices, partitions_dict):
         """Von Neumann entropy (base 2, Schmidt weights > 1e-10) of the sites
         keep_indices (qmb_state.py:245-291), from the subsystem/environment
-        maps of QuantumModel._get_partition.  The state is copied to the host
-        and the Schmidt values come from a dense SVD (small systems)."""
-        part = partitions_dict[tuple(sorted(keep_indices))]
-        M = np.zeros((part["subsys_dim"], part["env_dim"]), dtype=np.complex128)
-        M[part["subsys_map"], part["env_map"]] = self.psi
-        if min(M.shape) == 1:
-            return 0.0
-        lam = np.linalg.svd(M, compute_uv=False) ** 2
+        maps of QuantumModel._get_partition.  The coefficient matrix
+        psi[subsystem config, environment config] of a symmetry sector is block
+        diagonal up to permutations: the connected components of the
+        (subsystem, environment) pairs present in the sector are independent
+        blocks, so the Schmidt values are those of each block (one small dense
+        SVD each) -- the same spectrum as the reference's one dense SVD, at a
+        cost set by the largest block."""
+        lam = schmidt_weights(self.psi, partitions_dict[tuple(sorted(keep_indices))])
         lam = lam[lam > 1e-10]
         return float(-np.sum(lam * np.log2(lam)))
 
@@ -73,6 +73,36 @@ class QMB_state:
         for i in idx:
             logger.info(f"{sector_configs[i]}  {p[i]:.6f}")
         return sector_configs[idx], p[idx]
+
+
+def schmidt_weights(psi, part):
+    """Squared singular values of psi reshaped by the partition maps
+    (subsys_map, env_map), block by block over the connected components of
+    the bipartite graph of (subsystem, environment) pairs."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    a, e = np.asarray(part["subsys_map"]), np.asarray(part["env_map"])
+    na, ne = part["subsys_dim"], part["env_dim"]
+    if "blocks" not in part:
+        g = sp.coo_matrix((np.ones(a.size), (a, na + e)), shape=(na + ne, na + ne))
+        nc, lab = connected_components(g, directed=False)
+        comp = lab[a]  # component of every basis state
+        order = np.argsort(comp, kind="stable")
+        bounds = np.searchsorted(comp[order], np.arange(nc + 1))
+        part["blocks"] = (order, bounds)
+    order, bounds = part["blocks"]
+    out = []
+    for c in range(len(bounds) - 1):
+        idx = order[bounds[c]:bounds[c + 1]]
+        if idx.size == 0:
+            continue
+        ua, ia = np.unique(a[idx], return_inverse=True)
+        ue, ie = np.unique(e[idx], return_inverse=True)
+        M = np.zeros((ua.size, ue.size), dtype=np.complex128)
+        M[ia.ravel(), ie.ravel()] = psi[idx]
+        out.append(np.linalg.svd(M, compute_uv=False) ** 2 if min(M.shape) > 1
+                   else np.array([np.sum(np.abs(M) ** 2)]))
+    return np.concatenate(out) if out else np.zeros(0)
 
 
 def diagonalize_density_matrix(rho):

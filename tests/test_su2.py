@@ -210,3 +210,27 @@ def test_expval_oracle_equals_bruteforce():
         got = expval_nbody([orc.ops["Qpx_dag"], orc.ops["Qmx"]], [i, j], orc.configs, orc.keys, orc.loc_dims, psi)
         O = place({i: orc.ops["Qpx_dag"][i][:d, :d]}) @ place({j: orc.ops["Qmx"][j][:d, :d]})
         assert abs(got - np.vdot(full, O @ full)) < 1e-13
+
+
+def test_blockwise_schmidt_weights_equal_dense_svd():
+    """The engine's block-by-block Schmidt decomposition (connected components
+    of the subsystem/environment pairs) equals one dense SVD, L=10 OBC."""
+    from edlgt_amd.models import schmidt_weights
+    orc = SU2Oracle(False, [10], [True], sectors=[10])
+    rng = np.random.default_rng(9)
+    psi = rng.standard_normal(orc.N) + 1j * rng.standard_normal(orc.N)
+    psi /= np.linalg.norm(psi)
+    for keep in ([0, 1, 2, 3, 4], [0, 2, 5], [9]):
+        env = [i for i in range(10) if i not in keep]
+        _, a = np.unique(orc.configs[:, keep], axis=0, return_inverse=True)
+        _, e = np.unique(orc.configs[:, env], axis=0, return_inverse=True)
+        a, e = a.ravel(), e.ravel()
+        part = dict(subsys_map=a, env_map=e, subsys_dim=int(a.max()) + 1, env_dim=int(e.max()) + 1)
+        M = np.zeros((part["subsys_dim"], part["env_dim"]), dtype=complex)
+        M[a, e] = psi
+        ref = np.sort(np.linalg.svd(M, compute_uv=False) ** 2)[::-1]
+        got = np.sort(schmidt_weights(psi, part))[::-1]
+        n = min(ref.size, got.size)
+        assert np.max(np.abs(got[:n] - ref[:n])) < 1e-13
+        assert np.all(ref[n:] < 1e-13) and np.all(got[n:] < 1e-13)
+        assert abs(np.sum(got) - 1) < 1e-12
