@@ -290,7 +290,7 @@ def serve_leg(args):
     t_setup = time.perf_counter() - t0
     reqs = poisson_stream(args.serve_requests, args.serve_rate, list(lats),
                           lambda rng: [1.0, rng.uniform(0.3, 3.0)], seed=1)
-    svc = SolveService(csrs, n_eigs=4, tol=1e-8, max_basis=64, max_slots=16, warm_cache=64)
+    svc = SolveService(csrs, n_eigs=4, tol=1e-8, max_basis=args.serve_basis, max_slots=16, warm_cache=64)
     res = svc.run(reqs)
     served = latency_summary(res, svc.wall)
     served["mean_matvecs"] = {p: round(float(np.mean([r["matvecs"] for r in res.values() if r["problem"] == p])), 1)
@@ -302,7 +302,7 @@ def serve_leg(args):
 
     def solve_one(r):
         H = models[r["problem"]].build_Hamiltonian(r["coef"][0], r["coef"][1])
-        ev, _, _, _ = H.lanczos(4, tol=1e-8, max_basis=64, want_vectors=False)
+        ev, _, _, _ = H.lanczos(4, tol=1e-8, max_basis=args.serve_basis, want_vectors=False)
         return ev
 
     sample = sorted(reqs, key=lambda r: r["arrival"])[:args.plain_sample]
@@ -379,6 +379,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-basis", type=int, default=64,
+                    help="Lanczos basis size of the served and the plain solves")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch plain device copies of a known size (PMC byte-counter calibration)")
     args = ap.parse_args()
