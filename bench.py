@@ -267,10 +267,16 @@ def c2_sweep():
         [(i, mod.multi_term_coefficients(g), 7 + i) for i, g in enumerate(gs)])
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    # the same couplings squared (the configuration's g^2/2 E^2 and 1/(2 g^2)
+    # plaquette formula in the API's g), served on the same multi-term matrix
+    gsq = [g * g for g in gs]
+    res2 = Server(H3, n_eigs=4, tol=1e-10, max_basis=64, max_slots=3).solve_all(
+        [(i, mod.multi_term_coefficients(g), 17 + i) for i, g in enumerate(gsq)])
     return dict(workload="QED 4x2 PBC with matter, m=0.5, g in {0.5,1,2}, lowest 4 eigenvalues, tol 1e-10",
                 sector_dim=mod.sector_dim, sector_s=round(t_sector, 3), plain_s=round(t_plain, 3),
                 batched_s=round(t_batch, 3),
                 E0_per_site={str(g): float(plain[g][0] / 8) for g in gs},
+                E0_per_site_g_squared={str(g): float(res2[i][0][0] / 8) for i, g in enumerate(gsq)},
                 max_dE_batched_vs_plain=max(float(np.max(np.abs(res[i][0] - plain[g]))) for i, g in enumerate(gs)))
 
 

@@ -61,12 +61,14 @@ def test_sector_csr_energy(gpu, lvals, obc, pure, g, m, kat):
 
 def test_c2_sector_csr_and_four_eigenvalues(gpu):
     """2D QED 4x2 PBC (N=140,647): bit-exact basis and CSR pattern, lowest 4
-    eigenvalues at m=0.5 for g in {0.5, 1, 2} within 1e-10, KAT at g=m=1."""
+    eigenvalues at m=0.5 within 1e-10 for the literal couplings g in
+    {0.5, 1, 2} and for g = {0.25, 4}, the squares the configuration's
+    formula (g^2/2 E^2, 1/(2 g^2) plaquettes) maps them to; KAT at g=m=1."""
     orc = QEDOracle(1, False, [4, 2], [False, False])
     mod = _model([4, 2], [False, False], False)
     assert mod.sector_dim == 140647
     assert np.array_equal(mod.sector_configs, orc.configs)
-    for g in (0.5, 1.0, 2.0):
+    for g in (0.5, 1.0, 2.0, 0.25, 4.0):
         print(f"C2 g={g}", flush=True)
         Hg = mod.build_Hamiltonian(g, 0.5)
         Ho = orc.hamiltonian(g, 0.5)
