@@ -391,8 +391,8 @@ def main():
                     help="also launch plain device copies of a known size (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded design is "
-              "described in DESIGN.md and no multi-GPU path is built.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): no multi-GPU path is "
+              "built; one GPU runs every kernel at its full single-shard shape.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "su2c3":
         return su2_dynamics(args)
