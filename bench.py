@@ -45,7 +45,7 @@ line reports the build times, seconds per step for both precisions and, per
 step, the site occupations and the electric energy of both runs with their
 difference.
 
-Workload cpu_qed4x3: the full build + lowest-eigenvalue path of C5 on the
+Workload cpu_qed4x3: the full build + lowest-eigenvalue path of QED 4x3 PBC on the
 host cores (oracle/refcpu: C/OpenMP sector enumeration and two-pass triplets,
 SciPy coalescing into CSR, eigsh on the C SpMV), beside the same path on the
 GPU, for the build+E0 ratio.
@@ -128,7 +128,7 @@ def cpu_baseline(H, x_dev, max_seconds=12.0):
 
 
 def cpu_build_e0(args):
-    """C5 build + E0: host cores (refcpu restatement) vs GPU."""
+    """QED 4x3 PBC (N = 28,904,615) build + E0: host cores (refcpu restatement) vs GPU."""
     import torch
     from edlgt_amd import QED_Model
     from oracle import refcpu
@@ -147,7 +147,7 @@ def cpu_build_e0(args):
     t0 = time.perf_counter()
     e_cpu, N, nnz, T = refcpu.qed_build_and_e0([4, 3], [False, False], 1.0, 0.5, log=progress)
     t_cpu = time.perf_counter() - t0
-    out = {"metric": "C5 build + lowest eigenvalue, wall time", "value": round(t_gpu, 3), "unit": "s",
+    out = {"metric": "QED 4x3 PBC build + lowest eigenvalue, wall time", "value": round(t_gpu, 3), "unit": "s",
            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(1e3 * t_gpu, 1),
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
            "data": "synthetic: matrix built from the model", "config": {"workload": WORKLOADS["qed4x3"]["desc"],
@@ -155,7 +155,7 @@ def cpu_build_e0(args):
            "gpu": {"build_plus_E0_s": round(t_gpu, 3), "E0": e_gpu},
            "cpu_baseline": {"kind": "port", "cores": refcpu.threads(), "build_plus_E0_s": round(t_cpu, 2),
                             "phases_s": {k: round(v, 2) for k, v in T.items()}, "E0": e_cpu,
-                            "sample": "full C5 path once: sector, triplets, SciPy CSR, eigsh(k=1, SA)"},
+                            "sample": "full QED 4x3 path once: sector, triplets, SciPy CSR, eigsh(k=1, SA)"},
            "speedup": round(t_cpu / t_gpu, 1), "dE0_gpu_vs_cpu": abs(e_gpu - e_cpu)}
     print(json.dumps(out), flush=True)
 
@@ -501,7 +501,7 @@ def main():
     torch.cuda.empty_cache()
 
     serving = None
-    progress("C2 coupling sweep")
+    progress("QED 4x2 coupling sweep")
     sweep = c2_sweep()
     if not args.no_serve:
         progress("serving leg")

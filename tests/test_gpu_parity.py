@@ -69,7 +69,7 @@ def test_c2_sector_csr_and_four_eigenvalues(gpu):
     assert mod.sector_dim == 140647
     assert np.array_equal(mod.sector_configs, orc.configs)
     for g in (0.5, 1.0, 2.0, 0.25, 4.0):
-        print(f"C2 g={g}", flush=True)
+        print(f"QED 4x2 g={g}", flush=True)
         Hg = mod.build_Hamiltonian(g, 0.5)
         Ho = orc.hamiltonian(g, 0.5)
         _csr_match(Hg, Ho)
