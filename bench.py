@@ -242,7 +242,7 @@ def su2_dynamics(args):
     print(json.dumps(out), flush=True)
 
 
-def c2_sweep():
+def coupling_sweep():
     """QED 4x2 PBC with matter (N = 140,647), m = 0.5, g in {0.5, 1, 2}, lowest 4
     eigenvalues: one build + FP64 Lanczos per coupling (plain) vs one
     multi-term assembly solved for all couplings by one server (batched)."""
@@ -502,7 +502,7 @@ def main():
 
     serving = None
     progress("QED 4x2 coupling sweep")
-    sweep = c2_sweep()
+    sweep = coupling_sweep()
     if not args.no_serve:
         progress("serving leg")
         serving = serve_leg(args)
@@ -534,7 +534,7 @@ def main():
                       spmv_launch=launch_desc(tune), spmv_mixed_launch=launch_desc(tune32),
                       spmv_mixed_ms=round(ms32, 4), spmv_mixed_GBs=round(gbs32, 2),
                       pct_hbm_peak=round(100 * gbs / HBM_PEAK_GBS, 2), coded=coded, **extra),
-        "c2_sweep": sweep,
+        "coupling_sweep": sweep,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
