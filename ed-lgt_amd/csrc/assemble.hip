@@ -472,11 +472,27 @@ constexpr int kExpWaves = 4;
 __device__ inline double2 to_c2(double v) { return make_double2(v, 0.0); }
 __device__ inline double2 to_c2(double2 v) { return v; }
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline void store_y(double* y, int64_t r, double2 v) { y[r] = v.x; }
+__device__ inline void store_y(double2* y, int64_t r, double2 v) { y[r] = v; }
 
 // sum over placement P's transitions from this row of amp * psi[col]
+// Index of key k in the sorted keys: when `bucket` is given, bucket[b] is the
+// first index with key >= (b << shift), which narrows the search to one bucket.
+template <class KeyT>
+__device__ inline int64_t find_key(const KeyT* __restrict__ keys, int64_t n, KeyT k,
+                                   const int64_t* __restrict__ bucket, int shift, int64_t n_bucket) {
+  if (!bucket) return lower_bound_key<KeyT>(keys, n, k);
+  const KeyT b = k >> shift;
+  if (b >= (KeyT)n_bucket) return n;
+  const int64_t lo = bucket[(int64_t)b], hi = bucket[(int64_t)b + 1];
+  return lo + lower_bound_key<KeyT>(keys + lo, hi - lo, k);
+}
+
 template <class KeyT, class PT>
 __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, const uint8_t* cfg, KeyT key,
-                                          int64_t r, const PT* __restrict__ psi) {
+                                          int64_t r, const PT* __restrict__ psi,
+                                          const int64_t* __restrict__ bucket = nullptr, int shift = 0,
+                                          int64_t n_bucket = 0) {
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
   const KeyT* strides = static_cast<const KeyT*>(A.strides);
   if (P.nops == 1) {
@@ -521,7 +537,7 @@ __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, c
     const KeyT k2 = key + delta;
     int64_t col = r;
     if (k2 != key) {
-      col = lower_bound_key<KeyT>(keys, A.n_states, k2);
+      col = find_key<KeyT>(keys, A.n_states, k2, bucket, shift, n_bucket);
       if (col >= A.n_states || keys[col] != k2) {
         atomicOr(A.err, kErrMissing);
         col = -1;
@@ -590,7 +606,58 @@ __global__ void __launch_bounds__(256) expval_finish(const double2* __restrict__
   if (lane == 0) out[p] = s;
 }
 
+// ------------------------------------------------------------ matrix-free products
+// y[r] = sum over placements p and their transitions from row r of
+// strength_p * amp * x[col]: the stored matrix's row r applied without
+// storing it.  Four waves per block, one row per wave; lane l takes
+// placements l, l+64, ...; the lane sums are combined by a fixed butterfly
+// (deterministic).  Column lookups use a bucket index over the leading key
+// bits (a few probes instead of log2 N).
+template <class KeyT, class XT, class AT>
+__global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const int64_t* __restrict__ bucket, int shift,
+                                                    int64_t n_bucket, const XT* __restrict__ x,
+                                                    AT* __restrict__ y) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const KeyT* keys = static_cast<const KeyT*>(A.keys);
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < A.n_states; r += (int64_t)gridDim.x * 4) {
+    const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
+    const KeyT key = keys[r];
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 1
+    for (int p = lane; p < A.n_place; p += 64) {
+      const Placement P = A.place[p];
+      // a dagger placement's entries are conj(amp * strength), as assembled
+      const double2 sp = make_double2(P.s_re, P.dagger ? -P.s_im : P.s_im);
+      acc = cadd(acc, cmul(placement_apply<KeyT, XT>(A, P, cfg, key, r, x, bucket, shift, n_bucket), sp));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      acc.x += __shfl_xor(acc.x, o, 64);
+      acc.y += __shfl_xor(acc.y, o, 64);
+    }
+    if (lane == 0) store_y(y, r, acc);
+  }
+}
+
+template <class KeyT>
+__global__ void bucket_starts(const KeyT* __restrict__ keys, int64_t n, int shift, int64_t n_bucket,
+                              int64_t* __restrict__ bucket) {
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= n_bucket;
+       b += (int64_t)gridDim.x * blockDim.x)
+    bucket[b] = b == n_bucket ? n : lower_bound_key<KeyT>(keys, n, (KeyT)b << shift);
+}
+
 }  // namespace
+
+// The term program and the lookup index of a matrix-free operator.
+struct MatFree {
+  Program prog;
+  DevBuf bucket;  // int64 [n_bucket + 1]
+  int shift = 0;
+  int64_t n_bucket = 0;
+  int key_words = 1;
+  bool checked = false;  // the first product has been checked for kets outside the sector
+};
 
 int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
                       int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
@@ -665,6 +732,81 @@ int placements_expval(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   }
   EDLGT_HIP(hipMemcpyAsync(out_host, out.p, (size_t)n_place * sizeof(double2), hipMemcpyDeviceToHost, st));
   EDLGT_TRY(check_err(prog.err, st));  // synchronises
+  return EDLGT_OK;
+}
+
+int hamiltonian_build_matfree(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
+                              int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                              const int32_t* place_tables, const double* place_strength,
+                              const int32_t* place_dagger, int32_t value_type, hipStream_t st, Csr* out) {
+  auto mf = std::make_shared<MatFree>();
+  EDLGT_TRY(prepare_program(B, d_max, n_tables, tables, n_place, place_nops, place_sites, place_tables,
+                            place_strength, place_dagger, nullptr, 1, value_type, st, mf->prog));
+  const int64_t N = B.n_states;
+  mf->key_words = B.key_words;
+  // bucket index over the leading bits of the largest key: about 2^22 buckets
+  int bits = 0;
+  if (N > 0) {
+    if (B.key_words == 2) {
+      u128 kmax = 0;
+      EDLGT_HIP(hipMemcpy(&kmax, B.keys.as<u128>() + (N - 1), sizeof(u128), hipMemcpyDeviceToHost));
+      while (kmax) { ++bits; kmax >>= 1; }
+    } else {
+      uint64_t kmax = 0;
+      EDLGT_HIP(hipMemcpy(&kmax, B.keys.as<uint64_t>() + (N - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+      while (kmax) { ++bits; kmax >>= 1; }
+    }
+  }
+  mf->shift = std::max(0, bits - 22);
+  mf->n_bucket = N > 0 ? ((int64_t)1 << std::min(bits, 22)) : 1;
+  EDLGT_TRY(mf->bucket.alloc((size_t)(mf->n_bucket + 1) * sizeof(int64_t)));
+  const unsigned g = grid_for(mf->n_bucket + 1, 256, 16384);
+  if (B.key_words == 2)
+    hipLaunchKernelGGL(bucket_starts<u128>, dim3(g), dim3(256), 0, st, B.keys.as<u128>(), N, mf->shift,
+                       mf->n_bucket, mf->bucket.as<int64_t>());
+  else
+    hipLaunchKernelGGL(bucket_starts<uint64_t>, dim3(g), dim3(256), 0, st, B.keys.as<uint64_t>(), N, mf->shift,
+                       mf->n_bucket, mf->bucket.as<int64_t>());
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_HIP(hipStreamSynchronize(st));
+  out->n_rows = N;
+  out->nnz = 0;
+  out->value_type = value_type;
+  out->mf = std::move(mf);
+  return EDLGT_OK;
+}
+
+int matfree_spmm(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                 hipStream_t st) {
+  const MatFree& m = *h.mf;
+  const bool cplx = h.value_type == EDLGT_VALUE_C128;
+  if (cplx && xtype != EDLGT_VALUE_C128) return set_error(EDLGT_ERR_INVALID_ARG, "complex matrix needs complex vectors");
+  if (xtype != EDLGT_VALUE_C128 && xtype != EDLGT_VALUE_F64)
+    return set_error(EDLGT_ERR_INVALID_ARG, "matrix-free products take complex128 or float64 vectors");
+  const int64_t N = h.n_rows;
+  if (N == 0) return EDLGT_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>((N + 3) / 4, 65536);
+  const size_t es = xtype == EDLGT_VALUE_C128 ? 16 : 8;
+  for (int j = 0; j < n_rhs; ++j) {
+    const char* xj = static_cast<const char*>(x) + (size_t)j * ldx * es;
+    char* yj = static_cast<char*>(y) + (size_t)j * ldy * es;
+#define MF_LAUNCH(KT, XT)                                                                                \
+  hipLaunchKernelGGL((matfree_rows<KT, XT, XT>), dim3(grid), dim3(256), 0, st, m.prog.A, m.bucket.as<int64_t>(), \
+                     m.shift, m.n_bucket, reinterpret_cast<const XT*>(xj), reinterpret_cast<XT*>(yj))
+    if (m.key_words == 2) {
+      if (xtype == EDLGT_VALUE_C128) MF_LAUNCH(u128, double2); else MF_LAUNCH(u128, double);
+    } else {
+      if (xtype == EDLGT_VALUE_C128) MF_LAUNCH(uint64_t, double2); else MF_LAUNCH(uint64_t, double);
+    }
+#undef MF_LAUNCH
+    EDLGT_HIP(hipGetLastError());
+  }
+  // every product visits every transition of every row: checking the first
+  // one (synchronously) validates the program against the sector
+  if (!m.checked) {
+    EDLGT_TRY(check_err(m.prog.err, st));
+    h.mf->checked = true;
+  }
   return EDLGT_OK;
 }
 

@@ -208,6 +208,28 @@ int edlgt_expval_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables, co
                            place_strength, psi, vec_type, out_host, (hipStream_t)stream);
 }
 
+int edlgt_hamiltonian_build_matfree(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
+                                    const double* tables, int32_t n_place, const int32_t* place_nops,
+                                    const int32_t* place_sites, const int32_t* place_tables,
+                                    const double* place_strength, const int32_t* place_dagger,
+                                    int32_t value_type, void* stream, edlgt_csr** out) {
+  EDLGT_REQUIRE(b && out && tables, "null argument");
+  EDLGT_REQUIRE(n_place == 0 || (place_nops && place_sites && place_tables && place_strength && place_dagger),
+                "null placement arrays");
+  *out = nullptr;
+  auto h = std::make_unique<edlgt_csr>();
+  EDLGT_TRY(hamiltonian_build_matfree(*b, d_max, n_tables, tables, n_place, place_nops, place_sites, place_tables,
+                                      place_strength, place_dagger, value_type, (hipStream_t)stream, h.get()));
+  *out = h.release();
+  return EDLGT_OK;
+}
+
+int edlgt_csr_is_matrix_free(const edlgt_csr* h, int32_t* out) {
+  EDLGT_REQUIRE(h && out, "null argument");
+  *out = h->mf ? 1 : 0;
+  return EDLGT_OK;
+}
+
 int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
                             const double* tables, int32_t n_place, const int32_t* place_nops,
                             const int32_t* place_sites, const int32_t* place_tables,
@@ -257,6 +279,7 @@ int edlgt_hamiltonian_build_terms(const edlgt_basis* b, int32_t d_max, int32_t n
 
 int edlgt_csr_copy_terms(const edlgt_csr* h, uint8_t* term_host, int32_t* n_terms) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   EDLGT_REQUIRE(!term_host || h->n_terms == 1 || h->term.p, "the term array was released (coded matrix)");
   if (n_terms) *n_terms = h->n_terms;
   if (term_host && h->n_terms > 1 && h->nnz)
@@ -266,6 +289,7 @@ int edlgt_csr_copy_terms(const edlgt_csr* h, uint8_t* term_host, int32_t* n_term
 
 int edlgt_csr_compress(edlgt_csr* h, void* stream, int32_t* n_dict, int32_t* code_bytes) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   int done = 0;
   EDLGT_TRY(csr_compress(*h, (hipStream_t)stream, &done));
   if (n_dict) *n_dict = h->n_dict;
@@ -275,6 +299,7 @@ int edlgt_csr_compress(edlgt_csr* h, void* stream, int32_t* n_dict, int32_t* cod
 
 int edlgt_csr_release_values(edlgt_csr* h) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   EDLGT_REQUIRE(h->n_dict > 0, "only a coded matrix can release its value array");
   EDLGT_HIP(hipDeviceSynchronize());
   h->val.release();
@@ -293,6 +318,7 @@ int edlgt_csr_info(const edlgt_csr* h, int64_t* n_rows, int64_t* nnz, int32_t* v
 
 int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   EDLGT_REQUIRE(!val || h->val.p || h->nnz == 0, "the value array was released (coded matrix)");
   if (row_ptr)
     EDLGT_HIP(hipMemcpy(row_ptr, h->row_ptr.p, (h->n_rows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -305,6 +331,7 @@ int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val
 
 int edlgt_csr_device_ptrs(const edlgt_csr* h, void** row_ptr, void** col, void** val) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   if (row_ptr) *row_ptr = h->row_ptr.p;
   if (col) *col = h->col.p;
   if (val) *val = h->val.p;
@@ -357,6 +384,7 @@ int edlgt_spmm(edlgt_csr* h, int32_t precision, int32_t n_rhs, const void* X, vo
 int edlgt_spmm_coef(edlgt_csr* h, int32_t n_rhs, const void* X, void* Y, int32_t vec_type,
                     const double* coef_host, void* stream) {
   EDLGT_REQUIRE(h && X && Y && coef_host, "null argument");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   EDLGT_REQUIRE(n_rhs >= 1, "n_rhs must be >= 1");
   EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64, "vec_type must be C128 or F64");
   EDLGT_REQUIRE(!(vec_type == EDLGT_VALUE_F64 && h->value_type != EDLGT_VALUE_F64),
@@ -394,6 +422,7 @@ int edlgt_csr_autotune(edlgt_csr* h, int32_t precision, int32_t vec_type, int32_
                        const void* X, void* Y, void* stream, int32_t* out_lanes, int64_t* out_grid,
                        float* out_ms) {
   EDLGT_REQUIRE(h && X && Y && n_rhs >= 1, "null argument");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
   if (precision == EDLGT_PREC_MIXED) EDLGT_TRY(csr_make_low_precision(*h, (hipStream_t)stream));
   return csr_autotune(*h, precision, vec_type, n_rhs, X, Y, (hipStream_t)stream, out_lanes, out_grid,
@@ -402,6 +431,7 @@ int edlgt_csr_autotune(edlgt_csr* h, int32_t precision, int32_t vec_type, int32_
 
 int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid) {
   EDLGT_REQUIRE(h, "null matrix");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   EDLGT_REQUIRE(lanes_per_row == 0 || lanes_per_row == 4 || lanes_per_row == 8 ||
                     lanes_per_row == 16 || lanes_per_row == 32 || lanes_per_row == 64,
                 "lanes_per_row must be 0, 4, 8, 16, 32 or 64");
@@ -494,6 +524,7 @@ int edlgt_momentum_apply(const edlgt_momentum* m, int32_t forward, const void* i
 
 int edlgt_csr_momentum(const edlgt_csr* h, const edlgt_momentum* m, void* stream, edlgt_csr** out) {
   EDLGT_REQUIRE(h && m && out, "null argument");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   *out = nullptr;
   auto c = std::make_unique<edlgt_csr>();
   EDLGT_TRY(csr_momentum(*h, *m, (hipStream_t)stream, c.get()));
@@ -552,6 +583,7 @@ int edlgt_server_create(edlgt_csr* h, int32_t vec_type, int32_t max_slots, int32
                         int32_t max_basis, int64_t max_matvecs, int32_t warm_cache, void* stream,
                         edlgt_server** out) {
   EDLGT_REQUIRE(h && out, "null argument");
+  EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");
   EDLGT_REQUIRE(max_slots >= 1 && max_slots <= 64, "max_slots must be in [1, 64]");
   EDLGT_REQUIRE(n_eigs >= 1 && n_eigs <= h->n_rows && tol > 0.0, "bad n_eigs / tol");
   EDLGT_REQUIRE(vec_type == EDLGT_VALUE_F64 || vec_type == EDLGT_VALUE_C128, "vec_type must be F64 or C128");

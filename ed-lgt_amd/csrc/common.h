@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -90,6 +91,10 @@ struct Basis {
   DevBuf configs;             // uint8  [n_states][cfg_stride]
 };
 
+// On-the-fly Hamiltonian (term program over the sector basis), defined in
+// assemble.hip.
+struct MatFree;
+
 // CSR Hamiltonian kept on the device.
 struct Csr {
   int64_t n_rows = 0;
@@ -121,6 +126,9 @@ struct Csr {
   DevBuf dict_val;   // double2 [n_dict] (imaginary part 0 for real matrices)
   DevBuf dict_term;  // int32 [n_dict]
   DevBuf tab_unit;   // dict values with all term coefficients 1 (double or double2)
+  // matrix-free operator: no stored entries, every product generates them
+  // from the term program (nnz = 0, row_ptr / col / val empty)
+  std::shared_ptr<MatFree> mf;
 };
 
 // Momentum-sector projector B (n_rows x n_cols): CSC (col_ptr, row_idx,

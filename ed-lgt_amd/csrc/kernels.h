@@ -19,6 +19,12 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
                       const int32_t* place_tables, const double* place_strength,
                       const int32_t* place_dagger, const int32_t* place_term, int32_t n_terms,
                       int32_t value_type, hipStream_t st, Csr* out);
+int hamiltonian_build_matfree(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
+                              int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                              const int32_t* place_tables, const double* place_strength,
+                              const int32_t* place_dagger, int32_t value_type, hipStream_t st, Csr* out);
+int matfree_spmm(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                 hipStream_t st);
 int placements_expval(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables, int32_t n_place,
                       const int32_t* place_nops, const int32_t* place_sites, const int32_t* place_tables,
                       const double* place_strength, const void* psi, int32_t vec_type, double* out_host,

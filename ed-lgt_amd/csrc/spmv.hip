@@ -560,6 +560,10 @@ int spmm_packed(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx, 
 
 int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, int64_t ldx, void* y,
              int64_t ldy, hipStream_t st, void* pack, size_t pack_bytes) {
+  if (h.mf) {
+    if (precision != EDLGT_PREC_FP64) return set_error(EDLGT_ERR_INVALID_ARG, "matrix-free products are FP64");
+    return matfree_spmm(h, xtype, n_rhs, x, ldx, y, ldy, st);
+  }
   if (n_rhs > 1 && precision == EDLGT_PREC_FP64 && pack && pack_bytes >= spmm_pack_bytes(h, xtype) &&
       spmm_packable(h, xtype))
     return spmm_packed(h, xtype, n_rhs, x, ldx, y, ldy, pack, st);
@@ -641,6 +645,7 @@ int convert_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t st) 
 }
 
 int csr_make_low_precision(Csr& h, hipStream_t st) {
+  if (h.mf) return set_error(EDLGT_ERR_INVALID_ARG, "a matrix-free operator has no stored values (FP64 only)");
   if (h.n_dict > 0) return EDLGT_OK;  // coded values: products read the exact table
   if (h.val_lo.p) return EDLGT_OK;
   const int64_t ncomp = h.nnz * (h.value_type == EDLGT_VALUE_C128 ? 2 : 1);
@@ -987,6 +992,7 @@ size_t spmm_pack_bytes(const Csr& h, int xtype) {
 }
 
 bool spmm_packable(const Csr& h, int xtype) {
+  if (h.mf) return false;
   if (h.value_type == EDLGT_VALUE_C128) return xtype == EDLGT_VALUE_C128;
   return xtype == EDLGT_VALUE_F64 || xtype == EDLGT_VALUE_C128;
 }

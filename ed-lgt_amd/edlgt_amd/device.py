@@ -167,6 +167,24 @@ def _pack_placements(placements):
     return nops, sites, tabs, stren, dag, term
 
 
+def stored_bytes_bound(tables, placements, n_states, value_type):
+    """Upper bound of a stored CSR's device bytes: every placement's largest
+    transition count per row (products of the factors' largest row counts of
+    entries above 1e-10; single-factor terms are diagonal), times the states."""
+    tables = np.asarray(tables)
+    row_max = (np.abs(tables) > 1e-10).sum(axis=2).max(axis=1)
+    col_max = (np.abs(tables) > 1e-10).sum(axis=1).max(axis=1)
+    per_row = 0
+    for pl in placements:
+        if len(pl["tables"]) == 1:
+            per_row += 1
+            continue
+        m = col_max if pl.get("dagger") else row_max
+        per_row += int(np.prod([max(int(m[t]), 0) for t in pl["tables"]]))
+    s_v = 8 if value_type == L.VALUE_F64 else 16
+    return int(n_states) * (per_row * (4 + s_v) + 8)
+
+
 def expval_terms(basis, tables, placements, psi, stream=None):
     """<psi| strength_p prod_f O_f |psi> for every placement (complex array),
     one device pass over the basis for all of them (edlgt_expval_terms).
@@ -191,7 +209,7 @@ class CSRMatrix:
     """Canonical CSR Hamiltonian on the device (rows ascending, columns sorted
     within rows, duplicates summed, explicit zeros kept)."""
 
-    def __init__(self, basis, tables, placements, value_type, stream=None, n_terms=1):
+    def __init__(self, basis, tables, placements, value_type, stream=None, n_terms=1, matrix_free=False):
         """tables: (n_tables, d, d) complex; placements: list of dicts
         {sites, tables, strength, dagger[, term]}.  With n_terms > 1 every
         placement names its term (0 <= term < n_terms) and the matrix keeps the
@@ -203,7 +221,15 @@ class CSRMatrix:
         nops, sites, tabs, stren, dag, term = _pack_placements(placements)
         h = C.c_void_p()
         st = L.stream_handle(stream) if stream is not None else None
-        if n_terms == 1:
+        self.matrix_free = bool(matrix_free)
+        if matrix_free:
+            if n_terms != 1:
+                raise ValueError("a matrix-free operator has a single term")
+            L.check(lib.edlgt_hamiltonian_build_matfree(
+                basis.handle, d_max, n_tables, tables.ctypes.data, n_place, nops.ctypes.data,
+                sites.ctypes.data, tabs.ctypes.data, stren.ctypes.data, dag.ctypes.data,
+                int(value_type), st, C.byref(h)))
+        elif n_terms == 1:
             L.check(lib.edlgt_hamiltonian_build(
                 basis.handle, d_max, n_tables, tables.ctypes.data, n_place, nops.ctypes.data,
                 sites.ctypes.data, tabs.ctypes.data, stren.ctypes.data, dag.ctypes.data,
@@ -226,6 +252,7 @@ class CSRMatrix:
         self = cls.__new__(cls)
         self._h = h
         self.n_terms = int(n_terms)
+        self.matrix_free = False
         n, nnz, vt = C.c_int64(), C.c_int64(), C.c_int32()
         L.check(L.load().edlgt_csr_info(h, C.byref(n), C.byref(nnz), C.byref(vt)))
         self.n, self.nnz, self.value_type = int(n.value), int(nnz.value), int(vt.value)

@@ -235,6 +235,10 @@ class QuantumModel:
         self.n_sites = prod(self.lvals)
         self.ham_format = ham_format
         self.momentum_basis = None
+        # True: build the Hamiltonian as a matrix-free operator; "auto": only
+        # when the stored CSR's upper bound would exceed half the free device
+        # memory; False: always stored
+        self.matrix_free = "auto"
         self.staggered_basis = False
         self.basis = None
         self.res = {}
@@ -298,10 +302,23 @@ class QuantumModel:
             if add_dagger:
                 self._placements.append(dict(sites=list(sl), tables=tabs, strength=strength, dagger=1))
 
+    def _use_matrix_free(self, tables, vt):
+        if self.momentum_basis is not None or self.matrix_free is False:
+            return False
+        if self.matrix_free is True:
+            return True
+        import torch
+        from .device import stored_bytes_bound
+        free, _ = torch.cuda.mem_get_info()
+        return stored_bytes_bound(tables, self._placements, self.sector_dim, vt) > free // 2
+
     def _build_program(self, real):
         tables = np.stack(self._tables)
         vt = L.VALUE_F64 if real else L.VALUE_C128
-        csr = CSRMatrix(self.basis, tables, self._placements, vt)
+        mf = self._use_matrix_free(tables, vt)
+        csr = CSRMatrix(self.basis, tables, self._placements, vt, matrix_free=mf)
+        if mf:
+            logger.info("Hamiltonian built matrix-free (entries generated in every product)")
         if self.momentum_basis is not None:
             # B^H H B in the momentum sector (the reference's momentum triplets,
             # translational_sym.py:1394-1969, behind QMB_hamiltonian.build)

@@ -152,6 +152,28 @@ int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_table
                             const int32_t* place_sites, const int32_t* place_tables,
                             const double* place_strength, const int32_t* place_dagger,
                             int32_t value_type, void* stream, edlgt_csr** out);
+
+/* The same operator without storing it (for sectors whose CSR would not fit
+ * in device memory): the handle holds the term program (same arguments as
+ * edlgt_hamiltonian_build) and a bucket index over the sorted keys, and every
+ * product generates row r's entries on the fly and applies them,
+ * y_r = sum over placements and transitions of strength * amp * x[col]
+ * (one wave per row, placements over lanes, fixed-order sums).  Products go
+ * through edlgt_spmv / edlgt_spmm / edlgt_lanczos / edlgt_krylov_* like a
+ * stored matrix (FP64 only; complex128, or float64 for a real operator);
+ * entry points that read stored entries (copy, compress, autotune, servers,
+ * momentum projection) return EDLGT_ERR_INVALID_ARG.  The first product checks
+ * (synchronously) that every ket lies in the sector.  Replaces the
+ * ham_format="linear" LinearOperator seam of QMB_hamiltonian.build
+ * (edlgt/modeling/qmb_hamiltonian.py:186-216) with an operator that needs
+ * no triplets at all. */
+int edlgt_hamiltonian_build_matfree(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
+                                    const double* tables, int32_t n_place, const int32_t* place_nops,
+                                    const int32_t* place_sites, const int32_t* place_tables,
+                                    const double* place_strength, const int32_t* place_dagger,
+                                    int32_t value_type, void* stream, edlgt_csr** out);
+/* *out = 1 for a matrix-free operator, else 0. */
+int edlgt_csr_is_matrix_free(const edlgt_csr* h, int32_t* out);
 /* Multi-term variant: place_term[p] in [0, n_terms) tags every placement; the
  * result is H = sum_t H_t on one pattern whose entries keep their term id
  * (entries of different terms on the same (row, col) stay separate, sorted by
