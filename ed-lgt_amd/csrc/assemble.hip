@@ -65,6 +65,10 @@ struct AsmDev {
   const double2* trans_val;
   const double2* diag;            // [n_tables][d_max]  O[a][a]
   int32_t* err;
+  // bucket index of the keys: bucket[b] = first index with key >= (b << shift)
+  const int64_t* bucket;
+  int shift;
+  int64_t n_bucket;
 };
 
 __device__ inline int wave_excl_scan(int v, int* total) {
@@ -97,6 +101,26 @@ __device__ inline int64_t lower_bound_key(const K* __restrict__ a, int64_t n, K 
     }
   }
   return lo;
+}
+
+// Index of key k in the sorted keys: when `bucket` is given, bucket[b] is the
+// first index with key >= (b << shift), which narrows the search to one bucket.
+template <class KeyT>
+__device__ inline int64_t find_key(const KeyT* __restrict__ keys, int64_t n, KeyT k,
+                                   const int64_t* __restrict__ bucket, int shift, int64_t n_bucket) {
+  if (!bucket) return lower_bound_key<KeyT>(keys, n, k);
+  const KeyT b = k >> shift;
+  if (b >= (KeyT)n_bucket) return n;
+  const int64_t lo = bucket[(int64_t)b], hi = bucket[(int64_t)b + 1];
+  return lo + lower_bound_key<KeyT>(keys + lo, hi - lo, k);
+}
+
+template <class KeyT>
+__global__ void bucket_starts(const KeyT* __restrict__ keys, int64_t n, int shift, int64_t n_bucket,
+                              int64_t* __restrict__ bucket) {
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= n_bucket;
+       b += (int64_t)gridDim.x * blockDim.x)
+    bucket[b] = b == n_bucket ? n : lower_bound_key<KeyT>(keys, n, (KeyT)b << shift);
 }
 
 // Number of raw entries placement P emits from the row with local states cfg.
@@ -285,7 +309,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
         if (k == key) {
           col = r;
         } else {
-          col = lower_bound_key<KeyT>(keys, A.n_states, k);
+          col = find_key<KeyT>(keys, A.n_states, k, A.bucket, A.shift, A.n_bucket);
           if (col >= A.n_states || keys[col] != k) {
             atomicOr(A.err, kErrMissing);
             col = 0;
@@ -359,7 +383,7 @@ namespace {
 // column nonzeros above OP_TOL), the placements, the basis' keys, strides and
 // configurations, as the assembly and expectation kernels read them.
 struct Program {
-  DevBuf place, str, toff, tket, tval, diag, err;
+  DevBuf place, str, toff, tket, tval, diag, err, bucket;
   AsmDev A;
   int cap = 1;  // raw entries per row bound (assembly LDS size)
 };
@@ -452,6 +476,34 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
   A.trans_off = prog.toff.as<int32_t>(); A.trans_ket = prog.tket.as<uint8_t>();
   A.trans_val = prog.tval.as<double2>(); A.diag = prog.diag.as<double2>(); A.err = prog.err.as<int32_t>();
 
+  // bucket index over the leading bits of the largest key: about 2^22 buckets
+  {
+    const int64_t N = B.n_states;
+    int bits = 0;
+    if (N > 0) {
+      if (B.key_words == 2) {
+        u128 kmax = 0;
+        EDLGT_HIP(hipMemcpy(&kmax, B.keys.as<u128>() + (N - 1), sizeof(u128), hipMemcpyDeviceToHost));
+        while (kmax) { ++bits; kmax >>= 1; }
+      } else {
+        uint64_t kmax = 0;
+        EDLGT_HIP(hipMemcpy(&kmax, B.keys.as<uint64_t>() + (N - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+        while (kmax) { ++bits; kmax >>= 1; }
+      }
+    }
+    A.shift = std::max(0, bits - 22);
+    A.n_bucket = (int64_t)1 << std::min(bits, 22);
+    EDLGT_TRY(prog.bucket.alloc((size_t)(A.n_bucket + 1) * sizeof(int64_t)));
+    const unsigned g = grid_for(A.n_bucket + 1, 256, 16384);
+    if (B.key_words == 2)
+      hipLaunchKernelGGL(bucket_starts<u128>, dim3(g), dim3(256), 0, st, B.keys.as<u128>(), N, A.shift,
+                         A.n_bucket, prog.bucket.as<int64_t>());
+    else
+      hipLaunchKernelGGL(bucket_starts<uint64_t>, dim3(g), dim3(256), 0, st, B.keys.as<uint64_t>(), N, A.shift,
+                         A.n_bucket, prog.bucket.as<int64_t>());
+    EDLGT_HIP(hipGetLastError());
+    A.bucket = prog.bucket.as<int64_t>();
+  }
   return EDLGT_OK;
 }
 
@@ -476,23 +528,9 @@ __device__ inline void store_y(double* y, int64_t r, double2 v) { y[r] = v.x; }
 __device__ inline void store_y(double2* y, int64_t r, double2 v) { y[r] = v; }
 
 // sum over placement P's transitions from this row of amp * psi[col]
-// Index of key k in the sorted keys: when `bucket` is given, bucket[b] is the
-// first index with key >= (b << shift), which narrows the search to one bucket.
-template <class KeyT>
-__device__ inline int64_t find_key(const KeyT* __restrict__ keys, int64_t n, KeyT k,
-                                   const int64_t* __restrict__ bucket, int shift, int64_t n_bucket) {
-  if (!bucket) return lower_bound_key<KeyT>(keys, n, k);
-  const KeyT b = k >> shift;
-  if (b >= (KeyT)n_bucket) return n;
-  const int64_t lo = bucket[(int64_t)b], hi = bucket[(int64_t)b + 1];
-  return lo + lower_bound_key<KeyT>(keys + lo, hi - lo, k);
-}
-
 template <class KeyT, class PT>
 __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, const uint8_t* cfg, KeyT key,
-                                          int64_t r, const PT* __restrict__ psi,
-                                          const int64_t* __restrict__ bucket = nullptr, int shift = 0,
-                                          int64_t n_bucket = 0) {
+                                          int64_t r, const PT* __restrict__ psi) {
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
   const KeyT* strides = static_cast<const KeyT*>(A.strides);
   if (P.nops == 1) {
@@ -537,7 +575,7 @@ __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, c
     const KeyT k2 = key + delta;
     int64_t col = r;
     if (k2 != key) {
-      col = find_key<KeyT>(keys, A.n_states, k2, bucket, shift, n_bucket);
+      col = find_key<KeyT>(keys, A.n_states, k2, A.bucket, A.shift, A.n_bucket);
       if (col >= A.n_states || keys[col] != k2) {
         atomicOr(A.err, kErrMissing);
         col = -1;
@@ -614,9 +652,7 @@ __global__ void __launch_bounds__(256) expval_finish(const double2* __restrict__
 // (deterministic).  Column lookups use a bucket index over the leading key
 // bits (a few probes instead of log2 N).
 template <class KeyT, class XT, class AT>
-__global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const int64_t* __restrict__ bucket, int shift,
-                                                    int64_t n_bucket, const XT* __restrict__ x,
-                                                    AT* __restrict__ y) {
+__global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const XT* __restrict__ x, AT* __restrict__ y) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
   for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < A.n_states; r += (int64_t)gridDim.x * 4) {
@@ -628,7 +664,7 @@ __global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const int64_t* __r
       const Placement P = A.place[p];
       // a dagger placement's entries are conj(amp * strength), as assembled
       const double2 sp = make_double2(P.s_re, P.dagger ? -P.s_im : P.s_im);
-      acc = cadd(acc, cmul(placement_apply<KeyT, XT>(A, P, cfg, key, r, x, bucket, shift, n_bucket), sp));
+      acc = cadd(acc, cmul(placement_apply<KeyT, XT>(A, P, cfg, key, r, x), sp));
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -639,22 +675,12 @@ __global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const int64_t* __r
   }
 }
 
-template <class KeyT>
-__global__ void bucket_starts(const KeyT* __restrict__ keys, int64_t n, int shift, int64_t n_bucket,
-                              int64_t* __restrict__ bucket) {
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= n_bucket;
-       b += (int64_t)gridDim.x * blockDim.x)
-    bucket[b] = b == n_bucket ? n : lower_bound_key<KeyT>(keys, n, (KeyT)b << shift);
-}
 
 }  // namespace
 
 // The term program and the lookup index of a matrix-free operator.
 struct MatFree {
-  Program prog;
-  DevBuf bucket;  // int64 [n_bucket + 1]
-  int shift = 0;
-  int64_t n_bucket = 0;
+  Program prog;  // term program with the bucket index of the keys
   int key_words = 1;
   bool checked = false;  // the first product has been checked for kets outside the sector
 };
@@ -744,31 +770,6 @@ int hamiltonian_build_matfree(const Basis& B, int32_t d_max, int32_t n_tables, c
                             place_strength, place_dagger, nullptr, 1, value_type, st, mf->prog));
   const int64_t N = B.n_states;
   mf->key_words = B.key_words;
-  // bucket index over the leading bits of the largest key: about 2^22 buckets
-  int bits = 0;
-  if (N > 0) {
-    if (B.key_words == 2) {
-      u128 kmax = 0;
-      EDLGT_HIP(hipMemcpy(&kmax, B.keys.as<u128>() + (N - 1), sizeof(u128), hipMemcpyDeviceToHost));
-      while (kmax) { ++bits; kmax >>= 1; }
-    } else {
-      uint64_t kmax = 0;
-      EDLGT_HIP(hipMemcpy(&kmax, B.keys.as<uint64_t>() + (N - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
-      while (kmax) { ++bits; kmax >>= 1; }
-    }
-  }
-  mf->shift = std::max(0, bits - 22);
-  mf->n_bucket = N > 0 ? ((int64_t)1 << std::min(bits, 22)) : 1;
-  EDLGT_TRY(mf->bucket.alloc((size_t)(mf->n_bucket + 1) * sizeof(int64_t)));
-  const unsigned g = grid_for(mf->n_bucket + 1, 256, 16384);
-  if (B.key_words == 2)
-    hipLaunchKernelGGL(bucket_starts<u128>, dim3(g), dim3(256), 0, st, B.keys.as<u128>(), N, mf->shift,
-                       mf->n_bucket, mf->bucket.as<int64_t>());
-  else
-    hipLaunchKernelGGL(bucket_starts<uint64_t>, dim3(g), dim3(256), 0, st, B.keys.as<uint64_t>(), N, mf->shift,
-                       mf->n_bucket, mf->bucket.as<int64_t>());
-  EDLGT_HIP(hipGetLastError());
-  EDLGT_HIP(hipStreamSynchronize(st));
   out->n_rows = N;
   out->nnz = 0;
   out->value_type = value_type;
@@ -791,8 +792,8 @@ int matfree_spmm(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx,
     const char* xj = static_cast<const char*>(x) + (size_t)j * ldx * es;
     char* yj = static_cast<char*>(y) + (size_t)j * ldy * es;
 #define MF_LAUNCH(KT, XT)                                                                                \
-  hipLaunchKernelGGL((matfree_rows<KT, XT, XT>), dim3(grid), dim3(256), 0, st, m.prog.A, m.bucket.as<int64_t>(), \
-                     m.shift, m.n_bucket, reinterpret_cast<const XT*>(xj), reinterpret_cast<XT*>(yj))
+  hipLaunchKernelGGL((matfree_rows<KT, XT, XT>), dim3(grid), dim3(256), 0, st, m.prog.A,                \
+                     reinterpret_cast<const XT*>(xj), reinterpret_cast<XT*>(yj))
     if (m.key_words == 2) {
       if (xtype == EDLGT_VALUE_C128) MF_LAUNCH(u128, double2); else MF_LAUNCH(u128, double);
     } else {
