@@ -502,6 +502,16 @@ int edlgt_momentum_build(const edlgt_basis* b, int32_t n_dims, const int32_t* lv
   return EDLGT_OK;
 }
 
+int edlgt_momentum_build_tc(const edlgt_basis* b, int32_t d_loc, const int32_t* c_map, const double* c_sign,
+                            int32_t k, void* stream, edlgt_momentum** out) {
+  EDLGT_REQUIRE(b && c_map && c_sign && out, "null argument");
+  *out = nullptr;
+  auto m = std::make_unique<edlgt_momentum>();
+  EDLGT_TRY(momentum_build_tc(*b, d_loc, c_map, c_sign, k, (hipStream_t)stream, m.get()));
+  *out = m.release();
+  return EDLGT_OK;
+}
+
 int edlgt_momentum_info(const edlgt_momentum* m, int64_t* n_rows, int64_t* n_cols, int64_t* nnz,
                         int32_t* is_complex) {
   EDLGT_REQUIRE(m, "null argument");

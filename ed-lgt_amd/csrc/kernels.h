@@ -66,6 +66,8 @@ int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, 
                hipStream_t st);
 int momentum_build(const Basis& B, int n_dims, const int32_t* lvals, const int32_t* unit_cell,
                    const int32_t* k_vals, hipStream_t st, Momentum* out);
+int momentum_build_tc(const Basis& B, int d_loc, const int32_t* c_map, const double* c_sign, int32_t k,
+                      hipStream_t st, Momentum* out);
 int momentum_copy(const Momentum& m, int64_t* col_ptr, int32_t* row_idx, double* data);
 int momentum_apply(const Momentum& m, int forward, const void* in, void* out, hipStream_t st);
 int csr_momentum(const Csr& h, const Momentum& m, hipStream_t st, Csr* out);

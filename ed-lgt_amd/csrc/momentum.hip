@@ -80,6 +80,40 @@ __global__ void translate_rows(const uint8_t* __restrict__ configs, int cfg_stri
   }
 }
 
+
+// trans[r * L + t] = index of X^t configuration r, X = shift by one site after
+// the local map C: X^t puts C^t(cfg[j]) on site (j + t) % L, C applied for odd
+// t only (C is an involution); sign[r] = prod_j c_sign[cfg[j]].
+template <class KeyT>
+__global__ void translate_rows_tc(const uint8_t* __restrict__ configs, int cfg_stride, int L,
+                                  const KeyT* __restrict__ keys, int64_t n, const KeyT* __restrict__ strides,
+                                  const int32_t* __restrict__ c_map, const double* __restrict__ c_sign,
+                                  int32_t* __restrict__ trans, double* __restrict__ sign, int32_t* __restrict__ err) {
+  const int64_t total = n * L;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / L;
+    const int t = (int)(i % L);
+    const uint8_t* cfg = configs + r * cfg_stride;
+    KeyT key = 0;
+    double sg = 1.0;
+    for (int j = 0; j < L; ++j) {
+      const int a = cfg[j];
+      const int b = (t & 1) ? c_map[a] : a;
+      key += (KeyT)b * strides[(j + t) % L];
+      sg *= c_sign[a];
+    }
+    if (t == 0) sign[r] = sg;
+    const int64_t q = lower_bound_k<KeyT>(keys, n, key);
+    if (q >= n || keys[q] != key) {
+      atomicOr(err, 1);
+      trans[i] = 0;
+    } else {
+      trans[i] = (int32_t)q;
+    }
+  }
+}
+
 __device__ inline double2 cm(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -232,6 +266,143 @@ int check_flag(const DevBuf& err, hipStream_t st, int32_t* out) {
 
 }  // namespace
 
+// Host orbit scan over the translation table trans[N][S] (shift s decoded
+// row-major over the axis counts R): one column per orbit whose smallest
+// index is i and whose axis periods p satisfy (k_d p_d) % R_d == 0, the
+// period box summed with its Bloch phases, equal images accumulated in box
+// order.  Standard translations (tc_sign == nullptr): phases from per-axis
+// tables built by repeated multiplication, no column dropped for k = 0.  TC
+// translations (1D, tc_sign[i] = C sign of configuration i): odd shifts carry
+// the reference's sign, phases exp(-2 pi i k t / L) evaluated directly, and a
+// k = 0 column whose norm^2 is within 1e-8 of zero stays empty.
+static int orbit_columns(const std::vector<int32_t>& trans, int64_t N, int64_t S,
+                         const std::vector<int64_t>& R, const int32_t* k_vals, const double* tc_sign, hipStream_t st, Momentum* out) {
+  const int n_dims = (int)R.size();
+  std::vector<int64_t> sstride(n_dims);
+  {
+    int64_t st_ = 1;
+    for (int d = n_dims - 1; d >= 0; --d) {
+      sstride[d] = st_;
+      st_ *= R[d];
+    }
+  }
+  bool finite = false;
+  std::vector<int64_t> kd(n_dims);
+  for (int d = 0; d < n_dims; ++d) {
+    kd[d] = ((k_vals[d] % R[d]) + R[d]) % R[d];
+    finite = finite || k_vals[d] != 0;
+  }
+  using cd = std::complex<double>;
+  int64_t maxR = *std::max_element(R.begin(), R.end());
+  std::vector<cd> ptab((size_t)n_dims * maxR, cd(1.0, 0.0));
+  for (int d = 0; d < n_dims; ++d) {
+    if (R[d] <= 1) continue;
+    const cd step = std::exp(cd(0.0, -2.0 * M_PI * (double)kd[d] / (double)R[d]));
+    for (int64_t q = 1; q < R[d]; ++q) ptab[(size_t)d * maxR + q] = ptab[(size_t)d * maxR + q - 1] * step;
+  }
+  std::vector<int64_t> col_ptr(1, 0);
+  std::vector<int32_t> row_idx;
+  std::vector<double2> data;
+  std::vector<int32_t> row_col((size_t)N, -1);
+  std::vector<double2> row_val((size_t)N, make_double2(0.0, 0.0));
+  std::vector<int32_t> img;
+  std::vector<cd> amp;
+  std::vector<int64_t> p(n_dims), tl(n_dims);
+  for (int64_t i = 0; i < N; ++i) {
+    const int32_t* row = trans.data() + (size_t)i * S;
+    int32_t mn = row[0];
+    for (int64_t s = 1; s < S; ++s) mn = std::min(mn, row[s]);
+    if (mn != (int32_t)i) continue;  // not the orbit's smallest index
+    for (int d = 0; d < n_dims; ++d) {
+      p[d] = R[d];
+      for (int64_t q = 1; q <= R[d]; ++q)
+        if (row[(q % R[d]) * sstride[d]] == (int32_t)i) {
+          p[d] = q;
+          break;
+        }
+    }
+    bool keep = true;
+    for (int d = 0; d < n_dims; ++d)
+      if ((k_vals[d] * p[d]) % R[d] != 0) keep = false;
+    if (!keep) continue;
+    img.clear();
+    amp.clear();
+    int64_t box = 1;
+    for (int d = 0; d < n_dims; ++d) box *= p[d];
+    for (int64_t f = 0; f < box; ++f) {
+      int64_t q = f, flat = 0;
+      for (int d = n_dims - 1; d >= 0; --d) {
+        tl[d] = q % p[d];
+        q /= p[d];
+      }
+      cd ph(1.0, 0.0);
+      if (tc_sign) {
+        double arg = 0.0;
+        for (int d = 0; d < n_dims; ++d) {
+          flat += tl[d] * sstride[d];
+          arg += (double)(kd[d] * tl[d]) / (double)R[d];
+        }
+        if (finite) ph = std::exp(cd(0.0, -1.0) * (2.0 * M_PI * arg));
+        if (flat & 1) ph *= tc_sign[i];
+      } else {
+        for (int d = 0; d < n_dims; ++d) {
+          flat += tl[d] * sstride[d];
+          if (finite) ph *= ptab[(size_t)d * maxR + tl[d]];
+        }
+      }
+      const int32_t im = row[flat];
+      size_t u = 0;
+      while (u < img.size() && img[u] != im) ++u;
+      if (u == img.size()) {
+        img.push_back(im);
+        amp.push_back(ph);
+      } else {
+        amp[u] += ph;
+      }
+    }
+    double nrm = 0.0;
+    size_t kept = 0;
+    for (size_t u = 0; u < img.size(); ++u) {
+      if (finite && std::abs(amp[u]) <= kTolZero) continue;
+      img[kept] = img[u];
+      amp[kept] = amp[u];
+      nrm += std::norm(amp[u]);
+      ++kept;
+    }
+    img.resize(kept);
+    amp.resize(kept);
+    const int32_t col = (int32_t)(col_ptr.size() - 1);
+    const bool empty = finite ? (nrm <= kTolColNorm || kept == 0) : (tc_sign && std::fabs(nrm) <= 1e-8);
+    if (!empty) {
+      std::vector<size_t> ord(kept);
+      for (size_t u = 0; u < kept; ++u) ord[u] = u;
+      std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return img[a] < img[b]; });
+      const double inv = 1.0 / std::sqrt(nrm);
+      for (size_t u : ord) {
+        const cd v = amp[u] * inv;
+        row_idx.push_back(img[u]);
+        data.push_back(make_double2(v.real(), v.imag()));
+        row_col[(size_t)img[u]] = col;
+        row_val[(size_t)img[u]] = make_double2(v.real(), v.imag());
+      }
+    }
+    col_ptr.push_back((int64_t)row_idx.size());
+  }
+  out->n_rows = N;
+  out->n_cols = (int64_t)col_ptr.size() - 1;
+  out->nnz = (int64_t)row_idx.size();
+  out->is_complex = finite ? 1 : 0;
+  EDLGT_TRY(upload(out->col_ptr, col_ptr.data(), col_ptr.size(), st));
+  const int32_t z32 = 0;
+  const double2 z2 = make_double2(0.0, 0.0);
+  EDLGT_TRY(upload(out->row_idx, row_idx.empty() ? &z32 : row_idx.data(), std::max<size_t>(row_idx.size(), 1), st));
+  EDLGT_TRY(upload(out->data, data.empty() ? &z2 : data.data(), std::max<size_t>(data.size(), 1), st));
+  EDLGT_TRY(upload(out->row_col, row_col.empty() ? &z32 : row_col.data(), std::max<size_t>(row_col.size(), 1), st));
+  EDLGT_TRY(upload(out->row_val, row_val.empty() ? &z2 : row_val.data(), std::max<size_t>(row_val.size(), 1), st));
+  EDLGT_HIP(hipStreamSynchronize(st));
+  return EDLGT_OK;
+}
+
 int momentum_build(const Basis& B, int n_dims, const int32_t* lvals, const int32_t* unit_cell,
                    const int32_t* k_vals, hipStream_t st, Momentum* out) {
   if (n_dims < 1 || n_dims > 3) return set_error(EDLGT_ERR_INVALID_ARG, "n_dims must be 1..3");
@@ -296,119 +467,54 @@ int momentum_build(const Basis& B, int n_dims, const int32_t* lvals, const int32
   EDLGT_TRY(check_flag(b_err, st, &flag));
   if (flag) return set_error(EDLGT_ERR_KEY_NOT_FOUND, "the sector is not closed under the translations");
   b_trans.release();
-  // host orbit scan
-  std::vector<int64_t> sstride(n_dims);
-  {
-    int64_t st_ = 1;
-    for (int d = n_dims - 1; d >= 0; --d) {
-      sstride[d] = st_;
-      st_ *= R[d];
-    }
+  return orbit_columns(trans, N, S, R, k_vals, nullptr, st, out);
+}
+
+int momentum_build_tc(const Basis& B, int d_loc, const int32_t* c_map, const double* c_sign, int32_t k,
+                      hipStream_t st, Momentum* out) {
+  const int64_t N = B.n_states;
+  const int L = B.n_sites;
+  if (L < 1 || d_loc < 1 || d_loc > 256) return set_error(EDLGT_ERR_INVALID_ARG, "bad TC symmetry arguments");
+  for (int j = 0; j < L; ++j)
+    if (B.loc_dims[j] > d_loc) return set_error(EDLGT_ERR_INVALID_ARG, "local dimension above the TC map size");
+  for (int a = 0; a < d_loc; ++a)
+    if (c_map[a] < 0 || c_map[a] >= d_loc || c_map[c_map[a]] != a)
+      return set_error(EDLGT_ERR_INVALID_ARG, "the TC local map must be an involution of [0, d_loc)");
+  if (N > 0x7fffffffLL) return set_error(EDLGT_ERR_INDEX_OVERFLOW, "more than 2^31 - 1 sector states");
+  DevBuf b_map, b_sgn, b_str, b_trans, b_sign, b_err;
+  std::vector<int32_t> cm_(c_map, c_map + d_loc);
+  std::vector<double> cs_(c_sign, c_sign + d_loc);
+  EDLGT_TRY(upload(b_map, cm_.data(), cm_.size(), st));
+  EDLGT_TRY(upload(b_sgn, cs_.data(), cs_.size(), st));
+  if (B.key_words == 2)
+    EDLGT_TRY(upload(b_str, B.strides128.data(), B.strides128.size(), st));
+  else
+    EDLGT_TRY(upload(b_str, B.strides.data(), B.strides.size(), st));
+  EDLGT_TRY(b_trans.alloc((size_t)std::max<int64_t>(N * L, 1) * sizeof(int32_t)));
+  EDLGT_TRY(b_sign.alloc((size_t)std::max<int64_t>(N, 1) * sizeof(double)));
+  EDLGT_TRY(b_err.alloc(sizeof(int32_t)));
+  EDLGT_HIP(hipMemsetAsync(b_err.p, 0, sizeof(int32_t), st));
+  const unsigned g = grid_for(N * L, 256, 65536);
+  if (B.key_words == 2)
+    hipLaunchKernelGGL(translate_rows_tc<u128>, dim3(g), dim3(256), 0, st, B.configs.as<uint8_t>(), B.cfg_stride,
+                       L, B.keys.as<u128>(), N, b_str.as<u128>(), b_map.as<int32_t>(), b_sgn.as<double>(),
+                       b_trans.as<int32_t>(), b_sign.as<double>(), b_err.as<int32_t>());
+  else
+    hipLaunchKernelGGL(translate_rows_tc<uint64_t>, dim3(g), dim3(256), 0, st, B.configs.as<uint8_t>(),
+                       B.cfg_stride, L, B.keys.as<uint64_t>(), N, b_str.as<uint64_t>(), b_map.as<int32_t>(),
+                       b_sgn.as<double>(), b_trans.as<int32_t>(), b_sign.as<double>(), b_err.as<int32_t>());
+  EDLGT_HIP(hipGetLastError());
+  std::vector<int32_t> trans((size_t)N * L);
+  std::vector<double> sign((size_t)N);
+  if (N > 0) {
+    EDLGT_HIP(hipMemcpyAsync(trans.data(), b_trans.p, trans.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    EDLGT_HIP(hipMemcpyAsync(sign.data(), b_sign.p, sign.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   }
-  bool finite = false;
-  std::vector<int64_t> kd(n_dims);
-  for (int d = 0; d < n_dims; ++d) {
-    kd[d] = ((k_vals[d] % R[d]) + R[d]) % R[d];
-    finite = finite || k_vals[d] != 0;
-  }
-  using cd = std::complex<double>;
-  int64_t maxR = *std::max_element(R.begin(), R.end());
-  std::vector<cd> ptab((size_t)n_dims * maxR, cd(1.0, 0.0));
-  for (int d = 0; d < n_dims; ++d) {
-    if (R[d] <= 1) continue;
-    const cd step = std::exp(cd(0.0, -2.0 * M_PI * (double)kd[d] / (double)R[d]));
-    for (int64_t q = 1; q < R[d]; ++q) ptab[(size_t)d * maxR + q] = ptab[(size_t)d * maxR + q - 1] * step;
-  }
-  std::vector<int64_t> col_ptr(1, 0);
-  std::vector<int32_t> row_idx;
-  std::vector<double2> data;
-  std::vector<int32_t> row_col((size_t)N, -1);
-  std::vector<double2> row_val((size_t)N, make_double2(0.0, 0.0));
-  std::vector<int32_t> img;
-  std::vector<cd> amp;
-  std::vector<int64_t> p(n_dims), tl(n_dims);
-  for (int64_t i = 0; i < N; ++i) {
-    const int32_t* row = trans.data() + (size_t)i * S;
-    int32_t mn = row[0];
-    for (int64_t s = 1; s < S; ++s) mn = std::min(mn, row[s]);
-    if (mn != (int32_t)i) continue;  // not the orbit's smallest index
-    for (int d = 0; d < n_dims; ++d) {
-      p[d] = R[d];
-      for (int64_t q = 1; q <= R[d]; ++q)
-        if (row[(q % R[d]) * sstride[d]] == (int32_t)i) {
-          p[d] = q;
-          break;
-        }
-    }
-    bool keep = true;
-    for (int d = 0; d < n_dims; ++d)
-      if ((k_vals[d] * p[d]) % R[d] != 0) keep = false;
-    if (!keep) continue;
-    img.clear();
-    amp.clear();
-    int64_t box = 1;
-    for (int d = 0; d < n_dims; ++d) box *= p[d];
-    for (int64_t f = 0; f < box; ++f) {
-      int64_t q = f, flat = 0;
-      for (int d = n_dims - 1; d >= 0; --d) {
-        tl[d] = q % p[d];
-        q /= p[d];
-      }
-      cd ph(1.0, 0.0);
-      for (int d = 0; d < n_dims; ++d) {
-        flat += tl[d] * sstride[d];
-        if (finite) ph *= ptab[(size_t)d * maxR + tl[d]];
-      }
-      const int32_t im = row[flat];
-      size_t u = 0;
-      while (u < img.size() && img[u] != im) ++u;
-      if (u == img.size()) {
-        img.push_back(im);
-        amp.push_back(finite ? ph : cd(1.0, 0.0));
-      } else {
-        amp[u] += finite ? ph : cd(1.0, 0.0);
-      }
-    }
-    double nrm = 0.0;
-    size_t kept = 0;
-    for (size_t u = 0; u < img.size(); ++u) {
-      if (finite && std::abs(amp[u]) <= kTolZero) continue;
-      img[kept] = img[u];
-      amp[kept] = amp[u];
-      nrm += std::norm(amp[u]);
-      ++kept;
-    }
-    img.resize(kept);
-    amp.resize(kept);
-    const int32_t col = (int32_t)(col_ptr.size() - 1);
-    if (!(finite && (nrm <= kTolColNorm || kept == 0))) {
-      std::vector<size_t> ord(kept);
-      for (size_t u = 0; u < kept; ++u) ord[u] = u;
-      std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return img[a] < img[b]; });
-      const double inv = 1.0 / std::sqrt(nrm);
-      for (size_t u : ord) {
-        const cd v = amp[u] * inv;
-        row_idx.push_back(img[u]);
-        data.push_back(make_double2(v.real(), v.imag()));
-        row_col[(size_t)img[u]] = col;
-        row_val[(size_t)img[u]] = make_double2(v.real(), v.imag());
-      }
-    }
-    col_ptr.push_back((int64_t)row_idx.size());
-  }
-  out->n_rows = N;
-  out->n_cols = (int64_t)col_ptr.size() - 1;
-  out->nnz = (int64_t)row_idx.size();
-  out->is_complex = finite ? 1 : 0;
-  EDLGT_TRY(upload(out->col_ptr, col_ptr.data(), col_ptr.size(), st));
-  const int32_t z32 = 0;
-  const double2 z2 = make_double2(0.0, 0.0);
-  EDLGT_TRY(upload(out->row_idx, row_idx.empty() ? &z32 : row_idx.data(), std::max<size_t>(row_idx.size(), 1), st));
-  EDLGT_TRY(upload(out->data, data.empty() ? &z2 : data.data(), std::max<size_t>(data.size(), 1), st));
-  EDLGT_TRY(upload(out->row_col, row_col.empty() ? &z32 : row_col.data(), std::max<size_t>(row_col.size(), 1), st));
-  EDLGT_TRY(upload(out->row_val, row_val.empty() ? &z2 : row_val.data(), std::max<size_t>(row_val.size(), 1), st));
-  EDLGT_HIP(hipStreamSynchronize(st));
-  return EDLGT_OK;
+  int32_t flag = 0;
+  EDLGT_TRY(check_flag(b_err, st, &flag));
+  if (flag) return set_error(EDLGT_ERR_KEY_NOT_FOUND, "the sector is not closed under the TC translations");
+  std::vector<int64_t> R(1, L);
+  return orbit_columns(trans, N, L, R, &k, sign.data(), st, out);
 }
 
 int momentum_copy(const Momentum& m, int64_t* col_ptr, int32_t* row_idx, double* data) {

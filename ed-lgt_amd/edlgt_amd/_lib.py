@@ -83,6 +83,7 @@ SIGNATURES = {
     "edlgt_expval_terms": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "edlgt_krylov_evolve": (_i32, [_p, _p, _i32, _p, _f64, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
     "edlgt_momentum_build": (_i32, [_p, _i32, _p, _p, _p, _p, _p]),
+    "edlgt_momentum_build_tc": (_i32, [_p, _i32, _p, _p, _i32, _p, _p]),
     "edlgt_momentum_info": (_i32, [_p, _p, _p, _p, _p]),
     "edlgt_momentum_copy": (_i32, [_p, _p, _p, _p]),
     "edlgt_momentum_apply": (_i32, [_p, _i32, _p, _p, _p]),

@@ -533,15 +533,26 @@ class MomentumBasis:
     translation orbit; to_scipy() gives B, expand() / project() apply B and B^H
     to device vectors."""
 
-    def __init__(self, basis, lvals, unit_cell, k_vals, stream=None):
+    def __init__(self, basis, lvals, unit_cell, k_vals, stream=None, tc_map=None, tc_sign=None):
+        """tc_map / tc_sign given: the translation-combined symmetry of a 1D
+        ring (shift by one site after the local involution tc_map, odd powers
+        signed by prod tc_sign over the configuration), unit_cell ignored."""
         lv = np.ascontiguousarray(lvals, dtype=np.int32)
         uc = np.ascontiguousarray(unit_cell, dtype=np.int32)
         kv = np.ascontiguousarray(np.atleast_1d(k_vals), dtype=np.int32)
-        if not (lv.size == uc.size == kv.size):
-            raise ValueError("lvals, unit_cell and k_vals need one entry per axis")
         m = C.c_void_p()
-        L.check(L.load().edlgt_momentum_build(basis.handle, int(lv.size), lv.ctypes.data, uc.ctypes.data,
-                                              kv.ctypes.data, L.stream_handle(stream), C.byref(m)))
+        if tc_map is not None:
+            cm = np.ascontiguousarray(tc_map, dtype=np.int32)
+            cs = np.ascontiguousarray(tc_sign, dtype=np.float64)
+            if lv.size != 1 or kv.size != 1 or cs.size != cm.size:
+                raise ValueError("TC momentum sectors are 1D: one k and a sign per local state")
+            L.check(L.load().edlgt_momentum_build_tc(basis.handle, int(cm.size), cm.ctypes.data, cs.ctypes.data,
+                                                     int(kv[0]), L.stream_handle(stream), C.byref(m)))
+        else:
+            if not (lv.size == uc.size == kv.size):
+                raise ValueError("lvals, unit_cell and k_vals need one entry per axis")
+            L.check(L.load().edlgt_momentum_build(basis.handle, int(lv.size), lv.ctypes.data, uc.ctypes.data,
+                                                  kv.ctypes.data, L.stream_handle(stream), C.byref(m)))
         self._m = m
         self.basis = basis
         self.k_vals, self.unit_cell = kv.copy(), uc.copy()

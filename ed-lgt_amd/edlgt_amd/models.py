@@ -339,13 +339,17 @@ class QuantumModel:
             raise ValueError("symmetry sector_configs not defined yet")
         if self.has_obc[0]:
             raise ValueError("Momentum is not conserved with OBC.")
-        if TC_symmetry:
-            raise NotImplementedError("TC_symmetry momentum sectors are not on the device path")
         k_vals = np.atleast_1d(np.asarray(k_vals, dtype=np.int32))
         if k_vals.size != self.dim:
             raise ValueError(f"expected k_vals shape {self.dim}, got {k_vals.shape}")
         uc = np.ascontiguousarray(k_unit_cell_size, dtype=np.int32)
-        self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals)
+        if TC_symmetry:
+            tc = getattr(self, "tc_symmetry_tables", None)
+            if tc is None or self.dim != 1:
+                raise NotImplementedError("TC_symmetry momentum sectors exist for the 1D SU(2) chain only")
+            self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals, tc_map=tc[0], tc_sign=tc[1])
+        else:
+            self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals)
         logger.info(f"Momentum basis shape ({self.momentum_basis.n_rows},{self.momentum_basis.n_cols})")
 
     def _sector_state(self, psi_device):
@@ -783,6 +787,11 @@ class SU2_Model(QuantumModel):
         self.spin = 0.5
         self.pure_theory = bool(pure_theory)
         self.use_generic_model = False
+        # translation-combined symmetry of the 1D chain with matter: local
+        # involution on the six dressed states and the sign of each state
+        # (translational_sym.py:1013-1014, 1171-1172)
+        if not pure_theory and len(kwargs.get("lvals", [])) == 1:
+            self.tc_symmetry_tables = (np.array([4, 5, 2, 3, 0, 1]), np.array([1.0, 1.0, 1.0, -1.0, 1.0, 1.0]))
         self.background = 0
         site = SU2Site(self.pure_theory, self.dim)
         self._site = site

@@ -370,6 +370,16 @@ int edlgt_expval_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables, co
  * ====================================================================== */
 int edlgt_momentum_build(const edlgt_basis* b, int32_t n_dims, const int32_t* lvals, const int32_t* unit_cell,
                          const int32_t* k_vals, void* stream, edlgt_momentum** out);
+/* The same for the translation-combined symmetry of a 1D ring: X = shift by
+ * one site after the local involution c_map (d_loc entries), momentum k of
+ * the L-fold cyclic group of X; an odd power of X carries the reference
+ * configuration's sign prod_j c_sign[config_j]; the Bloch phases are
+ * exp(-2 pi i k t / L); at k = 0 a column whose norm^2 is within 1e-8 of zero
+ * stays empty.  Replaces build_TC_translations / momentum_basis_zero_k_TC /
+ * momentum_basis_finite_k_TC (translational_sym.py:111-128, 457-505,
+ * 953-1306), which hold c_map and c_sign for the SU(2) chain. */
+int edlgt_momentum_build_tc(const edlgt_basis* b, int32_t d_loc, const int32_t* c_map, const double* c_sign,
+                            int32_t k, void* stream, edlgt_momentum** out);
 int edlgt_momentum_info(const edlgt_momentum* m, int64_t* n_rows, int64_t* n_cols, int64_t* nnz,
                         int32_t* is_complex);
 int edlgt_momentum_copy(const edlgt_momentum* m, int64_t* col_ptr, int32_t* row_idx, double* data);

@@ -129,3 +129,89 @@ def momentum_basis(configs, keys, loc_dims, lvals, unit_cell, k_vals):
         r = c = np.zeros(0, dtype=np.int64)
         v = np.zeros(0, dtype=np.complex128)
     return sp.csc_matrix((v, (r, c)), shape=(N, n_cols))
+
+
+# Translation-combined (TC) symmetry of the 1D SU(2) chain with staggered
+# matter: X = T o C (the local map C, then a shift by one site), restated from
+# translational_sym.py:457-505 (build_TC_translations), :111-128
+# (precompute_C_sign_per_config), :953-1122 (momentum_basis_zero_k_TC) and
+# :1123-1306 (momentum_basis_finite_k_TC).
+SU2_TC_MAP = np.array([4, 5, 2, 3, 0, 1])
+SU2_TC_SIGN = np.array([1.0, 1.0, 1.0, -1.0, 1.0, 1.0])
+
+
+def tc_translation_table(configs, keys, loc_dims, c_map=SU2_TC_MAP):
+    """trans[i, t] = index of X^t |config_i> (X^t: C applied when t is odd,
+    then a shift by t sites)."""
+    from .model import encode
+    N, L = configs.shape
+    trans = np.empty((N, L), dtype=np.int64)
+    for t in range(L):
+        cur = configs if t % 2 == 0 else c_map[configs]
+        rolled = np.empty_like(cur)
+        rolled[:, (np.arange(L) + t) % L] = cur
+        k = encode(rolled, loc_dims)
+        idx = np.searchsorted(keys, k)
+        if np.any(idx >= N) or np.any(keys[np.minimum(idx, N - 1)] != k):
+            raise ValueError("the sector is not closed under the TC translations")
+        trans[:, t] = idx
+    return trans
+
+
+def momentum_basis_tc(configs, keys, loc_dims, k, c_map=SU2_TC_MAP, c_sign=SU2_TC_SIGN):
+    """B for TC momentum k (1D): reference = smallest index of its X-orbit,
+    period p = smallest p > 0 with X^p ref = ref, kept iff (k p) % L == 0;
+    column sum over t < p of exp(-2 pi i k t / L) s_t X^t|ref>, s_t = 1 for even
+    t and the reference's C sign for odd t; k = 0 drops columns whose norm^2 is
+    within 1e-8 of zero (np.isclose), k != 0 drops |a| <= 1e-14 and keeps
+    cancelled columns empty."""
+    trans = tc_translation_table(configs, keys, loc_dims, c_map)
+    N, L = configs.shape
+    sign = np.prod(c_sign[configs], axis=1)
+    refs = np.nonzero(trans.min(axis=1) == np.arange(N))[0]
+    finite = (k % L) != 0 or k != 0
+    rr, cc, vv = [], [], []
+    n_cols = 0
+    for ref in refs:
+        row = trans[ref]
+        p = L
+        for q in range(1, L + 1):
+            if row[q % L] == ref:
+                p = q
+                break
+        if (k * p) % L != 0:
+            continue
+        acc, order = {}, []
+        for t in range(p):
+            incr = sign[ref] if t % 2 == 1 else 1.0
+            ph = np.exp(-1j * 2.0 * np.pi * ((k % L) * t) / float(L)) if finite else 1.0
+            img = int(row[t])
+            if img not in acc:
+                acc[img] = 0.0
+                order.append(img)
+            acc[img] += incr * ph
+        rows = np.array(order, dtype=np.int64)
+        vals = np.array([acc[i] for i in order], dtype=np.complex128)
+        if finite:
+            keep = np.abs(vals) > 1e-14
+            rows, vals = rows[keep], vals[keep]
+            nrm = float(np.sum(np.abs(vals) ** 2))
+            if nrm <= 1e-30 or rows.size == 0:
+                n_cols += 1
+                continue
+        else:
+            nrm = float(np.sum(np.abs(vals) ** 2))
+            if np.isclose(nrm, 0.0):
+                n_cols += 1
+                continue
+        o = np.argsort(rows)
+        rr.append(rows[o])
+        cc.append(np.full(rows.size, n_cols))
+        vv.append(vals[o] / np.sqrt(nrm))
+        n_cols += 1
+    if rr:
+        r, c, v = np.concatenate(rr), np.concatenate(cc), np.concatenate(vv)
+    else:
+        r = c = np.zeros(0, dtype=np.int64)
+        v = np.zeros(0, dtype=np.complex128)
+    return sp.csc_matrix((v, (r, c)), shape=(N, n_cols))

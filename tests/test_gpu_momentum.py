@@ -85,3 +85,23 @@ def test_momentum_errors(gpu):
         mod.set_momentum_sector([4], [0])  # 4 does not divide 6
     with pytest.raises(ValueError):
         mod.set_momentum_sector([2], [0, 1])
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_su2_tc_momentum(gpu, k):
+    """TC_symmetry=True (unit cell 1, the L momenta of X = T o C) against the
+    oracle's restatement; the lowest sector eigenvalue over all k is the
+    reference's known answer."""
+    from edlgt_amd import SU2_Model
+    from oracle.momentum import momentum_basis_tc
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    orc = SU2Oracle(False, [6], [False], sectors=[6])
+    mod.set_momentum_sector([1], [k], TC_symmetry=True)
+    Hk = mod.build_Hamiltonian(1.0, 5.0)
+    Bo = momentum_basis_tc(orc.configs, orc.keys, orc.loc_dims, k)
+    _compare(mod.momentum_basis, Bo, Hk, orc.hamiltonian(1.0, 5.0))
+    mod.diagonalize_Hamiltonian(1, "sparse")
+    if k == 0:
+        assert abs(mod.H.Nenergies[0] - (-6.173752796132477)) < 1e-10
+    else:
+        assert mod.H.Nenergies[0] > -6.173752796132477 - 1e-10

@@ -65,3 +65,26 @@ def test_zero_k_is_real_and_uniform():
     for c in range(B.shape[1]):
         v = B[:, c].data
         assert np.allclose(v, v[0]) and abs(np.sum(np.abs(v) ** 2) - 1) < 1e-14
+
+
+def test_su2_tc_momentum_sectors():
+    """The translation-combined symmetry of the SU(2) chain (shift by one
+    site after the local map C): the L sectors k = 0..5 of the L=6 ring
+    partition the sector, commute with H and reproduce its spectrum."""
+    from oracle.momentum import momentum_basis_tc
+    orc = SU2Oracle(False, [6], [False], sectors=[6])
+    H = orc.hamiltonian(1.0, 5.0)
+    ev = np.linalg.eigvalsh(H.toarray())
+    tot, allk = 0, []
+    for k in range(6):
+        B = momentum_basis_tc(orc.configs, orc.keys, orc.loc_dims, k)
+        B = B[:, np.nonzero(np.diff(B.indptr) > 0)[0]]
+        G = (B.conj().T @ B).toarray()
+        assert np.max(np.abs(G - np.eye(B.shape[1]))) < 1e-14
+        Hk = (B.conj().T @ H @ B).tocsr()
+        D = H @ B - B @ Hk
+        assert (abs(D).max() if D.nnz else 0.0) < 1e-12
+        tot += B.shape[1]
+        allk.append(np.linalg.eigvalsh(Hk.toarray()))
+    assert tot == orc.N
+    assert np.max(np.abs(np.sort(np.concatenate(allk)) - ev)) < 1e-11
