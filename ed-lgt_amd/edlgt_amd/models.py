@@ -138,7 +138,13 @@ class QMB_hamiltonian:
         """Lowest n_eigs eigenpairs; Nenergies are energies per site
         (qmb_hamiltonian.py:260-332)."""
         if isinstance(n_eigs, str):
-            raise NotImplementedError("full spectra (n_eigs='full') are not on the device path")
+            if n_eigs != "full":
+                raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")
+            # every eigenpair: the device solver's full-basis path (the
+            # Krylov space is the whole sector), small sectors only
+            if self.shape[0] > 512:
+                raise NotImplementedError("n_eigs='full' is solved on the device for sectors of <= 512 states")
+            n_eigs = int(self.shape[0])
         if not isinstance(n_eigs, (int, np.integer)) or n_eigs < 1:
             raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")
         if n_eigs > self.shape[0]:

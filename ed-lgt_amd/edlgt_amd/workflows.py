@@ -148,7 +148,7 @@ def run_QED_spectrum(par):
     model = QED_build_model_and_hamiltonian(par)
     n_eigs = _get(par, ["hamiltonian", "n_eigs"], "full")
     model.diagonalize_Hamiltonian(n_eigs, model.ham_format, print_results=True)
-    n_points = int(n_eigs)
+    n_points = len(model.H.Nenergies)  # n_eigs, or every state for 'full'
     res, flags = QED_prepare_observables(model, par, n_points)
     res["energy"] = model.H.Nenergies
     if _get(par, ["hamiltonian", "save_psi"], False):
@@ -258,7 +258,7 @@ def run_SU2_spectrum(par):
     model = su2_build_model_and_hamiltonian(par)
     n_eigs = _get(par, ["hamiltonian", "n_eigs"], "full")
     model.diagonalize_Hamiltonian(n_eigs, model.ham_format, print_results=True)
-    n_points = int(n_eigs)
+    n_points = len(model.H.Nenergies)  # n_eigs, or every state for 'full'
     res, flags = su2_prepare_observables(model, par, n_points)
     res["energy"] = model.H.Nenergies
     if _get(par, ["hamiltonian", "save_psi"], False):

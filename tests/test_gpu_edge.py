@@ -48,6 +48,15 @@ def test_full_spectrum_of_a_small_sector(gpu):
         mod.diagonalize_Hamiltonian(4, "sparse")
     with pytest.raises(ValueError):
         mod.diagonalize_Hamiltonian(0, "sparse")
+    # n_eigs="full" (the reference's dense path) gives every eigenpair
+    mod.diagonalize_Hamiltonian("full", "sparse")
+    assert np.max(np.abs(mod.H.energies - ref)) < 1e-12 and len(mod.H.Npsi) == 3
+    big = _su2([6], [False], 6)
+    big.build_Hamiltonian(1.0, 5.0)
+    orc6 = SU2Oracle(False, [6], [False], sectors=[6])
+    big.diagonalize_Hamiltonian("full", "sparse")
+    ref6 = np.linalg.eigvalsh(orc6.hamiltonian(1.0, 5.0).toarray())
+    assert np.max(np.abs(big.H.energies - ref6)) < 1e-10
 
 
 def test_invalid_requests_raise(gpu):
