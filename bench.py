@@ -53,7 +53,6 @@ GPU, for the build+E0 ratio.
 """
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path

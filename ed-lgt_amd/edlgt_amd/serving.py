@@ -20,7 +20,6 @@ import time
 
 import numpy as np
 
-from . import _lib as L
 from .device import Server
 
 
