@@ -597,7 +597,15 @@ struct Solver {
   // On return X = Q, and Rout (nw x nw) with (X_in - V G) = Q Rout; columns
   // that were (numerically) dependent are replaced by random directions and
   // get zero rows in Rout.  Gout receives the V-coefficients.
-  int orthonormalize(int nv, S* X, int nw, Dense& Gout, Dense& Rout) {
+  //
+  // local > 0 (X directly follows V, nw <= 4): X = H V[:, nv-b:nv] of a
+  // Lanczos step whose exact coefficients are confined to the last `local`
+  // columns (block tridiagonal T).  Those are removed first (a Gram and an
+  // update over local columns only), then one full pass over V removes what
+  // rounding left along the older columns.  That pass cancels almost nothing,
+  // so the DGKS test below rarely asks for another: two sweeps over V per
+  // step instead of four.
+  int orthonormalize(int nv, S* X, int nw, Dense& Gout, Dense& Rout, int local = 0) {
     std::vector<double> n0(nw);
     Dense Gq;  // Gram matrix of X after the Gram-Schmidt passes
     Gout = Dense(nv, nw);
@@ -605,40 +613,56 @@ struct Solver {
       EDLGT_TRY(gram(X, nw, X, nw, Gq));
       for (int c = 0; c < nw; ++c) n0[c] = std::max(Gq(c, c).real(), 0.0);
     } else {
-      // pass 1: when X directly follows V, one Gram over [V | X] gives both
-      // V^H X and X^H X
-      Dense G(nv, nw), G0(nw, nw);
-      if (X == V.as<S>() + (int64_t)nv * N) {
+      const S* Vp = V.as<S>();
+      const bool follows = X == Vp + (int64_t)nv * N;
+      const int nl = (local > 0 && local < nv && nw <= 4 && follows) ? local : 0;
+      const int g0 = nv - (nl > 0 ? nl : nv);  // first column of pass 1
+      const int np = nv - g0;
+      // pass 1: when X directly follows V, one Gram over [V[:, g0:nv] | X]
+      // gives both the coefficients and X^H X
+      Dense G(np, nw), G0(nw, nw);
+      if (follows) {
         Dense GG;
-        EDLGT_TRY(gram(V.as<S>(), nv + nw, X, nw, GG));
+        EDLGT_TRY(gram(Vp + (int64_t)g0 * N, np + nw, X, nw, GG));
         for (int c = 0; c < nw; ++c) {
-          for (int j = 0; j < nv; ++j) G(j, c) = GG(j, c);
-          for (int i = 0; i < nw; ++i) G0(i, c) = GG(nv + i, c);
+          for (int j = 0; j < np; ++j) G(j, c) = GG(j, c);
+          for (int i = 0; i < nw; ++i) G0(i, c) = GG(np + i, c);
         }
       } else {
         EDLGT_TRY(gram(X, nw, X, nw, G0));
-        EDLGT_TRY(gram(V.as<S>(), nv, X, nw, G));
+        EDLGT_TRY(gram(Vp, nv, X, nw, G));
       }
       for (int c = 0; c < nw; ++c) n0[c] = std::max(G0(c, c).real(), 0.0);
       Dense mG = G;
       for (auto& z : mG.a) z = -z;
-      Gout = G;
-      // Second pass only where the first lost more than half of a column's
-      // squared norm (Daniel-Gragg-Kaufman-Stewart criterion): below that the
-      // cancellation leaves the result orthogonal to V to working precision.
+      for (int j = 0; j < np; ++j)
+        for (int c = 0; c < nw; ++c) Gout(g0 + j, c) = G(j, c);
+      // Second pass only where the last pass lost more than half of a
+      // column's squared norm (Daniel-Gragg-Kaufman-Stewart criterion): below
+      // that the cancellation leaves the result orthogonal to V to working
+      // precision.
       bool again = true;
       if (nw <= 4) {
         Dense G1;
-        EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1, &G1));
+        EDLGT_TRY(update(Vp + (int64_t)g0 * N, np, mG, X, nw, 1, &G1));
+        std::vector<double> nref(n0);
+        if (nl > 0) {
+          for (int c = 0; c < nw; ++c) nref[c] = std::max(G1(c, c).real(), 0.0);
+          EDLGT_TRY(gram(Vp, nv, X, nw, G));
+          mG = G;
+          for (auto& z : mG.a) z = -z;
+          EDLGT_TRY(update(Vp, nv, mG, X, nw, 1, &G1));
+          for (size_t i = 0; i < G.a.size(); ++i) Gout.a[i] += G.a[i];
+        }
         again = false;
         for (int c = 0; c < nw; ++c)
-          if (!(G1(c, c).real() >= kDgks * n0[c])) again = true;
+          if (!(G1(c, c).real() >= kDgks * nref[c])) again = true;
         if (!again) {
           Gq = G1;
           ++ortho_single;
         }
       } else {
-        EDLGT_TRY(update(V.as<S>(), nv, mG, X, nw, 1));
+        EDLGT_TRY(update(Vp, nv, mG, X, nw, 1));
       }
       if (again) {
         // pass 2, with the Gram matrix of the result formed in the same sweep
@@ -662,6 +686,14 @@ struct Solver {
     Dense Rpre(nw, nw);
     for (int c = 0; c < nw; ++c) Rpre(c, c) = 1.0;
     if (nw <= 4 && cholesky_upper(Gq, R1, floor2)) {
+      if (nw == 1) {
+        // one column: its squared norm from the fused Gram is exact to
+        // rounding, so one scaling normalises it (a second would change it
+        // by a factor 1 + O(eps))
+        EDLGT_TRY(transform(X, 1, upper_inverse(R1), nullptr));
+        Rout = R1;
+        return EDLGT_OK;
+      }
       Dense Gq2, R2;
       EDLGT_TRY(transform(X, nw, upper_inverse(R1), &Gq2));
       Rpre = R1;
@@ -771,8 +803,12 @@ struct Solver {
       EDLGT_TRY(gram(V.as<S>(), nv, Wp, b, G));
       Rb = Dense(b, b);
     } else {
-      // the residual block Q is orthonormalised in place: V[:, nv:nv+b]
-      EDLGT_TRY(orthonormalize(nv, Wp, b, G, Rb));
+      // the residual block Q is orthonormalised in place: V[:, nv:nv+b].
+      // Past the first block after a (re)start T is block tridiagonal, so
+      // the exact coefficients sit in the last two blocks; the first block
+      // after a thick restart also couples to every kept Ritz vector.
+      const int local = (nv - st_l >= 2 * b) ? 2 * b : 0;
+      EDLGT_TRY(orthonormalize(nv, Wp, b, G, Rb, local));
     }
     for (int i = 0; i < nv; ++i)
       for (int c = 0; c < b; ++c) {
