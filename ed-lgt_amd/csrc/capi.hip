@@ -456,12 +456,12 @@ int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid) {
 }
 
 int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis, int64_t max_iter,
-                  int32_t precision, int32_t vec_type, const void* v0_host, double* out_evals,
+                  int32_t precision, int32_t vec_type, const void* v0, double* out_evals,
                   void* out_evecs, double* out_resid, int64_t* out_matvecs, void* stream) {
   EDLGT_REQUIRE(h && out_evals, "null argument");
   EDLGT_REQUIRE(tol > 0.0, "tol must be positive");
   EDLGT_REQUIRE(precision == EDLGT_PREC_FP64 || precision == EDLGT_PREC_MIXED, "bad precision");
-  return lanczos_run(*h, n_eigs, tol, max_basis, max_iter, precision, vec_type, v0_host, out_evals,
+  return lanczos_run(*h, n_eigs, tol, max_basis, max_iter, precision, vec_type, v0, out_evals,
                      out_evecs, out_resid, out_matvecs, (hipStream_t)stream);
 }
 
@@ -477,6 +477,13 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
                             (hipStream_t)stream));
   EDLGT_HIP(hipMemcpy(out_host, out.p, (size_t)bins * sizeof(double), hipMemcpyDeviceToHost));
   return EDLGT_OK;
+}
+
+int edlgt_random_uniform(void* out, int64_t n, int32_t value_type, const uint64_t* state, void* stream) {
+  EDLGT_REQUIRE(n >= 0, "n must be >= 0");
+  EDLGT_REQUIRE(n == 0 || (out && state), "null argument");
+  EDLGT_REQUIRE(value_type == EDLGT_VALUE_F64 || value_type == EDLGT_VALUE_C128, "value_type must be F64 or C128");
+  return random_uniform(out, n, value_type == EDLGT_VALUE_C128, state, (hipStream_t)stream);
 }
 
 int edlgt_krylov_evolve(edlgt_csr* h, const edlgt_basis* b, int32_t precision, void* psi, double dt,

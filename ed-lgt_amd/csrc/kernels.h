@@ -58,7 +58,7 @@ int csr_autotune(Csr& h, int precision, int xtype, int n_rhs, const void* x, voi
 int convert_f64_to_f32(const double* in, float* out, int64_t n, hipStream_t st);
 
 int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter, int precision,
-                int vec_type, const void* v0_host, double* evals, void* evecs, double* resid,
+                int vec_type, const void* v0, double* evals, void* evecs, double* resid,
                 int64_t* matvecs, hipStream_t st);
 
 int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, int64_t n_steps,
@@ -90,5 +90,10 @@ int server_step(void* p, int max_done, int64_t* ids, double* evals, double* est,
 
 int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
                     hipStream_t st);
+
+// out[i] = i-th uniform [0, 1) double of the PCG64 stream with 128-bit state
+// and increment {state_hi, state_lo, inc_hi, inc_lo} (numpy's
+// default_rng(...).random(n)); complex output carries it in the real part.
+int random_uniform(void* out, int64_t n, bool complex_values, const uint64_t state[4], hipStream_t st);
 
 }  // namespace edlgt

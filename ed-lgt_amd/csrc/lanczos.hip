@@ -1034,7 +1034,7 @@ struct Solver {
 
 template <class S>
 int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, int precision,
-                  const void* v0_host, double* evals, void* evecs, double* resid, int64_t* matvecs,
+                  const void* v0, double* evals, void* evecs, double* resid, int64_t* matvecs,
                   hipStream_t st) {
   const int64_t N = h.n_rows;
   if (k < 1 || k > N) return set_error(EDLGT_ERR_INVALID_ARG, "n_eigs must be in [1, n]");
@@ -1059,8 +1059,9 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   sv.M = M;
   EDLGT_TRY(sv.setup());
   // start block: caller's v0 in column 0, random columns after it
-  if (v0_host) {
-    EDLGT_HIP(hipMemcpyAsync(sv.col(0), v0_host, (size_t)N * sizeof(S), hipMemcpyHostToDevice, st));
+  if (v0) {
+    // host or device memory (unified addressing)
+    EDLGT_HIP(hipMemcpyAsync(sv.col(0), v0, (size_t)N * sizeof(S), hipMemcpyDefault, st));
   } else {
     EDLGT_TRY(sv.randomize(sv.col(0), N));
   }
@@ -1430,15 +1431,15 @@ int krylov_run(Csr& h, const Basis* basis, int precision, void* psi, double dt, 
 }
 
 int lanczos_run(Csr& h, int n_eigs, double tol, int max_basis, int64_t max_iter, int precision,
-                int vec_type, const void* v0_host, double* evals, void* evecs, double* resid,
+                int vec_type, const void* v0, double* evals, void* evecs, double* resid,
                 int64_t* matvecs, hipStream_t st) {
   if (vec_type == EDLGT_VALUE_C128)
-    return lanczos_typed<double2>(h, n_eigs, tol, max_basis, max_iter, precision, v0_host, evals,
+    return lanczos_typed<double2>(h, n_eigs, tol, max_basis, max_iter, precision, v0, evals,
                                   evecs, resid, matvecs, st);
   if (vec_type == EDLGT_VALUE_F64) {
     if (h.value_type != EDLGT_VALUE_F64)
       return set_error(EDLGT_ERR_INVALID_ARG, "real vectors need a real matrix");
-    return lanczos_typed<double>(h, n_eigs, tol, max_basis, max_iter, precision, v0_host, evals,
+    return lanczos_typed<double>(h, n_eigs, tol, max_basis, max_iter, precision, v0, evals,
                                  evecs, resid, matvecs, st);
   }
   return set_error(EDLGT_ERR_INVALID_ARG, "vec_type must be C128 or F64");

@@ -92,6 +92,7 @@ SIGNATURES = {
     "edlgt_krylov_create": (_i32, [_p, _p, _i32, _i32, _i32, _i32, _p, _p]),
     "edlgt_krylov_run": (_i32, [_p, _p, _f64, _i64, _p, _p, _p]),
     "edlgt_krylov_destroy": (_i32, [_p]),
+    "edlgt_random_uniform": (_i32, [_p, _i64, _i32, _p, _p]),
 }
 
 _lib = None

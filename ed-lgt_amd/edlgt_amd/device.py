@@ -185,6 +185,20 @@ def stored_bytes_bound(tables, placements, n_states, value_type):
     return int(n_states) * (per_row * (4 + s_v) + 8)
 
 
+def random_uniform(n, complex_values=False, seed=0, stream=None):
+    """numpy.random.default_rng(seed).random(n) generated on the device
+    (edlgt_random_uniform, bit for bit the same PCG64 stream); complex
+    output holds the values in the real parts."""
+    torch = _torch()
+    st = np.random.default_rng(seed).bit_generator.state["state"]
+    m64 = (1 << 64) - 1
+    state = np.array([st["state"] >> 64, st["state"] & m64, st["inc"] >> 64, st["inc"] & m64], dtype=np.uint64)
+    out = torch.empty(int(n), dtype=torch.complex128 if complex_values else torch.float64, device="cuda")
+    L.check(L.load().edlgt_random_uniform(L.ptr(out), int(n), L.VALUE_C128 if complex_values else L.VALUE_F64,
+                                          state.ctypes.data, L.stream_handle(stream)))
+    return out
+
+
 def expval_terms(basis, tables, placements, psi, stream=None):
     """<psi| strength_p prod_f O_f |psi> for every placement (complex array),
     one device pass over the basis for all of them (edlgt_expval_terms).
@@ -378,8 +392,14 @@ class CSRMatrix:
         torch = _torch()
         vt = self.vec_type(complex_vectors)
         if v0 is not None:
-            v0 = np.ascontiguousarray(v0, dtype=np.complex128 if vt == L.VALUE_C128 else np.float64)
-            if v0.size != self.n:
+            if hasattr(v0, "data_ptr"):  # device tensor: used in place
+                want = torch.complex128 if vt == L.VALUE_C128 else torch.float64
+                if v0.dtype != want or not v0.is_cuda:
+                    raise ValueError(f"device v0 must be a {want} CUDA tensor")
+                v0 = v0.contiguous()
+            else:
+                v0 = np.ascontiguousarray(v0, dtype=np.complex128 if vt == L.VALUE_C128 else np.float64)
+            if (v0.numel() if hasattr(v0, "numel") else v0.size) != self.n:
                 raise ValueError("v0 has the wrong length")
         evals = np.zeros(n_eigs)
         resid = np.zeros(n_eigs)

@@ -151,11 +151,15 @@ class QMB_hamiltonian:
             raise ValueError(f"n_eigs must be < H.shape[0]={self.shape[0]}, got {n_eigs}")
         self.loc_dims = loc_dims
         self.n_eigs = int(n_eigs)
-        if v0 is None:
-            v0 = np.random.default_rng(0).random(self.shape[0])
         cplx = self.Ham.is_complex
-        v0 = np.asarray(v0, dtype=np.complex128 if cplx else np.float64)
-        v0 = v0 / np.linalg.norm(v0)
+        if v0 is None:
+            # default_rng(0).random(N), generated on the device (the solver
+            # normalises its start block)
+            from .device import random_uniform
+            v0 = random_uniform(self.shape[0], complex_values=cplx, seed=0)
+        else:
+            v0 = np.asarray(v0, dtype=np.complex128 if cplx else np.float64)
+            v0 = v0 / np.linalg.norm(v0)
         evals, evecs, resid, mvs = self.Ham.lanczos(self.n_eigs, tol=tol, max_basis=max_basis,
                                                     precision=precision, v0=v0)
         self.energies = evals

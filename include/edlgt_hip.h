@@ -256,7 +256,8 @@ int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid);
  * Lowest eigenpairs by thick-restart Lanczos with full reorthogonalisation.
  * Replaces scipy eigsh(H, k, which="SA") in QMB_hamiltonian.diagonalize
  * (edlgt/modeling/qmb_hamiltonian.py:260-332).
- *   v0_host      start vector (host, vec_type, length n); NULL -> all ones
+ *   v0           start vector (vec_type, length n) in host or device memory;
+ *                NULL -> deterministic pseudo-random entries
  *   tol          stop when every wanted residual ||H y - theta y|| <= tol * max(1,|theta|)
  *   max_basis    Krylov basis size m (>= n_eigs + 2); <= 0 selects
  *                max(2 n_eigs + 1, 20), eigsh's default ncv
@@ -271,9 +272,22 @@ int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid);
  * budget is exhausted.
  * ====================================================================== */
 int edlgt_lanczos(edlgt_csr* h, int32_t n_eigs, double tol, int32_t max_basis,
-                  int64_t max_iter, int32_t precision, int32_t vec_type, const void* v0_host,
+                  int64_t max_iter, int32_t precision, int32_t vec_type, const void* v0,
                   double* out_evals, void* out_evecs, double* out_resid,
                   int64_t* out_matvecs, void* stream);
+
+/* Uniform [0, 1) doubles on the device from a PCG64 stream: out[i] is the
+ * i-th value of numpy.random.Generator(PCG64).random(n) for the bit
+ * generator whose 128-bit state and increment are
+ * state = {state_hi, state_lo, inc_hi, inc_lo} (numpy's
+ * bit_generator.state["state"]), bit for bit.  value_type EDLGT_VALUE_F64
+ * writes doubles, EDLGT_VALUE_C128 writes (value, 0) pairs.
+ * No counterpart in the reference's code: its eigsh draws its own random
+ * start vector; the model layer here starts every solve from
+ * default_rng(0).random(n) (a fixed seed, so solves are reproducible), and
+ * this builds it on the device instead of on the host plus a full-length
+ * host-to-device copy. */
+int edlgt_random_uniform(void* out, int64_t n, int32_t value_type, const uint64_t* state, void* stream);
 
 /* ======================================================================
  * Served lowest-eigenvalue solves with continuous batching (no reference

@@ -55,3 +55,18 @@ def test_one_hip_runtime_whatever_the_import_order():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().splitlines()[-1] == "1"
+
+
+def test_random_uniform_argument_checks_without_gpu():
+    # value type, negative length and null pointers fail before any device call
+    import numpy as np
+    from edlgt_amd import _lib
+    lib = _lib.load()
+    state = np.zeros(4, dtype=np.uint64)
+    assert lib.edlgt_random_uniform(None, 0, _lib.VALUE_F64, None, None) == _lib.OK
+    assert lib.edlgt_random_uniform(None, 5, _lib.VALUE_F64, state.ctypes.data, None) == _lib.ERR_INVALID_ARG
+    assert lib.edlgt_random_uniform(None, -1, _lib.VALUE_F64, state.ctypes.data, None) == _lib.ERR_INVALID_ARG
+    buf = np.zeros(4)
+    assert lib.edlgt_random_uniform(buf.ctypes.data, 4, _lib.VALUE_F32, state.ctypes.data,
+                                    None) == _lib.ERR_INVALID_ARG
+    assert "value_type" in _lib.last_error()

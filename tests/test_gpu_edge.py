@@ -1,6 +1,7 @@
 """GPU edge cases: an empty symmetry sector, a one-state sector, a sector
 solved in full (k = N), invalid requests that must raise the reference's
-exception types, and observables/expectation values on tiny sectors."""
+exception types, observables/expectation values on tiny sectors, and the
+device start vector (numpy's PCG64 stream)."""
 import numpy as np
 import pytest
 
@@ -87,3 +88,32 @@ def test_expectation_of_a_ket_outside_the_sector_raises(gpu):
     tab = np.stack([mod.ops["Qpx_dag"][0]])
     with pytest.raises(ValueError):
         expval_terms(mod.basis, tab, [dict(sites=[0, 2], tables=[0, 0])], psi)
+
+
+@pytest.mark.parametrize("seed", [0, 7, 2**40 + 3])
+def test_device_start_vector_is_numpy_default_rng(gpu, seed):
+    """edlgt_random_uniform reproduces numpy.random.default_rng(seed).random(n)
+    bit for bit (PCG64 jump-ahead per thread), for lengths around the
+    4096-value block and a ragged large one; complex output in the real part."""
+    import torch
+    from edlgt_amd.device import random_uniform
+    for n in (1, 255, 4096, 4097, 1_000_003):
+        want = np.random.default_rng(seed).random(n)
+        got = random_uniform(n, complex_values=False, seed=seed).cpu().numpy()
+        assert np.array_equal(got, want), (seed, n)
+    gc = random_uniform(9001, complex_values=True, seed=seed).cpu().numpy()
+    assert np.array_equal(gc.real, np.random.default_rng(seed).random(9001)) and not gc.imag.any()
+    assert random_uniform(0, seed=seed).numel() == 0
+    torch.cuda.synchronize()
+
+
+def test_default_start_equals_host_start(gpu):
+    """diagonalize without v0 (device-generated default_rng(0) start) gives
+    the energies of the same solve started from the host vector."""
+    mod = _su2([6], [True], 6)
+    mod.build_Hamiltonian(1.0, 1.0)
+    mod.diagonalize_Hamiltonian(3, "sparse")
+    e_dev = np.array(mod.H.energies)
+    v0 = np.random.default_rng(0).random(mod.sector_dim)
+    mod.H.diagonalize(3, "sparse", mod.loc_dims, v0=v0)
+    assert np.allclose(e_dev, mod.H.energies, atol=1e-11, rtol=0)
