@@ -54,6 +54,7 @@ __device__ inline double2 to_c(double2 a) { return a; }
 __device__ inline double2 to_c(double a) { return make_double2(a, 0.0); }
 
 constexpr double kDgks = 0.5;  // squared-norm ratio above which one Gram-Schmidt pass suffices
+constexpr double kSkipLoss = 1e-14;  // measured loss of orthogonality left uncorrected
 constexpr int kGramBlocks = 1024;
 constexpr int kGramCols = 64;  // V columns per Gram launch (one per lane)
 constexpr int kGramRows = 8;   // rows per thread per sweep
@@ -414,6 +415,7 @@ struct Solver {
   DevBuf V, W, Tmp, part, gdev, cdev, shadow, pack;
   int64_t mv64 = 0, mv32 = 0;
   int64_t ortho_single = 0, ortho_double = 0;  // orthogonalisations with one / two passes
+  int64_t ortho_skipped = 0;  // full passes whose update was skipped (loss below kSkipLoss)
   uint64_t seed = 12345;
   bool need_tmp = true;      // restart / Ritz buffer (not needed by Krylov stepping)
   bool want_vectors = true;  // form the Ritz vectors when a run finishes
@@ -649,10 +651,29 @@ struct Solver {
         if (nl > 0) {
           for (int c = 0; c < nw; ++c) nref[c] = std::max(G1(c, c).real(), 0.0);
           EDLGT_TRY(gram(Vp, nv, X, nw, G));
-          mG = G;
-          for (auto& z : mG.a) z = -z;
-          EDLGT_TRY(update(Vp, nv, mG, X, nw, 1, &G1));
-          for (size_t i = 0; i < G.a.size(); ++i) Gout.a[i] += G.a[i];
+          // The Gram measures X's loss of orthogonality to the whole basis.
+          // Below kSkipLoss (relative to each column's norm) the update is
+          // skipped: X keeps those components, T keeps the exact local
+          // coefficients (H V = V T + Q R stays exact), and the basis stays
+          // orthogonal to kSkipLoss times one step's growth, because every
+          // step measures again and corrects once the loss exceeds it.
+          bool small = true;
+          for (int c = 0; c < nw && small; ++c) {
+            const double lim = kSkipLoss * kSkipLoss * nref[c];
+            for (int j = 0; j < nv; ++j)
+              if (!(std::norm(G(j, c)) <= lim)) {
+                small = false;
+                break;
+              }
+          }
+          if (small) {
+            ++ortho_skipped;
+          } else {
+            mG = G;
+            for (auto& z : mG.a) z = -z;
+            EDLGT_TRY(update(Vp, nv, mG, X, nw, 1, &G1));
+            for (size_t i = 0; i < G.a.size(); ++i) Gout.a[i] += G.a[i];
+          }
         }
         again = false;
         for (int c = 0; c < nw; ++c)
