@@ -35,7 +35,7 @@ constexpr int kMaxOps = 8;
 constexpr double kOpTol = 1e-10;   // sym_qmb_operations.py:21
 constexpr double kRealTol = 1e-12; // dtype_config.py:101-112
 
-enum : int32_t { kErrTooMany = 1, kErrMissing = 2, kErrNotReal = 4, kErrOverflow = 8 };
+enum : int32_t { kErrTooMany = 1, kErrMissing = 2, kErrNotReal = 4, kErrOverflow = 8, kErrCount = 16 };
 
 struct Placement {
   int32_t nops;
@@ -64,6 +64,7 @@ struct AsmDev {
   const uint8_t* trans_ket;
   const double2* trans_val;
   const double2* diag;            // [n_tables][d_max]  O[a][a]
+  const uint8_t* trans_self;      // [(2t+side)*d_max + a]: 1 if a's transition list holds a itself
   int32_t* err;
   // bucket index of the keys: bucket[b] = first index with key >= (b << shift)
   const int64_t* bucket;
@@ -217,6 +218,46 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
   }
 }
 
+// Stored entries of every row without enumerating them: per term, one
+// diagonal entry if any placement reaches the bra itself, plus every raw
+// entry that changes some site (a placement's factors act on distinct sites,
+// so a transition changes the configuration unless every factor keeps its
+// state, and each factor's list holds its own state at most once).  This is
+// the exact count unless two raw entries of one term reach the same ket; the
+// fill pass compares and flags kErrCount, and the host then counts by
+// enumeration.  One thread per row, placements read as broadcasts.
+__global__ void __launch_bounds__(256) predict_counts(AsmDev A, int32_t* __restrict__ row_count) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A.n_states;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
+    uint32_t diag_terms = 0;  // bit t: term t has a diagonal entry in this row
+    int64_t offd = 0;
+    for (int p = 0; p < A.n_place; ++p) {
+      const Placement& P = A.place[p];
+      if (P.nops == 1) {
+        const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
+        if (sqrt(v.x * v.x + v.y * v.y) >= kOpTol) diag_terms |= 1u << P.term;
+        continue;
+      }
+      int64_t c = 1;
+      int self = 1;
+#pragma unroll
+      for (int f = 0; f < kMaxOps; ++f) {
+        if (f < P.nops) {
+          const int a = cfg[P.site[f]];
+          const int side = 2 * P.table[f] + P.dagger;
+          const int32_t* off = A.trans_off + (size_t)side * (A.d_max + 1);
+          c *= off[a + 1] - off[a];
+          self &= A.trans_self[(size_t)side * A.d_max + a];
+        }
+      }
+      offd += c - self;
+      if (self) diag_terms |= 1u << P.term;
+    }
+    row_count[r] = (int32_t)(offd + __popc(diag_terms));
+  }
+}
+
 // One 64-thread block == one wavefront; LDS holds the row's raw entries.
 template <int kCap, bool kFill, class KeyT>
 __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restrict__ row_count,
@@ -328,6 +369,7 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
       rank_base += tot;
     }
     if (!kFill && lane == 0) row_count[r] = rank_base;
+    if (kFill && lane == 0 && rank_base != row_ptr[r + 1] - row_ptr[r]) atomicOr(A.err, kErrCount);
     __syncthreads();
   }
 }
@@ -383,7 +425,7 @@ namespace {
 // column nonzeros above OP_TOL), the placements, the basis' keys, strides and
 // configurations, as the assembly and expectation kernels read them.
 struct Program {
-  DevBuf place, str, toff, tket, tval, diag, err, bucket;
+  DevBuf place, str, toff, tket, tval, tself, diag, err, bucket;
   AsmDev A;
   int cap = 1;  // raw entries per row bound (assembly LDS size)
 };
@@ -401,7 +443,7 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
   // Host: transition lists (row and column nonzeros) of every table.
   using cd = std::complex<double>;
   std::vector<int32_t> trans_off((size_t)2 * n_tables * (d_max + 1));
-  std::vector<uint8_t> trans_ket;
+  std::vector<uint8_t> trans_ket, trans_self((size_t)2 * n_tables * d_max, 0);
   std::vector<double2> trans_val, diag((size_t)n_tables * d_max);
   for (int t = 0; t < n_tables; ++t) {
     const double* O = tables + (size_t)t * d_max * d_max * 2;
@@ -417,6 +459,7 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
           if (std::abs(v) > kOpTol) {
             trans_ket.push_back((uint8_t)b);
             trans_val.push_back(make_double2(v.real(), v.imag()));
+            if (b == a) trans_self[(size_t)(2 * t + side) * d_max + a] = 1;
           }
         }
       }
@@ -465,6 +508,7 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
   EDLGT_TRY(upload(prog.tket, trans_ket.empty() ? &z8 : trans_ket.data(), std::max<size_t>(trans_ket.size(), 1), st));
   EDLGT_TRY(upload(prog.tval, trans_val.empty() ? &z2 : trans_val.data(), std::max<size_t>(trans_val.size(), 1), st));
   EDLGT_TRY(upload(prog.diag, diag.data(), diag.size(), st));
+  EDLGT_TRY(upload(prog.tself, trans_self.data(), trans_self.size(), st));
   EDLGT_TRY(prog.err.alloc(sizeof(int32_t)));
   EDLGT_HIP(hipMemsetAsync(prog.err.p, 0, sizeof(int32_t), st));
 
@@ -475,6 +519,7 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
   A.keys = B.keys.p; A.configs = B.configs.as<uint8_t>();
   A.trans_off = prog.toff.as<int32_t>(); A.trans_ket = prog.tket.as<uint8_t>();
   A.trans_val = prog.tval.as<double2>(); A.diag = prog.diag.as<double2>(); A.err = prog.err.as<int32_t>();
+  A.trans_self = prog.tself.as<uint8_t>();
 
   // bucket index over the leading bits of the largest key: about 2^22 buckets
   {
@@ -702,24 +747,43 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   EDLGT_TRY(out->row_ptr.alloc((N + 1) * sizeof(int64_t)));
   DevBuf counts;
   EDLGT_TRY(counts.alloc(std::max<int64_t>(N, 1) * sizeof(int32_t)));
-  if (N > 0)
-    EDLGT_TRY(launch_any(B.key_words, cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr,
-                         nullptr, st));
-  EDLGT_TRY(check_err(b_err, st));
-  EDLGT_TRY(exclusive_scan_i32_i64(counts.as<int32_t>(), out->row_ptr.as<int64_t>(), N, st));
-  int64_t nnz = 0;
-  EDLGT_HIP(hipMemcpyAsync(&nnz, out->row_ptr.as<int64_t>() + N, sizeof(int64_t),
-                           hipMemcpyDeviceToHost, st));
-  EDLGT_HIP(hipStreamSynchronize(st));
-  out->nnz = nnz;
-  EDLGT_TRY(out->col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
-  EDLGT_TRY(out->val.alloc(std::max<int64_t>(nnz, 1) * value_bytes(value_type)));
   out->n_terms = n_terms;
-  if (n_terms > 1) EDLGT_TRY(out->term.alloc(std::max<int64_t>(nnz, 1)));
-  if (N > 0)
-    EDLGT_TRY(launch_any(B.key_words, cap, true, A, nullptr, out->row_ptr.as<int64_t>(),
-                         out->col.as<int32_t>(), out->val.as<double>(),
-                         n_terms > 1 ? out->term.as<uint8_t>() : nullptr, st));
+  // Row counts from the transition tables alone (predict_counts); if the fill
+  // finds two raw entries of a row reaching one ket (kErrCount), the rows are
+  // counted again by enumeration (the exact pass) and filled once more.
+  for (int exact = 0; exact < 2; ++exact) {
+    if (N > 0) {
+      if (exact) {
+        EDLGT_TRY(launch_any(B.key_words, cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr,
+                             nullptr, st));
+      } else {
+        hipLaunchKernelGGL(predict_counts, dim3(grid_for(N, 256, 16384)), dim3(256), 0, st, A,
+                           counts.as<int32_t>());
+        EDLGT_HIP(hipGetLastError());
+      }
+    }
+    EDLGT_TRY(check_err(b_err, st));
+    EDLGT_TRY(exclusive_scan_i32_i64(counts.as<int32_t>(), out->row_ptr.as<int64_t>(), N, st));
+    int64_t nnz = 0;
+    EDLGT_HIP(hipMemcpyAsync(&nnz, out->row_ptr.as<int64_t>() + N, sizeof(int64_t),
+                             hipMemcpyDeviceToHost, st));
+    EDLGT_HIP(hipStreamSynchronize(st));
+    out->nnz = nnz;
+    EDLGT_TRY(out->col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+    EDLGT_TRY(out->val.alloc(std::max<int64_t>(nnz, 1) * value_bytes(value_type)));
+    if (n_terms > 1) EDLGT_TRY(out->term.alloc(std::max<int64_t>(nnz, 1)));
+    if (N > 0)
+      EDLGT_TRY(launch_any(B.key_words, cap, true, A, nullptr, out->row_ptr.as<int64_t>(),
+                           out->col.as<int32_t>(), out->val.as<double>(),
+                           n_terms > 1 ? out->term.as<uint8_t>() : nullptr, st));
+    int32_t e = 0;
+    EDLGT_HIP(hipMemcpyAsync(&e, b_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    EDLGT_HIP(hipStreamSynchronize(st));
+    if (!(e & kErrCount) || exact) break;
+    if (e & ~kErrCount) break;  // a real error: reported below
+    // predicted counts were too high for some row: clear the flag, count exactly
+    EDLGT_HIP(hipMemsetAsync(b_err.p, 0, sizeof(int32_t), st));
+  }
   EDLGT_TRY(check_err(b_err, st));
   EDLGT_TRY(csr_prepare_stream(*out, st));
   return EDLGT_OK;

@@ -117,3 +117,33 @@ def test_default_start_equals_host_start(gpu):
     v0 = np.random.default_rng(0).random(mod.sector_dim)
     mod.H.diagonalize(3, "sparse", mod.loc_dims, v0=v0)
     assert np.allclose(e_dev, mod.H.energies, atol=1e-11, rtol=0)
+
+
+def test_repeated_placements_are_summed(gpu):
+    """Two raw entries of one row reaching the same ket (a placement listed
+    twice) are summed into one stored entry: the assembly's table-predicted
+    row counts are then too high, the fill pass detects it and the rows are
+    counted by enumeration.  Checked against the dense sum of Kronecker
+    products on the full 3-site product space (27 states)."""
+    import scipy.sparse as sp
+    from edlgt_amd.device import CSRMatrix, SectorBasis
+    import edlgt_amd._lib as L
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    D = np.diag(rng.normal(size=3)).astype(complex)
+    tables = np.stack([A, D])
+    pl = [dict(sites=[0, 1], tables=[0, 0], strength=0.7),
+          dict(sites=[0, 1], tables=[0, 0], strength=0.7),
+          dict(sites=[1, 2], tables=[0, 0], strength=0.3, dagger=True),
+          dict(sites=[2], tables=[1], strength=1.0)]
+    basis = SectorBasis([3, 3, 3])
+    H = CSRMatrix(basis, tables, pl, L.VALUE_C128)
+    rp, col, val = H.to_host()
+    got = sp.csr_matrix((val, col, rp), shape=(27, 27)).toarray()
+    I = np.eye(3)
+    want = 1.4 * np.kron(np.kron(A, A), I) + (0.3 * np.kron(I, np.kron(A, A))).conj().T \
+        + np.kron(np.kron(I, I), D)
+    assert np.allclose(got, want, atol=1e-13)
+    # one stored entry per distinct position, columns ascending in every row
+    assert H.nnz == np.count_nonzero(np.abs(want) > 1e-14)
+    assert all(np.all(np.diff(col[rp[r]:rp[r + 1]]) > 0) for r in range(27))
