@@ -54,6 +54,10 @@ struct SectorDev {
   const int32_t* nb_sites;
   const int32_t* loc_dims;     // [n_sites]
   const uint64_t* strides;     // [n_sites]
+  // link_mask[k * d_max + v] (d_max <= 64, else null): bit a set iff the
+  // activated link k holds with its later site in state a and its other site
+  // in state v
+  const uint64_t* link_mask;
 };
 
 // n-body constraint `id` on a full assignment of its sites: additive (U1),
@@ -251,6 +255,25 @@ __device__ inline bool check_links_nb(const SectorDev& D, int s, const uint8_t* 
   return true;
 }
 
+// States of site s that satisfy every link activated at s, given the
+// digits of the earlier sites (one table lookup per link).
+__device__ inline uint64_t allowed_states(const SectorDev& D, int s, const uint8_t* dig) {
+  const int ds = D.loc_dims[s];
+  uint64_t m = ds >= 64 ? ~0ull : ((1ull << ds) - 1);
+  for (int k = D.act_link_off[s]; k < D.act_link_off[s + 1]; ++k) {
+    const int i = D.act_link[3 * k + 1], j = D.act_link[3 * k + 2];
+    const int other = i == s ? j : i;  // i == j == s: the table row does not depend on v
+    m &= D.link_mask[(size_t)k * D.d_max + dig[other]];
+  }
+  return m;
+}
+
+__device__ inline bool check_nb(const SectorDev& D, int s, const uint8_t* dig) {
+  for (int k = D.act_nb_off[s]; k < D.act_nb_off[s + 1]; ++k)
+    if (!nbody_ok(D, D.act_nb[k], dig)) return false;
+  return true;
+}
+
 template <bool kFill, class KeyT>
 __global__ void __launch_bounds__(64) complete_prefixes_int(
     SectorDev D, IntGlob G, const KeyT* __restrict__ strides, const uint64_t* __restrict__ frontier,
@@ -284,8 +307,15 @@ __global__ void __launch_bounds__(64) complete_prefixes_int(
     } else {
       int s = p;
       int cur = -1;
+      const bool masks = D.link_mask != nullptr;
       while (true) {
-        ++cur;
+        if (masks) {
+          // next state above cur that the links activated at s allow
+          const uint64_t m = allowed_states(D, s, dig) & (cur >= 63 ? 0ull : (~0ull << (cur + 1)));
+          cur = m ? __ffsll((long long)m) - 1 : D.loc_dims[s];
+        } else {
+          ++cur;
+        }
         if (cur >= D.loc_dims[s]) {
           if (s == p) break;
           --s;
@@ -309,7 +339,7 @@ __global__ void __launch_bounds__(64) complete_prefixes_int(
               ok = gs == t;
           }
         }
-        if (!ok || !check_links_nb(D, s, dig)) continue;
+        if (!ok || !(masks ? check_nb(D, s, dig) : check_links_nb(D, s, dig))) continue;
         if (s == n - 1) {
           if (kFill) {
             keys[out] = key + (KeyT)cur * strides[s];
@@ -407,9 +437,32 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
     act_nb.insert(act_nb.end(), by_site_nb[s].begin(), by_site_nb[s].end());
     act_nb_off[s + 1] = (int32_t)act_nb.size();
   }
+  // Link masks for the depth-first completion (states of a link's later
+  // site allowed by each state of its other site), same test as check_site.
+  std::vector<uint64_t> link_mask;
+  const int n_act = (int)(act_link.size() / 3);
+  if (d_max <= 64 && n_act > 0) {
+    link_mask.assign((size_t)n_act * d_max, 0);
+    for (int s = 0; s < n_sites; ++s)
+      for (int k = act_link_off[s]; k < act_link_off[s + 1]; ++k) {
+        const int dir = act_link[3 * k], i = act_link[3 * k + 1], j = act_link[3 * k + 2];
+        for (int v = 0; v < d_max; ++v) {
+          uint64_t m = 0;
+          for (int a = 0; a < loc_dims[s]; ++a) {
+            // the later site s takes state a, the other site state v (i == j: both a)
+            const int ai = i == s ? a : v, aj = j == s ? a : v;
+            const double x = link_diags[(((size_t)dir * 2 + 0) * n_sites + i) * d_max + ai] +
+                             link_diags[(((size_t)dir * 2 + 1) * n_sites + j) * d_max + aj];
+            if (std::fabs(x - link_sectors[dir]) <= kSymTol) m |= 1ull << a;
+          }
+          link_mask[(size_t)k * d_max + v] = m;
+        }
+      }
+  }
   // Upload.
   DevBuf b_gd, b_gs, b_rmin, b_rmax, b_alo, b_al, b_ld, b_ls, b_ano, b_an, b_nd, b_ns, b_nso, b_nsi,
-      b_ldims, b_str;
+      b_ldims, b_str, b_lmask;
+  if (!link_mask.empty()) EDLGT_TRY(upload(b_lmask, link_mask.data(), link_mask.size(), st));
   const double zero = 0.0;
   const int32_t izero = 0;
   EDLGT_TRY(upload(b_gd, n_glob ? glob_diags : &zero, n_glob ? (size_t)n_glob * n_sites * d_max : 1, st));
@@ -440,6 +493,7 @@ int sector_build(int32_t n_sites, const int32_t* loc_dims, int32_t d_max, int32_
   D.nbody_type = nbody_type; D.nbody_diags = b_nd.as<double>(); D.nbody_sectors = b_ns.as<double>();
   D.nb_site_off = b_nso.as<int32_t>(); D.nb_sites = b_nsi.as<int32_t>();
   D.loc_dims = b_ldims.as<int32_t>(); D.strides = b_str.as<uint64_t>();
+  D.link_mask = link_mask.empty() ? nullptr : b_lmask.as<uint64_t>();
 
   // Exact integer form of the U(1) constraints: every diagonal value and
   // target scaled by 2^k (k <= 12) must be an integer of modest size.
