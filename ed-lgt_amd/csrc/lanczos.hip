@@ -1089,6 +1089,10 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   }
   sv.b = b;
   sv.M = M;
+  // convergence tested after every block step (a host Rayleigh-Ritz of the
+  // projected matrix, at most M x M), not only when the basis is full: the
+  // solve stops within one step of convergence instead of up to a basis later
+  if (M < N && M <= 128) sv.early_every = 1;
   EDLGT_TRY(sv.setup());
   // start block: caller's v0 in column 0, random columns after it
   if (v0) {
