@@ -50,6 +50,8 @@ struct Placement {
 struct AsmDev {
   int64_t n_states;
   int n_place;
+  int n_terms;                    // <= 16
+  int max_ops;                    // largest factor count of any placement
   int d_max;
   int cfg_stride;
   int value_real;                 // 1: float64 output (values must be real)
@@ -71,6 +73,17 @@ struct AsmDev {
   int shift;
   int64_t n_bucket;
 };
+
+// Sum over the 64 lanes by a fixed xor butterfly (the same order for every
+// row), complete in every lane.
+__device__ inline double2 wave_sum2(double2 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  return v;
+}
 
 __device__ inline int wave_excl_scan(int v, int* total) {
   const int lane = threadIdx.x & 63;
@@ -125,8 +138,11 @@ __global__ void bucket_starts(const KeyT* __restrict__ keys, int64_t n, int shif
 }
 
 // Number of raw entries placement P emits from the row with local states cfg.
-// Loops over factors are unrolled to kMaxOps with guards so that every
-// per-factor value stays in registers (no private-memory arrays).
+// Loops over factors are unrolled to kOps (>= every placement's factor count)
+// with guards so that every per-factor value stays in registers (no
+// private-memory arrays); kOps = 4 covers every model here and halves the
+// registers the per-factor state takes.
+template <int kOps = kMaxOps>
 __device__ inline int placement_count(const AsmDev& A, const Placement& P, const uint8_t* cfg) {
   if (P.nops == 1) {
     const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
@@ -134,7 +150,7 @@ __device__ inline int placement_count(const AsmDev& A, const Placement& P, const
   }
   int c = 1;
 #pragma unroll
-  for (int f = 0; f < kMaxOps; ++f) {
+  for (int f = 0; f < kOps; ++f) {
     if (f < P.nops) {
       const int a = cfg[P.site[f]];
       const int32_t* off = A.trans_off + (size_t)(2 * P.table[f] + P.dagger) * (A.d_max + 1);
@@ -145,7 +161,7 @@ __device__ inline int placement_count(const AsmDev& A, const Placement& P, const
 }
 
 // Enumerates placement P's entries into LDS slots [slot0, slot0 + count).
-template <bool kFill, class KeyT>
+template <bool kFill, class KeyT, int kOps = kMaxOps>
 __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const uint8_t* cfg,
                                       KeyT key, int slot0, KeyT* skey, uint16_t* sidx,
                                       uint8_t* sterm, double2* sval) {
@@ -162,11 +178,11 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
     }
     return;
   }
-  int beg[kMaxOps], cnt[kMaxOps], idx[kMaxOps];
-  KeyT stride[kMaxOps];
+  int beg[kOps], cnt[kOps], idx[kOps];
+  KeyT stride[kOps];
   KeyT base_delta = 0;  // modular (wrap-around) key arithmetic
 #pragma unroll
-  for (int f = 0; f < kMaxOps; ++f) {
+  for (int f = 0; f < kOps; ++f) {
     beg[f] = 0;
     cnt[f] = 1;
     idx[f] = 0;
@@ -185,7 +201,7 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
     KeyT delta = base_delta;
     double2 amp = make_double2(1.0, 0.0);
 #pragma unroll
-    for (int f = 0; f < kMaxOps; ++f) {
+    for (int f = 0; f < kOps; ++f) {
       if (f < P.nops) {
         const int e = beg[f] + idx[f];
         delta += (KeyT)A.trans_ket[e] * stride[f];
@@ -205,7 +221,7 @@ __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const
     // mixed-radix increment, last factor fastest (cnt == 1 beyond nops)
     bool carry = true;
 #pragma unroll
-    for (int f = kMaxOps - 1; f >= 0; --f) {
+    for (int f = kOps - 1; f >= 0; --f) {
       if (carry && f < P.nops) {
         if (++idx[f] < cnt[f]) {
           carry = false;
@@ -259,7 +275,7 @@ __global__ void __launch_bounds__(256) predict_counts(AsmDev A, int32_t* __restr
 }
 
 // One 64-thread block == one wavefront; LDS holds the row's raw entries.
-template <int kCap, bool kFill, class KeyT>
+template <int kCap, bool kFill, class KeyT, int kOps>
 __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restrict__ row_count,
                                                     const int64_t* __restrict__ row_ptr,
                                                     int32_t* __restrict__ out_col,
@@ -275,15 +291,41 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
     const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
     const KeyT key = keys[r];
     // 1+2: count placements, assign slots in placement order, emit.
+    // Single-factor (diagonal) placements take no slot: their values are
+    // summed per term across the wave by a fixed butterfly, rounds in
+    // placement order, and each term with any enters the sort as one entry.
     int base = 0;
     bool overflow = false;
+    uint32_t dmask = 0;                        // terms with a diagonal entry (wave-uniform)
+    double2 dacc = make_double2(0.0, 0.0);     // lane t: diagonal sum of term t
     for (int pb = 0; pb < A.n_place; pb += 64) {
       const int p = pb + lane;
       int c = 0;
+      bool dg = false;
       Placement P;
       if (p < A.n_place) {
         P = A.place[p];
-        c = placement_count(A, P, cfg);
+        c = placement_count<kOps>(A, P, cfg);
+        if (P.nops == 1) {
+          dg = c > 0;
+          c = 0;
+        }
+      }
+      if (__ballot(dg)) {
+        double2 v = make_double2(0.0, 0.0);
+        if (kFill && dg) {
+          v = cmul(A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]], make_double2(P.s_re, P.s_im));
+          if (A.value_real && fabs(v.y) > kRealTol) atomicOr(A.err, kErrNotReal);
+        }
+        for (int t = 0; t < A.n_terms; ++t) {
+          const bool mine = dg && P.term == t;
+          if (!__ballot(mine)) continue;
+          dmask |= 1u << t;
+          if (kFill) {
+            const double2 sum = wave_sum2(mine ? v : make_double2(0.0, 0.0));
+            if (lane == t) dacc = make_double2(dacc.x + sum.x, dacc.y + sum.y);
+          }
+        }
       }
       int tot;
       const int off = wave_excl_scan(c, &tot);
@@ -291,8 +333,23 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
         overflow = true;
         break;
       }
-      if (c > 0) placement_emit<kFill, KeyT>(A, P, cfg, key, base + off, skey, sidx, sterm, sval);
+      if (c > 0) placement_emit<kFill, KeyT, kOps>(A, P, cfg, key, base + off, skey, sidx, sterm, sval);
       base += tot;
+    }
+    if (!overflow) {
+      const int nd = __popc(dmask);
+      if (base + nd > kCap) {
+        overflow = true;
+      } else {
+        if (lane < 16 && ((dmask >> lane) & 1u)) {
+          const int slot = base + __popc(dmask & ((1u << lane) - 1u));
+          skey[slot] = key;
+          sidx[slot] = (uint16_t)slot;
+          sterm[slot] = (uint8_t)lane;
+          if (kFill) sval[slot] = dacc;
+        }
+        base += nd;
+      }
     }
     if (overflow) {
       if (lane == 0) atomicOr(A.err, kErrTooMany);
@@ -378,12 +435,15 @@ template <int kCap, class KeyT>
 int launch_pass(bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr, int32_t* col,
                 double* val, uint8_t* term, hipStream_t st) {
   const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(A.n_states, 1), 1 << 18);
-  if (fill)
-    hipLaunchKernelGGL((assemble_rows<kCap, true, KeyT>), dim3(grid), dim3(64), 0, st, A, counts,
-                       row_ptr, col, val, term);
-  else
-    hipLaunchKernelGGL((assemble_rows<kCap, false, KeyT>), dim3(grid), dim3(64), 0, st, A, counts,
-                       row_ptr, col, val, term);
+#define ASM_LAUNCH(FILL, OPS)                                                                       \
+  hipLaunchKernelGGL((assemble_rows<kCap, FILL, KeyT, OPS>), dim3(grid), dim3(64), 0, st, A, counts, \
+                     row_ptr, col, val, term)
+  if (A.max_ops <= 4) {
+    if (fill) ASM_LAUNCH(true, 4); else ASM_LAUNCH(false, 4);
+  } else {
+    if (fill) ASM_LAUNCH(true, kMaxOps); else ASM_LAUNCH(false, kMaxOps);
+  }
+#undef ASM_LAUNCH
   EDLGT_HIP(hipGetLastError());
   return EDLGT_OK;
 }
@@ -490,11 +550,12 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
         bound *= mx;
       }
     }
-    if (P.nops == 1) bound = 1;
+    if (P.nops == 1) bound = 0;  // folded into one entry per term (added below)
     cap_bound += bound;
     P.s_re = place_strength[2 * p];
     P.s_im = place_strength[2 * p + 1];
   }
+  cap_bound += n_terms;
   prog.cap = (int)std::min<int64_t>(std::max<int64_t>(cap_bound, 1), 1024);
 
   EDLGT_TRY(upload(prog.place, place.data(), std::max<size_t>(place.size(), 1), st));
@@ -513,7 +574,9 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
   EDLGT_HIP(hipMemsetAsync(prog.err.p, 0, sizeof(int32_t), st));
 
   AsmDev& A = prog.A;
-  A.n_states = B.n_states; A.n_place = n_place; A.d_max = d_max; A.cfg_stride = B.cfg_stride;
+  A.n_states = B.n_states; A.n_place = n_place; A.n_terms = n_terms; A.d_max = d_max; A.cfg_stride = B.cfg_stride;
+  A.max_ops = 1;
+  for (const Placement& P : place) A.max_ops = std::max(A.max_ops, (int)P.nops);
   A.value_real = value_type == EDLGT_VALUE_F64;
   A.place = prog.place.as<Placement>(); A.strides = prog.str.p;
   A.keys = B.keys.p; A.configs = B.configs.as<uint8_t>();
@@ -521,7 +584,7 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
   A.trans_val = prog.tval.as<double2>(); A.diag = prog.diag.as<double2>(); A.err = prog.err.as<int32_t>();
   A.trans_self = prog.tself.as<uint8_t>();
 
-  // bucket index over the leading bits of the largest key: about 2^22 buckets
+  // bucket index over the leading bits of the largest key
   {
     const int64_t N = B.n_states;
     int bits = 0;
@@ -536,8 +599,13 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
         while (kmax) { ++bits; kmax >>= 1; }
       }
     }
-    A.shift = std::max(0, bits - 22);
-    A.n_bucket = (int64_t)1 << std::min(bits, 22);
+    // about two buckets per state (2^10 .. 2^25 buckets): a search reads the
+    // bucket bounds and one or two keys
+    int target = 1;
+    while (target < 25 && ((int64_t)1 << target) < 2 * N) ++target;
+    target = std::max(target, 10);
+    A.shift = std::max(0, bits - target);
+    A.n_bucket = (int64_t)1 << std::min(bits, target);
     EDLGT_TRY(prog.bucket.alloc((size_t)(A.n_bucket + 1) * sizeof(int64_t)));
     const unsigned g = grid_for(A.n_bucket + 1, 256, 16384);
     if (B.key_words == 2)
