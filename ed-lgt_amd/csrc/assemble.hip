@@ -641,7 +641,7 @@ __device__ inline void store_y(double* y, int64_t r, double2 v) { y[r] = v.x; }
 __device__ inline void store_y(double2* y, int64_t r, double2 v) { y[r] = v; }
 
 // sum over placement P's transitions from this row of amp * psi[col]
-template <class KeyT, class PT>
+template <class KeyT, class PT, int kOps = kMaxOps>
 __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, const uint8_t* cfg, KeyT key,
                                           int64_t r, const PT* __restrict__ psi) {
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
@@ -651,12 +651,12 @@ __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, c
     if (sqrt(v.x * v.x + v.y * v.y) < kOpTol) return make_double2(0.0, 0.0);
     return cmul(v, to_c2(psi[r]));
   }
-  int beg[kMaxOps], cnt[kMaxOps], idx[kMaxOps];
-  KeyT stride[kMaxOps];
+  int beg[kOps], cnt[kOps], idx[kOps];
+  KeyT stride[kOps];
   KeyT base_delta = 0;
   bool empty = false;
 #pragma unroll
-  for (int f = 0; f < kMaxOps; ++f) {
+  for (int f = 0; f < kOps; ++f) {
     beg[f] = 0;
     cnt[f] = 1;
     idx[f] = 0;
@@ -677,7 +677,7 @@ __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, c
     KeyT delta = base_delta;
     double2 amp = make_double2(1.0, 0.0);
 #pragma unroll
-    for (int f = 0; f < kMaxOps; ++f) {
+    for (int f = 0; f < kOps; ++f) {
       if (f < P.nops) {
         const int e = beg[f] + idx[f];
         delta += (KeyT)A.trans_ket[e] * stride[f];
@@ -697,7 +697,7 @@ __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, c
     if (col >= 0) sum = cadd(sum, cmul(amp, to_c2(psi[col])));
     bool carry = true;
 #pragma unroll
-    for (int f = kMaxOps - 1; f >= 0; --f) {
+    for (int f = kOps - 1; f >= 0; --f) {
       if (carry && f < P.nops) {
         if (++idx[f] < cnt[f]) {
           carry = false;
@@ -711,7 +711,7 @@ __device__ inline double2 placement_apply(const AsmDev& A, const Placement& P, c
   return sum;
 }
 
-template <class KeyT, class PT>
+template <class KeyT, class PT, int kOps>
 __global__ void __launch_bounds__(64 * kExpWaves) expval_rows(AsmDev A, int p0, int np, const PT* __restrict__ psi,
                                                               double2* __restrict__ partial) {
   __shared__ double2 acc[kExpWaves][64 * kExpRounds];
@@ -727,7 +727,7 @@ __global__ void __launch_bounds__(64 * kExpWaves) expval_rows(AsmDev A, int p0, 
 #pragma unroll 1
     for (int p = lane; p < np; p += 64) {
       const Placement P = A.place[p0 + p];
-      const double2 s = cmul(placement_apply<KeyT, PT>(A, P, cfg, key, r, psi), make_double2(P.s_re, P.s_im));
+      const double2 s = cmul(placement_apply<KeyT, PT, kOps>(A, P, cfg, key, r, psi), make_double2(P.s_re, P.s_im));
       acc[wave][p] = cadd(acc[wave][p], cmul(pr, s));
     }
   }
@@ -764,7 +764,7 @@ __global__ void __launch_bounds__(256) expval_finish(const double2* __restrict__
 // placements l, l+64, ...; the lane sums are combined by a fixed butterfly
 // (deterministic).  Column lookups use a bucket index over the leading key
 // bits (a few probes instead of log2 N).
-template <class KeyT, class XT, class AT>
+template <class KeyT, class XT, class AT, int kOps>
 __global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const XT* __restrict__ x, AT* __restrict__ y) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
@@ -777,7 +777,7 @@ __global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const XT* __restri
       const Placement P = A.place[p];
       // a dagger placement's entries are conj(amp * strength), as assembled
       const double2 sp = make_double2(P.s_re, P.dagger ? -P.s_im : P.s_im);
-      acc = cadd(acc, cmul(placement_apply<KeyT, XT>(A, P, cfg, key, r, x), sp));
+      acc = cadd(acc, cmul(placement_apply<KeyT, XT, kOps>(A, P, cfg, key, r, x), sp));
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -875,9 +875,15 @@ int placements_expval(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   EDLGT_TRY(out.alloc((size_t)n_place * sizeof(double2)));
   for (int p0 = 0; p0 < n_place; p0 += kBatch) {
     const int np = std::min(kBatch, n_place - p0);
-#define EXP_LAUNCH(KT, PT)                                                                          \
-  hipLaunchKernelGGL((expval_rows<KT, PT>), dim3(nblk), dim3(64 * kExpWaves), 0, st, prog.A, p0, np, \
-                     static_cast<const PT*>(psi), partial.as<double2>())
+#define EXP_LAUNCH(KT, PT)                                                                            \
+  do {                                                                                                \
+    if (prog.A.max_ops <= 4)                                                                          \
+      hipLaunchKernelGGL((expval_rows<KT, PT, 4>), dim3(nblk), dim3(64 * kExpWaves), 0, st, prog.A, p0, \
+                         np, static_cast<const PT*>(psi), partial.as<double2>());                     \
+    else                                                                                              \
+      hipLaunchKernelGGL((expval_rows<KT, PT, kMaxOps>), dim3(nblk), dim3(64 * kExpWaves), 0, st, prog.A, \
+                         p0, np, static_cast<const PT*>(psi), partial.as<double2>());                 \
+  } while (0)
     if (B.key_words == 2) {
       if (vec_type == EDLGT_VALUE_C128) EXP_LAUNCH(u128, double2); else EXP_LAUNCH(u128, double);
     } else {
@@ -924,8 +930,14 @@ int matfree_spmm(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx,
     const char* xj = static_cast<const char*>(x) + (size_t)j * ldx * es;
     char* yj = static_cast<char*>(y) + (size_t)j * ldy * es;
 #define MF_LAUNCH(KT, XT)                                                                                \
-  hipLaunchKernelGGL((matfree_rows<KT, XT, XT>), dim3(grid), dim3(256), 0, st, m.prog.A,                \
-                     reinterpret_cast<const XT*>(xj), reinterpret_cast<XT*>(yj))
+  do {                                                                                               \
+    if (m.prog.A.max_ops <= 4)                                                                       \
+      hipLaunchKernelGGL((matfree_rows<KT, XT, XT, 4>), dim3(grid), dim3(256), 0, st, m.prog.A,         \
+                         reinterpret_cast<const XT*>(xj), reinterpret_cast<XT*>(yj));                   \
+    else                                                                                             \
+      hipLaunchKernelGGL((matfree_rows<KT, XT, XT, kMaxOps>), dim3(grid), dim3(256), 0, st, m.prog.A,   \
+                         reinterpret_cast<const XT*>(xj), reinterpret_cast<XT*>(yj));                   \
+  } while (0)
     if (m.key_words == 2) {
       if (xtype == EDLGT_VALUE_C128) MF_LAUNCH(u128, double2); else MF_LAUNCH(u128, double);
     } else {
