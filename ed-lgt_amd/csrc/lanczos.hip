@@ -525,9 +525,9 @@ struct Solver {
     hipLaunchKernelGGL((tall_update<S, NWC, false>), dim3(g), dim3(kTB), shmem, st, Vp, N, nv,     \
                        cdev.as<double2>(), nw, X, N, nw, N, beta, (double2*)nullptr);             \
     break;
-    if (nw > 32) return set_error(EDLGT_ERR_INVALID_ARG, "update width above 32");
-    int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : nw <= 8 ? 8 : nw <= 16 ? 16 : 32;
-    switch (nwp) { UPD_CASE(1) UPD_CASE(2) UPD_CASE(4) UPD_CASE(8) UPD_CASE(16) UPD_CASE(32) }
+    if (nw > 16) return set_error(EDLGT_ERR_INVALID_ARG, "update width above 16");
+    int nwp = nw <= 1 ? 1 : nw <= 2 ? 2 : nw <= 4 ? 4 : nw <= 8 ? 8 : 16;
+    switch (nwp) { UPD_CASE(1) UPD_CASE(2) UPD_CASE(4) UPD_CASE(8) UPD_CASE(16) }
 #undef UPD_CASE
     EDLGT_HIP(hipGetLastError());
     EDLGT_HIP(hipStreamSynchronize(st));  // cdev is reused by the next call
@@ -940,11 +940,12 @@ struct Solver {
     }
   }
 
-  // X[:, 0:nc] = V[:, 0:nv] C for any nc (chunks of up to 32 columns: V is
-  // read once per chunk).
+  // X[:, 0:nc] = V[:, 0:nv] C for any nc (chunks of up to 16 columns: V is
+  // read once per chunk; a 32-column kernel holds twice the accumulators and
+  // measured slower than two 16-column passes).
   int update_blocked(const S* Vp, int nv, const Dense& C, S* X, int nc) {
-    for (int c0 = 0; c0 < nc; c0 += 32) {
-      const int w = std::min(32, nc - c0);
+    for (int c0 = 0; c0 < nc; c0 += 16) {
+      const int w = std::min(16, nc - c0);
       Dense Cc(nv, w);
       for (int j = 0; j < nv; ++j)
         for (int c = 0; c < w; ++c) Cc(j, c) = C(j, c0 + c);
