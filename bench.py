@@ -183,6 +183,9 @@ def su2_dynamics(args):
     mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[Ls], lvals=[Ls], has_obc=[True])
     torch.cuda.synchronize()
     t_sector = time.perf_counter() - t0
+    # stored CSR, then coded here so that assembly and coding are timed apart
+    # (the model's automatic choice makes the same coded matrix in one step)
+    mod.matrix_free = False
     t0 = time.perf_counter()
     H = mod.build_Hamiltonian(1.0, 1.0)
     torch.cuda.synchronize()

@@ -160,24 +160,14 @@ __device__ inline int placement_count(const AsmDev& A, const Placement& P, const
   return c;
 }
 
-// Enumerates placement P's entries into LDS slots [slot0, slot0 + count).
+// Enumerates the entries of a multi-factor placement P into LDS slots
+// [slot0, slot0 + count) (single-factor placements are folded per term by
+// the caller).
 template <bool kFill, class KeyT, int kOps = kMaxOps>
 __device__ inline void placement_emit(const AsmDev& A, const Placement& P, const uint8_t* cfg,
                                       KeyT key, int slot0, KeyT* skey, uint16_t* sidx,
                                       uint8_t* sterm, double2* sval) {
   const KeyT* strides = static_cast<const KeyT*>(A.strides);
-  if (P.nops == 1) {
-    skey[slot0] = key;
-    sidx[slot0] = (uint16_t)slot0;
-    sterm[slot0] = (uint8_t)P.term;
-    if (kFill) {
-      const double2 v = A.diag[(size_t)P.table[0] * A.d_max + cfg[P.site[0]]];
-      double2 r = cmul(v, make_double2(P.s_re, P.s_im));
-      if (A.value_real && fabs(r.y) > kRealTol) atomicOr(A.err, kErrNotReal);
-      sval[slot0] = r;
-    }
-    return;
-  }
   int beg[kOps], cnt[kOps], idx[kOps];
   KeyT stride[kOps];
   KeyT base_delta = 0;  // modular (wrap-around) key arithmetic
@@ -896,6 +886,28 @@ int placements_expval(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   }
   EDLGT_HIP(hipMemcpyAsync(out_host, out.p, (size_t)n_place * sizeof(double2), hipMemcpyDeviceToHost, st));
   EDLGT_TRY(check_err(prog.err, st));  // synchronises
+  return EDLGT_OK;
+}
+
+int hamiltonian_count(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables, int32_t n_place,
+                      const int32_t* place_nops, const int32_t* place_sites, const int32_t* place_tables,
+                      const double* place_strength, const int32_t* place_dagger, hipStream_t st,
+                      int64_t* out_nnz) {
+  Program prog;
+  EDLGT_TRY(prepare_program(B, d_max, n_tables, tables, n_place, place_nops, place_sites, place_tables,
+                            place_strength, place_dagger, nullptr, 1, EDLGT_VALUE_C128, st, prog));
+  const int64_t N = B.n_states;
+  *out_nnz = 0;
+  if (N == 0) return EDLGT_OK;
+  DevBuf counts, offs;
+  EDLGT_TRY(counts.alloc((size_t)N * sizeof(int32_t)));
+  EDLGT_TRY(offs.alloc((size_t)(N + 1) * sizeof(int64_t)));
+  hipLaunchKernelGGL(predict_counts, dim3(grid_for(N, 256, 16384)), dim3(256), 0, st, prog.A,
+                     counts.as<int32_t>());
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_TRY(exclusive_scan_i32_i64(counts.as<int32_t>(), offs.as<int64_t>(), N, st));
+  EDLGT_HIP(hipMemcpyAsync(out_nnz, offs.as<int64_t>() + N, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  EDLGT_HIP(hipStreamSynchronize(st));
   return EDLGT_OK;
 }
 

@@ -208,6 +208,17 @@ int edlgt_expval_terms(const edlgt_basis* b, int32_t d_max, int32_t n_tables, co
                            place_strength, psi, vec_type, out_host, (hipStream_t)stream);
 }
 
+int edlgt_hamiltonian_count(const edlgt_basis* b, int32_t d_max, int32_t n_tables, const double* tables,
+                            int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                            const int32_t* place_tables, const double* place_strength,
+                            const int32_t* place_dagger, void* stream, int64_t* out_nnz) {
+  EDLGT_REQUIRE(b && out_nnz && tables, "null argument");
+  EDLGT_REQUIRE(n_place == 0 || (place_nops && place_sites && place_tables && place_strength && place_dagger),
+                "null placement arrays");
+  return hamiltonian_count(*b, d_max, n_tables, tables, n_place, place_nops, place_sites, place_tables,
+                           place_strength, place_dagger, (hipStream_t)stream, out_nnz);
+}
+
 int edlgt_hamiltonian_build_matfree(const edlgt_basis* b, int32_t d_max, int32_t n_tables,
                                     const double* tables, int32_t n_place, const int32_t* place_nops,
                                     const int32_t* place_sites, const int32_t* place_tables,

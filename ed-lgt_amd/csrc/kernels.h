@@ -19,6 +19,13 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
                       const int32_t* place_tables, const double* place_strength,
                       const int32_t* place_dagger, const int32_t* place_term, int32_t n_terms,
                       int32_t value_type, hipStream_t st, Csr* out);
+// Stored entries of edlgt_hamiltonian_build's CSR from the transition tables
+// alone (exact unless two raw entries of a row reach the same ket; never
+// below the true count).
+int hamiltonian_count(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables, int32_t n_place,
+                      const int32_t* place_nops, const int32_t* place_sites, const int32_t* place_tables,
+                      const double* place_strength, const int32_t* place_dagger, hipStream_t st,
+                      int64_t* out_nnz);
 int hamiltonian_build_matfree(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables,
                               int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
                               const int32_t* place_tables, const double* place_strength,

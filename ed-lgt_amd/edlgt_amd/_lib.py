@@ -57,6 +57,7 @@ SIGNATURES = {
     "edlgt_basis_destroy": (_i32, [_p]),
     "edlgt_hamiltonian_build": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "edlgt_hamiltonian_build_matfree": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _i32, _p, _p]),
+    "edlgt_hamiltonian_count": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _p, _p]),
     "edlgt_csr_is_matrix_free": (_i32, [_p, _p]),
     "edlgt_csr_info": (_i32, [_p, _p, _p, _p]),
     "edlgt_csr_copy": (_i32, [_p, _p, _p, _p]),

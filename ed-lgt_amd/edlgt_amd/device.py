@@ -167,22 +167,34 @@ def _pack_placements(placements):
     return nops, sites, tabs, stren, dag, term
 
 
-def stored_bytes_bound(tables, placements, n_states, value_type):
-    """Upper bound of a stored CSR's device bytes: every placement's largest
-    transition count per row (products of the factors' largest row counts of
-    entries above 1e-10; single-factor terms are diagonal), times the states."""
-    tables = np.asarray(tables)
-    row_max = (np.abs(tables) > 1e-10).sum(axis=2).max(axis=1)
-    col_max = (np.abs(tables) > 1e-10).sum(axis=1).max(axis=1)
-    per_row = 0
-    for pl in placements:
-        if len(pl["tables"]) == 1:
-            per_row += 1
-            continue
-        m = col_max if pl.get("dagger") else row_max
-        per_row += int(np.prod([max(int(m[t]), 0) for t in pl["tables"]]))
+def stored_entries(basis, tables, placements, stream=None):
+    """Entries the stored CSR of these placements would hold, counted on the
+    device from the transition tables (edlgt_hamiltonian_count: exact unless
+    two raw entries of a row reach the same ket, never below the true count)."""
+    tables = np.ascontiguousarray(tables, dtype=np.complex128)
+    n_tables, d_max = tables.shape[0], tables.shape[1]
+    nops, sites, tabs, stren, dag, _ = _pack_placements(placements)
+    out = C.c_int64()
+    L.check(L.load().edlgt_hamiltonian_count(
+        basis.handle, d_max, n_tables, tables.ctypes.data, len(placements), nops.ctypes.data,
+        sites.ctypes.data, tabs.ctypes.data, stren.ctypes.data, dag.ctypes.data,
+        L.stream_handle(stream) if stream is not None else None, C.byref(out)))
+    return int(out.value)
+
+
+def storage_plan(n_states, nnz, value_type, free_bytes):
+    """How to hold a Hamiltonian with nnz stored entries in free_bytes of
+    device memory, leaving about half of it to the solvers: "stored" (CSR
+    with values), "coded" (CSR built in full, then its values replaced by
+    codes of at most 2 bytes), or "free" (matrix-free operator)."""
     s_v = 8 if value_type == L.VALUE_F64 else 16
-    return int(n_states) * (per_row * (4 + s_v) + 8)
+    full = int(nnz) * (4 + s_v) + 8 * (int(n_states) + 1)
+    coded = int(nnz) * (4 + 2) + 8 * (int(n_states) + 1)
+    if full <= free_bytes // 2:
+        return "stored"
+    if full <= int(0.9 * free_bytes) and coded <= free_bytes // 2:
+        return "coded"
+    return "free"
 
 
 def random_uniform(n, complex_values=False, seed=0, stream=None):

@@ -245,9 +245,11 @@ class QuantumModel:
         self.n_sites = prod(self.lvals)
         self.ham_format = ham_format
         self.momentum_basis = None
-        # True: build the Hamiltonian as a matrix-free operator; "auto": only
-        # when the stored CSR's upper bound would exceed half the free device
-        # memory; False: always stored
+        # True: build the Hamiltonian as a matrix-free operator; False: always
+        # a stored CSR; "auto": from the stored entries counted on the device,
+        # a stored CSR while it takes at most half the free device memory, a
+        # dictionary-coded one while the full build fits and the coded form
+        # takes at most half, else matrix-free
         self.matrix_free = "auto"
         self.staggered_basis = False
         self.basis = None
@@ -312,22 +314,34 @@ class QuantumModel:
             if add_dagger:
                 self._placements.append(dict(sites=list(sl), tables=tabs, strength=strength, dagger=1))
 
-    def _use_matrix_free(self, tables, vt):
+    def _storage_mode(self, tables, vt):
         if self.momentum_basis is not None or self.matrix_free is False:
-            return False
+            return "stored"
         if self.matrix_free is True:
-            return True
+            return "free"
         import torch
-        from .device import stored_bytes_bound
+        from .device import storage_plan, stored_entries
         free, _ = torch.cuda.mem_get_info()
-        return stored_bytes_bound(tables, self._placements, self.sector_dim, vt) > free // 2
+        nnz = stored_entries(self.basis, tables, self._placements)
+        return storage_plan(self.sector_dim, nnz, vt, free)
 
     def _build_program(self, real):
         tables = np.stack(self._tables)
         vt = L.VALUE_F64 if real else L.VALUE_C128
-        mf = self._use_matrix_free(tables, vt)
-        csr = CSRMatrix(self.basis, tables, self._placements, vt, matrix_free=mf)
-        if mf:
+        mode = self._storage_mode(tables, vt)
+        csr = CSRMatrix(self.basis, tables, self._placements, vt, matrix_free=mode == "free")
+        if mode == "coded":
+            # values replaced by dictionary codes (products bit for bit the
+            # same); a matrix with too many distinct values is rebuilt
+            # matrix-free instead
+            if csr.compress() > 0:
+                csr.release_values()
+                logger.info("Hamiltonian stored dictionary-coded (values released)")
+            else:
+                del csr
+                mode = "free"
+                csr = CSRMatrix(self.basis, tables, self._placements, vt, matrix_free=True)
+        if mode == "free":
             logger.info("Hamiltonian built matrix-free (entries generated in every product)")
         if self.momentum_basis is not None:
             # B^H H B in the momentum sector (the reference's momentum triplets,

@@ -153,6 +153,19 @@ int edlgt_hamiltonian_build(const edlgt_basis* b, int32_t d_max, int32_t n_table
                             const double* place_strength, const int32_t* place_dagger,
                             int32_t value_type, void* stream, edlgt_csr** out);
 
+/* Number of entries edlgt_hamiltonian_build would store for the same
+ * arguments, from the transition tables alone (one thread per row, no
+ * enumeration or search): exact unless two raw entries of one row reach the
+ * same ket (then an upper bound).  Lets a caller size the CSR, or choose the
+ * matrix-free operator, before building.  Replaces the reference's counting
+ * pass (_count_row_nnz_from_transition_data + _get_nnz_offsets,
+ * edlgt/symmetries/sym_qmb_operations.py:95-164), which counts raw entries
+ * before coalescing; this counts the stored ones. */
+int edlgt_hamiltonian_count(const edlgt_basis* b, int32_t d_max, int32_t n_tables, const double* tables,
+                            int32_t n_place, const int32_t* place_nops, const int32_t* place_sites,
+                            const int32_t* place_tables, const double* place_strength,
+                            const int32_t* place_dagger, void* stream, int64_t* out_nnz);
+
 /* The same operator without storing it (for sectors whose CSR would not fit
  * in device memory): the handle holds the term program (same arguments as
  * edlgt_hamiltonian_build) and a bucket index over the sorted keys, and every

@@ -70,3 +70,19 @@ def test_random_uniform_argument_checks_without_gpu():
     assert lib.edlgt_random_uniform(buf.ctypes.data, 4, _lib.VALUE_F32, state.ctypes.data,
                                     None) == _lib.ERR_INVALID_ARG
     assert "value_type" in _lib.last_error()
+
+
+def test_storage_plan_thresholds():
+    """Stored while the CSR takes at most half the free memory, coded while
+    the full build fits in 90 % and the 2-byte-coded form in half, else
+    matrix-free (host-side rule; no device call)."""
+    from edlgt_amd import _lib as L
+    from edlgt_amd.device import storage_plan
+    n, nnz = 1000, 20000
+    full = nnz * 20 + 8 * (n + 1)
+    assert storage_plan(n, nnz, L.VALUE_C128, 2 * full) == "stored"
+    assert storage_plan(n, nnz, L.VALUE_C128, 2 * full - 2) == "coded"
+    assert storage_plan(n, nnz, L.VALUE_C128, int(full / 0.9) + 1) == "coded"
+    assert storage_plan(n, nnz, L.VALUE_C128, full) == "free"
+    # real values: 12 bytes per entry with values
+    assert storage_plan(n, nnz, L.VALUE_F64, 2 * (nnz * 12 + 8 * (n + 1))) == "stored"

@@ -82,3 +82,44 @@ def test_stored_only_entry_points_refuse(gpu):
         H.lanczos(1, precision=1)  # FP32 products need stored values
     with pytest.raises(ValueError):
         Server(H, n_eigs=1)
+
+
+@pytest.mark.parametrize("kind", ["qed2x2", "su2_l6", "z3_3x3"])
+def test_device_count_equals_stored_entries(gpu, kind):
+    """edlgt_hamiltonian_count (transition tables only, no enumeration) gives
+    the stored CSR's entry count exactly for these models."""
+    from edlgt_amd import QED_Model, SU2_Model, Zn_Model
+    from edlgt_amd.device import stored_entries
+    if kind == "qed2x2":
+        m, args = QED_Model(spin=1, pure_theory=False, lvals=[2, 2], has_obc=[False, False]), (0.7, 0.3)
+    elif kind == "su2_l6":
+        m, args = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False]), (1.0, 5.0)
+    else:
+        m, args = Zn_Model(3, lvals=[3, 3], has_obc=[False, False], flux_sector=(0, 0)), (1.0, 0.7)
+    m.matrix_free = False
+    H = m.build_Hamiltonian(*args)
+    assert stored_entries(m.basis, np.stack(m._tables), m._placements) == H.nnz
+
+
+@pytest.mark.parametrize("mode", ["stored", "coded", "free"])
+def test_automatic_storage_choice(gpu, monkeypatch, mode):
+    """matrix_free = "auto" under a pretended amount of free device memory:
+    plenty -> stored CSR; room for the full build but not for a solver beside
+    it -> dictionary-coded CSR with its values released; too little -> the
+    matrix-free operator.  Every form gives the stored spectrum (QED 2x2 PBC,
+    547 states)."""
+    import torch
+    from edlgt_amd import QED_Model
+    ref = QED_Model(spin=1, pure_theory=False, lvals=[2, 2], has_obc=[False, False])
+    ref.matrix_free = False
+    Hr = ref.build_Hamiltonian(0.7, 0.3)
+    er, _, _, _ = Hr.lanczos(3, tol=1e-12, want_vectors=False)
+    full = Hr.nnz * (4 + 16) + 8 * (Hr.n + 1)  # complex values
+    free = {"stored": 10 * full, "coded": int(1.5 * full), "free": full // 2}[mode]
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda *a: (free, 300 * 10**9))
+    m = QED_Model(spin=1, pure_theory=False, lvals=[2, 2], has_obc=[False, False])
+    H = m.build_Hamiltonian(0.7, 0.3)
+    got = "free" if H.matrix_free else ("coded" if getattr(H, "n_dict", 0) > 0 else "stored")
+    assert got == mode
+    e, _, _, _ = H.lanczos(3, tol=1e-12, want_vectors=False)
+    assert np.max(np.abs(e - er)) < 1e-10
