@@ -603,9 +603,10 @@ struct Solver {
   // local > 0 (X directly follows V, nw <= 4): X = H V[:, nv-b:nv] of a
   // Lanczos step whose exact coefficients are confined to the last `local`
   // columns (block tridiagonal T).  Those are removed first (a Gram and an
-  // update over local columns only), then one full pass over V removes what
-  // rounding left along the older columns.  That pass cancels almost nothing,
-  // so the DGKS test below rarely asks for another: two sweeps over V per
+  // update over local columns only), then a full Gram over V measures what
+  // rounding left along the older columns, and the full update runs only
+  // where that exceeds kSkipLoss.  The full pass cancels almost nothing, so
+  // the DGKS test below rarely asks for another: about one sweep over V per
   // step instead of four.
   int orthonormalize(int nv, S* X, int nw, Dense& Gout, Dense& Rout, int local = 0) {
     std::vector<double> n0(nw);

@@ -167,6 +167,23 @@ def _pack_placements(placements):
     return nops, sites, tabs, stren, dag, term
 
 
+def stored_entries_bound(tables, placements, n_states):
+    """Host-side upper bound of the stored entries: every placement's largest
+    transition count per row (products of the factors' largest row counts of
+    entries above 1e-10; single-factor terms are diagonal), times the states."""
+    tables = np.asarray(tables)
+    row_max = (np.abs(tables) > 1e-10).sum(axis=2).max(axis=1)
+    col_max = (np.abs(tables) > 1e-10).sum(axis=1).max(axis=1)
+    per_row = 0
+    for pl in placements:
+        if len(pl["tables"]) == 1:
+            per_row += 1
+            continue
+        m = col_max if pl.get("dagger") else row_max
+        per_row += int(np.prod([max(int(m[t]), 0) for t in pl["tables"]]))
+    return int(n_states) * per_row
+
+
 def stored_entries(basis, tables, placements, stream=None):
     """Entries the stored CSR of these placements would hold, counted on the
     device from the transition tables (edlgt_hamiltonian_count: exact unless

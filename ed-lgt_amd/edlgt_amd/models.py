@@ -320,8 +320,11 @@ class QuantumModel:
         if self.matrix_free is True:
             return "free"
         import torch
-        from .device import storage_plan, stored_entries
+        from .device import storage_plan, stored_entries, stored_entries_bound
         free, _ = torch.cuda.mem_get_info()
+        bound = stored_entries_bound(tables, self._placements, self.sector_dim)
+        if storage_plan(self.sector_dim, bound, vt, free) == "stored":
+            return "stored"  # even the host bound fits: no device count needed
         nnz = stored_entries(self.basis, tables, self._placements)
         return storage_plan(self.sector_dim, nnz, vt, free)
 

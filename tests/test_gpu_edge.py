@@ -147,3 +147,6 @@ def test_repeated_placements_are_summed(gpu):
     # one stored entry per distinct position, columns ascending in every row
     assert H.nnz == np.count_nonzero(np.abs(want) > 1e-14)
     assert all(np.all(np.diff(col[rp[r]:rp[r + 1]]) > 0) for r in range(27))
+    # the table-only count cannot see the collisions: an upper bound here
+    from edlgt_amd.device import stored_entries
+    assert stored_entries(basis, tables, pl) > H.nnz
