@@ -1263,7 +1263,9 @@ int krylov_typed(Csr& h, const Basis* basis, int precision, void* psi, double dt
 // SpMM is computed in the same order for any batch).  Warm (warm_cap > 0):
 // the server keeps the Ritz vectors of up to warm_cap finished requests on the
 // device and starts a new request from those of the nearest coefficient
-// vector; the result then agrees with the cold one to the requested tolerance.
+// vector, or, when a second cached coupling lies on the other side of the
+// request, from the two interpolated after a Procrustes rotation; the result
+// then agrees with the cold one to the requested tolerance.
 template <class S>
 struct Server {
   Csr& h;
@@ -1355,6 +1357,80 @@ struct Server {
     return best;
   }
 
+  // The cached coupling nearest to c among those with c between them and
+  // entry e1 (projection weight w of c on the segment e1 -> e in (0, 1)),
+  // or -1.
+  int bracket_warm(const double* c, int e1, double& w) const {
+    int best = -1;
+    double bd = 0.0;
+    for (int e = 0; e < warm_n; ++e) {
+      if (e == e1) continue;
+      double num = 0.0, den = 0.0, d = 0.0;
+      for (int t = 0; t < h.n_terms; ++t) {
+        const double s1 = warm_coef[e][t] - warm_coef[e1][t];
+        num += (c[t] - warm_coef[e1][t]) * s1;
+        den += s1 * s1;
+        d += (warm_coef[e][t] - c[t]) * (warm_coef[e][t] - c[t]);
+      }
+      if (!(den > 0.0)) continue;
+      const double we = num / den;
+      if (!(we > 0.0 && we < 1.0)) continue;
+      if (best < 0 || d < bd) {
+        best = e;
+        bd = d;
+        w = we;
+      }
+    }
+    return best;
+  }
+
+  // Start block between two cached couplings: the Ritz block Y2 is rotated
+  // onto Y1 (orthogonal Procrustes: Q = polar(Y2^H Y1), which fixes the
+  // signs, phases and the mixing inside degenerate groups) and the two are
+  // interpolated, X = (1 - w) Y1 + w Y2 Q, an approximation second order in
+  // the coupling step where either block alone is first order.  Leaves
+  // done = false (X untouched) when the two subspaces are too far apart.
+  int interpolate_warm(Solver<S>& sv, int e1, int e2, double w, bool& done) {
+    done = false;
+    const int64_t N = h.n_rows;
+    const S* Y1 = warm.as<S>() + (size_t)e1 * k * N;
+    const S* Y2 = warm.as<S>() + (size_t)e2 * k * N;
+    Dense G;
+    EDLGT_TRY(sv.gram(Y1, k, Y2, k, G));  // Y1^H Y2
+    Dense GG(k, k);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        cd z = 0.0;
+        for (int q = 0; q < k; ++q) z += G(i, q) * std::conj(G(j, q));
+        GG(i, j) = z;
+      }
+    std::vector<double> lam;
+    Dense W;
+    hermitian_eig(GG, lam, W);
+    if (!(lam[0] >= 0.25)) return EDLGT_OK;  // a principal angle above 60 degrees
+    Dense Sm(k, k);  // (G G^H)^(-1/2)
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) {
+        cd z = 0.0;
+        for (int q = 0; q < k; ++q) z += W(i, q) * (1.0 / std::sqrt(lam[q])) * std::conj(W(j, q));
+        Sm(i, j) = z;
+      }
+    Dense Gh(k, k);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) Gh(i, j) = std::conj(G(j, i));
+    const Dense Q = matmul(Gh, Sm);
+    Dense C1(k, b), C2(k, b);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < b; ++j) {
+        C1(i, j) = i == j ? cd(1.0 - w, 0.0) : cd(0.0, 0.0);
+        C2(i, j) = w * Q(i, j);
+      }
+    EDLGT_TRY(sv.update(Y1, k, C1, sv.col(0), b, 0));
+    EDLGT_TRY(sv.update(Y2, k, C2, sv.col(0), b, 1));
+    done = true;
+    return EDLGT_OK;
+  }
+
   // Prefers an allocated free slot; allocates a new one (up to max_slots)
   // otherwise.  *slot_out = -1 when every slot is busy.
   int submit(int64_t req, const double* coef, uint64_t seed, int* slot_out) {
@@ -1390,8 +1466,13 @@ struct Server {
     // pairs while a lower or degenerate eigenvector is still missing).
     sl.sv->early_every = e >= 0 ? 1 : 0;
     if (e >= 0) {
-      EDLGT_HIP(hipMemcpyAsync(sl.sv->col(0), warm.as<S>() + (size_t)e * k * N, (size_t)b * N * sizeof(S),
-                               hipMemcpyDeviceToDevice, st));
+      double w = 0.0;
+      const int e2 = (k == b && k <= 8) ? bracket_warm(sl.coef, e, w) : -1;
+      bool done = false;
+      if (e2 >= 0) EDLGT_TRY(interpolate_warm(*sl.sv, e, e2, w, done));
+      if (!done)
+        EDLGT_HIP(hipMemcpyAsync(sl.sv->col(0), warm.as<S>() + (size_t)e * k * N, (size_t)b * N * sizeof(S),
+                                 hipMemcpyDeviceToDevice, st));
     } else {
       EDLGT_TRY(sl.sv->randomize(sl.sv->col(0), (int64_t)b * N));
     }
