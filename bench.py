@@ -321,6 +321,16 @@ def serve_leg(args):
     served["peak_device_GB"] = round(svc.peak_bytes / 1e9, 2)
     del svc
     torch.cuda.empty_cache()
+    # capacity: the same requests all present at t = 0 (fresh servers and
+    # caches), so the rate is the servers' own, not the arrival rate
+    progress("served stream done; capacity run")
+    burst = [dict(r, arrival=0.0) for r in reqs]
+    svc = SolveService(csrs, n_eigs=4, tol=1e-8, max_basis=args.serve_basis, max_slots=16, warm_cache=64)
+    bres = svc.run(burst)
+    capacity = dict(requests=len(bres), wall_s=round(svc.wall, 3), solves_per_s=round(len(bres) / svc.wall, 2),
+                    all_converged=all(r["converged"] for r in bres.values()))
+    del svc
+    torch.cuda.empty_cache()
 
     def solve_one(r):
         H = models[r["problem"]].build_Hamiltonian(r["coef"][0], r["coef"][1])
@@ -341,8 +351,9 @@ def serve_leg(args):
                  "h~U[0.3,3], lowest 4 eigenvalues at tol 1e-8",
         stream=dict(requests=args.serve_requests, poisson_rate_per_s=args.serve_rate, seed=1),
         setup_s=round(t_setup, 3), served=served, plain_sample=plain,
-        served_on_plain_sample=served_prefix,
+        served_on_plain_sample=served_prefix, served_capacity=capacity,
         speedup_solves_per_s=round(served["solves_per_s"] / plain["service_solves_per_s"], 2),
+        speedup_capacity=round(capacity["solves_per_s"] / plain["service_solves_per_s"], 2),
         max_dE_served_vs_plain=max(d), max_dE0_served_vs_plain=max(d0))
 
 
