@@ -9,7 +9,7 @@
 //   against the whole basis with two classical Gram-Schmidt passes
 //   (G = V^H W, W -= V G), and factored W = Q R by Cholesky QR twice.  The
 //   projected matrix T = V^H H V (Hermitian, host) receives G and R.  When
-//   the basis is full, T is diagonalised (cyclic complex Jacobi), residual
+//   the basis is full, T is diagonalised (Householder + implicit QL), residual
 //   norms ||R S_i|| decide convergence, and the l lowest Ritz vectors are
 //   kept (V <- V S[:, :l]) together with the residual block Q and the
 //   coupling R S.  Rank loss of W (invariant subspace) is repaired column by
