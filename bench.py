@@ -459,9 +459,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-basis", type=int, default=128,
+    ap.add_argument("--serve-basis", type=int, default=160,
                     help="Lanczos basis size of the served solves (measured on the stream: p99 "
-                         "6.9 s at 64, 5.9 s at 80, 4.5 s at 128)")
+                         "3.8 s at 128, 3.4 s at 160, 3.7 s at 192)")
     ap.add_argument("--plain-basis", type=int, default=64,
                     help="Lanczos basis size of the plain solves, the best of 20 (eigsh's "
                          "default), 32, 48, 64 and 96 measured on the stream's first requests")
