@@ -1206,9 +1206,12 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   // that can hold the whole space uses single vectors and runs to exhaustion.
   int b = std::min(k, 4);
   // max_basis <= 0: ARPACK's default basis size ncv = max(2k + 1, 20), the
-  // size the reference's eigsh uses (measured the faster choice for a few
-  // eigenvalues: the reorthogonalisation cost grows with the basis)
-  if (max_basis <= 0) max_basis = std::max(2 * k + 1, 20);
+  // size the reference's eigsh uses (measured the faster choice for one
+  // eigenvalue: the reorthogonalisation cost grows with the basis), and at
+  // least 16 blocks when the block holds several vectors (k = 4 lowest of
+  // the Z3 4x4 sector: 5 blocks of 4 needed 2008 matvecs; 16 blocks were the
+  // fastest of 5 to 24 on the served couplings)
+  if (max_basis <= 0) max_basis = std::max(std::max(2 * k + 1, 20), b > 1 ? 16 * b : 0);
   int M = std::max(max_basis, 2 * k + 2 * b);
   M = (M + b - 1) / b * b;
   if (N <= 512 && M >= N) {

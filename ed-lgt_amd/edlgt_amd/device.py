@@ -417,7 +417,8 @@ class CSRMatrix:
     def lanczos(self, n_eigs, tol=1e-12, max_basis=0, max_iter=100000, precision=L.PREC_FP64,
                 v0=None, complex_vectors=None, want_vectors=True, stream=None):
         """Lowest eigenpairs: (evals ascending, evecs tensor (k, N) or None, resid, matvecs).
-        max_basis 0: eigsh's default basis size max(2k + 1, 20)."""
+        max_basis 0: eigsh's default basis size max(2k + 1, 20), at least 16
+        blocks of min(k, 4) vectors when k > 1."""
         torch = _torch()
         vt = self.vec_type(complex_vectors)
         if v0 is not None:

@@ -273,7 +273,8 @@ int edlgt_csr_set_launch(edlgt_csr* h, int32_t lanes_per_row, int64_t grid);
  *                NULL -> deterministic pseudo-random entries
  *   tol          stop when every wanted residual ||H y - theta y|| <= tol * max(1,|theta|)
  *   max_basis    Krylov basis size m (>= n_eigs + 2); <= 0 selects
- *                max(2 n_eigs + 1, 20), eigsh's default ncv
+ *                max(2 n_eigs + 1, 20), eigsh's default ncv, and at least
+ *                16 blocks of min(n_eigs, 4) vectors when n_eigs > 1
  *   max_iter     total matvec budget
  *   precision    EDLGT_PREC_FP64, or EDLGT_PREC_MIXED (FP32 SpMV, then FP64
  *                refinement restarted from the FP32 Ritz vectors)
