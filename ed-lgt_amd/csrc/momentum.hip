@@ -13,9 +13,12 @@
 // images accumulated in box order, amplitudes |a| <= 1e-14 dropped for k != 0
 // (a column that cancels completely stays, empty), rows sorted.
 //
-// Device: the translation table (one lookup per configuration and shift) and
-// the projected matrix.  Host: the orbit scan (a few operations per
-// configuration and shift over the table).
+// Device: the translation table (one lookup per configuration and shift), the
+// orbit scan of standard translations (every state reads its own table row:
+// representative = row minimum, rank in the orbit, Bloch sum over the shifts
+// that reach it; same columns as the box sum, since the phases are periodic
+// on the period lattice) and the projected matrix.  Host: the orbit scan of
+// the TC translations of the 1D chain (sign per representative).
 //
 // Projected matrix, one 64-lane wave per k-row c: for every member r of
 // column c (amplitude b_r) and every stored entry (j, h) of H's row r whose
@@ -258,6 +261,115 @@ __global__ void apply_b(int64_t n_rows, int64_t n_cols, const int64_t* __restric
   }
 }
 
+// Translation group geometry for the device orbit scan: shift s is decoded
+// row-major over the axis counts R (s_d = (s / sstride_d) % R_d), kd = k mod R.
+struct OrbitGeo {
+  int n_dims, S, max_r;
+  int r[3], sstride[3], kd[3];
+};
+
+__device__ inline int32_t row_min(const int32_t* __restrict__ row, int S) {
+  int32_t mn = row[0];
+  for (int s = 1; s < S; ++s) mn = min(mn, row[s]);
+  return mn;
+}
+
+// exp(-2 pi i k.s / R) == 1 for shift s, tested exactly on the integers
+// sum_d kd_d s_d (S / R_d) mod S
+__device__ inline bool k_trivial(const OrbitGeo& g, int s) {
+  int64_t num = 0;
+  for (int d = 0; d < g.n_dims; ++d)
+    num += (int64_t)g.kd[d] * ((s / g.sstride[d]) % g.r[d]) * (g.S / g.r[d]);
+  return num % g.S == 0;
+}
+
+// Per state j (one thread, its translation-table row): whether j is the
+// smallest index of its orbit and the orbit is kept ((k_d p_d) % R_d == 0 with
+// p_d the axis periods), and the stored entries of its column: the orbit size
+// S / |H| (H = shifts fixing j), or 0 when k is not trivial on H (then every
+// Bloch sum of the orbit cancels and the column stays, empty).
+__global__ void orbit_scan(const int32_t* __restrict__ trans, int64_t n, OrbitGeo g, int32_t* __restrict__ keep_rep,
+                           int32_t* __restrict__ n_members) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* row = trans + j * g.S;
+    int32_t flag = 0, cnt = 0;
+    if (row_min(row, g.S) == (int32_t)j) {
+      bool keep = true;
+      for (int d = 0; d < g.n_dims; ++d) {
+        int p = g.r[d];
+        for (int q = 1; q <= g.r[d]; ++q)
+          if (row[(q % g.r[d]) * g.sstride[d]] == (int32_t)j) {
+            p = q;
+            break;
+          }
+        if ((g.kd[d] * p) % g.r[d] != 0) keep = false;
+      }
+      if (keep) {
+        flag = 1;
+        int h = 0;
+        bool triv = true;
+        for (int s = 0; s < g.S; ++s)
+          if (row[s] == (int32_t)j) {
+            ++h;
+            triv = triv && k_trivial(g, s);
+          }
+        cnt = triv ? g.S / h : 0;
+      }
+    }
+    keep_rep[j] = flag;
+    n_members[j] = cnt;
+  }
+}
+
+// Per state j with representative m (its row's minimum): the column of m
+// (col_of = exclusive scan of keep_rep), the rank of j in the orbit
+// (#{s : T_s j < j} / |H|: the row lists every member |H| times) and the
+// amplitude sum over {s : T_s j = m} of exp(+2 pi i k.s / R) (the shifts
+// t = -s carrying m to j), normalised by sqrt(S |H|) (S / |H| members of
+// modulus |H|).  ptab[d][q] = exp(-2 pi i kd_d q / R_d).
+__global__ void orbit_fill(const int32_t* __restrict__ trans, int64_t n, OrbitGeo g, const double2* __restrict__ ptab,
+                           const int32_t* __restrict__ keep_rep, const int32_t* __restrict__ n_members,
+                           const int64_t* __restrict__ col_of, const int64_t* __restrict__ first,
+                           int64_t* __restrict__ col_ptr, int32_t* __restrict__ row_idx, double2* __restrict__ data,
+                           int32_t* __restrict__ row_col, double2* __restrict__ row_val) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* row = trans + j * g.S;
+    const int32_t m = row_min(row, g.S);
+    int32_t c = -1;
+    double2 v = make_double2(0.0, 0.0);
+    if (keep_rep[m]) {
+      const int64_t col = col_of[m];
+      if (m == (int32_t)j) col_ptr[col] = first[m];
+      if (n_members[m] > 0) {
+        int h = 0, less = 0;
+        double2 amp = make_double2(0.0, 0.0);
+        for (int s = 0; s < g.S; ++s) {
+          const int32_t x = row[s];
+          h += x == (int32_t)j;
+          less += x < (int32_t)j;
+          if (x == m) {
+            double2 ph = make_double2(1.0, 0.0);
+            for (int d = 0; d < g.n_dims; ++d) {
+              const int sd = (s / g.sstride[d]) % g.r[d];
+              ph = cm(ph, ptab[d * g.max_r + (g.r[d] - sd) % g.r[d]]);
+            }
+            amp.x += ph.x;
+            amp.y += ph.y;
+          }
+        }
+        const double inv = 1.0 / sqrt((double)g.S * (double)h);
+        v = make_double2(amp.x * inv, amp.y * inv);
+        const int64_t o = first[m] + less / h;
+        row_idx[o] = (int32_t)j;
+        data[o] = v;
+        c = (int32_t)col;
+      }
+    }
+    row_col[j] = c;
+    row_val[j] = v;
+  }
+}
+
 int check_flag(const DevBuf& err, hipStream_t st, int32_t* out) {
   EDLGT_HIP(hipMemcpyAsync(out, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   EDLGT_HIP(hipStreamSynchronize(st));
@@ -403,6 +515,76 @@ static int orbit_columns(const std::vector<int32_t>& trans, int64_t N, int64_t S
   return EDLGT_OK;
 }
 
+// Device orbit scan over the device translation table (standard
+// translations): the same columns as orbit_columns, each state placed by its
+// own table row (representative, rank, Bloch sum), so the table never leaves
+// the device.  Two passes: per-state flags and column sizes, scanned into
+// column numbers and column starts, then every state writes its entry.
+static int orbit_columns_device(const DevBuf& trans, int64_t N, int64_t S, const std::vector<int64_t>& R,
+                                const int32_t* k_vals, hipStream_t st, Momentum* out) {
+  OrbitGeo g{};
+  g.n_dims = (int)R.size();
+  g.S = (int)S;
+  g.max_r = (int)*std::max_element(R.begin(), R.end());
+  bool finite = false;
+  for (int d = g.n_dims - 1, stride = 1; d >= 0; --d) {
+    g.r[d] = (int)R[d];
+    g.sstride[d] = stride;
+    stride *= g.r[d];
+    g.kd[d] = (int)(((k_vals[d] % R[d]) + R[d]) % R[d]);
+    finite = finite || k_vals[d] != 0;
+  }
+  // per-axis phase tables by repeated multiplication, as orbit_columns builds them
+  using cd = std::complex<double>;
+  std::vector<double2> ptab((size_t)g.n_dims * g.max_r, make_double2(1.0, 0.0));
+  for (int d = 0; d < g.n_dims; ++d) {
+    const cd step = std::exp(cd(0.0, -2.0 * M_PI * (double)g.kd[d] / (double)g.r[d]));
+    cd v(1.0, 0.0);
+    for (int q = 1; q < g.r[d]; ++q) {
+      v *= step;
+      ptab[(size_t)d * g.max_r + q] = make_double2(v.real(), v.imag());
+    }
+  }
+  DevBuf b_ptab, b_keep, b_cnt, b_col, b_first;
+  EDLGT_TRY(upload(b_ptab, ptab.data(), ptab.size(), st));
+  EDLGT_TRY(b_keep.alloc((size_t)std::max<int64_t>(N, 1) * sizeof(int32_t)));
+  EDLGT_TRY(b_cnt.alloc((size_t)std::max<int64_t>(N, 1) * sizeof(int32_t)));
+  EDLGT_TRY(b_col.alloc((size_t)(N + 1) * sizeof(int64_t)));
+  EDLGT_TRY(b_first.alloc((size_t)(N + 1) * sizeof(int64_t)));
+  const unsigned grid = grid_for(N, 256, 65536);
+  if (N > 0) {
+    hipLaunchKernelGGL(orbit_scan, dim3(grid), dim3(256), 0, st, trans.as<int32_t>(), N, g, b_keep.as<int32_t>(),
+                       b_cnt.as<int32_t>());
+    EDLGT_HIP(hipGetLastError());
+  }
+  EDLGT_TRY(exclusive_scan_i32_i64(b_keep.as<int32_t>(), b_col.as<int64_t>(), N, st));
+  EDLGT_TRY(exclusive_scan_i32_i64(b_cnt.as<int32_t>(), b_first.as<int64_t>(), N, st));
+  int64_t tot[2] = {0, 0};
+  EDLGT_HIP(hipMemcpyAsync(&tot[0], b_col.as<int64_t>() + N, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  EDLGT_HIP(hipMemcpyAsync(&tot[1], b_first.as<int64_t>() + N, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  EDLGT_HIP(hipStreamSynchronize(st));
+  const int64_t n_cols = tot[0], nnz = tot[1];
+  EDLGT_TRY(out->col_ptr.alloc((size_t)(n_cols + 1) * sizeof(int64_t)));
+  EDLGT_TRY(out->row_idx.alloc((size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  EDLGT_TRY(out->data.alloc((size_t)std::max<int64_t>(nnz, 1) * sizeof(double2)));
+  EDLGT_TRY(out->row_col.alloc((size_t)std::max<int64_t>(N, 1) * sizeof(int32_t)));
+  EDLGT_TRY(out->row_val.alloc((size_t)std::max<int64_t>(N, 1) * sizeof(double2)));
+  EDLGT_HIP(hipMemcpyAsync(out->col_ptr.as<int64_t>() + n_cols, &tot[1], sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (N > 0) {
+    hipLaunchKernelGGL(orbit_fill, dim3(grid), dim3(256), 0, st, trans.as<int32_t>(), N, g, b_ptab.as<double2>(),
+                       b_keep.as<int32_t>(), b_cnt.as<int32_t>(), b_col.as<int64_t>(), b_first.as<int64_t>(),
+                       out->col_ptr.as<int64_t>(), out->row_idx.as<int32_t>(), out->data.as<double2>(),
+                       out->row_col.as<int32_t>(), out->row_val.as<double2>());
+    EDLGT_HIP(hipGetLastError());
+  }
+  EDLGT_HIP(hipStreamSynchronize(st));
+  out->n_rows = N;
+  out->n_cols = n_cols;
+  out->nnz = nnz;
+  out->is_complex = finite ? 1 : 0;
+  return EDLGT_OK;
+}
+
 int momentum_build(const Basis& B, int n_dims, const int32_t* lvals, const int32_t* unit_cell,
                    const int32_t* k_vals, hipStream_t st, Momentum* out) {
   if (n_dims < 1 || n_dims > 3) return set_error(EDLGT_ERR_INVALID_ARG, "n_dims must be 1..3");
@@ -460,14 +642,10 @@ int momentum_build(const Basis& B, int n_dims, const int32_t* lvals, const int32
                        B.cfg_stride, (int)n_sites, B.keys.as<uint64_t>(), N, b_str.as<uint64_t>(),
                        b_src.as<int32_t>(), (int)S, b_trans.as<int32_t>(), b_err.as<int32_t>());
   EDLGT_HIP(hipGetLastError());
-  std::vector<int32_t> trans((size_t)N * S);
-  if (N > 0)
-    EDLGT_HIP(hipMemcpyAsync(trans.data(), b_trans.p, trans.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   int32_t flag = 0;
   EDLGT_TRY(check_flag(b_err, st, &flag));
   if (flag) return set_error(EDLGT_ERR_KEY_NOT_FOUND, "the sector is not closed under the translations");
-  b_trans.release();
-  return orbit_columns(trans, N, S, R, k_vals, nullptr, st, out);
+  return orbit_columns_device(b_trans, N, S, R, k_vals, st, out);
 }
 
 int momentum_build_tc(const Basis& B, int d_loc, const int32_t* c_map, const double* c_sign, int32_t k,
