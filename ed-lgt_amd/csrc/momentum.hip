@@ -378,40 +378,17 @@ int check_flag(const DevBuf& err, hipStream_t st, int32_t* out) {
 
 }  // namespace
 
-// Host orbit scan over the translation table trans[N][S] (shift s decoded
-// row-major over the axis counts R): one column per orbit whose smallest
-// index is i and whose axis periods p satisfy (k_d p_d) % R_d == 0, the
-// period box summed with its Bloch phases, equal images accumulated in box
-// order.  Standard translations (tc_sign == nullptr): phases from per-axis
-// tables built by repeated multiplication, no column dropped for k = 0.  TC
-// translations (1D, tc_sign[i] = C sign of configuration i): odd shifts carry
-// the reference's sign, phases exp(-2 pi i k t / L) evaluated directly, and a
-// k = 0 column whose norm^2 is within 1e-8 of zero stays empty.
-static int orbit_columns(const std::vector<int32_t>& trans, int64_t N, int64_t S,
-                         const std::vector<int64_t>& R, const int32_t* k_vals, const double* tc_sign, hipStream_t st, Momentum* out) {
-  const int n_dims = (int)R.size();
-  std::vector<int64_t> sstride(n_dims);
-  {
-    int64_t st_ = 1;
-    for (int d = n_dims - 1; d >= 0; --d) {
-      sstride[d] = st_;
-      st_ *= R[d];
-    }
-  }
-  bool finite = false;
-  std::vector<int64_t> kd(n_dims);
-  for (int d = 0; d < n_dims; ++d) {
-    kd[d] = ((k_vals[d] % R[d]) + R[d]) % R[d];
-    finite = finite || k_vals[d] != 0;
-  }
+// Host orbit scan of the TC translations of a 1D chain of L sites over the
+// table trans[N][L] (shift t = X^t): one column per orbit whose smallest index
+// is i and whose period p satisfies (k p) % L == 0, the p images summed with
+// the phases exp(-2 pi i k t / L) (evaluated directly) times tc_sign[i] (the C
+// sign of configuration i) on odd t, equal images accumulated in shift order;
+// a k = 0 column whose norm^2 is within 1e-8 of zero stays empty.
+static int orbit_columns_tc(const std::vector<int32_t>& trans, int64_t N, int64_t L, int32_t k,
+                            const double* tc_sign, hipStream_t st, Momentum* out) {
+  const int64_t kd = ((k % L) + L) % L;
+  const bool finite = k != 0;
   using cd = std::complex<double>;
-  int64_t maxR = *std::max_element(R.begin(), R.end());
-  std::vector<cd> ptab((size_t)n_dims * maxR, cd(1.0, 0.0));
-  for (int d = 0; d < n_dims; ++d) {
-    if (R[d] <= 1) continue;
-    const cd step = std::exp(cd(0.0, -2.0 * M_PI * (double)kd[d] / (double)R[d]));
-    for (int64_t q = 1; q < R[d]; ++q) ptab[(size_t)d * maxR + q] = ptab[(size_t)d * maxR + q - 1] * step;
-  }
   std::vector<int64_t> col_ptr(1, 0);
   std::vector<int32_t> row_idx;
   std::vector<double2> data;
@@ -419,49 +396,24 @@ static int orbit_columns(const std::vector<int32_t>& trans, int64_t N, int64_t S
   std::vector<double2> row_val((size_t)N, make_double2(0.0, 0.0));
   std::vector<int32_t> img;
   std::vector<cd> amp;
-  std::vector<int64_t> p(n_dims), tl(n_dims);
   for (int64_t i = 0; i < N; ++i) {
-    const int32_t* row = trans.data() + (size_t)i * S;
+    const int32_t* row = trans.data() + (size_t)i * L;
     int32_t mn = row[0];
-    for (int64_t s = 1; s < S; ++s) mn = std::min(mn, row[s]);
+    for (int64_t s = 1; s < L; ++s) mn = std::min(mn, row[s]);
     if (mn != (int32_t)i) continue;  // not the orbit's smallest index
-    for (int d = 0; d < n_dims; ++d) {
-      p[d] = R[d];
-      for (int64_t q = 1; q <= R[d]; ++q)
-        if (row[(q % R[d]) * sstride[d]] == (int32_t)i) {
-          p[d] = q;
-          break;
-        }
-    }
-    bool keep = true;
-    for (int d = 0; d < n_dims; ++d)
-      if ((k_vals[d] * p[d]) % R[d] != 0) keep = false;
-    if (!keep) continue;
+    int64_t p = L;
+    for (int64_t q = 1; q <= L; ++q)
+      if (row[q % L] == (int32_t)i) {
+        p = q;
+        break;
+      }
+    if ((k * p) % L != 0) continue;
     img.clear();
     amp.clear();
-    int64_t box = 1;
-    for (int d = 0; d < n_dims; ++d) box *= p[d];
-    for (int64_t f = 0; f < box; ++f) {
-      int64_t q = f, flat = 0;
-      for (int d = n_dims - 1; d >= 0; --d) {
-        tl[d] = q % p[d];
-        q /= p[d];
-      }
+    for (int64_t flat = 0; flat < p; ++flat) {
       cd ph(1.0, 0.0);
-      if (tc_sign) {
-        double arg = 0.0;
-        for (int d = 0; d < n_dims; ++d) {
-          flat += tl[d] * sstride[d];
-          arg += (double)(kd[d] * tl[d]) / (double)R[d];
-        }
-        if (finite) ph = std::exp(cd(0.0, -1.0) * (2.0 * M_PI * arg));
-        if (flat & 1) ph *= tc_sign[i];
-      } else {
-        for (int d = 0; d < n_dims; ++d) {
-          flat += tl[d] * sstride[d];
-          if (finite) ph *= ptab[(size_t)d * maxR + tl[d]];
-        }
-      }
+      if (finite) ph = std::exp(cd(0.0, -1.0) * (2.0 * M_PI * ((double)(kd * flat) / (double)L)));
+      if (flat & 1) ph *= tc_sign[i];
       const int32_t im = row[flat];
       size_t u = 0;
       while (u < img.size() && img[u] != im) ++u;
@@ -484,7 +436,7 @@ static int orbit_columns(const std::vector<int32_t>& trans, int64_t N, int64_t S
     img.resize(kept);
     amp.resize(kept);
     const int32_t col = (int32_t)(col_ptr.size() - 1);
-    const bool empty = finite ? (nrm <= kTolColNorm || kept == 0) : (tc_sign && std::fabs(nrm) <= 1e-8);
+    const bool empty = finite ? (nrm <= kTolColNorm || kept == 0) : (std::fabs(nrm) <= 1e-8);
     if (!empty) {
       std::vector<size_t> ord(kept);
       for (size_t u = 0; u < kept; ++u) ord[u] = u;
@@ -516,7 +468,7 @@ static int orbit_columns(const std::vector<int32_t>& trans, int64_t N, int64_t S
 }
 
 // Device orbit scan over the device translation table (standard
-// translations): the same columns as orbit_columns, each state placed by its
+// translations): the same columns as the reference's period-box sum, each state placed by its
 // own table row (representative, rank, Bloch sum), so the table never leaves
 // the device.  Two passes: per-state flags and column sizes, scanned into
 // column numbers and column starts, then every state writes its entry.
@@ -534,7 +486,7 @@ static int orbit_columns_device(const DevBuf& trans, int64_t N, int64_t S, const
     g.kd[d] = (int)(((k_vals[d] % R[d]) + R[d]) % R[d]);
     finite = finite || k_vals[d] != 0;
   }
-  // per-axis phase tables by repeated multiplication, as orbit_columns builds them
+  // per-axis phase tables by repeated multiplication (the reference's phase table)
   using cd = std::complex<double>;
   std::vector<double2> ptab((size_t)g.n_dims * g.max_r, make_double2(1.0, 0.0));
   for (int d = 0; d < g.n_dims; ++d) {
@@ -691,8 +643,7 @@ int momentum_build_tc(const Basis& B, int d_loc, const int32_t* c_map, const dou
   int32_t flag = 0;
   EDLGT_TRY(check_flag(b_err, st, &flag));
   if (flag) return set_error(EDLGT_ERR_KEY_NOT_FOUND, "the sector is not closed under the TC translations");
-  std::vector<int64_t> R(1, L);
-  return orbit_columns(trans, N, L, R, &k, sign.data(), st, out);
+  return orbit_columns_tc(trans, N, L, k, sign.data(), st, out);
 }
 
 int momentum_copy(const Momentum& m, int64_t* col_ptr, int32_t* row_idx, double* data) {
