@@ -505,11 +505,13 @@ Dense matmul(const Dense& A, const Dense& B) {
   return C;
 }
 
-// Page-locked host memory (hipHostMalloc) for the small device-to-host reads
-// the solver waits on: a read into it takes about half the time of one into
-// pageable memory.
+// Page-locked, coherent host memory (hipHostMalloc) that the Gram reduction
+// kernels write directly (device address `dev`): the host reads a few KB of
+// Gram entries after a stream synchronisation with no copy in between
+// (device stores to coherent host memory bypass the GPU caches).
 struct HostPinned {
   void* p = nullptr;
+  void* dev = nullptr;
   size_t bytes = 0;
   HostPinned() = default;
   HostPinned(const HostPinned&) = delete;
@@ -519,16 +521,22 @@ struct HostPinned {
   }
   int alloc(size_t n) {
     if (p) (void)hipHostFree(p);
-    p = nullptr;
+    p = dev = nullptr;
     bytes = 0;
-    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&p, n, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
       p = nullptr;
       return set_error(EDLGT_ERR_OUT_OF_MEMORY, "hipHostMalloc of " + std::to_string(n) + " bytes failed");
+    }
+    if (hipHostGetDevicePointer(&dev, p, 0) != hipSuccess || !dev) {
+      (void)hipHostFree(p);
+      p = dev = nullptr;
+      return set_error(EDLGT_ERR_HIP, "hipHostGetDevicePointer failed on pinned Gram memory");
     }
     bytes = n;
     return EDLGT_OK;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
+  template <class T> T* dev_as() const { return static_cast<T*>(dev); }
 };
 
 template <class S>
@@ -539,8 +547,8 @@ struct Solver {
   int64_t N;
   int b;       // block size
   int M;       // basis capacity in vectors (multiple of b)
-  DevBuf V, W, Tmp, part, gdev, cdev, shadow, pack;
-  HostPinned hgram;  // host copy of gdev
+  DevBuf V, W, Tmp, part, cdev, shadow, pack;
+  HostPinned hgram;  // Gram entries, written by gram_finish
   int64_t mv64 = 0, mv32 = 0;
   int64_t ortho_single = 0, ortho_double = 0;  // orthogonalisations with one / two passes
   int64_t ortho_skipped = 0;  // full passes whose update was skipped (loss below kSkipLoss)
@@ -558,7 +566,6 @@ struct Solver {
     if (need_w) EDLGT_TRY(W.alloc((size_t)b * N * sizeof(S)));
     if (need_tmp) EDLGT_TRY(Tmp.alloc((size_t)std::max(M + b, 8) * N * sizeof(S)));
     EDLGT_TRY(part.alloc((size_t)kGramBlocks * kGramCols * 8 * sizeof(double2)));
-    EDLGT_TRY(gdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
     EDLGT_TRY(hgram.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
     EDLGT_TRY(cdev.alloc((size_t)(M + b) * std::max(M, 8) * sizeof(double2)));
     if (precision == EDLGT_PREC_MIXED) {
@@ -584,24 +591,22 @@ struct Solver {
     hipLaunchKernelGGL((gram_partial<S, NWC>), dim3(nblk), dim3(kTB), 0, st, Vc, N, nc, X, N, nw, N, \
                        part.as<double2>());                                                          \
     hipLaunchKernelGGL((gram_finish<NWC>), dim3(fin), dim3(kTB), 0, st, part.as<double2>(), nblk, nc, \
-                       nw, j0, gdev.as<double2>());                                                  \
+                       nw, j0, hgram.dev_as<double2>());                                                  \
     break;
       switch (nwp) { GRAM_CASE(1) GRAM_CASE(2) GRAM_CASE(4) GRAM_CASE(8) }
 #undef GRAM_CASE
       EDLGT_HIP(hipGetLastError());
     }
     const double2* hbuf = hgram.as<double2>();
-    EDLGT_HIP(hipMemcpyAsync(hgram.p, gdev.p, (size_t)nv * nw * sizeof(double2), hipMemcpyDeviceToHost, st));
     EDLGT_HIP(hipStreamSynchronize(st));
     for (int j = 0; j < nv; ++j)
       for (int c = 0; c < nw; ++c) G(j, c) = cd(hbuf[(size_t)j * nw + c].x, hbuf[(size_t)j * nw + c].y);
     return EDLGT_OK;
   }
 
-  // Reads back a Gram matrix reduced by gram_finish into gdev.
+  // Reads a Gram matrix that gram_finish wrote into hgram.
   int fetch_gram(int nw, Dense& G) {
     const double2* hbuf = hgram.as<double2>();
-    EDLGT_HIP(hipMemcpyAsync(hgram.p, gdev.p, (size_t)nw * nw * sizeof(double2), hipMemcpyDeviceToHost, st));
     EDLGT_HIP(hipStreamSynchronize(st));
     G = Dense(nw, nw);
     for (int j = 0; j < nw; ++j)
@@ -642,7 +647,7 @@ struct Solver {
     hipLaunchKernelGGL((tall_update<S, NWC, true>), dim3(g), dim3(kTB), shmem, st, Vp, N, nv,        \
                        cdev.as<double2>(), nw, X, N, nw, N, beta, part.as<double2>());              \
     hipLaunchKernelGGL((gram_finish<NWC>), dim3((NWC * NWC + 3) / 4), dim3(kTB), 0, st,              \
-                       part.as<double2>(), (int)g, NWC, nw, 0, gdev.as<double2>());                 \
+                       part.as<double2>(), (int)g, NWC, nw, 0, hgram.dev_as<double2>());                 \
     break;
       switch (nwp) { UPDG_CASE(1) UPDG_CASE(2) UPDG_CASE(4) }
 #undef UPDG_CASE
@@ -675,7 +680,7 @@ struct Solver {
       hipLaunchKernelGGL((row_transform<S, NWC, true>), dim3(g), dim3(kTB), 0, st, X, N, nw,         \
                          cdev.as<double2>(), N, part.as<double2>());                                \
       hipLaunchKernelGGL((gram_finish<NWC>), dim3((NWC * NWC + 3) / 4), dim3(kTB), 0, st,            \
-                         part.as<double2>(), (int)g, NWC, nw, 0, gdev.as<double2>());               \
+                         part.as<double2>(), (int)g, NWC, nw, 0, hgram.dev_as<double2>());               \
     } else {                                                                                        \
       hipLaunchKernelGGL((row_transform<S, NWC, false>), dim3(g), dim3(kTB), 0, st, X, N, nw,        \
                          cdev.as<double2>(), N, (double2*)nullptr);                                 \
@@ -694,7 +699,7 @@ struct Solver {
     const unsigned g = grid_for(N, kTB, 4096);
     hipLaunchKernelGGL(three_term<S>, dim3(g), dim3(kTB), 0, st, a, bv, w, c0, c1, c2, N, part.as<double2>());
     hipLaunchKernelGGL((gram_finish<1>), dim3(1), dim3(kTB), 0, st, part.as<double2>(), (int)g, 1, 1, 0,
-                       gdev.as<double2>());
+                       hgram.dev_as<double2>());
     EDLGT_HIP(hipGetLastError());
     Dense G;
     EDLGT_TRY(fetch_gram(1, G));
