@@ -452,7 +452,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3", "cpu_qed4x3", "momentum", "matfree"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="SpMV line only: no Lanczos solves, coupling sweep or serving leg")
     ap.add_argument("--sites", type=int, default=20)
     ap.add_argument("--dyn-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=512)
@@ -581,10 +582,11 @@ def main():
     del model, H, x, y, y_ref, x32
     torch.cuda.empty_cache()
 
-    serving = None
-    progress("QED 4x2 coupling sweep")
-    sweep = coupling_sweep()
-    if not args.no_serve:
+    serving = sweep = None
+    if not args.no_solve:  # the coupling sweep and the served stream are solves too
+        progress("QED 4x2 coupling sweep")
+        sweep = coupling_sweep()
+    if not args.no_serve and not args.no_solve:
         progress("serving leg")
         serving = serve_leg(args)
 
