@@ -174,61 +174,55 @@ class QMB_hamiltonian:
             for e in evals:
                 logger.info(f"{e:.10f}  {e / nsites:.10f}")
 
+    def time_evolution(self, initial_state, time_line, loc_dims, basis=None, d_max=None, m=30,
+                       reorth=True, precision=L.PREC_FP64, store_states=True):
+        """exp(-iHt) psi0 on a uniform time grid by device Krylov stepping
+        (replaces expm_multiply in QMB_hamiltonian.time_evolution,
+        qmb_hamiltonian.py:334-426).  psi_time[k] = exp(-i H t_k) psi0.  The
+        stepping state (m+1 basis vectors) stays on the device for the next call
+        with the same matrix and settings; release_krylov() frees it."""
+        import torch
+        from .device import KrylovSession
+        time_line = np.asarray(time_line, dtype=float)
+        if time_line.size < 2:
+            raise ValueError("time_line needs at least two points")
+        dt = float(time_line[1] - time_line[0])
+        if not np.allclose(np.diff(time_line), dt, rtol=1e-10, atol=1e-12):
+            raise ValueError("time_line must be uniformly spaced")
+        self.loc_dims = loc_dims
+        N = self.shape[0]
+        if hasattr(initial_state, "data_ptr"):
+            psi = initial_state.to(device="cuda", dtype=torch.complex128).clone()
+        else:
+            psi = torch.as_tensor(np.asarray(initial_state, dtype=np.complex128), device="cuda").clone()
+        if psi.numel() != N:
+            raise ValueError("initial_state has the wrong length")
+        dm = int(d_max or 0) if basis is not None else 0
+        key = (id(self.Ham), None if basis is None else id(basis), int(precision), int(m), bool(reorth), dm)
+        ks = getattr(self, "_krylov", None)
+        if ks is None or ks.key != key or ks.csr is not self.Ham:
+            self.release_krylov()
+            ks = self._krylov = KrylovSession(self.Ham, basis, precision, m, reorth, dm)
+        if abs(time_line[0]) > 0:
+            ks.run(psi, float(time_line[0]), 1)
+        n_steps = time_line.size - 1
+        hist = torch.empty((n_steps + 1, N), dtype=torch.complex128, device="cuda") if store_states else None
+        marg = None
+        if basis is not None:
+            marg = np.zeros((n_steps + 1, basis.n_sites, d_max))
+        err = np.zeros(max(n_steps, 1))
+        ks.run(psi, dt, n_steps, marg, err, hist)
+        self.krylov_err = err[:n_steps]
+        self.marginals = marg
+        self.psi_final = psi
+        self.psi_time = [QMB_state(hist[k], self.lvals, loc_dims) for k in range(n_steps + 1)] if store_states else None
 
-def _time_evolution(self, initial_state, time_line, loc_dims, basis=None, d_max=None, m=30,
-                    reorth=True, precision=L.PREC_FP64, store_states=True):
-    """exp(-iHt) psi0 on a uniform time grid by device Krylov stepping
-    (replaces expm_multiply in QMB_hamiltonian.time_evolution,
-    qmb_hamiltonian.py:334-426).  psi_time[k] = exp(-i H t_k) psi0.  The
-    stepping state (m+1 basis vectors) stays on the device for the next call
-    with the same matrix and settings; release_krylov() frees it."""
-    import torch
-    from .device import KrylovSession
-    time_line = np.asarray(time_line, dtype=float)
-    if time_line.size < 2:
-        raise ValueError("time_line needs at least two points")
-    dt = float(time_line[1] - time_line[0])
-    if not np.allclose(np.diff(time_line), dt, rtol=1e-10, atol=1e-12):
-        raise ValueError("time_line must be uniformly spaced")
-    self.loc_dims = loc_dims
-    N = self.shape[0]
-    if hasattr(initial_state, "data_ptr"):
-        psi = initial_state.to(device="cuda", dtype=torch.complex128).clone()
-    else:
-        psi = torch.as_tensor(np.asarray(initial_state, dtype=np.complex128), device="cuda").clone()
-    if psi.numel() != N:
-        raise ValueError("initial_state has the wrong length")
-    dm = int(d_max or 0) if basis is not None else 0
-    key = (id(self.Ham), None if basis is None else id(basis), int(precision), int(m), bool(reorth), dm)
-    ks = getattr(self, "_krylov", None)
-    if ks is None or ks.key != key or ks.csr is not self.Ham:
-        self.release_krylov()
-        ks = self._krylov = KrylovSession(self.Ham, basis, precision, m, reorth, dm)
-    if abs(time_line[0]) > 0:
-        ks.run(psi, float(time_line[0]), 1)
-    n_steps = time_line.size - 1
-    hist = torch.empty((n_steps + 1, N), dtype=torch.complex128, device="cuda") if store_states else None
-    marg = None
-    if basis is not None:
-        marg = np.zeros((n_steps + 1, basis.n_sites, d_max))
-    err = np.zeros(max(n_steps, 1))
-    ks.run(psi, dt, n_steps, marg, err, hist)
-    self.krylov_err = err[:n_steps]
-    self.marginals = marg
-    self.psi_final = psi
-    self.psi_time = [QMB_state(hist[k], self.lvals, loc_dims) for k in range(n_steps + 1)] if store_states else None
-
-
-def _release_krylov(self):
-    """Frees the device state kept by time_evolution."""
-    ks = getattr(self, "_krylov", None)
-    if ks is not None:
-        ks.close()
-    self._krylov = None
-
-
-QMB_hamiltonian.time_evolution = _time_evolution
-QMB_hamiltonian.release_krylov = _release_krylov
+    def release_krylov(self):
+        """Frees the device state kept by time_evolution."""
+        ks = getattr(self, "_krylov", None)
+        if ks is not None:
+            ks.close()
+        self._krylov = None
 
 
 class QuantumModel:
@@ -574,8 +568,17 @@ class QED_Model(QuantumModel):
     def __init__(self, spin, pure_theory, bg_list=None, plaq_basis=False, link_symmetries=True,
                  get_only_bulk=False, **kwargs):
         super().__init__(**kwargs)
-        if bg_list is not None and any(b != 0 for b in bg_list):
-            raise NotImplementedError("background charges are not supported by the device engine")
+        # static background charges (QED_model.py:56-59, 96-101): one integer
+        # charge per site; all zero is the same as none
+        self.bg_list = None
+        self.background = 0
+        if bg_list is not None:
+            bg = np.asarray(bg_list, dtype=float)
+            if bg.shape != (self.n_sites,):
+                raise ValueError(f"len(bg_list) must match n_sites = {self.n_sites}: got {bg.shape}")
+            self.background = int(np.max(np.abs(bg)))
+            if self.background != 0:
+                self.bg_list = bg.tolist()
         if plaq_basis or not link_symmetries or get_only_bulk:
             raise NotImplementedError("only the dressed-site basis with link symmetries is supported")
         if not isinstance(spin, (int, np.integer)) or spin < 1:
@@ -584,9 +587,27 @@ class QED_Model(QuantumModel):
         self.pure_theory = bool(pure_theory)
         self.staggered_basis = not self.pure_theory
         self.link_symmetries = True
-        site = opx.qed_site(self.spin, self.pure_theory, self.dim)
-        labels = [self.lattice.label(i, self.staggered_basis) for i in range(self.n_sites)]
-        self._project(site, lambda l: opx.qed_operators(site, self.spin, self.pure_theory, self.dim, l), labels)
+        charges = [0] * self.n_sites
+        if self.bg_list is not None:
+            charges = [int(round(q)) for q in self.bg_list]
+            for i, q in enumerate(self.bg_list):
+                if abs(q - charges[i]) > 1e-12:
+                    raise ValueError(f"No GI states at site {i} with bg {q}")
+        site = opx.qed_site(self.spin, self.pure_theory, self.dim, backgrounds=set(charges))
+        labels = []
+        for i in range(self.n_sites):
+            base = self.lattice.label(i, self.staggered_basis)
+            lab = opx.bg_label(base, charges[i])
+            if lab not in site.states:
+                raise ValueError(f"No GI states at site {i} label={base} with bg {float(charges[i])}")
+            labels.append(lab)
+        with_bg = self.background > 0
+        self._project(site, lambda l: opx.qed_operators(site, self.spin, self.pure_theory, self.dim, l,
+                                                        with_bg=with_bg), labels)
+        if with_bg:
+            # the reference's per-site gauge states lead with the background value
+            self.gauge_states_per_site = [np.column_stack([np.full(v.shape[0], q), v])
+                                          for q, v in zip(charges, self.gauge_states_per_site)]
         d_max = self.ops["E_px"].shape[1]
 
         def diag(name):
@@ -664,6 +685,65 @@ class QED_Model(QuantumModel):
                     bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
                     self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
         return self._build_program(real)
+
+
+# Minimal string configurations of 3D QED (2x2x2 OBC, matter, static charges
+# -1 at site 0 and +1 at site 7) per link truncation: local-state indices,
+# one per site (QED_model.py:855-897).  Data that names states of the
+# reference's basis order, kept as the reference defines it.
+_QED3D_STRINGS = {
+    1: {"S1": [4, 9, 9, 3, 6, 0, 3, 10], "S2": [3, 9, 6, 0, 9, 3, 3, 11], "S3": [1, 7, 9, 3, 9, 2, 3, 10],
+        "S4": [4, 9, 9, 3, 7, 3, 0, 8], "S5": [1, 6, 9, 2, 9, 3, 3, 11], "S6": [3, 9, 7, 3, 9, 3, 2, 8]},
+    2: {"S1": [11, 27, 27, 9, 22, 4, 9, 29], "S2": [10, 27, 22, 4, 27, 9, 9, 30],
+        "S3": [10, 27, 23, 9, 27, 9, 8, 25], "S4": [6, 22, 27, 8, 27, 9, 9, 30],
+        "S5": [6, 23, 27, 9, 27, 8, 9, 29], "S6": [11, 27, 27, 9, 23, 9, 4, 25]},
+    3: {"S1": [20, 54, 47, 11, 54, 18, 18, 58], "S2": [21, 54, 54, 18, 47, 11, 18, 57],
+        "S3": [21, 54, 54, 18, 48, 18, 11, 51], "S4": [20, 54, 48, 18, 54, 18, 17, 51],
+        "S5": [14, 48, 54, 18, 54, 17, 18, 57], "S6": [14, 47, 54, 17, 54, 18, 18, 58]},
+}
+
+
+def _qed_overlap_QMB_state(self, name):
+    """Named reference configurations (QED_model.py:778-899), one local-state
+    index per site, for get_qmb_state_from_configs:
+      1D with matter, spin 1: "V" (bare vacuum: every link E = 0, even sites
+        empty, odd sites filled; alias "PV") and "meson_center" (the central
+        (even, odd) pair turned into a meson: the even site filled with its
+        right link at E = +1);
+      3D, 2x2x2 with static charges bg_list = [-1, 0, 0, 0, 0, 0, 0, 1],
+        spin 1, 2 or 3: the minimal flux strings "S1".."S6"."""
+    if self.dim == 1 and not self.pure_theory:
+        key = name.strip().lower().replace(" ", "_")
+        if key == "pv":
+            key = "v"
+        if key in ("v", "meson_center"):
+            if self.spin != 1:
+                raise NotImplementedError("1D labels 'V'/'meson_center' are implemented for "
+                                          "spin='integrated' or truncated spin=1.")
+            n = self.n_sites
+            # spin-1 1D states in product order over (n, E_mx, E_px): index 1
+            # of an even site is (0, 0, 0), index 3 of an odd site (1, 0, 0);
+            # on an OBC edge the kept states are those with a zero outer link
+            cfg = np.array([1 if i % 2 == 0 else 3 for i in range(n)], dtype=np.int32)
+            if all(self.has_obc):
+                cfg[0], cfg[n - 1] = 0, 1
+            if key == "meson_center":
+                even = n // 2
+                if even % 2 == 1:
+                    even -= 1
+                even = max(0, min(even, n - 2))
+                cfg[even], cfg[even + 1] = 4, 1
+            return cfg
+    if self.dim == 3 and self.bg_list == [-1, 0, 0, 0, 0, 0, 0, 1]:
+        if self.spin not in _QED3D_STRINGS:
+            raise NotImplementedError("3D string states are tabulated for spin 1, 2 and 3")
+        if name not in _QED3D_STRINGS[self.spin]:
+            raise ValueError(f"Unknown String name: {name}")
+        return np.array(_QED3D_STRINGS[self.spin][name])
+    raise NotImplementedError("Only 3D QED states are supported")
+
+
+QED_Model.overlap_QMB_state = _qed_overlap_QMB_state
 
 
 def _qed_multi_term(self, m=None):

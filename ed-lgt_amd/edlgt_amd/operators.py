@@ -97,10 +97,20 @@ def _border_parts(dim):
     return parts
 
 
-def qed_site(spin, pure_theory, dim):
-    """Gauss-law states: sum_mu (E_p - E_m) = n - (0 even | 1 odd) with matter,
-    = 0 without; border labels additionally fix the named half-links to E=0
-    (QED_operators.py:410-538, lattice_geometry.py:345-487)."""
+def bg_label(label, q):
+    """Label of a site whose static background charge is q (q = 0: the plain label)."""
+    return label if q == 0 else f"{label}@bg{int(q)}"
+
+
+def qed_site(spin, pure_theory, dim, backgrounds=(0,)):
+    """Gauss-law states: sum_mu (E_p - E_m) = n - (0 even | 1 odd) + q with
+    matter, = q without, for every static background charge q in backgrounds
+    (labels of q != 0 carry the suffix "@bg<q>"); border labels additionally
+    fix the named half-links to E=0 (QED_operators.py:410-538,
+    lattice_geometry.py:345-487).  The reference makes the background a
+    leading dof of the dressed site and keeps, per site, the columns of its
+    own charge (quantum_model.py:440-466): the kept states are these, in the
+    same product order."""
     axes = "xyz"[:dim]
     E_vals = list(range(-spin, spin + 1))
     dofs = ([] if pure_theory else [("n", 2)]) + [(f"E_m{a}", len(E_vals)) for a in axes] + \
@@ -110,25 +120,29 @@ def qed_site(spin, pure_theory, dim):
     link_pos = {f"m{a}": off + i for i, a in enumerate(axes)}
     link_pos.update({f"p{a}": off + dim + i for i, a in enumerate(axes)})
     labels = {}
-    for core, odd in cores:
-        sets = {core: ([], [])}
-        for parts in _border_parts(dim):
-            sets[core + "_" + "_".join(parts)] = ([], [])
-        ranges = [range(s) for _, s in dofs]
-        for idx in product(*ranges):
-            val = list(idx)
-            for k in range(off, len(dofs)):
-                val[k] = E_vals[idx[k]]
-            charge = 0 if pure_theory else val[0] - odd
-            div = sum(val[off + dim + m] - val[off + m] for m in range(dim))
-            if div != charge:
-                continue
-            for lab, (ii, vv) in sets.items():
-                parts = lab.split("_")[1:]
-                if all(val[link_pos[p]] == 0 for p in parts):
-                    ii.append(idx)
-                    vv.append(val)
-        labels.update(sets)
+    for q in sorted(set(int(b) for b in backgrounds)):
+        for core, odd in cores:
+            sets = {core: ([], [])}
+            for parts in _border_parts(dim):
+                sets[core + "_" + "_".join(parts)] = ([], [])
+            ranges = [range(s) for _, s in dofs]
+            for idx in product(*ranges):
+                val = list(idx)
+                for k in range(off, len(dofs)):
+                    val[k] = E_vals[idx[k]]
+                charge = (0 if pure_theory else val[0] - odd) + q
+                div = sum(val[off + dim + m] - val[off + m] for m in range(dim))
+                if div != charge:
+                    continue
+                for lab, (ii, vv) in sets.items():
+                    parts = lab.split("_")[1:]
+                    if all(val[link_pos[p]] == 0 for p in parts):
+                        ii.append(idx)
+                        vv.append(val)
+            for lab, v in sets.items():
+                if not v[0]:
+                    continue  # no state of this label carries the charge q
+                labels[bg_label(lab, q)] = v
     return DressedSite(dofs, labels)
 
 
@@ -196,8 +210,9 @@ def qed_operator_specs(pure_theory, dim):
     return specs
 
 
-def qed_operators(site, spin, pure_theory, dim, label):
-    """All QED dressed-site operators on one label's gauge-invariant states."""
+def qed_operators(site, spin, pure_theory, dim, label, with_bg=False):
+    """All QED dressed-site operators on one label's gauge-invariant states
+    (with_bg: also the background charge operator "bg")."""
     mats = rishon_matrices(spin)
     if not pure_theory:
         mats.update(matter_matrices())
@@ -211,6 +226,10 @@ def qed_operators(site, spin, pure_theory, dim, label):
         for a in "xyz"[:dim]:
             for s in "mp":
                 out[f"Q_{s}{a}"] = out[f"Q_{s}{a}_dag"].conj().T.copy()
+    if with_bg:
+        # the background charge operator (QED_operators.py:318-326): q on every state
+        q = float(label.split("@bg")[1]) if "@bg" in label else 0.0
+        out["bg"] = q * np.eye(site.dim(label))
     if dim == 3:
         out["EzC_px,py"] = 1j * (out["E_pz"] + out["E_mz"]) @ out["C_px,py"]
         out["EyC_px,pz"] = -1j * (out["E_py"] + out["E_my"]) @ out["C_px,pz"]

@@ -51,15 +51,18 @@ def encode(configs, loc_dims):
 # ----------------------------------------------------------------------------
 # projection
 # ----------------------------------------------------------------------------
-def project_operators(ops, basis, labels, dtype=np.complex128):
-    """P^T O P per site, zero-padded to d_max (quantum_model.py:400-502)."""
-    loc_dims = np.array([basis[l].shape[1] for l in labels], dtype=np.int64)
+def project_operators(ops, basis, labels, dtype=np.complex128, site_cols=None):
+    """P^T O P per site, zero-padded to d_max (quantum_model.py:400-502);
+    site_cols[j] (optional) keeps only those columns of site j's isometry (the
+    states of its background charge, quantum_model.py:440-466)."""
+    proj = [basis[l] if site_cols is None else basis[l][:, site_cols[j]] for j, l in enumerate(labels)]
+    loc_dims = np.array([P.shape[1] for P in proj], dtype=np.int64)
     dmax = int(loc_dims.max())
     out = {}
     for name, O in ops.items():
         arr = np.zeros((len(labels), dmax, dmax), dtype=dtype)
         for j, l in enumerate(labels):
-            P = basis[l]
+            P = proj[j]
             eff = (P.conj().T @ O @ P).toarray()
             arr[j, : loc_dims[j], : loc_dims[j]] = eff
         out[name] = arr
@@ -303,19 +306,27 @@ class QEDOracle:
     """QED_Model (truncated spin, ladder U, no background) restated
     (models/QED_model.py:23-233, 235-394, 719-776)."""
 
-    def __init__(self, spin, pure_theory, lvals, has_obc, enumerate_sector=True):
+    def __init__(self, spin, pure_theory, lvals, has_obc, enumerate_sector=True, bg_list=None):
         self.spin, self.pure = spin, pure_theory
         self.lvals, self.has_obc = list(lvals), list(has_obc)
         self.dim = len(lvals)
         self.n_sites = prod(lvals)
         self.dirs = "xyz"[: self.dim]
-        basis, states = qed_ops.gauge_invariant_states(spin, pure_theory, self.dim)
-        ops = qed_ops.dressed_site_operators(spin, pure_theory, self.dim)
+        # static background charges (QED_model.py:56-59, 96-101; the bg dof
+        # leads every dressed site and each site keeps its own charge's states)
+        bg = int(max(abs(q) for q in bg_list)) if bg_list is not None else 0
+        self.bg_list = list(bg_list) if bg else None
+        basis, states = qed_ops.gauge_invariant_states(spin, pure_theory, self.dim, background=bg)
+        ops = qed_ops.dressed_site_operators(spin, pure_theory, self.dim, background=bg)
         self.labels = [
             lat.site_label(lat.zig_zag(self.lvals, i), self.lvals, self.has_obc, not pure_theory)
             for i in range(self.n_sites)
         ]
-        self.ops, self.loc_dims = project_operators(ops, basis, self.labels)
+        cols = None
+        if bg:
+            cols = [np.flatnonzero(np.isclose(states[l][:, 0], q, rtol=0.0, atol=1e-12))
+                    for l, q in zip(self.labels, self.bg_list)]
+        self.ops, self.loc_dims = project_operators(ops, basis, self.labels, site_cols=cols)
         self.gauge_states = states
         # symmetry generators (QED_model.py:117-199)
         dg = lambda name: np.real(np.einsum("sii->si", self.ops[name]))
@@ -347,6 +358,30 @@ class QEDOracle:
                                           link_sectors, pairs)
         self.keys = encode(self.configs, self.loc_dims)
         self.N = self.configs.shape[0]
+
+    def label_config(self, name):
+        """Local-state indices of the 1D spin-1 benchmark product states named
+        by QED_Model.overlap_QMB_state (QED_model.py:796-846), found from their
+        physical content in this oracle's own local bases rather than from the
+        reference's index table: "V" has every link at E = 0, even sites empty
+        and odd sites filled (no net charge anywhere); "meson_center" turns the
+        central (even, odd) pair into a meson (even site filled with its right
+        link at E = +1, odd site empty with its left link at E = +1)."""
+        if self.dim != 1 or self.pure:
+            raise NotImplementedError("1D QED with matter only")
+
+        def index(site, n, e_m, e_p):
+            st = self.gauge_states[self.labels[site]]
+            hit = np.flatnonzero((st[:, 0] == n) & (st[:, 1] == e_m) & (st[:, 2] == e_p))
+            assert hit.size == 1
+            return int(hit[0])
+
+        cfg = [index(i, i % 2, 0, 0) for i in range(self.n_sites)]
+        if name == "meson_center":
+            e = self.n_sites // 2
+            e -= e % 2
+            cfg[e], cfg[e + 1] = index(e, 1, 0, 1), index(e + 1, 0, 1, 0)
+        return np.array(cfg)
 
     def couplings(self, g, m=None, theta=0.0):
         """QED_Hamiltonian_couplings (QED_model.py:719-776)."""

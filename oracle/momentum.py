@@ -10,7 +10,9 @@ configuration (build_all_translations, :568-618), one reference per orbit
 (1/sqrt(norm)) sum over the period box of exp(-2 pi i k.t / R) T(t)|ref>, equal
 images accumulated, amplitudes |a| <= 1e-14 dropped for k != 0, rows sorted
 (momentum_basis_zero_k / _finite_k, :698-950, helpers :220-413).
-TC_symmetry is not restated.
+The translation-combined (TC_symmetry) variant of the 1D SU(2) chain is
+restated in momentum_basis_tc (build_TC_translations :111-128,
+momentum_basis_zero_k_TC / _finite_k_TC :457-505, 953-1306).
 """
 import numpy as np
 import scipy.sparse as sp

@@ -2,9 +2,10 @@
 
 Restates edlgt/operators/QED_operators.py:26-538 and
 edlgt/operators/bose_fermi_operators.py:14-70 with scipy Kronecker products,
-exactly in the reference's factor order (matter first, then the half-links
-m_x, m_y, ..., p_x, p_y, ...).  Only the ``U="ladder"``, fermionic,
-no-background branch used by the hot-path configurations is restated.
+exactly in the reference's factor order (static background charge first when
+present, then matter, then the half-links m_x, m_y, ..., p_x, p_y, ...).
+Only the ``U="ladder"``, fermionic branch used by the hot-path
+configurations is restated.
 """
 from itertools import product
 
@@ -58,8 +59,11 @@ def fermi_operators():
     return ops
 
 
-def dressed_site_operators(spin, pure_theory, lattice_dim):
-    """QED_dressed_site_operators for 1D/2D/3D (QED_operators.py:120-330)."""
+def dressed_site_operators(spin, pure_theory, lattice_dim, background=0):
+    """QED_dressed_site_operators for 1D/2D/3D (QED_operators.py:120-330);
+    background > 0 prepends the static-charge dof (values -background..
+    background) to every operator as an identity factor and adds its diagonal
+    operator "bg" (QED_operators.py:318-326)."""
     r = rishon_operators(spin)
     in_ops = dict(r)
     if not pure_theory:
@@ -148,15 +152,25 @@ def dressed_site_operators(spin, pure_theory, lattice_dim):
         for s in "mp":
             E2 = E2 + 0.5 * ops[f"E2_{s}{d}"]
     ops["E2"] = csr_matrix(E2)
+    if background > 0:
+        bg_vals = np.arange(-background, background + 1, dtype=float)
+        I_bg = identity(bg_vals.size, dtype=float)
+        for k in list(ops):
+            ops[k] = csr_matrix(kron(I_bg, ops[k]))
+        D = ops["E2"].shape[0] // bg_vals.size
+        ops["bg"] = csr_matrix(kron(diags(bg_vals, 0), identity(D, dtype=float)))
     return ops
 
 
-def gauge_invariant_states(spin, pure_theory, lattice_dim):
-    """QED_gauge_invariant_states without background (QED_operators.py:410-538).
+def gauge_invariant_states(spin, pure_theory, lattice_dim, background=0):
+    """QED_gauge_invariant_states (QED_operators.py:410-538).
 
     Returns (basis, states): basis[label] is the (D_full, d_loc) 0/1 isometry,
     states[label] the (d_loc, n_dof) local configurations, in itertools.product
-    order over (n, E_m..., E_p...) with E ascending from -spin."""
+    order over (bg, n, E_m..., E_p...) with E ascending from -spin; the static
+    charge bg (values -background..background, present when background > 0)
+    enters the Gauss law as div E = matter charge + bg and is ignored by the
+    border classification."""
     rish = list(range(-spin, spin + 1))
     dof = [rish] * (2 * lattice_dim)
     if pure_theory:
@@ -164,6 +178,8 @@ def gauge_invariant_states(spin, pure_theory, lattice_dim):
     else:
         cores, parity = ["even", "odd"], [1, -1]
         dof = [[0, 1]] + dof
+    if background > 0:
+        dof = [list(range(-background, background + 1))] + dof
     borders = border_labels(lattice_dim)
     rows, states = {}, {}
     D_full = 0
@@ -171,20 +187,21 @@ def gauge_invariant_states(spin, pure_theory, lattice_dim):
         keys = [core] + [f"{core}_{b}" for b in borders]
         for k in keys:
             rows[k], states[k] = [], []
-        for row, cfg in enumerate(product(*dof)):
+        for row, full in enumerate(product(*dof)):
             D_full = row + 1
+            q_bg, cfg = (full[0], full[1:]) if background > 0 else (0, full)
             if pure_theory:
-                charge, links = 0, cfg
+                charge, links = q_bg, cfg
             else:
-                charge = cfg[0] - (0 if parity[ii] > 0 else 1)
+                charge = cfg[0] - (0 if parity[ii] > 0 else 1) + q_bg
                 links = cfg[1:]
             div = sum(links[mu + lattice_dim] - links[mu] for mu in range(lattice_dim))
             if div == charge:
                 rows[core].append(row)
-                states[core].append(cfg)
+                states[core].append(full)
                 for b in border_configs(list(links), lattice_dim, 0):
                     rows[f"{core}_{b}"].append(row)
-                    states[f"{core}_{b}"].append(cfg)
+                    states[f"{core}_{b}"].append(full)
     basis = {}
     for k in rows:
         n = len(rows[k])

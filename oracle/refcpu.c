@@ -14,7 +14,10 @@
  *                                 nbody_data (edlgt/symmetries/sym_qmb_operations.py:95-396)
  *                                 with config_to_index_binarysearch (generate_configs.py:145-173)
  *
- * Build: oracle/Makefile (gcc -O3 -march=native -fopenmp -shared).
+ * Build: oracle/Makefile (gcc -O3 -march=x86-64-v3 -fopenmp -shared): AVX2 + FMA, the
+ * widest level every x86-64 host of this image supports; the library is built once and
+ * runs on hosts whose CPU model differs from the build machine's, where -march=native
+ * could emit instructions the other host lacks.
  */
 #include <math.h>
 #include <stdint.h>

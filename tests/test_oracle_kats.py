@@ -45,3 +45,77 @@ def test_qed_3d_theta_kat():
     M = QEDOracle(1, True, [2, 2, 2], [False, False, False])
     e = lowest_eigs(M.hamiltonian(2.3, theta=0.41), 1)[0] / 8
     assert abs(e - (-0.37555632787741716)) < 1e-10
+
+
+def _qed_ground_observables(M, psi):
+    """workflows/qed.py:134-155 on the oracle: E2 = link_avg('E2') over the
+    existing links, N = 0.5 even(N) + 0.5 odd(N_zero), mean xy plaquette."""
+    from oracle import lattice as lat
+    from oracle.model import expval_nbody
+    p2 = np.abs(psi) ** 2
+    cfg = M.configs
+
+    def local(name):
+        return np.array([np.sum(p2 * np.real(M.ops[name][s][cfg[:, s], cfg[:, s]])) for s in range(M.n_sites)])
+
+    e2, cnt = 0.0, 0
+    for d in M.dirs:
+        v = local(f"E2_p{d}")
+        for i in range(M.n_sites):
+            if lat.neighbor_sites(lat.zig_zag(M.lvals, i), M.lvals, d, M.has_obc) is not None:
+                e2 += v[i]
+                cnt += 1
+    even = np.array([lat.staggered_mask(M.lvals, "even")[lat.zig_zag(M.lvals, i)] for i in range(M.n_sites)])
+    n_obs = 0.5 * local("N")[even].mean() + 0.5 * local("N_zero")[~even].mean()
+    names = ["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"]
+    plaqs = [p for p in (lat.plaquette_sites(lat.zig_zag(M.lvals, i), M.lvals, ["x", "y"], M.has_obc)
+                         for i in range(M.n_sites)) if p]
+    plq = np.mean([expval_nbody([M.ops[n] for n in names], p, cfg, M.keys, M.loc_dims, psi).real for p in plaqs])
+    return e2 / cnt, n_obs, plq
+
+
+def test_qed_2d_stringbreaking_kat():
+    """validation/qed/test_qed_2d_stringbreaking.py:8-36: 3x2 OBC with matter
+    and static charges -1 (site 0) and +1 (site 5), g = m = 1: energy per
+    site, E2, N and the plaquette at atol 1e-10."""
+    M = QEDOracle(1, False, [3, 2], [True, True], bg_list=[-1, 0, 0, 0, 0, 1])
+    H = M.hamiltonian(1.0, 1.0)
+    w, v = np.linalg.eigh(H.toarray())
+    assert abs(w[0] / 6 - (-0.49619430243949)) < 1e-10
+    e2, n_obs, plq = _qed_ground_observables(M, v[:, 0])
+    assert abs(e2 - 0.4462942942081073) < 1e-10
+    assert abs(n_obs - 0.08360798337722186) < 1e-10
+    assert abs(plq - 0.4555429224294397) < 1e-10
+
+
+def test_background_charges_shift_gauss_law():
+    """With static charges the kept local states satisfy div E = matter
+    charge + q (QED_operators.py:503-510), the operator 'bg' is q on them, and
+    a zero background is the plain model."""
+    M = QEDOracle(1, False, [3, 2], [True, True], bg_list=[-1, 0, 0, 0, 0, 1])
+    M0 = QEDOracle(1, False, [3, 2], [True, True])
+    Mz = QEDOracle(1, False, [3, 2], [True, True], bg_list=[0] * 6)
+    assert np.array_equal(Mz.configs, M0.configs)
+    for s, q in enumerate([-1, 0, 0, 0, 0, 1]):
+        d = M.loc_dims[s]
+        assert np.allclose(M.ops["bg"][s][:d, :d], q * np.eye(d))
+        div = sum(np.real(np.diag(M.ops[f"E_p{a}"][s] - M.ops[f"E_m{a}"][s]))[:d] for a in M.dirs)
+        odd = 1 if s in (1, 3, 5) else 0  # 3x2: x fastest, (-1)^(x+y)
+        charge = np.real(np.diag(M.ops["N"][s]))[:d] - odd + q
+        assert np.allclose(div, charge)
+    assert M.N != M0.N
+
+
+def test_label_configs_are_vacuum_and_meson():
+    """The oracle's benchmark product states lie in the sector and carry the
+    stated physics: the vacuum has no electric energy and the staggered
+    filling; the meson adds one unit of flux on one link."""
+    M = QEDOracle(1, False, [10], [True])
+    rows = {tuple(r) for r in M.configs.tolist()}
+    v, mc = M.label_config("V"), M.label_config("meson_center")
+    assert tuple(v) in rows and tuple(mc) in rows
+    e2 = lambda c: sum(np.real(M.ops["E2"][s][c[s], c[s]]) for s in range(10))
+    n = lambda c: [np.real(M.ops["N"][s][c[s], c[s]]) for s in range(10)]
+    assert e2(v) == 0.0 and n(v) == [0.0, 1.0] * 5
+    assert e2(mc) == 1.0  # two half-links at E = +1, E2 = 1/2 sum of halves
+    assert np.sum(np.array(n(mc)) != np.array(n(v))) == 2
