@@ -490,6 +490,15 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
   return EDLGT_OK;
 }
 
+int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t vec_type,
+                                  int32_t d_max, double* out_dev, void* stream) {
+  EDLGT_REQUIRE(b && psi && out_dev, "null argument");
+  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64, "vec_type must be C128 or F64");
+  EDLGT_REQUIRE(d_max >= 1 && b->n_sites * d_max <= 4096, "n_sites * d_max must be <= 4096");
+  return basis_marginals_noalloc(*b, psi, vec_type == EDLGT_VALUE_C128, d_max, out_dev,
+                                 (hipStream_t)stream);
+}
+
 int edlgt_random_uniform(void* out, int64_t n, int32_t value_type, const uint64_t* state, void* stream) {
   EDLGT_REQUIRE(n >= 0, "n must be >= 0");
   EDLGT_REQUIRE(n == 0 || (out && state), "null argument");

@@ -97,6 +97,9 @@ int server_step(void* p, int max_done, int64_t* ids, double* evals, double* est,
 
 int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
                     hipStream_t st);
+// Same result without allocating: partial sums in a static per-device array.
+int basis_marginals_noalloc(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
+                            hipStream_t st);
 
 // out[i] = i-th uniform [0, 1) double of the PCG64 stream with 128-bit state
 // and increment {state_hi, state_lo, inc_hi, inc_lo} (numpy's

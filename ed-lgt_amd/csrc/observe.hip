@@ -13,6 +13,7 @@
 // scheduling); per-block partial histograms are summed in block order.  The
 // absolute error is at most N * 2^-62 (< 1e-10 for N < 4.6e8).
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 
@@ -61,32 +62,88 @@ __global__ void marginal_finish(const unsigned long long* __restrict__ partial, 
   out[i] = (double)s / kScale;
 }
 
+// Scratch of the allocation-free entry point: per-block partial histograms,
+// kScratchWords 64-bit words (4 MiB) per device, held until the process ends.
+// Host code never uses the array's name as an address: each device's copy is
+// resolved once with hipGetSymbolAddress and kept in g_scratch_dev.
+constexpr int64_t kScratchWords = int64_t(1) << 19;
+__device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+int marginal_blocks(int64_t n_states, int bins, int64_t max_words) {
+  int64_t nblk = std::min<int64_t>(1024, std::max<int64_t>(1, (n_states + 4095) / 4096));
+  nblk = std::min<int64_t>(nblk, max_words / bins);
+  return (int)nblk;
+}
+
+int marginals_launch(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
+                     unsigned long long* part, int nblk, hipStream_t st) {
+  const int bins = B.n_sites * d_max;
+  if (cplx)
+    hipLaunchKernelGGL(marginal_partial<true>, dim3(nblk), dim3(256), 0, st, (const double*)psi,
+                       B.n_states, B.configs.as<uint8_t>(), B.cfg_stride, B.n_sites, d_max, part);
+  else
+    hipLaunchKernelGGL(marginal_partial<false>, dim3(nblk), dim3(256), 0, st, (const double*)psi,
+                       B.n_states, B.configs.as<uint8_t>(), B.cfg_stride, B.n_sites, d_max, part);
+  EDLGT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(marginal_finish, dim3((bins + 255) / 256), dim3(256), 0, st, part, nblk, bins,
+                     out_dev);
+  EDLGT_HIP(hipGetLastError());
+  return EDLGT_OK;
+}
+
+int check_marginal_args(const Basis& B, int d_max) {
+  const int bins = B.n_sites * d_max;
+  if (bins > kMaxBins) return set_error(EDLGT_ERR_INVALID_ARG, "n_sites * d_max exceeds 4096");
+  for (int s = 0; s < B.n_sites; ++s)
+    if (B.loc_dims[s] > d_max) return set_error(EDLGT_ERR_INVALID_ARG, "d_max below a local dimension");
+  return EDLGT_OK;
+}
+
 }  // namespace
 
 // out_dev: device double [n_sites * d_max]
 int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
                     hipStream_t st) {
+  EDLGT_TRY(check_marginal_args(B, d_max));
   const int bins = B.n_sites * d_max;
-  if (bins > kMaxBins) return set_error(EDLGT_ERR_INVALID_ARG, "n_sites * d_max exceeds 4096");
-  for (int s = 0; s < B.n_sites; ++s)
-    if (B.loc_dims[s] > d_max) return set_error(EDLGT_ERR_INVALID_ARG, "d_max below a local dimension");
-  const int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (B.n_states + 4095) / 4096));
+  const int nblk = marginal_blocks(B.n_states, bins, int64_t(1024) * bins);
   DevBuf part;
   EDLGT_TRY(part.alloc((size_t)nblk * bins * sizeof(unsigned long long)));
-  if (cplx)
-    hipLaunchKernelGGL(marginal_partial<true>, dim3(nblk), dim3(256), 0, st, (const double*)psi,
-                       B.n_states, B.configs.as<uint8_t>(), B.cfg_stride, B.n_sites, d_max,
-                       part.as<unsigned long long>());
-  else
-    hipLaunchKernelGGL(marginal_partial<false>, dim3(nblk), dim3(256), 0, st, (const double*)psi,
-                       B.n_states, B.configs.as<uint8_t>(), B.cfg_stride, B.n_sites, d_max,
-                       part.as<unsigned long long>());
-  EDLGT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(marginal_finish, dim3((bins + 255) / 256), dim3(256), 0, st,
-                     part.as<unsigned long long>(), nblk, bins, out_dev);
-  EDLGT_HIP(hipGetLastError());
+  EDLGT_TRY(marginals_launch(B, psi, cplx, d_max, out_dev, part.as<unsigned long long>(), nblk, st));
   EDLGT_HIP(hipStreamSynchronize(st));
   return EDLGT_OK;
+}
+
+// The same histograms with the partial sums in the file-scope scratch array:
+// no allocation, no synchronisation (stream-ordered).  Every launch writes all
+// nblk * bins partial words before the finish kernel reads them, so nothing
+// needs clearing; calls that overlap on one device share the array and race.
+// The fixed-point sums are associative, so the result equals basis_marginals'
+// bit for bit although the block count may differ.
+int basis_marginals_noalloc(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
+                            hipStream_t st) {
+  EDLGT_TRY(check_marginal_args(B, d_max));
+  int dev = 0;
+  EDLGT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(EDLGT_ERR_INVALID_ARG, "device ordinal above 63");
+  void* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+      void* p = nullptr;
+      EDLGT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      g_scratch_dev[dev] = p;
+    }
+    part = g_scratch_dev[dev];
+  }
+  const int bins = B.n_sites * d_max;
+  const int nblk = marginal_blocks(B.n_states, bins, kScratchWords);
+  return marginals_launch(B, psi, cplx, d_max, out_dev, static_cast<unsigned long long*>(part), nblk,
+                          st);
 }
 
 }  // namespace edlgt

@@ -81,6 +81,7 @@ SIGNATURES = {
     "edlgt_server_destroy": (_i32, [_p]),
     "edlgt_lanczos": (_i32, [_p, _i32, _f64, _i32, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
     "edlgt_basis_marginals": (_i32, [_p, _p, _i32, _i32, _p, _p]),
+    "edlgt_basis_marginals_noalloc": (_i32, [_p, _p, _i32, _i32, _p, _p]),
     "edlgt_expval_terms": (_i32, [_p, _i32, _i32, _p, _i32, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "edlgt_krylov_evolve": (_i32, [_p, _p, _i32, _p, _f64, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
     "edlgt_momentum_build": (_i32, [_p, _i32, _p, _p, _p, _p, _p]),

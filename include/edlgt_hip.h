@@ -351,6 +351,17 @@ int edlgt_server_destroy(edlgt_server* s);
  * ====================================================================== */
 int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
                           double* out_host, void* stream);
+/* Allocation-free variant of edlgt_basis_marginals.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * out_dev: device double [n_sites * d_max], written in stream order (no
+ * synchronisation, no host copy).  The per-block partial sums live in a
+ * static 4 MiB array per device (the library's only fixed scratch; held until
+ * the process ends), so n_sites * d_max <= 4096 as above and nothing else is
+ * limited.  The result equals edlgt_basis_marginals' bit for bit.  Calls that
+ * overlap on one device share the array and race: order them on one stream. */
+int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
+                                  double* out_dev, void* stream);
 
 /* ======================================================================
  * Expectation values of term instances without building their matrices:
