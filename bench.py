@@ -29,11 +29,19 @@ Poisson arrivals (rate 20/s, seed 1), lattice drawn uniformly, each asking for
 the lowest 4 eigenvalues at tol 1e-8.  Served: resident bases and
 dictionary-coded multi-term matrices (assembled once per lattice), one
 continuous-batching server per lattice on its own stream, admission by device
-memory footprint, warm starts from cached Ritz blocks.  Plain: the same
-stream one request at a time, each re-assembling H(J, h) and running the FP64
-block Lanczos cold; timed on the first --plain-sample requests of the stream
-(its full 512-request run takes many minutes).  |dE| of the served against the
+memory footprint, warm starts from cached Ritz blocks.  The whole stream is
+served (latency, and capacity with every request present at t = 0).  Plain:
+one request at a time, each re-assembling H(J, h) and running the FP64 block
+Lanczos cold.  Served and plain are compared on the same requests: every
+stride-th request of the stream (--plain-sample of them, 64 by default, so the
+lattice mix is the stream's), at their stream arrival times (solves/s, p50,
+p99) and all present at t = 0 (capacity).  |dE| of the served against the
 plain eigenvalues is reported over that sample.
+
+Host-core side of the default line: the C/OpenMP restatement of csr_spmv
+(-march=native, built on the host) over the same matrix on all host threads
+and on one thread, with the host CPU model; and the full QED 4x3 build +
+lowest eigenvalue on the host cores beside the GPU's (--no-host-e0 skips it).
 
 Workload su2c3 (dynamics): 1D SU(2), spin-1/2 rishons with staggered
 two-colour matter, 20 sites, OBC, N_tot = 20, g = 1, m = 1 (N = 377,379,369
@@ -119,25 +127,52 @@ def time_launches(fn, steps, warmup):
     return s.elapsed_time(e) / steps
 
 
-def cpu_baseline(H, x_dev, max_seconds=12.0):
-    """oracle/refcpu csr_spmv (row-parallel, complex128) on the host cores over
-    the same CSR; returns GB/s by the same algorithmic-bytes formula."""
+def cpu_baseline(H, x_dev, max_seconds=12.0, one_thread_seconds=8.0):
+    """oracle/refcpu csr_spmv (row-parallel, complex128; built -march=native on
+    this host) on the host cores over the same CSR, then on one thread;
+    returns GB/s by the same algorithmic-bytes formula, with the host's CPU."""
     from oracle import refcpu
+    native = refcpu.use_native()
     rp, col, val = H.to_host()
     x = x_dev.cpu().numpy()
     y = np.empty_like(x)
-    refcpu.spmv(rp, col, val, x, y)  # warm
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        refcpu.spmv(rp, col, val, x, y)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > max_seconds or reps >= 50:
-            break
-    t = el / reps
-    return dict(value=round(spmv_bytes(H.n, H.nnz) / t / 1e9, 2), unit="GB/s", cores=refcpu.threads(),
-                kind="port", sample=f"{reps} full complex128 CSR SpMV passes over the same matrix "
-                                    f"(N={H.n}, nnz={H.nnz}), {t*1e3:.1f} ms each")
+    B = spmv_bytes(H.n, H.nnz)
+
+    def timed(limit, max_reps):
+        refcpu.spmv(rp, col, val, x, y)  # warm
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            refcpu.spmv(rp, col, val, x, y)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > limit or reps >= max_reps:
+                return reps, el / reps
+
+    threads = refcpu.threads()
+    reps, t = timed(max_seconds, 50)
+    refcpu.set_threads(1)
+    reps1, t1 = timed(one_thread_seconds, 5)
+    refcpu.set_threads(threads)
+    model, sockets, cores, cpus = refcpu.host_cpu()
+    return dict(value=round(B / t / 1e9, 2), unit="GB/s", cores=threads, kind="port",
+                sample=f"{reps} full complex128 CSR SpMV passes over the same matrix "
+                       f"(N={H.n}, nnz={H.nnz}), {t*1e3:.1f} ms each, {threads} threads; "
+                       f"{reps1} passes on 1 thread, {t1*1e3:.0f} ms each",
+                one_thread_GBs=round(B / t1 / 1e9, 2),
+                host=dict(model=model, sockets=sockets, physical_cores=cores, logical_cpus=cpus),
+                build="gcc -O3 -fopenmp " + ("-march=native (built on this host)" if native else
+                                             "-march=x86-64-v3 (no compiler on this host)"))
+
+
+def host_build_e0():
+    """The full QED 4x3 build + lowest-eigenvalue path on the host cores
+    (oracle/refcpu: C/OpenMP sector and two-pass triplets, SciPy coalescing,
+    eigsh on the C SpMV), timed once."""
+    from oracle import refcpu
+    t0 = time.perf_counter()
+    e_cpu, N, nnz, T = refcpu.qed_build_and_e0([4, 3], [False, False], 1.0, 0.5, log=progress)
+    return dict(build_plus_E0_s=round(time.perf_counter() - t0, 2), threads=refcpu.threads(), E0=e_cpu,
+                phases_s={k: round(v, 2) for k, v in T.items()}, sector_dim=N, nnz=nnz)
 
 
 def cpu_build_e0(args):
@@ -337,25 +372,51 @@ def serve_leg(args):
         ev, _, _, _ = H.lanczos(4, tol=1e-8, max_basis=args.plain_basis, want_vectors=False)
         return ev
 
-    sample = sorted(reqs, key=lambda r: r["arrival"])[:args.plain_sample]
-    progress(f"served {served['requests']} requests in {served['wall_s']:.1f}s; plain sample")
+    # the same requests through both paths: every stride-th request of the
+    # whole stream (so the three lattice sizes appear as in the stream), at
+    # their stream arrival times, and all present at t = 0 (capacity)
+    ordered = sorted(reqs, key=lambda r: r["arrival"])
+    stride = max(1, len(ordered) // args.plain_sample)
+    sample = ordered[::stride][:args.plain_sample]
+    progress(f"served {served['requests']} requests in {served['wall_s']:.1f}s; "
+             f"same-sample comparison on {len(sample)} requests")
+    svc = SolveService(csrs, n_eigs=4, tol=1e-8, max_basis=args.serve_basis, max_slots=16, warm_cache=64)
+    sres = svc.run(sample)
+    served_s = latency_summary(sres, svc.wall)
+    del svc
+    svc = SolveService(csrs, n_eigs=4, tol=1e-8, max_basis=args.serve_basis, max_slots=16, warm_cache=64)
+    cres = svc.run([dict(r, arrival=0.0) for r in sample])
+    served_cap = len(cres) / svc.wall
+    del svc
+    torch.cuda.empty_cache()
     pres = serve_plain(solve_one, sample)
     plain = latency_summary(pres, serve_plain.wall)
     busy = sum(r["done"] - r["admitted"] for r in pres.values())
-    plain["service_solves_per_s"] = len(pres) / busy
-    d = [float(np.max(np.abs(res[i]["evals"] - pres[i]["evals"]))) for i in pres]
-    d0 = [float(abs(res[i]["evals"][0] - pres[i]["evals"][0])) for i in pres]
-    served_prefix = latency_summary({i: res[i] for i in pres}, max(res[i]["done"] for i in pres))
+    plain_cap = len(pres) / busy
+    d = [float(np.max(np.abs(sres[i]["evals"] - pres[i]["evals"]))) for i in pres]
+    d0 = [float(abs(sres[i]["evals"][0] - pres[i]["evals"][0])) for i in pres]
+    mix = {p: sum(1 for r in sample if r["problem"] == p) for p in lats}
+    same = dict(
+        requests=len(sample), stride=stride, lattice_mix=mix,
+        arrival_rate_per_s=round(args.serve_rate / stride, 3),
+        served=dict(solves_per_s=round(served_s["solves_per_s"], 3), p50_ms=round(served_s["p50_ms"], 1),
+                    p99_ms=round(served_s["p99_ms"], 1), mean_ms=round(served_s["mean_ms"], 1)),
+        plain=dict(solves_per_s=round(plain["solves_per_s"], 3), p50_ms=round(plain["p50_ms"], 1),
+                   p99_ms=round(plain["p99_ms"], 1), mean_ms=round(plain["mean_ms"], 1)),
+        capacity_solves_per_s=dict(served=round(served_cap, 2), plain=round(plain_cap, 3),
+                                   ratio=round(served_cap / plain_cap, 2)),
+        p50_ratio=round(plain["p50_ms"] / served_s["p50_ms"], 2),
+        p99_ratio=round(plain["p99_ms"] / served_s["p99_ms"], 2),
+        max_dE_served_vs_plain=max(d), max_dE0_served_vs_plain=max(d0))
     return dict(
         workload="Z3 pure gauge 3x3/4x3/4x4 PBC zero-flux sectors (6561/177147/14348907 states), J=1, "
                  "h~U[0.3,3], lowest 4 eigenvalues at tol 1e-8",
         stream=dict(requests=args.serve_requests, poisson_rate_per_s=args.serve_rate, seed=1),
         lanczos_basis=dict(served=args.serve_basis, plain=args.plain_basis),
-        setup_s=round(t_setup, 3), served=served, plain_sample=plain,
-        served_on_plain_sample=served_prefix, served_capacity=capacity,
-        speedup_solves_per_s=round(served["solves_per_s"] / plain["service_solves_per_s"], 2),
-        speedup_capacity=round(capacity["solves_per_s"] / plain["service_solves_per_s"], 2),
-        max_dE_served_vs_plain=max(d), max_dE0_served_vs_plain=max(d0))
+        setup_s=round(t_setup, 3), served=served, served_capacity=capacity,
+        same_sample=same,
+        speedup_capacity_same_sample=same["capacity_solves_per_s"]["ratio"],
+        speedup_capacity_vs_plain_service_rate=round(capacity["solves_per_s"] / plain_cap, 2))
 
 
 def momentum_sectors(args):
@@ -452,13 +513,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3", "cpu_qed4x3", "momentum", "matfree"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host-e0", action="store_true",
+                    help="skip the host-core build + lowest eigenvalue of QED 4x3 (about two minutes)")
     ap.add_argument("--no-solve", action="store_true",
                     help="SpMV line only: no Lanczos solves, coupling sweep or serving leg")
     ap.add_argument("--sites", type=int, default=20)
     ap.add_argument("--dyn-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="requests in the same-sample served/plain comparison (every stride-th of the stream)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-basis", type=int, default=160,
                     help="Lanczos basis size of the served solves (measured on the stream: p99 "
@@ -557,6 +621,7 @@ def main():
     if not args.no_cpu:
         progress("CPU baseline")
         cpu = cpu_baseline(H, x)
+        torch.cuda.empty_cache()
     progress("coded SpMV")
 
     # dictionary-coded values: same products bit for bit, 5 bytes per entry
@@ -589,6 +654,14 @@ def main():
     if not args.no_serve and not args.no_solve:
         progress("serving leg")
         serving = serve_leg(args)
+
+    if cpu is not None and not args.no_host_e0 and args.workload == "qed4x3" and not args.no_solve:
+        progress("host-core build + E0 of the same sector")
+        he = host_build_e0()
+        he["gpu_build_plus_E0_s"] = extra["build_plus_E0_s"]
+        he["speedup"] = round(he["build_plus_E0_s"] / extra["build_plus_E0_s"], 1)
+        he["dE0_gpu_vs_host"] = abs(he["E0"] - extra["E0_fp64"])
+        cpu["build_plus_E0"] = he
 
     traffic = None
     tf = ROOT / "profiles" / f"spmv_{args.workload}_traffic.json"

@@ -26,117 +26,17 @@
 #include <memory>
 
 #include "common.h"
+#include "gram.h"
 #include "kernels.h"
 
 namespace edlgt {
 namespace {
 
+using namespace gramk;
 using cd = std::complex<double>;
-
-// ---------------------------------------------------------------- scalars
-__device__ inline double2 cj_mul(double2 a, double2 b) {  // conj(a) * b
-  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
-}
-__device__ inline double cj_mul(double a, double b) { return a * b; }
-__device__ inline double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double add(double a, double b) { return a + b; }
-__device__ inline double2 mul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-// V element times a complex coefficient, in the vector's scalar type (real
-// vectors use the real part: their coefficients are real).
-__device__ inline double2 mul_s(double2 v, double2 c) { return mul(v, c); }
-__device__ inline double mul_s(double v, double2 c) { return v * c.x; }
-template <class S> __device__ inline S zero();
-template <> __device__ inline double zero<double>() { return 0.0; }
-template <> __device__ inline double2 zero<double2>() { return make_double2(0.0, 0.0); }
-__device__ inline double2 to_c(double2 a) { return a; }
-__device__ inline double2 to_c(double a) { return make_double2(a, 0.0); }
 
 constexpr double kDgks = 0.5;  // squared-norm ratio above which one Gram-Schmidt pass suffices
 constexpr double kSkipLoss = 1e-14;  // measured loss of orthogonality left uncorrected
-constexpr int kGramBlocks = 1024;
-constexpr int kGramCols = 64;  // V columns per Gram launch (one per lane)
-constexpr int kGramRows = 8;   // rows per thread per sweep
-constexpr int kTB = 256;
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ inline double2 wave_sum(double2 v) { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
-
-// partial[blk][j][c] = sum over the block's rows of conj(V_j[r]) W_c[r],
-// j < nv <= 64, c < nw <= NW.  Each thread keeps kGramRows rows of W in
-// registers for a sweep, so W and V are each read once; for every column j
-// the lane products are summed across the wave with a fixed xor butterfly
-// and lane j keeps the running sum of column j.  The four waves' sums are
-// combined in fixed order: the result is bitwise reproducible.
-template <class S, int NW>
-__global__ void __launch_bounds__(kTB) gram_partial(const S* __restrict__ V, int64_t ldv, int nv,
-                                                    const S* __restrict__ W, int64_t ldw, int nw,
-                                                    int64_t n, double2* __restrict__ partial) {
-  __shared__ double2 red[kTB / 64][kGramCols][NW];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = (int64_t)blockIdx.x * chunk;
-  const int64_t r1 = min(n, r0 + chunk);
-  S mine[NW];
-#pragma unroll
-  for (int c = 0; c < NW; ++c) mine[c] = zero<S>();
-  for (int64_t base = r0; base < r1; base += (int64_t)kTB * kGramRows) {
-    S w[kGramRows][NW];
-#pragma unroll
-    for (int i = 0; i < kGramRows; ++i) {
-      const int64_t r = base + (int64_t)i * kTB + threadIdx.x;
-#pragma unroll
-      for (int c = 0; c < NW; ++c) w[i][c] = (r < r1 && c < nw) ? W[c * ldw + r] : zero<S>();
-    }
-    for (int j = 0; j < nv; ++j) {
-      S acc[NW];
-#pragma unroll
-      for (int c = 0; c < NW; ++c) acc[c] = zero<S>();
-#pragma unroll
-      for (int i = 0; i < kGramRows; ++i) {
-        const int64_t r = base + (int64_t)i * kTB + threadIdx.x;
-        const S v = r < r1 ? V[(int64_t)j * ldv + r] : zero<S>();
-#pragma unroll
-        for (int c = 0; c < NW; ++c) acc[c] = add(acc[c], cj_mul(v, w[i][c]));
-      }
-#pragma unroll
-      for (int c = 0; c < NW; ++c) {
-        const S t = wave_sum(acc[c]);
-        if (lane == j) mine[c] = add(mine[c], t);
-      }
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < NW; ++c) red[wave][lane][c] = to_c(mine[c]);
-  __syncthreads();
-  for (int t = threadIdx.x; t < nv * NW; t += kTB) {
-    const int j = t / NW, c = t % NW;
-    double2 s = red[0][j][c];
-#pragma unroll
-    for (int q = 1; q < kTB / 64; ++q) s = add(s, red[q][j][c]);
-    partial[((int64_t)blockIdx.x * nv + j) * NW + c] = s;
-  }
-}
-
-// out[(j0 + j) * nw_real + c] = sum over blocks of partial[b][j][c]: one wave
-// per output element, lanes over blocks, fixed butterfly.
-template <int NW>
-__global__ void __launch_bounds__(kTB) gram_finish(const double2* __restrict__ partial, int nblk, int nv,
-                                                   int nw_real, int j0, double2* __restrict__ out) {
-  const int e = blockIdx.x * (kTB / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (e >= nv * NW) return;  // whole waves exit together
-  const int j = e / NW, c = e % NW;
-  double2 s = make_double2(0.0, 0.0);
-  for (int b = lane; b < nblk; b += 64) s = add(s, partial[((int64_t)b * nv + j) * NW + c]);
-  s = wave_sum(s);
-  if (lane == 0 && c < nw_real) out[(int64_t)(j0 + j) * nw_real + c] = s;
-}
 
 // Per-thread Gram accumulator of the NW values a thread writes for one row
 // (g[i][c] += conj(y_i) y_c), reduced over the block at the end (fixed wave
@@ -176,6 +76,14 @@ struct RowGram {
   }
 };
 
+// Rows per thread per loop iteration of the streaming updates: independent
+// rows whose loads are all issued before any is used, so that a wave keeps
+// several memory requests in flight (one row per iteration ran the QED 4x3
+// local update at 3.2 TB/s).  A thread still visits its rows in the order
+// r, r + stride, r + 2 stride, ..., so its Gram sums are added in the same
+// order as with one row per iteration: results are unchanged bit for bit.
+template <int NW> constexpr int rows_unroll() { return NW <= 2 ? 4 : NW <= 4 ? 2 : 1; }
+
 // W[:, c] = beta * W[:, c] + sum_j V_j C[j][c]    (C: nv x nw, row-major, complex)
 // With kGram the Gram matrix of the new W columns is accumulated on the way
 // (partial[blk][i][c], NW x NW per block).
@@ -184,26 +92,43 @@ __global__ void __launch_bounds__(kTB) tall_update(const S* __restrict__ V, int6
                                                    const double2* __restrict__ C, int ldc,
                                                    S* __restrict__ W, int64_t ldw, int nw,
                                                    int64_t n, int beta, double2* __restrict__ partial) {
+  constexpr int RU = rows_unroll<NW>();
   extern __shared__ double2 cs[];
   for (int i = threadIdx.x; i < nv * nw; i += blockDim.x) cs[i] = C[(i / nw) * ldc + (i % nw)];
   __syncthreads();
   RowGram<S, kGram ? NW : 1> rg;
   if (kGram) rg.clear();
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    S acc[NW];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r0 < n; r0 += stride * RU) {
+    S acc[RU][NW];
 #pragma unroll
-    for (int c = 0; c < NW; ++c) acc[c] = (beta && c < nw) ? W[c * ldw + r] : zero<S>();
+    for (int q = 0; q < RU; ++q) {
+      const int64_t r = r0 + q * stride;
+#pragma unroll
+      for (int c = 0; c < NW; ++c) acc[q][c] = (beta && c < nw && r < n) ? W[c * ldw + r] : zero<S>();
+    }
     for (int j = 0; j < nv; ++j) {
-      const S v = V[(int64_t)j * ldv + r];
+      S v[RU];
 #pragma unroll
-      for (int c = 0; c < NW; ++c)
-        if (c < nw) acc[c] = add(acc[c], mul_s(v, cs[j * nw + c]));
+      for (int q = 0; q < RU; ++q) {
+        const int64_t r = r0 + q * stride;
+        v[q] = r < n ? V[(int64_t)j * ldv + r] : zero<S>();
+      }
+#pragma unroll
+      for (int q = 0; q < RU; ++q)
+#pragma unroll
+        for (int c = 0; c < NW; ++c)
+          if (c < nw) acc[q][c] = add(acc[q][c], mul_s(v[q], cs[j * nw + c]));
     }
 #pragma unroll
-    for (int c = 0; c < NW; ++c)
-      if (c < nw) W[c * ldw + r] = acc[c];
-    if constexpr (kGram) rg.add_row(acc);
+    for (int q = 0; q < RU; ++q) {
+      const int64_t r = r0 + q * stride;
+      if (r >= n) continue;
+#pragma unroll
+      for (int c = 0; c < NW; ++c)
+        if (c < nw) W[c * ldw + r] = acc[q][c];
+      if constexpr (kGram) rg.add_row(acc[q]);
+    }
   }
   if constexpr (kGram) rg.flush(partial);
 }
@@ -214,27 +139,38 @@ __global__ void __launch_bounds__(kTB) tall_update(const S* __restrict__ V, int6
 template <class S, int NW, bool kGram>
 __global__ void __launch_bounds__(kTB) row_transform(S* X, int64_t ldx, int nw, const double2* __restrict__ C,
                                                      int64_t n, double2* __restrict__ partial) {
+  constexpr int RU = rows_unroll<NW>();
   __shared__ double2 cs[NW * NW];
   for (int i = threadIdx.x; i < nw * nw; i += blockDim.x) cs[i] = C[i];
   __syncthreads();
   RowGram<S, kGram ? NW : 1> rg;
   if (kGram) rg.clear();
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    S x[NW], y[NW];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r0 < n; r0 += stride * RU) {
+    S x[RU][NW];
 #pragma unroll
-    for (int j = 0; j < NW; ++j) x[j] = j < nw ? X[j * ldx + r] : zero<S>();
+    for (int q = 0; q < RU; ++q) {
+      const int64_t r = r0 + q * stride;
 #pragma unroll
-    for (int c = 0; c < NW; ++c) {
-      y[c] = zero<S>();
-#pragma unroll
-      for (int j = 0; j < NW; ++j)
-        if (j < nw && c < nw) y[c] = add(y[c], mul_s(x[j], cs[j * nw + c]));
+      for (int j = 0; j < NW; ++j) x[q][j] = (j < nw && r < n) ? X[j * ldx + r] : zero<S>();
     }
 #pragma unroll
-    for (int c = 0; c < NW; ++c)
-      if (c < nw) X[c * ldx + r] = y[c];
-    if constexpr (kGram) rg.add_row(y);
+    for (int q = 0; q < RU; ++q) {
+      const int64_t r = r0 + q * stride;
+      if (r >= n) continue;
+      S y[NW];
+#pragma unroll
+      for (int c = 0; c < NW; ++c) {
+        y[c] = zero<S>();
+#pragma unroll
+        for (int j = 0; j < NW; ++j)
+          if (j < nw && c < nw) y[c] = add(y[c], mul_s(x[q][j], cs[j * nw + c]));
+      }
+#pragma unroll
+      for (int c = 0; c < NW; ++c)
+        if (c < nw) X[c * ldx + r] = y[c];
+      if constexpr (kGram) rg.add_row(y);
+    }
   }
   if constexpr (kGram) rg.flush(partial);
 }
@@ -248,14 +184,30 @@ __device__ inline double2 rscale(double c, double2 v) { return make_double2(c * 
 template <class S>
 __global__ void __launch_bounds__(kTB) three_term(const S* a, const S* b, S* w, double c0, double c1, double c2,
                                                   int64_t n, double2* __restrict__ partial) {
+  constexpr int RU = 4;
   RowGram<S, 1> rg;
   rg.clear();
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    S y[1];
-    y[0] = add(add(rscale(c2, w[r]), rscale(c1, b[r])), rscale(c0, a[r]));
-    w[r] = y[0];
-    rg.add_row(y);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r0 < n; r0 += stride * RU) {
+    S wa[RU], wb[RU], ww[RU];
+#pragma unroll
+    for (int q = 0; q < RU; ++q) {
+      const int64_t r = r0 + q * stride;
+      if (r < n) {
+        ww[q] = w[r];
+        wb[q] = b[r];
+        wa[q] = a[r];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RU; ++q) {
+      const int64_t r = r0 + q * stride;
+      if (r >= n) continue;
+      S y[1];
+      y[0] = add(add(rscale(c2, ww[q]), rscale(c1, wb[q])), rscale(c0, wa[q]));
+      w[r] = y[0];
+      rg.add_row(y);
+    }
   }
   rg.flush(partial);
 }
@@ -1200,6 +1152,36 @@ struct Solver {
   }
 };
 
+// Basis capacity M (a multiple of the block size b) for k wanted pairs.  A
+// basis that can hold the whole space (N <= 4096) switches to single vectors
+// run to exhaustion: always for N <= 512 when M >= N, and for larger N when
+// the half-sector cap would leave fewer than k + 2b vectors (a thick restart
+// keeps k + b of them and needs room for one more block, else the highest
+// wanted pairs are dropped at every restart and the solve cannot converge).
+// Above 4096 states such a k is an error: the projected matrix would have N
+// rows and is factored on the host.
+int choose_basis(int64_t N, int k, int max_basis, int& b, int& M) {
+  M = std::max(max_basis, 2 * k + 2 * b);
+  M = (M + b - 1) / b * b;
+  if (N <= 512 && M >= N) {
+    b = 1;
+    M = (int)N;
+    return EDLGT_OK;
+  }
+  if (M > N / 2) {
+    M = (int)std::max<int64_t>(2 * b, (N / 2) / b * b);
+    if (M < k + 2 * b) {
+      if (N > 4096)
+        return set_error(EDLGT_ERR_INVALID_ARG,
+                         "n_eigs too large for the sector: a restarted basis of at most N/2 vectors "
+                         "cannot hold n_eigs + 2 blocks");
+      b = 1;
+      M = (int)N;
+    }
+  }
+  return EDLGT_OK;
+}
+
 template <class S>
 int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, int precision,
                   const void* v0, double* evals, void* evecs, double* resid, int64_t* matvecs,
@@ -1217,15 +1199,8 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   // the Z3 4x4 sector: 5 blocks of 4 needed 2008 matvecs; 16 blocks were the
   // fastest of 5 to 24 on the served couplings)
   if (max_basis <= 0) max_basis = std::max(std::max(2 * k + 1, 20), b > 1 ? 16 * b : 0);
-  int M = std::max(max_basis, 2 * k + 2 * b);
-  M = (M + b - 1) / b * b;
-  if (N <= 512 && M >= N) {
-    // the basis can hold the whole space: single vectors, run to exhaustion
-    b = 1;
-    M = (int)N;
-  } else if (M > N / 2) {
-    M = (int)std::max<int64_t>(2 * b, (N / 2) / b * b);
-  }
+  int M = 0;
+  EDLGT_TRY(choose_basis(N, k, max_basis, b, M));
   sv.b = b;
   sv.M = M;
   // convergence tested after every block step (a host Rayleigh-Ritz of the
@@ -1409,14 +1384,7 @@ struct Server {
     max_slots = max_slots_;
     warm_cap = warm_cap_;
     b = std::min(k, 4);
-    M = std::max(max_basis, 2 * k + 2 * b);
-    M = (M + b - 1) / b * b;
-    if (N <= 512 && M >= N) {
-      b = 1;
-      M = (int)N;
-    } else if (M > N / 2) {
-      M = (int)std::max<int64_t>(2 * b, (N / 2) / b * b);
-    }
+    EDLGT_TRY(choose_basis(N, k, max_basis, b, M));
     slots.resize(max_slots);
     EDLGT_TRY(xs.alloc(kCoefMaxRhs * sizeof(void*)));
     EDLGT_TRY(ys.alloc(kCoefMaxRhs * sizeof(void*)));
@@ -1424,12 +1392,16 @@ struct Server {
     return EDLGT_OK;
   }
 
+  // device bytes of one slot's solver (Solver::setup with need_w = false):
+  // basis and restart buffer, Gram partials and the coefficient buffer; its
+  // Gram result buffer (same size as the coefficients) is pinned host memory
   size_t slot_bytes() const {
     const size_t N = (size_t)h.n_rows;
     return ((size_t)(M + b) + std::max(M + b, 8)) * N * sizeof(S) +
            (size_t)kGramBlocks * kGramCols * 8 * sizeof(double2) +
-           2 * (size_t)(M + b) * std::max(M, 8) * sizeof(double2);
+           (size_t)(M + b) * std::max(M, 8) * sizeof(double2);
   }
+  size_t slot_host_bytes() const { return (size_t)(M + b) * std::max(M, 8) * sizeof(double2); }
   // warm cache plus the packed right-hand-side workspace of the fused SpMM
   size_t shared_bytes() const {
     return (size_t)warm_cap * k * (size_t)h.n_rows * sizeof(S) + coef_pack_bytes(h, vec_type, kCoefMaxRhs);

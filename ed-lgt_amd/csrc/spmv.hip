@@ -167,6 +167,20 @@ __device__ inline void reduce_rows(const AT* prod, const int64_t* __restrict__ r
   }
 }
 
+// Tile of a stream-kernel block.  The hardware hands workgroup b to XCD
+// b % 8 (8 XCDs, each with its own L2); this gives every XCD runs of
+// kXcdRun consecutive tiles instead of every eighth tile, so rows that share
+// near-diagonal x lines run on one L2 (measured on QED 4x3: 3.13 -> 3.04 ms;
+// runs of 4 and 16 were slower than 8).
+constexpr int64_t kXcdRun = 8;
+__device__ inline int64_t stream_tile(int64_t b, int64_t n_tiles) {
+  constexpr int64_t per = 8 * kXcdRun;
+  const int64_t full = n_tiles / per * per;
+  if (b >= full) return b;
+  const int64_t r = b % per;
+  return b / per * per + (r % 8) * kXcdRun + r / 8;
+}
+
 template <class VT, class XT, class AT, int BS, int TILE, int U>
 __global__ void __launch_bounds__(BS) csr_stream_kernel(
     int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
@@ -174,13 +188,17 @@ __global__ void __launch_bounds__(BS) csr_stream_kernel(
     const int64_t* __restrict__ row_blocks, const int64_t* __restrict__ block_pos) {
   constexpr int CAP = TILE + 256;
   __shared__ AT prod[CAP];
-  const int64_t r0 = row_blocks[blockIdx.x], r1 = row_blocks[blockIdx.x + 1];
+  const int64_t t = stream_tile(blockIdx.x, gridDim.x);
+  const int64_t r0 = row_blocks[t], r1 = row_blocks[t + 1];
   if (r0 >= r1) return;
-  const int64_t p0 = block_pos[blockIdx.x], p1 = block_pos[blockIdx.x + 1];
+  const int64_t p0 = block_pos[t], p1 = block_pos[t + 1];
   const int64_t cnt = p1 - p0;
   const int64_t nr = r1 - r0;
   const int tid = threadIdx.x;
   if (cnt <= CAP) {
+    // memory phase at raised wave priority, so that the waves of blocks still
+    // loading issue ahead of those reducing from LDS (measured: -0.3 %)
+    __builtin_amdgcn_s_setprio(2);
     for (int64_t base = 0; base < cnt; base += BS * U) {
       int32_t c[U];
       decltype(load_val16(val, 0)) v[U];
@@ -202,6 +220,7 @@ __global__ void __launch_bounds__(BS) csr_stream_kernel(
         }
       }
     }
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     if (nr <= BS / 8) reduce_rows<8, AT, BS>(prod, row_ptr, r0, nr, p0, y);
     else if (nr <= BS / 4) reduce_rows<4, AT, BS>(prod, row_ptr, r0, nr, p0, y);
@@ -277,13 +296,15 @@ __global__ void __launch_bounds__(BS) csr_stream_coded(
     __syncthreads();
   }
   const TT* tb = kLds ? stab : tab;
-  const int64_t r0 = row_blocks[blockIdx.x], r1 = row_blocks[blockIdx.x + 1];
+  const int64_t t = stream_tile(blockIdx.x, gridDim.x);
+  const int64_t r0 = row_blocks[t], r1 = row_blocks[t + 1];
   if (r0 >= r1) return;
-  const int64_t p0 = block_pos[blockIdx.x], p1 = block_pos[blockIdx.x + 1];
+  const int64_t p0 = block_pos[t], p1 = block_pos[t + 1];
   const int64_t cnt = p1 - p0;
   const int64_t nr = r1 - r0;
   const int tid = threadIdx.x;
   if (cnt <= CAP) {
+    __builtin_amdgcn_s_setprio(2);
     for (int64_t base = 0; base < cnt; base += BS * U) {
       int32_t c[U];
       int k[U];
@@ -305,6 +326,7 @@ __global__ void __launch_bounds__(BS) csr_stream_coded(
         }
       }
     }
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     if (nr <= BS / 8) reduce_rows<8, AT, BS>(prod, row_ptr, r0, nr, p0, y);
     else if (nr <= BS / 4) reduce_rows<4, AT, BS>(prod, row_ptr, r0, nr, p0, y);

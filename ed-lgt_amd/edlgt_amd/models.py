@@ -141,9 +141,10 @@ class QMB_hamiltonian:
             if n_eigs != "full":
                 raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")
             # every eigenpair: the device solver's full-basis path (the
-            # Krylov space is the whole sector), small sectors only
-            if self.shape[0] > 512:
-                raise NotImplementedError("n_eigs='full' is solved on the device for sectors of <= 512 states")
+            # Krylov space is the whole sector, then one host factorization
+            # of the N x N projected matrix), sectors of up to 4096 states
+            if self.shape[0] > 4096:
+                raise NotImplementedError("n_eigs='full' is solved on the device for sectors of <= 4096 states")
             n_eigs = int(self.shape[0])
         if not isinstance(n_eigs, (int, np.integer)) or n_eigs < 1:
             raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")

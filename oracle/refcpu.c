@@ -14,10 +14,10 @@
  *                                 nbody_data (edlgt/symmetries/sym_qmb_operations.py:95-396)
  *                                 with config_to_index_binarysearch (generate_configs.py:145-173)
  *
- * Build: oracle/Makefile (gcc -O3 -march=x86-64-v3 -fopenmp -shared): AVX2 + FMA, the
- * widest level every x86-64 host of this image supports; the library is built once and
- * runs on hosts whose CPU model differs from the build machine's, where -march=native
- * could emit instructions the other host lacks.
+ * Build: oracle/Makefile.  librefcpu.so: gcc -O3 -march=x86-64-v3 -fopenmp -shared (AVX2 +
+ * FMA, loadable on every x86-64 host of this image: the checker library is built once and
+ * travels).  librefcpu_native.so (`make native`): the same with -march=native, built on the
+ * host that runs the CPU baseline, which is timed with it.
  */
 #include <math.h>
 #include <stdint.h>
@@ -33,6 +33,11 @@ int refcpu_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* OpenMP team size of the following calls (1 = the one-thread baseline). */
+void refcpu_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
 }
 
 /* y = H x, complex128 values/vectors as interleaved (re, im) doubles. */

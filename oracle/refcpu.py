@@ -8,11 +8,53 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "librefcpu.so"
+LIB_NATIVE = HERE / "librefcpu_native.so"
 _lib = None
 
 
 def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def use_native():
+    """Builds librefcpu_native.so (-march=native) on this host and makes the
+    following calls use it; returns False (keeping the portable build) when
+    the host has no compiler.  Call before the first use of the library."""
+    global LIB, _lib
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", str(HERE), "native"], check=True, capture_output=True)
+    except (OSError, subprocess.CalledProcessError):
+        return False
+    LIB, _lib = LIB_NATIVE, None
+    return True
+
+
+def set_threads(n):
+    """OpenMP team size of the following calls."""
+    load().refcpu_set_threads(int(n))
+
+
+def host_cpu():
+    """(model name, sockets, physical cores, logical CPUs) of this host."""
+    model, phys = "unknown", set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if cur:
+                    phys.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+                cur = {}
+                continue
+            k, v = (t.strip() for t in line.split(":", 1))
+            cur[k] = v
+            if k == "model name":
+                model = v
+        if cur:
+            phys.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+    except OSError:
+        pass
+    sockets = len({p[0] for p in phys}) or 1
+    return model, sockets, len(phys) or os.cpu_count(), os.cpu_count()
 
 
 def load():
@@ -23,6 +65,7 @@ def load():
         lib = C.CDLL(str(LIB))
         p, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
         lib.refcpu_threads.restype = i32
+        lib.refcpu_set_threads.argtypes = [i32]
         lib.refcpu_csr_spmv_c128.argtypes = [i64, p, p, p, p, p]
         lib.refcpu_csr_spmv_f64.argtypes = [i64, p, p, p, p, p]
         lib.refcpu_sector.restype = i64

@@ -137,3 +137,21 @@ def test_qed_3d_theta_sector_csr_energy(gpu):
     _csr_match(Hg, Ho)
     mod.diagonalize_Hamiltonian(1, "sparse", tol=1e-12)
     assert abs(mod.H.Nenergies[0] - (-0.37555632787741716)) < 1e-10
+
+
+@pytest.mark.parametrize("k", [270, 300, "full"])
+def test_many_eigenpairs_above_512_states(gpu, k):
+    """QED 2x2 PBC with matter (547 states, above the 512-state full-basis
+    limit): n_eigs near or above half the sector, where a basis capped at
+    N/2 vectors could not hold the wanted pairs, and n_eigs='full' -- every
+    eigenvalue against the oracle's dense spectrum (eigsh accepts such k)."""
+    orc = QEDOracle(1, False, [2, 2], [False, False])
+    mod = _model([2, 2], [False, False], False)
+    assert orc.N == 547
+    mod.build_Hamiltonian(0.7, 0.3)
+    w = np.linalg.eigvalsh(orc.hamiltonian(0.7, 0.3).toarray())
+    mod.diagonalize_Hamiltonian(k, "sparse")
+    got = mod.H.Nenergies * 4
+    kk = orc.N if k == "full" else k
+    assert got.size == kk
+    assert np.max(np.abs(got - w[:kk])) < 1e-10

@@ -152,3 +152,25 @@ def test_qed_c2_g_sweep_multi_term(gpu):
     for i, g in enumerate(gs):
         ref = lowest_eigs(orc.hamiltonian(g, 0.5), 4)
         assert np.max(np.abs(res[i][0] - ref)) < 1e-9
+
+
+def test_slot_bytes_match_device_allocation(gpu):
+    """edlgt_server_info's slot_bytes is the device memory one slot takes: the
+    first submission allocates it (free memory drops by slot_bytes, within the
+    allocator's rounding); the Gram result buffer is pinned host memory."""
+    import torch
+    from edlgt_amd import Server
+    _, mod = _zn([4, 3])
+    H2 = mod.build_multi_term()
+    srv = Server(H2, n_eigs=4, tol=1e-8, max_basis=64, max_slots=2)
+    info = srv.info()
+    assert info["allocated"] == 0
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    assert srv.submit(0, [1.0, 0.9], 5)
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    assert srv.info()["allocated"] == 1
+    used = free0 - free1
+    assert abs(used - info["slot_bytes"]) <= 4 << 20, (used, info["slot_bytes"])
+    srv.solve_all([])
