@@ -102,3 +102,34 @@ def test_plain_baseline_is_serial():
     done = [res[i]["done"] for i in calls]
     starts = [res[i]["admitted"] for i in calls]
     assert all(starts[i + 1] >= done[i] for i in range(len(calls) - 1))
+
+
+def test_cold_start_limit_until_first_result():
+    """With warm starts, a burst runs at most cold_limit requests of a problem
+    until one has finished (its cache is empty before that); afterwards the
+    slots fill up to max_slots.  Without warm starts there is no limit."""
+    for warm, limit, expect_first in ((64, 2, 2), (0, 2, 8)):
+        fakes = {}
+
+        def factory(p, _):
+            fakes[p] = FakeServer(1, 0, 8, None)
+            return fakes[p]
+
+        svc = SolveService({"a": None}, hbm_budget=10_000, server_factory=factory, warm_cache=warm,
+                           cold_limit=limit)
+        reqs = [dict(id=i, problem="a", coef=[1.0, 1.0 + (i % 3)], seed=i, arrival=0.0) for i in range(20)]
+        seen = []
+        orig = fakes["a"].submit
+
+        def submit(rid, coef, seed, orig=orig, f=fakes["a"]):
+            ok = orig(rid, coef, seed)
+            if ok:
+                seen.append((rid, f.info()["active"], svc._finished["a"]))
+            return ok
+
+        fakes["a"].submit = submit
+        res = svc.run(reqs)
+        assert sorted(res) == list(range(20))
+        before = [a for _, a, fin in seen if fin == 0]
+        assert max(before) == expect_first
+        assert fakes["a"].max_active == 8
