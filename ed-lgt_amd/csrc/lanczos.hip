@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -1365,16 +1366,34 @@ struct Server {
     double coef[16] = {0};
   };
   std::vector<Slot> slots;
+  // per step: device pointer arrays and coefficients of every active column
+  // (xs, ys, cf), staged in pinned host memory (hxp, hyp, hcp) so that a
+  // captured graph's copy nodes read the current step's values
   DevBuf xs, ys, cf, pack;
-  std::vector<const void*> hx;
-  std::vector<void*> hy;
-  std::vector<double> hc;
+  HostPinned hxp, hyp, hcp;
+  int max_cols = 0;
+  // the fused-SpMM chain of a step (uploads, coefficient tables, pack + SpMM
+  // per group) captured once per column count and replayed (the chain's
+  // launches are fixed by the column count, the matrix and the workspace)
+  struct StepGraph {
+    int ncol;
+    void* pack;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs;
+  bool use_graphs = false;
+  int64_t graph_launches = 0, graph_captures = 0;
   // warm-start cache
   int warm_cap = 0, warm_n = 0;
   DevBuf warm;
   std::vector<std::vector<double>> warm_coef;
 
   Server(Csr& hh, hipStream_t s, int vt) : h(hh), st(s), vec_type(vt) {}
+  ~Server() { drop_graphs(); }
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
 
   int setup(int max_slots_, int n_eigs, double tol_, int max_basis, int64_t max_mv_, int warm_cap_) {
     const int64_t N = h.n_rows;
@@ -1386,9 +1405,18 @@ struct Server {
     b = std::min(k, 4);
     EDLGT_TRY(choose_basis(N, k, max_basis, b, M));
     slots.resize(max_slots);
-    EDLGT_TRY(xs.alloc(kCoefMaxRhs * sizeof(void*)));
-    EDLGT_TRY(ys.alloc(kCoefMaxRhs * sizeof(void*)));
-    EDLGT_TRY(cf.alloc(kCoefMaxRhs * 16 * sizeof(double)));
+    max_cols = max_slots * b;
+    EDLGT_TRY(xs.alloc(max_cols * sizeof(void*)));
+    EDLGT_TRY(ys.alloc(max_cols * sizeof(void*)));
+    EDLGT_TRY(cf.alloc((size_t)(max_cols + kCoefMaxRhs) * 16 * sizeof(double)));
+    EDLGT_TRY(hxp.alloc(max_cols * sizeof(void*)));
+    EDLGT_TRY(hyp.alloc(max_cols * sizeof(void*)));
+    EDLGT_TRY(hcp.alloc((size_t)(max_cols + kCoefMaxRhs) * 16 * sizeof(double)));
+    std::memset(hcp.p, 0, hcp.bytes);
+    // graph capture needs a stream other than the legacy default one;
+    // EDLGT_SERVER_GRAPHS=0 launches the chain eagerly (same results)
+    const char* env = std::getenv("EDLGT_SERVER_GRAPHS");
+    use_graphs = st != nullptr && !(env && env[0] == '0');
     return EDLGT_OK;
   }
 
@@ -1587,32 +1615,67 @@ struct Server {
   // Fused SpMM over the active blocks of all active slots, in chunks of at most
   // 16 columns; each slot's product lands in V[:, nv:nv+b] (out_block).
   int fused_matvec() {
-    hx.clear();
-    hy.clear();
-    hc.clear();
+    auto* px = hxp.as<const void*>();
+    auto* py = hyp.as<void*>();
+    auto* pc = hcp.as<double>();
+    int ncol = 0;
     for (auto& sl : slots) {
       if (!sl.active) continue;
       for (int c = 0; c < b; ++c) {
-        hx.push_back(sl.sv->active_block() + (int64_t)c * h.n_rows);
-        hy.push_back(sl.sv->out_block() + (int64_t)c * h.n_rows);
-        for (int t = 0; t < 16; ++t) hc.push_back(sl.coef[t]);
+        px[ncol] = sl.sv->active_block() + (int64_t)c * h.n_rows;
+        py[ncol] = sl.sv->out_block() + (int64_t)c * h.n_rows;
+        for (int t = 0; t < 16; ++t) pc[(size_t)ncol * 16 + t] = sl.coef[t];
         sl.sv->mv64 += 1;
+        ++ncol;
       }
     }
+    if (ncol == 0) return EDLGT_OK;
     const size_t need = coef_pack_bytes(h, vec_type, kCoefMaxRhs);
-    if (pack.bytes < need) EDLGT_TRY(pack.alloc(need));  // also after the matrix was coded
-    hc.resize((hx.size() + kCoefMaxRhs) * 16, 0.0);
-    size_t done = 0;
-    while (done < hx.size()) {
-      const int nb = (int)std::min<size_t>(hx.size() - done, kCoefMaxRhs);
-      EDLGT_HIP(hipMemcpyAsync(xs.p, hx.data() + done, nb * sizeof(void*), hipMemcpyHostToDevice, st));
-      EDLGT_HIP(hipMemcpyAsync(ys.p, hy.data() + done, nb * sizeof(void*), hipMemcpyHostToDevice, st));
-      EDLGT_HIP(hipMemcpyAsync(cf.p, hc.data() + done * 16, kCoefMaxRhs * 16 * sizeof(double),
-                               hipMemcpyHostToDevice, st));
-      EDLGT_TRY(csr_spmm_coef(h, vec_type, nb, xs.p, ys.p, hx.data() + done, hy.data() + done, cf.as<double>(),
-                              pack.p, st));
-      EDLGT_HIP(hipStreamSynchronize(st));  // the pointer/coefficient buffers are reused
-      done += nb;
+    if (pack.bytes < need) {  // also after the matrix was coded
+      drop_graphs();
+      EDLGT_TRY(pack.alloc(need));
+    }
+    if (!use_graphs) {
+      EDLGT_TRY(issue_matvec(ncol));
+    } else {
+      hipGraphExec_t exec = nullptr;
+      for (auto& g : graphs)
+        if (g.ncol == ncol && g.pack == pack.p) exec = g.exec;
+      if (!exec) {
+        EDLGT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int s1 = issue_matvec(ncol);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (s1 != EDLGT_OK) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return s1;
+        }
+        EDLGT_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        EDLGT_HIP(ei);
+        graphs.push_back(StepGraph{ncol, pack.p, exec});
+        ++graph_captures;
+      }
+      EDLGT_HIP(hipGraphLaunch(exec, st));
+      ++graph_launches;
+    }
+    EDLGT_HIP(hipStreamSynchronize(st));  // the staging buffers are rewritten next step
+    return EDLGT_OK;
+  }
+
+  // The launches of one fused product over ncol columns (eager, or recorded
+  // into a graph): pointer and coefficient uploads from the pinned staging
+  // buffers, then one coefficient SpMM per chunk of at most 16 columns.
+  int issue_matvec(int ncol) {
+    EDLGT_HIP(hipMemcpyAsync(xs.p, hxp.p, ncol * sizeof(void*), hipMemcpyHostToDevice, st));
+    EDLGT_HIP(hipMemcpyAsync(ys.p, hyp.p, ncol * sizeof(void*), hipMemcpyHostToDevice, st));
+    EDLGT_HIP(hipMemcpyAsync(cf.p, hcp.p, (size_t)(ncol + kCoefMaxRhs) * 16 * sizeof(double),
+                             hipMemcpyHostToDevice, st));
+    for (int done = 0; done < ncol; done += kCoefMaxRhs) {
+      const int nb = std::min(ncol - done, kCoefMaxRhs);
+      EDLGT_TRY(csr_spmm_coef(h, vec_type, nb, xs.as<void*>() + done, ys.as<void*>() + done, nullptr, nullptr,
+                              cf.as<double>() + (size_t)done * 16, pack.p, st));
     }
     return EDLGT_OK;
   }

@@ -174,3 +174,33 @@ def test_slot_bytes_match_device_allocation(gpu):
     used = free0 - free1
     assert abs(used - info["slot_bytes"]) <= 4 << 20, (used, info["slot_bytes"])
     srv.solve_all([])
+
+
+@pytest.mark.parametrize("coded", [False, True])
+def test_server_graph_replay_equals_eager(gpu, coded, monkeypatch):
+    """The fused product of a served step is captured once per column count
+    into a HIP graph and replayed with the step's pointers and coefficients
+    from pinned staging memory; eager launches (EDLGT_SERVER_GRAPHS=0) give
+    the same results bit for bit.  Six slots of 3 columns: 18 columns in two
+    chunks, then fewer as requests finish, each count replayed many times
+    with the solvers' own kernels and synchronisations between replays.
+    (k = 3: the next level of Z3 3x3 is four-fold degenerate.)"""
+    import torch
+    from edlgt_amd import Server
+    orc, mod = _zn([3, 3])
+    H2 = mod.build_multi_term()
+    if coded:
+        assert H2.compress() > 0
+    reqs = [(i, [1.0, h], 300 + i) for i, h in enumerate([0.5, 0.8, 1.3, 1.9, 2.4, 0.7, 2.9])]
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("EDLGT_SERVER_GRAPHS", mode)
+        srv = Server(H2, n_eigs=3, tol=1e-10, max_basis=32, max_slots=6, stream=torch.cuda.Stream())
+        out[mode] = srv.solve_all(reqs)
+        del srv
+    for rid, coef, _ in reqs:
+        assert np.array_equal(out["1"][rid][0], out["0"][rid][0])
+        assert np.array_equal(out["1"][rid][1], out["0"][rid][1])
+        assert out["1"][rid][2] == out["0"][rid][2]
+        ref = lowest_eigs(orc.hamiltonian(*coef), 3)
+        assert np.max(np.abs(out["1"][rid][0] - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
