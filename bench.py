@@ -6,6 +6,7 @@
     python bench.py --workload cpu_qed4x3
     python bench.py --workload momentum
     python bench.py --workload matfree
+    python bench.py --workload serving_graphs
 
 Workload (default): 2D U(1) QED, 4x3 periodic lattice, staggered fermions,
 E in {-1,0,1}, g=1, m=0.5, Gauss-law + zero-charge sector (N = 28,904,615).
@@ -419,6 +420,57 @@ def serve_leg(args):
         speedup_capacity_vs_plain_service_rate=round(capacity["solves_per_s"] / plain_cap, 2))
 
 
+def serving_graphs_ab(args):
+    """The served Z3 stream with each step's fused-SpMM chain replayed from
+    HIP graphs against the same chain launched eagerly
+    (EDLGT_SERVER_GRAPHS=1 / 0, read when a server is created), two
+    repetitions: per-lattice capacity, whole-stream capacity, the 64-request
+    same-sample burst and the stream's p50 / p99."""
+    import os
+    import torch
+    from edlgt_amd import Zn_Model
+    from edlgt_amd.serving import SolveService, latency_summary, poisson_stream
+    lats = {"3x3": [3, 3], "4x3": [4, 3], "4x4": [4, 4]}
+    csrs = {}
+    for name, lv in lats.items():
+        csrs[name] = Zn_Model(3, lvals=lv, has_obc=[False, False], flux_sector=(0, 0)).build_multi_term()
+        csrs[name].compress()
+    reqs = poisson_stream(args.serve_requests, args.serve_rate, list(lats),
+                          lambda rng: [1.0, rng.uniform(0.3, 3.0)], seed=1)
+    ordered = sorted(reqs, key=lambda r: r["arrival"])
+    sample = ordered[::max(1, len(ordered) // args.plain_sample)][:args.plain_sample]
+
+    def service(problems):
+        return SolveService(problems, n_eigs=4, tol=1e-8, max_basis=args.serve_basis, max_slots=16, warm_cache=64)
+
+    runs = []
+    for rep in range(2):
+        for mode in ("1", "0"):
+            os.environ["EDLGT_SERVER_GRAPHS"] = mode
+            row = dict(graphs=mode == "1", rep=rep)
+            for p in lats:
+                sub = [dict(r, arrival=0.0) for r in reqs if r["problem"] == p]
+                svc = service({p: csrs[p]})
+                svc.run(sub)
+                row[f"{p}_alone_solves_per_s"] = round(len(sub) / svc.wall, 2)
+            svc = service(csrs)
+            res = svc.run([dict(r, arrival=0.0) for r in reqs])
+            row["capacity_solves_per_s"] = round(len(res) / svc.wall, 2)
+            svc = service(csrs)
+            res = svc.run([dict(r, arrival=0.0) for r in sample])
+            row["sample_burst_solves_per_s"] = round(len(res) / svc.wall, 2)
+            svc = service(csrs)
+            res = svc.run(reqs)
+            m = latency_summary(res, svc.wall)
+            row.update(stream_p50_ms=round(m["p50_ms"], 1), stream_p99_ms=round(m["p99_ms"], 1))
+            progress(json.dumps(row))
+            runs.append(row)
+            del svc
+            torch.cuda.empty_cache()
+    os.environ.pop("EDLGT_SERVER_GRAPHS", None)
+    print(json.dumps({"metric": "served Z3 stream, graph-replayed vs eager step chain", "runs": runs}), flush=True)
+
+
 def momentum_sectors(args):
     """Z3 pure gauge 4x4 PBC zero-flux sector (N = 14,348,907), J = 1, h = 0.7:
     full-sector build + lowest 4 eigenvalues vs the zero-momentum sector of
@@ -511,7 +563,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3", "cpu_qed4x3", "momentum", "matfree"])
+    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3", "cpu_qed4x3", "momentum", "matfree", "serving_graphs"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host-e0", action="store_true",
                     help="skip the host-core build + lowest eigenvalue of QED 4x3 (about two minutes)")
@@ -545,6 +597,8 @@ def main():
         return momentum_sectors(args)
     if args.workload == "matfree":
         return matrix_free(args)
+    if args.workload == "serving_graphs":
+        return serving_graphs_ab(args)
     import torch
     from edlgt_amd import QED_Model, PREC_MIXED
     from edlgt_amd import _lib as L
