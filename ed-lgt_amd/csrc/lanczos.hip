@@ -38,6 +38,17 @@ using cd = std::complex<double>;
 
 constexpr double kDgks = 0.5;  // squared-norm ratio above which one Gram-Schmidt pass suffices
 constexpr double kSkipLoss = 1e-14;  // measured loss of orthogonality left uncorrected
+// Mixed precision, one eigenpair: relative residual of the FP32-product
+// Lanczos phase, the relative CG residual of each Newton correction, and the
+// CG iterations allowed per correction (EDLGT_MIXED_TOL32 / EDLGT_MIXED_ETA
+// override the first two for tuning runs)
+constexpr double kMixedTol32 = 1e-3;
+constexpr double kMixedEta = 1e-4;
+constexpr int kCgMaxIter = 600;
+double env_double(const char* name, double dflt) {
+  const char* e = std::getenv(name);
+  return e && *e ? std::atof(e) : dflt;
+}
 
 // Per-thread Gram accumulator of the NW values a thread writes for one row
 // (g[i][c] += conj(y_i) y_c), reduced over the block at the end (fixed wave
@@ -211,6 +222,163 @@ __global__ void __launch_bounds__(kTB) three_term(const S* a, const S* b, S* w, 
     }
   }
   rg.flush(partial);
+}
+
+// ---------------------------------------------------------------- CG refinement
+// Conjugate gradients on the projected correction equation of one eigenpair
+// (P (H32 - theta) P) delta = -r, P = I - x x^H, with the FP32 matrix H32 (the
+// mixed-precision products).  The CG vectors (r, p, q, delta, g and a copy of
+// x) are stored in FP32 (L = float2 / float) and combined in FP64 registers:
+// the correction only needs a relative accuracy eta >= 1e-6, and halving
+// their bytes halves the streaming kernels' time.  Every scalar of an
+// iteration (beta, gamma, alpha) is reduced on the device from the previous
+// kernel's per-block partial sums, each block summing the kCgBlocks partials
+// in the same fixed order, so an iteration is three streaming kernels and one
+// SpMV with no host synchronisation; the host reads |r|^2 every few
+// iterations.
+constexpr int kCgBlocks = 1024;
+
+__device__ inline double2 sum_partials(const double2* __restrict__ part) {
+  __shared__ double2 red[kTB];
+  double2 v = make_double2(0.0, 0.0);
+  for (int i = threadIdx.x; i < kCgBlocks; i += kTB) v = add(v, part[i]);
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kTB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = add(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double2 out = red[0];
+  __syncthreads();
+  return out;
+}
+
+__device__ inline void store_block_sum(double2 v, double2* __restrict__ part) {
+  __shared__ double2 red[kTB / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 s = red[0];
+#pragma unroll
+    for (int q = 1; q < kTB / 64; ++q) s = add(s, red[q]);
+    part[blockIdx.x] = s;
+  }
+}
+
+__device__ inline float2 to_lo(double2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ inline float to_lo(double v) { return (float)v; }
+__device__ inline double2 to_hi(float2 v) { return make_double2((double)v.x, (double)v.y); }
+__device__ inline double to_hi(float v) { return (double)v; }
+__device__ inline double2 axpy(double a, double2 x, double2 y) { return make_double2(y.x + a * x.x, y.y + a * x.y); }
+__device__ inline double axpy(double a, double x, double y) { return y + a * x; }
+__device__ inline double2 cmul_s(double2 c, double2 x) { return mul(c, x); }  // complex scalar times x
+__device__ inline double cmul_s(double2 c, double x) { return c.x * x; }
+__device__ inline double nrm2(double2 v) { return v.x * v.x + v.y * v.y; }
+__device__ inline double nrm2(double v) { return v * v; }
+__device__ inline double2 sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double sub(double a, double b) { return a - b; }
+__device__ inline double2 neg(double2 a) { return make_double2(-a.x, -a.y); }
+__device__ inline double neg(double a) { return -a; }
+
+// partials of |hx - theta x|^2 (the FP64 residual norm, not stored)
+template <class S>
+__global__ void __launch_bounds__(kTB) resid_rr(const S* __restrict__ hx, const S* __restrict__ x, double theta,
+                                                int64_t n, double2* __restrict__ part) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTB)
+    acc += nrm2(axpy(-theta, x[i], hx[i]));
+  store_block_sum(make_double2(acc, 0.0), part);
+}
+
+// Start of one correction: g = y - theta x (y = H32 x_lo, the product the CG
+// operator projects with), r = -(hx - theta x) (the FP64 residual), delta = 0,
+// and the partials of |r|^2.
+template <class S, class L>
+__global__ void __launch_bounds__(kTB) cg_init(const S* __restrict__ y, const S* __restrict__ hx,
+                                               const S* __restrict__ x, double theta, int64_t n, L* __restrict__ g,
+                                               L* __restrict__ r, L* __restrict__ delta,
+                                               double2* __restrict__ part_rr) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTB) {
+    const S xv = x[i];
+    g[i] = to_lo(axpy(-theta, xv, y[i]));
+    const L rv = to_lo(neg(axpy(-theta, xv, hx[i])));
+    r[i] = rv;
+    delta[i] = to_lo(zero<S>());
+    acc += nrm2(to_hi(rv));
+  }
+  store_block_sum(make_double2(acc, 0.0), part_rr);
+}
+
+// p = r + beta p (beta = |r|^2 / |r_prev|^2, 0 on the first iteration) and the
+// partials of gamma = g^H p
+template <class L>
+__global__ void __launch_bounds__(kTB) cg_dir(const L* __restrict__ r, L* __restrict__ p, const L* __restrict__ g,
+                                              const double2* __restrict__ part_rr,
+                                              const double2* __restrict__ part_rr_prev, int first, int64_t n,
+                                              double2* __restrict__ part_g) {
+  double beta = 0.0;
+  if (!first) {
+    const double rr = sum_partials(part_rr).x, rp = sum_partials(part_rr_prev).x;
+    beta = rp > 0.0 ? rr / rp : 0.0;
+  }
+  double2 acc = make_double2(0.0, 0.0);
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTB) {
+    const L v = first ? r[i] : to_lo(axpy(beta, to_hi(p[i]), to_hi(r[i])));
+    p[i] = v;
+    acc = add(acc, to_c(cj_mul(to_hi(g[i]), to_hi(v))));
+  }
+  store_block_sum(acc, part_g);
+}
+
+// q = y - theta p - gamma x (y = H32 p) and the partials of p^H q
+template <class S, class L>
+__global__ void __launch_bounds__(kTB) cg_apply(const L* __restrict__ p, const S* __restrict__ y,
+                                                const L* __restrict__ x, L* __restrict__ q,
+                                                const double2* __restrict__ part_g, double theta, int64_t n,
+                                                double2* __restrict__ part_pq) {
+  const double2 gamma = sum_partials(part_g);
+  double2 acc = make_double2(0.0, 0.0);
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTB) {
+    const S pv = to_hi(p[i]);
+    const S qv = sub(axpy(-theta, pv, y[i]), cmul_s(gamma, to_hi(x[i])));
+    q[i] = to_lo(qv);
+    acc = add(acc, to_c(cj_mul(pv, qv)));
+  }
+  store_block_sum(acc, part_pq);
+}
+
+// alpha = |r|^2 / p^H q; delta += alpha p; r -= alpha q; partials of the new |r|^2
+// (alpha = 0 when p^H q is not positive: the host sees |r| stall and stops)
+template <class L>
+__global__ void __launch_bounds__(kTB) cg_step(L* __restrict__ delta, L* __restrict__ r, const L* __restrict__ p,
+                                               const L* __restrict__ q, const double2* __restrict__ part_rr,
+                                               const double2* __restrict__ part_pq, int64_t n,
+                                               double2* __restrict__ part_rr_out) {
+  const double rr = sum_partials(part_rr).x, pq = sum_partials(part_pq).x;
+  const double alpha = pq > 0.0 ? rr / pq : 0.0;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTB) {
+    delta[i] = to_lo(axpy(alpha, to_hi(p[i]), to_hi(delta[i])));
+    const L rv = to_lo(axpy(-alpha, to_hi(q[i]), to_hi(r[i])));
+    r[i] = rv;
+    acc += nrm2(to_hi(rv));
+  }
+  store_block_sum(make_double2(acc, 0.0), part_rr_out);
+}
+
+// x += delta and the partials of the new |x|^2
+template <class S, class L>
+__global__ void __launch_bounds__(kTB) cg_accept(S* __restrict__ x, const L* __restrict__ delta, int64_t n,
+                                                 double2* __restrict__ part) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTB) {
+    const S v = add(x[i], to_hi(delta[i]));
+    x[i] = v;
+    acc += nrm2(v);
+  }
+  store_block_sum(make_double2(acc, 0.0), part);
 }
 
 // fills x[i] with deterministic pseudo-random values in [-0.5, 0.5)
@@ -509,6 +677,12 @@ struct Solver {
   bool need_tmp = true;      // restart / Ritz buffer (not needed by Krylov stepping)
   bool want_vectors = true;  // form the Ritz vectors when a run finishes
   bool need_w = true;        // spare block (Krylov stepping, true residuals)
+  // Orthogonalise each new block against the last two blocks only (after the
+  // first step of a start or restart).  Cheap and accurate while no Ritz pair
+  // has converged far (residuals well above sqrt(unit roundoff) of the
+  // products): used for the loose FP32 phase of the mixed mode, whose result
+  // the FP64 refinement checks.
+  bool local_only = false;
 
   Solver(Csr& hh, hipStream_t s, int prec) : h(hh), st(s), precision(prec), N(hh.n_rows) {}
 
@@ -736,7 +910,12 @@ struct Solver {
         Dense G1;
         EDLGT_TRY(update(Vp + (int64_t)g0 * N, np, mG, X, nw, 1, &G1));
         std::vector<double> nref(n0);
-        if (nl > 0) {
+        if (nl > 0 && local_only) {
+          // no full pass: the loss of orthogonality to older columns is left
+          // to grow (plain block Lanczos)
+          for (int c = 0; c < nw; ++c) nref[c] = std::max(G1(c, c).real(), 0.0);
+          ++ortho_skipped;
+        } else if (nl > 0) {
           for (int c = 0; c < nw; ++c) nref[c] = std::max(G1(c, c).real(), 0.0);
           EDLGT_TRY(gram(Vp, nv, X, nw, G));
           // The Gram measures X's loss of orthogonality to the whole basis.
@@ -1134,6 +1313,116 @@ struct Solver {
   }
 
   // True residual norms ||H y_i - theta_i y_i|| in FP64 for Y = Tmp[:, 0:k].
+  // Inexact Newton refinement of the lowest eigenpair in place (x: unit
+  // FP64 vector, e.g. the Ritz vector of the FP32 phase), for the mixed
+  // precision mode: each outer step forms the FP64 residual r = H x - theta x
+  // (theta the Rayleigh quotient) and solves the projected correction equation
+  // (P (H32 - theta) P) delta = -r, delta orthogonal to x, by conjugate
+  // gradients with the FP32 matrix to a relative residual eta; then
+  // x <- (x + delta) / |x + delta|.  On the complement of x, H - theta is
+  // positive definite with smallest eigenvalue lambda_1 - lambda_0, so CG
+  // converges at the rate Lanczos does, but needs no basis and no
+  // reorthogonalisation.  ok = false (x left as a start vector for the FP64
+  // Lanczos phase) when the residual stops falling or the budget is spent.
+  int refine_newton(S* x, double tol, double eta, int64_t max_mv, double& theta, bool& ok) {
+    ok = false;
+    if (M + b < 5 || !shadow.p || h.mf) return EDLGT_OK;
+    using L = typename std::conditional<sizeof(S) == 16, float2, float>::type;
+    // FP64 columns 0-1 hold H x and the FP32 products; columns 2-4 each hold
+    // two FP32 vectors
+    S *hx = col(0), *y = col(1);
+    L* xlo = reinterpret_cast<L*>(col(2));
+    L *g = xlo + N, *r = reinterpret_cast<L*>(col(3)), *pp = r + N, *delta = reinterpret_cast<L*>(col(4)),
+      *q = delta + N;
+    double2* pg = part.as<double2>();
+    double2* ppq = pg + kCgBlocks;
+    double2* prr[2] = {pg + 2 * kCgBlocks, pg + 3 * kCgBlocks};
+    const int lt = sizeof(S) == 16 ? EDLGT_VALUE_C64 : EDLGT_VALUE_F32;
+    auto read_sum = [&](const double2* pr, double& out) -> int {
+      EDLGT_HIP(hipGetLastError());
+      hipLaunchKernelGGL((gram_finish<1>), dim3(1), dim3(kTB), 0, st, pr, kCgBlocks, 1, 1, 0,
+                         hgram.dev_as<double2>());
+      EDLGT_HIP(hipGetLastError());
+      Dense G;
+      EDLGT_TRY(fetch_gram(1, G));
+      out = G(0, 0).real();
+      return EDLGT_OK;
+    };
+    const int saved = precision;
+    double prev = INFINITY;
+    int64_t used = 0;
+    for (int outer = 0; outer < 6; ++outer) {
+      // FP64 residual of the current vector
+      precision = EDLGT_PREC_FP64;
+      EDLGT_TRY(matvec(x, hx, 1));
+      ++used;
+      Dense xh;
+      EDLGT_TRY(gram(x, 1, hx, 1, xh));
+      theta = xh(0, 0).real();
+      hipLaunchKernelGGL((resid_rr<S>), dim3(kCgBlocks), dim3(kTB), 0, st, hx, x, theta, N, prr[0]);
+      double rn = 0.0;
+      EDLGT_TRY(read_sum(prr[0], rn));
+      const double res = std::sqrt(std::max(rn, 0.0));
+      if (res <= tol * std::max(1.0, std::fabs(theta))) {
+        ok = true;
+        break;
+      }
+      if (!(res < 0.5 * prev) || used >= max_mv) break;
+      prev = res;
+      // y = H32 x_lo; g = y - theta x; r = -(H x - theta x); delta = 0
+      const int64_t ncomp = N * (int64_t)(sizeof(S) / 8);
+      EDLGT_TRY(convert_f64_to_f32(reinterpret_cast<const double*>(x), reinterpret_cast<float*>(xlo), ncomp, st));
+      EDLGT_TRY(csr_spmm(h, EDLGT_PREC_MIXED, lt, 1, xlo, N, y, N, st));
+      ++used;
+      ++mv32;
+      hipLaunchKernelGGL((cg_init<S, L>), dim3(kCgBlocks), dim3(kTB), 0, st, y, hx, x, theta, N, g, r, delta,
+                         prr[0]);
+      double rr0 = 0.0;
+      EDLGT_TRY(read_sum(prr[0], rr0));
+      const double target = eta * eta * rr0;
+      bool inner_ok = false;
+      double hist[4] = {rr0, rr0, rr0, rr0};  // |r|^2 at the last four checks
+      for (int it = 0; it < kCgMaxIter && used < max_mv; ++it) {
+        hipLaunchKernelGGL((cg_dir<L>), dim3(kCgBlocks), dim3(kTB), 0, st, r, pp, g, prr[it & 1],
+                           prr[(it + 1) & 1], it == 0 ? 1 : 0, N, pg);
+        EDLGT_HIP(hipGetLastError());
+        EDLGT_TRY(csr_spmm(h, EDLGT_PREC_MIXED, lt, 1, pp, N, y, N, st));
+        ++used;
+        ++mv32;
+        hipLaunchKernelGGL((cg_apply<S, L>), dim3(kCgBlocks), dim3(kTB), 0, st, pp, y, xlo, q, pg, theta, N, ppq);
+        hipLaunchKernelGGL((cg_step<L>), dim3(kCgBlocks), dim3(kTB), 0, st, delta, r, pp, q, prr[it & 1], ppq, N,
+                           prr[(it + 1) & 1]);
+        EDLGT_HIP(hipGetLastError());
+        if (it % 4 == 3) {
+          double rr = 0.0;
+          EDLGT_TRY(read_sum(prr[(it + 1) & 1], rr));
+          if (!(rr <= 1e8 * rr0)) break;  // diverging (or NaN): not positive definite
+          if (rr <= target) {
+            inner_ok = true;
+            break;
+          }
+          // stalled at the FP32 floor (|r| fell less than 2x over 16
+          // iterations): keep the correction if it reduced |r| 100-fold
+          if (it >= 15 && rr > 0.25 * hist[(it / 4) & 3]) {
+            inner_ok = rr <= 1e-4 * rr0;
+            break;
+          }
+          hist[(it / 4) & 3] = rr;
+        }
+      }
+      if (!inner_ok) break;
+      // x <- (x + delta) / |x + delta|
+      hipLaunchKernelGGL((cg_accept<S, L>), dim3(kCgBlocks), dim3(kTB), 0, st, x, delta, N, prr[0]);
+      double xn = 0.0;
+      EDLGT_TRY(read_sum(prr[0], xn));
+      Dense sc(1, 1);
+      sc(0, 0) = 1.0 / std::sqrt(std::max(xn, 1e-300));
+      EDLGT_TRY(transform(x, 1, sc, nullptr));
+    }
+    precision = saved;
+    return EDLGT_OK;
+  }
+
   int residuals(int k, const std::vector<double>& theta, std::vector<double>& out) {
     out.assign(k, 0.0);
     const int saved = precision;
@@ -1219,10 +1508,27 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
   if (b > 1) EDLGT_TRY(sv.randomize(sv.col(1), (int64_t)(b - 1) * N));
   std::vector<double> theta, est;
   bool conv = false;
-  if (precision == EDLGT_PREC_MIXED) {
+  bool refined = false;
+  if (precision == EDLGT_PREC_MIXED && k == 1) {
+    // one eigenpair: FP32-product Lanczos to a loose tolerance, then inexact
+    // Newton refinement (FP64 residuals, FP32 CG corrections); the FP64
+    // Lanczos phase below only if the refinement stalls
+    const double tol32 = std::max(tol, env_double("EDLGT_MIXED_TOL32", kMixedTol32));
+    sv.local_only = env_double("EDLGT_MIXED_LOCAL", 1.0) != 0.0;
+    EDLGT_TRY(sv.run(k, tol32, max_iter, theta, est, conv));
+    sv.local_only = false;
+    double th = 0.0;
+    EDLGT_TRY(sv.refine_newton(sv.Tmp.template as<S>(), tol, env_double("EDLGT_MIXED_ETA", kMixedEta),
+                               max_iter - sv.mv64 - sv.mv32, th, refined));
+    if (refined) {
+      theta.assign(1, th);
+      conv = true;
+    }
+  }
+  if (precision == EDLGT_PREC_MIXED && !refined) {
     // FP32 phase to a loose tolerance, then FP64 from its Ritz block
     const double tol32 = std::max(tol, 2e-6);
-    EDLGT_TRY(sv.run(k, tol32, max_iter, theta, est, conv));
+    if (k != 1) EDLGT_TRY(sv.run(k, tol32, max_iter, theta, est, conv));
     const int nk = std::min(k, b);
     EDLGT_HIP(hipMemcpyAsync(sv.col(0), sv.Tmp.p, (size_t)nk * N * sizeof(S), hipMemcpyDeviceToDevice, st));
     if (nk < b) EDLGT_TRY(sv.randomize(sv.col(nk), (int64_t)(b - nk) * N));
@@ -1235,7 +1541,7 @@ int lanczos_typed(Csr& h, int k, double tol, int max_basis, int64_t max_iter, in
     }
     sv.precision = EDLGT_PREC_FP64;
   }
-  EDLGT_TRY(sv.run(k, tol, max_iter, theta, est, conv));
+  if (!refined) EDLGT_TRY(sv.run(k, tol, max_iter, theta, est, conv));
   for (int i = 0; i < k; ++i) evals[i] = theta[i];
   if (evecs)
     EDLGT_HIP(hipMemcpyAsync(evecs, sv.Tmp.p, (size_t)k * N * sizeof(S), hipMemcpyDeviceToDevice, st));

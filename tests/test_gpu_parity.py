@@ -110,6 +110,32 @@ def test_mixed_precision_lanczos(gpu):
     assert mod.H.matvecs[1] > 0  # the FP32 phase ran
 
 
+@pytest.mark.parametrize("lvals,obc,pure,g,m", [
+    ([4, 2], [False, False], False, 1.0, 0.5),   # complex128 vectors
+    ([3, 3], [False, False], True, 1.0, None),   # float64 vectors
+    ([10], [True], False, 1.0, 1.0),
+])
+def test_mixed_precision_newton_one_eigenpair(gpu, lvals, obc, pure, g, m):
+    """One eigenpair in mixed precision: loose FP32 Lanczos, then inexact
+    Newton steps (FP64 residual, FP32 CG correction).  The lowest eigenvalue
+    within 1e-10 of the oracle, the FP64 residual within tol, and the
+    refinement path taken (a handful of FP64 products, not an FP64 Lanczos
+    phase)."""
+    from edlgt_amd import PREC_MIXED
+    orc = QEDOracle(1, pure, lvals, obc)
+    mod = _model(lvals, obc, pure)
+    mod.build_Hamiltonian(g, m)
+    ref = lowest_eigs(orc.hamiltonian(g, m), 1)
+    mod.diagonalize_Hamiltonian(1, "sparse", precision=PREC_MIXED)
+    e = float(mod.H.Nenergies[0] * np.prod(lvals))
+    assert abs(e - ref[0]) < 1e-10, (e, ref[0])
+    assert mod.H.residuals[0] <= 1e-9 * max(1.0, abs(e))
+    mv64, mv32 = (int(v) for v in mod.H.matvecs)
+    assert mv32 > 0 and mv64 <= 8, (mv64, mv32)
+    # the returned vector is a unit vector
+    assert abs(mod.H.Npsi[0].get_norm() - 1.0) < 1e-12
+
+
 def test_lookup_and_initial_state(gpu):
     mod = _model([10], [True], False)
     cfg = mod.sector_configs
