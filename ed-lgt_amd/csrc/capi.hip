@@ -499,6 +499,23 @@ int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t
                                  (hipStream_t)stream);
 }
 
+int edlgt_schmidt_gram(const void* psi, int32_t vec_type, int64_t n_states, const int64_t* pos, int32_t n_blocks,
+                       const int64_t* blocks, void* gram, int64_t gram_elems, void* stream) {
+  EDLGT_REQUIRE(n_states >= 0 && n_blocks >= 0 && gram_elems >= 0, "negative size");
+  EDLGT_REQUIRE(n_states == 0 || (psi && pos), "null psi or pos");
+  EDLGT_REQUIRE(n_blocks == 0 || (blocks && gram), "null blocks or gram");
+  EDLGT_REQUIRE(vec_type == EDLGT_VALUE_C128 || vec_type == EDLGT_VALUE_F64, "vec_type must be C128 or F64");
+  return schmidt_gram(psi, vec_type == EDLGT_VALUE_C128, n_states, pos, n_blocks, blocks, gram, gram_elems,
+                      (hipStream_t)stream);
+}
+
+int edlgt_hermitian_eigvals(void* gram, int64_t gram_elems, int32_t n_blocks, const int64_t* blocks, double* evals,
+                            int64_t n_evals, void* stream) {
+  EDLGT_REQUIRE(n_blocks >= 0 && gram_elems >= 0 && n_evals >= 0, "negative size");
+  EDLGT_REQUIRE(n_blocks == 0 || (gram && blocks && evals), "null argument");
+  return hermitian_eigvals(gram, gram_elems, n_blocks, blocks, evals, n_evals, (hipStream_t)stream);
+}
+
 int edlgt_random_uniform(void* out, int64_t n, int32_t value_type, const uint64_t* state, void* stream) {
   EDLGT_REQUIRE(n >= 0, "n must be >= 0");
   EDLGT_REQUIRE(n == 0 || (out && state), "null argument");

@@ -106,4 +106,12 @@ int basis_marginals_noalloc(const Basis& B, const void* psi, bool cplx, int d_ma
 // default_rng(...).random(n)); complex output carries it in the real part.
 int random_uniform(void* out, int64_t n, bool complex_values, const uint64_t state[4], hipStream_t st);
 
+// Schmidt blocks of a state (schmidt.hip): G_b = M_b M_b^H with psi scattered
+// to the packed blocks by pos; eigenvalues of Hermitian blocks by Jacobi.
+// blocks: host int64 [n_blocks][4] = (d, c, moff, goff).
+int schmidt_gram(const void* psi, bool cplx, int64_t n, const int64_t* pos, int n_blocks, const int64_t* blocks,
+                 void* gram, int64_t gram_elems, hipStream_t st);
+int hermitian_eigvals(void* gram, int64_t gram_elems, int n_blocks, const int64_t* blocks, double* evals,
+                      int64_t n_evals, hipStream_t st);
+
 }  // namespace edlgt

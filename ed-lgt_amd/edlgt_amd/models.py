@@ -46,23 +46,29 @@ class QMB_state:
         keep_indices (qmb_state.py:245-291), from the subsystem/environment
         maps of QuantumModel._get_partition.  The coefficient matrix
         psi[subsystem config, environment config] of a symmetry sector is block
-        diagonal up to permutations: the connected components of the
-        (subsystem, environment) pairs present in the sector are independent
-        blocks, so the Schmidt values are those of each block (one small dense
-        SVD each) -- the same spectrum as the reference's one dense SVD, at a
-        cost set by the largest block."""
-        lam = schmidt_weights(self.psi, partitions_dict[tuple(sorted(keep_indices))])
+        diagonal up to permutations; its squared singular values are computed
+        on the device block by block (schmidt_spectrum) -- the spectrum of the
+        reference's one dense SVD."""
+        part = partitions_dict[tuple(sorted(keep_indices))]
+        if min(part["subsys_dim"], part["env_dim"]) == 1:
+            return 0.0  # rank one, as in the reference
+        lam = schmidt_spectrum(self.psi_device, part)
         lam = lam[lam > 1e-10]
         return float(-np.sum(lam * np.log2(lam)))
 
     def reduced_density_matrix(self, keep_indices, partitions_dict):
         """Dense reduced density matrix of the sites keep_indices, psi_matrix
         psi_matrix^H over the sector's subsystem configurations
-        (qmb_state.py:211-243); host, small systems."""
+        (qmb_state.py:211-243): its diagonal blocks are formed on the device
+        and placed on the host."""
         part = partitions_dict[tuple(sorted(keep_indices))]
-        M = np.zeros((part["subsys_dim"], part["env_dim"]), dtype=np.complex128)
-        M[part["subsys_map"], part["env_map"]] = self.psi
-        return M @ M.conj().T
+        lay = schmidt_layout(part, "subsys")
+        G = schmidt_gram(self.psi_device, lay).cpu().numpy()
+        rho = np.zeros((part["subsys_dim"], part["subsys_dim"]), dtype=np.complex128)
+        for b, rows in enumerate(lay["row_ids"]):
+            d, _, _, goff = lay["blocks"][b]
+            rho[np.ix_(rows, rows)] = G[goff:goff + d * d].reshape(d, d)
+        return rho
 
     def get_state_configurations(self, threshold, sector_configs):
         """(configurations, probabilities) of the basis states with |psi|^2 >
@@ -75,34 +81,92 @@ class QMB_state:
         return sector_configs[idx], p[idx]
 
 
-def schmidt_weights(psi, part):
-    """Squared singular values of psi reshaped by the partition maps
-    (subsys_map, env_map), block by block over the connected components of
-    the bipartite graph of (subsystem, environment) pairs."""
+def schmidt_layout(part, rows="small"):
+    """Block layout of psi[subsystem, environment] for the device Schmidt
+    kernels (edlgt_schmidt_gram), cached in the partition dict.  The blocks
+    are the connected components of the bipartite graph of (subsystem,
+    environment) pairs present in the sector; block b is a dense d x c
+    row-major matrix whose rows are its subsystem configurations
+    (rows="subsys", for the reduced density matrix) or its smaller side
+    (rows="small", for the spectrum), ordered by configuration index.
+    Returns dict(blocks=int64 [nb, 4] (d, c, moff, goff), pos_host=int64
+    element position of every basis state in the packed blocks (copied to the
+    device on first use), row_ids=subsystem indices of each block's rows
+    (rows="subsys"), n_vals, m_elems, g_elems)."""
+    key = ("schmidt", rows)
+    if key in part:
+        return part[key]
     import scipy.sparse as sp
     from scipy.sparse.csgraph import connected_components
-    a, e = np.asarray(part["subsys_map"]), np.asarray(part["env_map"])
-    na, ne = part["subsys_dim"], part["env_dim"]
-    if "blocks" not in part:
-        g = sp.coo_matrix((np.ones(a.size), (a, na + e)), shape=(na + ne, na + ne))
-        nc, lab = connected_components(g, directed=False)
-        comp = lab[a]  # component of every basis state
-        order = np.argsort(comp, kind="stable")
-        bounds = np.searchsorted(comp[order], np.arange(nc + 1))
-        part["blocks"] = (order, bounds)
-    order, bounds = part["blocks"]
-    out = []
-    for c in range(len(bounds) - 1):
-        idx = order[bounds[c]:bounds[c + 1]]
-        if idx.size == 0:
-            continue
-        ua, ia = np.unique(a[idx], return_inverse=True)
-        ue, ie = np.unique(e[idx], return_inverse=True)
-        M = np.zeros((ua.size, ue.size), dtype=np.complex128)
-        M[ia.ravel(), ie.ravel()] = psi[idx]
-        out.append(np.linalg.svd(M, compute_uv=False) ** 2 if min(M.shape) > 1
-                   else np.array([np.sum(np.abs(M) ** 2)]))
-    return np.concatenate(out) if out else np.zeros(0)
+    a, e = np.asarray(part["subsys_map"], np.int64), np.asarray(part["env_map"], np.int64)
+    na, ne = int(part["subsys_dim"]), int(part["env_dim"])
+    g = sp.coo_matrix((np.ones(a.size), (a, na + e)), shape=(na + ne, na + ne))
+    nc, lab = connected_components(g, directed=False)
+    comp = lab[a].astype(np.int64)  # component of every basis state
+    # rank of each state's subsystem (environment) configuration inside its component
+    ka, ia = np.unique(comp * na + a, return_inverse=True)
+    ke, ie = np.unique(comp * ne + e, return_inverse=True)
+    ia, ie = ia.ravel().astype(np.int64), ie.ravel().astype(np.int64)
+    ca, ce = ka // na, ke // ne  # component of each unique key (sorted)
+    fa = np.searchsorted(ca, np.arange(nc + 1))
+    fe = np.searchsorted(ce, np.arange(nc + 1))
+    ra, re_ = ia - fa[comp], ie - fe[comp]
+    da, de = np.diff(fa), np.diff(fe)
+    used = np.flatnonzero(da > 0)  # components with basis states
+    if rows == "subsys":
+        sub_rows = np.ones(nc, bool)
+    elif rows == "small":
+        sub_rows = da <= de
+    else:
+        raise ValueError(f"rows must be 'small' or 'subsys', not {rows!r}")
+    d = np.where(sub_rows, da, de)[used]
+    c = np.where(sub_rows, de, da)[used]
+    moff = np.concatenate([[0], np.cumsum(d * c)[:-1]])
+    goff = np.concatenate([[0], np.cumsum(d * d)[:-1]])
+    bidx = np.full(nc, -1, np.int64)
+    bidx[used] = np.arange(used.size)
+    b = bidx[comp]
+    r = np.where(sub_rows[comp], ra, re_)
+    col = np.where(sub_rows[comp], re_, ra)
+    pos = moff[b] + r * c[b] + col
+    lay = dict(blocks=np.ascontiguousarray(np.stack([d, c, moff, goff], axis=1).astype(np.int64)),
+               pos_host=np.ascontiguousarray(pos, dtype=np.int64),
+               n_vals=int(d.sum()), m_elems=int((d * c).sum()), g_elems=int((d * d).sum()),
+               row_ids=[ka[fa[k]:fa[k + 1]] - k * na for k in used] if rows == "subsys" else None)
+    part[key] = lay
+    return lay
+
+
+def schmidt_gram(psi_device, lay):
+    """Device complex128 array of the Gram blocks G_b = M_b M_b^H of the
+    layout (edlgt_schmidt_gram)."""
+    import torch
+    psi = psi_device.reshape(-1)
+    if psi.dtype not in (torch.complex128, torch.float64):
+        psi = psi.to(torch.complex128)
+    psi = psi.contiguous()
+    if psi.numel() != lay["pos_host"].size:
+        raise ValueError(f"state has {psi.numel()} entries, the partition {lay['pos_host'].size}")
+    if "pos" not in lay:
+        lay["pos"] = torch.as_tensor(lay["pos_host"], device=psi.device)
+    vt = L.VALUE_C128 if psi.dtype == torch.complex128 else L.VALUE_F64
+    G = torch.empty(max(lay["g_elems"], 1), dtype=torch.complex128, device=psi.device)
+    blocks = lay["blocks"]
+    L.check(L.load().edlgt_schmidt_gram(L.ptr(psi), vt, psi.numel(), L.ptr(lay["pos"]), blocks.shape[0],
+                                        L.ptr(blocks), L.ptr(G), G.numel(), L.stream_handle()))
+    return G
+
+
+def schmidt_spectrum(psi_device, part):
+    """Squared Schmidt values of the state over the partition (every block's
+    Gram eigenvalues, edlgt_hermitian_eigvals; unsorted)."""
+    lay = schmidt_layout(part, "small")
+    G = schmidt_gram(psi_device, lay)
+    ev = np.empty(max(lay["n_vals"], 1))
+    blocks = lay["blocks"]
+    L.check(L.load().edlgt_hermitian_eigvals(L.ptr(G), G.numel(), blocks.shape[0], L.ptr(blocks), L.ptr(ev),
+                                             ev.size, L.stream_handle()))
+    return ev[:lay["n_vals"]]
 
 
 def diagonalize_density_matrix(rho):

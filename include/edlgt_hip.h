@@ -364,6 +364,36 @@ int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t
                                   double* out_dev, void* stream);
 
 /* ======================================================================
+ * Schmidt blocks of a state over a bipartition of the lattice.  Replaces the
+ * psi_matrix reshape of QMB_state._get_psi_matrix and the dense products and
+ * SVD of reduced_density_matrix / entanglement_entropy
+ * (edlgt/modeling/qmb_state.py:160-291; partition maps of
+ * QuantumModel._get_partition, edlgt/models/quantum_model.py:715-791).
+ * In a symmetry sector psi[subsystem config, environment config] is block
+ * diagonal up to permutations (connected components of the pairs present);
+ * the caller lays the blocks out: blocks (host int64 [n_blocks][4]) =
+ * (d, c, moff, goff), block b a dense d x c row-major matrix M_b at element
+ * offset moff of a packed array, and state i at element pos[i] (pos: device
+ * int64 [n_states]; a position outside the blocks is EDLGT_ERR_INVALID_ARG).
+ *
+ * edlgt_schmidt_gram: gram (device complex128, gram_elems elements) receives
+ * G_b = M_b M_b^H at offset goff (d x d row-major, exactly Hermitian).  psi:
+ * device, vec_type C128 or F64.  With rows = the subsystem side, G_b is the
+ * block of the reduced density matrix; with rows = the smaller side its
+ * eigenvalues are the squared Schmidt values.
+ *
+ * edlgt_hermitian_eigvals: eigenvalues of the Hermitian blocks of gram (same
+ * block table, only d and goff read) by cyclic Jacobi on the device;
+ * evals (host double, n_evals >= sum of d) receives block b's d values at
+ * offset d_0 + ... + d_{b-1}, unsorted.  gram is overwritten.
+ * EDLGT_ERR_NOT_CONVERGED after 60 sweeps.
+ * ====================================================================== */
+int edlgt_schmidt_gram(const void* psi, int32_t vec_type, int64_t n_states, const int64_t* pos, int32_t n_blocks,
+                       const int64_t* blocks, void* gram, int64_t gram_elems, void* stream);
+int edlgt_hermitian_eigvals(void* gram, int64_t gram_elems, int32_t n_blocks, const int64_t* blocks, double* evals,
+                            int64_t n_evals, void* stream);
+
+/* ======================================================================
  * Expectation values of term instances without building their matrices:
  *   out_host[2p], out_host[2p+1] = Re, Im of <psi| strength_p prod_f O_f |psi>
  * for every placement p (same tables / placement arrays as

@@ -213,9 +213,9 @@ def test_expval_oracle_equals_bruteforce():
 
 
 def test_blockwise_schmidt_weights_equal_dense_svd():
-    """The engine's block-by-block Schmidt decomposition (connected components
+    """The oracle's block-by-block Schmidt decomposition (connected components
     of the subsystem/environment pairs) equals one dense SVD, L=10 OBC."""
-    from edlgt_amd.models import schmidt_weights
+    from oracle.entropy import block_schmidt_weights as schmidt_weights
     orc = SU2Oracle(False, [10], [True], sectors=[10])
     rng = np.random.default_rng(9)
     psi = rng.standard_normal(orc.N) + 1j * rng.standard_normal(orc.N)
