@@ -626,6 +626,26 @@ class QuantumModel:
         return state
 
 
+def _integrated_operators():
+    """Pauli matrices in the reference's normalisation (spin_operators.py:62-72:
+    Sz = diag(1, -1), Sp = |0><1|) with N = (Sz + I)/2 and N_zero = I - N
+    (QED_operators.py:541-545)."""
+    I = np.eye(2)
+    Sz = np.diag([1.0, -1.0])
+    Sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    Sm = Sp.T.copy()
+    N = 0.5 * (Sz + I)
+    return {"I": I, "Sz": Sz, "Sp": Sp, "Sm": Sm, "Sx": Sp + Sm, "Sy": -1j * (Sp - Sm), "N": N, "N_zero": I - N}
+
+
+class _PauliSite:
+    """Two-state site of integrated QED (no gauge states)."""
+    values = {"site": np.arange(2)[:, None]}
+
+    def dim(self, label):
+        return 2
+
+
 class QED_Model(QuantumModel):
     """U(1) lattice gauge theory with optional staggered fermions
     (edlgt/models/QED_model.py:20-394, 719-776)."""
@@ -637,10 +657,21 @@ class QED_Model(QuantumModel):
         # charge per site; all zero is the same as none
         self.bg_list = None
         self.background = 0
+        bg = None
         if bg_list is not None:
             bg = np.asarray(bg_list, dtype=float)
             if bg.shape != (self.n_sites,):
                 raise ValueError(f"len(bg_list) must match n_sites = {self.n_sites}: got {bg.shape}")
+        if isinstance(spin, str):
+            # gauge fields integrated out by Gauss's law (QED_model.py:64-82)
+            if spin != "integrated":
+                raise ValueError(f"spin must be integer or 'integrated': got {spin}")
+            if pure_theory or self.dim != 1:
+                raise ValueError("QED with integrated gauge fields is only in 1D with matter")
+            self.bg_list = None if bg is None else bg.tolist()
+            self._init_integrated()
+            return
+        if bg is not None:
             self.background = int(np.max(np.abs(bg)))
             if self.background != 0:
                 self.bg_list = bg.tolist()
@@ -697,6 +728,146 @@ class QED_Model(QuantumModel):
         logger.info(f"SEC DIM: {self.basis.n_states}")
         self.default_params()
 
+    def _init_integrated(self):
+        """1D QED with the gauge fields integrated out: two states per site
+        (Pauli basis, index 0 = occupied, QED_operators.py:541-545 with
+        spin_operators.py:62-72), sector N_tot = n_sites // 2, no link
+        constraints (QED_model.py:64-82, 114-140)."""
+        self.spin = "integrated"
+        self.pure_theory = False
+        self.plaq_basis = False
+        self.link_symmetries = False
+        self.staggered_basis = False
+        ops = _integrated_operators()
+        site = _PauliSite()
+        self._project(site, lambda l: ops, ["site"] * self.n_sites)
+        glob = (np.real(np.einsum("sii->si", self.ops["N"]))[None], np.array([float(self.n_sites // 2)]), "U")
+        self.basis = SectorBasis(self.loc_dims, glob=glob)
+        logger.info(f"SEC DIM: {self.basis.n_states}")
+        self.default_params()
+
+    def _integrated_static_charges(self):
+        """QED_model.py:494-503."""
+        if self.bg_list is None:
+            return np.zeros(self.n_sites)
+        return np.asarray(self.bg_list, dtype=float)
+
+    def _build_integrated(self, g, m, theta, dtype_mode):
+        """build_integrated_Hamiltonian (QED_model.py:396-492): staggered mass,
+        hopping S+ S- with t = -i/2, and the electric energy
+        E sum_{n<L-1} [sum_{k<=n} (q_k + (Sz_k + (-1)^k)/2)]^2 expanded into
+        one-body Sz terms h_i, two-body Sz_i Sz_j terms and a constant."""
+        mode = dtype_mode
+        if mode in ("auto", None):
+            mode = "complex"
+        real = self._dtype_real(theta, mode)
+        if m is None:
+            raise ValueError("m is required with matter fields")
+        self.QED_Hamiltonian_couplings(g, m, theta)
+        self._new_program()
+        self._integrated_matter(self.coeffs)
+        self._integrated_electric(self.coeffs["E"])
+        return self._build_program(real)
+
+    def _integrated_matter(self, c):
+        """Staggered mass and hopping placements (QED_model.py:409-427)."""
+        lat = self.lattice
+        even = lat.even_mask()
+        self._add_local("N", c["m_even"], [i for i in range(self.n_sites) if even[i]])
+        self._add_local("N", c["m_odd"], [i for i in range(self.n_sites) if not even[i]])
+        self._add_nbody(["Sp", "Sm"], lat.bonds("x"), c["t"], True)
+
+    def _integrated_electric(self, E):
+        """Electric energy placements at coupling E (QED_model.py:428-491):
+        h_i Sz_i with h_i = E sum_{i<=n<L-1} A_n (A the prefix sums of
+        q_k + (-1)^k / 2), J_ij Sz_i Sz_j with J_ij = E (L-1-j) / 2 for
+        i < j < L-1, and the constant E [sum_n A_n^2 + n_links (n_links+1) / 8]
+        as a diagonal term on site 0."""
+        n = self.n_sites
+        n_links = n - 1
+        q = self._integrated_static_charges()
+        offsets = q + 0.5 * np.array([1.0 if i % 2 == 0 else -1.0 for i in range(n)])
+        prefix = np.cumsum(offsets)
+        for k in range(n_links):
+            h = E * np.sum(prefix[k:n_links])
+            if abs(h) >= 1e-14:
+                self._add_local("Sz", h, [k])
+        for j in range(1, n_links):
+            w = 0.5 * E * (n - 1 - j)
+            if abs(w) > 1e-14:
+                self._add_nbody(["Sz", "Sz"], [(i, j) for i in range(j)], w, False)
+        if n_links > 0:
+            const = E * (np.sum(prefix[:n_links] ** 2) + 0.25 * 0.5 * n_links * (n_links + 1))
+            if abs(const) > 1e-14:
+                self._add_local("I", float(const), [0])
+
+    def reconstruct_integrated_E2_from_N(self, density_obs_name="N", density_corr_obs_name="N_N",
+                                         state_index=None, dynamics=False, compute_density_corr=True,
+                                         print_values=True):
+        """Link-resolved <E^2> of integrated 1D QED from the measured density
+        (QED_model.py:505-652): E_n = sum_{k<=n} [q_k + N_k + ((-1)^k - 1)/2],
+        so <E_n^2> = B_n^2 + 2 B_n sum_{k<=n} <N_k> + sum_{k,l<=n} <N_k N_l>
+        with B_n = sum_{k<=n} [q_k + ((-1)^k - 1)/2].  The correlator is read
+        from res or measured on the device."""
+        if self.spin != "integrated" or self.dim != 1 or self.pure_theory:
+            raise ValueError("Integrated E2 reconstruction is defined only for 1D QED with matter")
+        if density_obs_name not in self.res:
+            raise KeyError(f"Missing '{density_obs_name}' in self.res. Measure local density first.")
+        dens = np.asarray(self.res[density_obs_name], dtype=float)
+        if dens.shape != (self.n_sites,):
+            raise ValueError(f"Expected {density_obs_name}.shape=({self.n_sites},) got {dens.shape}.")
+        n_links = self.n_sites - 1
+        if n_links <= 0:
+            self.res["E2"] = np.zeros(0)
+            self.res["E2_avg"] = 0.0
+            return self.res["E2"]
+        if density_corr_obs_name in self.res:
+            corr = np.asarray(self.res[density_corr_obs_name], dtype=float)
+        else:
+            if not compute_density_corr:
+                raise KeyError(f"Missing '{density_corr_obs_name}' in self.res. "
+                               "Measure ['N','N_N'] or set compute_density_corr=True.")
+            states = self.H.psi_time if dynamics else getattr(self.H, "Npsi", None)
+            if states is None:
+                raise ValueError("Cannot compute density correlator: no states available.")
+            if state_index is None:
+                if len(states) != 1:
+                    raise ValueError("state_index is required to compute density correlator "
+                                     "when several states are available.")
+                state_index = 0
+            psi_dev = self._sector_state(states[state_index].psi_device)
+            saved = getattr(self, "_obs_groups", {})
+            pairs = [(i, j) for i in range(self.n_sites) for j in range(i + 1, self.n_sites)]
+            self._obs_groups = {"N_N": ("twobody", ["N", "N"], pairs)}
+            try:
+                v = np.real(self._group_expvals(psi_dev)["N_N"])
+            finally:
+                self._obs_groups = saved
+            corr = np.zeros((self.n_sites, self.n_sites))
+            for (i, j), x in zip(pairs, v):
+                corr[i, j] = corr[j, i] = x
+            self.res[density_corr_obs_name] = corr
+        if corr.shape != (self.n_sites, self.n_sites):
+            raise ValueError(f"Expected '{density_corr_obs_name}' shape ({self.n_sites}, {self.n_sites}), "
+                             f"got {corr.shape}.")
+        corr = np.array(corr, dtype=float, copy=True)
+        np.fill_diagonal(corr, dens)  # N_k^2 = N_k for fermions
+        q = self._integrated_static_charges()
+        k = np.arange(self.n_sites, dtype=float)
+        b = np.cumsum(q + 0.5 * ((-1.0) ** k - 1.0))
+        pd = np.cumsum(dens)
+        pc = np.diag(corr.cumsum(axis=0).cumsum(axis=1))
+        e2 = b[:n_links] ** 2 + 2.0 * b[:n_links] * pd[:n_links] + pc[:n_links]
+        self.res["E2"] = e2
+        self.res["E2_avg"] = float(np.mean(e2))
+        if print_values:
+            logger.info("----------------------------------------------------")
+            logger.info("E2 ")
+            for i in range(n_links):
+                logger.info(f"{(i,)} {format(e2[i], '.16f')}")
+            logger.info(f"{format(self.res['E2_avg'], '.16f')}")
+        return e2
+
     def QED_Hamiltonian_couplings(self, g, m=None, theta=0.0):
         """QED_model.py:719-776 (dressed-site normalisation)."""
         c = {"g": g, "E": g / 2, "B": -1 / (2 * g), "theta": -theta * g}
@@ -718,7 +889,10 @@ class QED_Model(QuantumModel):
         return mode == "real"
 
     def build_Hamiltonian(self, g, m=None, theta=0.0, dtype_mode="auto"):
-        """build_truncated_Hamiltonian term order (QED_model.py:235-394)."""
+        """build_truncated_Hamiltonian term order (QED_model.py:235-394), or
+        build_integrated_Hamiltonian for spin='integrated'."""
+        if self.spin == "integrated":
+            return self._build_integrated(g, m, theta, dtype_mode)
         real = self._dtype_real(theta, dtype_mode)
         self.QED_Hamiltonian_couplings(g, m, theta)
         c = self.coeffs
@@ -751,6 +925,122 @@ class QED_Model(QuantumModel):
                     self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
         return self._build_program(real)
 
+    def overlap_QMB_state(self, name):
+        """Named reference configurations (QED_model.py:778-899), one local-state
+        index per site, for get_qmb_state_from_configs:
+          1D with matter, spin 1: "V" (bare vacuum: every link E = 0, even sites
+            empty, odd sites filled; alias "PV") and "meson_center" (the central
+            (even, odd) pair turned into a meson: the even site filled with its
+            right link at E = +1); the same two for spin='integrated' as Pauli
+            occupations;
+          3D, 2x2x2 with static charges bg_list = [-1, 0, 0, 0, 0, 0, 0, 1],
+            spin 1, 2 or 3: the minimal flux strings "S1".."S6"."""
+        if self.dim == 1 and not self.pure_theory:
+            key = name.strip().lower().replace(" ", "_")
+            if key == "pv":
+                key = "v"
+            if key in ("v", "meson_center") and self.spin == "integrated":
+                # occupations in the Pauli basis (index 0 = occupied): even sites
+                # empty, odd sites filled; the meson flips the central (even, odd)
+                # pair (QED_model.py:803-818)
+                n = self.n_sites
+                cfg = np.array([1 if i % 2 == 0 else 0 for i in range(n)], dtype=np.int32)
+                if key == "meson_center":
+                    even = n // 2
+                    if even % 2 == 1:
+                        even -= 1
+                    even = max(0, min(even, n - 2))
+                    cfg[even], cfg[even + 1] = 0, 1
+                return cfg
+            if key in ("v", "meson_center"):
+                if self.spin != 1:
+                    raise NotImplementedError("1D labels 'V'/'meson_center' are implemented for "
+                                              "spin='integrated' or truncated spin=1.")
+                n = self.n_sites
+                # spin-1 1D states in product order over (n, E_mx, E_px): index 1
+                # of an even site is (0, 0, 0), index 3 of an odd site (1, 0, 0);
+                # on an OBC edge the kept states are those with a zero outer link
+                cfg = np.array([1 if i % 2 == 0 else 3 for i in range(n)], dtype=np.int32)
+                if all(self.has_obc):
+                    cfg[0], cfg[n - 1] = 0, 1
+                if key == "meson_center":
+                    even = n // 2
+                    if even % 2 == 1:
+                        even -= 1
+                    even = max(0, min(even, n - 2))
+                    cfg[even], cfg[even + 1] = 4, 1
+                return cfg
+        if self.dim == 3 and self.bg_list == [-1, 0, 0, 0, 0, 0, 0, 1]:
+            if self.spin not in _QED3D_STRINGS:
+                raise NotImplementedError("3D string states are tabulated for spin 1, 2 and 3")
+            if name not in _QED3D_STRINGS[self.spin]:
+                raise ValueError(f"Unknown String name: {name}")
+            return np.array(_QED3D_STRINGS[self.spin][name])
+        raise NotImplementedError("Only 3D QED states are supported")
+
+    def multi_term_program(self, m=None):
+        """(tables, placements with term ids) of H(g) = (g/2) H_E + (-1/(2g)) H_B + H_0
+        for a g sweep at fixed m: term 0 = E2, term 1 = xy plaquettes (+ h.c.; the
+        3D planes too), term 2 = mass + hopping at mass m (empty without matter).
+        Coefficients of a coupling g: QED_Model.multi_term_coefficients(g).
+        spin='integrated': term 0 = the electric energy at E = 1, term 1
+        empty, term 2 = mass + hopping."""
+        lat = self.lattice
+        self._new_program()
+        if self.spin == "integrated":
+            if m is None:
+                raise ValueError("m is required with matter fields")
+            self._integrated_electric(1.0)
+            for p in self._placements:
+                p["term"] = 0
+            k0 = len(self._placements)
+            self.QED_Hamiltonian_couplings(1.0, m)
+            self._integrated_matter(self.coeffs)
+            for p in self._placements[k0:]:
+                p["term"] = 2
+            return np.stack(self._tables), list(self._placements)
+        self._add_local("E2", 1.0)
+        for p in self._placements:
+            p["term"] = 0
+        k0 = len(self._placements)
+        if self.dim > 1:
+            self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], lat.plaquettes(("x", "y")), 1.0, True)
+        if self.dim == 3:
+            self._add_nbody(["C_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], lat.plaquettes(("x", "z")), 1.0, True)
+            self._add_nbody(["C_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], lat.plaquettes(("y", "z")), 1.0, True)
+        for p in self._placements[k0:]:
+            p["term"] = 1
+        k1 = len(self._placements)
+        if not self.pure_theory:
+            if m is None:
+                raise ValueError("m is required with matter fields")
+            self.QED_Hamiltonian_couplings(1.0, m)
+            c = self.coeffs
+            even = lat.even_mask()
+            self._add_local("N", c["m_even"], [i for i in range(self.n_sites) if even[i]])
+            self._add_local("N", c["m_odd"], [i for i in range(self.n_sites) if not even[i]])
+            for d in self.directions:
+                for stag in ("even", "odd"):
+                    bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
+                    self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
+        for p in self._placements[k1:]:
+            p["term"] = 2
+        return np.stack(self._tables), list(self._placements)
+
+
+    def build_multi_term(self, m=None, dtype_mode="auto"):
+        """Assembles the three coupling-independent parts of H(g) once on one
+        pattern; the matrix at coupling g is sum_t multi_term_coefficients(g)[t] H_t
+        (products: CSRMatrix.matvec_coef, Server)."""
+        tables, placements = self.multi_term_program(m)
+        real = self._dtype_real(0.0, dtype_mode)
+        return CSRMatrix(self.basis, tables, placements, L.VALUE_F64 if real else L.VALUE_C128, n_terms=3)
+
+
+    def multi_term_coefficients(self, g):
+        """Term coefficients of H(g) (QED_Hamiltonian_couplings, QED_model.py:719-776)."""
+        return [g / 2, -1 / (2 * g), 1.0]
+
 
 # Minimal string configurations of 3D QED (2x2x2 OBC, matter, static charges
 # -1 at site 0 and +1 at site 7) per link truncation: local-state indices,
@@ -766,104 +1056,6 @@ _QED3D_STRINGS = {
         "S3": [21, 54, 54, 18, 48, 18, 11, 51], "S4": [20, 54, 48, 18, 54, 18, 17, 51],
         "S5": [14, 48, 54, 18, 54, 17, 18, 57], "S6": [14, 47, 54, 17, 54, 18, 18, 58]},
 }
-
-
-def _qed_overlap_QMB_state(self, name):
-    """Named reference configurations (QED_model.py:778-899), one local-state
-    index per site, for get_qmb_state_from_configs:
-      1D with matter, spin 1: "V" (bare vacuum: every link E = 0, even sites
-        empty, odd sites filled; alias "PV") and "meson_center" (the central
-        (even, odd) pair turned into a meson: the even site filled with its
-        right link at E = +1);
-      3D, 2x2x2 with static charges bg_list = [-1, 0, 0, 0, 0, 0, 0, 1],
-        spin 1, 2 or 3: the minimal flux strings "S1".."S6"."""
-    if self.dim == 1 and not self.pure_theory:
-        key = name.strip().lower().replace(" ", "_")
-        if key == "pv":
-            key = "v"
-        if key in ("v", "meson_center"):
-            if self.spin != 1:
-                raise NotImplementedError("1D labels 'V'/'meson_center' are implemented for "
-                                          "spin='integrated' or truncated spin=1.")
-            n = self.n_sites
-            # spin-1 1D states in product order over (n, E_mx, E_px): index 1
-            # of an even site is (0, 0, 0), index 3 of an odd site (1, 0, 0);
-            # on an OBC edge the kept states are those with a zero outer link
-            cfg = np.array([1 if i % 2 == 0 else 3 for i in range(n)], dtype=np.int32)
-            if all(self.has_obc):
-                cfg[0], cfg[n - 1] = 0, 1
-            if key == "meson_center":
-                even = n // 2
-                if even % 2 == 1:
-                    even -= 1
-                even = max(0, min(even, n - 2))
-                cfg[even], cfg[even + 1] = 4, 1
-            return cfg
-    if self.dim == 3 and self.bg_list == [-1, 0, 0, 0, 0, 0, 0, 1]:
-        if self.spin not in _QED3D_STRINGS:
-            raise NotImplementedError("3D string states are tabulated for spin 1, 2 and 3")
-        if name not in _QED3D_STRINGS[self.spin]:
-            raise ValueError(f"Unknown String name: {name}")
-        return np.array(_QED3D_STRINGS[self.spin][name])
-    raise NotImplementedError("Only 3D QED states are supported")
-
-
-QED_Model.overlap_QMB_state = _qed_overlap_QMB_state
-
-
-def _qed_multi_term(self, m=None):
-    """(tables, placements with term ids) of H(g) = (g/2) H_E + (-1/(2g)) H_B + H_0
-    for a g sweep at fixed m: term 0 = E2, term 1 = xy plaquettes (+ h.c.; the
-    3D planes too), term 2 = mass + hopping at mass m (empty without matter).
-    Coefficients of a coupling g: QED_Model.multi_term_coefficients(g)."""
-    lat = self.lattice
-    self._new_program()
-    self._add_local("E2", 1.0)
-    for p in self._placements:
-        p["term"] = 0
-    k0 = len(self._placements)
-    if self.dim > 1:
-        self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], lat.plaquettes(("x", "y")), 1.0, True)
-    if self.dim == 3:
-        self._add_nbody(["C_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], lat.plaquettes(("x", "z")), 1.0, True)
-        self._add_nbody(["C_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], lat.plaquettes(("y", "z")), 1.0, True)
-    for p in self._placements[k0:]:
-        p["term"] = 1
-    k1 = len(self._placements)
-    if not self.pure_theory:
-        if m is None:
-            raise ValueError("m is required with matter fields")
-        self.QED_Hamiltonian_couplings(1.0, m)
-        c = self.coeffs
-        even = lat.even_mask()
-        self._add_local("N", c["m_even"], [i for i in range(self.n_sites) if even[i]])
-        self._add_local("N", c["m_odd"], [i for i in range(self.n_sites) if not even[i]])
-        for d in self.directions:
-            for stag in ("even", "odd"):
-                bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
-                self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
-    for p in self._placements[k1:]:
-        p["term"] = 2
-    return np.stack(self._tables), list(self._placements)
-
-
-def _qed_build_multi_term(self, m=None, dtype_mode="auto"):
-    """Assembles the three coupling-independent parts of H(g) once on one
-    pattern; the matrix at coupling g is sum_t multi_term_coefficients(g)[t] H_t
-    (products: CSRMatrix.matvec_coef, Server)."""
-    tables, placements = self.multi_term_program(m)
-    real = self._dtype_real(0.0, dtype_mode)
-    return CSRMatrix(self.basis, tables, placements, L.VALUE_F64 if real else L.VALUE_C128, n_terms=3)
-
-
-def _qed_coefficients(self, g):
-    """Term coefficients of H(g) (QED_Hamiltonian_couplings, QED_model.py:719-776)."""
-    return [g / 2, -1 / (2 * g), 1.0]
-
-
-QED_Model.multi_term_program = _qed_multi_term
-QED_Model.build_multi_term = _qed_build_multi_term
-QED_Model.multi_term_coefficients = _qed_coefficients
 
 
 class Zn_Model(QuantumModel):

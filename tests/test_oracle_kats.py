@@ -119,3 +119,32 @@ def test_label_configs_are_vacuum_and_meson():
     assert e2(v) == 0.0 and n(v) == [0.0, 1.0] * 5
     assert e2(mc) == 1.0  # two half-links at E = +1, E2 = 1/2 sum of halves
     assert np.sum(np.array(n(mc)) != np.array(n(v))) == 2
+
+
+def test_integrated_qed_oracle_pins():
+    """oracle/integrated.py: free fermions at g = 0 (sum of the L/2 lowest
+    one-particle energies) and Gauss's law on product states."""
+    from oracle import integrated as iq
+    for L, m in ((4, 0.7), (6, 0.3), (8, 1.1)):
+        H = iq.hamiltonian(L, 0.0, m, np.zeros(L))
+        assert abs(np.linalg.eigvalsh(H.toarray())[0] - iq.free_fermion_e0(L, m)) < 1e-12
+        assert np.max(np.abs((H - H.conj().T).toarray())) < 1e-15
+    L, g, m = 6, 1.3, 0.4
+    cfg = iq.sector_configs(L)
+    vac = np.array([1 if i % 2 == 0 else 0 for i in range(L)])
+    meson = vac.copy()
+    meson[2], meson[3] = 0, 1  # central (even, odd) pair flipped
+    rows = {tuple(c): r for r, c in enumerate(cfg.tolist())}
+    mass = lambda c: sum((m if i % 2 == 0 else -m) * (c[i] == 0) for i in range(L))
+    Hg = iq.hamiltonian(L, g, m, np.zeros(L)).toarray()
+    assert abs(Hg[rows[tuple(vac)], rows[tuple(vac)]] - mass(vac)) < 1e-13  # no flux
+    # meson: one unit of flux on the link between its two sites
+    assert abs(Hg[rows[tuple(meson)], rows[tuple(meson)]] - (mass(meson) + g / 2)) < 1e-13
+    # a static charge +1 on site 0 and -1 on the last site: flux 1 on every link
+    q = np.zeros(L)
+    q[0], q[-1] = 1.0, -1.0
+    Hq = iq.hamiltonian(L, g, m, q).toarray()
+    assert abs(Hq[rows[tuple(vac)], rows[tuple(vac)]] - (mass(vac) + g / 2 * (L - 1))) < 1e-13
+    psi = np.zeros(len(cfg))
+    psi[rows[tuple(meson)]] = 1.0
+    assert np.allclose(iq.link_e2(L, np.zeros(L), psi), [0, 0, 1, 0, 0])
