@@ -675,14 +675,21 @@ class QED_Model(QuantumModel):
             self.background = int(np.max(np.abs(bg)))
             if self.background != 0:
                 self.bg_list = bg.tolist()
-        if plaq_basis or not link_symmetries or get_only_bulk:
-            raise NotImplementedError("only the dressed-site basis with link symmetries is supported")
+        if plaq_basis:
+            raise NotImplementedError("the plaquette basis of the pure 2D theory is not built; "
+                                      "use the dressed-site basis")
+        if get_only_bulk and any(self.has_obc):
+            # only the bulk site bases exist: border sites have none to project on
+            raise ValueError("get_only_bulk=True keeps no border bases: it needs periodic boundaries")
         if not isinstance(spin, (int, np.integer)) or spin < 1:
             raise NotImplementedError("spin must be a positive integer truncation")
         self.spin = int(spin)
         self.pure_theory = bool(pure_theory)
         self.staggered_basis = not self.pure_theory
-        self.link_symmetries = True
+        # link_symmetries=False: the sector drops the link constraints and H
+        # carries the penalty 100 E2 - 100 E_p E_m instead (QED_model.py:131-141, 381-393)
+        self.link_symmetries = bool(link_symmetries)
+        self.get_only_bulk = bool(get_only_bulk)
         charges = [0] * self.n_sites
         if self.bg_list is not None:
             charges = [int(round(q)) for q in self.bg_list]
@@ -724,7 +731,8 @@ class QED_Model(QuantumModel):
             # electric flux through the three coordinate planes at the origin
             nbody = (np.stack([diag(f"E_p{d}") for d in self.directions]), np.zeros(3),
                      [self.lattice.origin_face(d) for d in self.directions], "U")
-        self.basis = SectorBasis(self.loc_dims, glob=glob, link=link, nbody=nbody)
+        self.basis = SectorBasis(self.loc_dims, glob=glob, link=link if self.link_symmetries else None,
+                                 nbody=nbody)
         logger.info(f"SEC DIM: {self.basis.n_states}")
         self.default_params()
 
@@ -923,6 +931,11 @@ class QED_Model(QuantumModel):
                 for stag in ("even", "odd"):
                     bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
                     self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
+        if not self.link_symmetries:
+            # link penalties: 100 sum E2 - 100 sum_links E_p(i) E_m(i + d)
+            self._add_local("E2", 100.0)
+            for d in self.directions:
+                self._add_nbody([f"E_p{d}", f"E_m{d}"], lat.bonds(d), -100.0, False)
         return self._build_program(real)
 
     def overlap_QMB_state(self, name):
@@ -981,7 +994,8 @@ class QED_Model(QuantumModel):
     def multi_term_program(self, m=None):
         """(tables, placements with term ids) of H(g) = (g/2) H_E + (-1/(2g)) H_B + H_0
         for a g sweep at fixed m: term 0 = E2, term 1 = xy plaquettes (+ h.c.; the
-        3D planes too), term 2 = mass + hopping at mass m (empty without matter).
+        3D planes too), term 2 = mass + hopping at mass m (empty without matter)
+        and the link penalties when link_symmetries=False.
         Coefficients of a coupling g: QED_Model.multi_term_coefficients(g).
         spin='integrated': term 0 = the electric energy at E = 1, term 1
         empty, term 2 = mass + hopping."""
@@ -1023,6 +1037,10 @@ class QED_Model(QuantumModel):
                 for stag in ("even", "odd"):
                     bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
                     self._add_nbody([f"Q_p{d}_dag", f"Q_m{d}"], bonds, c[f"t{d}_{stag}"], True)
+        if not self.link_symmetries:
+            self._add_local("E2", 100.0)
+            for d in self.directions:
+                self._add_nbody([f"E_p{d}", f"E_m{d}"], lat.bonds(d), -100.0, False)
         for p in self._placements[k1:]:
             p["term"] = 2
         return np.stack(self._tables), list(self._placements)

@@ -306,8 +306,12 @@ class QEDOracle:
     """QED_Model (truncated spin, ladder U, no background) restated
     (models/QED_model.py:23-233, 235-394, 719-776)."""
 
-    def __init__(self, spin, pure_theory, lvals, has_obc, enumerate_sector=True, bg_list=None):
+    def __init__(self, spin, pure_theory, lvals, has_obc, enumerate_sector=True, bg_list=None,
+                 link_symmetries=True):
         self.spin, self.pure = spin, pure_theory
+        # link_symmetries=False: no link constraints in the sector, penalty
+        # terms in H instead (QED_model.py:131-141, 381-393)
+        self.link_symmetries = bool(link_symmetries)
         self.lvals, self.has_obc = list(lvals), list(has_obc)
         self.dim = len(lvals)
         self.n_sites = prod(lvals)
@@ -336,6 +340,8 @@ class QEDOracle:
         self.pairs, self.link_diags = pairs, link_diags
         if not enumerate_sector:
             return
+        if not self.link_symmetries:
+            link_diags = None
         if not pure_theory:
             glob = dg("N")[None]
             self.configs = sector_configs(self.loc_dims, glob, np.array([self.n_sites // 2], float),
@@ -435,6 +441,17 @@ class QEDOracle:
                     if bonds:
                         add_nbody_list(T, [self.ops[f"Q_p{d}_dag"], self.ops[f"Q_m{d}"]], bonds, *args,
                                        c[f"t{d}_{stag}"], True)
+        if not self.link_symmetries:
+            # 100 E2 - 100 sum_links E_p(i) E_m(i + d) = 50 sum (E_p - E_m)^2 + boundary terms
+            add_local(T, self.ops["E2"], *args, 100.0)
+            for d in self.dirs:
+                bonds = []
+                for i in range(self.n_sites):
+                    b = lat.neighbor_sites(lat.zig_zag(self.lvals, i), self.lvals, d, self.has_obc)
+                    if b is not None:
+                        bonds.append(b)
+                if bonds:
+                    add_nbody_list(T, [self.ops[f"E_p{d}"], self.ops[f"E_m{d}"]], bonds, *args, -100.0, False)
         return T
 
     def hamiltonian(self, g, m=None, theta=0.0):

@@ -148,3 +148,20 @@ def test_integrated_qed_oracle_pins():
     psi = np.zeros(len(cfg))
     psi[rows[tuple(meson)]] = 1.0
     assert np.allclose(iq.link_e2(L, np.zeros(L), psi), [0, 0, 1, 0, 0])
+
+
+def test_link_penalty_spectrum_equals_link_sector():
+    """QED_Model(link_symmetries=False) (QED_model.py:131-141, 381-393): the
+    sector without link constraints plus 100 E2 - 100 E_p E_m per link (=
+    50 (E_p - E_m)^2 per link, commuting with H) has the link-symmetric
+    sector's spectrum at the bottom, exactly."""
+    from oracle.model import QEDOracle, lowest_eigs
+    for lv, obc, pure, g, m in (([4], [True], False, 1.0, 1.0), ([4], [False], False, 1.0, 1.0),
+                                ([2, 2], [True, True], True, 1.0, None), ([2, 2], [False, False], True, 0.7, None)):
+        a = QEDOracle(1, pure, lv, obc)
+        b = QEDOracle(1, pure, lv, obc, link_symmetries=False)
+        assert b.N > a.N
+        k = min(3, a.N - 1)
+        ea = lowest_eigs(a.hamiltonian(g, m), k)
+        eb = lowest_eigs(b.hamiltonian(g, m), k)
+        assert np.max(np.abs(ea - eb)) < 1e-10, (lv, obc, ea, eb)
