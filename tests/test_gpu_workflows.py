@@ -105,3 +105,27 @@ def test_run_su2_spectrum_in_momentum_sector(gpu):
     assert abs(res["energy"][0] - (-6.173752796132477)) < 1e-10
     assert abs(res["E2"][0] - 0.0553604161539725) < 1e-10
     assert res["entropy"][0] == 0.0  # not measured in a momentum sector, as in the reference
+
+
+def test_driver_through_run_sim_result_layout(gpu, tmp_path):
+    """A QED KAT case generated from a config template, run through
+    run_sim with the spectrum driver and read back with get_sim from
+    results/<uid>/res.npz (the reference scripts' simsio flow,
+    scripts/script_QED.py:18-40)."""
+    import yaml
+    from edlgt_amd import simio
+    from edlgt_amd.workflows import check_observables, run_QED_spectrum
+    common = _qed_par([4], [True], False, 1.0, 1.0)
+    del common["g"], common["m"]
+    cfg = tmp_path / "configs" / "QED"
+    cfg.mkdir(parents=True)
+    header = {"===": {"template": "$uid:\n  <<<: common\n  g: $g\n  m: $m\n", "common": common}}
+    (cfg / "kat.yaml").write_text(yaml.safe_dump(header, sort_keys=False))
+    (uid,) = simio.gen_configs("template", {"g": [1.0], "m": [1.0]}, "QED/kat", tmp_path / "configs")
+    with simio.run_sim(uid, tmp_path / "configs", tmp_path / "results") as sim:
+        sim.res.update(run_QED_spectrum(sim.par))
+    back = simio.get_sim(uid, tmp_path / "configs", tmp_path / "results")
+    check_observables(back.res, {"energy": -0.5691411466075982, "E2": 0.03161445624423755,
+                                 "N": 0.0474013240232069}, atol=1e-10, tag="QED 1D via run_sim")
+    q = simio.SimsQuery("QED/kat", tmp_path / "configs")
+    assert q.uids == [uid]
