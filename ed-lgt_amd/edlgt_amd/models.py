@@ -1148,34 +1148,45 @@ class Zn_Model(QuantumModel):
 
 
 class SU2_Model(QuantumModel):
-    """SU(2) lattice gauge theory with spin-1/2 rishons and optional two-colour
-    staggered matter (edlgt/models/SU2_model.py:28-355, 486-624), 1D and 2D.
+    """SU(2) lattice gauge theory with optional two-colour staggered matter
+    (edlgt/models/SU2_model.py:28-355, 357-484, 486-624), 1D to 3D: the
+    hard-coded spin-1/2 operator set, or the generalised one (any rishon
+    truncation spin, use_generic_model=True or spin >= 1).
 
     Sector: global N_tot = sectors[0] with matter, link constraints
-    T2_p{d}(i) = T2_m{d}(i+d), and for the pure theory on periodic 2D lattices
-    the parity of the flux through the origin lines fixed to +1."""
+    T2_p{d}(i) = T2_m{d}(i+d), and for the pure theory on periodic lattices
+    the parity of the flux through the origin lines (2D) or faces (3D) fixed
+    to +1."""
 
     def __init__(self, spin, pure_theory, bg_list=None, sectors=None, use_generic_model=False, **kwargs):
         super().__init__(**kwargs)
-        if float(spin) != 0.5:
-            raise NotImplementedError("the device engine builds the spin-1/2 SU(2) model")
+        from fractions import Fraction
+        s2 = 2 * float(spin)
+        if s2 < 1 or abs(s2 - round(s2)) > 1e-12:
+            raise ValueError(f"spin must be a positive multiple of 1/2, not {spin}")
         if bg_list is not None and any(b != 0 for b in bg_list):
             raise NotImplementedError("background charges are not supported by the device engine")
-        if use_generic_model:
-            raise NotImplementedError("the generalised SU(2) operator set is not on the device path")
-        if self.dim > 2:
-            raise NotImplementedError("SU(2) models are built in 1D and 2D")
+        if self.dim > 3:
+            raise ValueError("SU(2) models are defined in 1D, 2D and 3D")
         from .su2_operators import SU2Site
-        self.spin = 0.5
+        s_max = Fraction(int(round(s2)), 2)
+        generic = bool(use_generic_model) or s_max >= 1
+        if self.dim == 3 and s_max > Fraction(1, 2):
+            raise NotImplementedError("3D SU(2) is built for spin 1/2 rishons (a spin-1 site has "
+                                      "2,985,984 states before projection)")
+        self.spin = float(spin)
         self.pure_theory = bool(pure_theory)
-        self.use_generic_model = False
+        self.use_generic_model = generic
         # translation-combined symmetry of the 1D chain with matter: local
         # involution on the six dressed states and the sign of each state
         # (translational_sym.py:1013-1014, 1171-1172)
-        if not pure_theory and len(kwargs.get("lvals", [])) == 1:
+        if not pure_theory and self.dim == 1 and not generic:
             self.tc_symmetry_tables = (np.array([4, 5, 2, 3, 0, 1]), np.array([1.0, 1.0, 1.0, -1.0, 1.0, 1.0]))
         self.background = 0
-        site = SU2Site(self.pure_theory, self.dim)
+        site = SU2Site(self.pure_theory, self.dim, s_max, generic)
+        if max(site.dim(l) for l in site.basis) > 255:
+            raise NotImplementedError(f"a dressed site of {max(site.dim(l) for l in site.basis)} states "
+                                      "exceeds the engine's 255 local states")
         self._site = site
         labels = [self.lattice.label(i, False) for i in range(self.n_sites)]
         self._project(site, site.operators, labels)
@@ -1196,6 +1207,9 @@ class SU2_Model(QuantumModel):
             line_of = {"x": "y", "y": "x"}
             nbody = (np.stack([diag(f"P_p{d}") for d in self.directions]), np.ones(self.dim),
                      [self.lattice.origin_line(line_of[d]) for d in self.directions], "Z")
+        elif self.pure_theory and not any(self.has_obc) and self.dim == 3:
+            nbody = (np.stack([diag(f"P_p{d}") for d in self.directions]), np.ones(3),
+                     [self.lattice.origin_face(d) for d in self.directions], "Z")
         self.basis = SectorBasis(self.loc_dims, glob=glob, link=link, nbody=nbody)
         if self.basis.n_states == 0:
             raise ValueError("No configurations found for the given symmetry sectors")
@@ -1229,12 +1243,17 @@ class SU2_Model(QuantumModel):
         if abs(theta) > 1e-12 or lambda_noise:
             raise NotImplementedError("theta and Gauss-law noise terms are not on the device path")
         self.SU2_Hamiltonian_couplings(g, m, theta)
+        if self.use_generic_model:
+            return self._build_generic(m, real)
         c = self.coeffs
         lat = self.lattice
         self._new_program()
         self._add_local("E_square", c["E"])
         if self.dim > 1:
             self._add_nbody(["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"], lat.plaquettes(("x", "y")), c["B"], True)
+        if self.dim == 3:
+            self._add_nbody(["C_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], lat.plaquettes(("x", "z")), c["B"], True)
+            self._add_nbody(["C_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], lat.plaquettes(("y", "z")), c["B"], True)
         if not self.pure_theory:
             if m is None:
                 raise ValueError("m is required with matter fields")
@@ -1247,8 +1266,46 @@ class SU2_Model(QuantumModel):
                     self._add_nbody([f"Qp{d}_dag", f"Qm{d}"], bonds, c[f"t{d}_{stag}"], True)
         return self._build_program(real)
 
+    # corner combinations of the generalised plaquette (SU2_model.py:420-437)
+    _GEN_PLAQ = ["ABABABAB", "AAABBBAB", "ABABAABB", "AAABBABB", "ABBBABAA", "AABBBBAA", "ABBBAABA",
+                 "AABBBABA", "BBAAABAB", "BAAABBAB", "BBAAAABB", "BAAABABB", "BBBAABAA", "BABABBAA",
+                 "BBBAAABA", "BABABABA"]
+
+    def _build_generic(self, m, real):
+        """build_gen_Hamiltonian (SU2_model.py:357-484): electric Casimir,
+        staggered mass N_tot, the two hoppings [Q1_p^dag, Q2_m] and
+        [Q2_p^dag, Q1_m] per direction and parity, and per plane the sixteen
+        species combinations of the plaquette."""
+        c = self.coeffs
+        lat = self.lattice
+        self._new_program()
+        self._add_local("E_square", c["E"])
+        if not self.pure_theory:
+            if m is None:
+                raise ValueError("m is required with matter fields")
+            even = lat.even_mask()
+            self._add_local("N_tot", c["m_even"], [i for i in range(self.n_sites) if even[i]])
+            self._add_local("N_tot", c["m_odd"], [i for i in range(self.n_sites) if not even[i]])
+            for d in self.directions:
+                for stag in ("even", "odd"):
+                    bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
+                    for names in ([f"Q1_p{d}_dag", f"Q2_m{d}"], [f"Q2_p{d}_dag", f"Q1_m{d}"]):
+                        self._add_nbody(names, bonds, c[f"t{d}_{stag}"], True)
+        if self.dim > 1:
+            planes = ["xy"] if self.dim == 2 else ["xy", "xz", "yz"]
+            for a, b in planes:
+                plaq = lat.plaquettes((a, b))
+                for sp4 in self._GEN_PLAQ:
+                    k = [sp4[2 * q:2 * q + 2] for q in range(4)]
+                    names = [f"C{k[0]}_p{a},p{b}", f"C{k[1]}_p{b},m{a}", f"C{k[2]}_m{b},p{a}", f"C{k[3]}_m{a},m{b}"]
+                    self._add_nbody(names, plaq, c["B"], True)
+        return self._build_program(real)
+
     def overlap_QMB_state(self, name):
-        """Reference product configurations (SU2_model.py:486-553)."""
+        """Reference product configurations (SU2_model.py:486-553), local-state
+        indices of the hard-coded spin-1/2 basis."""
+        if self.use_generic_model:
+            raise NotImplementedError("named states index the hard-coded spin-1/2 basis")
         if self.dim == 1:
             table = {"V": (0, 4, 0, 2), "PV": (1, 5, 1, 1), "T": (6, 12, 1, 1), "M": (2, 3, 1, 1),
                      "B": (7, 11, 1, 1)}

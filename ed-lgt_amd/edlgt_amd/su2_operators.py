@@ -1,18 +1,23 @@
-"""SU(2) dressed sites (spin-1/2 rishons, optional two-colour staggered matter).
+"""SU(2) dressed sites: rishons of every irrep j <= s_max per half-link, with
+optional two-colour staggered matter.
 
 A dressed site is matter (4 states: vacuum, r = m +1/2, g = m -1/2, pair)
-times one rishon per half-link (3 states: j = 0; j = 1/2, m = +1/2; j = 1/2,
-m = -1/2), half-links ordered m_x, (m_y, m_z,) p_x, (p_y, p_z)
-(edlgt/operators/SU2_operators.py:35-300, SU2_rishons.py:50-80,
-bose_fermi_operators.py:14-70).  The gauge-invariant local states are the
-SU(2) singlets of the site's spins, built by coupling the spins one after the
-other (matter first) and grouped by their intermediate total spins
-(SU2_singlets.py:30-249); their order follows SU2_gauge_invariant_states
-(SU2_operators.py:625-714): spin configurations in product order, then
-singlets sorted by intermediate spins.  Operators are Kronecker products on the
-full site space projected onto each label's singlet basis, B^T O B
-(quantum_model.py:400-502); the site spaces are tiny (36 states in 1D, 324 in
-2D).
+times one rishon per half-link (the irreps j = 0, 1/2, ..., s_max, each with
+m = j, ..., -j: 3 states for s_max = 1/2), half-links ordered m_x, (m_y,
+m_z,) p_x, (p_y, p_z) (edlgt/operators/SU2_operators.py:35-300, 302-592,
+SU2_rishons.py:50-234, bose_fermi_operators.py:14-70).  The gauge-invariant
+local states are the SU(2) singlets of the site's spins, built by coupling
+the spins one after the other (matter first) and grouped by their
+intermediate total spins (SU2_singlets.py:30-249); their order follows
+SU2_gauge_invariant_states (SU2_operators.py:625-714): spin configurations in
+product order, then singlets sorted by intermediate spins.  Operators are
+sparse Kronecker products on the full site space projected onto each
+label's singlet basis, B^T O B (quantum_model.py:400-502).
+
+Two operator sets: the hard-coded spin-1/2 one (SU2_dressed_site_operators,
+rishons Z_r, Z_g that raise and lower j) and the generalised one of any
+s_max (SU2_gen_dressed_site_operators: rishons that lower j by 1/2, in two
+species A and B, hoppings Q1/Q2 and sixteen corner combinations per plaquette).
 
 Clebsch-Gordan coefficients come from the Racah formula in exact rational
 arithmetic (a coefficient is sign * sqrt(rational); a singlet amplitude, a
@@ -23,6 +28,7 @@ from itertools import product
 from math import factorial, sqrt
 
 import numpy as np
+import scipy.sparse as sp
 
 HALF = Fraction(1, 2)
 
@@ -164,36 +170,108 @@ def matter_matrices():
     return m
 
 
+def link_irreps(s_max):
+    """j = 0, 1/2, ..., s_max."""
+    return [Fraction(k, 2) for k in range(int(2 * s_max) + 1)]
+
+
+def rishon_states(s_max):
+    """(j, m) of every rishon index: irreps in increasing j, m descending."""
+    return [(j, m) for j in link_irreps(s_max) for m in m_values(j)]
+
+
+def generators(s_max):
+    """Block-diagonal SU(2) generators on the rishon space (spin_operators.py:
+    75-127, get_spin_operators :31-60): Tz = m, Tp |j m> = sqrt(j(j+1) -
+    m(m+1)) |j m+1>, T2 = j(j+1), parity P = (-1)^(2j)."""
+    st = rishon_states(s_max)
+    n = len(st)
+    Tz = np.diag([float(m) for _, m in st])
+    T2 = np.diag([float(j * (j + 1)) for j, _ in st])
+    P = np.diag([(-1.0) ** int(2 * j) for j, _ in st])
+    Tp = np.zeros((n, n))
+    for a, (ja, ma) in enumerate(st):
+        for b, (jb, mb) in enumerate(st):
+            if ja == jb and ma == mb + 1:
+                Tp[a, b] = sqrt(float(jb * (jb + 1) - mb * (mb + 1)))
+    Tm = Tp.T.copy()
+    return {"Iz": np.eye(n), "P": P, "Tz": Tz, "Tp": Tp, "Tm": Tm, "Tx": 0.5 * (Tp + Tm),
+            "Ty": -0.5j * (Tp - Tm), "T2": T2}
+
+
+def _chi(j, color, m):
+    """Coefficient of the generalised rishons (SU2_rishons.py:208-234)."""
+    num = float(j + m + 1) if color == "r" else float(j - m + 1)
+    return sqrt(num / sqrt(float((2 * j + 1) * (2 * j + 2))))
+
+
+def rishon_matrices_gen(s_max):
+    """Generalised rishons (SU2_Rishon_gen, SU2_rishons.py:106-196): Z_r maps
+    |j + 1/2, m + 1/2> to |j, m> and Z_g maps |j + 1/2, m - 1/2> to |j, m> with
+    coefficient chi(j, color, m); species B = Z, species A: ZA_r = ZB_g^dag,
+    ZA_g = -ZB_r^dag; products with the parity P for the corner operators."""
+    st = rishon_states(s_max)
+    idx = {jm: k for k, jm in enumerate(st)}
+    n = len(st)
+    m = generators(s_max)
+    Z = {"r": np.zeros((n, n)), "g": np.zeros((n, n))}
+    for (j, mm), a in idx.items():
+        if j == s_max:
+            continue
+        Z["r"][a, idx[(j + HALF, mm + HALF)]] = _chi(j, "r", mm)
+        Z["g"][a, idx[(j + HALF, mm - HALF)]] = _chi(j, "g", mm)
+    P = m["P"]
+    for c in "rg":
+        m[f"Z{c}"] = Z[c]
+        m[f"Z{c}_dag"] = Z[c].T.copy()
+        m[f"ZB_{c}"] = Z[c]
+        m[f"ZB_{c}_dag"] = Z[c].T.copy()
+    m["ZA_r"] = m["ZB_g_dag"].copy()
+    m["ZA_g"] = -m["ZB_r_dag"]
+    for sp_ in "AB":
+        for c in "rg":
+            z = m[f"Z{sp_}_{c}"]
+            m[f"Z{sp_}_{c}_dag"] = z.T.copy()
+            m[f"Z{sp_}_{c}_P"] = z @ P
+            m[f"P_Z{sp_}_{c}_dag"] = P @ z.T
+    return m
+
+
 def _kron(mats):
-    out = np.array([[1.0]])
+    out = sp.identity(1, format="csr")
     for a in mats:
-        out = np.kron(out, a)
+        out = sp.kron(out, sp.csr_matrix(a), format="csr")
     return out
 
 
 # ---------------------------------------------------------------- dressed site
-MATTER_J = [("V", Fraction(0)), ("S", HALF), ("S", HALF), ("P", Fraction(0))]
-MATTER_M = [Fraction(0), HALF, -HALF, Fraction(0)]
-RISHON_J = [Fraction(0), HALF, HALF]
-RISHON_M = [Fraction(0), HALF, -HALF]
-
-
 class SU2Site:
     """Singlet bases per site label ('site' and OBC border labels) as
-    isometries basis[label] (full site dim x d_loc)."""
+    isometries basis[label] (full site dim x d_loc), for rishon irreps up to
+    s_max; generic=True selects the generalised operator set."""
 
-    def __init__(self, pure_theory, dim):
+    def __init__(self, pure_theory, dim, s_max=HALF, generic=False):
         self.pure = bool(pure_theory)
         self.dim_lat = int(dim)
+        self.s_max = Fraction(s_max).limit_denominator(2)
+        self.generic = bool(generic)
+        if not self.generic and self.s_max != HALF:
+            raise ValueError("the hard-coded SU(2) operator set is the spin-1/2 one")
         axes = "xyz"[:dim]
         self.links = [f"m{a}" for a in axes] + [f"p{a}" for a in axes]
         n_link = len(self.links)
-        self.full_dim = (1 if self.pure else 4) * 3 ** n_link
+        self.rdim = len(rishon_states(self.s_max))
+        self._roff = {}
+        k = 0
+        for j in link_irreps(self.s_max):
+            self._roff[j] = k
+            k += int(2 * j) + 1
+        self.full_dim = (1 if self.pure else 4) * self.rdim ** n_link
         from .operators import _border_parts
         labels = ["site"] + ["site_" + "_".join(p) for p in _border_parts(dim)]
         cols = {lab: [] for lab in labels}
         vals = {lab: [] for lab in labels}
-        link_choices = [[Fraction(0), HALF]] * n_link
+        link_choices = [link_irreps(self.s_max)] * n_link
         matter_choices = [] if self.pure else [["V", "S", "P"]]
         for combo in product(*(matter_choices + link_choices)):
             if self.pure:
@@ -233,62 +311,125 @@ class SU2Site:
             idx = mi
             k0 = 1
         for s, m in zip(spins[k0:], ms[k0:]):
-            ri = 0 if s == 0 else (1 if m > 0 else 2)
-            idx = idx * 3 + ri
+            idx = idx * self.rdim + self._roff[s] + int(s - m)
         return idx
 
     def dim(self, label):
         return self.basis[label].shape[1]
 
-    def full_operators(self):
-        """Kronecker-product operators on the full site space
-        (SU2_operators.py:35-300, spin 1/2; 1D and 2D)."""
-        R = rishon_matrices()
-        Mt = None if self.pure else matter_matrices()
+    def _op_factory(self, R, Mt):
         links = self.links
-        n_link = len(links)
 
         def op(assign, matter="ID_psi"):
             mats = [] if self.pure else [Mt[matter]]
             mats += [R[assign.get(l, "Iz")] for l in links]
             return _kron(mats)
+        return op
 
+    def _corner_lists(self):
+        """Corner operator factor lists per (corner, colour) as (half-link ->
+        factor name) maps, for the planes of the lattice (SU2_operators.py:
+        104-131, 200-257, 383-399, 431-492); the names use a species prefix
+        ('' for the hard-coded set, 'A'/'B' pairs for the generalised one)."""
+        if self.dim_lat == 2:
+            return {"px,py": {"px": "{a}_P", "py": "{b}_dag"},
+                    "py,mx": {"mx": "P_{b}_dag", "my": "P", "px": "P", "py": "{a}"},
+                    "mx,my": {"mx": "{a}_P", "my": "{b}_dag"},
+                    "my,px": {"my": "{a}_P", "px": "{b}_dag"}}
+        if self.dim_lat == 3:
+            return {"px,py": {"px": "{a}_P", "py": "{b}_dag"},
+                    "py,mx": {"mx": "P_{b}_dag", "my": "P", "mz": "P", "px": "P", "py": "{a}"},
+                    "mx,my": {"mx": "{a}_P", "my": "{b}_dag"},
+                    "my,px": {"my": "{a}_P", "mz": "P", "px": "{b}_dag"},
+                    "px,pz": {"px": "{a}_P", "py": "P", "pz": "{b}_dag"},
+                    "pz,mx": {"mx": "P_{b}_dag", "my": "P", "mz": "P", "px": "P", "py": "P", "pz": "{a}"},
+                    "mx,mz": {"mx": "{a}_P", "my": "P", "mz": "{b}_dag"},
+                    "mz,px": {"mz": "{a}_P", "px": "{b}_dag"},
+                    "py,pz": {"py": "{a}_P", "pz": "{b}_dag"},
+                    "pz,my": {"my": "P_{b}_dag", "mz": "P", "px": "P", "py": "P", "pz": "{a}"},
+                    "my,mz": {"my": "{a}_P", "mz": "{b}_dag"},
+                    "mz,py": {"mz": "{a}_P", "px": "P", "py": "{b}_dag"}}
+        return {}
+
+    def full_operators(self):
+        """Sparse Kronecker-product operators on the full site space."""
+        return self._generic_operators() if self.generic else self._standard_operators()
+
+    def _standard_operators(self):
+        """SU2_dressed_site_operators (SU2_operators.py:35-300), spin 1/2, 1D-3D."""
+        R = rishon_matrices()
+        Mt = None if self.pure else matter_matrices()
+        links = self.links
+        op = self._op_factory(R, Mt)
         ops = {}
         for name in ("T2", "P", "Tx", "Ty", "Tz"):
             for l in links:
                 ops[f"{name}_{l}"] = op({l: name})
         if self.dim_lat == 1:
             ops["U"] = sum(op({"mx": f"Z{c}_dag_P", "px": f"Z{c}"}) for c in "rg") / np.sqrt(2)
-        elif self.dim_lat == 2:
-            ops["C_px,py"] = sum(op({"px": f"Z{c}_P", "py": f"Z{c}_dag"}) for c in "rg")
-            ops["C_py,mx"] = sum(op({"mx": f"P_Z{c}_dag", "my": "P", "px": "P", "py": f"Z{c}"}) for c in "rg")
-            ops["C_mx,my"] = sum(op({"mx": f"Z{c}_P", "my": f"Z{c}_dag"}) for c in "rg")
-            ops["C_my,px"] = sum(op({"my": f"Z{c}_P", "px": f"Z{c}_dag"}) for c in "rg")
-        else:
-            raise NotImplementedError("SU(2) dressed sites are built for 1D and 2D lattices")
+        for corner, fac in self._corner_lists().items():
+            ops[f"C_{corner}"] = sum(op({l: f.format(a=f"Z{c}", b=f"Z{c}") for l, f in fac.items()})
+                                     for c in "rg")
         if not self.pure:
-            axes = "xyz"[: self.dim_lat]
-            for a in axes:
-                ops[f"Qm{a}_dag"] = 0
-                ops[f"Qp{a}_dag"] = 0
-            for c in "rg":
-                # string of parities over the half-links before the acted one
-                for k, l in enumerate(links):
-                    assign = {links[q]: "P" for q in range(k)}
-                    assign[l] = f"Z{c}"
-                    ops[f"Q{l}_dag"] = ops[f"Q{l}_dag"] + op(assign, matter=f"psi_{c}_dag_P")
-            for a in axes:
-                for s in "mp":
-                    ops[f"Q{s}{a}"] = ops[f"Q{s}{a}_dag"].conj().T.copy()
-            for lab in ("r", "g", "tot", "single", "pair", "zero"):
-                ops[f"N_{lab}"] = op({}, matter=f"N_{lab}")
-            ops["N-1"] = ops["N_tot"] - np.eye(self.full_dim)
+            self._add_matter(ops, op)
         ops["E_square"] = sum(0.5 * ops[f"T2_{l}"] for l in links)
         return ops
+
+    def _generic_operators(self):
+        """SU2_gen_dressed_site_operators (SU2_operators.py:302-592), any s_max, 1D-3D."""
+        R = rishon_matrices_gen(self.s_max)
+        Mt = None if self.pure else matter_matrices()
+        links = self.links
+        op = self._op_factory(R, Mt)
+        ops = {}
+        for name in ("T2", "P"):
+            for l in links:
+                ops[f"{name}_{l}"] = op({l: name})
+        for l1, l2 in product("AB", repeat=2):
+            for corner, fac in self._corner_lists().items():
+                ops[f"C{l1}{l2}_{corner}"] = sum(
+                    op({l: f.format(a=f"Z{l1}_{c}", b=f"Z{l2}_{c}") for l, f in fac.items()}) for c in "rg")
+        if not self.pure:
+            # Q1^dag = psi_r^dag Zg^dag - psi_g^dag Zr^dag, Q2^dag = psi_r^dag Zr + psi_g^dag Zg,
+            # parities on the half-links before the acted one
+            for k, l in enumerate(links):
+                string = {links[q]: "P" for q in range(k)}
+                ops[f"Q1_{l}_dag"] = (op({**string, l: "Zg_dag"}, matter="psi_r_dag_P")
+                                      - op({**string, l: "Zr_dag"}, matter="psi_g_dag_P"))
+                ops[f"Q2_{l}_dag"] = (op({**string, l: "Zr"}, matter="psi_r_dag_P")
+                                      + op({**string, l: "Zg"}, matter="psi_g_dag_P"))
+            for l in links:
+                for q in ("Q1", "Q2"):
+                    ops[f"{q}_{l}"] = ops[f"{q}_{l}_dag"].conj().T.tocsr()
+            for lab in ("r", "g", "tot", "single", "pair", "zero"):
+                ops[f"N_{lab}"] = op({}, matter=f"N_{lab}")
+        ops["E_square"] = sum(0.5 * ops[f"T2_{l}"] for l in links)
+        return ops
+
+    def _add_matter(self, ops, op):
+        """Hard-coded set: hoppings Q_{side axis}^dag = sum_c psi_c^dag Z_c with
+        the parity string, number operators and N - 1."""
+        links = self.links
+        axes = "xyz"[: self.dim_lat]
+        for a in axes:
+            ops[f"Qm{a}_dag"] = 0
+            ops[f"Qp{a}_dag"] = 0
+        for c in "rg":
+            # string of parities over the half-links before the acted one
+            for k, l in enumerate(links):
+                assign = {links[q]: "P" for q in range(k)}
+                assign[l] = f"Z{c}"
+                ops[f"Q{l}_dag"] = ops[f"Q{l}_dag"] + op(assign, matter=f"psi_{c}_dag_P")
+        for a in axes:
+            for s_ in "mp":
+                ops[f"Q{s_}{a}"] = ops[f"Q{s_}{a}_dag"].conj().T.tocsr()
+        for lab in ("r", "g", "tot", "single", "pair", "zero"):
+            ops[f"N_{lab}"] = op({}, matter=f"N_{lab}")
+        ops["N-1"] = ops["N_tot"] - sp.identity(self.full_dim, format="csr")
 
     def operators(self, label):
         """Every operator projected on the label's singlet basis, B^T O B."""
         if not hasattr(self, "_full"):
             self._full = self.full_operators()
         B = self.basis[label]
-        return {k: B.T @ v @ B for k, v in self._full.items()}
+        return {k: np.asarray(B.T @ (v @ B)) for k, v in self._full.items()}

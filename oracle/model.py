@@ -524,18 +524,23 @@ class ZnOracle:
 
 
 class SU2Oracle:
-    """SU2_Model (j = 1/2, hard-coded operators, no background) restated
-    (models/SU2_model.py:31-171, 200-355, 555-624), 1D and 2D."""
+    """SU2_Model (no background) restated (models/SU2_model.py:31-171,
+    200-355, 357-484, 555-624): the hard-coded spin-1/2 operators, or the
+    generalised ones (generic=True; any rishon truncation spin), 1D-3D."""
 
-    def __init__(self, pure_theory, lvals, has_obc, sectors=None):
+    def __init__(self, pure_theory, lvals, has_obc, sectors=None, spin=0.5, generic=False):
+        from sympy import Rational
         from . import su2_ops
         self.pure = pure_theory
         self.lvals, self.has_obc = list(lvals), list(has_obc)
         self.dim = len(lvals)
         self.n_sites = prod(lvals)
         self.dirs = "xyz"[: self.dim]
-        basis = su2_ops.gauge_invariant_states(pure_theory, self.dim)
-        ops = su2_ops.dressed_site_operators(pure_theory, self.dim)
+        j_max = Rational(int(round(2 * spin)), 2)
+        self.generic = bool(generic) or j_max >= 1
+        basis = su2_ops.gauge_invariant_states(pure_theory, self.dim, j_max)
+        ops = (su2_ops.dressed_site_operators_gen(pure_theory, self.dim, j_max) if self.generic
+               else su2_ops.dressed_site_operators(pure_theory, self.dim))
         self.labels = [lat.site_label(lat.zig_zag(self.lvals, i), self.lvals, self.has_obc, False)
                        for i in range(self.n_sites)]
         self.site_basis = basis
@@ -553,6 +558,12 @@ class SU2Oracle:
             self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, link_sectors, pairs,
                                           np.stack([dg(f"P_p{d}") for d in self.dirs]), np.ones(self.dim),
                                           [lines[line_of[d]] for d in self.dirs], "Z")
+        elif not any(self.has_obc) and self.dim == 3:
+            faces = lat.origin_faces_3d(self.lvals)
+            face_of = {"x": "yz", "y": "xz", "z": "xy"}
+            self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, link_sectors, pairs,
+                                          np.stack([dg(f"P_p{d}") for d in self.dirs]), np.ones(3),
+                                          [faces[face_of[d]] for d in self.dirs], "Z")
         else:
             self.configs = sector_configs(self.loc_dims, None, None, "U", link_diags, link_sectors, pairs)
         self.keys = encode(self.configs, self.loc_dims)
@@ -564,20 +575,36 @@ class SU2Oracle:
         if not self.pure and m is not None:
             t = 2 * np.sqrt(2)
             c.update({"tx_even": -1j * t, "tx_odd": -1j * t, "ty_even": -t, "ty_odd": t,
-                      "m_even": m, "m_odd": -m})
+                      "tz_even": -t, "tz_odd": t, "m_even": m, "m_odd": -m})
         return c
 
+    def _plaqs(self, axes):
+        return [p for p in (lat.plaquette_sites(lat.zig_zag(self.lvals, i), self.lvals, list(axes), self.has_obc)
+                            for i in range(self.n_sites)) if p]
+
+    def _bonds(self, d, stag):
+        mask = lat.staggered_mask(self.lvals, stag)
+        bonds = []
+        for i in range(self.n_sites):
+            co = lat.zig_zag(self.lvals, i)
+            b = lat.neighbor_sites(co, self.lvals, d, self.has_obc)
+            if b is not None and mask[co]:
+                bonds.append(b)
+        return bonds
+
     def triplets(self, g, m=None):
-        """build_base_Hamiltonian term order (SU2_model.py:200-355)."""
+        """build_base_Hamiltonian (SU2_model.py:200-355) or build_gen_Hamiltonian
+        (:357-484) term order."""
+        if self.generic:
+            return self._triplets_gen(g, m)
         c = self.couplings(g, m)
         T = Triplets(self.N)
         args = (self.configs, self.keys, self.loc_dims)
         add_local(T, self.ops["E_square"], *args, c["E"])
-        if self.dim > 1:
-            names = ["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"]
-            plaqs = [p for p in (lat.plaquette_sites(lat.zig_zag(self.lvals, i), self.lvals, ["x", "y"],
-                                                     self.has_obc) for i in range(self.n_sites)) if p]
-            add_nbody_list(T, [self.ops[n] for n in names], plaqs, *args, c["B"], True)
+        planes = [] if self.dim == 1 else ["xy"] if self.dim == 2 else ["xy", "xz", "yz"]
+        for a, b in planes:
+            names = [f"C_p{a},p{b}", f"C_p{b},m{a}", f"C_m{b},p{a}", f"C_m{a},m{b}"]
+            add_nbody_list(T, [self.ops[n] for n in names], self._plaqs((a, b)), *args, c["B"], True)
         if not self.pure:
             for stag in ["even", "odd"]:
                 mask = lat.staggered_mask(self.lvals, stag)
@@ -595,6 +622,39 @@ class SU2Oracle:
                     if bonds:
                         add_nbody_list(T, [self.ops[f"Qp{d}_dag"], self.ops[f"Qm{d}"]], bonds, *args,
                                        c[f"t{d}_{stag}"], True)
+        return T
+
+    PLAQ_SETS = [["AB", "AB", "AB", "AB"], ["AA", "AB", "BB", "AB"], ["AB", "AB", "AA", "BB"],
+                 ["AA", "AB", "BA", "BB"], ["AB", "BB", "AB", "AA"], ["AA", "BB", "BB", "AA"],
+                 ["AB", "BB", "AA", "BA"], ["AA", "BB", "BA", "BA"], ["BB", "AA", "AB", "AB"],
+                 ["BA", "AA", "BB", "AB"], ["BB", "AA", "AA", "BB"], ["BA", "AA", "BA", "BB"],
+                 ["BB", "BA", "AB", "AA"], ["BA", "BA", "BB", "AA"], ["BB", "BA", "AA", "BA"],
+                 ["BA", "BA", "BA", "BA"]]
+
+    def _triplets_gen(self, g, m=None):
+        c = self.couplings(g, m)
+        T = Triplets(self.N)
+        args = (self.configs, self.keys, self.loc_dims)
+        add_local(T, self.ops["E_square"], *args, c["E"])
+        if not self.pure:
+            for stag in ["even", "odd"]:
+                mask = lat.staggered_mask(self.lvals, stag)
+                ms = [mask[lat.zig_zag(self.lvals, i)] for i in range(self.n_sites)]
+                add_local(T, self.ops["N_tot"], *args, c[f"m_{stag}"], ms)
+            for d in self.dirs:
+                for stag in ["even", "odd"]:
+                    bonds = self._bonds(d, stag)
+                    if not bonds:
+                        continue
+                    for names in ([f"Q1_p{d}_dag", f"Q2_m{d}"], [f"Q2_p{d}_dag", f"Q1_m{d}"]):
+                        add_nbody_list(T, [self.ops[n] for n in names], bonds, *args, c[f"t{d}_{stag}"], True)
+        planes = [] if self.dim == 1 else ["xy"] if self.dim == 2 else ["xy", "xz", "yz"]
+        for a, b in planes:
+            plaqs = self._plaqs((a, b))
+            for ps in self.PLAQ_SETS:
+                names = [f"C{ps[0]}_p{a},p{b}", f"C{ps[1]}_p{b},m{a}", f"C{ps[2]}_m{b},p{a}",
+                         f"C{ps[3]}_m{a},m{b}"]
+                add_nbody_list(T, [self.ops[n] for n in names], plaqs, *args, c["B"], True)
         return T
 
     def hamiltonian(self, g, m=None):

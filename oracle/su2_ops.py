@@ -1,4 +1,4 @@
-"""SU(2) spin-1/2 dressed sites (oracle; test infrastructure).
+"""SU(2) dressed sites (oracle; test infrastructure).
 
 Restates, with SymPy's Clebsch-Gordan coefficients as the reference uses them:
   * couple_two_spins / add_new_spin / group_sorted_spin_configs
@@ -7,9 +7,15 @@ Restates, with SymPy's Clebsch-Gordan coefficients as the reference uses them:
                                    operators/SU2_singlets.py:303-420
   * SU2_gauge_invariant_states     operators/SU2_operators.py:625-714
   * SU2_Rishon (j = 1/2)           operators/SU2_rishons.py:50-80
-  * SU2_generators                 operators/spin_operators.py:91-127
+  * SU2_Rishon_gen (any j_max)     operators/SU2_rishons.py:106-234
+  * SU2_generators                 operators/spin_operators.py:75-127
   * fermi_operators (two colours)  operators/bose_fermi_operators.py:14-70
-  * SU2_dressed_site_operators     operators/SU2_operators.py:35-300 (1D, 2D)
+  * SU2_dressed_site_operators     operators/SU2_operators.py:35-300 (1D-3D)
+  * SU2_gen_dressed_site_operators operators/SU2_operators.py:302-592 (1D-3D)
+Pins (tests): the rishon algebra the reference checks (SU2_rishons.py:
+80-103, 179-194), singlet spaces equal to the kernel of the total SU(2)
+generators, and the generalised and hard-coded spin-1/2 models sharing their
+sector and spectrum (validation/su2/test_su2_hamiltonians.py).
 Kronecker order follows qmb_operator (modeling/qmb_operations.py:28-73):
 the first listed factor is the most significant.
 """
@@ -69,18 +75,23 @@ def singlets(spins):
     return groups
 
 
-def gauge_invariant_states(pure_theory, dim):
-    """SU2_gauge_invariant_states for j_max = 1/2: basis[label] (csr, D x d)."""
+def _irreps(j_max):
+    return [S(k) / 2 for k in range(int(2 * j_max) + 1)]
+
+
+def gauge_invariant_states(pure_theory, dim, j_max=S(1) / 2):
+    """SU2_gauge_invariant_states: basis[label] (csr, D x d)."""
     n_link = 2 * dim
     links = [f"m{a}" for a in "xyz"[:dim]] + [f"p{a}" for a in "xyz"[:dim]]
-    rish_j, rish_m = [S(0), S(1) / 2, S(1) / 2], [S(0), S(1) / 2, -S(1) / 2]
+    rish_j = [j for j in _irreps(j_max) for _ in _mvals(j)]
+    rish_m = [m for j in _irreps(j_max) for m in _mvals(j)]
     j_list = [rish_j] * n_link
     m_list = [rish_m] * n_link
     if not pure_theory:
         j_list = [["V", S(1) / 2, S(1) / 2, "P"]] + j_list
         m_list = [[S(0), S(1) / 2, -S(1) / 2, S(0)]] + m_list
-    full = list(zip(product(*j_list), product(*m_list)))
-    spins = [[S(0), S(1) / 2]] * n_link
+    full = {tuple(jc) + tuple(mc): k for k, (jc, mc) in enumerate(zip(product(*j_list), product(*m_list)))}
+    spins = [_irreps(j_max)] * n_link
     if not pure_theory:
         spins = [[S(0), S(1) / 2, S(0)]] + spins
     labels = ["site"] + [f"site_{b}" for b in lat.border_labels(dim)]
@@ -101,11 +112,8 @@ def gauge_invariant_states(pure_theory, dim):
                 Jcfg[0] = "P"
         for _, terms in singlets(cfg):
             vec = np.zeros(len(full))
-            jm = {tuple(Jcfg) + ms: amp for ms, amp in terms}
-            for k, (jc, mc) in enumerate(full):
-                key = tuple(jc) + tuple(mc)
-                if key in jm:
-                    vec[k] = float(jm[key])
+            for ms, amp in terms:
+                vec[full[tuple(Jcfg) + tuple(ms)]] = float(amp)
             link_j = cfg[1:] if not pure_theory else cfg
             zero = {links[k] for k in range(n_link) if link_j[k] == 0}
             for l in labels:
@@ -113,6 +121,66 @@ def gauge_invariant_states(pure_theory, dim):
                 if all(p in zero for p in parts):
                     cols[l].append(vec)
     return {l: sp.csr_matrix(np.column_stack(c)) for l, c in cols.items() if c}
+
+
+def generators(j_max):
+    """SU2_generators(j_max): block_diag over the irreps 0..j_max of the spin
+    matrices of get_spin_operators (spin_operators.py:31-60, 75-127)."""
+    blocks = {"Tz": [sp.csr_matrix((1, 1))], "Tp": [sp.csr_matrix((1, 1))], "T2": [sp.csr_matrix((1, 1))]}
+    for j in _irreps(j_max)[1:]:
+        jj = float(j)
+        mz = np.array([float(m) for m in _mvals(j)])
+        blocks["Tz"].append(sp.diags(mz))
+        blocks["Tp"].append(sp.diags(np.sqrt(jj * (jj + 1) - mz[1:] * (mz[1:] + 1)), 1,
+                                     shape=(len(mz), len(mz))))
+        blocks["T2"].append(sp.diags([jj * (jj + 1)] * len(mz)))
+    out = {k: sp.block_diag(v, format="csr") for k, v in blocks.items()}
+    out["Tm"] = out["Tp"].T.tocsr()
+    out["Tx"] = 0.5 * (out["Tp"] + out["Tm"])
+    out["Ty"] = -0.5j * (out["Tp"] - out["Tm"])
+    return out
+
+
+def rishon_gen(j_max):
+    """SU2_Rishon_gen(j_max).ops: superdiagonal bands of chi coefficients."""
+    largest = int(2 * j_max + 1)
+    size = sum(range(1, largest + 1))
+    shape = (size, size)
+    P_diag = []
+    for s_size in range(largest):
+        P_diag += [(-1) ** s_size] * (s_size + 1)
+    ops = {"P": sp.diags(P_diag, 0, shape, format="csr"), "Iz": sp.identity(size, format="csr")}
+
+    def chi(s, color, m):
+        num = (s + m + 1) if color == "r" else (s - m + 1)
+        return np.sqrt(num / np.sqrt((2 * s + 1) * (2 * s + 2)))
+
+    for color, d0 in (("r", 1), ("g", 2)):
+        entries, offsets = [], []
+        in_zeros, diag = 0, d0
+        for s_size in range(largest - 1):
+            spin = s_size / 2
+            c = [chi(spin, color, float(m)) for m in _mvals(S(s_size) / 2)]
+            out_zeros = size - len(c) - diag - in_zeros
+            entries.append([0.0] * in_zeros + c + [0.0] * out_zeros)
+            offsets.append(diag)
+            diag += 1
+            in_zeros += s_size + 1
+        Z = sp.diags(entries, offsets, shape, format="csr")
+        ops[f"Z{color}"] = Z
+        ops[f"Z{color}_dag"] = Z.T.tocsr()
+        ops[f"ZB_{color}"] = Z
+        ops[f"ZB_{color}_dag"] = Z.T.tocsr()
+        ops[f"ZB_{color}_P"] = Z @ ops["P"]
+        ops[f"P_ZB_{color}_dag"] = ops["P"] @ Z.T
+    ops["ZA_r"] = ops["ZB_g_dag"]
+    ops["ZA_g"] = -ops["ZB_r_dag"]
+    for color in "rg":
+        ops[f"ZA_{color}_dag"] = ops[f"ZA_{color}"].T.tocsr()
+        ops[f"ZA_{color}_P"] = ops[f"ZA_{color}"] @ ops["P"]
+        ops[f"P_ZA_{color}_dag"] = ops["P"] @ ops[f"ZA_{color}_dag"]
+    ops.update(generators(j_max))
+    return ops
 
 
 def _rishon():
@@ -180,24 +248,15 @@ def dressed_site_operators(pure_theory, dim):
             ops[f"{op}_{l}"] = _kron(ins, names)
     if dim == 1:
         ops["U"] = sum(_kron(ins, [f"Z{c}_dag_P", f"Z{c}"]) for c in "rg") / np.sqrt(2)
-    elif dim == 2:
-        ops["C_px,py"] = sum(_kron(ins, ["Iz", "Iz", f"Z{c}_P", f"Z{c}_dag"]) for c in "rg")
-        ops["C_py,mx"] = sum(_kron(ins, [f"P_Z{c}_dag", "P", "P", f"Z{c}"]) for c in "rg")
-        ops["C_mx,my"] = sum(_kron(ins, [f"Z{c}_P", f"Z{c}_dag", "Iz", "Iz"]) for c in "rg")
-        ops["C_my,px"] = sum(_kron(ins, ["Iz", f"Z{c}_P", f"Z{c}_dag", "Iz"]) for c in "rg")
     else:
-        raise NotImplementedError
+        for corner, lst in _corner_lists(dim).items():
+            ops[f"C_{corner}"] = sum(_kron(ins, [f.format(a=f"Z{c}", b=f"Z{c}") for f in lst]) for c in "rg")
     if not pure_theory:
         for k in list(ops):
             ops[k] = sp.kron(ins["ID_psi"], ops[k]).tocsr()
-        if dim == 1:
-            lists = {"Qpx_dag": lambda c: [f"psi_{c}_dag_P", "P", f"Z{c}"],
-                     "Qmx_dag": lambda c: [f"psi_{c}_dag_P", f"Z{c}", "Iz"]}
-        else:
-            lists = {"Qmx_dag": lambda c: [f"psi_{c}_dag_P", f"Z{c}", "Iz", "Iz", "Iz"],
-                     "Qmy_dag": lambda c: [f"psi_{c}_dag_P", "P", f"Z{c}", "Iz", "Iz"],
-                     "Qpx_dag": lambda c: [f"psi_{c}_dag_P", "P", "P", f"Z{c}", "Iz"],
-                     "Qpy_dag": lambda c: [f"psi_{c}_dag_P", "P", "P", "P", f"Z{c}"]}
+        halves = [f"m{a}" for a in "xyz"[:dim]] + [f"p{a}" for a in "xyz"[:dim]]
+        lists = {f"Q{h}_dag": (lambda c, k=k: [f"psi_{c}_dag_P"] + ["P"] * k + [f"Z{c}"] + ["Iz"] * (n_link - k - 1))
+                 for k, h in enumerate(halves)}
         for name, fn in lists.items():
             ops[name] = sum(_kron(ins, fn(c)) for c in "rg")
         for name in list(lists):
@@ -209,10 +268,69 @@ def dressed_site_operators(pure_theory, dim):
     return ops
 
 
-def total_spin_operators(pure_theory, dim):
+def _corner_lists(dim):
+    """Factor lists of the corner operators, half-links in the order m_x,
+    m_y, (m_z,) p_x, p_y, (p_z) (SU2_operators.py:104-131, 200-257, 383-399,
+    431-492); {a} / {b} stand for the first / second rishon of the corner."""
+    if dim == 2:
+        return {"px,py": ["Iz", "Iz", "{a}_P", "{b}_dag"], "py,mx": ["P_{b}_dag", "P", "P", "{a}"],
+                "mx,my": ["{a}_P", "{b}_dag", "Iz", "Iz"], "my,px": ["Iz", "{a}_P", "{b}_dag", "Iz"]}
+    if dim == 3:
+        return {"px,py": ["Iz", "Iz", "Iz", "{a}_P", "{b}_dag", "Iz"],
+                "py,mx": ["P_{b}_dag", "P", "P", "P", "{a}", "Iz"],
+                "mx,my": ["{a}_P", "{b}_dag", "Iz", "Iz", "Iz", "Iz"],
+                "my,px": ["Iz", "{a}_P", "P", "{b}_dag", "Iz", "Iz"],
+                "px,pz": ["Iz", "Iz", "Iz", "{a}_P", "P", "{b}_dag"],
+                "pz,mx": ["P_{b}_dag", "P", "P", "P", "P", "{a}"],
+                "mx,mz": ["{a}_P", "P", "{b}_dag", "Iz", "Iz", "Iz"],
+                "mz,px": ["Iz", "Iz", "{a}_P", "{b}_dag", "Iz", "Iz"],
+                "py,pz": ["Iz", "Iz", "Iz", "Iz", "{a}_P", "{b}_dag"],
+                "pz,my": ["Iz", "P_{b}_dag", "P", "P", "P", "{a}"],
+                "my,mz": ["Iz", "{a}_P", "{b}_dag", "Iz", "Iz", "Iz"],
+                "mz,py": ["Iz", "Iz", "{a}_P", "P", "{b}_dag", "Iz"]}
+    return {}
+
+
+def dressed_site_operators_gen(pure_theory, dim, j_max):
+    """SU2_gen_dressed_site_operators(j_max, pure_theory, dim) (no background)."""
+    ins = rishon_gen(j_max)
+    if not pure_theory:
+        ins.update(_matter())
+    n_link = 2 * dim
+    halves = [f"m{a}" for a in "xyz"[:dim]] + [f"p{a}" for a in "xyz"[:dim]]
+    ops = {}
+    for op in ["T2", "P"]:
+        for k, l in enumerate(halves):
+            names = ["Iz"] * n_link
+            names[k] = op
+            ops[f"{op}_{l}"] = _kron(ins, names)
+    for l1, l2 in product("AB", repeat=2):
+        for corner, lst in _corner_lists(dim).items():
+            ops[f"C{l1}{l2}_{corner}"] = sum(
+                _kron(ins, [f.format(a=f"Z{l1}_{c}", b=f"Z{l2}_{c}") for f in lst]) for c in "rg")
+    if not pure_theory:
+        for k in list(ops):
+            ops[k] = sp.kron(ins["ID_psi"], ops[k]).tocsr()
+        for k, h in enumerate(halves):
+            pre, post = ["P"] * k, ["Iz"] * (n_link - k - 1)
+            ops[f"Q1_{h}_dag"] = (_kron(ins, ["psi_r_dag_P"] + pre + ["Zg_dag"] + post)
+                                  - _kron(ins, ["psi_g_dag_P"] + pre + ["Zr_dag"] + post))
+            ops[f"Q2_{h}_dag"] = (_kron(ins, ["psi_r_dag_P"] + pre + ["Zr"] + post)
+                                  + _kron(ins, ["psi_g_dag_P"] + pre + ["Zg"] + post))
+        for h in halves:
+            for q in ("Q1", "Q2"):
+                ops[f"{q}_{h}"] = ops[f"{q}_{h}_dag"].conj().T.tocsr()
+        for lab in ["r", "g", "tot", "single", "pair", "zero"]:
+            ops[f"N_{lab}"] = _kron(ins, [f"N_{lab}"] + ["Iz"] * n_link)
+    ops["E_square"] = sum(0.5 * ops[f"T2_{s}{d}"] for s in "pm" for d in "xyz"[:dim])
+    return ops
+
+
+def total_spin_operators(pure_theory, dim, j_max=S(1) / 2):
     """Sum of the site's SU(2) generators (matter + every half-link), the
     Gauss-law generators whose common kernel is the singlet space."""
-    ins = _rishon()
+    ins = generators(j_max)
+    ins["Iz"] = sp.identity(ins["Tz"].shape[0], format="csr")
     n_link = 2 * dim
     out = {}
     for a in ("x", "y", "z"):
@@ -229,6 +347,6 @@ def total_spin_operators(pure_theory, dim):
             sz = sp.diags([0.0, 0.5, -0.5, 0.0])
             sp_ = sp.csr_matrix(([1.0], ([1], [2])), shape=(4, 4))
             mat = {"z": sz, "x": 0.5 * (sp_ + sp_.T), "y": -0.5j * (sp_ - sp_.T)}[a]
-            tot = tot + sp.kron(mat, sp.identity(3 ** n_link))
+            tot = tot + sp.kron(mat, sp.identity(ins["Iz"].shape[0] ** n_link))
         out[a] = sp.csr_matrix(tot)
     return out

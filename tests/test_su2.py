@@ -234,3 +234,95 @@ def test_blockwise_schmidt_weights_equal_dense_svd():
         assert np.max(np.abs(got[:n] - ref[:n])) < 1e-13
         assert np.all(ref[n:] < 1e-13) and np.all(got[n:] < 1e-13)
         assert abs(np.sum(got) - 1) < 1e-12
+
+
+# ---------------------------------------------------------------- generalised and 3D operator sets
+@pytest.mark.parametrize("j2", [1, 2, 3])
+def test_generalised_rishon_algebra(j2):
+    """SU2_Rishon_gen's own checks (SU2_rishons.py:179-194): the rishons
+    transform as a spin-1/2 doublet under the generators and anticommute with
+    the parity, for rishon truncations j_max = 1/2, 1, 3/2."""
+    from sympy import Rational
+    o = su2_ops.rishon_gen(Rational(j2, 2))
+    comm = lambda a, b: (a @ b - b @ a).toarray()
+    Zr, Zg = o["Zr"], o["Zg"]
+    assert np.allclose(2 * comm(Zr, o["Tx"]), Zg.toarray(), atol=1e-14)
+    assert np.allclose(2 * comm(Zg, o["Tx"]), Zr.toarray(), atol=1e-14)
+    assert np.allclose(comm(Zr, o["Ty"]), -0.5j * Zg.toarray(), atol=1e-14)
+    assert np.allclose(comm(Zg, o["Ty"]), 0.5j * Zr.toarray(), atol=1e-14)
+    assert np.allclose(2 * comm(Zr, o["Tz"]), Zr.toarray(), atol=1e-14)
+    assert np.allclose(2 * comm(Zg, o["Tz"]), -Zg.toarray(), atol=1e-14)
+    for c in "rg":
+        Z = o[f"Z{c}"]
+        assert np.max(np.abs((Z @ o["P"] + o["P"] @ Z).toarray())) < 1e-15
+    # the engine's dense rishons and generators are the same matrices
+    from fractions import Fraction
+
+    from edlgt_amd.su2_operators import rishon_matrices_gen
+    mine = rishon_matrices_gen(Fraction(j2, 2))
+    for name in ("Zr", "Zg", "ZA_r", "ZA_g", "ZA_r_P", "P_ZB_g_dag", "T2", "Tz", "Tx", "Ty", "P"):
+        assert np.max(np.abs(mine[name] - o[name].toarray())) < 1e-15, name
+
+
+@pytest.mark.parametrize("pure,dim,j2", [(False, 1, 2), (True, 2, 2), (False, 1, 3), (True, 3, 1)])
+def test_generalised_singlet_space_is_the_gauss_kernel(pure, dim, j2):
+    from sympy import Rational
+    j = Rational(j2, 2)
+    basis = su2_ops.gauge_invariant_states(pure, dim, j)
+    B = basis["site"].toarray()
+    T = su2_ops.total_spin_operators(pure, dim, j)
+    assert np.allclose(B.T @ B, np.eye(B.shape[1]), atol=1e-14)
+    for a in "xyz":
+        assert np.max(np.abs(T[a] @ B)) < 1e-13
+    C = sum((T[a].conj().T @ T[a]).toarray() for a in "xyz")
+    assert int(np.sum(np.linalg.eigvalsh(C) < 1e-10)) == B.shape[1]
+
+
+@pytest.mark.parametrize("pure,dim,j2,generic", [(False, 1, 1, True), (True, 2, 1, True), (False, 2, 1, True),
+                                                  (False, 1, 2, True), (True, 2, 2, True), (True, 3, 1, False),
+                                                  (False, 3, 1, False)])
+def test_engine_site_matches_oracle_general(pure, dim, j2, generic):
+    """Bases and every projected operator of the engine's site against the
+    oracle's, for the generalised set and the 3D hard-coded set."""
+    from fractions import Fraction
+
+    from sympy import Rational
+
+    from edlgt_amd.su2_operators import SU2Site
+    site = SU2Site(pure, dim, Fraction(j2, 2), generic)
+    j = Rational(j2, 2)
+    basis = su2_ops.gauge_invariant_states(pure, dim, j)
+    ops = su2_ops.dressed_site_operators_gen(pure, dim, j) if generic else su2_ops.dressed_site_operators(pure, dim)
+    assert set(site.basis) == set(basis)
+    for lab in ("site", sorted(basis)[-1]):
+        Bo = basis[lab].toarray()
+        assert np.max(np.abs(site.basis[lab] - Bo)) < 1e-15
+        mine = site.operators(lab)
+        assert set(mine) == set(ops)
+        for name, O in ops.items():
+            ref = Bo.T @ (O @ Bo)
+            assert np.max(np.abs(mine[name] - ref)) < 1e-14, (lab, name)
+
+
+@pytest.mark.parametrize("lvals,obc,pure", [([4], [True], False), ([4], [False], False),
+                                            ([2, 2], [True, True], False), ([2, 2], [False, False], False),
+                                            ([2, 2], [True, True], True), ([2, 2], [False, False], True),
+                                            ([2, 2, 2], [True] * 3, True)])
+def test_generalised_equals_hard_coded_at_spin_half(lvals, obc, pure):
+    """validation/su2/test_su2_hamiltonians.py (compare_SU2_models,
+    workflows/su2.py:36-89): the generalised operator set at spin 1/2 gives
+    the hard-coded model's sector and its lowest eigenvalues (g = m = 0.1,
+    atol 1e-10)."""
+    from oracle.model import lowest_eigs
+    n = int(np.prod(lvals))
+    sec = None if pure else [n]
+    m = None if pure else 0.1
+    a = SU2Oracle(pure, lvals, obc, sectors=sec)
+    b = SU2Oracle(pure, lvals, obc, sectors=sec, generic=True)
+    assert np.array_equal(a.configs, b.configs)
+    k = min(4 if len(lvals) <= 2 else 2, a.N - 1)
+    if k < 1:
+        return
+    ea = np.linalg.eigvalsh(a.hamiltonian(0.1, m).toarray())[:k] if a.N <= 600 else lowest_eigs(a.hamiltonian(0.1, m), k)
+    eb = np.linalg.eigvalsh(b.hamiltonian(0.1, m).toarray())[:k] if b.N <= 600 else lowest_eigs(b.hamiltonian(0.1, m), k)
+    assert np.max(np.abs(ea - eb)) / n < 1e-10, (ea, eb)
