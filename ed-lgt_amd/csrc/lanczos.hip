@@ -45,6 +45,11 @@ constexpr double kSkipLoss = 1e-14;  // measured loss of orthogonality left unco
 constexpr double kMixedTol32 = 1e-3;
 constexpr double kMixedEta = 1e-4;
 constexpr int kCgMaxIter = 600;
+// Krylov exponential: a step's basis stops growing once its a-posteriori
+// error estimate (relative to |psi|) is below kKrylovTol, tested from
+// kKrylovMin vectors on (the requested m is the maximum)
+constexpr double kKrylovTol = 1e-13;
+constexpr int kKrylovMin = 4;
 double env_double(const char* name, double dflt) {
   const char* e = std::getenv(name);
   return e && *e ? std::atof(e) : dflt;
@@ -1221,11 +1226,34 @@ struct Solver {
     return EDLGT_OK;
   }
 
-  // One Krylov step psi <- exp(-i H dt) psi (Lanczos basis of dimension m,
-  // three-term recurrence plus optional full CGS2 reorthogonalisation).
-  // V[:, 0..m] holds the basis; the projected exponential is formed on the
-  // host from the eigendecomposition of the real tridiagonal T.
-  // err receives the standard a-posteriori estimate beta_m |(e^{-iT dt} e_1)_m|.
+  // a-posteriori error of the Krylov exponential with the first k basis
+  // vectors: beta_k |(e^{-i T_k dt} e_1)_k| (T_k the leading k x k block)
+  static double krylov_estimate(const std::vector<double>& alpha, const std::vector<double>& beta, int k,
+                                double dt) {
+    Dense T(k, k);
+    for (int j = 0; j < k; ++j) {
+      T(j, j) = alpha[j];
+      if (j + 1 < k) {
+        T(j, j + 1) = beta[j];
+        T(j + 1, j) = beta[j];
+      }
+    }
+    std::vector<double> lam;
+    Dense Sv;
+    hermitian_eig(T, lam, Sv);
+    cd z = 0.0;
+    for (int l = 0; l < k; ++l) z += Sv(k - 1, l) * std::exp(cd(0.0, -lam[l] * dt)) * std::conj(Sv(0, l));
+    return beta[k - 1] * std::abs(z);
+  }
+
+  // One Krylov step psi <- exp(-i H dt) psi (Lanczos basis of at most m
+  // vectors, three-term recurrence plus optional full CGS2
+  // reorthogonalisation).  V[:, 0..m] holds the basis; the projected
+  // exponential is formed on the host from the eigendecomposition of the real
+  // tridiagonal T.  The recurrence stops early once the standard
+  // a-posteriori estimate beta_k |(e^{-iT_k dt} e_1)_k| (relative to |psi|)
+  // is below kKrylovTol, tested from kKrylovMin vectors on; err receives that
+  // estimate for the basis used (0 for an invariant subspace).
   int krylov_step(S* psi, double dt, int m, bool reorth, double& err, int& m_used) {
     Dense nn;
     EDLGT_TRY(gram(psi, 1, psi, 1, nn));
@@ -1245,6 +1273,7 @@ struct Solver {
     std::vector<double> nrm(1, 1.0);
     double hnorm = 0.0;
     int me = m;
+    bool exact = false;
     for (int j = 0; j < m; ++j) {
       double a = 0.0, b = 0.0;
       if (reorth) {
@@ -1277,6 +1306,11 @@ struct Solver {
       hnorm = std::max(hnorm, std::fabs(a) + b + (j > 0 ? beta[j - 1] : 0.0));
       if (b <= 1e-14 * std::max(hnorm, 1e-300)) {  // invariant subspace: exact
         me = j + 1;
+        exact = true;
+        break;
+      }
+      if (j + 1 >= kKrylovMin && j + 1 < m && krylov_estimate(alpha, beta, j + 1, dt) <= kKrylovTol) {
+        me = j + 1;
         break;
       }
       if (reorth) {
@@ -1305,7 +1339,7 @@ struct Solver {
       for (int l = 0; l < me; ++l) z += Sv(k, l) * std::exp(cd(0.0, -lam[l] * dt)) * std::conj(Sv(0, l));
       c(k, 0) = beta0 * z;
     }
-    err = (me == m) ? beta[m - 1] * std::abs(c(m - 1, 0)) / beta0 : 0.0;
+    err = exact ? 0.0 : beta[me - 1] * std::abs(c(me - 1, 0)) / beta0;
     for (int k = 0; k < me; ++k) c(k, 0) /= nrm[k];
     EDLGT_TRY(update(V.as<S>(), me, c, psi, 1, 0));
     m_used = me;
