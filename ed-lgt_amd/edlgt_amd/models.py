@@ -1240,9 +1240,12 @@ class SU2_Model(QuantumModel):
             real = mode == "real"
         else:
             raise ValueError(f"dtype_mode must be one of ['auto', 'complex', 'real'], got {dtype_mode!r}")
-        if abs(theta) > 1e-12 or lambda_noise:
-            raise NotImplementedError("theta and Gauss-law noise terms are not on the device path")
+        # the Gauss-law-violating noise term needs background charges and periodic
+        # boundaries (SU2_model.py:338-354); without background charges the
+        # reference adds nothing, as here
         self.SU2_Hamiltonian_couplings(g, m, theta)
+        if abs(self.coeffs["theta"]) > 10e-10 and self.dim == 3 and self.use_generic_model:
+            raise NotImplementedError("the theta term is built with the hard-coded spin-1/2 operators")
         if self.use_generic_model:
             return self._build_generic(m, real)
         c = self.coeffs
@@ -1264,6 +1267,12 @@ class SU2_Model(QuantumModel):
                 for stag in ("even", "odd"):
                     bonds = [b for b in lat.bonds(d) if even[b[0]] == (stag == "even")]
                     self._add_nbody([f"Qp{d}_dag", f"Qm{d}"], bonds, c[f"t{d}_{stag}"], True)
+        if self.dim == 3 and abs(c["theta"]) > 10e-10:
+            # topological term: E_normal times the plaquette, per plane (SU2_model.py:303-336)
+            for names, axes in ((["EzC_px,py", "C_py,mx", "C_my,px", "C_mx,my"], ("x", "y")),
+                                (["EyC_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], ("x", "z")),
+                                (["ExC_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], ("y", "z"))):
+                self._add_nbody(names, lat.plaquettes(axes), c["theta"], True)
         return self._build_program(real)
 
     # corner combinations of the generalised plaquette (SU2_model.py:420-437)

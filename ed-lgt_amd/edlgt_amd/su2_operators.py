@@ -370,6 +370,27 @@ class SU2Site:
         for corner, fac in self._corner_lists().items():
             ops[f"C_{corner}"] = sum(op({l: f.format(a=f"Z{c}", b=f"Z{c}") for l, f in fac.items()})
                                      for c in "rg")
+        if self.dim_lat == 3:
+            # theta-term corners E_nu C (SU2_operators.py:231-250): the first
+            # corner of each plane times the electric field normal to it,
+            # contracted with the spin-1/2 matrices over the rishon colours
+            half = {"x": np.array([[0.0, 0.5], [0.5, 0.0]]), "y": np.array([[0.0, -0.5j], [0.5j, 0.0]]),
+                    "z": np.array([[0.5, 0.0], [0.0, -0.5]])}
+            planes = (("EzC_px,py", "z", {"px": "{1}_P", "py": "{2}_dag"}, 1j),
+                      ("EyC_px,pz", "y", {"px": "{1}_P", "py": "P", "pz": "{2}_dag"}, -1j),
+                      ("ExC_py,pz", "x", {"py": "{1}_P", "pz": "{2}_dag"}, 1j))
+            for name, normal, fac, phase in planes:
+                acc = 0
+                for nu in "xyz":
+                    E = ops[f"T{nu}_p{normal}"] + ops[f"T{nu}_m{normal}"]
+                    for ii, c1 in enumerate("rg"):
+                        for jj, c2 in enumerate("rg"):
+                            f = half[nu][jj, ii]
+                            if f == 0:
+                                continue
+                            corner = op({l: x.format("", f"Z{c1}", f"Z{c2}") for l, x in fac.items()})
+                            acc = acc + phase * (corner @ (f * E))
+                ops[name] = sp.csr_matrix(acc)
         if not self.pure:
             self._add_matter(ops, op)
         ops["E_square"] = sum(0.5 * ops[f"T2_{l}"] for l in links)

@@ -569,9 +569,10 @@ class SU2Oracle:
         self.keys = encode(self.configs, self.loc_dims)
         self.N = self.configs.shape[0]
 
-    def couplings(self, g, m=None):
+    def couplings(self, g, m=None, theta=0.0):
         """SU2_Hamiltonian_couplings (SU2_model.py:555-624)."""
-        c = {"E": 8 * g / 3 if self.dim == 1 else g / 2, "B": 0 if self.dim == 1 else -1 / (2 * g)}
+        c = {"E": 8 * g / 3 if self.dim == 1 else g / 2, "B": 0 if self.dim == 1 else -1 / (2 * g),
+             "theta": -theta * g}
         if not self.pure and m is not None:
             t = 2 * np.sqrt(2)
             c.update({"tx_even": -1j * t, "tx_odd": -1j * t, "ty_even": -t, "ty_odd": t,
@@ -592,12 +593,12 @@ class SU2Oracle:
                 bonds.append(b)
         return bonds
 
-    def triplets(self, g, m=None):
+    def triplets(self, g, m=None, theta=0.0):
         """build_base_Hamiltonian (SU2_model.py:200-355) or build_gen_Hamiltonian
         (:357-484) term order."""
         if self.generic:
             return self._triplets_gen(g, m)
-        c = self.couplings(g, m)
+        c = self.couplings(g, m, theta)
         T = Triplets(self.N)
         args = (self.configs, self.keys, self.loc_dims)
         add_local(T, self.ops["E_square"], *args, c["E"])
@@ -622,6 +623,11 @@ class SU2Oracle:
                     if bonds:
                         add_nbody_list(T, [self.ops[f"Qp{d}_dag"], self.ops[f"Qm{d}"]], bonds, *args,
                                        c[f"t{d}_{stag}"], True)
+        if self.dim == 3 and abs(c["theta"]) > 10e-10:
+            for names, axes in ((["EzC_px,py", "C_py,mx", "C_my,px", "C_mx,my"], "xy"),
+                                (["EyC_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], "xz"),
+                                (["ExC_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], "yz")):
+                add_nbody_list(T, [self.ops[n] for n in names], self._plaqs(axes), *args, c["theta"], True)
         return T
 
     PLAQ_SETS = [["AB", "AB", "AB", "AB"], ["AA", "AB", "BB", "AB"], ["AB", "AB", "AA", "BB"],
@@ -657,5 +663,5 @@ class SU2Oracle:
                 add_nbody_list(T, [self.ops[n] for n in names], plaqs, *args, c["B"], True)
         return T
 
-    def hamiltonian(self, g, m=None):
-        return self.triplets(g, m).csr(real=self.pure)
+    def hamiltonian(self, g, m=None, theta=0.0):
+        return self.triplets(g, m, theta).csr(real=self.pure and abs(theta) <= 1e-12)

@@ -251,6 +251,22 @@ def dressed_site_operators(pure_theory, dim):
     else:
         for corner, lst in _corner_lists(dim).items():
             ops[f"C_{corner}"] = sum(_kron(ins, [f.format(a=f"Z{c}", b=f"Z{c}") for f in lst]) for c in "rg")
+    if dim == 3:
+        # theta-term corners (SU2_operators.py:231-250)
+        sig = {"x": 0.5 * np.array([[0, 1], [1, 0]]), "y": 0.5 * np.array([[0, -1j], [1j, 0]]),
+               "z": 0.5 * np.array([[1, 0], [0, -1]])}
+        for k in ("EzC_px,py", "ExC_py,pz", "EyC_px,pz"):
+            ops[k] = 0
+        for nu in "xyz":
+            Ez = ops[f"T{nu}_pz"] + ops[f"T{nu}_mz"]
+            Ey = ops[f"T{nu}_py"] + ops[f"T{nu}_my"]
+            Ex = ops[f"T{nu}_px"] + ops[f"T{nu}_mx"]
+            for ii, c1 in enumerate("rg"):
+                for jj, c2 in enumerate("rg"):
+                    factor = sig[nu][jj, ii]
+                    ops["EzC_px,py"] = ops["EzC_px,py"] + 1j * _kron(ins, ["Iz", "Iz", "Iz", f"Z{c1}_P", f"Z{c2}_dag", "Iz"]) @ (factor * Ez)
+                    ops["EyC_px,pz"] = ops["EyC_px,pz"] - 1j * _kron(ins, ["Iz", "Iz", "Iz", f"Z{c1}_P", "P", f"Z{c2}_dag"]) @ (factor * Ey)
+                    ops["ExC_py,pz"] = ops["ExC_py,pz"] + 1j * _kron(ins, ["Iz", "Iz", "Iz", "Iz", f"Z{c1}_P", f"Z{c2}_dag"]) @ (factor * Ex)
     if not pure_theory:
         for k in list(ops):
             ops[k] = sp.kron(ins["ID_psi"], ops[k]).tocsr()

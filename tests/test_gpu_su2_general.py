@@ -96,3 +96,23 @@ def test_arguments(gpu):
     gen = SU2_Model(spin=1.0, pure_theory=False, sectors=[4], lvals=[4], has_obc=[True])
     with pytest.raises(NotImplementedError):
         gen.overlap_QMB_state("V")
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_3d_theta_term(gpu, pure):
+    """3D SU(2) with the topological term (SU2_model.py:303-336): sector and
+    CSR bit-exact against the oracle, complex H, energies within 1e-10."""
+    from edlgt_amd import SU2_Model
+    print(f"SU(2) 3D theta pure={pure}", flush=True)
+    sec = None if pure else [8]
+    m = None if pure else 0.5
+    orc = SU2Oracle(pure, [2, 2, 2], [True] * 3, sectors=sec)
+    mod = SU2_Model(spin=0.5, pure_theory=pure, sectors=sec, lvals=[2, 2, 2], has_obc=[True] * 3)
+    Hg = mod.build_Hamiltonian(1.1, m, theta=0.45)
+    assert Hg.is_complex
+    Ho = orc.hamiltonian(1.1, m, theta=0.45)
+    _csr_match(Hg, Ho)
+    k = min(3, orc.N - 1)
+    mod.diagonalize_Hamiltonian(k, "sparse")
+    ref = lowest_eigs(Ho, k) if orc.N > 600 else np.linalg.eigvalsh(Ho.toarray())[:k]
+    assert np.max(np.abs(mod.H.Nenergies * 8 - ref)) < 1e-10

@@ -326,3 +326,33 @@ def test_generalised_equals_hard_coded_at_spin_half(lvals, obc, pure):
     ea = np.linalg.eigvalsh(a.hamiltonian(0.1, m).toarray())[:k] if a.N <= 600 else lowest_eigs(a.hamiltonian(0.1, m), k)
     eb = np.linalg.eigvalsh(b.hamiltonian(0.1, m).toarray())[:k] if b.N <= 600 else lowest_eigs(b.hamiltonian(0.1, m), k)
     assert np.max(np.abs(ea - eb)) / n < 1e-10, (ea, eb)
+
+
+def test_su2_3d_theta_term_oracle_pins():
+    """3D theta term (E.B per plane, SU2_model.py:303-336): H(theta) is
+    Hermitian, H(-theta) = conj(H(theta)) (the term is odd under time
+    reversal), so the spectrum is even in theta; and the term changes it."""
+    orc = SU2Oracle(True, [2, 2, 2], [True] * 3)
+    H0 = orc.hamiltonian(1.2).toarray()
+    Hp = orc.hamiltonian(1.2, theta=0.7).toarray()
+    Hm = orc.hamiltonian(1.2, theta=-0.7).toarray()
+    assert np.max(np.abs(Hp - Hp.conj().T)) < 1e-14
+    assert np.max(np.abs(Hm - Hp.conj())) < 1e-14
+    ep, em, e0 = (np.linalg.eigvalsh(h) for h in (Hp, Hm, H0))
+    assert np.max(np.abs(ep - em)) < 1e-12
+    assert np.max(np.abs(ep - e0)) > 1e-3
+
+
+def test_engine_su2_3d_theta_operators_match_oracle():
+    from fractions import Fraction
+
+    from edlgt_amd.su2_operators import SU2Site
+    site = SU2Site(True, 3, Fraction(1, 2), False)
+    basis = su2_ops.gauge_invariant_states(True, 3)
+    ops = su2_ops.dressed_site_operators(True, 3)
+    B = basis["site"].toarray()
+    mine = site.operators("site")
+    for name in ("EzC_px,py", "EyC_px,pz", "ExC_py,pz"):
+        ref = B.T @ (ops[name] @ B)
+        assert np.max(np.abs(mine[name] - ref)) < 1e-14, name
+        assert np.max(np.abs(ref)) > 0.1
