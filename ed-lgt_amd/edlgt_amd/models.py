@@ -7,6 +7,7 @@ assembly, eigensolves and time evolution run on the GPU through
 libedlgt_hip; only the tiny per-site operator tables are built on the host.
 """
 import logging
+from fractions import Fraction
 from math import prod
 
 import numpy as np
@@ -1160,12 +1161,22 @@ class SU2_Model(QuantumModel):
 
     def __init__(self, spin, pure_theory, bg_list=None, sectors=None, use_generic_model=False, **kwargs):
         super().__init__(**kwargs)
-        from fractions import Fraction
         s2 = 2 * float(spin)
         if s2 < 1 or abs(s2 - round(s2)) > 1e-12:
             raise ValueError(f"spin must be a positive multiple of 1/2, not {spin}")
-        if bg_list is not None and any(b != 0 for b in bg_list):
-            raise NotImplementedError("background charges are not supported by the device engine")
+        # static background charges (SU2_model.py:63-64, 100-104): one spin
+        # per site, the sites' bases restricted to their charge
+        background = Fraction(0)
+        self.bg_list = None
+        if bg_list is not None:
+            bgv = np.asarray(bg_list, dtype=float)
+            if bgv.shape != (self.n_sites,):
+                raise ValueError(f"len(bg_list) must match n_sites = {self.n_sites}: got {bgv.shape}")
+            if np.any(bgv < 0) or np.any(np.abs(2 * bgv - np.round(2 * bgv)) > 1e-12):
+                raise ValueError("background charges are spins: non-negative multiples of 1/2")
+            background = Fraction(int(round(2 * bgv.max())), 2)
+            if background > 0:
+                self.bg_list = bgv.tolist()
         if self.dim > 3:
             raise ValueError("SU(2) models are defined in 1D, 2D and 3D")
         from .su2_operators import SU2Site
@@ -1182,13 +1193,20 @@ class SU2_Model(QuantumModel):
         # (translational_sym.py:1013-1014, 1171-1172)
         if not pure_theory and self.dim == 1 and not generic:
             self.tc_symmetry_tables = (np.array([4, 5, 2, 3, 0, 1]), np.array([1.0, 1.0, 1.0, -1.0, 1.0, 1.0]))
-        self.background = 0
-        site = SU2Site(self.pure_theory, self.dim, s_max, generic)
+        self.background = float(background)
+        site = SU2Site(self.pure_theory, self.dim, s_max, generic, background)
         if max(site.dim(l) for l in site.basis) > 255:
             raise NotImplementedError(f"a dressed site of {max(site.dim(l) for l in site.basis)} states "
                                       "exceeds the engine's 255 local states")
         self._site = site
-        labels = [self.lattice.label(i, False) for i in range(self.n_sites)]
+        labels = []
+        for i in range(self.n_sites):
+            lab = self.lattice.label(i, False)
+            if self.bg_list is not None:
+                lab = f"{lab}@bg{int(round(2 * self.bg_list[i]))}"
+            if lab not in site.basis:
+                raise ValueError(f"No GI states at site {i} label={lab}")
+            labels.append(lab)
         self._project(site, site.operators, labels)
 
         def diag(name):
@@ -1273,6 +1291,12 @@ class SU2_Model(QuantumModel):
                                 (["EyC_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], ("x", "z")),
                                 (["ExC_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], ("y", "z"))):
                 self._add_nbody(names, lat.plaquettes(axes), c["theta"], True)
+        if self.background > 0 and lambda_noise > 0 and not any(self.has_obc):
+            # Gauss-law-violating hopping through the background charges (SU2_model.py:338-354)
+            if "Vpx_dag" not in self.ops:
+                raise NotImplementedError("the noise term exists for the 1D chain with matter and "
+                                          "spin-1/2 background charges")
+            self._add_nbody(["Vpx_dag", "Vmx"], lat.bonds("x"), -lambda_noise, True)
         return self._build_program(real)
 
     # corner combinations of the generalised plaquette (SU2_model.py:420-437)

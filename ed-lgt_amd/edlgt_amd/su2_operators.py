@@ -250,11 +250,17 @@ class SU2Site:
     isometries basis[label] (full site dim x d_loc), for rishon irreps up to
     s_max; generic=True selects the generalised operator set."""
 
-    def __init__(self, pure_theory, dim, s_max=HALF, generic=False):
+    def __init__(self, pure_theory, dim, s_max=HALF, generic=False, background=0):
         self.pure = bool(pure_theory)
         self.dim_lat = int(dim)
         self.s_max = Fraction(s_max).limit_denominator(2)
         self.generic = bool(generic)
+        # static background charge: a leading spin of irrep 0..background per
+        # site, coupled first (SU2_operators.py:625-714, SU2_singlets.py:329-387);
+        # labels get the suffix "@bg<2j>" of the site's charge
+        self.bg = Fraction(background).limit_denominator(2)
+        self.bg_irreps = link_irreps(self.bg) if self.bg > 0 else []
+        self.bgdim = len(rishon_states(self.bg)) if self.bg > 0 else 1
         if not self.generic and self.s_max != HALF:
             raise ValueError("the hard-coded SU(2) operator set is the spin-1/2 one")
         axes = "xyz"[:dim]
@@ -266,20 +272,27 @@ class SU2Site:
         for j in link_irreps(self.s_max):
             self._roff[j] = k
             k += int(2 * j) + 1
-        self.full_dim = (1 if self.pure else 4) * self.rdim ** n_link
+        self.full_dim = self.bgdim * (1 if self.pure else 4) * self.rdim ** n_link
         from .operators import _border_parts
-        labels = ["site"] + ["site_" + "_".join(p) for p in _border_parts(dim)]
+        base_labels = ["site"] + ["site_" + "_".join(p) for p in _border_parts(dim)]
+        labels = base_labels if not self.bg_irreps else [f"{b}@bg{int(2 * j)}" for b in base_labels
+                                                         for j in self.bg_irreps]
         cols = {lab: [] for lab in labels}
         vals = {lab: [] for lab in labels}
         link_choices = [link_irreps(self.s_max)] * n_link
         matter_choices = [] if self.pure else [["V", "S", "P"]]
-        for combo in product(*(matter_choices + link_choices)):
+        bg_choices = [self.bg_irreps] if self.bg_irreps else []
+        for combo in product(*(bg_choices + matter_choices + link_choices)):
+            combo = list(combo)
+            jbg = combo.pop(0) if self.bg_irreps else None
             if self.pure:
-                mkind, links = None, list(combo)
-                spins = links
+                mkind, links = None, combo
+                spins = list(links)
             else:
-                mkind, links = combo[0], list(combo[1:])
-                spins = [HALF if mkind == "S" else Fraction(0)] + links
+                mkind, links = combo[0], combo[1:]
+                spins = [HALF if mkind == "S" else Fraction(0)] + list(links)
+            if jbg is not None:
+                spins = [jbg] + spins
             j2 = [int(2 * s) for s in spins]
             tot = sum(j2)
             if tot % 2 or max(j2) > tot - max(j2):
@@ -287,19 +300,28 @@ class SU2Site:
             for Js, terms in su2_singlets(spins):
                 vec = np.zeros(self.full_dim)
                 for ms, amp in terms:
-                    vec[self._index(mkind, spins, ms)] = amp
+                    vec[self._index(mkind, spins, ms, jbg)] = amp
                 zero_links = {self.links[k] for k, s in enumerate(links) if s == 0}
-                jv = ([{"V": 0, "S": 1, "P": 2}[mkind]] if not self.pure else []) + [int(2 * s) for s in links]
-                for lab in labels:
-                    parts = lab.split("_")[1:]
+                jv = ([int(2 * jbg)] if jbg is not None else []) + \
+                     ([{"V": 0, "S": 1, "P": 2}[mkind]] if not self.pure else []) + [int(2 * s) for s in links]
+                for b in base_labels:
+                    parts = b.split("_")[1:]
                     if all(p in zero_links for p in parts):
+                        lab = b if jbg is None else f"{b}@bg{int(2 * jbg)}"
                         cols[lab].append(vec)
                         vals[lab].append(jv)
         self.basis = {lab: np.column_stack(cols[lab]) for lab in labels if cols[lab]}
         self.values = {lab: np.asarray(vals[lab], dtype=np.int64) for lab in labels if cols[lab]}
 
-    def _index(self, mkind, spins, ms):
+    def _index(self, mkind, spins, ms, jbg=None):
+        """Full-site index: background (if any, most significant), matter
+        (vacuum, r, g, pair), then one rishon index per half-link."""
         idx = 0
+        if jbg is not None:
+            # background irreps 0..bg, m descending within each
+            off = sum(int(2 * j) + 1 for j in self.bg_irreps if j < jbg)
+            idx = off + int(jbg - ms[0])
+            spins, ms = spins[1:], ms[1:]
         k0 = 0
         if not self.pure:
             if mkind == "V":
@@ -308,10 +330,10 @@ class SU2Site:
                 mi = 3
             else:
                 mi = 1 if ms[0] > 0 else 2
-            idx = mi
+            idx = idx * 4 + mi
             k0 = 1
-        for s, m in zip(spins[k0:], ms[k0:]):
-            idx = idx * self.rdim + self._roff[s] + int(s - m)
+        for s_, m in zip(spins[k0:], ms[k0:]):
+            idx = idx * self.rdim + self._roff[s_] + int(s_ - m)
         return idx
 
     def dim(self, label):
@@ -319,12 +341,36 @@ class SU2Site:
 
     def _op_factory(self, R, Mt):
         links = self.links
+        bg = generators(self.bg) if self.bg > 0 else None
+        if bg is not None:
+            # the background slot carries its generators and, for the noise
+            # term, the spin-1/2 rishons (SU2_operators.py:272-298)
+            bg = dict(bg)
+            if self.bg == HALF:
+                r12 = rishon_matrices()
+                for c in "rg":
+                    bg[f"Z{c}"] = r12[f"Z{c}"]
+                    bg[f"Z{c}_dag"] = r12[f"Z{c}_dag"]
 
-        def op(assign, matter="ID_psi"):
-            mats = [] if self.pure else [Mt[matter]]
+        def op(assign, matter="ID_psi", background="Iz"):
+            mats = [] if bg is None else [bg[background]]
+            mats += [] if self.pure else [Mt[matter]]
             mats += [R[assign.get(l, "Iz")] for l in links]
             return _kron(mats)
         return op
+
+    def _add_background(self, ops, op):
+        """The site's background Casimir and, in 1D with matter and charges of
+        spin 1/2, the Gauss-law-violating hoppings V (SU2_operators.py:272-298)."""
+        if self.bg == 0:
+            return
+        ops["bg"] = op({}, background="T2")
+        if self.dim_lat == 1 and not self.pure and self.bg == HALF:
+            ops["Vmx_dag"] = sum(op({"mx": f"Z{c}"}, matter="P_psi", background=f"Z{c}_dag") for c in "rg")
+            ops["Vpx_dag"] = sum(op({"mx": "P", "px": f"Z{c}"}, matter="P_psi", background=f"Z{c}_dag")
+                                 for c in "rg")
+            ops["Vmx"] = ops["Vmx_dag"].conj().T.tocsr()
+            ops["Vpx"] = ops["Vpx_dag"].conj().T.tocsr()
 
     def _corner_lists(self):
         """Corner operator factor lists per (corner, colour) as (half-link ->
@@ -394,6 +440,7 @@ class SU2Site:
         if not self.pure:
             self._add_matter(ops, op)
         ops["E_square"] = sum(0.5 * ops[f"T2_{l}"] for l in links)
+        self._add_background(ops, op)
         return ops
 
     def _generic_operators(self):
@@ -425,6 +472,8 @@ class SU2Site:
             for lab in ("r", "g", "tot", "single", "pair", "zero"):
                 ops[f"N_{lab}"] = op({}, matter=f"N_{lab}")
         ops["E_square"] = sum(0.5 * ops[f"T2_{l}"] for l in links)
+        if self.bg > 0:
+            ops["bg"] = op({}, background="T2")
         return ops
 
     def _add_matter(self, ops, op):

@@ -528,7 +528,7 @@ class SU2Oracle:
     200-355, 357-484, 555-624): the hard-coded spin-1/2 operators, or the
     generalised ones (generic=True; any rishon truncation spin), 1D-3D."""
 
-    def __init__(self, pure_theory, lvals, has_obc, sectors=None, spin=0.5, generic=False):
+    def __init__(self, pure_theory, lvals, has_obc, sectors=None, spin=0.5, generic=False, bg_list=None):
         from sympy import Rational
         from . import su2_ops
         self.pure = pure_theory
@@ -538,13 +538,22 @@ class SU2Oracle:
         self.dirs = "xyz"[: self.dim]
         j_max = Rational(int(round(2 * spin)), 2)
         self.generic = bool(generic) or j_max >= 1
-        basis = su2_ops.gauge_invariant_states(pure_theory, self.dim, j_max)
+        # static background charges (SU2_model.py:63-64; quantum_model.py:440-466)
+        bg = Rational(int(round(2 * max(bg_list))), 2) if bg_list is not None else 0
+        self.bg_list = list(bg_list) if bg else None
+        if self.generic and bg:
+            raise NotImplementedError("background charges are restated for the hard-coded set")
+        basis, bgcol = su2_ops.gauge_invariant_states(pure_theory, self.dim, j_max, bg, return_bg=True)
         ops = (su2_ops.dressed_site_operators_gen(pure_theory, self.dim, j_max) if self.generic
-               else su2_ops.dressed_site_operators(pure_theory, self.dim))
+               else su2_ops.dressed_site_operators(pure_theory, self.dim, bg))
         self.labels = [lat.site_label(lat.zig_zag(self.lvals, i), self.lvals, self.has_obc, False)
                        for i in range(self.n_sites)]
         self.site_basis = basis
-        self.ops, self.loc_dims = project_operators(ops, basis, self.labels)
+        cols = None
+        if bg:
+            cols = [np.flatnonzero(np.isclose(bgcol[l], q, rtol=0.0, atol=1e-12))
+                    for l, q in zip(self.labels, self.bg_list)]
+        self.ops, self.loc_dims = project_operators(ops, basis, self.labels, site_cols=cols)
         dg = lambda name: np.real(np.einsum("sii->si", self.ops[name]))
         pairs = lat.link_pairs(self.lvals, self.has_obc)
         link_diags = np.stack([np.stack([dg(f"T2_p{d}"), -dg(f"T2_m{d}")]) for d in self.dirs])
@@ -593,7 +602,7 @@ class SU2Oracle:
                 bonds.append(b)
         return bonds
 
-    def triplets(self, g, m=None, theta=0.0):
+    def triplets(self, g, m=None, theta=0.0, noise=0.0):
         """build_base_Hamiltonian (SU2_model.py:200-355) or build_gen_Hamiltonian
         (:357-484) term order."""
         if self.generic:
@@ -628,6 +637,10 @@ class SU2Oracle:
                                 (["EyC_px,pz", "C_pz,mx", "C_mz,px", "C_mx,mz"], "xz"),
                                 (["ExC_py,pz", "C_pz,my", "C_mz,py", "C_my,mz"], "yz")):
                 add_nbody_list(T, [self.ops[n] for n in names], self._plaqs(axes), *args, c["theta"], True)
+        if self.bg_list is not None and noise > 0 and not any(self.has_obc):
+            bonds = [b for b in (lat.neighbor_sites(lat.zig_zag(self.lvals, i), self.lvals, "x", self.has_obc)
+                                 for i in range(self.n_sites)) if b is not None]
+            add_nbody_list(T, [self.ops["Vpx_dag"], self.ops["Vmx"]], bonds, *args, -noise, True)
         return T
 
     PLAQ_SETS = [["AB", "AB", "AB", "AB"], ["AA", "AB", "BB", "AB"], ["AB", "AB", "AA", "BB"],
@@ -663,5 +676,5 @@ class SU2Oracle:
                 add_nbody_list(T, [self.ops[n] for n in names], plaqs, *args, c["B"], True)
         return T
 
-    def hamiltonian(self, g, m=None, theta=0.0):
-        return self.triplets(g, m, theta).csr(real=self.pure and abs(theta) <= 1e-12)
+    def hamiltonian(self, g, m=None, theta=0.0, noise=0.0):
+        return self.triplets(g, m, theta, noise).csr(real=self.pure and abs(theta) <= 1e-12)

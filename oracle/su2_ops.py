@@ -79,8 +79,10 @@ def _irreps(j_max):
     return [S(k) / 2 for k in range(int(2 * j_max) + 1)]
 
 
-def gauge_invariant_states(pure_theory, dim, j_max=S(1) / 2):
-    """SU2_gauge_invariant_states: basis[label] (csr, D x d)."""
+def gauge_invariant_states(pure_theory, dim, j_max=S(1) / 2, background=0, return_bg=False):
+    """SU2_gauge_invariant_states: basis[label] (csr, D x d); with a
+    background (spin irreps 0..background as the leading, first-coupled dof)
+    and return_bg, also the background spin of every column per label."""
     n_link = 2 * dim
     links = [f"m{a}" for a in "xyz"[:dim]] + [f"p{a}" for a in "xyz"[:dim]]
     rish_j = [j for j in _irreps(j_max) for _ in _mvals(j)]
@@ -90,13 +92,22 @@ def gauge_invariant_states(pure_theory, dim, j_max=S(1) / 2):
     if not pure_theory:
         j_list = [["V", S(1) / 2, S(1) / 2, "P"]] + j_list
         m_list = [[S(0), S(1) / 2, -S(1) / 2, S(0)]] + m_list
+    bg = S(background)
+    if bg:
+        j_list = [[j for j in _irreps(bg) for _ in _mvals(j)]] + j_list
+        m_list = [[m for j in _irreps(bg) for m in _mvals(j)]] + m_list
     full = {tuple(jc) + tuple(mc): k for k, (jc, mc) in enumerate(zip(product(*j_list), product(*m_list)))}
     spins = [_irreps(j_max)] * n_link
     if not pure_theory:
         spins = [[S(0), S(1) / 2, S(0)]] + spins
+    vind = 0
+    if bg:
+        spins = [_irreps(bg)] + spins
+        vind = 1
     labels = ["site"] + [f"site_{b}" for b in lat.border_labels(dim)]
     cols = {l: [] for l in labels}
-    n_after = int(np.prod([len(s) for s in spins[1:]]))
+    bgs = {l: [] for l in labels}
+    n_after = int(np.prod([len(s) for s in spins[vind + 1:]]))
     for ii, cfg in enumerate(product(*spins)):
         cfg = list(cfg)
         j2 = [int(2 * x) for x in cfg]
@@ -107,20 +118,24 @@ def gauge_invariant_states(pure_theory, dim, j_max=S(1) / 2):
         if not pure_theory:
             sector = (ii // n_after) % 3
             if sector == 0:
-                Jcfg[0] = "V"
+                Jcfg[vind] = "V"
             elif sector == 2:
-                Jcfg[0] = "P"
+                Jcfg[vind] = "P"
         for _, terms in singlets(cfg):
             vec = np.zeros(len(full))
             for ms, amp in terms:
                 vec[full[tuple(Jcfg) + tuple(ms)]] = float(amp)
-            link_j = cfg[1:] if not pure_theory else cfg
+            link_j = cfg[vind + (0 if pure_theory else 1):]
             zero = {links[k] for k in range(n_link) if link_j[k] == 0}
             for l in labels:
                 parts = l.split("_")[1:]
                 if all(p in zero for p in parts):
                     cols[l].append(vec)
-    return {l: sp.csr_matrix(np.column_stack(c)) for l, c in cols.items() if c}
+                    bgs[l].append(float(cfg[0]) if bg else 0.0)
+    out = {l: sp.csr_matrix(np.column_stack(c)) for l, c in cols.items() if c}
+    if return_bg:
+        return out, {l: np.array(bgs[l]) for l in out}
+    return out
 
 
 def generators(j_max):
@@ -234,8 +249,31 @@ def _kron(ops, names):
     return sp.csr_matrix(out)
 
 
-def dressed_site_operators(pure_theory, dim):
-    """SU2_dressed_site_operators(1/2, pure_theory, dim) for dim 1, 2."""
+def _with_background(ops, pure_theory, dim, background):
+    """kron(identity(bg), op) for every operator, the background Casimir
+    'bg' and, in 1D with matter, the V hoppings (SU2_operators.py:272-298)."""
+    gen = generators(S(background))
+    nbg = gen["T2"].shape[0]
+    out = {k: sp.kron(sp.identity(nbg), v).tocsr() for k, v in ops.items()}
+    ins = _rishon()
+    ins["T2_bg"] = gen["T2"]
+    n_link = 2 * dim
+    ids = ["Iz"] * n_link if pure_theory else ["ID_psi"] + ["Iz"] * n_link
+    if not pure_theory:
+        ins.update(_matter())
+    out["bg"] = _kron(ins, ["T2_bg"] + ids)
+    if dim == 1 and not pure_theory:
+        out["Vmx_dag"] = sum(_kron(ins, [f"Z{c}_dag", "P_psi", f"Z{c}", "Iz"]) for c in "rg")
+        out["Vpx_dag"] = sum(_kron(ins, [f"Z{c}_dag", "P_psi", "P", f"Z{c}"]) for c in "rg")
+        out["Vmx"] = out["Vmx_dag"].conj().T.tocsr()
+        out["Vpx"] = out["Vpx_dag"].conj().T.tocsr()
+    return out
+
+
+def dressed_site_operators(pure_theory, dim, background=0):
+    """SU2_dressed_site_operators(1/2, pure_theory, dim, background)."""
+    if background:
+        return _with_background(dressed_site_operators(pure_theory, dim), pure_theory, dim, background)
     ins = _rishon()
     if not pure_theory:
         ins.update(_matter())

@@ -116,3 +116,50 @@ def test_3d_theta_term(gpu, pure):
     mod.diagonalize_Hamiltonian(k, "sparse")
     ref = lowest_eigs(Ho, k) if orc.N > 600 else np.linalg.eigvalsh(Ho.toarray())[:k]
     assert np.max(np.abs(mod.H.Nenergies * 8 - ref)) < 1e-10
+
+
+BG_CASES = [
+    # lvals, has_obc, pure, bg_list, noise
+    ([4], [True], False, [0.5, 0, 0, 0.5], 0.0),
+    ([4], [False], False, [0.5, 0, 0.5, 0], 0.0),
+    ([4], [False], False, [0.5, 0, 0.5, 0], 0.3),   # Gauss-law-violating hopping
+    ([2, 2], [True, True], True, [0.5, 0, 0, 0.5], 0.0),
+    ([2, 2], [True, True], False, [0.5, 0, 0, 0.5], 0.0),
+]
+
+
+@pytest.mark.parametrize("lvals,obc,pure,bg,noise", BG_CASES)
+def test_background_charges(gpu, lvals, obc, pure, bg, noise):
+    """SU2_Model(bg_list=...) (SU2_model.py:63-104, 338-354;
+    quantum_model.py:440-466): sector and CSR bit-exact against the oracle,
+    energies within 1e-10."""
+    from edlgt_amd import SU2_Model
+    print(f"SU(2) background lvals={lvals} obc={obc} pure={pure} bg={bg} noise={noise}", flush=True)
+    n = int(np.prod(lvals))
+    sec = None if pure else [n]
+    m = None if pure else 0.8
+    orc = SU2Oracle(pure, lvals, obc, sectors=sec, bg_list=bg)
+    mod = SU2_Model(spin=0.5, pure_theory=pure, sectors=sec, lvals=lvals, has_obc=obc, bg_list=bg)
+    assert mod.background == 0.5 and mod.bg_list == [float(b) for b in bg]
+    assert np.array_equal(mod.loc_dims, orc.loc_dims)
+    assert np.array_equal(mod.sector_configs, orc.configs)
+    Hg = mod.build_Hamiltonian(1.1, m, lambda_noise=noise)
+    Ho = orc.hamiltonian(1.1, m, noise=noise)
+    _csr_match(Hg, Ho)
+    k = min(2, orc.N - 1) if orc.N > 1 else 1
+    mod.diagonalize_Hamiltonian(k, "sparse")
+    ref = np.linalg.eigvalsh(Ho.toarray())[:k]
+    assert np.max(np.abs(mod.H.Nenergies * n - ref)) < 1e-10
+
+
+def test_background_flux_string_closed_form(gpu):
+    """Pure chain, charges 1/2 at sites 1 and 4 of 6 (OBC): one state, energy
+    2 g per link between the charges."""
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=0.5, pure_theory=True, lvals=[6], has_obc=[True], bg_list=[0, 0.5, 0, 0, 0.5, 0])
+    assert mod.sector_dim == 1
+    mod.build_Hamiltonian(1.3)
+    mod.diagonalize_Hamiltonian(1, "sparse")
+    assert abs(mod.H.Nenergies[0] * 6 - 2 * 1.3 * 3) < 1e-12
+    with pytest.raises(ValueError):
+        SU2_Model(spin=0.5, pure_theory=True, lvals=[4], has_obc=[True], bg_list=[0.3, 0, 0, 0])

@@ -356,3 +356,51 @@ def test_engine_su2_3d_theta_operators_match_oracle():
         ref = B.T @ (ops[name] @ B)
         assert np.max(np.abs(mine[name] - ref)) < 1e-14, name
         assert np.max(np.abs(ref)) > 0.1
+
+
+# ---------------------------------------------------------------- background charges
+@pytest.mark.parametrize("L,qpos", [(4, (0, 3)), (6, (0, 5)), (6, (2, 3)), (5, (1, 4))])
+def test_su2_background_flux_string_closed_form(L, qpos):
+    """Pure 1D SU(2), OBC, spin-1/2 static charges at two sites: Gauss's law
+    forces j = 1/2 on every link between them and j = 0 elsewhere, a single
+    sector state whose energy is E sum_half-links T2/2 = (8g/3)(3/4) per link
+    = 2g per link (SU2_model.py:555-624 couplings, E_square = sum T2/2)."""
+    bg = [0.0] * L
+    bg[qpos[0]] = bg[qpos[1]] = 0.5
+    g = 1.3
+    orc = SU2Oracle(True, [L], [True], bg_list=bg)
+    assert orc.N == 1
+    assert abs(orc.hamiltonian(g).toarray()[0, 0] - 2 * g * (qpos[1] - qpos[0])) < 1e-13
+    # every site's background Casimir is its charge's j(j+1)
+    for i in range(L):
+        d = orc.loc_dims[i]
+        assert np.allclose(np.diag(orc.ops["bg"][i][:d, :d]).real, bg[i] * (bg[i] + 1))
+
+
+def test_su2_zero_background_is_the_plain_model():
+    a = SU2Oracle(False, [4], [True], sectors=[4])
+    b = SU2Oracle(False, [4], [True], sectors=[4], bg_list=[0, 0, 0, 0])
+    assert b.bg_list is None and np.array_equal(a.configs, b.configs)
+
+
+@pytest.mark.parametrize("pure,dim", [(False, 1), (True, 1), (True, 2), (False, 2)])
+def test_engine_site_with_background_matches_oracle(pure, dim):
+    from fractions import Fraction
+
+    from edlgt_amd.su2_operators import SU2Site
+    site = SU2Site(pure, dim, Fraction(1, 2), False, background=Fraction(1, 2))
+    basis, bgcol = su2_ops.gauge_invariant_states(pure, dim, background=Fraction(1, 2), return_bg=True)
+    ops = su2_ops.dressed_site_operators(pure, dim, background=Fraction(1, 2))
+    for lab, Bo in basis.items():
+        for k in (0, 1):
+            cols = np.flatnonzero(np.isclose(bgcol[lab], k / 2))
+            mine_lab = f"{lab}@bg{k}"
+            if cols.size == 0:
+                assert mine_lab not in site.basis
+                continue
+            B = Bo.toarray()[:, cols]
+            assert np.max(np.abs(site.basis[mine_lab] - B)) < 1e-15
+            mine = site.operators(mine_lab)
+            assert set(mine) == set(ops), (set(mine) ^ set(ops))
+            for name, O in ops.items():
+                assert np.max(np.abs(mine[name] - B.T @ (O @ B))) < 1e-14, (mine_lab, name)
