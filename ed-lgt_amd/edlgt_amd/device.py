@@ -364,6 +364,17 @@ class CSRMatrix:
             m.sum_duplicates()
         return m
 
+    def toarray(self):
+        """Dense host copy (the reference's ham_format="dense" Hamiltonian,
+        qmb_hamiltonian.py:218-225), for sectors of at most 16384 states."""
+        if self.shape[0] > 16384:
+            raise MemoryError(f"a dense copy of a {self.shape[0]}-state Hamiltonian is not made; use to_scipy()")
+        return self.to_scipy().toarray()
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.toarray()
+        return a if dtype is None else a.astype(dtype)
+
     def vec_type(self, complex_vectors=None):
         cplx = self.is_complex if complex_vectors is None else complex_vectors
         return L.VALUE_C128 if cplx else L.VALUE_F64

@@ -34,7 +34,8 @@ def test_integrated_hamiltonian_energy_e2(gpu, L, g, m, q):
     assert list(mod.loc_dims) == [2] * L
     H = mod.build_Hamiltonian(g, m)
     Ho = iq.hamiltonian(L, g, m, qq).toarray()
-    Hg = H.to_scipy().toarray()
+    # the reference's own check reads the dense matrix: np.asarray(model.H.Ham)
+    Hg = np.asarray(mod.H.Ham, dtype=np.complex128)
     assert np.max(np.abs(Hg - Ho)) <= 1e-12 * max(1.0, np.max(np.abs(Ho)))
     w, v = np.linalg.eigh(Ho)
     mod.diagonalize_Hamiltonian(1, "sparse")
