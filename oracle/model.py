@@ -524,7 +524,7 @@ class ZnOracle:
 
 
 class SU2Oracle:
-    """SU2_Model (no background) restated (models/SU2_model.py:31-171,
+    """SU2_Model restated (models/SU2_model.py:31-171,
     200-355, 357-484, 555-624): the hard-coded spin-1/2 operators, or the
     generalised ones (generic=True; any rishon truncation spin), 1D-3D."""
 
@@ -541,10 +541,8 @@ class SU2Oracle:
         # static background charges (SU2_model.py:63-64; quantum_model.py:440-466)
         bg = Rational(int(round(2 * max(bg_list))), 2) if bg_list is not None else 0
         self.bg_list = list(bg_list) if bg else None
-        if self.generic and bg:
-            raise NotImplementedError("background charges are restated for the hard-coded set")
         basis, bgcol = su2_ops.gauge_invariant_states(pure_theory, self.dim, j_max, bg, return_bg=True)
-        ops = (su2_ops.dressed_site_operators_gen(pure_theory, self.dim, j_max) if self.generic
+        ops = (su2_ops.dressed_site_operators_gen(pure_theory, self.dim, j_max, bg) if self.generic
                else su2_ops.dressed_site_operators(pure_theory, self.dim, bg))
         self.labels = [lat.site_label(lat.zig_zag(self.lvals, i), self.lvals, self.has_obc, False)
                        for i in range(self.n_sites)]

@@ -249,19 +249,27 @@ def _kron(ops, names):
     return sp.csr_matrix(out)
 
 
-def _with_background(ops, pure_theory, dim, background):
-    """kron(identity(bg), op) for every operator, the background Casimir
-    'bg' and, in 1D with matter, the V hoppings (SU2_operators.py:272-298)."""
+def _background_slot(ops, ins, pure_theory, dim, background):
+    """kron(identity(bg), op) for every operator and the background Casimir
+    'bg' = T2_bg x identity (SU2_operators.py:270-281, 580-590); ins holds
+    the single-dof operators of the set ('Iz' of its rishons)."""
     gen = generators(S(background))
     nbg = gen["T2"].shape[0]
     out = {k: sp.kron(sp.identity(nbg), v).tocsr() for k, v in ops.items()}
-    ins = _rishon()
+    ins = dict(ins)
     ins["T2_bg"] = gen["T2"]
     n_link = 2 * dim
     ids = ["Iz"] * n_link if pure_theory else ["ID_psi"] + ["Iz"] * n_link
     if not pure_theory:
         ins.update(_matter())
     out["bg"] = _kron(ins, ["T2_bg"] + ids)
+    return out, ins
+
+
+def _with_background(ops, pure_theory, dim, background):
+    """The background slot and, in 1D with matter, the V hoppings
+    (SU2_operators.py:270-298)."""
+    out, ins = _background_slot(ops, _rishon(), pure_theory, dim, background)
     if dim == 1 and not pure_theory:
         out["Vmx_dag"] = sum(_kron(ins, [f"Z{c}_dag", "P_psi", f"Z{c}", "Iz"]) for c in "rg")
         out["Vpx_dag"] = sum(_kron(ins, [f"Z{c}_dag", "P_psi", "P", f"Z{c}"]) for c in "rg")
@@ -345,8 +353,14 @@ def _corner_lists(dim):
     return {}
 
 
-def dressed_site_operators_gen(pure_theory, dim, j_max):
-    """SU2_gen_dressed_site_operators(j_max, pure_theory, dim) (no background)."""
+def dressed_site_operators_gen(pure_theory, dim, j_max, background=0):
+    """SU2_gen_dressed_site_operators(j_max, pure_theory, dim, background)
+    (SU2_operators.py:302-590); a background adds its slot and Casimir only
+    (no V hoppings in this set)."""
+    if background:
+        out, _ = _background_slot(dressed_site_operators_gen(pure_theory, dim, j_max), rishon_gen(j_max),
+                                  pure_theory, dim, background)
+        return out
     ins = rishon_gen(j_max)
     if not pure_theory:
         ins.update(_matter())

@@ -119,28 +119,36 @@ def test_3d_theta_term(gpu, pure):
 
 
 BG_CASES = [
-    # lvals, has_obc, pure, bg_list, noise
-    ([4], [True], False, [0.5, 0, 0, 0.5], 0.0),
-    ([4], [False], False, [0.5, 0, 0.5, 0], 0.0),
-    ([4], [False], False, [0.5, 0, 0.5, 0], 0.3),   # Gauss-law-violating hopping
-    ([2, 2], [True, True], True, [0.5, 0, 0, 0.5], 0.0),
-    ([2, 2], [True, True], False, [0.5, 0, 0, 0.5], 0.0),
+    # lvals, has_obc, pure, bg_list, noise, spin, generic
+    ([4], [True], False, [0.5, 0, 0, 0.5], 0.0, 0.5, False),
+    ([4], [False], False, [0.5, 0, 0.5, 0], 0.0, 0.5, False),
+    ([4], [False], False, [0.5, 0, 0.5, 0], 0.3, 0.5, False),   # Gauss-law-violating hopping
+    ([2, 2], [True, True], True, [0.5, 0, 0, 0.5], 0.0, 0.5, False),
+    ([2, 2], [True, True], False, [0.5, 0, 0, 0.5], 0.0, 0.5, False),
+    # the generalised set with background slots (SU2_operators.py:580-590)
+    ([4], [False], False, [0.5, 0, 0.5, 0], 0.0, 0.5, True),
+    ([4], [True], False, [0.5, 0, 1, 0.5], 0.0, 1.0, True),
+    ([4], [False], False, [1, 0, 1, 0], 0.0, 1.0, True),
+    ([2, 2], [False, False], True, [0, 1, 0, 1], 0.0, 1.0, True),
 ]
 
 
-@pytest.mark.parametrize("lvals,obc,pure,bg,noise", BG_CASES)
-def test_background_charges(gpu, lvals, obc, pure, bg, noise):
+@pytest.mark.parametrize("lvals,obc,pure,bg,noise,spin,generic", BG_CASES)
+def test_background_charges(gpu, lvals, obc, pure, bg, noise, spin, generic):
     """SU2_Model(bg_list=...) (SU2_model.py:63-104, 338-354;
     quantum_model.py:440-466): sector and CSR bit-exact against the oracle,
     energies within 1e-10."""
     from edlgt_amd import SU2_Model
-    print(f"SU(2) background lvals={lvals} obc={obc} pure={pure} bg={bg} noise={noise}", flush=True)
+    print(f"SU(2) background lvals={lvals} obc={obc} pure={pure} bg={bg} noise={noise} spin={spin} "
+          f"generic={generic}", flush=True)
     n = int(np.prod(lvals))
     sec = None if pure else [n]
     m = None if pure else 0.8
-    orc = SU2Oracle(pure, lvals, obc, sectors=sec, bg_list=bg)
-    mod = SU2_Model(spin=0.5, pure_theory=pure, sectors=sec, lvals=lvals, has_obc=obc, bg_list=bg)
-    assert mod.background == 0.5 and mod.bg_list == [float(b) for b in bg]
+    orc = SU2Oracle(pure, lvals, obc, sectors=sec, spin=spin, generic=generic, bg_list=bg)
+    mod = SU2_Model(spin=spin, pure_theory=pure, sectors=sec, lvals=lvals, has_obc=obc, bg_list=bg,
+                    use_generic_model=generic)
+    assert mod.background == max(bg) and mod.bg_list == [float(b) for b in bg]
+    assert mod.use_generic_model == (generic or spin >= 1)
     assert np.array_equal(mod.loc_dims, orc.loc_dims)
     assert np.array_equal(mod.sector_configs, orc.configs)
     Hg = mod.build_Hamiltonian(1.1, m, lambda_noise=noise)
@@ -148,8 +156,20 @@ def test_background_charges(gpu, lvals, obc, pure, bg, noise):
     _csr_match(Hg, Ho)
     k = min(2, orc.N - 1) if orc.N > 1 else 1
     mod.diagonalize_Hamiltonian(k, "sparse")
-    ref = np.linalg.eigvalsh(Ho.toarray())[:k]
+    ref = lowest_eigs(Ho, k) if orc.N > 600 else np.linalg.eigvalsh(Ho.toarray())[:k]
     assert np.max(np.abs(mod.H.Nenergies * n - ref)) < 1e-10
+
+
+def test_background_spin1_flux_string_closed_form(gpu):
+    """Pure chain at spin-1 truncation, charges 1/2, 1, 1/2 at sites 0, 2, 3
+    (OBC): every link carries j = 1/2, one state of energy
+    (8g/3) * 3 * (3/4)."""
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=1, pure_theory=True, lvals=[4], has_obc=[True], bg_list=[0.5, 0, 1, 0.5])
+    assert mod.sector_dim == 1 and mod.use_generic_model
+    mod.build_Hamiltonian(0.7)
+    mod.diagonalize_Hamiltonian(1, "sparse")
+    assert abs(mod.H.Nenergies[0] * 4 - (8 * 0.7 / 3) * 3 * 0.75) < 1e-12
 
 
 def test_background_flux_string_closed_form(gpu):

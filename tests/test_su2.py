@@ -404,3 +404,74 @@ def test_engine_site_with_background_matches_oracle(pure, dim):
             assert set(mine) == set(ops), (set(mine) ^ set(ops))
             for name, O in ops.items():
                 assert np.max(np.abs(mine[name] - B.T @ (O @ B))) < 1e-14, (mine_lab, name)
+
+
+@pytest.mark.parametrize("pure,dim,j2,bg2", [(False, 1, 1, 1), (True, 1, 2, 2), (False, 1, 2, 1), (True, 2, 1, 1),
+                                             (True, 2, 2, 1)])
+def test_engine_site_with_background_matches_oracle_general(pure, dim, j2, bg2):
+    """The generalised set with a background slot (SU2_operators.py:302-590):
+    bases per background charge and every projected operator of the
+    engine's site against the oracle's."""
+    from fractions import Fraction
+
+    from sympy import Rational
+
+    from edlgt_amd.su2_operators import SU2Site
+    site = SU2Site(pure, dim, Fraction(j2, 2), True, background=Fraction(bg2, 2))
+    j, bg = Rational(j2, 2), Rational(bg2, 2)
+    basis, bgcol = su2_ops.gauge_invariant_states(pure, dim, j, background=bg, return_bg=True)
+    ops = su2_ops.dressed_site_operators_gen(pure, dim, j, background=bg)
+    assert "bg" in ops and "Vpx_dag" not in ops
+    n_checked = 0
+    for lab, Bo in basis.items():
+        for k in range(bg2 + 1):
+            cols = np.flatnonzero(np.isclose(bgcol[lab], k / 2))
+            mine_lab = f"{lab}@bg{k}"
+            if cols.size == 0:
+                assert mine_lab not in site.basis
+                continue
+            B = Bo.toarray()[:, cols]
+            assert np.max(np.abs(site.basis[mine_lab] - B)) < 1e-15
+            mine = site.operators(mine_lab)
+            assert set(mine) == set(ops), (set(mine) ^ set(ops))
+            for name, O in ops.items():
+                assert np.max(np.abs(mine[name] - B.T @ (O @ B))) < 1e-14, (mine_lab, name)
+            # the background Casimir is diagonal with the charge's j(j+1)
+            assert np.allclose(mine["bg"], (k / 2) * (k / 2 + 1) * np.eye(B.shape[1]))
+            n_checked += 1
+    assert n_checked >= 2
+
+
+@pytest.mark.parametrize("lvals,obc,pure,bg", [([4], [True], False, [0, 0.5, 0, 0.5]),
+                                               ([4], [False], False, [0.5, 0, 0.5, 0]),
+                                               ([4], [False], True, [0.5, 0, 0, 0.5]),
+                                               ([2, 2], [True, True], True, [0.5, 0, 0, 0.5])])
+def test_generalised_equals_hard_coded_with_background(lvals, obc, pure, bg):
+    """With spin-1/2 background charges the generalised set at spin 1/2 gives
+    the hard-coded model's sector and lowest eigenvalues (g = m = 0.1)."""
+    n = int(np.prod(lvals))
+    sec = None if pure else [n]
+    m = None if pure else 0.1
+    a = SU2Oracle(pure, lvals, obc, sectors=sec, bg_list=bg)
+    b = SU2Oracle(pure, lvals, obc, sectors=sec, generic=True, bg_list=bg)
+    assert np.array_equal(a.configs, b.configs)
+    k = min(4, a.N)
+    ea = np.linalg.eigvalsh(a.hamiltonian(0.1, m).toarray())[:k]
+    eb = np.linalg.eigvalsh(b.hamiltonian(0.1, m).toarray())[:k]
+    assert np.max(np.abs(ea - eb)) / n < 1e-10, (ea, eb)
+
+
+@pytest.mark.parametrize("L,bg,links", [(4, [1, 0, 0, 1], [1, 1, 1]), (5, [0, 1, 0, 1, 0], [0, 1, 1, 0]),
+                                        (4, [0.5, 0, 1, 0.5], [0.5, 0.5, 0.5]),
+                                        (5, [0.5, 0, 0, 0.5, 0], [0.5, 0.5, 0.5, 0])])
+def test_su2_spin1_background_flux_string_closed_form(L, bg, links):
+    """Pure 1D SU(2), OBC, rishons truncated at spin 1, static charges of
+    spin 1/2 and 1: Gauss's law fixes every link's spin (j on a link equals
+    the coupling of the charges to its left, within the truncation), so the
+    sector holds one state, of energy (8g/3) sum_links j(j+1)
+    (two half-links of T2/2 each; SU2_model.py:555-624)."""
+    g = 0.7
+    orc = SU2Oracle(True, [L], [True], spin=1, bg_list=bg)
+    assert orc.generic and orc.N == 1
+    e = orc.hamiltonian(g).toarray()[0, 0]
+    assert abs(e - (8 * g / 3) * sum(j * (j + 1) for j in links)) < 1e-12
