@@ -265,19 +265,26 @@ __global__ void __launch_bounds__(256) predict_counts(AsmDev A, int32_t* __restr
 }
 
 // One 64-thread block == one wavefront; LDS holds the row's raw entries.
+// Rows are 0 .. n_iter-1, or rows[0 .. n_iter-1] when a row list is given.
+// A row with more than kCap raw entries is appended to ovf (when given: the
+// host re-runs those rows with a larger kCap) or flagged as an error.
 template <int kCap, bool kFill, class KeyT, int kOps>
 __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restrict__ row_count,
                                                     const int64_t* __restrict__ row_ptr,
                                                     int32_t* __restrict__ out_col,
                                                     double* __restrict__ out_val,
-                                                    uint8_t* __restrict__ out_term) {
+                                                    uint8_t* __restrict__ out_term,
+                                                    const int64_t* __restrict__ rows, int64_t n_iter,
+                                                    int64_t* __restrict__ ovf,
+                                                    unsigned long long* __restrict__ n_ovf) {
   __shared__ KeyT skey[kCap];
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
   __shared__ uint16_t sidx[kCap];
   __shared__ uint8_t sterm[kCap];
   __shared__ double2 sval[kFill ? kCap : 1];
   const int lane = threadIdx.x;
-  for (int64_t r = blockIdx.x; r < A.n_states; r += gridDim.x) {
+  for (int64_t it = blockIdx.x; it < n_iter; it += gridDim.x) {
+    const int64_t r = rows ? rows[it] : it;
     const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
     const KeyT key = keys[r];
     // 1+2: count placements, assign slots in placement order, emit.
@@ -342,7 +349,10 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
       }
     }
     if (overflow) {
-      if (lane == 0) atomicOr(A.err, kErrTooMany);
+      if (lane == 0) {
+        if (ovf) ovf[atomicAdd(n_ovf, 1ull)] = r;
+        else atomicOr(A.err, kErrTooMany);
+      }
       if (!kFill) row_count[r] = 0;
       __syncthreads();
       continue;
@@ -421,13 +431,22 @@ __global__ void __launch_bounds__(64) assemble_rows(AsmDev A, int32_t* __restric
   }
 }
 
+// Row list and overflow list of an assembly pass (see assemble_rows).
+struct RowSet {
+  const int64_t* rows = nullptr;  // null: every row
+  int64_t n_iter = 0;
+  int64_t* ovf = nullptr;         // null: an overflowing row is an error
+  unsigned long long* n_ovf = nullptr;
+};
+
 template <int kCap, class KeyT>
 int launch_pass(bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr, int32_t* col,
-                double* val, uint8_t* term, hipStream_t st) {
-  const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(A.n_states, 1), 1 << 18);
+                double* val, uint8_t* term, const RowSet& rs, hipStream_t st) {
+  if (rs.n_iter <= 0) return EDLGT_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>(rs.n_iter, 1 << 18);
 #define ASM_LAUNCH(FILL, OPS)                                                                       \
   hipLaunchKernelGGL((assemble_rows<kCap, FILL, KeyT, OPS>), dim3(grid), dim3(64), 0, st, A, counts, \
-                     row_ptr, col, val, term)
+                     row_ptr, col, val, term, rs.rows, rs.n_iter, rs.ovf, rs.n_ovf)
   if (A.max_ops <= 4) {
     if (fill) ASM_LAUNCH(true, 4); else ASM_LAUNCH(false, 4);
   } else {
@@ -438,20 +457,28 @@ int launch_pass(bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_
   return EDLGT_OK;
 }
 
+// LDS slots of the assembly: rows up to kCapSmall raw entries run in the
+// first pass (sized to the program's bound), longer ones are re-run with
+// kCapLarge slots (140 KB of LDS with 128-bit keys).
+constexpr int kCapSmall = 1024;
+constexpr int kCapLarge = 4096;
+
 template <class KeyT>
 int launch_cap(int cap, bool fill, const AsmDev& A, int32_t* counts, const int64_t* row_ptr,
-               int32_t* col, double* val, uint8_t* term, hipStream_t st) {
-  if (cap <= 64) return launch_pass<64, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
-  if (cap <= 128) return launch_pass<128, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
-  if (cap <= 256) return launch_pass<256, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
-  if (cap <= 512) return launch_pass<512, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
-  return launch_pass<1024, KeyT>(fill, A, counts, row_ptr, col, val, term, st);
+               int32_t* col, double* val, uint8_t* term, const RowSet& rs, hipStream_t st) {
+  if (cap <= 64) return launch_pass<64, KeyT>(fill, A, counts, row_ptr, col, val, term, rs, st);
+  if (cap <= 128) return launch_pass<128, KeyT>(fill, A, counts, row_ptr, col, val, term, rs, st);
+  if (cap <= 256) return launch_pass<256, KeyT>(fill, A, counts, row_ptr, col, val, term, rs, st);
+  if (cap <= 512) return launch_pass<512, KeyT>(fill, A, counts, row_ptr, col, val, term, rs, st);
+  if (cap <= kCapSmall) return launch_pass<kCapSmall, KeyT>(fill, A, counts, row_ptr, col, val, term, rs, st);
+  return launch_pass<kCapLarge, KeyT>(fill, A, counts, row_ptr, col, val, term, rs, st);
 }
 
 int launch_any(int key_words, int cap, bool fill, const AsmDev& A, int32_t* counts,
-               const int64_t* row_ptr, int32_t* col, double* val, uint8_t* term, hipStream_t st) {
-  if (key_words == 2) return launch_cap<u128>(cap, fill, A, counts, row_ptr, col, val, term, st);
-  return launch_cap<uint64_t>(cap, fill, A, counts, row_ptr, col, val, term, st);
+               const int64_t* row_ptr, int32_t* col, double* val, uint8_t* term, const RowSet& rs,
+               hipStream_t st) {
+  if (key_words == 2) return launch_cap<u128>(cap, fill, A, counts, row_ptr, col, val, term, rs, st);
+  return launch_cap<uint64_t>(cap, fill, A, counts, row_ptr, col, val, term, rs, st);
 }
 
 int check_err(const DevBuf& err, hipStream_t st) {
@@ -463,7 +490,7 @@ int check_err(const DevBuf& err, hipStream_t st) {
   if (e & kErrNotReal)
     return set_error(EDLGT_ERR_NOT_REAL, "complex matrix element in real value mode (|Im| > 1e-12)");
   if (e & kErrTooMany)
-    return set_error(EDLGT_ERR_INDEX_OVERFLOW, "more than 1024 raw entries in one row");
+    return set_error(EDLGT_ERR_INDEX_OVERFLOW, "more than 4096 raw entries in one row");
   return EDLGT_OK;
 }
 
@@ -477,7 +504,8 @@ namespace {
 struct Program {
   DevBuf place, str, toff, tket, tval, tself, diag, err, bucket;
   AsmDev A;
-  int cap = 1;  // raw entries per row bound (assembly LDS size)
+  int cap = 1;            // LDS slots of the first assembly pass
+  int64_t cap_bound = 1;  // bound on the raw entries of any row
 };
 
 int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const double* tables, int32_t n_place,
@@ -546,7 +574,8 @@ int prepare_program(const Basis& B, int32_t d_max, int32_t n_tables, const doubl
     P.s_im = place_strength[2 * p + 1];
   }
   cap_bound += n_terms;
-  prog.cap = (int)std::min<int64_t>(std::max<int64_t>(cap_bound, 1), 1024);
+  prog.cap_bound = std::max<int64_t>(cap_bound, 1);
+  prog.cap = (int)std::min<int64_t>(prog.cap_bound, kCapSmall);
 
   EDLGT_TRY(upload(prog.place, place.data(), std::max<size_t>(place.size(), 1), st));
   if (B.key_words == 2)
@@ -806,14 +835,38 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   DevBuf counts;
   EDLGT_TRY(counts.alloc(std::max<int64_t>(N, 1) * sizeof(int32_t)));
   out->n_terms = n_terms;
+  // Rows whose raw entries exceed the first pass's LDS slots (possible only
+  // when the program's bound is above kCapSmall) are listed by that pass and
+  // run again with kCapLarge slots.
+  DevBuf ovf, n_ovf;
+  const bool two_level = prog.cap_bound > kCapSmall && N > 0;
+  if (two_level) {
+    EDLGT_TRY(ovf.alloc((size_t)N * sizeof(int64_t)));
+    EDLGT_TRY(n_ovf.alloc(sizeof(unsigned long long)));
+  }
+  auto run = [&](bool fill, int32_t* cnt, const int64_t* rp, int32_t* c, double* v, uint8_t* tm) -> int {
+    RowSet all;
+    all.n_iter = N;
+    if (!two_level) return launch_any(B.key_words, cap, fill, A, cnt, rp, c, v, tm, all, st);
+    all.ovf = ovf.as<int64_t>();
+    all.n_ovf = n_ovf.as<unsigned long long>();
+    EDLGT_HIP(hipMemsetAsync(n_ovf.p, 0, sizeof(unsigned long long), st));
+    EDLGT_TRY(launch_any(B.key_words, cap, fill, A, cnt, rp, c, v, tm, all, st));
+    unsigned long long n_long = 0;
+    EDLGT_HIP(hipMemcpyAsync(&n_long, n_ovf.p, sizeof(n_long), hipMemcpyDeviceToHost, st));
+    EDLGT_HIP(hipStreamSynchronize(st));
+    RowSet longer;
+    longer.rows = ovf.as<int64_t>();
+    longer.n_iter = (int64_t)n_long;
+    return launch_any(B.key_words, kCapLarge, fill, A, cnt, rp, c, v, tm, longer, st);
+  };
   // Row counts from the transition tables alone (predict_counts); if the fill
   // finds two raw entries of a row reaching one ket (kErrCount), the rows are
   // counted again by enumeration (the exact pass) and filled once more.
   for (int exact = 0; exact < 2; ++exact) {
     if (N > 0) {
       if (exact) {
-        EDLGT_TRY(launch_any(B.key_words, cap, false, A, counts.as<int32_t>(), nullptr, nullptr, nullptr,
-                             nullptr, st));
+        EDLGT_TRY(run(false, counts.as<int32_t>(), nullptr, nullptr, nullptr, nullptr));
       } else {
         hipLaunchKernelGGL(predict_counts, dim3(grid_for(N, 256, 16384)), dim3(256), 0, st, A,
                            counts.as<int32_t>());
@@ -831,9 +884,8 @@ int hamiltonian_build(const Basis& B, int32_t d_max, int32_t n_tables, const dou
     EDLGT_TRY(out->val.alloc(std::max<int64_t>(nnz, 1) * value_bytes(value_type)));
     if (n_terms > 1) EDLGT_TRY(out->term.alloc(std::max<int64_t>(nnz, 1)));
     if (N > 0)
-      EDLGT_TRY(launch_any(B.key_words, cap, true, A, nullptr, out->row_ptr.as<int64_t>(),
-                           out->col.as<int32_t>(), out->val.as<double>(),
-                           n_terms > 1 ? out->term.as<uint8_t>() : nullptr, st));
+      EDLGT_TRY(run(true, nullptr, out->row_ptr.as<int64_t>(), out->col.as<int32_t>(), out->val.as<double>(),
+                    n_terms > 1 ? out->term.as<uint8_t>() : nullptr));
     int32_t e = 0;
     EDLGT_HIP(hipMemcpyAsync(&e, b_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     EDLGT_HIP(hipStreamSynchronize(st));

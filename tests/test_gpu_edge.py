@@ -150,3 +150,70 @@ def test_repeated_placements_are_summed(gpu):
     # the table-only count cannot see the collisions: an upper bound here
     from edlgt_amd.device import stored_entries
     assert stored_entries(basis, tables, pl) > H.nnz
+
+
+def _kron_chain(factors):
+    import scipy.sparse as sp
+    out = sp.csr_matrix(np.ones((1, 1), complex))
+    for f in factors:
+        out = sp.kron(out, sp.csr_matrix(f), format="csr")
+    return out
+
+
+def test_rows_above_1024_raw_entries(gpu):
+    """Rows whose raw entries exceed the first assembly pass's 1024 LDS slots
+    are listed and assembled again with 4096 slots.  Six 4-state sites, 4-site
+    placements of a table whose state 0 has four transitions and the others
+    two: a row has 96 (no site in state 0) to 1536 (all in state 0) raw
+    entries, so both passes run, with the exact-count fallback (overlapping
+    placements of one term reach the same kets).  Checked against the sum of
+    Kronecker products on the full 4096-state space."""
+    from edlgt_amd.device import CSRMatrix, SectorBasis
+    import edlgt_amd._lib as L
+    rng = np.random.default_rng(11)
+    A = np.zeros((4, 4), complex)
+    A[0, :] = rng.normal(size=4) + 1j * rng.normal(size=4)
+    A[:, 0] = rng.normal(size=4) + 1j * rng.normal(size=4)
+    for a in range(1, 4):
+        A[a, a] = rng.normal() + 1j * rng.normal()
+    D = np.diag(rng.normal(size=4)).astype(complex)
+    I = np.eye(4)
+    pl, want = [], None
+    for w, s in zip(range(3), (0.7, -0.4, 1.3)):
+        pl.append(dict(sites=list(range(w, w + 4)), tables=[0] * 4, strength=s))
+        pl.append(dict(sites=list(range(w, w + 4)), tables=[0] * 4, strength=0.5 * s, dagger=True))
+        term = _kron_chain([I] * w + [A] * 4 + [I] * (2 - w))
+        want = (s * term if want is None else want + s * term) + (0.5 * s * term).conj().T
+    for site in range(6):
+        pl.append(dict(sites=[site], tables=[1], strength=0.25))
+        want = want + 0.25 * _kron_chain([I] * site + [D] + [I] * (5 - site))
+    H = CSRMatrix(SectorBasis([4] * 6), np.stack([A, D]), pl, L.VALUE_C128)
+    rp, col, val = H.to_host()
+    want = want.tocsr()
+    want.sum_duplicates()
+    want.eliminate_zeros()
+    want.sort_indices()
+    assert np.array_equal(rp, want.indptr) and np.array_equal(col, want.indices)
+    assert np.max(np.abs(val - want.data)) <= 1e-13 * np.max(np.abs(want.data))
+    # raw entries per row: both directions of each window, product of the
+    # factors' transition counts (4 from state 0, else 2): both passes ran
+    per_site = np.where(np.indices([4] * 6).reshape(6, -1).T == 0, 4, 2)
+    raw = sum(2 * per_site[:, w:w + 4].prod(axis=1) for w in range(3))
+    assert raw.max() == 1536 and (raw > 1024).any() and (raw <= 1024).any()
+
+
+def test_rows_above_4096_raw_entries_raise(gpu):
+    """More than 4096 raw entries in one row is an error (the second pass's
+    LDS bound), not a silent truncation: two 5-site placements of a dense
+    4x4 table and their conjugates give 4096 per row, one 2-site placement
+    more crosses the bound."""
+    from edlgt_amd.device import CSRMatrix, SectorBasis
+    import edlgt_amd._lib as L
+    A = np.random.default_rng(12).normal(size=(4, 4)).astype(complex)
+    pl = []
+    for w in range(2):
+        pl.append(dict(sites=list(range(w, w + 5)), tables=[0] * 5, strength=1.0))
+        pl.append(dict(sites=list(range(w, w + 5)), tables=[0] * 5, strength=1.0, dagger=True))
+    pl.append(dict(sites=[0, 1], tables=[0, 0], strength=1.0))
+    with pytest.raises(OverflowError, match="4096 raw entries"):
+        CSRMatrix(SectorBasis([4] * 6), A[None], pl, L.VALUE_C128)
