@@ -128,6 +128,48 @@ def time_launches(fn, steps, warmup):
     return s.elapsed_time(e) / steps
 
 
+def real_gauge_leg(model, N, nnz, e64, args):
+    """QED with staggered matter solved in the real gauge: D^H H D (float64
+    values) built from the same term program, its SpMV timed on real vectors,
+    then the FP64 and mixed lowest-eigenvalue solves (eigenvectors mapped back
+    through D).  The headline SpMV line stays the complex128 matrix's."""
+    import torch
+    from edlgt_amd import PREC_MIXED
+    progress("real-gauge build and solves")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rg = model.real_gauge_matrix()
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    if rg is None:
+        return None
+    Hr = rg[0]
+    tune = Hr.autotune(n_rhs=1)
+    Hr.set_launch(*tune[:2])
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    xr = torch.randn(N, dtype=torch.float64, device="cuda", generator=gen)
+    yr = torch.empty_like(xr)
+    ms = time_launches(lambda: Hr.matvec(xr, yr), args.steps, args.warmup)
+    B = spmv_bytes(N, nnz, s_v=8, s_x=8)
+    out = dict(assembly_s=round(t_build, 3), spmv_ms=round(ms, 4), spmv_alg_bytes=B,
+               spmv_alg_GBs=round(B / (ms * 1e-3) / 1e9, 1),
+               spmv_frac=round(B / (ms * 1e-3) / HBM_PEAK_GBS / 1e9, 4))
+    del xr, yr
+    for name, prec in (("fp64", None), ("mixed", PREC_MIXED)):
+        kw = {} if prec is None else dict(precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        model.diagonalize_Hamiltonian(1, "sparse", real_gauge=True, **kw)
+        torch.cuda.synchronize()
+        out[f"lanczos_{name}_s"] = round(time.perf_counter() - t0, 3)
+        out[f"lanczos_{name}_matvecs"] = [int(v) for v in model.H.matvecs]
+        out[f"dE0_{name}_vs_complex_fp64"] = abs(float(model.H.energies[0]) - e64)
+    model.H.Npsi = model.H.GSpsi = None
+    model._rg = None
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(H, x_dev, max_seconds=12.0, one_thread_seconds=8.0):
     """oracle/refcpu csr_spmv (row-parallel, complex128; built -march=native on
     this host) on the host cores over the same CSR, then on one thread;
@@ -670,6 +712,7 @@ def main():
                      lanczos_mixed_matvecs=[int(v) for v in model.H.matvecs],
                      dE0_mixed_vs_fp64=abs(emix - e64),
                      build_plus_E0_s=round(t_sector + t_assembly + t_fp64, 3))
+        extra["real_gauge"] = real_gauge_leg(model, N, nnz, e64, args)
 
     cpu = None
     if not args.no_cpu:

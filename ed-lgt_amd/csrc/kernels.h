@@ -100,6 +100,9 @@ int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, doubl
 // Same result without allocating: partial sums in a static per-device array.
 int basis_marginals_noalloc(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
                             hipStream_t st);
+// y = i^e x per basis state, e_r = sum_s tab[s][c_r[s]] mod 4 (real x, complex y).
+int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int n_vec, const double* x,
+                        void* y, hipStream_t st);
 
 // out[i] = i-th uniform [0, 1) double of the PCG64 stream with 128-bit state
 // and increment {state_hi, state_lo, inc_hi, inc_lo} (numpy's

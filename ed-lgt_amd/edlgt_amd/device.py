@@ -124,6 +124,23 @@ class SectorBasis:
         w = out.reshape(-1, 2)
         return [int(lo) | (int(hi) << 64) for lo, hi in w]
 
+    def quarter_phase(self, site_exp, x, stream=None):
+        """y[..., r] = i^e_r x[..., r] with e_r = sum_s site_exp[s, c_r[s]] mod 4,
+        for real device vectors x ((N,) or (k, N) float64): complex128 result
+        (edlgt_basis_quarter_phase)."""
+        torch = _torch()
+        tab = np.ascontiguousarray(site_exp, dtype=np.int32)
+        if tab.ndim != 2 or tab.shape[0] != self.n_sites or tab.shape[1] < int(self.loc_dims.max()):
+            raise ValueError("site_exp must be (n_sites, d_max) with d_max >= every local dimension")
+        if x.dtype != torch.float64 or not x.is_cuda or x.shape[-1] != self.n_states:
+            raise ValueError("x must be a float64 CUDA tensor over the sector")
+        x = x.contiguous()
+        n_vec = 1 if x.dim() == 1 else int(x.shape[0])
+        y = torch.empty(x.shape, dtype=torch.complex128, device=x.device)
+        L.check(L.load().edlgt_basis_quarter_phase(self._h, tab.ctypes.data, tab.shape[1], n_vec, L.ptr(x),
+                                                   L.ptr(y), L.stream_handle(stream)))
+        return y
+
     def lookup(self, configs):
         """Indices of configurations (raises ValueError if one is not in the sector)."""
         q = np.ascontiguousarray(np.atleast_2d(configs), dtype=np.uint16)

@@ -199,9 +199,11 @@ class QMB_hamiltonian:
         self.nnz = csr.nnz
 
     def diagonalize(self, n_eigs, format, loc_dims, print_results=False, tol=1e-12,
-                    max_basis=0, precision=L.PREC_FP64, v0=None):
+                    max_basis=0, precision=L.PREC_FP64, v0=None, gauge=None):
         """Lowest n_eigs eigenpairs; Nenergies are energies per site
-        (qmb_hamiltonian.py:260-332)."""
+        (qmb_hamiltonian.py:260-332).  gauge = (real CSR of D^H H D, sector
+        basis, per-site exponent table): solve that matrix and return the
+        eigenvectors D x of H (QuantumModel.real_gauge_matrix)."""
         if isinstance(n_eigs, str):
             if n_eigs != "full":
                 raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")
@@ -217,17 +219,22 @@ class QMB_hamiltonian:
             raise ValueError(f"n_eigs must be < H.shape[0]={self.shape[0]}, got {n_eigs}")
         self.loc_dims = loc_dims
         self.n_eigs = int(n_eigs)
-        cplx = self.Ham.is_complex
+        H = self.Ham if gauge is None else gauge[0]
+        cplx = H.is_complex
         if v0 is None:
             # default_rng(0).random(N), generated on the device (the solver
             # normalises its start block)
             from .device import random_uniform
             v0 = random_uniform(self.shape[0], complex_values=cplx, seed=0)
+        elif gauge is not None:
+            raise ValueError("a start vector is not taken by the real-gauge solve")
         else:
             v0 = np.asarray(v0, dtype=np.complex128 if cplx else np.float64)
             v0 = v0 / np.linalg.norm(v0)
-        evals, evecs, resid, mvs = self.Ham.lanczos(self.n_eigs, tol=tol, max_basis=max_basis,
-                                                    precision=precision, v0=v0)
+        evals, evecs, resid, mvs = H.lanczos(self.n_eigs, tol=tol, max_basis=max_basis,
+                                             precision=precision, v0=v0)
+        if gauge is not None:
+            evecs = gauge[1].quarter_phase(gauge[2], evecs)
         self.energies = evals
         self.residuals = resid
         self.matvecs = mvs
@@ -374,7 +381,8 @@ class QuantumModel:
             if add_dagger:
                 self._placements.append(dict(sites=list(sl), tables=tabs, strength=strength, dagger=1))
 
-    def _storage_mode(self, tables, vt):
+    def _storage_mode(self, tables, vt, placements=None):
+        placements = self._placements if placements is None else placements
         if self.momentum_basis is not None or self.matrix_free is False:
             return "stored"
         if self.matrix_free is True:
@@ -382,17 +390,15 @@ class QuantumModel:
         import torch
         from .device import storage_plan, stored_entries, stored_entries_bound
         free, _ = torch.cuda.mem_get_info()
-        bound = stored_entries_bound(tables, self._placements, self.sector_dim)
+        bound = stored_entries_bound(tables, placements, self.sector_dim)
         if storage_plan(self.sector_dim, bound, vt, free) == "stored":
             return "stored"  # even the host bound fits: no device count needed
-        nnz = stored_entries(self.basis, tables, self._placements)
+        nnz = stored_entries(self.basis, tables, placements)
         return storage_plan(self.sector_dim, nnz, vt, free)
 
-    def _build_program(self, real):
-        tables = np.stack(self._tables)
-        vt = L.VALUE_F64 if real else L.VALUE_C128
-        mode = self._storage_mode(tables, vt)
-        csr = CSRMatrix(self.basis, tables, self._placements, vt, matrix_free=mode == "free")
+    def _make_csr(self, tables, placements, vt):
+        mode = self._storage_mode(tables, vt, placements)
+        csr = CSRMatrix(self.basis, tables, placements, vt, matrix_free=mode == "free")
         if mode == "coded":
             # values replaced by dictionary codes (products bit for bit the
             # same); a matrix with too many distinct values is rebuilt
@@ -403,9 +409,16 @@ class QuantumModel:
             else:
                 del csr
                 mode = "free"
-                csr = CSRMatrix(self.basis, tables, self._placements, vt, matrix_free=True)
+                csr = CSRMatrix(self.basis, tables, placements, vt, matrix_free=True)
         if mode == "free":
             logger.info("Hamiltonian built matrix-free (entries generated in every product)")
+        return csr
+
+    def _build_program(self, real):
+        tables = np.stack(self._tables)
+        vt = L.VALUE_F64 if real else L.VALUE_C128
+        csr = self._make_csr(tables, self._placements, vt)
+        self._rg = None  # the real-gauge matrix of the previous build is stale
         if self.momentum_basis is not None:
             # B^H H B in the momentum sector (the reference's momentum triplets,
             # translational_sym.py:1394-1969, behind QMB_hamiltonian.build)
@@ -446,10 +459,88 @@ class QuantumModel:
             return mb.expand(psi_device)
         return psi_device
 
-    def diagonalize_Hamiltonian(self, n_eigs, format, print_results=False, **kw):
-        self.H.diagonalize(n_eigs, format, self.loc_dims, print_results, **kw)
+    def diagonalize_Hamiltonian(self, n_eigs, format, print_results=False, real_gauge=False, **kw):
+        """quantum_model.py:606-611.  real_gauge=True solves D^H H D instead of H
+        when a diagonal phase D makes it real (real_gauge_matrix): the same
+        eigenvalues with real float64 vectors and values (about half the bytes
+        per product and per orthogonalisation), the eigenvectors mapped back
+        through D on the device.  Where no such phase applies the complex
+        matrix is solved; H.real_gauge tells which ran."""
+        gauge = None
+        if real_gauge:
+            rg = self.real_gauge_matrix()
+            if rg is not None:
+                gauge = (rg[0], self.basis, rg[1])
+        self.H.diagonalize(n_eigs, format, self.loc_dims, print_results, gauge=gauge, **kw)
+        self.H.real_gauge = gauge is not None
         self.n_eigs = self.H.n_eigs
         self.res["energy"] = self.H.Nenergies
+
+    # -- real gauge (no counterpart in the reference: its complex solve gives
+    # the same eigenpairs)
+    def _gauge_weights(self):
+        """(a, name): per-site weights a_j and the number operator n_j of a
+        diagonal phase D = diag(i^(sum_j a_j n_j)) under which the model's
+        Hamiltonian is real, or None when the model knows no such phase."""
+        return None
+
+    def _real_gauge_placements(self):
+        """The program of D^H H D: each placement's strength times
+        i^(-sum_f a_f q_f), where q_f is the change of n (bra minus ket) that
+        factor f's table makes at its site.  A placement maps a ket c to a bra r
+        with s_r - s_c = sum_f a_f q_f (s = sum_j a_j n_j), so every entry of
+        the placement takes the same factor i^(s_c - s_r); its Hermitian
+        conjugate takes the conjugate factor, which the assembly's conj(strength)
+        of dagger placements already gives.  None when a table changes n by
+        different amounts on different entries (no fixed factor)."""
+        w = self._gauge_weights()
+        if w is None or self.momentum_basis is not None:
+            return None
+        a, name = w
+        nd = np.real(np.einsum("sii->si", self.ops[name]))
+        quarter = (1.0, 1j, -1.0, -1j)
+        charge = {}
+        out = []
+        for P in self._placements:
+            e = 0
+            for site, t in zip(P["sites"], P["tables"]):
+                if (t, site) not in charge:
+                    bra, ket = np.nonzero(np.abs(self._tables[t]) > 1e-10)
+                    q = nd[site][bra] - nd[site][ket]
+                    fixed = q.size == 0 or (np.all(np.abs(q - q[0]) <= 1e-9) and abs(q[0] - round(q[0])) <= 1e-9)
+                    charge[(t, site)] = (int(round(q[0])) if q.size else 0) if fixed else None
+                if charge[(t, site)] is None:
+                    return None
+                e += int(a[site]) * charge[(t, site)]
+            Q = dict(P)
+            Q["strength"] = P["strength"] * quarter[(-e) % 4]
+            out.append(Q)
+        return out
+
+    def real_gauge_matrix(self):
+        """(float64 CSR of D^H H D, per-site exponent table a_j n_j(state))
+        for the last build_Hamiltonian, or None when the model has no such
+        phase, a momentum sector is set, or D^H H D is not real (QED with matter
+        on a lattice periodic along x or z with an odd extent).  Built once per
+        build_Hamiltonian by the same assembly, which rejects any entry with an
+        imaginary part above 1e-12."""
+        cached = getattr(self, "_rg", None)
+        if cached is not None:
+            return cached or None
+        self._rg = False
+        placements = self._real_gauge_placements()
+        if placements is None:
+            return None
+        try:
+            csr = self._make_csr(np.stack(self._tables), placements, L.VALUE_F64)
+        except ValueError:
+            logger.info("no real gauge: D^H H D has complex entries")
+            return None
+        a, name = self._gauge_weights()
+        nd = np.rint(np.real(np.einsum("sii->si", self.ops[name]))).astype(np.int64)
+        site_exp = (np.asarray(a, dtype=np.int64)[:, None] * nd).astype(np.int32)
+        self._rg = (csr, site_exp)
+        return self._rg
 
     def time_evolution_Hamiltonian(self, initial_state, time_line, **kw):
         """quantum_model.py:613-615; per-step site marginals are kept in
@@ -938,6 +1029,22 @@ class QED_Model(QuantumModel):
             for d in self.directions:
                 self._add_nbody([f"E_p{d}", f"E_m{d}"], lat.bonds(d), -100.0, False)
         return self._build_program(real)
+
+    def _gauge_weights(self):
+        """Staggered fermions hop with the coefficients -i/2 along x, -i/2 and
+        +i/2 along z and real ones along y (QED_Hamiltonian_couplings, from
+        QED_model.py:719-776), and every hopping entry moves one fermion
+        between the bond's sites.  With a_j = (x_j + z_j) mod 2 the phase
+        i^(sum_j a_j N_j) therefore turns each hopping entry real; the electric,
+        mass and plaquette terms keep every N_j and stay real.  A lattice
+        periodic along x or z with an odd extent has a wrap-around bond whose
+        ends share a_j: its entries stay imaginary and real_gauge_matrix
+        refuses the real build."""
+        if self.spin == "integrated" or self.pure_theory:
+            return None
+        a = np.array([(c[0] + (c[2] if self.dim == 3 else 0)) % 2
+                      for c in map(self.lattice.coords, range(self.n_sites))], dtype=np.int64)
+        return a, "N"
 
     def overlap_QMB_state(self, name):
         """Named reference configurations (QED_model.py:778-899), one local-state

@@ -363,6 +363,18 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
 int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
                                   double* out_dev, void* stream);
 
+/* Diagonal quarter-turn phase of the basis states:
+ *   y[v][r] = i^(e_r) x[v][r],  e_r = (sum_s site_exp[s * d_max + config_r[s]]) mod 4
+ * x: device float64 (n_vec x N), y: device complex128 (n_vec x N).  The
+ * real-gauge solve of QED with staggered matter (QED_Model real_gauge=True)
+ * diagonalises D^H H D, D = diag(i^e), which is real; this maps its real
+ * eigenvectors back to eigenvectors of H.  No counterpart in the reference
+ * (its QED-with-matter solve, eigsh in QMB_hamiltonian.diagonalize,
+ * edlgt/modeling/qmb_hamiltonian.py:260-332, stays complex); the results are
+ * the same eigenpairs.  site_exp: host int32 [n_sites * d_max]. */
+int edlgt_basis_quarter_phase(const edlgt_basis* b, const int32_t* site_exp, int32_t d_max, int32_t n_vec,
+                              const double* x, void* y, void* stream);
+
 /* ======================================================================
  * Schmidt blocks of a state over a bipartition of the lattice.  Replaces the
  * psi_matrix reshape of QMB_state._get_psi_matrix and the dense products and

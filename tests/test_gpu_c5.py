@@ -93,6 +93,18 @@ def test_qed4x3_lowest_eigenvalue_every_form(qed4x3):
     mod.diagonalize_Hamiltonian(1, "sparse", precision=PREC_MIXED)
     emix = float(mod.H.energies[0])
     assert abs(emix - e64) <= 1e-10, (emix, e64)
+    print("QED 4x3: real-gauge FP64 and mixed Lanczos", flush=True)
+    mod.diagonalize_Hamiltonian(1, "sparse", real_gauge=True)
+    assert mod.H.real_gauge
+    erg = float(mod.H.energies[0])
+    assert abs(erg - e64) <= 1e-10, (erg, e64)
+    psi = mod.H.Npsi[0].psi_device
+    assert float(torch.linalg.vector_norm(H.matvec(psi) - erg * psi)) <= 1e-8
+    mod.diagonalize_Hamiltonian(1, "sparse", real_gauge=True, precision=PREC_MIXED)
+    assert abs(float(mod.H.energies[0]) - e64) <= 1e-10
+    del psi
+    mod._rg = None
+    torch.cuda.empty_cache()
     print("QED 4x3: coded Lanczos", flush=True)
     assert H.compress() > 0
     ev, _, _, _ = H.lanczos(1, tol=1e-12, want_vectors=False)

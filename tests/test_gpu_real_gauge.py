@@ -1,0 +1,121 @@
+"""GPU: the real-gauge solve of QED with staggered matter
+(QED_Model.diagonalize_Hamiltonian(..., real_gauge=True)).
+
+The device builds D^H H D, D = diag(i^(sum_j a_j N_j)), a_j = (x_j + z_j) mod 2,
+as a float64 CSR from the same term program with one power of i per
+placement, solves it with real vectors and maps the eigenvectors back through
+D (edlgt_basis_quarter_phase).  Checked against the oracle: the real matrix
+equals D^H H D (pattern bit-exact, values <= 1e-14 relative, the oracle's
+definition pinned in tests/test_real_gauge.py), the eigenvalues equal the
+complex solve's and the oracle's (<= 1e-10), the mapped vectors are
+eigenvectors of the complex H (residual through the complex CSR: a wrong phase
+map fails it), a non-degenerate ground state equals the complex solve's up to
+a global phase, and the observables agree.  A lattice periodic along x with an odd extent has no real
+gauge: the complex matrix is solved and H.real_gauge says so."""
+import numpy as np
+import pytest
+
+from oracle.model import QEDOracle, lowest_eigs
+
+pytestmark = pytest.mark.gpu
+
+CASES = [([10], [True]), ([6], [False]), ([2, 2], [False, False]), ([3, 2], [True, True]),
+         ([4, 2], [False, False]), ([2, 2, 2], [True, True, True])]
+
+
+def _exponents(orc, mod):
+    a, name = mod._gauge_weights()
+    nd = np.rint(np.real(np.einsum("sii->si", orc.ops[name]))).astype(np.int64)
+    s = np.zeros(orc.N, dtype=np.int64)
+    for j in range(orc.n_sites):
+        s += a[j] * nd[j][orc.configs[:, j]]
+    return s % 4
+
+
+@pytest.mark.parametrize("lvals,obc", CASES)
+def test_real_gauge_matrix_and_solve(gpu, lvals, obc):
+    import torch
+    from edlgt_amd import QED_Model
+    print(f"real gauge: QED lvals={lvals} obc={obc}", flush=True)
+    orc = QEDOracle(1, False, lvals, obc)
+    mod = QED_Model(spin=1, pure_theory=False, lvals=lvals, has_obc=obc)
+    assert np.array_equal(mod.sector_configs, orc.configs)
+    Hc = mod.build_Hamiltonian(1.3, 0.7)
+    Ho = orc.hamiltonian(1.3, 0.7)
+    rg = mod.real_gauge_matrix()
+    assert rg is not None
+    Hr, site_exp = rg
+    assert not Hr.is_complex
+    # D^H H D from the oracle's complex matrix
+    ph = 1j ** _exponents(orc, mod)
+    C = Ho.tocoo()
+    want = C.copy()
+    want.data = np.conj(ph[C.row]) * C.data * ph[C.col]
+    want = want.tocsr()
+    want.sort_indices()
+    assert np.max(np.abs(want.data.imag)) == 0.0
+    rp, col, val = Hr.to_host()
+    assert np.array_equal(rp, want.indptr.astype(np.int64)) and np.array_equal(col, want.indices.astype(np.int32))
+    assert np.max(np.abs(val - want.data.real)) <= 1e-14 * np.max(np.abs(want.data))
+
+    k = min(3, orc.N - 1)
+    obs = dict(local_obs=["E2", "N"])
+    if len(lvals) > 1:
+        obs["plaquette_obs"] = [["C_px,py", "C_py,mx", "C_my,px", "C_mx,my"]]
+    mod.get_observables(**obs)
+    mod.diagonalize_Hamiltonian(k, "sparse")
+    assert not mod.H.real_gauge
+    e_c = mod.H.energies.copy()
+    psi_c = mod.H.Npsi[0].psi_device.clone()
+    mod.measure_observables(0)
+    res_c = {key: np.array(v, copy=True) for key, v in mod.res.items() if key != "energy"}
+    mod.diagonalize_Hamiltonian(k, "sparse", real_gauge=True)
+    assert mod.H.real_gauge
+    e_r = mod.H.energies
+    ref = lowest_eigs(Ho, k) if orc.N > 600 else np.linalg.eigvalsh(Ho.toarray())[:k]
+    assert np.max(np.abs(e_r - e_c)) < 1e-10 and np.max(np.abs(e_r - ref)) < 1e-10
+    for i in range(k):
+        psi = mod.H.Npsi[i].psi_device
+        assert psi.dtype == torch.complex128
+        r = Hc.matvec(psi) - e_r[i] * psi
+        assert float(torch.linalg.vector_norm(r)) <= 1e-8 * max(1.0, abs(e_r[i]))
+    if e_c[1] - e_c[0] > 1e-6:  # non-degenerate ground state: same vector up to a phase
+        ov = abs(complex(torch.vdot(psi_c, mod.H.Npsi[0].psi_device)))
+        assert abs(ov - 1.0) < 1e-9
+        mod.measure_observables(0)
+        for key, v in res_c.items():
+            assert np.max(np.abs(np.asarray(mod.res[key]) - v)) < 1e-9, key
+
+
+def test_odd_periodic_extent_solves_the_complex_matrix(gpu):
+    from edlgt_amd import QED_Model
+    orc = QEDOracle(1, False, [3, 2], [False, False])
+    mod = QED_Model(spin=1, pure_theory=False, lvals=[3, 2], has_obc=[False, False])
+    mod.build_Hamiltonian(1.3, 0.7)
+    assert mod.real_gauge_matrix() is None
+    mod.diagonalize_Hamiltonian(2, "sparse", real_gauge=True)
+    assert not mod.H.real_gauge
+    ref = lowest_eigs(orc.hamiltonian(1.3, 0.7), 2)
+    assert np.max(np.abs(mod.H.energies - ref)) < 1e-10
+
+
+def test_pure_theory_has_no_real_gauge(gpu):
+    from edlgt_amd import QED_Model
+    mod = QED_Model(spin=1, pure_theory=True, lvals=[2, 2], has_obc=[False, False])
+    mod.build_Hamiltonian(1.3)
+    assert mod.real_gauge_matrix() is None
+
+
+def test_real_gauge_mixed_precision_and_rebuild(gpu):
+    """4x2 periodic (140,647 states): the mixed FP32/FP64 solve of the real
+    matrix reaches the FP64 eigenvalue; a second build_Hamiltonian at another
+    coupling replaces the cached real matrix."""
+    from edlgt_amd import PREC_MIXED, QED_Model
+    orc = QEDOracle(1, False, [4, 2], [False, False])
+    mod = QED_Model(spin=1, pure_theory=False, lvals=[4, 2], has_obc=[False, False])
+    for g in (1.0, 2.0):
+        mod.build_Hamiltonian(g, 0.5)
+        ref = lowest_eigs(orc.hamiltonian(g, 0.5), 1)[0]
+        mod.diagonalize_Hamiltonian(1, "sparse", real_gauge=True, precision=PREC_MIXED)
+        assert mod.H.real_gauge
+        assert abs(mod.H.energies[0] - ref) < 1e-10, (g, mod.H.energies[0], ref)

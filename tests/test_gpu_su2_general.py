@@ -31,6 +31,7 @@ CASES = [
     ([6], [True], False, 1.0, True),
     ([4], [True], False, 1.5, True),
     ([2, 2], [True, True], False, 1.0, True),
+    ([2, 2], [False, False], False, 1.0, True),  # 95,252 states, up to 356 raw entries per row
     ([2, 2], [False, False], True, 1.0, True),
     ([3, 2], [True, True], True, 1.0, True),
     ([2, 2], [False, False], False, 0.5, True),
