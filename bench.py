@@ -7,6 +7,7 @@
     python bench.py --workload momentum
     python bench.py --workload matfree
     python bench.py --workload serving_graphs
+    python bench.py --workload realgauge
 
 Workload (default): 2D U(1) QED, 4x3 periodic lattice, staggered fermions,
 E in {-1,0,1}, g=1, m=0.5, Gauss-law + zero-charge sector (N = 28,904,615).
@@ -54,6 +55,11 @@ recurrence).  The matrix is dictionary-coded and its value array released
 line reports the build times, seconds per step for both precisions and, per
 step, the site occupations and the electric energy of both runs with their
 difference.
+
+Workload realgauge: QED 4x3 PBC solved in the real gauge (D^H H D with
+D = diag(i^(sum_j a_j N_j)), float64 values) beside the complex FP64 solve: the
+real matrix's assembly, its SpMV on real vectors, and the FP64 and mixed
+lowest-eigenvalue solves with their |dE0| against the complex one.
 
 Workload cpu_qed4x3: the full build + lowest-eigenvalue path of QED 4x3 PBC on the
 host cores (oracle/refcpu: C/OpenMP sector enumeration and two-pass triplets,
@@ -554,6 +560,33 @@ def momentum_sectors(args):
     print(json.dumps(out), flush=True)
 
 
+def real_gauge_workload(args):
+    """QED 4x3 PBC (N = 28,904,615) solved in the real gauge beside the complex
+    matrix: the complex FP64 lowest eigenvalue, then the real matrix D^H H D,
+    its SpMV on real vectors and its FP64 and mixed solves (real_gauge_leg)."""
+    import torch
+    from edlgt_amd import QED_Model
+    wl = WORKLOADS["qed4x3"]
+    warm_up()
+    progress("QED 4x3: complex build and FP64 solve")
+    model = QED_Model(spin=1, pure_theory=False, lvals=wl["lvals"], has_obc=[False, False])
+    H = model.build_Hamiltonian(wl["g"], wl["m"])
+    N, nnz = H.n, H.nnz
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model.diagonalize_Hamiltonian(1, "sparse")
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    e64 = float(model.H.energies[0])
+    rg = real_gauge_leg(model, N, nnz, e64, args)
+    out = {"metric": "real-gauge FP64 lowest-eigenvalue solve time", "value": rg["lanczos_fp64_s"], "unit": "s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (default_rng(0) start vector)",
+           "config": {"workload": wl["desc"], "sector_dim": N, "stored_nnz": nnz},
+           "complex_fp64": {"lanczos_s": round(t64, 3), "E0": e64}, "real_gauge": rg}
+    print(json.dumps(out), flush=True)
+
+
 def matrix_free(args):
     """QED 4x3 PBC (N = 28,904,615): the matrix-free operator's product (entries
     generated from the term program in every product) against the stored CSR
@@ -605,7 +638,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3", "cpu_qed4x3", "momentum", "matfree", "serving_graphs"])
+    ap.add_argument("--workload", default="qed4x3", choices=sorted(WORKLOADS) + ["su2c3", "cpu_qed4x3", "momentum", "matfree", "serving_graphs",
+                                                                  "realgauge"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host-e0", action="store_true",
                     help="skip the host-core build + lowest eigenvalue of QED 4x3 (about two minutes)")
@@ -639,6 +673,8 @@ def main():
         return momentum_sectors(args)
     if args.workload == "matfree":
         return matrix_free(args)
+    if args.workload == "realgauge":
+        return real_gauge_workload(args)
     if args.workload == "serving_graphs":
         return serving_graphs_ab(args)
     import torch
