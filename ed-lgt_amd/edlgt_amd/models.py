@@ -42,6 +42,66 @@ class QMB_state:
         import torch
         return float(torch.linalg.vector_norm(self.psi_device).item())
 
+    def normalize(self, threshold=1e-14):
+        """Rescales the state to unit norm when its norm differs from 1 by more
+        than threshold; returns the norm before (qmb_state.py:79-95)."""
+        norm = self.get_norm()
+        if abs(norm - 1.0) > threshold:
+            self.psi_device = self.psi_device / norm
+            self._host = None
+        return norm
+
+    def truncate(self, threshold=1e-14):
+        """Host copy of the amplitudes with |psi_i| <= threshold set to zero
+        (qmb_state.py:97-110, truncation :466-482); the state is unchanged."""
+        if not np.isscalar(threshold) or threshold < 0:
+            raise ValueError(f"threshold must be a non-negative scalar, not {threshold!r}")
+        psi = self.psi
+        return np.where(np.abs(psi) > threshold, psi, 0)
+
+    def expectation_value(self, operator, component="real"):
+        """<psi|O|psi> (qmb_state.py:112-157) for O given as (rows, cols, values)
+        triplets (summed like exp_val_data, duplicates included), a dense
+        matrix, a SciPy sparse matrix, or a CSRMatrix on the device (one device
+        product).  The first three are applied to the host copy of psi, as the
+        reference does; component: "real" or "imag"."""
+        import scipy.sparse as sps
+        if component not in ("real", "imag"):
+            raise ValueError(f"component must be 'real' or 'imag', not {component!r}")
+        if isinstance(operator, CSRMatrix):
+            import torch
+            psi = self.psi_device
+            if operator.n != psi.numel():
+                raise ValueError(f"quantum state dim {psi.numel()} is NOT the operator dim {operator.n}")
+            if not operator.is_complex and psi.is_complex():
+                y = torch.complex(operator.matvec(psi.real.contiguous()), operator.matvec(psi.imag.contiguous()))
+            else:
+                y = operator.matvec(psi.contiguous())
+            val = complex(torch.vdot(psi, y).item())
+        elif isinstance(operator, tuple) and len(operator) == 3:
+            r, c, v = (np.asarray(a) for a in operator)
+            psi = self.psi
+            val = complex(np.sum(np.conj(psi[r]) * v * psi[c]))
+        elif isinstance(operator, np.ndarray) or sps.issparse(operator):
+            psi = self.psi
+            if operator.ndim != 2 or operator.shape[0] != operator.shape[1] or operator.shape[0] != psi.shape[0]:
+                raise ValueError(f"quantum state dim {psi.shape[0]} is NOT the operator dim {operator.shape[0]}")
+            val = complex(np.vdot(psi, sps.csr_matrix(operator) @ psi))
+        else:
+            raise TypeError("Operator must be provided as a tuple (row_list, col_list, value_list), "
+                            "a dense matrix (np.ndarray), or a sparse matrix (scipy sparse matrix).")
+        return val.real if component == "real" else val.imag
+
+    def participation_renyi_entropy(self, alpha=2):
+        """PE_alpha = log(sum_i |psi_i|^(2 alpha) + 1e-16) / (1 - alpha) in the
+        sector basis (qmb_state.py:357-391), reduced on the device."""
+        import torch
+        if alpha <= 0 or alpha == 1:
+            raise ValueError(f"alpha must be > 0 and != 1. Got alpha={alpha}.")
+        p = torch.abs(self.psi_device).double() ** 2
+        ipr = float(torch.sum(p ** float(alpha)).item())
+        return float(np.log(ipr + 1e-16) / (1.0 - float(alpha)))
+
     def entanglement_entropy(self, keep_indices, partitions_dict):
         """Von Neumann entropy (base 2, Schmidt weights > 1e-10) of the sites
         keep_indices (qmb_state.py:245-291), from the subsystem/environment

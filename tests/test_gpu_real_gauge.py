@@ -79,6 +79,14 @@ def test_real_gauge_matrix_and_solve(gpu, lvals, obc):
         assert psi.dtype == torch.complex128
         r = Hc.matvec(psi) - e_r[i] * psi
         assert float(torch.linalg.vector_norm(r)) <= 1e-8 * max(1.0, abs(e_r[i]))
+    # QMB_state.expectation_value through the device CSR (complex H, and the
+    # real matrix on the gauge-transformed vector), and through SciPy on the host
+    st = mod.H.Npsi[0]
+    assert abs(st.expectation_value(Hc) - e_r[0]) < 1e-9 * max(1.0, abs(e_r[0]))
+    assert abs(st.expectation_value(Hc, "imag")) < 1e-9
+    if orc.N <= 200_000:
+        assert abs(st.expectation_value(Ho) - e_r[0]) < 1e-9 * max(1.0, abs(e_r[0]))
+    assert 0.0 <= st.participation_renyi_entropy(2) <= np.log(orc.N) + 1e-12
     if e_c[1] - e_c[0] > 1e-6:  # non-degenerate ground state: same vector up to a phase
         ov = abs(complex(torch.vdot(psi_c, mod.H.Npsi[0].psi_device)))
         assert abs(ov - 1.0) < 1e-9

@@ -1,0 +1,69 @@
+"""CPU: QMB_state's post-processing helpers (qmb_state.py:79-157, 357-391):
+normalize, truncate, expectation_value and the participation Renyi entropy,
+checked on host tensors against closed forms: Pauli expectation values of a
+known qubit state in the three operator formats, Renyi entropies of a basis
+state (0), of a uniform superposition over N states (log N for every alpha),
+and of a two-state superposition (log(p^a + q^a) / (1 - a)); plus the
+reference's argument errors."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from edlgt_amd.models import QMB_state
+
+
+def _state(vec):
+    return QMB_state(torch.as_tensor(np.asarray(vec, dtype=np.complex128)), [len(vec)], [len(vec)])
+
+
+def test_normalize_and_truncate():
+    s = _state([3.0, 4.0j, 1e-15])
+    assert abs(s.normalize() - 5.0) < 1e-14
+    assert abs(s.get_norm() - 1.0) < 1e-15
+    assert np.allclose(s.psi, [0.6, 0.8j, 2e-16])
+    assert abs(s.normalize() - 1.0) < 1e-15  # already normalised: unchanged
+    t = s.truncate(1e-14)
+    assert t[2] == 0 and t[0] == s.psi[0] and t[1] == s.psi[1] and s.psi[2] != 0
+    with pytest.raises(ValueError):
+        s.truncate(-1.0)
+
+
+def test_pauli_expectation_values_in_every_format():
+    th, ph = 0.37, 1.1
+    # |psi> = cos(th)|0> + e^{i ph} sin(th)|1>: <X> = sin 2th cos ph, <Y> = sin 2th sin ph, <Z> = cos 2th
+    s = _state([np.cos(th), np.exp(1j * ph) * np.sin(th)])
+    X = np.array([[0, 1], [1, 0]], complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    Z = np.diag([1.0, -1.0]).astype(complex)
+    want = {"X": np.sin(2 * th) * np.cos(ph), "Y": np.sin(2 * th) * np.sin(ph), "Z": np.cos(2 * th)}
+    for name, M in (("X", X), ("Y", Y), ("Z", Z)):
+        r, c = np.nonzero(M)
+        for op in (M, sp.csr_matrix(M), sp.coo_matrix(M), (r, c, M[r, c])):
+            assert abs(s.expectation_value(op) - want[name]) < 1e-15
+            assert abs(s.expectation_value(op, "imag")) < 1e-15
+    # a non-Hermitian operator has an imaginary part: <psi|i Z|psi> = i cos 2th
+    assert abs(s.expectation_value(1j * Z, "imag") - np.cos(2 * th)) < 1e-15
+    with pytest.raises(TypeError):
+        s.expectation_value([[1, 0], [0, 1]])
+    with pytest.raises(ValueError):
+        s.expectation_value(np.eye(3))
+    with pytest.raises(ValueError):
+        s.expectation_value(Z, "abs")
+
+
+@pytest.mark.parametrize("alpha", [0.5, 2, 3])
+def test_participation_renyi_entropy_closed_forms(alpha):
+    assert abs(_state([0, 1, 0, 0]).participation_renyi_entropy(alpha)) < 1e-15
+    n = 64
+    u = _state(np.full(n, 1 / np.sqrt(n)) * np.exp(1j * np.arange(n)))
+    assert abs(u.participation_renyi_entropy(alpha) - np.log(n)) < 1e-12
+    p = 0.3
+    w = _state([np.sqrt(p), 1j * np.sqrt(1 - p)])
+    assert abs(w.participation_renyi_entropy(alpha) - np.log(p ** alpha + (1 - p) ** alpha) / (1 - alpha)) < 1e-14
+
+
+def test_participation_renyi_entropy_rejects_alpha():
+    for a in (0, 1, -2):
+        with pytest.raises(ValueError):
+            _state([1, 0]).participation_renyi_entropy(a)
