@@ -1401,6 +1401,18 @@ class SU2_Model(QuantumModel):
         logger.info(f"SEC DIM: {self.basis.n_states}")
         self.default_params()
 
+    def _gauge_weights(self):
+        """The hopping coefficients are -2 sqrt(2) i along x and real along y
+        and z (SU2_Hamiltonian_couplings, from SU2_model.py:555-624), and every
+        hopping entry moves one fermion across its bond: with a_j = x_j mod 2
+        the phase i^(sum_j a_j N_tot_j) turns the x entries real; the other
+        terms keep every N_tot_j.  A chain or lattice periodic along x with an
+        odd extent keeps imaginary wrap-around entries (real build refused)."""
+        if self.pure_theory:
+            return None
+        a = np.array([c[0] % 2 for c in map(self.lattice.coords, range(self.n_sites))], dtype=np.int64)
+        return a, "N_tot"
+
     def SU2_Hamiltonian_couplings(self, g, m=None, theta=0.0):
         """SU2_model.py:555-624 (hopping rescaled to 2 sqrt(2))."""
         if self.dim == 1:

@@ -1,5 +1,5 @@
-"""GPU: the real-gauge solve of QED with staggered matter
-(QED_Model.diagonalize_Hamiltonian(..., real_gauge=True)).
+"""GPU: the real-gauge solve of QED and SU(2) with staggered matter
+(diagonalize_Hamiltonian(..., real_gauge=True)).
 
 The device builds D^H H D, D = diag(i^(sum_j a_j N_j)), a_j = (x_j + z_j) mod 2,
 as a float64 CSR from the same term program with one power of i per
@@ -10,8 +10,9 @@ definition pinned in tests/test_real_gauge.py), the eigenvalues equal the
 complex solve's and the oracle's (<= 1e-10), the mapped vectors are
 eigenvectors of the complex H (residual through the complex CSR: a wrong phase
 map fails it), a non-degenerate ground state equals the complex solve's up to
-a global phase, and the observables agree.  A lattice periodic along x with an odd extent has no real
-gauge: the complex matrix is solved and H.real_gauge says so."""
+a global phase, and the observables agree.  A lattice periodic along x with
+an odd extent has no real gauge: the complex matrix is solved and
+H.real_gauge says so.  SU(2) takes a_j = x_j mod 2 with N_tot."""
 import numpy as np
 import pytest
 
@@ -127,3 +128,56 @@ def test_real_gauge_mixed_precision_and_rebuild(gpu):
         mod.diagonalize_Hamiltonian(1, "sparse", real_gauge=True, precision=PREC_MIXED)
         assert mod.H.real_gauge
         assert abs(mod.H.energies[0] - ref) < 1e-10, (g, mod.H.energies[0], ref)
+
+
+@pytest.mark.parametrize("lvals,obc,spin", [([6], [True], 0.5), ([6], [False], 0.5), ([2, 2], [False, False], 0.5),
+                                            ([4], [True], 1.0), ([2, 2, 2], [True, True, True], 0.5)])
+def test_su2_real_gauge(gpu, lvals, obc, spin):
+    """SU(2) with matter (x hopping -2 sqrt(2) i, a_j = x_j mod 2, N_tot): the
+    real matrix equals the oracle's D^H H D, and the real-gauge eigenpairs are
+    the complex solve's (eigenvalues <= 1e-10, residual through the complex H)."""
+    import torch
+    from edlgt_amd import SU2_Model
+    from oracle.model import SU2Oracle
+    print(f"real gauge: SU(2) lvals={lvals} obc={obc} spin={spin}", flush=True)
+    n = int(np.prod(lvals))
+    orc = SU2Oracle(False, lvals, obc, sectors=[n], spin=spin, generic=spin > 0.5)
+    mod = SU2_Model(spin=spin, pure_theory=False, sectors=[n], lvals=lvals, has_obc=obc)
+    Hc = mod.build_Hamiltonian(1.3, 0.7)
+    Ho = orc.hamiltonian(1.3, 0.7)
+    Hr, _ = mod.real_gauge_matrix()
+    a, name = mod._gauge_weights()
+    nd = np.rint(np.real(np.einsum("sii->si", orc.ops[name]))).astype(np.int64)
+    s = np.zeros(orc.N, dtype=np.int64)
+    for j in range(n):
+        s += a[j] * nd[j][orc.configs[:, j]]
+    ph = 1j ** (s % 4)
+    C = Ho.tocoo()
+    want = C.copy()
+    want.data = np.conj(ph[C.row]) * C.data * ph[C.col]
+    want = want.tocsr()
+    want.sort_indices()
+    rp, col, val = Hr.to_host()
+    assert np.array_equal(rp, want.indptr.astype(np.int64)) and np.array_equal(col, want.indices.astype(np.int32))
+    assert np.max(np.abs(val - want.data.real)) <= 1e-14 * np.max(np.abs(want.data))
+    k = min(3, orc.N - 1)
+    mod.diagonalize_Hamiltonian(k, "sparse", real_gauge=True)
+    assert mod.H.real_gauge
+    ref = lowest_eigs(Ho, k) if orc.N > 600 else np.linalg.eigvalsh(Ho.toarray())[:k]
+    assert np.max(np.abs(mod.H.energies - ref)) < 1e-10
+    for i in range(k):
+        psi = mod.H.Npsi[i].psi_device
+        r = Hc.matvec(psi) - mod.H.energies[i] * psi
+        assert float(torch.linalg.vector_norm(r)) <= 1e-8 * max(1.0, abs(mod.H.energies[i]))
+
+
+def test_su2_odd_periodic_chain_solves_the_complex_matrix(gpu):
+    from edlgt_amd import SU2_Model
+    from oracle.model import SU2Oracle
+    orc = SU2Oracle(False, [5], [False], sectors=[4])
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[4], lvals=[5], has_obc=[False])
+    mod.build_Hamiltonian(1.3, 0.7)
+    assert mod.real_gauge_matrix() is None
+    mod.diagonalize_Hamiltonian(2, "sparse", real_gauge=True)
+    assert not mod.H.real_gauge
+    assert np.max(np.abs(mod.H.energies - np.linalg.eigvalsh(orc.hamiltonian(1.3, 0.7).toarray())[:2])) < 1e-10

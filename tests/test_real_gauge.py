@@ -1,4 +1,4 @@
-"""CPU: the real gauge of QED with staggered matter.
+"""CPU: the real gauge of QED and SU(2) with staggered matter.
 
 D = diag(i^(sum_j a_j N_j)), a_j = (x_j + z_j) mod 2, turns the complex QED
 Hamiltonian with matter (hopping -i/2 along x, -i/2 / +i/2 along z, real along
@@ -11,6 +11,8 @@ Pinned here against the definition, with the oracle's complex H:
     placement's strength times a fixed power of i), assembled here on the host
     with the oracle's triplet routine, equals D^H H D entry for entry, and the
     unmodified program equals H (which pins this host assembly).
+The same holds for SU(2) with matter (hopping -2 sqrt(2) i along x, real
+along y and z: SU2_model.py:555-624) with a_j = x_j mod 2 and N_tot.
 The engine's model is built on the host with the sector taken from the oracle
 (the device enumeration is pinned bit-exactly elsewhere)."""
 import numpy as np
@@ -18,22 +20,22 @@ import pytest
 
 import edlgt_amd.models as models
 from oracle import lattice as lat
-from oracle.model import QEDOracle, Triplets, nbody_triplets
+from oracle.model import QEDOracle, SU2Oracle, Triplets, nbody_triplets
 
 
-def _phase_exponents(orc, a):
-    nd = np.rint(np.real(np.einsum("sii->si", orc.ops["N"]))).astype(np.int64)
+def _phase_exponents(orc, a, name="N"):
+    nd = np.rint(np.real(np.einsum("sii->si", orc.ops[name]))).astype(np.int64)
     s = np.zeros(orc.N, dtype=np.int64)
     for j in range(orc.n_sites):
         s += a[j] * nd[j][orc.configs[:, j]]
     return s % 4
 
 
-def _weights(lvals):
+def _weights(lvals, with_z=True):
     a = []
     for i in range(int(np.prod(lvals))):
         co = lat.zig_zag(lvals, i)
-        a.append((co[0] + (co[2] if len(co) > 2 else 0)) % 2)
+        a.append((co[0] + (co[2] if len(co) > 2 and with_z else 0)) % 2)
     return np.array(a, dtype=np.int64)
 
 
@@ -137,3 +139,48 @@ def test_no_gauge_for_the_pure_theory_and_odd_periodic_program(monkeypatch):
     pure = object.__new__(models.QED_Model)
     pure.spin, pure.pure_theory = 1, True
     assert pure._gauge_weights() is None
+
+
+SU2_CASES = [([6], [True], 0.5), ([6], [False], 0.5), ([2, 2], [False, False], 0.5), ([4], [True], 1.0),
+             ([2, 2, 2], [True, True, True], 0.5)]
+
+
+def _su2(lvals, obc, spin):
+    n = int(np.prod(lvals))
+    return SU2Oracle(False, lvals, obc, sectors=[n], spin=spin, generic=spin > 0.5)
+
+
+@pytest.mark.parametrize("lvals,obc,spin", SU2_CASES)
+def test_su2_gauge_makes_the_hamiltonian_real(lvals, obc, spin):
+    orc = _su2(lvals, obc, spin)
+    H = orc.hamiltonian(1.3, 0.7)
+    assert np.max(np.abs(H.data.imag)) > 1.0
+    G = _gauged(H, _phase_exponents(orc, _weights(lvals, with_z=False), "N_tot"))
+    assert np.max(np.abs(G.data.imag)) == 0.0
+
+
+def test_su2_odd_periodic_chain_has_no_real_gauge():
+    orc = SU2Oracle(False, [5], [False], sectors=[4])
+    G = _gauged(orc.hamiltonian(1.3, 0.7), _phase_exponents(orc, _weights([5], False), "N_tot"))
+    assert np.max(np.abs(G.data.imag)) > 1.0
+
+
+@pytest.mark.parametrize("lvals,obc,spin", SU2_CASES[:4])
+def test_su2_engine_program_is_the_gauged_hamiltonian(monkeypatch, lvals, obc, spin):
+    orc = _su2(lvals, obc, spin)
+    monkeypatch.setattr(models, "SectorBasis", lambda *a, **k: _HostSector(orc))
+    n = int(np.prod(lvals))
+    mod = models.SU2_Model(spin=spin, pure_theory=False, sectors=[n], lvals=lvals, has_obc=obc)
+    mod._build_program = lambda real: None
+    mod.build_Hamiltonian(1.3, 0.7)
+    a, name = mod._gauge_weights()
+    assert name == "N_tot" and np.array_equal(a, _weights(lvals, with_z=False))
+    H = orc.hamiltonian(1.3, 0.7)
+    plain = _assemble(mod._placements, mod._tables, orc)
+    assert np.array_equal(plain.indptr, H.indptr) and np.array_equal(plain.indices, H.indices)
+    assert np.max(np.abs(plain.data - H.data)) <= 1e-14 * np.max(np.abs(H.data))
+    G = _assemble(mod._real_gauge_placements(), mod._tables, orc)
+    want = _gauged(H, _phase_exponents(orc, a, name))
+    assert np.array_equal(G.indptr, want.indptr) and np.array_equal(G.indices, want.indices)
+    assert np.max(np.abs(G.data.imag)) == 0.0
+    assert np.max(np.abs(G.data - want.data)) <= 1e-14 * np.max(np.abs(H.data))
