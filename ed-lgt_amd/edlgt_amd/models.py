@@ -474,11 +474,18 @@ class QuantumModel:
             logger.info("Hamiltonian built matrix-free (entries generated in every product)")
         return csr
 
+    def _keep_program(self):
+        """Records the term program of the Hamiltonian being built (the
+        real-gauge matrix is built from it later; build_multi_term rewrites the
+        working program) and drops the real-gauge matrix of the previous build."""
+        self._h_program = (list(self._tables), [dict(p) for p in self._placements])
+        self._rg = None
+
     def _build_program(self, real):
+        self._keep_program()
         tables = np.stack(self._tables)
         vt = L.VALUE_F64 if real else L.VALUE_C128
         csr = self._make_csr(tables, self._placements, vt)
-        self._rg = None  # the real-gauge matrix of the previous build is stale
         if self.momentum_basis is not None:
             # B^H H B in the momentum sector (the reference's momentum triplets,
             # translational_sym.py:1394-1969, behind QMB_hamiltonian.build)
@@ -554,18 +561,20 @@ class QuantumModel:
         of dagger placements already gives.  None when a table changes n by
         different amounts on different entries (no fixed factor)."""
         w = self._gauge_weights()
-        if w is None or self.momentum_basis is not None:
+        prog = getattr(self, "_h_program", None)
+        if w is None or prog is None or self.momentum_basis is not None:
             return None
+        tables, placements = prog
         a, name = w
         nd = np.real(np.einsum("sii->si", self.ops[name]))
         quarter = (1.0, 1j, -1.0, -1j)
         charge = {}
         out = []
-        for P in self._placements:
+        for P in placements:
             e = 0
             for site, t in zip(P["sites"], P["tables"]):
                 if (t, site) not in charge:
-                    bra, ket = np.nonzero(np.abs(self._tables[t]) > 1e-10)
+                    bra, ket = np.nonzero(np.abs(tables[t]) > 1e-10)
                     q = nd[site][bra] - nd[site][ket]
                     fixed = q.size == 0 or (np.all(np.abs(q - q[0]) <= 1e-9) and abs(q[0] - round(q[0])) <= 1e-9)
                     charge[(t, site)] = (int(round(q[0])) if q.size else 0) if fixed else None
@@ -592,7 +601,7 @@ class QuantumModel:
         if placements is None:
             return None
         try:
-            csr = self._make_csr(np.stack(self._tables), placements, L.VALUE_F64)
+            csr = self._make_csr(np.stack(self._h_program[0]), placements, L.VALUE_F64)
         except ValueError:
             logger.info("no real gauge: D^H H D has complex entries")
             return None

@@ -128,6 +128,12 @@ def test_real_gauge_mixed_precision_and_rebuild(gpu):
         mod.diagonalize_Hamiltonian(1, "sparse", real_gauge=True, precision=PREC_MIXED)
         assert mod.H.real_gauge
         assert abs(mod.H.energies[0] - ref) < 1e-10, (g, mod.H.energies[0], ref)
+    # the multi-term program of a coupling sweep does not replace the program
+    # of the last build_Hamiltonian (g = 2) that the real matrix comes from
+    mod.build_Hamiltonian(2.0, 0.5)
+    mod.build_multi_term(0.5)
+    mod.diagonalize_Hamiltonian(1, "sparse", real_gauge=True)
+    assert mod.H.real_gauge and abs(mod.H.energies[0] - ref) < 1e-10, (mod.H.energies[0], ref)
 
 
 @pytest.mark.parametrize("lvals,obc,spin", [([6], [True], 0.5), ([6], [False], 0.5), ([2, 2], [False, False], 0.5),

@@ -84,7 +84,7 @@ class _HostSector:
 def _engine_model(monkeypatch, orc, lvals, obc):
     monkeypatch.setattr(models, "SectorBasis", lambda *a, **k: _HostSector(orc))
     mod = models.QED_Model(spin=1, pure_theory=False, lvals=lvals, has_obc=obc)
-    mod._build_program = lambda real: None  # keep the program on the host
+    mod._build_program = lambda real: mod._keep_program()  # keep the program on the host
     mod.build_Hamiltonian(1.3, 0.7)
     return mod
 
@@ -171,7 +171,7 @@ def test_su2_engine_program_is_the_gauged_hamiltonian(monkeypatch, lvals, obc, s
     monkeypatch.setattr(models, "SectorBasis", lambda *a, **k: _HostSector(orc))
     n = int(np.prod(lvals))
     mod = models.SU2_Model(spin=spin, pure_theory=False, sectors=[n], lvals=lvals, has_obc=obc)
-    mod._build_program = lambda real: None
+    mod._build_program = lambda real: mod._keep_program()
     mod.build_Hamiltonian(1.3, 0.7)
     a, name = mod._gauge_weights()
     assert name == "N_tot" and np.array_equal(a, _weights(lvals, with_z=False))
