@@ -1226,8 +1226,14 @@ class QED_Model(QuantumModel):
     def build_multi_term(self, m=None, dtype_mode="auto"):
         """Assembles the three coupling-independent parts of H(g) once on one
         pattern; the matrix at coupling g is sum_t multi_term_coefficients(g)[t] H_t
-        (products: CSRMatrix.matvec_coef, Server)."""
-        tables, placements = self.multi_term_program(m)
+        (products: CSRMatrix.matvec_coef, Server).  The couplings of the last
+        build_Hamiltonian (self.coeffs) are kept."""
+        saved = getattr(self, "coeffs", None)
+        try:
+            tables, placements = self.multi_term_program(m)
+        finally:
+            if saved is not None:
+                self.coeffs = saved
         real = self._dtype_real(0.0, dtype_mode)
         return CSRMatrix(self.basis, tables, placements, L.VALUE_F64 if real else L.VALUE_C128, n_terms=3)
 

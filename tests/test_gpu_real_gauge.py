@@ -132,6 +132,7 @@ def test_real_gauge_mixed_precision_and_rebuild(gpu):
     # of the last build_Hamiltonian (g = 2) that the real matrix comes from
     mod.build_Hamiltonian(2.0, 0.5)
     mod.build_multi_term(0.5)
+    assert mod.coeffs["g"] == 2.0  # the couplings of the last build are kept
     mod.diagonalize_Hamiltonian(1, "sparse", real_gauge=True)
     assert mod.H.real_gauge and abs(mod.H.energies[0] - ref) < 1e-10, (mod.H.energies[0], ref)
 
