@@ -105,3 +105,56 @@ def test_su2_tc_momentum(gpu, k):
         assert abs(mod.H.Nenergies[0] - (-6.173752796132477)) < 1e-10
     else:
         assert mod.H.Nenergies[0] > -6.173752796132477 - 1e-10
+
+
+def _finite_k_par(model, k_vals, unit_cell, **couplings):
+    par = {"model": model, "hamiltonian": {"n_eigs": 1, "save_psi": False},
+           "momentum": {"get_momentum_basis": True, "unit_cell_size": unit_cell, "momentum_k_vals": k_vals,
+                        "TC_symmetry": False},
+           "observables": {"measure_obs": False, "get_entropy": False, "get_state_configs": False,
+                           "get_overlap": False}}
+    par.update(couplings)
+    return par
+
+
+def test_reference_finite_k_models(gpu):
+    """The two models of the reference's finite-momentum check
+    (validation/symmetries/test_momentum_nbody_generic_finite_k_models.py:40-136),
+    run through the parameter-dictionary drivers with its parameters: the
+    6-site periodic SU(2) chain with matter at unit cell 1 and k = pi (g = 1,
+    m = 5), and the 2x2x2 periodic pure QED lattice with theta = 0.41 at
+    k = (pi, pi, pi) (g = 2.3; complex matrix, 146,343-state sector).  The
+    device projector and the ground-state energy are compared with the oracle's
+    projector and the lowest eigenvalue of its B^H H B."""
+    from edlgt_amd import QED_Model, SU2_Model
+    from edlgt_amd.workflows import run_QED_spectrum, run_SU2_spectrum
+    from oracle.model import QEDOracle
+    print("finite k: SU(2) L=6 PBC, unit cell 1, k = 3", flush=True)
+    par = _finite_k_par({"lvals": [6], "sectors": [6], "has_obc": [False], "spin": 0.5, "pure_theory": False,
+                         "ham_format": "sparse"}, [3], [1], g=1.0, m=5.0)
+    res = run_SU2_spectrum(par)
+    orc = SU2Oracle(False, [6], [False], sectors=[6])
+    Bo = momentum_basis(orc.configs, orc.keys, orc.loc_dims, [6], [1], [3])
+    ref = np.linalg.eigvalsh((Bo.conj().T @ orc.hamiltonian(1.0, 5.0) @ Bo).toarray())[0] / 6
+    assert abs(float(res["energy"][0]) - ref) < 1e-10, (res["energy"][0], ref)
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    mod.set_momentum_sector([1], [3])
+    _compare(mod.momentum_basis, Bo, mod.build_Hamiltonian(1.0, 5.0), orc.hamiltonian(1.0, 5.0))
+
+    print("finite k: QED 2x2x2 PBC pure, theta = 0.41, k = (1, 1, 1)", flush=True)
+    par = _finite_k_par({"lvals": [2, 2, 2], "has_obc": [False] * 3, "spin": 1, "pure_theory": True,
+                         "ham_format": "sparse"}, [1, 1, 1], [1, 1, 1], g=2.3, theta=0.41)
+    res = run_QED_spectrum(par)
+    q = QEDOracle(1, True, [2, 2, 2], [False] * 3)
+    Hq = q.hamiltonian(2.3, None, 0.41)
+    Bq = momentum_basis(q.configs, q.keys, q.loc_dims, [2, 2, 2], [1, 1, 1], [1, 1, 1])
+    ref = lowest_eigs((Bq.conj().T @ Hq @ Bq).tocsr(), 1)[0] / 8
+    assert abs(float(res["energy"][0]) - ref) < 1e-10, (res["energy"][0], ref)
+    mod = QED_Model(spin=1, pure_theory=True, lvals=[2, 2, 2], has_obc=[False] * 3)
+    mod.set_momentum_sector([1, 1, 1], [1, 1, 1])
+    Hk = mod.build_Hamiltonian(2.3, None, 0.41)
+    B = mod.momentum_basis.to_scipy()
+    assert B.shape == Bq.shape and np.array_equal(B.indptr, Bq.indptr) and np.array_equal(B.indices, Bq.indices)
+    assert np.max(np.abs(B.data - Bq.data)) < 1e-15
+    diff = (Hk.to_scipy() - (Bq.conj().T @ Hq @ Bq)).tocsr()  # 18,255 momentum states: compared sparse
+    assert np.max(np.abs(diff.data), initial=0.0) < 1e-13 * np.max(np.abs(Hq.data))
