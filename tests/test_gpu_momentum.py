@@ -158,3 +158,42 @@ def test_reference_finite_k_models(gpu):
     assert np.max(np.abs(B.data - Bq.data)) < 1e-15
     diff = (Hk.to_scipy() - (Bq.conj().T @ Hq @ Bq)).tocsr()  # 18,255 momentum states: compared sparse
     assert np.max(np.abs(diff.data), initial=0.0) < 1e-13 * np.max(np.abs(Hq.data))
+
+
+@pytest.mark.parametrize("lvals,sites2,sites4", [([6], [1, 4], [0, 2, 3, 5]), ([2, 1, 2], [0, 3], [0, 1, 2, 3])])
+def test_reference_generic_nbody_momentum(gpu, lvals, sites2, sites4):
+    """The reference's generic n-body momentum cases
+    (validation/symmetries/test_momentum_nbody_generic.py:16-41, 164-184): the
+    full space of qubits on a 6-site ring and on a 2x1x2 lattice, k = 0 of the
+    unit translations, and a two-site and a four-site product of its complex
+    2x2 operators (op f: [[0.7 c, 0.2 + 0.05i c], [-0.3 + 0.11i c, -0.4 c]],
+    c = f + 1) on non-adjacent sites.  B^H O B on the device (a stored CSR of
+    O projected by edlgt_csr_momentum) against the oracle's projector applied
+    to the Kronecker product of the factors."""
+    import scipy.sparse as sp
+    import edlgt_amd._lib as L
+    from edlgt_amd.device import CSRMatrix, MomentumBasis, SectorBasis
+    from oracle.model import encode
+    n = int(np.prod(lvals))
+    configs = np.array(np.unravel_index(np.arange(2 ** n), [2] * n)).T.astype(np.uint16)
+    keys = encode(configs, np.full(n, 2))
+    Bo = momentum_basis(configs, keys, np.full(n, 2), lvals, [1] * len(lvals), [0] * len(lvals))
+    basis = SectorBasis(np.full(n, 2))
+    assert np.array_equal(basis.configs(), configs)
+    mb = MomentumBasis(basis, lvals, [1] * len(lvals), [0] * len(lvals))
+    B = mb.to_scipy()
+    assert np.array_equal(B.indptr, Bo.indptr) and np.array_equal(B.indices, Bo.indices)
+    assert np.max(np.abs(B.data - Bo.data)) < 1e-15
+    for sites in (sites2, sites4):
+        ops = [np.array([[0.7 * (f + 1), 0.2 + 0.05j * (f + 1)], [-0.3 + 0.11j * (f + 1), -0.4 * (f + 1)]])
+               for f in range(len(sites))]
+        H = CSRMatrix(basis, np.stack(ops), [dict(sites=list(sites), tables=list(range(len(sites))), strength=1.0)],
+                      L.VALUE_C128)
+        full = sp.identity(1, dtype=complex, format="csr")
+        for s in range(n):
+            f = sites.index(s) if s in sites else None
+            full = sp.kron(full, sp.csr_matrix(ops[f]) if f is not None else sp.identity(2), format="csr")
+        want = (Bo.conj().T @ full @ Bo).toarray()
+        got = H.momentum(mb).to_scipy().toarray()
+        assert got.shape == want.shape
+        assert np.max(np.abs(got - want)) < 1e-13
