@@ -499,6 +499,14 @@ int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t
                                  (hipStream_t)stream);
 }
 
+int edlgt_basis_signed_image(const edlgt_basis* b, const int32_t* site_perm, const int32_t* loc_perm,
+                             const double* loc_sign, int32_t d_max, int64_t* out_index, double* out_sign,
+                             void* stream) {
+  EDLGT_REQUIRE(b && site_perm && loc_perm && loc_sign, "null argument");
+  EDLGT_REQUIRE(b->n_states == 0 || (out_index && out_sign), "null output");
+  return basis_signed_image(*b, site_perm, loc_perm, loc_sign, d_max, out_index, out_sign, (hipStream_t)stream);
+}
+
 int edlgt_basis_quarter_phase(const edlgt_basis* b, const int32_t* site_exp, int32_t d_max, int32_t n_vec,
                               const double* x, void* y, void* stream) {
   EDLGT_REQUIRE(b && site_exp, "null argument");

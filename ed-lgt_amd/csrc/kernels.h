@@ -100,6 +100,10 @@ int basis_marginals(const Basis& B, const void* psi, bool cplx, int d_max, doubl
 // Same result without allocating: partial sums in a static per-device array.
 int basis_marginals_noalloc(const Basis& B, const void* psi, bool cplx, int d_max, double* out_dev,
                             hipStream_t st);
+// Index and sign of every state's image under a site permutation with per-site
+// local maps (host outputs; -1 / an error when an image leaves the sector).
+int basis_signed_image(const Basis& B, const int32_t* site_perm, const int32_t* loc_perm, const double* loc_sign,
+                       int d_max, int64_t* out_idx, double* out_sign, hipStream_t st);
 // y = i^e x per basis state, e_r = sum_s tab[s][c_r[s]] mod 4 (real x, complex y).
 int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int n_vec, const double* x,
                         void* y, hipStream_t st);

@@ -117,6 +117,38 @@ __global__ void translate_rows_tc(const uint8_t* __restrict__ configs, int cfg_s
   }
 }
 
+
+// Signed image of every configuration under a site permutation with per-site
+// local maps: the image puts loc_perm[j][c_j] on site site_perm[j] and carries
+// the sign prod_j loc_sign[j][c_j]; out_idx[r] = its basis index (-1 and *err
+// set when the image is not in the sector).  The spatial inversion of
+// SU2_Model.get_parity_inversion_operator (inversion_sym.py:77-161).
+template <class KeyT>
+__global__ void signed_image_rows(const uint8_t* __restrict__ configs, int cfg_stride, int n_sites,
+                                  const KeyT* __restrict__ keys, int64_t n, const KeyT* __restrict__ strides,
+                                  const int32_t* __restrict__ site_perm, const int32_t* __restrict__ loc_perm,
+                                  const double* __restrict__ loc_sign, int d_max, int64_t* __restrict__ out_idx,
+                                  double* __restrict__ out_sign, int32_t* __restrict__ err) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t* cfg = configs + r * cfg_stride;
+    KeyT key = 0;
+    double sg = 1.0;
+    for (int j = 0; j < n_sites; ++j) {
+      const int a = cfg[j];
+      key += (KeyT)loc_perm[j * d_max + a] * strides[site_perm[j]];
+      sg *= loc_sign[j * d_max + a];
+    }
+    const int64_t q = lower_bound_k<KeyT>(keys, n, key);
+    if (q >= n || keys[q] != key) {
+      atomicOr(err, 1);
+      out_idx[r] = -1;
+    } else {
+      out_idx[r] = q;
+    }
+    out_sign[r] = sg;
+  }
+}
+
 __device__ inline double2 cm(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -598,6 +630,55 @@ int momentum_build(const Basis& B, int n_dims, const int32_t* lvals, const int32
   EDLGT_TRY(check_flag(b_err, st, &flag));
   if (flag) return set_error(EDLGT_ERR_KEY_NOT_FOUND, "the sector is not closed under the translations");
   return orbit_columns_device(b_trans, N, S, R, k_vals, st, out);
+}
+
+int basis_signed_image(const Basis& B, const int32_t* site_perm, const int32_t* loc_perm, const double* loc_sign,
+                       int d_max, int64_t* out_idx, double* out_sign, hipStream_t st) {
+  const int n_sites = B.n_sites;
+  const int64_t N = B.n_states;
+  if (d_max < 1 || d_max > 256) return set_error(EDLGT_ERR_INVALID_ARG, "d_max must be in [1, 256]");
+  std::vector<int> seen(n_sites, 0);
+  for (int j = 0; j < n_sites; ++j) {
+    if (site_perm[j] < 0 || site_perm[j] >= n_sites || seen[site_perm[j]]++)
+      return set_error(EDLGT_ERR_INVALID_ARG, "site_perm is not a permutation of the sites");
+    if (B.loc_dims[j] > d_max) return set_error(EDLGT_ERR_INVALID_ARG, "d_max below a local dimension");
+  }
+  for (int j = 0; j < n_sites; ++j)
+    for (int a = 0; a < B.loc_dims[j]; ++a) {
+      const int b = loc_perm[j * d_max + a];
+      if (b < 0 || b >= B.loc_dims[site_perm[j]])
+        return set_error(EDLGT_ERR_INVALID_ARG, "loc_perm maps a local state outside its image site's basis");
+    }
+  if (N == 0) return EDLGT_OK;
+  DevBuf b_sp, b_lp, b_ls, b_str, b_idx, b_sg, b_err;
+  EDLGT_TRY(upload(b_sp, site_perm, (size_t)n_sites, st));
+  EDLGT_TRY(upload(b_lp, loc_perm, (size_t)n_sites * d_max, st));
+  EDLGT_TRY(upload(b_ls, loc_sign, (size_t)n_sites * d_max, st));
+  if (B.key_words == 2)
+    EDLGT_TRY(upload(b_str, B.strides128.data(), B.strides128.size(), st));
+  else
+    EDLGT_TRY(upload(b_str, B.strides.data(), B.strides.size(), st));
+  EDLGT_TRY(b_idx.alloc((size_t)N * sizeof(int64_t)));
+  EDLGT_TRY(b_sg.alloc((size_t)N * sizeof(double)));
+  EDLGT_TRY(b_err.alloc(sizeof(int32_t)));
+  EDLGT_HIP(hipMemsetAsync(b_err.p, 0, sizeof(int32_t), st));
+  const unsigned g = grid_for(N, 256, 65536);
+  if (B.key_words == 2)
+    hipLaunchKernelGGL(signed_image_rows<u128>, dim3(g), dim3(256), 0, st, B.configs.as<uint8_t>(), B.cfg_stride,
+                       n_sites, B.keys.as<u128>(), N, b_str.as<u128>(), b_sp.as<int32_t>(), b_lp.as<int32_t>(),
+                       b_ls.as<double>(), d_max, b_idx.as<int64_t>(), b_sg.as<double>(), b_err.as<int32_t>());
+  else
+    hipLaunchKernelGGL(signed_image_rows<uint64_t>, dim3(g), dim3(256), 0, st, B.configs.as<uint8_t>(),
+                       B.cfg_stride, n_sites, B.keys.as<uint64_t>(), N, b_str.as<uint64_t>(), b_sp.as<int32_t>(),
+                       b_lp.as<int32_t>(), b_ls.as<double>(), d_max, b_idx.as<int64_t>(), b_sg.as<double>(),
+                       b_err.as<int32_t>());
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_HIP(hipMemcpyAsync(out_idx, b_idx.p, (size_t)N * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  EDLGT_HIP(hipMemcpyAsync(out_sign, b_sg.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, st));
+  int32_t flag = 0;
+  EDLGT_TRY(check_flag(b_err, st, &flag));  // synchronises
+  if (flag) return set_error(EDLGT_ERR_KEY_NOT_FOUND, "the sector is not closed under the map");
+  return EDLGT_OK;
 }
 
 int momentum_build_tc(const Basis& B, int d_loc, const int32_t* c_map, const double* c_sign, int32_t k,

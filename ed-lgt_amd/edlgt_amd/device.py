@@ -124,6 +124,28 @@ class SectorBasis:
         w = out.reshape(-1, 2)
         return [int(lo) | (int(hi) << 64) for lo, hi in w]
 
+    def signed_image(self, site_perm, loc_perm, loc_sign, stream=None):
+        """(index, sign) of every state's image under a site permutation with
+        per-site local maps: the image puts loc_perm[j][c_j] on site
+        site_perm[j] and carries prod_j loc_sign[j][c_j]
+        (edlgt_basis_signed_image).  loc_perm / loc_sign: (n_sites, d) or one
+        row for every site."""
+        sp = np.ascontiguousarray(site_perm, dtype=np.int32)
+        lp = np.atleast_2d(np.asarray(loc_perm, dtype=np.int32))
+        ls = np.atleast_2d(np.asarray(loc_sign, dtype=np.float64))
+        if lp.shape[0] == 1:
+            lp = np.repeat(lp, self.n_sites, axis=0)
+            ls = np.repeat(ls, self.n_sites, axis=0)
+        lp, ls = np.ascontiguousarray(lp), np.ascontiguousarray(ls)
+        if sp.shape != (self.n_sites,) or lp.shape != ls.shape or lp.shape[0] != self.n_sites:
+            raise ValueError("site_perm needs one entry per site, loc_perm / loc_sign one row per site")
+        idx = np.empty(self.n_states, dtype=np.int64)
+        sign = np.empty(self.n_states, dtype=np.float64)
+        L.check(L.load().edlgt_basis_signed_image(self._h, sp.ctypes.data, lp.ctypes.data, ls.ctypes.data,
+                                                  lp.shape[1], idx.ctypes.data, sign.ctypes.data,
+                                                  L.stream_handle(stream)))
+        return idx, sign
+
     def quarter_phase(self, site_exp, x, stream=None):
         """y[..., r] = i^e_r x[..., r] with e_r = sum_s site_exp[s, c_r[s]] mod 4,
         for real device vectors x ((N,) or (k, N) float64): complex128 result

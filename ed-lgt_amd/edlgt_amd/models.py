@@ -1528,6 +1528,72 @@ class SU2_Model(QuantumModel):
                     self._add_nbody(names, plaq, c["B"], True)
         return self._build_program(real)
 
+    def local_parity_labels(self, wrt_site):
+        """Local action of the spatial inversion on the six dressed states of
+        the 1D spin-1/2 chain with matter (SU2_model.py:708-741): (loc_perm,
+        loc_phase); wrt_site=False composes it with the particle-hole map."""
+        if self.dim > 1:
+            raise ValueError(f"Does not work in D={self.dim}>1")
+        if self.spin > 0.5:
+            raise ValueError(f"Does not work spin j={self.spin}>0.5")
+        if wrt_site:
+            return np.array([0, 1, 3, 2, 4, 5], dtype=np.int32), np.array([1, -1, 1, -1, 1, -1], dtype=np.int32)
+        return np.array([4, 5, 3, 2, 0, 1], dtype=np.int32), np.array([1, -1, 1, 1, 1, -1], dtype=np.int32)
+
+    def get_parity_inversion_operator(self, wrt_site):
+        """Spatial inversion of the chain about site L/2 - 1 (wrt_site) or
+        about the bond (L/2 - 1, L/2) composed with particle-hole, as a signed
+        permutation of the sector states: self.parityOP (SciPy CSR, one +-1
+        per column; SU2_model.py:743-763, inversion_sym.py:18-161).  The image
+        of every state is found on the device (edlgt_basis_signed_image)."""
+        import scipy.sparse as sps
+        if self.dim != 1:
+            raise NotImplementedError(f"Cannot apply PARITY in D={self.dim}>1")
+        loc_perm, loc_phase = self.local_parity_labels(wrt_site)
+        L_ = self.n_sites
+        center2 = 2 * (L_ // 2 - 1) + (0 if wrt_site else 1)
+        site_perm = np.array([(center2 - j) % L_ for j in range(L_)], dtype=np.int32)
+        row, sign = self.basis.signed_image(site_perm, loc_perm, loc_phase)
+        N = self.sector_dim
+        self._parity_image = row
+        self.parityOP = sps.csr_matrix((sign, (row, np.arange(N))), shape=(N, N))
+        return self.parityOP
+
+    def check_parity_inversion_operator(self):
+        """The checks of SU2_model.py:765-829: |PH - HP|, |P^2 - I|, |P^H - P|
+        and |P^H P - I| (returned), one +-1 per row and column, and, for every
+        basis state i, <i|H|i> = <Pi|H|Pi> and HP|i> = PH|i> (the reference's
+        per-state loop, vectorised: the diagonal of H against its permuted
+        diagonal, and the column norms of HP - PH) to 1e-14 times the largest
+        |H| entry (the reference's 1e-14 taken relative, as the entries carry
+        their own rounding); ValueError on the first state that fails.  H is
+        taken from the device (host copy)."""
+        import scipy.sparse as sps
+        from scipy.sparse.linalg import norm as spnorm
+        P = self.parityOP
+        H = self.H.Ham.to_scipy()
+        if self.momentum_basis is not None:
+            Pk = self.momentum_basis.to_scipy()
+            H = (Pk @ H @ Pk.conj().T).tocsr()
+        I = sps.identity(self.sector_dim, format="csr")
+        comm = (P @ H - H @ P).tocsc()
+        errs = {"PH-HP": float(spnorm(comm)), "P^2-I": float(spnorm(P @ P - I)),
+                "P^H-P": float(spnorm(P.conj().T - P)), "P^HP-I": float(spnorm(P.conj().T @ P - I))}
+        for k, v in errs.items():
+            logger.info(f"|{k}| = {v}")
+        if not (np.all(np.diff(P.indptr) == 1) and np.all(np.diff(P.tocsc().indptr) == 1)):
+            raise ValueError("the parity operator is not a signed permutation")
+        if self.momentum_basis is None:
+            d = np.real(H.diagonal())
+            img = self._parity_image  # P|i> = +-|img[i]>
+            cols = np.sqrt(np.asarray(abs(comm).power(2).sum(axis=0))).ravel()
+            tol = 1e-14 * max(1.0, float(abs(H).max()))
+            bad = np.nonzero((np.abs(d[img] - d) > tol) | (cols > tol))[0]
+            if bad.size:
+                ii = int(bad[0])
+                raise ValueError(f"config {ii} {self.sector_configs[ii]} not symmetrized")
+        return errs
+
     def overlap_QMB_state(self, name):
         """Reference product configurations (SU2_model.py:486-553), local-state
         indices of the hard-coded spin-1/2 basis."""

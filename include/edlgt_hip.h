@@ -363,6 +363,20 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
 int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
                                   double* out_dev, void* stream);
 
+/* Signed image of every basis state under a site permutation with per-site
+ * local maps: the image of configuration r puts loc_perm[j * d_max + c_r[j]]
+ * on site site_perm[j], with sign prod_j loc_sign[j * d_max + c_r[j]];
+ * out_index[r] = its basis index, out_sign[r] the sign (host arrays of N).
+ * Replaces build_parity_operator / parity_image_config
+ * (edlgt/symmetries/inversion_sym.py:77-161) behind
+ * SU2_Model.get_parity_inversion_operator (edlgt/models/SU2_model.py:708-763).
+ * An image outside the sector is EDLGT_ERR_KEY_NOT_FOUND (the reference
+ * writes -1 there); site_perm must be a permutation and loc_perm[j] must map
+ * into site site_perm[j]'s local basis (EDLGT_ERR_INVALID_ARG otherwise). */
+int edlgt_basis_signed_image(const edlgt_basis* b, const int32_t* site_perm, const int32_t* loc_perm,
+                             const double* loc_sign, int32_t d_max, int64_t* out_index, double* out_sign,
+                             void* stream);
+
 /* Diagonal quarter-turn phase of the basis states:
  *   y[v][r] = i^(e_r) x[v][r],  e_r = (sum_s site_exp[s * d_max + config_r[s]]) mod 4
  * x: device float64 (n_vec x N), y: device complex128 (n_vec x N).  The

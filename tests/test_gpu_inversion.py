@@ -1,0 +1,54 @@
+"""GPU: SU2_Model.get_parity_inversion_operator / check_parity_inversion_operator
+(SU2_model.py:708-829): the device images of every sector state against the
+oracle's inversion (oracle/inversion.py, pinned in tests/test_inversion.py),
+bit-exact; the model's checks pass on periodic chains in both modes and in a
+momentum sector; the ground state's parity <psi|P|psi> is +-1; an open chain
+(edge sites with three of the six dressed states) raises."""
+import numpy as np
+import pytest
+
+from oracle.inversion import parity_operator
+from oracle.model import SU2Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("L", [6, 8, 10])
+@pytest.mark.parametrize("wrt_site", [True, False])
+def test_parity_operator_matches_the_oracle(gpu, L, wrt_site):
+    from edlgt_amd import SU2_Model
+    print(f"parity: SU(2) L={L} PBC wrt_site={wrt_site}", flush=True)
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[L], lvals=[L], has_obc=[False])
+    orc = SU2Oracle(False, [L], [False], sectors=[L])
+    P = mod.get_parity_inversion_operator(wrt_site)
+    Po = parity_operator(orc.configs, orc.keys, orc.loc_dims, wrt_site)
+    assert np.array_equal(P.indptr, Po.indptr) and np.array_equal(P.indices, Po.indices)
+    assert np.array_equal(P.data, Po.data)
+    mod.build_Hamiltonian(1.0, 5.0)
+    errs = mod.check_parity_inversion_operator()
+    assert errs["P^2-I"] == 0.0 and errs["P^H-P"] == 0.0 and errs["P^HP-I"] == 0.0
+    assert errs["PH-HP"] < 1e-12
+    mod.diagonalize_Hamiltonian(2, "sparse")
+    if mod.H.energies[1] - mod.H.energies[0] > 1e-8:
+        psi = mod.H.Npsi[0].psi
+        assert abs(abs(np.real(np.vdot(psi, P @ psi))) - 1.0) < 1e-9
+
+
+def test_parity_check_in_a_momentum_sector(gpu):
+    """With a momentum basis the check compares P with B H_k B^H (the
+    reference's branch, SU2_model.py:767-773)."""
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    mod.get_parity_inversion_operator(True)
+    mod.set_momentum_sector([2], [0])
+    mod.build_Hamiltonian(1.0, 5.0)
+    errs = mod.check_parity_inversion_operator()
+    assert errs["PH-HP"] < 1e-12
+
+
+def test_open_chain_raises(gpu):
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[True])
+    # edge sites hold 3 of the 6 dressed states: the local map leaves their basis
+    with pytest.raises(ValueError, match="outside its image site's basis|not closed"):
+        mod.get_parity_inversion_operator(True)
