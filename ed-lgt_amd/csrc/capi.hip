@@ -513,7 +513,16 @@ int edlgt_basis_quarter_phase(const edlgt_basis* b, const int32_t* site_exp, int
   EDLGT_REQUIRE(n_vec >= 0, "negative vector count");
   EDLGT_REQUIRE(n_vec == 0 || b->n_states == 0 || (x && y), "null vector");
   EDLGT_REQUIRE(d_max >= 1 && b->n_sites * d_max <= 4096, "n_sites * d_max must be <= 4096");
-  return basis_quarter_phase(*b, site_exp, d_max, n_vec, x, y, (hipStream_t)stream);
+  return basis_quarter_phase(*b, site_exp, d_max, n_vec, 1, false, x, y, (hipStream_t)stream);
+}
+
+int edlgt_basis_quarter_phase_c(const edlgt_basis* b, const int32_t* site_exp, int32_t d_max, int32_t n_vec,
+                                int32_t direction, const void* x, void* y, void* stream) {
+  EDLGT_REQUIRE(b && site_exp, "null argument");
+  EDLGT_REQUIRE(n_vec >= 0, "negative vector count");
+  EDLGT_REQUIRE(n_vec == 0 || b->n_states == 0 || (x && y), "null vector");
+  EDLGT_REQUIRE(d_max >= 1 && b->n_sites * d_max <= 4096, "n_sites * d_max must be <= 4096");
+  return basis_quarter_phase(*b, site_exp, d_max, n_vec, direction, true, x, y, (hipStream_t)stream);
 }
 
 int edlgt_schmidt_gram(const void* psi, int32_t vec_type, int64_t n_states, const int64_t* pos, int32_t n_blocks,

@@ -104,9 +104,10 @@ int basis_marginals_noalloc(const Basis& B, const void* psi, bool cplx, int d_ma
 // local maps (host outputs; -1 / an error when an image leaves the sector).
 int basis_signed_image(const Basis& B, const int32_t* site_perm, const int32_t* loc_perm, const double* loc_sign,
                        int d_max, int64_t* out_idx, double* out_sign, hipStream_t st);
-// y = i^e x per basis state, e_r = sum_s tab[s][c_r[s]] mod 4 (real x, complex y).
-int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int n_vec, const double* x,
-                        void* y, hipStream_t st);
+// y = i^(dir e) x per basis state, e_r = sum_s tab[s][c_r[s]] mod 4 (real or
+// complex x, complex y; y may alias a complex x).
+int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int n_vec, int dir, bool cplx_in,
+                        const void* x, void* y, hipStream_t st);
 
 // out[i] = i-th uniform [0, 1) double of the PCG64 stream with 128-bit state
 // and increment {state_hi, state_lo, inc_hi, inc_lo} (numpy's

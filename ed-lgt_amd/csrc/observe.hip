@@ -105,15 +105,19 @@ int check_marginal_args(const Basis& B, int d_max) {
 
 
 // ---- diagonal quarter-turn phases of basis states.
-// e_r = (sum_s tab[s * d_max + c_r[s]]) mod 4 and y[v][r] = i^e_r x[v][r]:
-// maps a real vector of a Hamiltonian D^H H D that the diagonal phase
-// D = diag(i^e_r) made real back to a vector of H.  The per-site table sits
+// e_r = (sum_s tab[s * d_max + c_r[s]]) mod 4 and y[v][r] = i^(dir e_r) x[v][r]
+// (dir = +1 or -1): maps vectors between a Hamiltonian H and D^H H D, which the
+// diagonal phase D = diag(i^e_r) made real (D x for an eigenvector x of the
+// real matrix, D^H psi for a state evolved under it).  The per-site table sits
 // in LDS; multiplying by a power of i only swaps and negates components, so
-// the map is exact.
+// the map is exact.  Real input (kCplxIn false) or complex input; every
+// thread reads its own element before writing it, so y may alias x for
+// complex input.
+template <bool kCplxIn>
 __global__ void __launch_bounds__(256) quarter_phase_rows(const uint8_t* __restrict__ cfg, int64_t n,
                                                          int cfg_stride, int n_sites, int d_max,
-                                                         const int32_t* __restrict__ tab, int n_vec,
-                                                         const double* __restrict__ x, double2* __restrict__ y) {
+                                                         const int32_t* __restrict__ tab, int n_vec, int dir,
+                                                         const double* x, double2* y) {
   __shared__ int32_t stab[kMaxBins];
   const int bins = n_sites * d_max;
   for (int i = threadIdx.x; i < bins; i += blockDim.x) stab[i] = tab[i] & 3;
@@ -122,15 +126,18 @@ __global__ void __launch_bounds__(256) quarter_phase_rows(const uint8_t* __restr
     const uint8_t* row = cfg + r * cfg_stride;
     int e = 0;
     for (int s = 0; s < n_sites; ++s) e += stab[s * d_max + row[s]];
-    e &= 3;
+    e = (dir < 0 ? 4 - (e & 3) : e) & 3;
     for (int v = 0; v < n_vec; ++v) {
-      const double a = x[(int64_t)v * n + r];
+      const int64_t i = (int64_t)v * n + r;
+      double2 a;
+      if (kCplxIn) a = reinterpret_cast<const double2*>(x)[i];
+      else a = make_double2(x[i], 0.0);
       double2 o;
-      if (e == 0) o = make_double2(a, 0.0);
-      else if (e == 1) o = make_double2(0.0, a);
-      else if (e == 2) o = make_double2(-a, 0.0);
-      else o = make_double2(0.0, -a);
-      y[(int64_t)v * n + r] = o;
+      if (e == 0) o = a;
+      else if (e == 1) o = make_double2(-a.y, a.x);
+      else if (e == 2) o = make_double2(-a.x, -a.y);
+      else o = make_double2(a.y, -a.x);
+      y[i] = o;
     }
   }
 }
@@ -177,20 +184,28 @@ int basis_marginals_noalloc(const Basis& B, const void* psi, bool cplx, int d_ma
                           st);
 }
 
-// y (complex, n_vec x N) = i^e x (real, n_vec x N), e_r from the per-site
-// integer table tab_host [n_sites][d_max] (entries taken mod 4).
-int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int n_vec, const double* x,
-                        void* y, hipStream_t st) {
+// y (complex, n_vec x N) = i^(dir e) x (x real or complex, n_vec x N), e_r
+// from the per-site integer table tab_host [n_sites][d_max] (entries taken
+// mod 4).
+int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int n_vec, int dir, bool cplx_in,
+                        const void* x, void* y, hipStream_t st) {
   EDLGT_TRY(check_marginal_args(B, d_max));
   if (n_vec < 0) return set_error(EDLGT_ERR_INVALID_ARG, "negative vector count");
+  if (dir != 1 && dir != -1) return set_error(EDLGT_ERR_INVALID_ARG, "direction must be +1 or -1");
   if (B.n_states == 0 || n_vec == 0) return EDLGT_OK;
   const int bins = B.n_sites * d_max;
   DevBuf tab;
   EDLGT_TRY(tab.alloc((size_t)bins * sizeof(int32_t)));
   EDLGT_HIP(hipMemcpyAsync(tab.p, tab_host, (size_t)bins * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(quarter_phase_rows, dim3(grid_for(B.n_states, 256, 16384)), dim3(256), 0, st,
-                     B.configs.as<uint8_t>(), B.n_states, B.cfg_stride, B.n_sites, d_max, tab.as<int32_t>(),
-                     n_vec, x, static_cast<double2*>(y));
+  const unsigned g = grid_for(B.n_states, 256, 16384);
+  if (cplx_in)
+    hipLaunchKernelGGL(quarter_phase_rows<true>, dim3(g), dim3(256), 0, st, B.configs.as<uint8_t>(), B.n_states,
+                       B.cfg_stride, B.n_sites, d_max, tab.as<int32_t>(), n_vec, dir, static_cast<const double*>(x),
+                       static_cast<double2*>(y));
+  else
+    hipLaunchKernelGGL(quarter_phase_rows<false>, dim3(g), dim3(256), 0, st, B.configs.as<uint8_t>(), B.n_states,
+                       B.cfg_stride, B.n_sites, d_max, tab.as<int32_t>(), n_vec, dir, static_cast<const double*>(x),
+                       static_cast<double2*>(y));
   EDLGT_HIP(hipGetLastError());
   EDLGT_HIP(hipStreamSynchronize(st));  // tab is freed on return
   return EDLGT_OK;

@@ -83,6 +83,7 @@ SIGNATURES = {
     "edlgt_basis_marginals": (_i32, [_p, _p, _i32, _i32, _p, _p]),
     "edlgt_basis_marginals_noalloc": (_i32, [_p, _p, _i32, _i32, _p, _p]),
     "edlgt_basis_quarter_phase": (_i32, [_p, _p, _i32, _i32, _p, _p, _p]),
+    "edlgt_basis_quarter_phase_c": (_i32, [_p, _p, _i32, _i32, _i32, _p, _p, _p]),
     "edlgt_basis_signed_image": (_i32, [_p, _p, _p, _p, _i32, _p, _p, _p]),
     "edlgt_schmidt_gram": (_i32, [_p, _i32, _i64, _p, _i32, _p, _p, _i64, _p]),
     "edlgt_hermitian_eigvals": (_i32, [_p, _i64, _i32, _p, _p, _i64, _p]),

@@ -146,21 +146,32 @@ class SectorBasis:
                                                   L.stream_handle(stream)))
         return idx, sign
 
-    def quarter_phase(self, site_exp, x, stream=None):
-        """y[..., r] = i^e_r x[..., r] with e_r = sum_s site_exp[s, c_r[s]] mod 4,
-        for real device vectors x ((N,) or (k, N) float64): complex128 result
-        (edlgt_basis_quarter_phase)."""
+    def quarter_phase(self, site_exp, x, direction=1, out=None, stream=None):
+        """y[..., r] = i^(direction e_r) x[..., r] with e_r = sum_s site_exp[s, c_r[s]]
+        mod 4, for device vectors x ((N,) or (k, N)): float64 x (direction +1,
+        edlgt_basis_quarter_phase) or complex128 x (either direction, out may
+        be x itself, edlgt_basis_quarter_phase_c); complex128 result."""
         torch = _torch()
         tab = np.ascontiguousarray(site_exp, dtype=np.int32)
         if tab.ndim != 2 or tab.shape[0] != self.n_sites or tab.shape[1] < int(self.loc_dims.max()):
             raise ValueError("site_exp must be (n_sites, d_max) with d_max >= every local dimension")
-        if x.dtype != torch.float64 or not x.is_cuda or x.shape[-1] != self.n_states:
-            raise ValueError("x must be a float64 CUDA tensor over the sector")
-        x = x.contiguous()
+        if x.dtype not in (torch.float64, torch.complex128) or not x.is_cuda or x.shape[-1] != self.n_states:
+            raise ValueError("x must be a float64 or complex128 CUDA tensor over the sector")
+        if not x.is_contiguous():
+            raise ValueError("x must be contiguous")
         n_vec = 1 if x.dim() == 1 else int(x.shape[0])
-        y = torch.empty(x.shape, dtype=torch.complex128, device=x.device)
-        L.check(L.load().edlgt_basis_quarter_phase(self._h, tab.ctypes.data, tab.shape[1], n_vec, L.ptr(x),
-                                                   L.ptr(y), L.stream_handle(stream)))
+        y = torch.empty(x.shape, dtype=torch.complex128, device=x.device) if out is None else out
+        if y.shape != x.shape or y.dtype != torch.complex128 or not y.is_contiguous():
+            raise ValueError("out must be a contiguous complex128 tensor shaped like x")
+        if x.dtype == torch.float64:
+            if direction != 1:
+                raise ValueError("a real vector is mapped with direction +1")
+            L.check(L.load().edlgt_basis_quarter_phase(self._h, tab.ctypes.data, tab.shape[1], n_vec, L.ptr(x),
+                                                       L.ptr(y), L.stream_handle(stream)))
+        else:
+            L.check(L.load().edlgt_basis_quarter_phase_c(self._h, tab.ctypes.data, tab.shape[1], n_vec,
+                                                         int(direction), L.ptr(x), L.ptr(y),
+                                                         L.stream_handle(stream)))
         return y
 
     def lookup(self, configs):

@@ -308,12 +308,16 @@ class QMB_hamiltonian:
                 logger.info(f"{e:.10f}  {e / nsites:.10f}")
 
     def time_evolution(self, initial_state, time_line, loc_dims, basis=None, d_max=None, m=30,
-                       reorth=True, precision=L.PREC_FP64, store_states=True):
+                       reorth=True, precision=L.PREC_FP64, store_states=True, gauge=None):
         """exp(-iHt) psi0 on a uniform time grid by device Krylov stepping
         (replaces expm_multiply in QMB_hamiltonian.time_evolution,
         qmb_hamiltonian.py:334-426).  psi_time[k] = exp(-i H t_k) psi0.  The
         stepping state (m+1 basis vectors) stays on the device for the next call
-        with the same matrix and settings; release_krylov() frees it."""
+        with the same matrix and settings; release_krylov() frees it.
+        gauge = (real CSR of D^H H D, sector basis, per-site exponent table):
+        evolve D^H psi0 under that matrix (exp(-iHt) = D exp(-i D^H H D t) D^H)
+        and map the states back through D; the site marginals are the same in
+        both gauges."""
         import torch
         from .device import KrylovSession
         time_line = np.asarray(time_line, dtype=float)
@@ -330,12 +334,15 @@ class QMB_hamiltonian:
             psi = torch.as_tensor(np.asarray(initial_state, dtype=np.complex128), device="cuda").clone()
         if psi.numel() != N:
             raise ValueError("initial_state has the wrong length")
+        H = self.Ham if gauge is None else gauge[0]
+        if gauge is not None:
+            gauge[1].quarter_phase(gauge[2], psi, direction=-1, out=psi)  # D^H psi0
         dm = int(d_max or 0) if basis is not None else 0
-        key = (id(self.Ham), None if basis is None else id(basis), int(precision), int(m), bool(reorth), dm)
+        key = (id(H), None if basis is None else id(basis), int(precision), int(m), bool(reorth), dm)
         ks = getattr(self, "_krylov", None)
-        if ks is None or ks.key != key or ks.csr is not self.Ham:
+        if ks is None or ks.key != key or ks.csr is not H:
             self.release_krylov()
-            ks = self._krylov = KrylovSession(self.Ham, basis, precision, m, reorth, dm)
+            ks = self._krylov = KrylovSession(H, basis, precision, m, reorth, dm)
         if abs(time_line[0]) > 0:
             ks.run(psi, float(time_line[0]), 1)
         n_steps = time_line.size - 1
@@ -345,6 +352,10 @@ class QMB_hamiltonian:
             marg = np.zeros((n_steps + 1, basis.n_sites, d_max))
         err = np.zeros(max(n_steps, 1))
         ks.run(psi, dt, n_steps, marg, err, hist)
+        if gauge is not None:  # back to the gauge of H
+            gauge[1].quarter_phase(gauge[2], psi, direction=1, out=psi)
+            if hist is not None:
+                gauge[1].quarter_phase(gauge[2], hist, direction=1, out=hist)
         self.krylov_err = err[:n_steps]
         self.marginals = marg
         self.psi_final = psi
@@ -611,13 +622,23 @@ class QuantumModel:
         self._rg = (csr, site_exp)
         return self._rg
 
-    def time_evolution_Hamiltonian(self, initial_state, time_line, **kw):
+    def time_evolution_Hamiltonian(self, initial_state, time_line, real_gauge=False, **kw):
         """quantum_model.py:613-615; per-step site marginals are kept in
-        self.H.marginals so that dynamics observables need no extra pass."""
+        self.H.marginals so that dynamics observables need no extra pass.
+        real_gauge=True evolves under the real matrix D^H H D when a diagonal
+        phase D makes it real (real_gauge_matrix; complex states, float64
+        values: 12 instead of 20 bytes per stored entry in every product) and
+        maps the states back; H.real_gauge tells which ran."""
         d_max = int(self.loc_dims.max())
         # per-step marginals need sector-basis states: not in a momentum sector
         basis = self.basis if self.momentum_basis is None else None
-        self.H.time_evolution(initial_state, time_line, self.loc_dims, basis=basis, d_max=d_max, **kw)
+        gauge = None
+        if real_gauge:
+            rg = self.real_gauge_matrix()
+            if rg is not None:
+                gauge = (rg[0], self.basis, rg[1])
+        self.H.time_evolution(initial_state, time_line, self.loc_dims, basis=basis, d_max=d_max, gauge=gauge, **kw)
+        self.H.real_gauge = gauge is not None
 
     # -- observables (quantum_model.py:1158-1333)
     def get_observables(self, local_obs=[], twobody_obs=[], plaquette_obs=[], nbody_obs=[],

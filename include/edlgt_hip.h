@@ -388,6 +388,15 @@ int edlgt_basis_signed_image(const edlgt_basis* b, const int32_t* site_perm, con
  * the same eigenpairs.  site_exp: host int32 [n_sites * d_max]. */
 int edlgt_basis_quarter_phase(const edlgt_basis* b, const int32_t* site_exp, int32_t d_max, int32_t n_vec,
                               const double* x, void* y, void* stream);
+/* The same phase on complex vectors, either way: y[v][r] = i^(direction e_r)
+ * x[v][r], direction +1 (D x) or -1 (D^H x); x, y: device complex128
+ * (n_vec x N), y may equal x.  The real-gauge time evolution
+ * (time_evolution_Hamiltonian(..., real_gauge=True)) evolves D^H psi0 under
+ * the real D^H H D and maps the states back; the reference's expm_multiply on
+ * the complex matrix (edlgt/modeling/qmb_hamiltonian.py:334-426) gives the
+ * same states.  No counterpart in the reference. */
+int edlgt_basis_quarter_phase_c(const edlgt_basis* b, const int32_t* site_exp, int32_t d_max, int32_t n_vec,
+                                int32_t direction, const void* x, void* y, void* stream);
 
 /* ======================================================================
  * Schmidt blocks of a state over a bipartition of the lattice.  Replaces the

@@ -188,3 +188,34 @@ def test_su2_odd_periodic_chain_solves_the_complex_matrix(gpu):
     mod.diagonalize_Hamiltonian(2, "sparse", real_gauge=True)
     assert not mod.H.real_gauge
     assert np.max(np.abs(mod.H.energies - np.linalg.eigvalsh(orc.hamiltonian(1.3, 0.7).toarray())[:2])) < 1e-10
+
+
+@pytest.mark.parametrize("lvals,obc", [([10], [True]), ([2, 2], [False, False]), ([6], [False])])
+def test_real_gauge_time_evolution(gpu, lvals, obc):
+    """exp(-iHt) psi0 = D exp(-i D^H H D t) D^H psi0: the real-gauge Krylov
+    states equal the complex run's (<= 1e-10) and the oracle's expm_multiply
+    path (<= 1e-8), with the same per-step site marginals, from a random
+    complex start state (QED with matter)."""
+    from edlgt_amd import QED_Model
+    from oracle.model import expm_path
+    print(f"real-gauge dynamics: QED lvals={lvals} obc={obc}", flush=True)
+    orc = QEDOracle(1, False, lvals, obc)
+    mod = QED_Model(spin=1, pure_theory=False, lvals=lvals, has_obc=obc)
+    mod.build_Hamiltonian(1.3, 0.7)
+    rng = np.random.default_rng(3)
+    psi0 = rng.normal(size=orc.N) + 1j * rng.normal(size=orc.N)
+    psi0 /= np.linalg.norm(psi0)
+    tl = np.linspace(0.0, 0.5, 6)
+    mod.time_evolution_Hamiltonian(psi0, tl)
+    assert not mod.H.real_gauge
+    ref = [st.psi.copy() for st in mod.H.psi_time]
+    marg = mod.H.marginals.copy()
+    mod.time_evolution_Hamiltonian(psi0, tl, real_gauge=True)
+    assert mod.H.real_gauge
+    got = [st.psi for st in mod.H.psi_time]
+    assert max(np.max(np.abs(a - b)) for a, b in zip(got, ref)) < 1e-10
+    assert np.max(np.abs(mod.H.marginals - marg)) < 1e-10
+    assert np.max(np.abs(mod.H.psi_final.cpu().numpy() - ref[-1])) < 1e-10
+    exact = expm_path(orc.hamiltonian(1.3, 0.7), psi0, 0.0, 0.5, 6)
+    assert max(np.max(np.abs(got[k] - exact[k])) for k in range(6)) < 1e-8
+    mod.H.release_krylov()
