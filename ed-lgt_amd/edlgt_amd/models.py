@@ -562,8 +562,9 @@ class QuantumModel:
         Hamiltonian is real, or None when the model knows no such phase."""
         return None
 
-    def _real_gauge_placements(self):
-        """The program of D^H H D: each placement's strength times
+    def _real_gauge_placements(self, program=None):
+        """The program of D^H H D (of the last build_Hamiltonian, or the given
+        (tables, placements)): each placement's strength times
         i^(-sum_f a_f q_f), where q_f is the change of n (bra minus ket) that
         factor f's table makes at its site.  A placement maps a ket c to a bra r
         with s_r - s_c = sum_f a_f q_f (s = sum_j a_j n_j), so every entry of
@@ -572,7 +573,7 @@ class QuantumModel:
         of dagger placements already gives.  None when a table changes n by
         different amounts on different entries (no fixed factor)."""
         w = self._gauge_weights()
-        prog = getattr(self, "_h_program", None)
+        prog = getattr(self, "_h_program", None) if program is None else program
         if w is None or prog is None or self.momentum_basis is not None:
             return None
         tables, placements = prog
@@ -1244,19 +1245,34 @@ class QED_Model(QuantumModel):
         return np.stack(self._tables), list(self._placements)
 
 
-    def build_multi_term(self, m=None, dtype_mode="auto"):
+    def build_multi_term(self, m=None, dtype_mode="auto", real_gauge=False):
         """Assembles the three coupling-independent parts of H(g) once on one
         pattern; the matrix at coupling g is sum_t multi_term_coefficients(g)[t] H_t
         (products: CSRMatrix.matvec_coef, Server).  The couplings of the last
-        build_Hamiltonian (self.coeffs) are kept."""
+        build_Hamiltonian (self.coeffs) are kept.  real_gauge=True assembles
+        the parts of D^H H(g) D instead (the term coefficients are real, so one
+        phase per placement serves every g; see real_gauge_matrix) with float64
+        values: the same eigenvalues, eigenvectors in the gauge of D
+        (csr.real_gauge says whether the real build applied)."""
         saved = getattr(self, "coeffs", None)
         try:
             tables, placements = self.multi_term_program(m)
         finally:
             if saved is not None:
                 self.coeffs = saved
+        if real_gauge:
+            rg = self._real_gauge_placements((list(tables), placements))
+            if rg is not None:
+                try:
+                    csr = CSRMatrix(self.basis, tables, rg, L.VALUE_F64, n_terms=3)
+                    csr.real_gauge = True
+                    return csr
+                except ValueError:
+                    logger.info("no real gauge: D^H H D has complex entries")
         real = self._dtype_real(0.0, dtype_mode)
-        return CSRMatrix(self.basis, tables, placements, L.VALUE_F64 if real else L.VALUE_C128, n_terms=3)
+        csr = CSRMatrix(self.basis, tables, placements, L.VALUE_F64 if real else L.VALUE_C128, n_terms=3)
+        csr.real_gauge = False
+        return csr
 
 
     def multi_term_coefficients(self, g):

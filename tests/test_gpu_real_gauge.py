@@ -219,3 +219,26 @@ def test_real_gauge_time_evolution(gpu, lvals, obc):
     exact = expm_path(orc.hamiltonian(1.3, 0.7), psi0, 0.0, 0.5, 6)
     assert max(np.max(np.abs(got[k] - exact[k])) for k in range(6)) < 1e-8
     mod.H.release_krylov()
+
+
+def test_real_gauge_multi_term_sweep(gpu):
+    """build_multi_term(real_gauge=True): the parts of D^H H(g) D on one real
+    pattern; one server solves a coupling sweep on it with the eigenvalues of
+    the complex per-coupling builds (<= 1e-10).  A lattice without a real gauge
+    falls back to the complex multi-term matrix."""
+    from edlgt_amd import QED_Model, Server
+    mod = QED_Model(spin=1, pure_theory=False, lvals=[2, 2], has_obc=[False, False])
+    gs = [0.5, 1.0, 2.0]
+    plain = {}
+    for g in gs:
+        H = mod.build_Hamiltonian(g, 0.5)
+        plain[g] = H.lanczos(4, tol=1e-12, want_vectors=False)[0]
+    H3 = mod.build_multi_term(0.5, real_gauge=True)
+    assert H3.real_gauge and not H3.is_complex
+    res = Server(H3, n_eigs=4, tol=1e-12, max_basis=64, max_slots=3).solve_all(
+        [(i, mod.multi_term_coefficients(g), 7 + i) for i, g in enumerate(gs)])
+    for i, g in enumerate(gs):
+        assert np.max(np.abs(res[i][0] - plain[g])) < 1e-10, (g, res[i][0], plain[g])
+    odd = QED_Model(spin=1, pure_theory=False, lvals=[3, 2], has_obc=[False, False])
+    H3o = odd.build_multi_term(0.5, real_gauge=True)
+    assert not H3o.real_gauge and H3o.is_complex
