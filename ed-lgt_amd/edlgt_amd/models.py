@@ -530,6 +530,15 @@ class QuantumModel:
             self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals)
         logger.info(f"Momentum basis shape ({self.momentum_basis.n_rows},{self.momentum_basis.n_cols})")
 
+    def _basis_Pk_as_csr(self):
+        """The momentum projector B (sector rows x momentum columns) as SciPy
+        CSR, or None without a momentum sector (quantum_model.py:626-632)."""
+        return None if self.momentum_basis is None else self.momentum_basis.to_scipy().tocsr()
+
+    def _basis_Pk_as_csc(self):
+        """The same projector as SciPy CSC (quantum_model.py:615-624)."""
+        return None if self.momentum_basis is None else self.momentum_basis.to_scipy().tocsc()
+
     def _sector_state(self, psi_device):
         """A state over the sector basis (momentum states expanded through B)."""
         mb = self.momentum_basis
@@ -1137,6 +1146,20 @@ class QED_Model(QuantumModel):
                       for c in map(self.lattice.coords, range(self.n_sites))], dtype=np.int64)
         return a, "N"
 
+    def build_truncated_Hamiltonian(self, g, m=None, theta=0.0, dtype_mode="auto"):
+        """The truncated-link builder by name (QED_model.py:235-394);
+        build_Hamiltonian dispatches to it for integer spins."""
+        if self.spin == "integrated":
+            raise ValueError("this model integrates the gauge fields out: use build_integrated_Hamiltonian")
+        return self.build_Hamiltonian(g, m, theta, dtype_mode)
+
+    def build_integrated_Hamiltonian(self, g, m=None, theta=0.0, dtype_mode="auto"):
+        """The integrated-gauge builder by name (QED_model.py:396-652), for
+        spin='integrated' models."""
+        if self.spin != "integrated":
+            raise ValueError("this model has truncated links: use build_truncated_Hamiltonian")
+        return self.build_Hamiltonian(g, m, theta, dtype_mode)
+
     def overlap_QMB_state(self, name):
         """Named reference configurations (QED_model.py:778-899), one local-state
         index per site, for get_qmb_state_from_configs:
@@ -1529,6 +1552,21 @@ class SU2_Model(QuantumModel):
                                           "spin-1/2 background charges")
             self._add_nbody(["Vpx_dag", "Vmx"], lat.bonds("x"), -lambda_noise, True)
         return self._build_program(real)
+
+    def build_base_Hamiltonian(self, g, m=None, theta=0.0, lambda_noise=0.0, dtype_mode="auto"):
+        """The hard-coded spin-1/2 builder by name (SU2_model.py:200-355), for
+        models built with that operator set; build_Hamiltonian dispatches to it."""
+        if self.use_generic_model:
+            raise ValueError("this model carries the generalised operators: use build_gen_Hamiltonian")
+        return self.build_Hamiltonian(g, m, theta, lambda_noise, dtype_mode)
+
+    def build_gen_Hamiltonian(self, g, m=None, dtype_mode="auto"):
+        """The generalised builder by name (SU2_model.py:357-484), for models
+        built with use_generic_model=True or spin >= 1."""
+        if not self.use_generic_model:
+            raise ValueError("this model carries the hard-coded spin-1/2 operators: construct it with "
+                             "use_generic_model=True to build the generalised Hamiltonian")
+        return self.build_Hamiltonian(g, m, dtype_mode=dtype_mode)
 
     # corner combinations of the generalised plaquette (SU2_model.py:420-437)
     _GEN_PLAQ = ["ABABABAB", "AAABBBAB", "ABABAABB", "AAABBABB", "ABBBABAA", "AABBBBAA", "ABBBAABA",

@@ -197,3 +197,15 @@ def test_reference_generic_nbody_momentum(gpu, lvals, sites2, sites4):
         got = H.momentum(mb).to_scipy().toarray()
         assert got.shape == want.shape
         assert np.max(np.abs(got - want)) < 1e-13
+
+
+def test_momentum_projector_as_scipy(gpu):
+    """_basis_Pk_as_csr / _basis_Pk_as_csc (quantum_model.py:615-632): the
+    momentum projector as SciPy, None without a momentum sector."""
+    from edlgt_amd import SU2_Model
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    assert mod._basis_Pk_as_csr() is None
+    mod.set_momentum_sector([2], [1])
+    B = mod.momentum_basis.to_scipy()
+    for M in (mod._basis_Pk_as_csr(), mod._basis_Pk_as_csc()):
+        assert M.shape == B.shape and abs(M - B).max() == 0

@@ -184,3 +184,28 @@ def test_background_flux_string_closed_form(gpu):
     assert abs(mod.H.Nenergies[0] * 6 - 2 * 1.3 * 3) < 1e-12
     with pytest.raises(ValueError):
         SU2_Model(spin=0.5, pure_theory=True, lvals=[4], has_obc=[True], bg_list=[0.3, 0, 0, 0])
+
+
+def test_named_builders(gpu):
+    """The builders by their reference names (SU2_model.py:200, 357;
+    QED_model.py:235, 396) give the dispatched matrix, and refuse the other
+    operator set."""
+    from edlgt_amd import QED_Model, SU2_Model
+    gen = SU2_Model(spin=1.0, pure_theory=False, sectors=[4], lvals=[4], has_obc=[True])
+    a = gen.build_Hamiltonian(1.3, 0.7).to_host()
+    b = gen.build_gen_Hamiltonian(1.3, 0.7).to_host()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    with pytest.raises(ValueError):
+        gen.build_base_Hamiltonian(1.3, 0.7)
+    base = SU2_Model(spin=0.5, pure_theory=False, sectors=[4], lvals=[4], has_obc=[True])
+    a = base.build_Hamiltonian(1.3, 0.7).to_host()
+    b = base.build_base_Hamiltonian(1.3, 0.7).to_host()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    with pytest.raises(ValueError):
+        base.build_gen_Hamiltonian(1.3, 0.7)
+    q = QED_Model(spin=1, pure_theory=False, lvals=[4], has_obc=[True])
+    a = q.build_Hamiltonian(1.0, 1.0).to_host()
+    b = q.build_truncated_Hamiltonian(1.0, 1.0).to_host()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    with pytest.raises(ValueError):
+        q.build_integrated_Hamiltonian(1.0, 1.0)
