@@ -530,6 +530,39 @@ class QuantumModel:
             self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals)
         logger.info(f"Momentum basis shape ({self.momentum_basis.n_rows},{self.momentum_basis.n_cols})")
 
+    def get_abelian_symmetry_sector(self, global_ops=None, global_sectors=None, global_sym_type="U",
+                                    link_ops=None, link_sectors=None, nbody_ops=None, nbody_sectors=None,
+                                    nbody_sites_list=None, nbody_sym_type=None):
+        """The symmetry sector of abelian generators given as per-site operator
+        tables (quantum_model.py:504-604): global generators with their
+        sectors ("U" sums or "Z" products), link generators [O_plus, O_minus]
+        per direction on the lattice's link pairs, and n-body generators on
+        site lists (used with link generators only, as in the reference).  The
+        diagonals of the tables define the constraints; the sector is
+        enumerated on the device and replaces this model's basis (the
+        Hamiltonian and the per-site tables stay as they are).  Without any
+        generator the sector is the whole product space."""
+        def diag(op):
+            return np.real(np.einsum("sii->si", np.asarray(op)))
+
+        glob = link = nbody = None
+        if global_ops is not None:
+            glob = (np.stack([diag(o) for o in global_ops]), np.asarray(global_sectors, dtype=float),
+                    global_sym_type)
+        if link_ops is not None:
+            ld = np.stack([np.stack([diag(pair[0]), diag(pair[1])]) for pair in link_ops])
+            link = (ld, np.asarray(link_sectors, dtype=float), self.lattice.link_pairs())
+            if nbody_ops is not None and global_ops is None:
+                nbody = (np.stack([diag(o) for o in nbody_ops]), np.asarray(nbody_sectors, dtype=float),
+                         [list(map(int, sl)) for sl in nbody_sites_list], nbody_sym_type or "U")
+        self.basis = SectorBasis(self.loc_dims, glob=glob, link=link, nbody=nbody)
+        if self.basis.n_states == 0:
+            raise ValueError("No configurations found for the given symmetry sectors")
+        logger.info(f"SEC DIM: {self.basis.n_states}")
+        self.momentum_basis = None
+        self.default_params()
+        return self.basis
+
     def _basis_Pk_as_csr(self):
         """The momentum projector B (sector rows x momentum columns) as SciPy
         CSR, or None without a momentum sector (quantum_model.py:626-632)."""

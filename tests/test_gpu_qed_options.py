@@ -77,3 +77,35 @@ def test_get_only_bulk_and_plaquette_basis(gpu):
         QED_Model(spin=1, pure_theory=False, lvals=[4], has_obc=[True], get_only_bulk=True)
     with pytest.raises(NotImplementedError):
         QED_Model(spin=1, pure_theory=True, lvals=[2, 2], has_obc=[False, False], plaq_basis=True)
+
+
+def test_get_abelian_symmetry_sector_rebuilds_the_model_sectors(gpu):
+    """QuantumModel.get_abelian_symmetry_sector (quantum_model.py:504-604) with
+    the generators the reference's QED constructor passes (QED_model.py:118-199):
+    the global N_tot = L/2 with the link constraints (matter), the link
+    constraints with the origin-line flux constraints (pure, periodic 2D), and
+    a global-only sector, against the constructor's sector and the oracle."""
+    from edlgt_amd import QED_Model, SU2_Model
+    from oracle.model import QEDOracle
+    m = QED_Model(spin=1, pure_theory=False, lvals=[4, 2], has_obc=[False, False])
+    want = m.sector_configs.copy()
+    m.get_abelian_symmetry_sector(global_ops=[m.ops["N"]], global_sectors=[4],
+                                  link_ops=[[m.ops[f"E_p{d}"], -m.ops[f"E_m{d}"]] for d in "xy"],
+                                  link_sectors=[0, 0])
+    assert np.array_equal(m.sector_configs, want)
+    p = QED_Model(spin=1, pure_theory=True, lvals=[2, 2], has_obc=[False, False])
+    want = p.sector_configs.copy()
+    p.get_abelian_symmetry_sector(link_ops=[[p.ops[f"E_p{d}"], -p.ops[f"E_m{d}"]] for d in "xy"],
+                                  link_sectors=[0, 0], nbody_ops=[p.ops["E_px"], p.ops["E_py"]],
+                                  nbody_sectors=[0, 0],
+                                  nbody_sites_list=[p.lattice.origin_line("y"), p.lattice.origin_line("x")],
+                                  nbody_sym_type="U")
+    assert np.array_equal(p.sector_configs, want)
+    assert np.array_equal(want, QEDOracle(1, True, [2, 2], [False, False]).configs)
+    s = SU2_Model(spin=0.5, pure_theory=False, sectors=[4], lvals=[4], has_obc=[True])
+    s.get_abelian_symmetry_sector(global_ops=[s.ops["N_tot"]], global_sectors=[4])
+    cfg = s.sector_configs
+    nt = np.real(np.einsum("sii->si", s.ops["N_tot"]))
+    assert np.all(np.rint(nt[np.arange(4)[None, :], cfg].sum(axis=1)) == 4)
+    full = np.array(np.meshgrid(*[np.arange(d) for d in s.loc_dims], indexing="ij")).reshape(4, -1).T
+    assert cfg.shape[0] == int(np.sum(np.rint(nt[np.arange(4)[None, :], full].sum(axis=1)) == 4))
