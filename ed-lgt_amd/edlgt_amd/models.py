@@ -840,6 +840,116 @@ class QuantumModel:
             logger.info(f"FIDELITY: {fid}")
         return fid
 
+    # -- ensemble averages over a fully diagonalised sector (quantum_model.py:904-1156)
+    def _density_values(self, rho, local_obs_list, special_norms, staggered_avgs, prefix):
+        """The reference's per-observable reductions (mean over sites,
+        staggered mean, or dot with a normalisation vector / L) of the
+        diagonal local observables in the density rho(r) (real device tensor),
+        from one site-marginal pass over sqrt(rho).  Both reductions are linear
+        in the per-site values, so a weighted sum of eigenstate densities gives
+        the weighted sum of the eigenstates' values."""
+        import torch
+        marg = self._marginals(torch.sqrt(torch.clamp(rho, min=0.0)).contiguous())
+        out = {}
+        for obs in local_obs_list:
+            v = self.local_expvals(marg, obs)
+            if special_norms.get(obs) is not None:
+                val = float(np.dot(v, special_norms[obs]) / self.n_sites)
+            elif staggered_avgs.get(obs) is not None:
+                self.res[obs] = v
+                val = self.stag_avg(obs, staggered_avgs[obs])
+            else:
+                val = float(np.mean(v))
+            out[f"{prefix}_{obs}"] = val
+        return out
+
+    def _eigvec_matrix(self):
+        import torch
+        return torch.stack([s.psi_device.to(torch.complex128) for s in self.H.Npsi])
+
+    def _row_density(self, M, w=None):
+        """sum_r w_r |S m_r|^2 over the rows m_r of M (device), S the map to the
+        sector basis (the identity, or B for a momentum basis)."""
+        import torch
+        if self._sector_state(M[0]) is M[0]:
+            a2 = M.abs() ** 2
+            return a2.sum(dim=0) if w is None else (w[:, None] * a2).sum(dim=0)
+        rho = None
+        for r in range(M.shape[0]):
+            t = self._sector_state(M[r].contiguous()).abs() ** 2
+            t = t if w is None else w[r] * t
+            rho = t if rho is None else rho + t
+        return rho
+
+    def _state_tensor(self, state):
+        import torch
+        if hasattr(state, "data_ptr"):
+            st = state.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
+        else:
+            st = torch.as_tensor(np.asarray(state, dtype=np.complex128).reshape(-1), device="cuda")
+        if st.numel() != self.H.shape[0]:
+            raise ValueError(f"len(state) must be {self.H.shape[0]} not {st.numel()}")
+        return st
+
+    def microcanonical_avg(self, local_obs_list, state, special_norms=None, staggered_avgs=None):
+        """Microcanonical averages of diagonal local observables
+        (quantum_model.py:942-1045): the shell |e_i - <H>/L| < sqrt(<H^2> -
+        <H>^2)/L of the per-site eigenvalues around the state, equal weights.
+        Returns (coherent shell state (host), {"ME_<obs>": value}); <H> and
+        <H^2> = |H psi|^2 from one device product."""
+        import torch
+        special_norms = special_norms or {}
+        staggered_avgs = staggered_avgs or {}
+        st = self._state_tensor(state)
+        Hs = self.H.Ham.matvec(st) if self.H.Ham.is_complex else torch.complex(
+            self.H.Ham.matvec(st.real.contiguous()), self.H.Ham.matvec(st.imag.contiguous()))
+        Eq = float(torch.vdot(st, Hs).real.item())
+        E2q = float(torch.vdot(Hs, Hs).real.item())
+        e_q, d_e = Eq / self.n_sites, np.sqrt(max(E2q - Eq ** 2, 0.0)) / self.n_sites
+        shell = np.nonzero(np.abs(np.asarray(self.H.Nenergies) - e_q) < d_e)[0]
+        if shell.size == 0:
+            raise ValueError(f"ME shell empty: no eigstate |e_i-e_q|<delta_e={d_e}.")
+        V = self._eigvec_matrix()[torch.as_tensor(shell, device="cuda")]
+        psi_th = (V.sum(dim=0) / np.sqrt(shell.size)).cpu().numpy()
+        rho = self._row_density(V) / shell.size
+        return psi_th, self._density_values(rho, local_obs_list, special_norms, staggered_avgs, "ME")
+
+    def diagonal_avg(self, local_obs_list, state, special_norms=None, staggered_avgs=None, tol_deg=1e-10):
+        """Diagonal-ensemble averages (quantum_model.py:1047-1156): weights
+        |<E_i|psi>|^2, or, where eigenvalues are degenerate within tol_deg, the
+        projections Psi_b of the state on each degenerate block (sum_b
+        <Psi_b|O|Psi_b>).  Needs every eigenpair."""
+        import torch
+        if self.n_eigs != self.H.shape[0]:
+            raise ValueError(f"Need all H eigvals {self.H.shape[0]}, not only {self.n_eigs}")
+        special_norms = special_norms or {}
+        staggered_avgs = staggered_avgs or {}
+        st = self._state_tensor(state)
+        E = self._eigvec_matrix()
+        coef = torch.mv(E.conj(), st)
+        ev = np.asarray(self.H.Nenergies)
+        block = np.concatenate([[0], np.cumsum(np.abs(np.diff(ev)) >= tol_deg)]).astype(np.int64)
+        n_b = int(block[-1]) + 1
+        P = torch.zeros((n_b, E.shape[1]), dtype=torch.complex128, device="cuda")
+        P.index_add_(0, torch.as_tensor(block, device="cuda"), coef[:, None] * E)
+        return self._density_values(self._row_density(P), local_obs_list, special_norms, staggered_avgs, "DE")
+
+    def canonical_avg(self, local_obs, beta):
+        """Tr(O e^{-beta H}) / (Z L) with O the sum over sites of a diagonal local
+        operator (quantum_model.py:912-940), from the full spectrum: weights
+        e^{-beta (E_i - E_0)} (the shift cancels in the ratio).  The reference
+        forms e^{-beta H} densely; here every eigenpair must have been computed
+        (diagonalize_Hamiltonian("full", ...))."""
+        import torch
+        if self.n_eigs != self.H.shape[0]:
+            raise ValueError("canonical_avg needs every eigenpair: diagonalize_Hamiltonian('full', ...)")
+        E = self._eigvec_matrix()
+        en = np.asarray(self.H.energies, dtype=float)
+        w = np.exp(-beta * (en - en.min()))
+        w_t = torch.as_tensor(w / w.sum(), device="cuda")
+        vals = self._density_values(self._row_density(E, w_t), [local_obs], {}, {}, "C")
+        return vals[f"C_{local_obs}"]
+
     def get_qmb_state_from_configs(self, configs):
         """Equal-weight normalised superposition of basis configurations
         (quantum_model.py:826-862); raises ValueError for a config outside
