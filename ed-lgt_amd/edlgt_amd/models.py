@@ -257,6 +257,96 @@ class QMB_hamiltonian:
         self.Ham = csr
         self.Ham_type = fmt
         self.nnz = csr.nnz
+        self._all_pairs = None
+
+    # -- spectral sums (qmb_hamiltonian.py:428-601)
+    def full_eigenpairs(self):
+        """(energies, eigenvector rows (N, N) on the device) of every level:
+        the stored ones after diagonalize("full"), else one device solve over
+        the whole sector (the full-basis Lanczos path, <= 4096 states), kept
+        until the matrix changes.  The reference forms exp(-beta H) densely
+        where the spectrum is not stored; every trace it takes is a sum over
+        these levels."""
+        import torch
+        n = self.shape[0]
+        if getattr(self, "n_eigs", 0) == n:
+            return np.asarray(self.energies, dtype=float), torch.stack(
+                [s.psi_device.to(torch.complex128) for s in self.Npsi])
+        if getattr(self, "_all_pairs", None) is None:
+            if n > 4096:
+                raise NotImplementedError("every level is solved on the device for sectors of <= 4096 states")
+            from .device import random_uniform
+            v0 = random_uniform(n, complex_values=self.Ham.is_complex, seed=0)
+            evals, evecs, _, _ = self.Ham.lanczos(n, tol=1e-12, v0=v0)
+            self._all_pairs = (np.asarray(evals, dtype=float), evecs.to(torch.complex128))
+        return self._all_pairs
+
+    def _levels(self):
+        """The levels the reference's sums run over: the per-site energies when
+        every eigenpair is stored (its full-spectrum branch, qmb_hamiltonian.py:
+        446-447), otherwise the eigenvalues of H (its expm branch, :448-449:
+        Tr f(H) = sum_i f(E_i))."""
+        if getattr(self, "n_eigs", 0) == self.shape[0]:
+            return np.asarray(self.Nenergies, dtype=float)
+        return self.full_eigenpairs()[0]
+
+    def partition_function(self, beta):
+        """Z = sum_i exp(-beta e_i) (qmb_hamiltonian.py:428-452); ValueError
+        unless Z > 0."""
+        Z = float(np.sum(np.exp(-beta * self._levels())))
+        if Z <= 0:
+            raise ValueError(f"Z must be positive, not {Z}.")
+        return Z
+
+    def free_energy(self, beta):
+        """-log(Z) / beta (qmb_hamiltonian.py:454-469)."""
+        return -1 / beta * np.log(self.partition_function(beta))
+
+    def thermal_average(self, beta):
+        """sum_i e_i exp(-beta e_i) / Z (qmb_hamiltonian.py:471-489)."""
+        e = self._levels()
+        return float(e.dot(np.exp(-beta * e)) / self.partition_function(beta))
+
+    def F_prime(self, beta):
+        """-<e^2>_beta - <e>_beta^2, the slope the reference's Newton search
+        uses (qmb_hamiltonian.py:491-508; kept as written there)."""
+        e = self._levels()
+        e2 = np.square(e).dot(np.exp(-beta * e)) / self.partition_function(beta)
+        return float(-e2 - self.thermal_average(beta) ** 2)
+
+    def get_beta(self, state, threshold=1e-10, max_iter=1000):
+        """Newton search for beta with thermal_average(beta) = <state|H|state>
+        (qmb_hamiltonian.py:510-564): from beta = 1e-7, beta <- beta - F/F',
+        F = thermal_average(beta) - <H>, until the step is below threshold or
+        after max_iter steps (a warning).  As in the reference, <H> is the
+        total energy while the full-spectrum branch weighs per-site energies."""
+        import torch
+        st = state if hasattr(state, "data_ptr") else torch.as_tensor(
+            np.asarray(state, dtype=np.complex128).reshape(-1), device="cuda")
+        state_energy = QMB_state(st.to(torch.complex128), self.lvals, None).expectation_value(self.Ham)
+        beta, accuracy, it = 1e-7, 1.0, 0
+        while accuracy > threshold and it < max_iter:
+            it += 1
+            F = self.thermal_average(beta) - state_energy
+            Fp = self.F_prime(beta)
+            prev, beta = beta, beta - F / Fp
+            accuracy = abs(beta - prev)
+        if it >= max_iter:
+            logger.warning("Maximum iterations reached without convergence.")
+        logger.info(f"BETA {beta}")
+        return beta
+
+    def get_r_value(self):
+        """Adjacent-gap ratios r_i = min(d_i, d_i+1) / max(d_i, d_i+1) of the
+        sorted per-site spectrum; self.r_value = their mean over the middle
+        80 % (qmb_hamiltonian.py:566-601).  Needs every eigenpair."""
+        if getattr(self, "n_eigs", 0) != self.shape[0]:
+            raise ValueError("The Hamiltonian has to be fully diagonalized.")
+        d = np.diff(np.sort(np.asarray(self.Nenergies, dtype=float)))
+        r = np.minimum(d[:-1], d[1:]) / np.maximum(d[:-1], d[1:])
+        self.r_value = float(np.mean(r[int(0.1 * len(r)):int(0.9 * len(r))]))
+        logger.info(f"R value {self.r_value}")
+        return r
 
     def diagonalize(self, n_eigs, format, loc_dims, print_results=False, tol=1e-12,
                     max_basis=0, precision=L.PREC_FP64, v0=None, gauge=None):
@@ -936,19 +1026,31 @@ class QuantumModel:
 
     def canonical_avg(self, local_obs, beta):
         """Tr(O e^{-beta H}) / (Z L) with O the sum over sites of a diagonal local
-        operator (quantum_model.py:912-940), from the full spectrum: weights
-        e^{-beta (E_i - E_0)} (the shift cancels in the ratio).  The reference
-        forms e^{-beta H} densely; here every eigenpair must have been computed
-        (diagonalize_Hamiltonian("full", ...))."""
+        operator (quantum_model.py:912-940), as a sum over every level with
+        weights e^{-beta (E_i - E_0)} (the shift cancels in the ratio): the
+        stored eigenpairs after diagonalize_Hamiltonian("full", ...), else one
+        device solve of the whole sector (QMB_hamiltonian.full_eigenpairs) where
+        the reference forms e^{-beta H} densely."""
         import torch
-        if self.n_eigs != self.H.shape[0]:
-            raise ValueError("canonical_avg needs every eigenpair: diagonalize_Hamiltonian('full', ...)")
-        E = self._eigvec_matrix()
-        en = np.asarray(self.H.energies, dtype=float)
+        en, E = self.H.full_eigenpairs()
         w = np.exp(-beta * (en - en.min()))
         w_t = torch.as_tensor(w / w.sum(), device="cuda")
         vals = self._density_values(self._row_density(E, w_t), [local_obs], {}, {}, "C")
         return vals[f"C_{local_obs}"]
+
+    def compute_expH(self, beta):
+        """exp(-beta H) as a SciPy CSC matrix (quantum_model.py:904-906), from
+        every level: V^T diag(e^{-beta E}) conj(V) on the device."""
+        import scipy.sparse as sps
+        import torch
+        en, E = self.H.full_eigenpairs()
+        w = torch.as_tensor(np.exp(-beta * en), device="cuda", dtype=torch.complex128)
+        M = (E.transpose(0, 1) * w[None, :]).contiguous() @ E.conj().contiguous()
+        return sps.csc_matrix(M.cpu().numpy())
+
+    def get_thermal_beta(self, state, threshold):
+        """Effective inverse temperature of a state (quantum_model.py:908-910)."""
+        return self.H.get_beta(state, threshold)
 
     def get_qmb_state_from_configs(self, configs):
         """Equal-weight normalised superposition of basis configurations

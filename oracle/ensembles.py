@@ -10,7 +10,10 @@ with O the sum of the site operators).  Observables are diagonal local
 operators (per-site diagonals `diag`, shape (L, d)); `post` is the reference's
 per-observable reduction of the per-site values (mean, staggered mean, or dot
 with a normalisation vector / L).  Everything in loops over eigenstates, as
-the reference does it.
+the reference does it.  Beside them the spectral sums behind the reference's
+partition_function / thermal_average / F_prime / get_beta / get_r_value
+(edlgt/modeling/qmb_hamiltonian.py:428-601), with the dense exponential where
+the reference uses it.
 """
 import numpy as np
 
@@ -71,3 +74,42 @@ def canonical(energies, evecs, configs, diag, beta, n_sites):
     w = np.exp(-beta * (energies - energies.min()))
     Z = np.sum(w)
     return float(sum(w[i] * np.sum(site_values(evecs[:, i], configs, diag)) for i in range(len(w))) / (Z * n_sites))
+
+
+# -- spectral sums and the beta search (qmb_hamiltonian.py:428-601)
+def expm_sums(H, beta):
+    """(Z, <H>, <H^2>) from the dense exponential, as the reference's branch
+    without a stored spectrum computes them (qmb_hamiltonian.py:448-449,
+    :487-488, :505-506)."""
+    from scipy.linalg import expm
+    X = expm(-beta * H)
+    Z = np.real(np.trace(X))
+    return Z, np.real(np.trace(H @ X)) / Z, np.real(np.trace(H @ H @ X)) / Z
+
+
+def level_sums(levels, beta):
+    """(Z, <e>, <e^2>) as sums over levels (the full-spectrum branch, over the
+    per-site energies)."""
+    w = np.exp(-beta * levels)
+    Z = np.sum(w)
+    return Z, levels.dot(w) / Z, np.square(levels).dot(w) / Z
+
+
+def newton_beta(sums, target, threshold=1e-10, max_iter=1000):
+    """The reference's iteration (qmb_hamiltonian.py:531-562): beta from 1e-7,
+    beta <- beta - F / F', F = <e> - target, F' = -<e^2> - <e>^2."""
+    beta, acc, it = 1e-7, 1.0, 0
+    while acc > threshold and it < max_iter:
+        it += 1
+        _, E, E2 = sums(beta)
+        prev, beta = beta, beta - (E - target) / (-E2 - E ** 2)
+        acc = abs(beta - prev)
+    return beta, it
+
+
+def r_values(n_energies):
+    """Adjacent-gap ratios of the sorted spectrum and their mean over the
+    middle 80 % (qmb_hamiltonian.py:566-601), in a loop."""
+    d = np.diff(np.sort(n_energies))
+    r = np.array([min(d[i], d[i + 1]) / max(d[i], d[i + 1]) for i in range(len(d) - 1)])
+    return r, np.mean(r[int(0.1 * len(r)):int(0.9 * len(r))])

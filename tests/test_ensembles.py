@@ -65,3 +65,38 @@ def test_microcanonical_shell(chain):
     assert abs(me["ME_N_single"] - want) < 1e-12
     with pytest.raises(ValueError):
         ens.microcanonical(H, v, e / 6, np.zeros(len(e)), orc.configs, {"N_single": diag}, {"N_single": np.mean}, 6)
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.05, 0.4])
+def test_level_sums_are_the_exponential_traces(chain, beta):
+    """Tr f(H) = sum_i f(E_i): the sums over the total energies equal the
+    dense-exponential traces (the reference's other branch); at beta = 0,
+    Z = N and <H> = Tr(H) / N."""
+    orc, H, e, v, diag = chain
+    a = ens.expm_sums(H, beta)
+    b = ens.level_sums(e, beta)
+    for x, y in zip(a, b):
+        assert abs(x - y) <= 1e-11 * max(1.0, abs(x))
+    if beta == 0.0:
+        assert a[0] == pytest.approx(len(e), rel=1e-14) and a[1] == pytest.approx(np.trace(H) / len(e), rel=1e-13)
+
+
+def test_newton_beta_reaches_the_state_energy(chain):
+    """Where the search converges, <e>_beta is the target (F = 0 up to the stop
+    rule's |F| < threshold |F'|): over the
+    total energies for a configuration of energy 2 (beta > 0, below the
+    infinite-temperature mean 6) and of energy 10 (beta < 0)."""
+    orc, H, e, v, diag = chain
+    for target, sign in ((2.0, 1), (10.0, -1)):
+        beta, it = ens.newton_beta(lambda b: ens.level_sums(e, b), target)
+        assert it < 1000 and np.sign(beta) == sign
+        _, E, E2 = ens.level_sums(e, beta)
+        # the loop stops on a step below 1e-10, i.e. |F| < 1e-10 |F'|
+        assert abs(E - target) < 1e-10 * (E2 + E ** 2) * 1.01
+
+
+def test_r_values_closed_forms():
+    r, mean = ens.r_values(np.arange(12.0) ** 1)  # equal spacing: r = 1
+    assert np.all(r == 1.0) and mean == 1.0
+    r, _ = ens.r_values(np.array([3.0, 0.0, 1.0, 7.0]))  # gaps 1, 2, 4
+    assert np.allclose(r, [0.5, 0.5])

@@ -38,7 +38,7 @@ def setup(gpu):
     kw = dict(special_norms={"N_tot": norm}, staggered_avgs={"N_pair": "even"})
     rng = np.random.default_rng(7)
     psi = rng.normal(size=orc.N) + 1j * rng.normal(size=orc.N)
-    states = {"random": psi / np.linalg.norm(psi), "config": np.eye(orc.N)[0]}
+    states = {"random": psi / np.linalg.norm(psi), "config": np.eye(orc.N)[0], "config1": np.eye(orc.N)[1]}
     return orc, H, e, v, mod, diags, post, kw, states
 
 
@@ -98,5 +98,51 @@ def test_real_gauge_solve_and_partial_spectrum(setup):
     m2.diagonalize_Hamiltonian(4, "sparse", tol=1e-12)
     with pytest.raises(ValueError):
         m2.diagonal_avg(["N_single"], states["random"])
+    # the canonical average needs every level: one device solve of the sector
+    for k in diags:
+        assert abs(m2.canonical_avg(k, 0.3) - ens.canonical(e, v, orc.configs, diags[k], 0.3, 6)) < TOL, k
+
+
+def test_spectral_sums_beta_and_r(setup):
+    """partition_function / free_energy / thermal_average / F_prime / get_beta
+    (qmb_hamiltonian.py:428-564) in both of the reference's branches: over the
+    stored per-site energies after the full solve, and, after a partial solve,
+    over the eigenvalues of H (the oracle takes the dense exponential's
+    traces there); get_r_value; compute_expH against scipy's expm."""
+    from scipy.linalg import expm
+    from edlgt_amd import SU2_Model
+    orc, H, e, v, mod, diags, post, kw, states = setup
+    print("spectral sums", flush=True)
+    Hh = mod.H
+    for beta in (0.05, 0.4):
+        Z, E, E2 = ens.level_sums(e / 6, beta)
+        assert Hh.partition_function(beta) == pytest.approx(Z, rel=1e-10)
+        assert abs(Hh.thermal_average(beta) - E) < 1e-10
+        assert abs(Hh.F_prime(beta) - (-E2 - E ** 2)) < 1e-10 * (E2 + E ** 2)
+        assert abs(Hh.free_energy(beta) + np.log(Z) / beta) < 1e-9
+    cfg = states["config1"]
+    assert abs(np.vdot(cfg, H @ cfg).real - 2.0) < 1e-12
+    want, it = ens.newton_beta(lambda b: ens.level_sums(e / 6, b), 2.0)
+    assert it < 1000 and abs(mod.get_thermal_beta(cfg, 1e-10) - want) < 1e-8
+    r_o, mean_o = ens.r_values(e / 6)
+    r = Hh.get_r_value()
+    d = np.diff(np.sort(e / 6))
+    ok = np.maximum(d[:-1], d[1:]) > 1e-6
+    assert ok.sum() > 0.9 * ok.size
+    assert np.max(np.abs(r[ok] - r_o[ok])) < 1e-6 and abs(Hh.r_value - mean_o) < 1e-6
+    X = mod.compute_expH(0.3).toarray()
+    Xo = expm(-0.3 * H.toarray())
+    assert np.max(np.abs(X - Xo)) < 1e-12 * np.max(np.abs(Xo))
+    # partial solve: the reference's exponential branch, over the total energies
+    m2 = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    m2.build_Hamiltonian(1.0, 5.0)
+    m2.diagonalize_Hamiltonian(2, "sparse", tol=1e-12)
+    for beta in (0.05, 0.4):
+        Z, E, E2 = ens.expm_sums(H.toarray(), beta)
+        assert m2.H.partition_function(beta) == pytest.approx(Z, rel=1e-10)
+        assert abs(m2.H.thermal_average(beta) - E) < 1e-9
+        assert abs(m2.H.F_prime(beta) - (-E2 - E ** 2)) < 1e-10 * (E2 + E ** 2)
+    want, it = ens.newton_beta(lambda b: ens.expm_sums(H.toarray(), b), 2.0)
+    assert it < 1000 and abs(m2.get_thermal_beta(cfg, 1e-10) - want) < 1e-8
     with pytest.raises(ValueError):
-        m2.canonical_avg("N_single", 1.0)
+        m2.H.get_r_value()
