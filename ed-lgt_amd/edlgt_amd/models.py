@@ -230,6 +230,15 @@ def schmidt_spectrum(psi_device, part):
     return ev[:lay["n_vals"]]
 
 
+def get_entropy_partition(lvals, option="half"):
+    """Sites of the standard half bipartition (qmb_hamiltonian.py:635-664): the
+    first floor(Lx/2) sites of a chain, or of each of the first two rows."""
+    if option != "half":
+        raise NotImplementedError("for the moment it works only for option=half")
+    half = list(range(int(lvals[0] / 2)))
+    return half if len(lvals) == 1 else half + [lvals[0] + i for i in half]
+
+
 def diagonalize_density_matrix(rho):
     """(eigenvalues ascending, eigenvectors) of a Hermitian density matrix
     (qmb_state.py:526-546)."""
@@ -410,6 +419,8 @@ class QMB_hamiltonian:
         both gauges."""
         import torch
         from .device import KrylovSession
+        if self.Ham_type == "dense":
+            return self._exact_time_evolution(initial_state, time_line, loc_dims)
         time_line = np.asarray(time_line, dtype=float)
         if time_line.size < 2:
             raise ValueError("time_line needs at least two points")
@@ -450,6 +461,31 @@ class QMB_hamiltonian:
         self.marginals = marg
         self.psi_final = psi
         self.psi_time = [QMB_state(hist[k], self.lvals, loc_dims) for k in range(n_steps + 1)] if store_states else None
+
+    def _exact_time_evolution(self, initial_state, time_line, loc_dims):
+        """The reference's path for ham_format="dense" (qmb_hamiltonian.py:
+        363-376, exact_time_evolution :792-836): every eigenpair (solved
+        first where not stored), then psi(t) = sum_i e^{-i E_i t} <E_i|psi0> E_i
+        at each time of the grid (any spacing), as one product on the device."""
+        import torch
+        if getattr(self, "n_eigs", 0) != self.shape[0]:
+            self.diagonalize("full", "dense", loc_dims)
+        self.loc_dims = loc_dims
+        en, E = self.full_eigenpairs()
+        if hasattr(initial_state, "data_ptr"):
+            psi0 = initial_state.to(device="cuda", dtype=torch.complex128).reshape(-1)
+        else:
+            psi0 = torch.as_tensor(np.asarray(initial_state, dtype=np.complex128).reshape(-1), device="cuda")
+        if psi0.numel() != self.shape[0]:
+            raise ValueError("initial_state has the wrong length")
+        t = np.asarray(time_line, dtype=float).reshape(-1)
+        coef = torch.mv(E.conj(), psi0)
+        ph = torch.as_tensor(np.exp(-1j * np.outer(t, en)), device="cuda")
+        hist = (ph * coef[None, :]) @ E
+        self.krylov_err = np.zeros(max(t.size - 1, 1))
+        self.marginals = None
+        self.psi_final = hist[-1].clone()
+        self.psi_time = [QMB_state(hist[k], self.lvals, loc_dims) for k in range(t.size)]
 
     def release_krylov(self):
         """Frees the device state kept by time_evolution."""
@@ -662,6 +698,41 @@ class QuantumModel:
         """The same projector as SciPy CSC (quantum_model.py:615-624)."""
         return None if self.momentum_basis is None else self.momentum_basis.to_scipy().tocsc()
 
+    def _project_state_with_basis(self, state_realspace):
+        """B^H x: sector-basis coefficients to the momentum basis
+        (quantum_model.py:634-671), on the device; the identity (as complex128)
+        without a momentum sector.  Returns a host array."""
+        import torch
+        x = np.asarray(state_realspace.cpu().numpy() if hasattr(state_realspace, "data_ptr") else state_realspace)
+        if self.momentum_basis is None:
+            return x.astype(np.complex128, copy=False)
+        t = torch.as_tensor(np.ascontiguousarray(x, dtype=np.complex128).reshape(-1), device="cuda")
+        return self.momentum_basis.project(t).cpu().numpy()
+
+    def momentum_basis_projection(self, operator):
+        """B^H A B in the momentum basis (quantum_model.py:673-713).  "H": the
+        device Hamiltonian, when it was built in the sector basis before the
+        momentum sector was set, is projected on the device in place (built
+        after set_momentum_sector it already is B^H H B); an operator given as a
+        dense or SciPy matrix over the sector basis is projected with SciPy, as
+        the reference does, and returned as CSR."""
+        import scipy.sparse as sps
+        mb = self.momentum_basis
+        if mb is None:
+            raise ValueError("Basis projector B is not set.")
+        if isinstance(operator, str) and operator == "H":
+            H = self.H.Ham
+            if H.n == mb.n_rows and getattr(H, "momentum_basis", None) is None:
+                P = H.momentum(mb)
+                self.H.shape = (P.n, P.n)
+                self.H.set_matrix(P, self.H.Ham_type)
+            elif H.n != mb.n_cols:
+                raise ValueError(f"H has {H.n} rows, the momentum basis maps {mb.n_rows} -> {mb.n_cols}")
+            return None
+        A = sps.csr_matrix(operator)
+        B = self._basis_Pk_as_csc()
+        return (B.conj().T @ (A @ B)).tocsr()
+
     def _sector_state(self, psi_device):
         """A state over the sector basis (momentum states expanded through B)."""
         mb = self.momentum_basis
@@ -770,6 +841,8 @@ class QuantumModel:
             rg = self.real_gauge_matrix()
             if rg is not None:
                 gauge = (rg[0], self.basis, rg[1])
+        if self.H.Ham_type == "dense":
+            gauge = None  # exact evolution from the stored spectrum
         self.H.time_evolution(initial_state, time_line, self.loc_dims, basis=basis, d_max=d_max, gauge=gauge, **kw)
         self.H.real_gauge = gauge is not None
 

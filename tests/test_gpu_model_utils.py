@@ -1,0 +1,94 @@
+"""GPU: the remaining model utilities of the reference's API on the 6-site
+periodic SU(2) chain (282 states).
+  * ham_format="dense": time evolution from every eigenpair
+    (qmb_hamiltonian.py:363-376, exact_time_evolution :792-836) against the
+    oracle's expm_multiply at 1e-9 on a uniform grid and against the dense
+    exponential on a non-uniform one (which the Krylov path refuses);
+  * _project_state_with_basis (quantum_model.py:634-671): B^H x with the
+    oracle's momentum projector B at 1e-14;
+  * momentum_basis_projection (quantum_model.py:673-713): "H" built in the
+    sector basis, projected on the device in place, equal to the matrix built
+    after set_momentum_sector (same pattern, values at 1e-14) and to the
+    oracle's B^H H B; a host operator projected by SciPy equals B^H A B."""
+import numpy as np
+import pytest
+
+from oracle.model import SU2Oracle, expm_path
+from oracle.momentum import momentum_basis
+
+pytestmark = pytest.mark.gpu
+
+
+def _chain(**kw):
+    from edlgt_amd import SU2_Model
+    return SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False], **kw)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return SU2Oracle(False, [6], [False], sectors=[6])
+
+
+def test_dense_format_exact_evolution(gpu, orc):
+    from scipy.linalg import expm
+    print("dense-format exact evolution", flush=True)
+    mod = _chain(ham_format="dense")
+    mod.build_Hamiltonian(1.0, 5.0)
+    psi0 = mod.get_qmb_state_from_configs([mod.overlap_QMB_state("V")])
+    psi0 = psi0.cpu().numpy() if hasattr(psi0, "data_ptr") else np.asarray(psi0)
+    tl = np.arange(21) * 0.05
+    mod.time_evolution_Hamiltonian(psi0, tl)
+    assert mod.H.n_eigs == orc.N  # every eigenpair solved first, as the reference does
+    H = orc.hamiltonian(1.0, 5.0)
+    ref = expm_path(H, np.asarray(psi0, dtype=complex), tl[0], tl[-1], len(tl))
+    got = np.stack([s.psi for s in mod.H.psi_time])
+    assert np.max(np.abs(got - ref)) < 1e-9
+    # any spacing: against the dense exponential
+    tl2 = np.array([0.0, 0.013, 0.2, 0.71, 1.9])
+    mod.time_evolution_Hamiltonian(psi0, tl2)
+    Hd = H.toarray()
+    for k, t in enumerate(tl2):
+        assert np.max(np.abs(mod.H.psi_time[k].psi - expm(-1j * t * Hd) @ psi0)) < 1e-10
+    mod.get_observables(["N_single"])
+    mod.measure_observables(4, dynamics=True)
+    assert np.all(np.isfinite(mod.res["N_single"]))
+    # the sparse format keeps the Krylov path (uniform grids only)
+    sp = _chain()
+    sp.build_Hamiltonian(1.0, 5.0)
+    with pytest.raises(ValueError):
+        sp.time_evolution_Hamiltonian(psi0, tl2)
+
+
+def test_momentum_state_and_operator_projection(gpu, orc):
+    import torch
+    print("momentum projections", flush=True)
+    Bo = momentum_basis(orc.configs, orc.keys, orc.loc_dims, [6], [2], [1])
+    Ho = orc.hamiltonian(1.0, 5.0)
+    # H built in the sector basis, then projected in place
+    mod = _chain()
+    mod.build_Hamiltonian(1.0, 5.0)
+    x = np.random.default_rng(3).normal(size=orc.N) + 1j * np.random.default_rng(4).normal(size=orc.N)
+    assert np.array_equal(mod._project_state_with_basis(x), x.astype(np.complex128))
+    mod.set_momentum_sector([2], [1])
+    assert np.max(np.abs(mod._project_state_with_basis(x) - Bo.conj().T @ x)) < 1e-13
+    assert np.max(np.abs(mod._project_state_with_basis(torch.as_tensor(x)) - Bo.conj().T @ x)) < 1e-13
+    mod.momentum_basis_projection("H")
+    assert mod.H.shape == (Bo.shape[1], Bo.shape[1])
+    ref = _chain()
+    ref.set_momentum_sector([2], [1])
+    Hk = ref.build_Hamiltonian(1.0, 5.0)
+    a, b = mod.H.Ham.to_scipy(), Hk.to_scipy()
+    assert np.array_equal(a.indptr, b.indptr) and np.array_equal(a.indices, b.indices)
+    assert np.max(np.abs(a.data - b.data)) < 1e-14 * np.max(np.abs(b.data))
+    want = (Bo.conj().T @ Ho @ Bo).toarray()
+    assert np.max(np.abs(a.toarray() - want)) < 1e-13 * np.max(np.abs(want))
+    mod.momentum_basis_projection("H")  # already projected: unchanged
+    assert mod.H.shape == (Bo.shape[1], Bo.shape[1])
+    mod.diagonalize_Hamiltonian(2, "sparse")
+    assert np.max(np.abs(mod.H.energies - np.linalg.eigvalsh(want)[:2])) < 1e-10
+    # a host operator over the sector basis
+    A = np.diag(np.real(np.einsum("sii->si", orc.ops["N_single"]))[np.arange(6)[None, :], orc.configs].sum(axis=1))
+    got = mod.momentum_basis_projection(A).toarray()
+    assert np.max(np.abs(got - (Bo.conj().T @ A @ Bo))) < 1e-13
+    with pytest.raises(ValueError):
+        _chain().momentum_basis_projection("H")

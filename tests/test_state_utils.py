@@ -67,3 +67,14 @@ def test_participation_renyi_entropy_rejects_alpha():
     for a in (0, 1, -2):
         with pytest.raises(ValueError):
             _state([1, 0]).participation_renyi_entropy(a)
+
+
+def test_entropy_partition_half():
+    """get_entropy_partition (qmb_hamiltonian.py:635-664): the first half of a
+    chain, or the first half of each of the first two rows of a 2D lattice."""
+    from edlgt_amd.models import get_entropy_partition
+    assert get_entropy_partition([6]) == [0, 1, 2]
+    assert get_entropy_partition([5]) == [0, 1]
+    assert get_entropy_partition([4, 3]) == [0, 1, 4, 5]
+    with pytest.raises(NotImplementedError):
+        get_entropy_partition([4], option="quarter")
