@@ -268,6 +268,23 @@ class QMB_hamiltonian:
         self.nnz = csr.nnz
         self._all_pairs = None
 
+    def convert_hamiltonian(self, format):
+        """qmb_hamiltonian.py:227-258.  The device matrix serves every format
+        (all products run on the device); the label selects the time-evolution
+        path as in the reference ("dense": exact, from every eigenpair)."""
+        if format not in ("dense", "sparse", "linear"):
+            raise ValueError(f"Invalid format: {format}. Use 'dense', 'sparse', or 'linear'.")
+        logger.info(f"CONVERT HAMILTONIAN from {self.Ham_type} to {format}")
+        self.Ham_type = format
+
+    def get_sparsity(self):
+        """Stored entries over N^2 (qmb_hamiltonian.py:614-619 counts the
+        triplets before duplicates are summed; the device matrix keeps the
+        summed ones)."""
+        sparsity = self.nnz / (self.shape[0] ** 2)
+        logger.info(f"SPARSITY: {round(sparsity, 16)}")
+        return sparsity
+
     # -- spectral sums (qmb_hamiltonian.py:428-601)
     def full_eigenpairs(self):
         """(energies, eigenvector rows (N, N) on the device) of every level:

@@ -52,11 +52,18 @@ def test_dense_format_exact_evolution(gpu, orc):
     mod.get_observables(["N_single"])
     mod.measure_observables(4, dynamics=True)
     assert np.all(np.isfinite(mod.res["N_single"]))
-    # the sparse format keeps the Krylov path (uniform grids only)
+    # the sparse format keeps the Krylov path (uniform grids only) ...
     sp = _chain()
     sp.build_Hamiltonian(1.0, 5.0)
     with pytest.raises(ValueError):
         sp.time_evolution_Hamiltonian(psi0, tl2)
+    assert sp.H.get_sparsity() == H.nnz / orc.N ** 2
+    # ... until converted (qmb_hamiltonian.py:227-258)
+    sp.H.convert_hamiltonian("dense")
+    sp.time_evolution_Hamiltonian(psi0, tl2)
+    assert np.max(np.abs(sp.H.psi_time[-1].psi - expm(-1j * tl2[-1] * Hd) @ psi0)) < 1e-10
+    with pytest.raises(ValueError):
+        sp.H.convert_hamiltonian("csr")
 
 
 def test_momentum_state_and_operator_projection(gpu, orc):
