@@ -1142,6 +1142,43 @@ class QuantumModel:
         """Effective inverse temperature of a state (quantum_model.py:908-910)."""
         return self.H.get_beta(state, threshold)
 
+    # -- configuration printout (QED_model.py:901-946, SU2_model.py:831-871)
+    _pad_state_values = True  # QED pads each local value to the column width
+
+    def _local_state_labels(self):
+        """Columns of a local basis state: bg (with background charges), M
+        (with matter), then the links -x.. and +x.. of the lattice's axes."""
+        labels = ["bg"] if getattr(self, "background", 0) > 0 else []
+        if not getattr(self, "pure_theory", True):
+            labels.append("M")
+        dirs = "xyz"[: int(self.dim)]
+        return labels + [f"-{d}" for d in dirs] + [f"+{d}" for d in dirs]
+
+    def _format_header(self, entry_width):
+        return "[" + " ".join(f"{lab:>{entry_width}s}" for lab in self._local_state_labels()) + " ]"
+
+    def print_state_config(self, config, amplitude=None):
+        """Logs a configuration site by site: the local state index and its
+        gauge-invariant basis state (matter, link values)."""
+        if getattr(self, "spin", None) == "integrated":
+            raise NotImplementedError("It does not work with spin='integrated'")
+        logger.info("----------------------------------------------------")
+        msg = f"CONFIG {config}"
+        if amplitude is not None:
+            msg += f" |psi|^2={np.abs(amplitude) ** 2:.8f}"
+        logger.info(msg)
+        logger.info("----------------------------------------------------")
+        max_abs = max(1, int(abs(getattr(self, "spin", 1))), int(abs(getattr(self, "background", 0))))
+        width = len(str(max_abs)) + 2
+        logger.info(f"{'':>19s}{self._format_header(width)}")
+        for ii, idx in enumerate(config):
+            st = self.gauge_states_per_site[ii][int(idx)]
+            if self._pad_state_values:
+                body = " ".join(f"{int(round(v)):>{width}d}" for v in st)
+            else:
+                body = " ".join(f"{v}" for v in st)
+            logger.info(f"SITE {ii:>2d} state {int(idx):>3d}: [{body} ]")
+
     def get_qmb_state_from_configs(self, configs):
         """Equal-weight normalised superposition of basis configurations
         (quantum_model.py:826-862); raises ValueError for a config outside
@@ -1735,6 +1772,7 @@ class SU2_Model(QuantumModel):
     T2_p{d}(i) = T2_m{d}(i+d), and for the pure theory on periodic lattices
     the parity of the flux through the origin lines (2D) or faces (3D) fixed
     to +1."""
+    _pad_state_values = False  # SU(2) prints the local values as they are
 
     def __init__(self, spin, pure_theory, bg_list=None, sectors=None, use_generic_model=False, **kwargs):
         super().__init__(**kwargs)

@@ -14,7 +14,8 @@ Pinned here against the definition, with the oracle's complex H:
 The same holds for SU(2) with matter (hopping -2 sqrt(2) i along x, real
 along y and z: SU2_model.py:555-624) with a_j = x_j mod 2 and N_tot.
 The engine's model is built on the host with the sector taken from the oracle
-(the device enumeration is pinned bit-exactly elsewhere)."""
+(the device enumeration is pinned bit-exactly elsewhere); the configuration
+printout, which reads the same host-side model, is checked here too."""
 import numpy as np
 import pytest
 
@@ -184,3 +185,23 @@ def test_su2_engine_program_is_the_gauged_hamiltonian(monkeypatch, lvals, obc, s
     assert np.array_equal(G.indptr, want.indptr) and np.array_equal(G.indices, want.indices)
     assert np.max(np.abs(G.data.imag)) == 0.0
     assert np.max(np.abs(G.data - want.data)) <= 1e-14 * np.max(np.abs(H.data))
+
+
+def test_print_state_config(monkeypatch, caplog):
+    """print_state_config (QED_model.py:901-946): one line per site with the
+    local index and its gauge-invariant state (matter, -x -y, +x +y links),
+    the rows of the oracle's local basis for the site's label."""
+    import logging
+    orc = QEDOracle(1, False, [2, 2], [False, False])
+    mod = _engine_model(monkeypatch, orc, [2, 2], [False, False])
+    cfg = orc.configs[5]
+    with caplog.at_level(logging.INFO, logger=models.logger.name):
+        mod.print_state_config(cfg, amplitude=0.5)
+    lines = [r.getMessage() for r in caplog.records]
+    assert any("|psi|^2=0.25000000" in l for l in lines)
+    assert any(l.strip().startswith("[") and l.split() == ["[", "M", "-x", "-y", "+x", "+y", "]"] for l in lines)
+    rows = [l for l in lines if l.startswith("SITE")]
+    assert len(rows) == 4
+    for i, l in enumerate(rows):
+        vals = [int(v) for v in l.split("[")[1].rstrip(" ]").split()]
+        assert vals == [int(round(v)) for v in orc.gauge_states[orc.labels[i]][cfg[i]]]
