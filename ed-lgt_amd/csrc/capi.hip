@@ -349,6 +349,11 @@ int edlgt_csr_device_ptrs(const edlgt_csr* h, void** row_ptr, void** col, void**
   return EDLGT_OK;
 }
 
+int edlgt_csr_dense_eigh(const edlgt_csr* h, double* evals, void* evecs, void* stream) {
+  EDLGT_REQUIRE(h, "null matrix");
+  return csr_dense_eigh(*h, evals, evecs, (hipStream_t)stream);
+}
+
 int edlgt_csr_make_low_precision(edlgt_csr* h, void* stream) {
   EDLGT_REQUIRE(h, "null matrix");
   return csr_make_low_precision(*h, (hipStream_t)stream);

@@ -48,6 +48,11 @@ int csr_spmm(const Csr& h, int precision, int xtype, int n_rhs, const void* x, i
 size_t spmm_pack_bytes(const Csr& h, int xtype);
 bool spmm_packable(const Csr& h, int xtype);
 int csr_make_low_precision(Csr& h, hipStream_t st);
+// Every eigenpair of a stored single-term matrix (dense.hip): evals (host,
+// ascending), evecs (device, N*N values of the matrix's type, row k =
+// eigenvector k); at most kDenseEighMaxN states.
+constexpr int64_t kDenseEighMaxN = 32768;
+int csr_dense_eigh(const Csr& h, double* evals, void* evecs, hipStream_t st);
 // Fused multi-coefficient SpMM (see spmv.hip): xs/ys device pointer arrays,
 // coef device [nb][16]; nb in {1, 2, 4, 8, 16}.
 constexpr int kCoefMaxRhs = 16;

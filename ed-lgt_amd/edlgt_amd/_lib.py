@@ -62,6 +62,7 @@ SIGNATURES = {
     "edlgt_csr_info": (_i32, [_p, _p, _p, _p]),
     "edlgt_csr_copy": (_i32, [_p, _p, _p, _p]),
     "edlgt_csr_device_ptrs": (_i32, [_p, _p, _p, _p]),
+    "edlgt_csr_dense_eigh": (_i32, [_p, _p, _p, _p]),
     "edlgt_csr_make_low_precision": (_i32, [_p, _p]),
     "edlgt_csr_destroy": (_i32, [_p]),
     "edlgt_spmv": (_i32, [_p, _i32, _p, _p, _i32, _p]),

@@ -381,6 +381,19 @@ class CSRMatrix:
         L.check(L.load().edlgt_csr_copy(self._h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
         return rp, col, val
 
+    def dense_eigh(self, stream=None):
+        """Every eigenpair from the dense matrix on the device
+        (edlgt_csr_dense_eigh, rocSOLVER): (evals ascending, evecs tensor
+        (N, N), row k = eigenvector k), for up to 32768 states."""
+        torch = _torch()
+        n = self.n
+        if n > 32768:
+            raise NotImplementedError("the dense eigensolve takes sectors of at most 32768 states")
+        evals = np.zeros(n)
+        evecs = torch.empty((n, n), dtype=torch.complex128 if self.is_complex else torch.float64, device="cuda")
+        L.check(L.load().edlgt_csr_dense_eigh(self._h, evals.ctypes.data, L.ptr(evecs), L.stream_handle(stream)))
+        return evals, evecs
+
     def compress(self, stream=None):
         """Dictionary-codes the values (edlgt_csr_compress); returns the number
         of distinct (value, term) pairs, 0 when the matrix stays uncoded."""

@@ -289,8 +289,8 @@ class QMB_hamiltonian:
     def full_eigenpairs(self):
         """(energies, eigenvector rows (N, N) on the device) of every level:
         the stored ones after diagonalize("full"), else one device solve over
-        the whole sector (the full-basis Lanczos path, <= 4096 states), kept
-        until the matrix changes.  The reference forms exp(-beta H) densely
+        the whole sector (the full-basis Lanczos path up to 4096 states, the
+        dense rocSOLVER factorization above), kept until the matrix changes.  The reference forms exp(-beta H) densely
         where the spectrum is not stored; every trace it takes is a sum over
         these levels."""
         import torch
@@ -300,10 +300,11 @@ class QMB_hamiltonian:
                 [s.psi_device.to(torch.complex128) for s in self.Npsi])
         if getattr(self, "_all_pairs", None) is None:
             if n > 4096:
-                raise NotImplementedError("every level is solved on the device for sectors of <= 4096 states")
-            from .device import random_uniform
-            v0 = random_uniform(n, complex_values=self.Ham.is_complex, seed=0)
-            evals, evecs, _, _ = self.Ham.lanczos(n, tol=1e-12, v0=v0)
+                evals, evecs = self.Ham.dense_eigh()
+            else:
+                from .device import random_uniform
+                v0 = random_uniform(n, complex_values=self.Ham.is_complex, seed=0)
+                evals, evecs, _, _ = self.Ham.lanczos(n, tol=1e-12, v0=v0)
             self._all_pairs = (np.asarray(evals, dtype=float), evecs.to(torch.complex128))
         return self._all_pairs
 
@@ -383,11 +384,13 @@ class QMB_hamiltonian:
         if isinstance(n_eigs, str):
             if n_eigs != "full":
                 raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")
-            # every eigenpair: the device solver's full-basis path (the
-            # Krylov space is the whole sector, then one host factorization
-            # of the N x N projected matrix), sectors of up to 4096 states
+            # every eigenpair: up to 4096 states the device solver's
+            # full-basis path (the Krylov space is the whole sector, then one
+            # host factorization of the N x N projected matrix); above, the
+            # dense matrix factored on the device (rocSOLVER), as the
+            # reference factors its dense array (qmb_hamiltonian.py:293-301)
             if self.shape[0] > 4096:
-                raise NotImplementedError("n_eigs='full' is solved on the device for sectors of <= 4096 states")
+                return self._diagonalize_dense(loc_dims, print_results, gauge)
             n_eigs = int(self.shape[0])
         if not isinstance(n_eigs, (int, np.integer)) or n_eigs < 1:
             raise ValueError(f"n_eigs must be int or 'full', not {n_eigs}.")
@@ -411,17 +414,9 @@ class QMB_hamiltonian:
                                              precision=precision, v0=v0)
         if gauge is not None:
             evecs = gauge[1].quarter_phase(gauge[2], evecs)
-        self.energies = evals
         self.residuals = resid
         self.matvecs = mvs
-        nsites = prod(self.lvals)
-        self.Nenergies = evals / nsites
-        self.Npsi = [QMB_state(evecs[i], self.lvals, loc_dims) for i in range(self.n_eigs)]
-        self.GSenergy = self.Nenergies[0]
-        self.GSpsi = self.Npsi[0]
-        if print_results:
-            for e in evals:
-                logger.info(f"{e:.10f}  {e / nsites:.10f}")
+        self._store_eigenpairs(evals, evecs, loc_dims, print_results)
 
     def time_evolution(self, initial_state, time_line, loc_dims, basis=None, d_max=None, m=30,
                        reorth=True, precision=L.PREC_FP64, store_states=True, gauge=None):
@@ -478,6 +473,31 @@ class QMB_hamiltonian:
         self.marginals = marg
         self.psi_final = psi
         self.psi_time = [QMB_state(hist[k], self.lvals, loc_dims) for k in range(n_steps + 1)] if store_states else None
+
+    def _store_eigenpairs(self, evals, evecs, loc_dims, print_results):
+        self.energies = evals
+        nsites = prod(self.lvals)
+        self.Nenergies = evals / nsites
+        self.Npsi = [QMB_state(evecs[i], self.lvals, loc_dims) for i in range(self.n_eigs)]
+        self.GSenergy = self.Nenergies[0]
+        self.GSpsi = self.Npsi[0]
+        if print_results:
+            for e in evals:
+                logger.info(f"{e:.10f}  {e / nsites:.10f}")
+
+    def _diagonalize_dense(self, loc_dims, print_results, gauge):
+        """n_eigs="full" above the Lanczos full-basis limit: the stored matrix
+        (or the real-gauge one, eigenvectors mapped back) factored densely on
+        the device."""
+        self.loc_dims = loc_dims
+        H = self.Ham if gauge is None else gauge[0]
+        evals, evecs = H.dense_eigh()
+        if gauge is not None:
+            evecs = gauge[1].quarter_phase(gauge[2], evecs)
+        self.n_eigs = int(self.shape[0])
+        self.residuals = np.zeros(self.n_eigs)
+        self.matvecs = np.zeros(2, dtype=np.int64)
+        self._store_eigenpairs(evals, evecs, loc_dims, print_results)
 
     def _exact_time_evolution(self, initial_state, time_line, loc_dims):
         """The reference's path for ham_format="dense" (qmb_hamiltonian.py:

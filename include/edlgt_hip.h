@@ -222,6 +222,14 @@ int edlgt_csr_copy(const edlgt_csr* h, int64_t* row_ptr, int32_t* col, void* val
 int edlgt_csr_device_ptrs(const edlgt_csr* h, void** row_ptr, void** col, void** val);
 /* Build an FP32 (complex64 / float32) copy of the values for EDLGT_PREC_MIXED. */
 int edlgt_csr_make_low_precision(edlgt_csr* h, void* stream);
+/* Every eigenpair of a stored single-term matrix of up to 32768 states, the
+ * dense route of QMB_hamiltonian.diagonalize(n_eigs="full")
+ * (edlgt/modeling/qmb_hamiltonian.py:293-301: the dense array factored by
+ * scipy.linalg.eigh), here on the device (rocSOLVER zheevd / dsyevd).
+ * evals: host double [n] ascending; evecs: device [n * n] of the matrix's
+ * value type (complex128 or float64), row k = eigenvector k.  A
+ * matrix-free, value-released or multi-term matrix is EDLGT_ERR_INVALID_ARG. */
+int edlgt_csr_dense_eigh(const edlgt_csr* h, double* evals, void* evecs, void* stream);
 int edlgt_csr_destroy(edlgt_csr* h);
 
 /* ======================================================================
