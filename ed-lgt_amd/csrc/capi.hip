@@ -512,6 +512,14 @@ int edlgt_basis_signed_image(const edlgt_basis* b, const int32_t* site_perm, con
   return basis_signed_image(*b, site_perm, loc_perm, loc_sign, d_max, out_index, out_sign, (hipStream_t)stream);
 }
 
+int edlgt_basis_sre2(const edlgt_basis* b, const void* psi, double prob_threshold, double* sre2,
+                     int64_t* support_size, double* discarded_weight, int64_t* n_strings, void* stream) {
+  EDLGT_REQUIRE(b && psi, "null argument");
+  EDLGT_REQUIRE(sre2 && support_size && discarded_weight && n_strings, "null output");
+  return basis_sre2(*b, psi, prob_threshold, sre2, support_size, discarded_weight, n_strings,
+                    (hipStream_t)stream);
+}
+
 int edlgt_basis_quarter_phase(const edlgt_basis* b, const int32_t* site_exp, int32_t d_max, int32_t n_vec,
                               const double* x, void* y, void* stream) {
   EDLGT_REQUIRE(b && site_exp, "null argument");

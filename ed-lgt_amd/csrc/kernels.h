@@ -53,6 +53,12 @@ int csr_make_low_precision(Csr& h, hipStream_t st);
 // eigenvector k); at most kDenseEighMaxN states.
 constexpr int64_t kDenseEighMaxN = 32768;
 int csr_dense_eigh(const Csr& h, double* evals, void* evecs, hipStream_t st);
+// Stabilizer Renyi-2 entropy of a complex128 sector state on the support
+// holding >= 1 - delta of its weight (magic.hip); at most kSreMaxSupport
+// support states.
+constexpr int64_t kSreMaxSupport = 16383;
+int basis_sre2(const Basis& B, const void* psi, double delta, double* sre2, int64_t* support, double* discarded,
+               int64_t* n_strings, hipStream_t st);
 // Fused multi-coefficient SpMM (see spmv.hip): xs/ys device pointer arrays,
 // coef device [nb][16]; nb in {1, 2, 4, 8, 16}.
 constexpr int kCoefMaxRhs = 16;

@@ -6,10 +6,23 @@ CSRMatrix    -- canonical CSR Hamiltonian on the GPU, with SpMV/SpMM and Lanczos
 Server       -- continuous-batching lowest-eigenvalue solver on a multi-term matrix
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import _lib as L
+
+
+# id of a configs() table -> its basis, so that API calls given the model's
+# sector_configs (as the reference's are) can reach the device basis
+_CONFIG_OWNER = {}
+
+
+def basis_of_configs(configs):
+    """The SectorBasis whose configs() returned this very array, or None."""
+    ref = _CONFIG_OWNER.get(id(configs))
+    b = ref() if ref is not None else None
+    return b if b is not None and b._configs is configs else None
 
 
 def _torch():
@@ -111,7 +124,23 @@ class SectorBasis:
             out = np.empty((self.n_states, self.n_sites), dtype=np.uint16)
             L.check(L.load().edlgt_basis_copy_configs(self._h, out.ctypes.data))
             self._configs = out
+            _CONFIG_OWNER[id(out)] = weakref.ref(self)
         return self._configs
+
+    def sre2(self, psi, prob_threshold=1e-2, stream=None):
+        """Stabilizer Renyi-2 entropy of a sector state on the support holding
+        >= 1 - prob_threshold of its weight (edlgt_basis_sre2): (SRE2,
+        support size, discarded weight, number of X-strings)."""
+        torch = _torch()
+        t = psi if hasattr(psi, "data_ptr") else torch.as_tensor(np.asarray(psi, dtype=np.complex128))
+        t = t.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
+        if t.numel() != self.n_states:
+            raise ValueError(f"the state has {t.numel()} entries, the sector {self.n_states}")
+        out, disc = np.zeros(1), np.zeros(1)
+        k, ns = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
+        L.check(L.load().edlgt_basis_sre2(self._h, L.ptr(t), float(prob_threshold), out.ctypes.data, k.ctypes.data,
+                                          disc.ctypes.data, ns.ctypes.data, L.stream_handle(stream)))
+        return float(out[0]), int(k[0]), float(disc[0]), int(ns[0])
 
     def keys(self):
         """Keys as uint64 (n,) or, for 128-bit keys, Python ints."""

@@ -102,6 +102,19 @@ class QMB_state:
         ipr = float(torch.sum(p ** float(alpha)).item())
         return float(np.log(ipr + 1e-16) / (1.0 - float(alpha)))
 
+    def stabilizer_renyi_entropy(self, sector_configs, prob_threshold=1e-2):
+        """Stabilizer Renyi-2 entropy on the support holding >= 1 -
+        prob_threshold of the weight (qmb_state.py:393-465), on the device.
+        sector_configs: the model's sector_configs (or its SectorBasis), whose
+        device basis the state is expanded in."""
+        from .device import SectorBasis, basis_of_configs
+        basis = sector_configs if isinstance(sector_configs, SectorBasis) else basis_of_configs(sector_configs)
+        if basis is None:
+            raise ValueError("sector_configs must be the model's sector_configs (the device basis of the state)")
+        sre, K, disc, n_str = basis.sre2(self.psi_device, prob_threshold)
+        logger.info(f"Support size = {K}, discarded weight = {disc:.3e}, {n_str} X-strings; SRE2 = {sre:.12f}")
+        return sre
+
     def entanglement_entropy(self, keep_indices, partitions_dict):
         """Von Neumann entropy (base 2, Schmidt weights > 1e-10) of the sites
         keep_indices (qmb_state.py:245-291), from the subsystem/environment

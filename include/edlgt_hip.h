@@ -371,6 +371,16 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
 int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
                                   double* out_dev, void* stream);
 
+/* Stabilizer Renyi-2 entropy of a complex128 state psi (device, n_states) on
+ * the support holding >= 1 - prob_threshold of its weight: replaces
+ * QMB_state.stabilizer_renyi_entropy (edlgt/modeling/qmb_state.py:393-465,
+ * with extract_support :708-790 and edlgt/tools/magic.py:283-649).  Outputs
+ * (host): the SRE2, the support size, the discarded weight and the number of
+ * X-strings.  prob_threshold outside (0, 1), a support above 16383 states or
+ * keys above 64 bits are EDLGT_ERR_INVALID_ARG. */
+int edlgt_basis_sre2(const edlgt_basis* b, const void* psi, double prob_threshold, double* sre2,
+                     int64_t* support_size, double* discarded_weight, int64_t* n_strings, void* stream);
+
 /* Signed image of every basis state under a site permutation with per-site
  * local maps: the image of configuration r puts loc_perm[j * d_max + c_r[j]]
  * on site site_perm[j], with sign prod_j loc_sign[j * d_max + c_r[j]];
