@@ -260,6 +260,29 @@ def diagonalize_density_matrix(rho):
     return np.linalg.eigh(np.asarray(rho))
 
 
+def get_projector_for_efficient_density_matrix(rho_eigvals, rho_eigvecs, threshold):
+    """Columns of the eigenvectors whose eigenvalues exceed threshold, largest
+    first; the threshold divided by 10 until at least two pass
+    (qmb_state.py:549-592)."""
+    order = np.argsort(rho_eigvals)[::-1]
+    vals = np.asarray(rho_eigvals)[order]
+    vecs = np.asarray(rho_eigvecs)[:, order]
+    n_cols = int(np.sum(vals > threshold))
+    while n_cols < 2:
+        threshold /= 10
+        n_cols = int(np.sum(vals > threshold))
+    logger.info(f"SIGNIFICANT EIGENVALUES {n_cols} with threshold {threshold}")
+    return vecs[:, :n_cols].astype(complex)
+
+
+def mixed_exp_val_data(psi1, psi2, row_list, col_list, value_list):
+    """<psi1|O|psi2> of an operator given as triplets, duplicates summed
+    (qmb_state.py:657-679)."""
+    psi1, psi2 = np.asarray(psi1), np.asarray(psi2)
+    r, c = np.asarray(row_list), np.asarray(col_list)
+    return complex(np.sum(np.conj(psi1[r]) * np.asarray(value_list) * psi2[c]))
+
+
 class QMB_hamiltonian:
     """Hamiltonian container (edlgt/modeling/qmb_hamiltonian.py:24-426) whose
     matrix lives on the GPU as canonical CSR."""
