@@ -1982,6 +1982,57 @@ class SU2_Model(QuantumModel):
             self._add_nbody(["Vpx_dag", "Vmx"], lat.bonds("x"), -lambda_noise, True)
         return self._build_program(real)
 
+    def build_local_Hamiltonian(self, g, m, R0):
+        """The energy density around site R0 of the 1D chain with matter
+        (SU2_model.py:626-699), stored in self.Hlocal: the electric term and
+        the staggered mass (-1)^R0 m at R0, and half of each hopping on the
+        bonds (R0, R0+1) and (R0-1, R0) with its conjugate written out as the
+        reference writes it ([Qpx_dag, Qmx] times t/2, [Qpx, Qmx_dag] times
+        -t/2).  Summed over R0 it is the chain's Hamiltonian."""
+        from .device import CSRMatrix
+        if self.dim > 1:
+            raise ValueError(f"Local Hamiltonian valid only for D=1, got {self.dim}")
+        if self.pure_theory or self.use_generic_model:
+            raise ValueError("the local Hamiltonian is defined for the spin-1/2 chain with matter")
+        if self.momentum_basis is not None:
+            raise NotImplementedError("the local Hamiltonian is built in the sector basis (no momentum sector)")
+        if not 0 <= int(R0) < self.n_sites:
+            raise ValueError(f"R0 must be a site of the chain, got {R0}")
+        R0 = int(R0)
+        # the working program and couplings of the last build stay as they were
+        saved = (self._tables, self._table_ids, self._placements, dict(self.coeffs)) \
+            if hasattr(self, "_placements") and hasattr(self, "coeffs") else None
+        try:
+            self.SU2_Hamiltonian_couplings(g, m)
+            c = self.coeffs
+            self._new_program()
+            self._add_local("E_square", c["E"], [R0])
+            self._add_local("N-1", ((-1) ** R0) * c["m"], [R0])
+            half = 0.5 * c["tx_even"]
+            fwd = [b for b in self.lattice.bonds("x") if b[0] == R0]
+            bwd = [b for b in self.lattice.bonds("x") if b[0] == (R0 - 1) % self.n_sites]
+            for names, strength in ((["Qpx_dag", "Qmx"], half), (["Qpx", "Qmx_dag"], -half)):
+                self._add_nbody(names, fwd, strength, False)
+                self._add_nbody(names, bwd, strength, False)
+            csr = CSRMatrix(self.basis, np.stack(self._tables), self._placements, L.VALUE_C128)
+        finally:
+            if saved is not None:
+                self._tables, self._table_ids, self._placements, self.coeffs = saved
+            else:
+                for a in ("_tables", "_table_ids", "_placements"):
+                    self.__dict__.pop(a, None)
+        self.Hlocal = QMB_hamiltonian(self.lvals, self.sector_dim)
+        self.Hlocal.set_matrix(csr, "sparse")
+        return csr
+
+    def get_mask(self, sites_list):
+        """Boolean mask over the lattice selecting the given sites
+        (SU2_model.py:701-706)."""
+        mask = np.zeros(self.lvals, dtype=bool)
+        for site in sites_list:
+            mask[site] = True
+        return mask
+
     def build_base_Hamiltonian(self, g, m=None, theta=0.0, lambda_noise=0.0, dtype_mode="auto"):
         """The hard-coded spin-1/2 builder by name (SU2_model.py:200-355), for
         models built with that operator set; build_Hamiltonian dispatches to it."""
