@@ -304,6 +304,26 @@ class QMB_hamiltonian:
         self.nnz = csr.nnz
         self._all_pairs = None
 
+    def _triplets(self):
+        H = self.Ham
+        if hasattr(H, "to_scipy"):
+            H = H.to_scipy()
+        C = H.tocoo()
+        return C.row.astype(np.int32), C.col.astype(np.int32), C.data
+
+    @property
+    def row_list(self):
+        """Rows of the stored entries (the reference keeps its triplets here)."""
+        return self._triplets()[0]
+
+    @property
+    def col_list(self):
+        return self._triplets()[1]
+
+    @property
+    def value_list(self):
+        return self._triplets()[2]
+
     def convert_hamiltonian(self, format):
         """qmb_hamiltonian.py:227-258.  The device matrix serves every format
         (all products run on the device); the label selects the time-evolution
@@ -711,6 +731,10 @@ class QuantumModel:
         the reference's translational_sym module).  The Hamiltonian becomes
         B^H H B on the n_cols momentum states; measure_observables maps the
         eigenvectors back through B."""
+        self.momentum_basis = self._momentum_projector(k_unit_cell_size, k_vals, TC_symmetry)
+        logger.info(f"Momentum basis shape ({self.momentum_basis.n_rows},{self.momentum_basis.n_cols})")
+
+    def _momentum_projector(self, k_unit_cell_size, k_vals, TC_symmetry):
         from .device import MomentumBasis
         if self.basis is None:
             raise ValueError("symmetry sector_configs not defined yet")
@@ -724,10 +748,42 @@ class QuantumModel:
             tc = getattr(self, "tc_symmetry_tables", None)
             if tc is None or self.dim != 1:
                 raise NotImplementedError("TC_symmetry momentum sectors exist for the 1D SU(2) chain only")
-            self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals, tc_map=tc[0], tc_sign=tc[1])
-        else:
-            self.momentum_basis = MomentumBasis(self.basis, self.lvals, uc, k_vals)
-        logger.info(f"Momentum basis shape ({self.momentum_basis.n_rows},{self.momentum_basis.n_cols})")
+            return MomentumBasis(self.basis, self.lvals, uc, k_vals, tc_map=tc[0], tc_sign=tc[1])
+        return MomentumBasis(self.basis, self.lvals, uc, k_vals)
+
+    def set_momentum_pair(self, k_left, k_right, k_unit_cell_size, TC_symmetry):
+        """Two momentum projectors P_kL, P_kR for rectangular blocks
+        P_kL^dagger O P_kR of operators built afterwards
+        (quantum_model.py:278-348): kept in self.momentum_pair; the sector
+        basis and its Hamiltonian are unchanged."""
+        mb_l = self._momentum_projector(k_unit_cell_size, k_left, TC_symmetry)
+        mb_r = self._momentum_projector(k_unit_cell_size, k_right, TC_symmetry)
+        logger.info(f"Combined Momenta {k_left} {k_right}")
+        self.momentum_pair = {"L": mb_l, "R": mb_r, "k_left": np.atleast_1d(np.asarray(k_left, dtype=np.int32)),
+                              "k_right": np.atleast_1d(np.asarray(k_right, dtype=np.int32))}
+
+    def check_momentum_pair(self):
+        """||P_L^H P_L - I||_F and ||P_R^H P_R - I||_F <= 1e-12, and, for
+        different momenta, ||P_L^H P_R||_F <= 1e-12 (quantum_model.py:350-398);
+        ValueError otherwise."""
+        import scipy.sparse as sps
+        from scipy.sparse.linalg import norm
+        mp = getattr(self, "momentum_pair", None)
+        if mp is None:
+            raise ValueError("no momentum pair set (set_momentum_pair)")
+        PL, PR = mp["L"].to_scipy().tocsc(), mp["R"].to_scipy().tocsr()
+        nl = norm(PL.conj().T @ PL - sps.identity(PL.shape[1]))
+        nr = norm(PR.conj().T @ PR - sps.identity(PR.shape[1]))
+        nmix = norm(PL.conj().T @ PR)
+        logger.info(f"norm: (PL^dag @ PL) -1: {nl}")
+        logger.info(f"norm: (PR^dag @ PR) -1: {nr}")
+        logger.info(f"norm: (PL^dag @ PR): {nmix}")
+        if nl > 1e-12:
+            raise ValueError("PL is not a projector")
+        if nr > 1e-12:
+            raise ValueError("RL is not a projector")
+        if not np.array_equal(mp["k_left"], mp["k_right"]) and nmix > 1e-12:
+            raise ValueError("The two basis are not orthogonal")
 
     def get_abelian_symmetry_sector(self, global_ops=None, global_sectors=None, global_sym_type="U",
                                     link_ops=None, link_sectors=None, nbody_ops=None, nbody_sectors=None,
@@ -1988,14 +2044,17 @@ class SU2_Model(QuantumModel):
         the staggered mass (-1)^R0 m at R0, and half of each hopping on the
         bonds (R0, R0+1) and (R0-1, R0) with its conjugate written out as the
         reference writes it ([Qpx_dag, Qmx] times t/2, [Qpx, Qmx_dag] times
-        -t/2).  Summed over R0 it is the chain's Hamiltonian."""
+        -t/2).  Summed over R0 it is the chain's Hamiltonian.  After
+        set_momentum_pair, Hlocal holds the rectangular block P_kL^dagger H_R0
+        P_kR (SciPy CSR) instead."""
         from .device import CSRMatrix
         if self.dim > 1:
             raise ValueError(f"Local Hamiltonian valid only for D=1, got {self.dim}")
         if self.pure_theory or self.use_generic_model:
             raise ValueError("the local Hamiltonian is defined for the spin-1/2 chain with matter")
         if self.momentum_basis is not None:
-            raise NotImplementedError("the local Hamiltonian is built in the sector basis (no momentum sector)")
+            raise NotImplementedError("the local Hamiltonian is built in the sector basis or, after "
+                                      "set_momentum_pair, between two momentum sectors")
         if not 0 <= int(R0) < self.n_sites:
             raise ValueError(f"R0 must be a site of the chain, got {R0}")
         R0 = int(R0)
@@ -2023,6 +2082,15 @@ class SU2_Model(QuantumModel):
                     self.__dict__.pop(a, None)
         self.Hlocal = QMB_hamiltonian(self.lvals, self.sector_dim)
         self.Hlocal.set_matrix(csr, "sparse")
+        mp = getattr(self, "momentum_pair", None)
+        if mp is not None:
+            # the rectangular block P_kL^dagger H_R0 P_kR, as the reference's
+            # pair-mode terms give it (host product of the device matrices)
+            A = (mp["L"].to_scipy().conj().T @ csr.to_scipy() @ mp["R"].to_scipy()).tocsr()
+            A.sum_duplicates()
+            self.Hlocal.Ham, self.Hlocal.shape = A, A.shape
+            self.Hlocal.nnz = A.nnz
+            return A
         return csr
 
     def get_mask(self, sites_list):
