@@ -275,6 +275,51 @@ def get_projector_for_efficient_density_matrix(rho_eigvals, rho_eigvecs, thresho
     return vecs[:, :n_cols].astype(complex)
 
 
+def build_energy_block_ids(evals, tol=1e-10):
+    """(block id per eigenvalue, number of blocks): consecutive sorted
+    eigenvalues closer than tol share a degenerate block
+    (quantum_model.py:1451-1478)."""
+    E = np.real_if_close(np.asarray(evals))
+    if E.size == 0:
+        return np.zeros(0, dtype=np.int32), 0
+    block = np.concatenate([[0], np.cumsum(np.abs(np.diff(E)) >= tol)]).astype(np.int32)
+    return block, int(block[-1]) + 1
+
+
+def apply_projection(projector, operator):
+    """P^dagger O P in the dense or sparse family of the inputs
+    (quantum_model.py:1411-1448)."""
+    import scipy.sparse as sps
+    if projector.shape[0] != operator.shape[0]:
+        raise ValueError(f"Projector and operator have incompatible shapes {projector.shape} {operator.shape}")
+    if sps.issparse(projector) and sps.issparse(operator):
+        return (projector.conj().T @ operator @ projector).tocsr()
+    if not sps.issparse(projector) and not sps.issparse(operator):
+        P = np.asarray(projector)
+        return P.conj().T @ np.asarray(operator) @ P
+    raise TypeError("projector and operator must both be dense or both be sparse")
+
+
+def format_loc_dims(loc_dims, lvals, pad=3):
+    """Logs the local dimensions on the lattice's geometry: one line per row
+    of x; in 3D the z layers of a row side by side (quantum_model.py:
+    1481-1523)."""
+    arr = np.asarray(loc_dims)
+
+    def row(vals):
+        return "[" + " ".join(f"{v:>{pad}d}" for v in vals) + "]"
+
+    if len(lvals) == 1:
+        logger.info(row(arr.reshape(lvals[0])))
+    elif len(lvals) == 2:
+        for r in arr.reshape(lvals[1], lvals[0]):
+            logger.info(row(r))
+    elif len(lvals) == 3:
+        g = arr.reshape(lvals[2], lvals[1], lvals[0])  # [z, y, x]
+        for yy in range(lvals[1]):
+            logger.info("   ".join(row(g[zz, yy]) for zz in range(lvals[2])))
+
+
 def mixed_exp_val_data(psi1, psi2, row_list, col_list, value_list):
     """<psi1|O|psi2> of an operator given as triplets, duplicates summed
     (qmb_state.py:657-679)."""
@@ -1219,9 +1264,8 @@ class QuantumModel:
         st = self._state_tensor(state)
         E = self._eigvec_matrix()
         coef = torch.mv(E.conj(), st)
-        ev = np.asarray(self.H.Nenergies)
-        block = np.concatenate([[0], np.cumsum(np.abs(np.diff(ev)) >= tol_deg)]).astype(np.int64)
-        n_b = int(block[-1]) + 1
+        block, n_b = build_energy_block_ids(self.H.Nenergies, tol_deg)
+        block = block.astype(np.int64)
         P = torch.zeros((n_b, E.shape[1]), dtype=torch.complex128, device="cuda")
         P.index_add_(0, torch.as_tensor(block, device="cuda"), coef[:, None] * E)
         return self._density_values(self._row_density(P), local_obs_list, special_norms, staggered_avgs, "DE")

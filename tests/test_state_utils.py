@@ -103,3 +103,28 @@ def test_density_matrix_projector_and_mixed_expectation():
     a = rng.normal(size=4) + 1j * rng.normal(size=4)
     b = rng.normal(size=4) + 1j * rng.normal(size=4)
     assert abs(mixed_exp_val_data(a, b, r, c, v) - np.vdot(a, O @ b)) < 1e-14
+
+
+def test_block_ids_projection_and_loc_dims_printout(caplog):
+    """build_energy_block_ids (quantum_model.py:1451-1478), apply_projection
+    (:1411-1448) and format_loc_dims (:1481-1523) on closed cases."""
+    import logging
+    import edlgt_amd.models as M
+    b, n = M.build_energy_block_ids(np.array([-2.0, -2.0 + 1e-12, -1.0, 0.5, 0.5]), 1e-10)
+    assert list(b) == [0, 0, 1, 2, 2] and n == 3
+    b, n = M.build_energy_block_ids(np.array([1.0]))
+    assert list(b) == [0] and n == 1
+    P = np.array([[1.0, 0.0], [0.0, 1j], [0.0, 0.0]])
+    O = np.arange(9.0).reshape(3, 3)
+    want = P.conj().T @ O @ P
+    assert np.allclose(M.apply_projection(P, O), want)
+    got = M.apply_projection(sp.csr_matrix(P), sp.csr_matrix(O))
+    assert sp.issparse(got) and np.allclose(got.toarray(), want)
+    with pytest.raises(TypeError):
+        M.apply_projection(sp.csr_matrix(P), O)
+    with pytest.raises(ValueError):
+        M.apply_projection(P[:2], O)
+    with caplog.at_level(logging.INFO, logger=M.logger.name):
+        M.format_loc_dims(np.arange(12), [2, 3, 2], pad=2)
+    lines = [r.getMessage() for r in caplog.records]
+    assert lines == ["[ 0  1]   [ 6  7]", "[ 2  3]   [ 8  9]", "[ 4  5]   [10 11]"]
