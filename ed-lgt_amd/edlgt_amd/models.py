@@ -275,6 +275,24 @@ def get_projector_for_efficient_density_matrix(rho_eigvals, rho_eigvecs, thresho
     return vecs[:, :n_cols].astype(complex)
 
 
+def truncation(array, threshold=1e-14):
+    """Entries with |a| <= threshold set to zero (qmb_state.py:466-482)."""
+    a = np.asarray(array)
+    return np.where(np.abs(a) > threshold, a, 0)
+
+
+def get_norm(psi):
+    """Euclidean norm of a complex vector (qmb_state.py:484-502)."""
+    return float(np.sqrt(np.sum(np.abs(np.asarray(psi)) ** 2)))
+
+
+def get_sorted_indices(data):
+    """Indices ordering data by descending magnitude, ties by descending real
+    part (qmb_state.py:504-523: lexsort on (real, abs), reversed)."""
+    data = np.asarray(data)
+    return np.lexsort((np.real(data), np.abs(data)))[::-1]
+
+
 def build_energy_block_ids(evals, tol=1e-10):
     """(block id per eigenvalue, number of blocks): consecutive sorted
     eigenvalues closer than tol share a degenerate block

@@ -128,3 +128,15 @@ def test_block_ids_projection_and_loc_dims_printout(caplog):
         M.format_loc_dims(np.arange(12), [2, 3, 2], pad=2)
     lines = [r.getMessage() for r in caplog.records]
     assert lines == ["[ 0  1]   [ 6  7]", "[ 2  3]   [ 8  9]", "[ 4  5]   [10 11]"]
+
+
+def test_truncation_norm_and_sorted_indices():
+    """truncation, get_norm, get_sorted_indices (qmb_state.py:466-523)."""
+    import edlgt_amd.models as M
+    a = np.array([1e-15, -0.5, 2e-14j, 0.3 + 0.4j])
+    assert np.array_equal(M.truncation(a), np.array([0, -0.5, 2e-14j, 0.3 + 0.4j]))
+    assert np.array_equal(M.truncation(a, 1e-13), np.array([0, -0.5, 0, 0.3 + 0.4j]))
+    assert M.get_norm(np.array([3.0, 4j])) == 5.0
+    # |.| = 0.5 three times: ties by real part, largest first; then 0.2
+    d = np.array([0.2, -0.5, 0.5, 0.5j])
+    assert list(M.get_sorted_indices(d)) == [2, 3, 1, 0]
