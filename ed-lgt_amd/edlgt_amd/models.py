@@ -1169,12 +1169,28 @@ class QuantumModel:
         if key not in self._partition_cache:
             cfg = self.sector_configs
             env = [i for i in range(self.n_sites) if i not in key]
-            _, sub_map = np.unique(cfg[:, list(key)], axis=0, return_inverse=True)
-            _, env_map = np.unique(cfg[:, env], axis=0, return_inverse=True)
+            sub_u, sub_map = np.unique(cfg[:, list(key)], axis=0, return_inverse=True)
+            env_u, env_map = np.unique(cfg[:, env], axis=0, return_inverse=True)
             sub_map, env_map = sub_map.ravel(), env_map.ravel()
-            self._partition_cache[key] = dict(subsys_map=sub_map, env_map=env_map,
+            self._partition_cache[key] = dict(subsys_indices=list(key), env_indices=env,
+                                              unique_subsys_configs=sub_u, unique_env_configs=env_u,
+                                              subsys_map=sub_map, env_map=env_map,
                                               subsys_dim=int(sub_map.max()) + 1, env_dim=int(env_map.max()) + 1)
         return self._partition_cache[key]
+
+    def build_projector_from_sector_to_fullspace(self, indices):
+        """Column isometry from the subsystem's sector configurations to the
+        subsystem's full product space (quantum_model.py:793-824): entry 1 at
+        (mixed-radix index of configuration j, rightmost site fastest; j)."""
+        import scipy.sparse as sps
+        u = self._get_partition(indices)["unique_subsys_configs"].astype(np.int64)
+        dims = np.asarray(self.loc_dims, dtype=np.int64)[sorted(indices)]
+        strides = np.ones(dims.size, dtype=np.int64)
+        for k in range(dims.size - 2, -1, -1):
+            strides[k] = strides[k + 1] * dims[k + 1]
+        rows = u @ strides
+        n = u.shape[0]
+        return sps.csc_matrix((np.ones(n), (rows, np.arange(n))), shape=(int(np.prod(dims)), n))
 
     def measure_fidelity(self, state, index, dynamics=False, print_value=False):
         """|<state|psi_index>|^2 with an eigenstate or a time-evolved state

@@ -99,3 +99,30 @@ def test_momentum_state_and_operator_projection(gpu, orc):
     assert np.max(np.abs(got - (Bo.conj().T @ A @ Bo))) < 1e-13
     with pytest.raises(ValueError):
         _chain().momentum_basis_projection("H")
+
+
+def test_projector_from_sector_to_fullspace(gpu, orc):
+    """build_projector_from_sector_to_fullspace (quantum_model.py:793-824):
+    an isometry whose embedding of the sector reduced density matrix of the
+    ground state (sites 0-2 of the 6-site chain) equals the reduced density
+    matrix of the same state written out on the full 6^6 product space."""
+    print("sector-to-fullspace projector", flush=True)
+    mod = _chain()
+    mod.build_Hamiltonian(1.0, 5.0)
+    mod.diagonalize_Hamiltonian(1, "sparse", tol=1e-12)
+    keep = [0, 1, 2]
+    part = mod._get_partition(keep)
+    P = mod.build_projector_from_sector_to_fullspace(keep)
+    d = int(np.prod(mod.loc_dims[keep]))
+    assert P.shape == (d, part["subsys_dim"])
+    assert np.array_equal((P.T @ P).toarray(), np.eye(part["subsys_dim"]))
+    rho = mod.H.Npsi[0].reduced_density_matrix(keep, mod._partition_cache)
+    full = (P @ rho @ P.T.toarray())
+    dims = np.asarray(mod.loc_dims, dtype=np.int64)
+    psi = mod.H.Npsi[0].psi
+    cfg = np.asarray(mod.sector_configs, dtype=np.int64)
+    sub = cfg[:, :3] @ np.array([dims[1] * dims[2], dims[2], 1])
+    env = cfg[:, 3:] @ np.array([dims[4] * dims[5], dims[5], 1])
+    M = np.zeros((d, int(np.prod(dims[3:]))), dtype=complex)
+    M[sub, env] = psi
+    assert np.max(np.abs(full - M @ M.conj().T)) < 1e-13
