@@ -293,6 +293,21 @@ def get_sorted_indices(data):
     return np.lexsort((np.real(data), np.abs(data)))[::-1]
 
 
+def exact_time_evolution(tline, eigenvalues, eigenvectors, initial_state):
+    """psi(t) = sum_i e^{-i E_i t} <E_i|psi0> E_i for every t of tline, rows
+    of the result; eigenvectors by columns (qmb_hamiltonian.py:792-836)."""
+    V = np.asarray(eigenvectors)
+    c = V.conj().T @ np.asarray(initial_state, dtype=np.complex128)
+    ph = np.exp(-1j * np.outer(np.asarray(tline, dtype=float), np.asarray(eigenvalues, dtype=float)))
+    return (ph * c[None, :]) @ V.T
+
+
+def compute_overlap_with_eigenstates(eigenvectors, state):
+    """<E_i|state> for the eigenvectors given by columns
+    (qmb_hamiltonian.py:838-861)."""
+    return np.asarray(eigenvectors).conj().T @ np.asarray(state, dtype=np.complex128)
+
+
 def build_energy_block_ids(evals, tol=1e-10):
     """(block id per eigenvalue, number of blocks): consecutive sorted
     eigenvalues closer than tol share a degenerate block
@@ -1331,6 +1346,37 @@ class QuantumModel:
     def get_thermal_beta(self, state, threshold):
         """Effective inverse temperature of a state (quantum_model.py:908-910)."""
         return self.H.get_beta(state, threshold)
+
+    # -- numeric mode (quantum_model.py:1342-1409)
+    @staticmethod
+    def _normalize_dtype_mode(dtype_mode, allow_auto=True):
+        """bool (True = complex), "real" / "complex" / "auto" (any case) or
+        None (= "auto") to the mode's label; TypeError / ValueError otherwise."""
+        if isinstance(dtype_mode, (bool, np.bool_)):
+            return "complex" if bool(dtype_mode) else "real"
+        if dtype_mode is None:
+            if allow_auto:
+                return "auto"
+            raise ValueError("dtype_mode cannot be None in this context")
+        if not isinstance(dtype_mode, str):
+            raise TypeError(f"dtype_mode must be bool/str/None, got {type(dtype_mode)}")
+        mode = dtype_mode.strip().lower()
+        valid = {"real", "complex", "auto"} if allow_auto else {"real", "complex"}
+        if mode not in valid:
+            raise ValueError(f"dtype_mode must be one of {sorted(valid)}, got {dtype_mode!r}")
+        return mode
+
+    def configure_dtype_mode(self, dtype_mode="auto", auto_mode="complex"):
+        """The numeric mode ("real" or "complex") that dtype_mode selects,
+        auto_mode standing in for "auto".  The device build takes its mode
+        per call (build_Hamiltonian(..., dtype_mode=...): float64 values only
+        where every entry is real), so this records the choice in
+        self.dtype_mode and returns it."""
+        mode = self._normalize_dtype_mode(dtype_mode, allow_auto=True)
+        if mode == "auto":
+            mode = self._normalize_dtype_mode(auto_mode, allow_auto=False)
+        self.dtype_mode = mode
+        return mode
 
     # -- configuration printout (QED_model.py:901-946, SU2_model.py:831-871)
     _pad_state_values = True  # QED pads each local value to the column width

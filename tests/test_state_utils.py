@@ -140,3 +140,32 @@ def test_truncation_norm_and_sorted_indices():
     # |.| = 0.5 three times: ties by real part, largest first; then 0.2
     d = np.array([0.2, -0.5, 0.5, 0.5j])
     assert list(M.get_sorted_indices(d)) == [2, 3, 1, 0]
+
+
+def test_exact_evolution_overlaps_and_dtype_modes():
+    """exact_time_evolution / compute_overlap_with_eigenstates
+    (qmb_hamiltonian.py:792-861) against scipy's expm on a random Hermitian
+    matrix; the dtype-mode labels (quantum_model.py:1342-1409)."""
+    from scipy.linalg import expm
+    import edlgt_amd.models as M
+    rng = np.random.default_rng(2)
+    A = rng.normal(size=(7, 7)) + 1j * rng.normal(size=(7, 7))
+    H = A + A.conj().T
+    e, V = np.linalg.eigh(H)
+    psi = rng.normal(size=7) + 1j * rng.normal(size=7)
+    tl = np.array([0.0, 0.3, 1.7])
+    out = M.exact_time_evolution(tl, e, V, psi)
+    for k, t in enumerate(tl):
+        assert np.max(np.abs(out[k] - expm(-1j * t * H) @ psi)) < 1e-12
+    assert np.max(np.abs(M.compute_overlap_with_eigenstates(V, psi) - V.conj().T @ psi)) < 1e-14
+    norm = M.QuantumModel._normalize_dtype_mode
+    assert norm(True) == "complex" and norm(False) == "real" and norm(None) == "auto" and norm(" Real ") == "real"
+    with pytest.raises(ValueError):
+        norm(None, allow_auto=False)
+    with pytest.raises(ValueError):
+        norm("half")
+    with pytest.raises(TypeError):
+        norm(3)
+    qm = object.__new__(M.QuantumModel)
+    assert qm.configure_dtype_mode() == "complex" and qm.configure_dtype_mode("auto", "real") == "real"
+    assert qm.configure_dtype_mode(False) == "real" and qm.dtype_mode == "real"
