@@ -1347,6 +1347,23 @@ class QuantumModel:
         """Effective inverse temperature of a state (quantum_model.py:908-910)."""
         return self.H.get_beta(state, threshold)
 
+    # -- link symmetries of measured fields (local_term.py:240-284)
+    def check_link_symmetry(self, axis, obs1, obs2, value=0, sign=1):
+        """res[obs1][i] + sign * res[obs2][j] == value (within 1e-10) on every
+        bond (i, j) along axis, from the measured per-site values
+        (local_term.py:240-284); ValueError naming the first violating site."""
+        a, b = np.asarray(self.res[obs1]), np.asarray(self.res[obs2])
+        for i in range(self.n_sites):
+            j = self.lattice.step(i, axis)
+            if j is None:
+                continue
+            if abs(a[i] + sign * b[j] - value) > 1e-10:
+                logger.info(f"{a[i]}")
+                logger.info(f"{b[j]}")
+                raise ValueError(f"{axis}-Link Symmetry is violated at index {i}")
+        logger.info("----------------------------------------------------")
+        logger.info(f"{axis}-LINK SYMMETRY IS SATISFIED")
+
     # -- numeric mode (quantum_model.py:1342-1409)
     @staticmethod
     def _normalize_dtype_mode(dtype_mode, allow_auto=True):
@@ -1675,6 +1692,16 @@ class QED_Model(QuantumModel):
                 logger.info(f"{(i,)} {format(e2[i], '.16f')}")
             logger.info(f"{format(self.res['E2_avg'], '.16f')}")
         return e2
+
+    def check_symmetries(self):
+        """E_p{d}(i) - E_m{d}(i+d) = 0 on every link, from the measured fields
+        (QED_model.py:707-717; measured by get_observables([... "E_p{d}",
+        "E_m{d}" ...]) and measure_observables).  The relation checked is the
+        one the sector imposes ([E_p, -E_m] summing to 0, QED_model.py:136);
+        the reference's check, which reads LocalTerm objects it no longer
+        keeps, passes the opposite sign."""
+        for d in self.directions:
+            self.check_link_symmetry(d, f"E_p{d}", f"E_m{d}", value=0, sign=-1)
 
     def QED_Hamiltonian_couplings(self, g, m=None, theta=0.0):
         """QED_model.py:719-776 (dressed-site normalisation)."""
@@ -2096,6 +2123,12 @@ class SU2_Model(QuantumModel):
             return None
         a = np.array([c[0] % 2 for c in map(self.lattice.coords, range(self.n_sites))], dtype=np.int64)
         return a, "N_tot"
+
+    def check_symmetries(self):
+        """T2_p{d}(i) - T2_m{d}(i+d) = 0 on every link, from the measured
+        Casimirs (SU2_model.py:474-484)."""
+        for d in self.directions:
+            self.check_link_symmetry(d, f"T2_p{d}", f"T2_m{d}", value=0, sign=-1)
 
     def SU2_Hamiltonian_couplings(self, g, m=None, theta=0.0):
         """SU2_model.py:555-624 (hopping rescaled to 2 sqrt(2))."""

@@ -126,3 +126,27 @@ def test_projector_from_sector_to_fullspace(gpu, orc):
     M = np.zeros((d, int(np.prod(dims[3:]))), dtype=complex)
     M[sub, env] = psi
     assert np.max(np.abs(full - M @ M.conj().T)) < 1e-13
+
+
+def test_link_symmetry_checks(gpu):
+    """check_symmetries (QED_model.py:707-717, SU2_model.py:474-484) on
+    measured fields of ground states: satisfied, and a planted violation in
+    the measured values is reported with its site."""
+    from edlgt_amd import QED_Model
+    print("link symmetry checks", flush=True)
+    q = QED_Model(spin=1, pure_theory=False, lvals=[2, 2], has_obc=[False, False])
+    q.build_Hamiltonian(1.0, 0.5)
+    q.diagonalize_Hamiltonian(1, "sparse")
+    q.get_observables([f"E_{s}{d}" for s in "pm" for d in q.directions])
+    q.measure_observables(0)
+    q.check_symmetries()
+    q.res["E_px"] = np.array(q.res["E_px"], dtype=float)
+    q.res["E_px"][3] += 1e-6
+    with pytest.raises(ValueError, match="x-Link Symmetry is violated at index 3"):
+        q.check_symmetries()
+    s = _chain()
+    s.build_Hamiltonian(1.0, 5.0)
+    s.diagonalize_Hamiltonian(1, "sparse")
+    s.get_observables(["T2_px", "T2_mx"])
+    s.measure_observables(0)
+    s.check_symmetries()
