@@ -23,14 +23,27 @@ __all__ = ["QuantumModel", "QED_Model", "SU2_Model", "Zn_Model", "QMB_hamiltonia
 
 
 class QMB_state:
-    """State vector wrapper (edlgt/modeling/qmb_state.py:41).  `psi` is the host
-    copy (complex128 numpy), `psi_device` the device tensor."""
+    """State vector wrapper (edlgt/modeling/qmb_state.py:41-77).  `psi` is the
+    host copy (numpy), `psi_device` the device tensor; constructed from either
+    (a host array is copied to the device once)."""
 
-    def __init__(self, psi_device, lvals, loc_dims):
-        self.psi_device = psi_device
+    def __init__(self, psi, lvals=None, loc_dims=None, symmetry_sector=True, debug_mode=False):
+        if hasattr(psi, "data_ptr"):
+            self.psi_device = psi
+            self._host = None
+        else:
+            import torch
+            host = np.asarray(psi)
+            if host.ndim != 1 or host.size == 0:
+                raise ValueError(f"psi must be a non-empty 1D array, got shape {host.shape}")
+            if not np.iscomplexobj(host):
+                host = host.astype(np.float64)
+            self.psi_device = torch.as_tensor(np.ascontiguousarray(host), device="cuda")
+            self._host = None
         self.lvals = lvals
         self.loc_dims = loc_dims
-        self._host = None
+        self.symmetry_sector = symmetry_sector
+        self.debug_mode = debug_mode
 
     @property
     def psi(self):

@@ -150,3 +150,23 @@ def test_link_symmetry_checks(gpu):
     s.get_observables(["T2_px", "T2_mx"])
     s.measure_observables(0)
     s.check_symmetries()
+
+
+def test_state_from_host_array(gpu, orc):
+    """QMB_state(psi, lvals, loc_dims) as the reference constructs it
+    (qmb_state.py:44-77): a host array is copied to the device; the entropy
+    and expectation values through it equal those of the device-built state."""
+    print("state from a host array", flush=True)
+    from edlgt_amd import QMB_state
+    mod = _chain()
+    mod.build_Hamiltonian(1.0, 5.0)
+    mod.diagonalize_Hamiltonian(1, "sparse", tol=1e-12)
+    gs = mod.H.Npsi[0]
+    st = QMB_state(psi=gs.psi.copy(), lvals=mod.lvals, loc_dims=mod.loc_dims)
+    assert st.psi_device.is_cuda and np.array_equal(st.psi, gs.psi)
+    mod._get_partition([0, 1, 2])
+    assert abs(st.entanglement_entropy([0, 1, 2], mod._partition_cache)
+               - gs.entanglement_entropy([0, 1, 2], mod._partition_cache)) < 1e-14
+    assert abs(st.expectation_value(mod.H.Ham) - gs.expectation_value(mod.H.Ham)) < 1e-12
+    with pytest.raises(ValueError):
+        QMB_state(np.zeros((2, 2)), mod.lvals, mod.loc_dims)
