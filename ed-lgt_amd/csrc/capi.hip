@@ -512,6 +512,13 @@ int edlgt_basis_signed_image(const edlgt_basis* b, const int32_t* site_perm, con
   return basis_signed_image(*b, site_perm, loc_perm, loc_sign, d_max, out_index, out_sign, (hipStream_t)stream);
 }
 
+int edlgt_ensemble_density(const void* rows_matrix, int64_t n, int64_t n_rows, int64_t n_members,
+                           const int64_t* member_rows, const double* member_coef, const int64_t* block_start,
+                           int64_t n_blocks, double* density, void* stream) {
+  return ensemble_density(rows_matrix, n, n_rows, n_members, member_rows, member_coef, block_start, n_blocks,
+                          density, (hipStream_t)stream);
+}
+
 int edlgt_basis_sre2(const edlgt_basis* b, const void* psi, double prob_threshold, double* sre2,
                      int64_t* support_size, double* discarded_weight, int64_t* n_strings, void* stream) {
   EDLGT_REQUIRE(b && psi, "null argument");

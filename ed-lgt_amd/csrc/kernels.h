@@ -117,6 +117,12 @@ int basis_signed_image(const Basis& B, const int32_t* site_perm, const int32_t* 
                        int d_max, int64_t* out_idx, double* out_sign, hipStream_t st);
 // y = i^(dir e) x per basis state, e_r = sum_s tab[s][c_r[s]] mod 4 (real or
 // complex x, complex y; y may alias a complex x).
+// Ensemble density rho[r] = sum_b |sum_{t in block b} coef[t] E[rows[t]][r]|^2
+// over the rows of a complex128 matrix E (n_rows_E x n, row-major, device);
+// members, coefficients (complex, host) and block starts (host, n_blocks + 1).
+int ensemble_density(const void* E, int64_t n, int64_t n_rows_E, int64_t k, const int64_t* rows_host,
+                     const double* coef_host, const int64_t* bstart_host, int64_t n_blocks, double* rho_dev,
+                     hipStream_t st);
 int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int n_vec, int dir, bool cplx_in,
                         const void* x, void* y, hipStream_t st);
 

@@ -211,4 +211,57 @@ int basis_quarter_phase(const Basis& B, const int32_t* tab_host, int d_max, int 
   return EDLGT_OK;
 }
 
+namespace {
+
+// rho[r] = sum_b |sum_{t in block b} coef[t] E[rows[t]][r]|^2, one thread per
+// basis state r (coalesced reads of the eigenvector rows), blocks and their
+// members in the order given (deterministic).
+__global__ void __launch_bounds__(256) ensemble_density_rows(const double2* __restrict__ E, int64_t n,
+                                                            const int64_t* __restrict__ rows,
+                                                            const double2* __restrict__ coef,
+                                                            const int64_t* __restrict__ bstart, int64_t n_blocks,
+                                                            double* __restrict__ rho) {
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) {
+    double acc = 0.0;
+    for (int64_t b = 0; b < n_blocks; ++b) {
+      double sx = 0.0, sy = 0.0;
+      for (int64_t t = bstart[b]; t < bstart[b + 1]; ++t) {
+        const double2 e = E[rows[t] * n + r], c = coef[t];
+        sx += c.x * e.x - c.y * e.y;
+        sy += c.x * e.y + c.y * e.x;
+      }
+      acc += sx * sx + sy * sy;
+    }
+    rho[r] = acc;
+  }
+}
+
+}  // namespace
+
+int ensemble_density(const void* E, int64_t n, int64_t n_rows_E, int64_t k, const int64_t* rows_host,
+                     const double* coef_host, const int64_t* bstart_host, int64_t n_blocks, double* rho_dev,
+                     hipStream_t st) {
+  if (n < 0 || k < 0 || n_blocks < 0) return set_error(EDLGT_ERR_INVALID_ARG, "negative size");
+  if (n == 0) return EDLGT_OK;
+  if (!E || !rho_dev || (k && (!rows_host || !coef_host)) || !bstart_host)
+    return set_error(EDLGT_ERR_INVALID_ARG, "null argument");
+  if (bstart_host[0] != 0 || bstart_host[n_blocks] != k)
+    return set_error(EDLGT_ERR_INVALID_ARG, "block starts must run from 0 to the member count");
+  for (int64_t b = 0; b < n_blocks; ++b)
+    if (bstart_host[b + 1] < bstart_host[b]) return set_error(EDLGT_ERR_INVALID_ARG, "block starts must not decrease");
+  for (int64_t t = 0; t < k; ++t)
+    if (rows_host[t] < 0 || rows_host[t] >= n_rows_E)
+      return set_error(EDLGT_ERR_INVALID_ARG, "member row " + std::to_string(rows_host[t]) + " outside the matrix");
+  DevBuf rows, coef, bs;
+  EDLGT_TRY(upload(rows, rows_host, (size_t)k, st));
+  EDLGT_TRY(upload(coef, reinterpret_cast<const double2*>(coef_host), (size_t)k, st));
+  EDLGT_TRY(upload(bs, bstart_host, (size_t)n_blocks + 1, st));
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65535);
+  hipLaunchKernelGGL(ensemble_density_rows, dim3(g), dim3(256), 0, st, static_cast<const double2*>(E), n,
+                     rows.as<int64_t>(), coef.as<double2>(), bs.as<int64_t>(), n_blocks, rho_dev);
+  EDLGT_HIP(hipGetLastError());
+  EDLGT_HIP(hipStreamSynchronize(st));  // the member lists are freed on return
+  return EDLGT_OK;
+}
+
 }  // namespace edlgt

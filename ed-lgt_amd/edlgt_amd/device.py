@@ -25,6 +25,22 @@ def basis_of_configs(configs):
     return b if b is not None and b._configs is configs else None
 
 
+def ensemble_density(E, members, coef, block_start, stream=None):
+    """rho (device float64, n) = sum_b |sum_{t in block b} coef[t] E[members[t]]|^2
+    over the rows of a complex128 device matrix E (edlgt_ensemble_density)."""
+    torch = _torch()
+    E = E.to(device="cuda", dtype=torch.complex128).contiguous()
+    n_rows, n = E.shape
+    members = np.ascontiguousarray(members, dtype=np.int64)
+    coef = np.ascontiguousarray(coef, dtype=np.complex128)
+    bs = np.ascontiguousarray(block_start, dtype=np.int64)
+    rho = torch.empty(n, dtype=torch.float64, device="cuda")
+    L.check(L.load().edlgt_ensemble_density(L.ptr(E), n, n_rows, members.size, members.ctypes.data,
+                                            coef.ctypes.data, bs.ctypes.data, bs.size - 1, L.ptr(rho),
+                                            L.stream_handle(stream)))
+    return rho
+
+
 def _torch():
     import torch
     if not torch.cuda.is_available():

@@ -1266,19 +1266,18 @@ class QuantumModel:
         import torch
         return torch.stack([s.psi_device.to(torch.complex128) for s in self.H.Npsi])
 
-    def _row_density(self, M, w=None):
-        """sum_r w_r |S m_r|^2 over the rows m_r of M (device), S the map to the
-        sector basis (the identity, or B for a momentum basis)."""
+    def _ensemble_rho(self, E, members, coef, bstart):
+        """rho(r) = sum_b |sum_{t in block b} coef_t S E[members_t](r)|^2 on the
+        device (edlgt_ensemble_density), S the map to the sector basis (the
+        identity, or B for a momentum basis, whose member rows are expanded
+        first)."""
         import torch
-        if self._sector_state(M[0]) is M[0]:
-            a2 = M.abs() ** 2
-            return a2.sum(dim=0) if w is None else (w[:, None] * a2).sum(dim=0)
-        rho = None
-        for r in range(M.shape[0]):
-            t = self._sector_state(M[r].contiguous()).abs() ** 2
-            t = t if w is None else w[r] * t
-            rho = t if rho is None else rho + t
-        return rho
+        from .device import ensemble_density
+        members = np.asarray(members, dtype=np.int64)
+        if self._sector_state(E[0]) is not E[0]:
+            E = torch.stack([self._sector_state(E[int(m)].contiguous()) for m in members])
+            members = np.arange(members.size, dtype=np.int64)
+        return ensemble_density(E, members, coef, bstart)
 
     def _state_tensor(self, state):
         import torch
@@ -1308,9 +1307,10 @@ class QuantumModel:
         shell = np.nonzero(np.abs(np.asarray(self.H.Nenergies) - e_q) < d_e)[0]
         if shell.size == 0:
             raise ValueError(f"ME shell empty: no eigstate |e_i-e_q|<delta_e={d_e}.")
-        V = self._eigvec_matrix()[torch.as_tensor(shell, device="cuda")]
-        psi_th = (V.sum(dim=0) / np.sqrt(shell.size)).cpu().numpy()
-        rho = self._row_density(V) / shell.size
+        E = self._eigvec_matrix()
+        psi_th = (E[torch.as_tensor(shell, device="cuda")].sum(dim=0) / np.sqrt(shell.size)).cpu().numpy()
+        rho = self._ensemble_rho(E, shell, np.full(shell.size, 1.0 / np.sqrt(shell.size), dtype=complex),
+                                 np.arange(shell.size + 1))
         return psi_th, self._density_values(rho, local_obs_list, special_norms, staggered_avgs, "ME")
 
     def diagonal_avg(self, local_obs_list, state, special_norms=None, staggered_avgs=None, tol_deg=1e-10):
@@ -1325,12 +1325,12 @@ class QuantumModel:
         staggered_avgs = staggered_avgs or {}
         st = self._state_tensor(state)
         E = self._eigvec_matrix()
-        coef = torch.mv(E.conj(), st)
+        coef = torch.mv(E.conj(), st).cpu().numpy()
         block, n_b = build_energy_block_ids(self.H.Nenergies, tol_deg)
-        block = block.astype(np.int64)
-        P = torch.zeros((n_b, E.shape[1]), dtype=torch.complex128, device="cuda")
-        P.index_add_(0, torch.as_tensor(block, device="cuda"), coef[:, None] * E)
-        return self._density_values(self._row_density(P), local_obs_list, special_norms, staggered_avgs, "DE")
+        # levels are contiguous in the ascending spectrum: block b = [starts[b], starts[b+1])
+        starts = np.concatenate([[0], np.nonzero(np.diff(block))[0] + 1, [block.size]])
+        rho = self._ensemble_rho(E, np.arange(block.size), coef, starts)
+        return self._density_values(rho, local_obs_list, special_norms, staggered_avgs, "DE")
 
     def canonical_avg(self, local_obs, beta):
         """Tr(O e^{-beta H}) / (Z L) with O the sum over sites of a diagonal local
@@ -1342,8 +1342,8 @@ class QuantumModel:
         import torch
         en, E = self.H.full_eigenpairs()
         w = np.exp(-beta * (en - en.min()))
-        w_t = torch.as_tensor(w / w.sum(), device="cuda")
-        vals = self._density_values(self._row_density(E, w_t), [local_obs], {}, {}, "C")
+        rho = self._ensemble_rho(E, np.arange(en.size), np.sqrt(w / w.sum()).astype(complex), np.arange(en.size + 1))
+        vals = self._density_values(rho, [local_obs], {}, {}, "C")
         return vals[f"C_{local_obs}"]
 
     def compute_expH(self, beta):

@@ -371,6 +371,20 @@ int edlgt_basis_marginals(const edlgt_basis* b, const void* psi, int32_t vec_typ
 int edlgt_basis_marginals_noalloc(const edlgt_basis* b, const void* psi, int32_t vec_type, int32_t d_max,
                                   double* out_dev, void* stream);
 
+/* Ensemble density of eigenvector rows: density[r] (device, n) =
+ * sum_b |sum_{t in block b} member_coef[t] * rows_matrix[member_rows[t]][r]|^2
+ * for a complex128 row-major matrix (device, n_rows x n); members,
+ * coefficients (complex128 as re, im pairs) and block_start (n_blocks + 1,
+ * from 0 to n_members) on the host.  The weighted sums behind
+ * microcanonical_avg / diagonal_avg / canonical_avg
+ * (edlgt/models/quantum_model.py:912-1156): equal weights over the shell, the
+ * state's projection on each degenerate level, Boltzmann weights (one member
+ * per block, coefficient sqrt(w)).  Rows outside the matrix or block starts
+ * that are not a partition are EDLGT_ERR_INVALID_ARG. */
+int edlgt_ensemble_density(const void* rows_matrix, int64_t n, int64_t n_rows, int64_t n_members,
+                           const int64_t* member_rows, const double* member_coef, const int64_t* block_start,
+                           int64_t n_blocks, double* density, void* stream);
+
 /* Stabilizer Renyi-2 entropy of a complex128 state psi (device, n_states) on
  * the support holding >= 1 - prob_threshold of its weight: replaces
  * QMB_state.stabilizer_renyi_entropy (edlgt/modeling/qmb_state.py:393-465,

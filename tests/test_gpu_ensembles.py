@@ -146,3 +146,24 @@ def test_spectral_sums_beta_and_r(setup):
     assert it < 1000 and abs(m2.get_thermal_beta(cfg, 1e-10) - want) < 1e-8
     with pytest.raises(ValueError):
         m2.H.get_r_value()
+
+
+def test_ensemble_density_entry_point(gpu):
+    """edlgt_ensemble_density on a random 5 x 7 complex matrix: blocks
+    {0, 1}, {2, 3, 4} of rows (4, 0, 1, 3, 2) against numpy; members outside
+    the matrix and block starts that do not partition the members are
+    errors."""
+    import torch
+    from edlgt_amd.device import ensemble_density
+    print("ensemble density entry point", flush=True)
+    rng = np.random.default_rng(11)
+    E = rng.normal(size=(5, 7)) + 1j * rng.normal(size=(5, 7))
+    rows = np.array([4, 0, 1, 3, 2])
+    c = rng.normal(size=5) + 1j * rng.normal(size=5)
+    got = ensemble_density(torch.as_tensor(E, device="cuda"), rows, c, [0, 2, 5]).cpu().numpy()
+    want = np.abs(c[0] * E[4] + c[1] * E[0]) ** 2 + np.abs(c[2] * E[1] + c[3] * E[3] + c[4] * E[2]) ** 2
+    assert np.max(np.abs(got - want)) < 1e-13
+    with pytest.raises(ValueError):
+        ensemble_density(torch.as_tensor(E, device="cuda"), [5], [1.0], [0, 1])
+    with pytest.raises(ValueError):
+        ensemble_density(torch.as_tensor(E, device="cuda"), rows, c, [0, 2, 4])
