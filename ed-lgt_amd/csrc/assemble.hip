@@ -779,34 +779,29 @@ __global__ void __launch_bounds__(256) expval_finish(const double2* __restrict__
 // ------------------------------------------------------------ matrix-free products
 // y[r] = sum over placements p and their transitions from row r of
 // strength_p * amp * x[col]: the stored matrix's row r applied without
-// storing it.  Four waves per block, one row per wave; lane l takes
-// placements l, l+64, ...; the lane sums are combined by a fixed butterfly
-// (deterministic).  Column lookups use a bucket index over the leading key
-// bits (a few probes instead of log2 N).
+// storing it.  Column lookups use a bucket index over the leading key bits (a
+// few probes instead of log2 N).
+// One thread per row walking every placement in program order: the 64 rows
+// of a wave run the same placement together (uniform control flow,
+// neighbouring configurations and keys), and a row's sum needs no cross-lane
+// reduction (deterministic).  Measured on QED 4x3 PBC: 26.6 ms per product
+// against 78.9 ms with one row per wave and the lanes over placements.
 template <class KeyT, class XT, class AT, int kOps>
 __global__ void __launch_bounds__(256) matfree_rows(AsmDev A, const XT* __restrict__ x, AT* __restrict__ y) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
-  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < A.n_states; r += (int64_t)gridDim.x * 4) {
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < A.n_states; r += (int64_t)gridDim.x * 256) {
     const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
     const KeyT key = keys[r];
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll 1
-    for (int p = lane; p < A.n_place; p += 64) {
+    for (int p = 0; p < A.n_place; ++p) {
       const Placement P = A.place[p];
-      // a dagger placement's entries are conj(amp * strength), as assembled
       const double2 sp = make_double2(P.s_re, P.dagger ? -P.s_im : P.s_im);
       acc = cadd(acc, cmul(placement_apply<KeyT, XT, kOps>(A, P, cfg, key, r, x), sp));
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      acc.x += __shfl_xor(acc.x, o, 64);
-      acc.y += __shfl_xor(acc.y, o, 64);
-    }
-    if (lane == 0) store_y(y, r, acc);
+    store_y(y, r, acc);
   }
 }
-
 
 }  // namespace
 
@@ -988,7 +983,7 @@ int matfree_spmm(const Csr& h, int xtype, int n_rhs, const void* x, int64_t ldx,
     return set_error(EDLGT_ERR_INVALID_ARG, "matrix-free products take complex128 or float64 vectors");
   const int64_t N = h.n_rows;
   if (N == 0) return EDLGT_OK;
-  const unsigned grid = (unsigned)std::min<int64_t>((N + 3) / 4, 65536);
+  const unsigned grid = (unsigned)std::min<int64_t>((N + 255) / 256, 65536);
   const size_t es = xtype == EDLGT_VALUE_C128 ? 16 : 8;
   for (int j = 0; j < n_rhs; ++j) {
     const char* xj = static_cast<const char*>(x) + (size_t)j * ldx * es;
