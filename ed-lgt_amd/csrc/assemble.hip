@@ -737,17 +737,30 @@ __global__ void __launch_bounds__(64 * kExpWaves) expval_rows(AsmDev A, int p0, 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = lane; i < 64 * kExpRounds; i += 64) acc[wave][i] = make_double2(0.0, 0.0);
   const KeyT* keys = static_cast<const KeyT*>(A.keys);
-  for (int64_t r = (int64_t)blockIdx.x * kExpWaves + wave; r < A.n_states;
-       r += (int64_t)gridDim.x * kExpWaves) {
-    const uint8_t* cfg = A.configs + (size_t)r * A.cfg_stride;
-    const KeyT key = keys[r];
-    double2 pr = to_c2(psi[r]);
+  // one row per lane, 64 consecutive rows per wave, every placement in turn:
+  // the lanes run the same placement together, and a fixed butterfly sums
+  // the wave's rows for each placement
+  for (int64_t base = ((int64_t)blockIdx.x * kExpWaves + wave) * 64; base < A.n_states;
+       base += (int64_t)gridDim.x * kExpWaves * 64) {
+    const int64_t r = base + lane;
+    const bool live = r < A.n_states;
+    const int64_t rr = live ? r : base;
+    const uint8_t* cfg = A.configs + (size_t)rr * A.cfg_stride;
+    const KeyT key = keys[rr];
+    double2 pr = live ? to_c2(psi[r]) : make_double2(0.0, 0.0);
     pr.y = -pr.y;  // conj(psi_r)
 #pragma unroll 1
-    for (int p = lane; p < np; p += 64) {
+    for (int p = 0; p < np; ++p) {
       const Placement P = A.place[p0 + p];
-      const double2 s = cmul(placement_apply<KeyT, PT, kOps>(A, P, cfg, key, r, psi), make_double2(P.s_re, P.s_im));
-      acc[wave][p] = cadd(acc[wave][p], cmul(pr, s));
+      double2 v = make_double2(0.0, 0.0);
+      if (live)
+        v = cmul(pr, cmul(placement_apply<KeyT, PT, kOps>(A, P, cfg, key, r, psi), make_double2(P.s_re, P.s_im)));
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o, 64);
+        v.y += __shfl_xor(v.y, o, 64);
+      }
+      if (lane == 0) acc[wave][p] = cadd(acc[wave][p], v);
     }
   }
   __syncthreads();
@@ -906,7 +919,7 @@ int placements_expval(const Basis& B, int32_t d_max, int32_t n_tables, const dou
   if (n_place == 0) return EDLGT_OK;
   const int64_t N = B.n_states;
   constexpr int kBatch = 64 * kExpRounds;
-  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((N + kExpWaves - 1) / kExpWaves, 2048));
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>((N + 64 * kExpWaves - 1) / (64 * kExpWaves), 2048));
   DevBuf partial, out;
   EDLGT_TRY(partial.alloc((size_t)nblk * kBatch * sizeof(double2)));
   EDLGT_TRY(out.alloc((size_t)n_place * sizeof(double2)));
