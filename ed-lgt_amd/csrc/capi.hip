@@ -622,6 +622,37 @@ int edlgt_momentum_apply(const edlgt_momentum* m, int32_t forward, const void* i
   return momentum_apply(*m, forward ? 1 : 0, in, out, (hipStream_t)stream);
 }
 
+int edlgt_csr_create(int64_t n, int64_t nnz, const int64_t* row_ptr, const int32_t* col, const void* val,
+                     int32_t value_type, void* stream, edlgt_csr** out) {
+  EDLGT_REQUIRE(out, "null output");
+  *out = nullptr;
+  EDLGT_REQUIRE(n >= 0 && nnz >= 0, "negative size");
+  EDLGT_REQUIRE(row_ptr && (nnz == 0 || (col && val)), "null argument");
+  EDLGT_REQUIRE(value_type == EDLGT_VALUE_C128 || value_type == EDLGT_VALUE_F64,
+                "value_type must be EDLGT_VALUE_C128 or EDLGT_VALUE_F64");
+  EDLGT_REQUIRE(n < 0x7fffffffLL, "row count above int32 column indices");
+  EDLGT_REQUIRE(row_ptr[0] == 0 && row_ptr[n] == nnz, "row_ptr must run from 0 to nnz");
+  for (int64_t r = 0; r < n; ++r)
+    EDLGT_REQUIRE(row_ptr[r + 1] >= row_ptr[r], "row_ptr must not decrease");
+  for (int64_t p = 0; p < nnz; ++p)
+    EDLGT_REQUIRE(col[p] >= 0 && col[p] < n, "column index outside the matrix");
+  hipStream_t st = (hipStream_t)stream;
+  auto c = std::make_unique<edlgt_csr>();
+  c->n_rows = n;
+  c->nnz = nnz;
+  c->value_type = value_type;
+  EDLGT_TRY(upload(c->row_ptr, row_ptr, (size_t)n + 1, st));
+  EDLGT_TRY(upload(c->col, col, (size_t)nnz, st));
+  if (value_type == EDLGT_VALUE_C128)
+    EDLGT_TRY(upload(c->val, static_cast<const double2*>(val), (size_t)nnz, st));
+  else
+    EDLGT_TRY(upload(c->val, static_cast<const double*>(val), (size_t)nnz, st));
+  EDLGT_TRY(csr_prepare_stream(*c, st));
+  EDLGT_HIP(hipStreamSynchronize(st));  // the host arrays may go once this returns
+  *out = c.release();
+  return EDLGT_OK;
+}
+
 int edlgt_csr_momentum(const edlgt_csr* h, const edlgt_momentum* m, void* stream, edlgt_csr** out) {
   EDLGT_REQUIRE(h && m && out, "null argument");
   EDLGT_REQUIRE(!h->mf, "not available for a matrix-free operator (no stored entries)");

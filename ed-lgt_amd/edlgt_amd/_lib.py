@@ -63,6 +63,7 @@ SIGNATURES = {
     "edlgt_csr_copy": (_i32, [_p, _p, _p, _p]),
     "edlgt_csr_device_ptrs": (_i32, [_p, _p, _p, _p]),
     "edlgt_csr_dense_eigh": (_i32, [_p, _p, _p, _p]),
+    "edlgt_csr_create": (_i32, [C.c_int64, C.c_int64, _p, _p, _p, C.c_int32, _p, _p]),
     "edlgt_basis_sre2": (_i32, [_p, _p, C.c_double, _p, _p, _p, _p, _p]),
     "edlgt_ensemble_density": (_i32, [_p, C.c_int64, C.c_int64, C.c_int64, _p, _p, _p, C.c_int64, _p, _p]),
     "edlgt_csr_make_low_precision": (_i32, [_p, _p]),

@@ -397,6 +397,26 @@ class CSRMatrix:
         self.basis = basis
         return self
 
+    @classmethod
+    def from_scipy(cls, A, stream=None):
+        """A device copy of a square SciPy matrix, duplicates summed
+        (edlgt_csr_create): complex128 values when any entry has an imaginary
+        part, float64 otherwise."""
+        import scipy.sparse as sp
+        M = sp.csr_matrix(A)
+        M.sum_duplicates()
+        if M.shape[0] != M.shape[1]:
+            raise ValueError(f"the matrix must be square, got {M.shape}")
+        cplx = np.iscomplexobj(M.data) and M.nnz > 0 and np.any(M.data.imag != 0)
+        val = np.ascontiguousarray(M.data if cplx else np.real(M.data), dtype=np.complex128 if cplx else np.float64)
+        rp = np.ascontiguousarray(M.indptr, dtype=np.int64)
+        col = np.ascontiguousarray(M.indices, dtype=np.int32)
+        h = C.c_void_p()
+        L.check(L.load().edlgt_csr_create(M.shape[0], M.nnz, rp.ctypes.data, col.ctypes.data, val.ctypes.data,
+                                          L.VALUE_C128 if cplx else L.VALUE_F64, L.stream_handle(stream),
+                                          C.byref(h)))
+        return cls._wrap(h, None)
+
     def momentum(self, mb, stream=None):
         """B^H H B on momentum sector mb (edlgt_csr_momentum): a new matrix of
         mb.n_cols rows; this one is left unchanged."""

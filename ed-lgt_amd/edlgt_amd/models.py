@@ -395,7 +395,57 @@ class QMB_hamiltonian:
         self.nnz = csr.nnz
         self._all_pairs = None
 
+    def add_term(self, term):
+        """Accumulates a contribution given as (row_list, col_list,
+        value_list) triplets, a dense matrix (its nonzero entries) or a SciPy
+        sparse matrix (qmb_hamiltonian.py:155-171); build() assembles them."""
+        import scipy.sparse as sps
+        if isinstance(term, tuple) and len(term) == 3:
+            r, c, v = (np.asarray(a) for a in term)
+        elif isinstance(term, np.ndarray):
+            r, c = np.nonzero(term)
+            v = term[r, c]
+        elif sps.issparse(term):
+            C_ = sps.coo_matrix(term)
+            r, c, v = C_.row, C_.col, C_.data
+        else:
+            raise TypeError("Unsupported term type: expect (dense/sparse) or (row_list, col_list, value_list) ")
+        r, c, v = np.ravel(r).astype(np.int64), np.ravel(c).astype(np.int64), np.ravel(v)
+        if not (r.size == c.size == v.size):
+            raise ValueError("row_list, col_list and value_list must have the same length")
+        if not hasattr(self, "_pending") or self._pending is None:
+            self._pending = []
+        self._pending.append((r, c, v))
+
+    def build(self, format):
+        """The accumulated triplets as SciPy's coalesced CSR (duplicates summed,
+        explicit zeros kept), uploaded as the device matrix
+        (qmb_hamiltonian.py:173-225); every format is served by the device
+        matrix, the label selects the time-evolution path."""
+        import scipy.sparse as sps
+        from .device import CSRMatrix
+        if format not in ("dense", "sparse", "linear"):
+            raise ValueError(f"Invalid format: {format}. Use 'dense', 'sparse', or 'linear'.")
+        parts = getattr(self, "_pending", None) or []
+        n = self.shape[0]
+        if parts:
+            r = np.concatenate([p[0] for p in parts])
+            c = np.concatenate([p[1] for p in parts])
+            v = np.concatenate([np.asarray(p[2], dtype=np.complex128) for p in parts])
+        else:
+            r = c = np.zeros(0, dtype=np.int64)
+            v = np.zeros(0, dtype=np.complex128)
+        M = sps.csr_matrix((v, (r, c)), shape=(n, n))
+        logger.info(f"Construct {format} Hamiltonian")
+        self.set_matrix(CSRMatrix.from_scipy(M), format)
+        self.get_sparsity()
+
     def _triplets(self):
+        parts = getattr(self, "_pending", None)
+        if self.Ham is None and parts:
+            return (np.concatenate([p[0] for p in parts]).astype(np.int32),
+                    np.concatenate([p[1] for p in parts]).astype(np.int32),
+                    np.concatenate([np.asarray(p[2]) for p in parts]))
         H = self.Ham
         if hasattr(H, "to_scipy"):
             H = H.to_scipy()

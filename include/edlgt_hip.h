@@ -230,6 +230,14 @@ int edlgt_csr_make_low_precision(edlgt_csr* h, void* stream);
  * value type (complex128 or float64), row k = eigenvector k.  A
  * matrix-free, value-released or multi-term matrix is EDLGT_ERR_INVALID_ARG. */
 int edlgt_csr_dense_eigh(const edlgt_csr* h, double* evals, void* evecs, void* stream);
+/* A matrix from host CSR arrays (n rows and columns, row_ptr int64 [n+1]
+ * from 0 to nnz, col int32 [nnz] in [0, n), val complex128 or float64 [nnz]):
+ * the device form of QMB_hamiltonian.add_term + build("sparse")
+ * (edlgt/modeling/qmb_hamiltonian.py:155-225) for terms a caller assembles
+ * itself; the caller passes SciPy's coalesced form (duplicates summed).
+ * Arrays that are not such a CSR are EDLGT_ERR_INVALID_ARG. */
+int edlgt_csr_create(int64_t n, int64_t nnz, const int64_t* row_ptr, const int32_t* col, const void* val,
+                     int32_t value_type, void* stream, edlgt_csr** out);
 int edlgt_csr_destroy(edlgt_csr* h);
 
 /* ======================================================================

@@ -170,3 +170,44 @@ def test_state_from_host_array(gpu, orc):
     assert abs(st.expectation_value(mod.H.Ham) - gs.expectation_value(mod.H.Ham)) < 1e-12
     with pytest.raises(ValueError):
         QMB_state(np.zeros((2, 2)), mod.lvals, mod.loc_dims)
+
+
+def test_hamiltonian_from_added_terms(gpu, orc):
+    """QMB_hamiltonian.add_term / build (qmb_hamiltonian.py:155-225): the
+    oracle's SU(2) Hamiltonian handed over in three parts (triplets, a dense
+    block, a SciPy matrix) and a zero-sum pair of duplicates is SciPy's
+    coalesced matrix on the device (pattern bit-exact, explicit zero kept);
+    its lowest eigenvalues match the oracle's; a real matrix is stored as
+    float64."""
+    import scipy.sparse as sps
+    from edlgt_amd import QMB_hamiltonian
+    from oracle.model import lowest_eigs
+    print("Hamiltonian from added terms", flush=True)
+    Ho = orc.hamiltonian(1.0, 5.0).tocoo()
+    k = Ho.nnz // 3
+    H = QMB_hamiltonian([6], orc.N)
+    H.add_term((Ho.row[:k], Ho.col[:k], Ho.data[:k]))
+    D = np.zeros((orc.N, orc.N), dtype=complex)
+    D[Ho.row[k:2 * k], Ho.col[k:2 * k]] = Ho.data[k:2 * k]
+    H.add_term(D)
+    H.add_term(sps.csr_matrix((Ho.data[2 * k:], (Ho.row[2 * k:], Ho.col[2 * k:])), shape=Ho.shape))
+    H.add_term((np.array([0, 0]), np.array([5, 5]), np.array([0.25, -0.25])))
+    assert H.row_list.size == Ho.nnz + 2
+    H.build("sparse")
+    want = sps.csr_matrix((np.concatenate([Ho.data, [0.25, -0.25]]),
+                           (np.concatenate([Ho.row, [0, 0]]), np.concatenate([Ho.col, [5, 5]]))), shape=Ho.shape)
+    want.sum_duplicates()
+    got = H.Ham.to_scipy()
+    assert np.array_equal(got.indptr, want.indptr) and np.array_equal(got.indices, want.indices)
+    assert np.max(np.abs(got.data - want.data)) <= 1e-15 * np.max(np.abs(want.data))
+    H.diagonalize(3, "sparse", np.full(6, 6))
+    assert np.max(np.abs(H.energies - lowest_eigs(Ho.tocsr(), 3))) < 1e-10
+    R = QMB_hamiltonian([2], 3)
+    R.add_term(np.array([[1.0, 2.0, 0.0], [2.0, 0.0, 1.0], [0.0, 1.0, -1.0]]))
+    R.build("sparse")
+    assert not R.Ham.is_complex and np.array_equal(R.Ham.to_scipy().toarray(),
+                                                   np.array([[1.0, 2.0, 0.0], [2.0, 0.0, 1.0], [0.0, 1.0, -1.0]]))
+    with pytest.raises(ValueError):
+        R.build("csr")
+    with pytest.raises(TypeError):
+        R.add_term([1, 2, 3])
