@@ -417,6 +417,26 @@ class CSRMatrix:
                                           C.byref(h)))
         return cls._wrap(h, None)
 
+    # SciPy-style sums and scalings (H + c P, as the reference's scripts add a
+    # penalty to H.Ham): formed on the host from the stored entries and
+    # uploaded as a new device matrix
+    def __add__(self, other):
+        import scipy.sparse as sp
+        if isinstance(other, CSRMatrix):
+            other = other.to_scipy()
+        if not (sp.issparse(other) or isinstance(other, np.ndarray)):
+            return NotImplemented
+        return CSRMatrix.from_scipy(self.to_scipy() + sp.csr_matrix(other))
+
+    __radd__ = __add__
+
+    def __mul__(self, a):
+        if not np.isscalar(a):
+            return NotImplemented
+        return CSRMatrix.from_scipy(self.to_scipy() * a)
+
+    __rmul__ = __mul__
+
     def momentum(self, mb, stream=None):
         """B^H H B on momentum sector mb (edlgt_csr_momentum): a new matrix of
         mb.n_cols rows; this one is left unchanged."""

@@ -52,3 +52,31 @@ def test_open_chain_raises(gpu):
     # edge sites hold 3 of the 6 dressed states: the local map leaves their basis
     with pytest.raises(ValueError, match="outside its image site's basis|not closed"):
         mod.get_parity_inversion_operator(True)
+
+
+def test_parity_penalty_added_to_the_hamiltonian(gpu):
+    """The reference's scripts add a parity penalty to H.Ham
+    (scripts/script_SU2_dynamics.py: H + 300 (1 - P)/2): the sum of the
+    device matrix and SciPy matrices is a device matrix whose spectrum keeps
+    the P = +1 levels and lifts the P = -1 ones by 300."""
+    import scipy.sparse as sps
+    from edlgt_amd import SU2_Model
+    from edlgt_amd.device import CSRMatrix
+    print("parity penalty", flush=True)
+    mod = SU2_Model(spin=0.5, pure_theory=False, sectors=[6], lvals=[6], has_obc=[False])
+    mod.build_Hamiltonian(1.0, 5.0)
+    mod.get_parity_inversion_operator(wrt_site=False)
+    P = mod.parityOP
+    Pminus = 0.5 * (sps.identity(P.shape[0], format="csr") - P)
+    H0 = mod.H.Ham.to_scipy().toarray()
+    mod.H.Ham = mod.H.Ham + 300 * Pminus
+    assert isinstance(mod.H.Ham, CSRMatrix)
+    Hp = mod.H.Ham.to_scipy().toarray()
+    assert np.max(np.abs(Hp - (H0 + 300 * Pminus.toarray()))) < 1e-12
+    Pd = P.toarray()
+    w, V = np.linalg.eigh(H0 + 1e-9 * Pd)  # levels labelled by parity
+    par = np.real(np.einsum("ij,ik,kj->j", V.conj(), Pd, V))
+    plus = np.sort(w[par > 0])
+    mod.diagonalize_Hamiltonian(3, "sparse", tol=1e-12)
+    assert np.max(np.abs(mod.H.energies - plus[:3])) < 1e-6
+    assert np.max(np.abs((2.0 * mod.H.Ham) .to_scipy().toarray() - 2 * Hp)) == 0.0
