@@ -389,11 +389,22 @@ class QMB_hamiltonian:
         logger.info("====================================================")
         logger.info(f"{en_state} ENERGY: {round(self.Nenergies[en_state], 16)}")
 
+    @property
+    def Ham(self):
+        return self.__dict__.get("_Ham")
+
+    @Ham.setter
+    def Ham(self, value):
+        """Any new matrix (also H.Ham = H.Ham + c P, as the reference's scripts
+        write it) drops the levels solved for the previous one."""
+        self._Ham = value
+        self._all_pairs = None
+        if value is not None and hasattr(value, "nnz"):
+            self.nnz = value.nnz
+
     def set_matrix(self, csr, fmt):
         self.Ham = csr
         self.Ham_type = fmt
-        self.nnz = csr.nnz
-        self._all_pairs = None
 
     def add_term(self, term):
         """Accumulates a contribution given as (row_list, col_list,

@@ -69,6 +69,7 @@ def test_parity_penalty_added_to_the_hamiltonian(gpu):
     P = mod.parityOP
     Pminus = 0.5 * (sps.identity(P.shape[0], format="csr") - P)
     H0 = mod.H.Ham.to_scipy().toarray()
+    before = mod.H.full_eigenpairs()[0]  # cached levels of H, dropped by the assignment below
     mod.H.Ham = mod.H.Ham + 300 * Pminus
     assert isinstance(mod.H.Ham, CSRMatrix)
     Hp = mod.H.Ham.to_scipy().toarray()
@@ -79,4 +80,6 @@ def test_parity_penalty_added_to_the_hamiltonian(gpu):
     plus = np.sort(w[par > 0])
     mod.diagonalize_Hamiltonian(3, "sparse", tol=1e-12)
     assert np.max(np.abs(mod.H.energies - plus[:3])) < 1e-6
-    assert np.max(np.abs((2.0 * mod.H.Ham) .to_scipy().toarray() - 2 * Hp)) == 0.0
+    assert np.max(np.abs((2.0 * mod.H.Ham).to_scipy().toarray() - 2 * Hp)) == 0.0
+    assert np.max(np.abs(before - np.linalg.eigvalsh(H0))) < 1e-9
+    assert np.max(np.abs(mod.H.full_eigenpairs()[0] - np.linalg.eigvalsh(Hp))) < 1e-9
